@@ -1,0 +1,374 @@
+"""ctypes front-end of the CPU FP64 oracle -- TEST INFRASTRUCTURE ONLY.
+
+The oracle restates the reference's JAX solver path in C++
+(``oracle/pdq_oracle.cpp``).  Only ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py`` may import this module; the product
+package never does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libpdq_oracle.so")
+
+FACT = {"dense": 0, "isotropic": 1, "blockdiag": 2}
+CONSTRAINT = {"ts0": 0, "ts1": 1}
+STRATEGY = {"filter": 0, "fixedpoint": 1, "fixedinterval": 2}
+SOLVER = {"solver": 0, "dynamic": 1, "mle": 2}
+ERROR_NORM = {"scale_then_rms": 0, "rms_then_scale": 1}
+CONTROL = {"integral": 0, "proportional_integral": 1}
+VF = {"lotka_volterra": 0, "van_der_pol": 1, "burgers": 2, "linear": 3, "logistic": 4}
+
+
+class _Cfg(ctypes.Structure):
+    _fields_ = [
+        ("fact", ctypes.c_int),
+        ("nu", ctypes.c_int),
+        ("d", ctypes.c_int),
+        ("constraint", ctypes.c_int),
+        ("strategy", ctypes.c_int),
+        ("solver", ctypes.c_int),
+        ("error_norm", ctypes.c_int),
+        ("control", ctypes.c_int),
+        ("vf_id", ctypes.c_int),
+        ("clip_dt", ctypes.c_int),
+        ("rtol", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("dt0", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("damp", ctypes.c_double),
+        ("safety", ctypes.c_double),
+        ("factor_min", ctypes.c_double),
+        ("factor_max", ctypes.c_double),
+        ("exponent_integral", ctypes.c_double),
+        ("exponent_proportional", ctypes.c_double),
+        ("base_scale", ctypes.c_double),
+        ("correct_asymptotic_underconfidence", ctypes.c_int),
+        ("max_attempts", ctypes.c_int),
+    ]
+
+
+_P = ctypes.POINTER(ctypes.c_double)
+_PL = ctypes.POINTER(ctypes.c_long)
+
+
+class _Out(ctypes.Structure):
+    _fields_ = [
+        ("t", _P),
+        ("mean", _P),
+        ("chol", _P),
+        ("os", _P),
+        ("num_steps", _PL),
+        ("filt_mean", _P),
+        ("filt_chol", _P),
+        ("cond_A", _P),
+        ("cond_mean", _P),
+        ("cond_chol", _P),
+        ("cond_tl", _P),
+        ("cond_to", _P),
+        ("term_mean", _P),
+        ("term_chol", _P),
+        ("attempts", ctypes.c_long),
+        ("accepted", ctypes.c_long),
+        ("status", ctypes.c_int),
+    ]
+
+
+@dataclasses.dataclass
+class Config:
+    """Solver configuration (mirrors the reference's constructor kwargs)."""
+
+    fact: str = "dense"
+    nu: int = 4
+    d: int = 2
+    constraint: str = "ts0"
+    strategy: str = "filter"
+    solver: str = "dynamic"
+    error_norm: str = "scale_then_rms"
+    control: str = "integral"
+    vf: str = "lotka_volterra"
+    clip_dt: bool = False
+    rtol: float = 1e-6
+    atol: float = 1e-8
+    dt0: float = 0.1
+    eps: float = 1e-8
+    damp: float = 0.0
+    safety: float = 0.95
+    factor_min: float = 0.2
+    factor_max: float = 10.0
+    exponent_integral: float = 0.3
+    exponent_proportional: float = 0.4
+    base_scale: float = 1.0
+    correct_asymptotic_underconfidence: bool = True
+    max_attempts: int = 0
+
+    def c(self) -> _Cfg:
+        return _Cfg(
+            FACT[self.fact],
+            self.nu,
+            self.d,
+            CONSTRAINT[self.constraint],
+            STRATEGY[self.strategy],
+            SOLVER[self.solver],
+            ERROR_NORM[self.error_norm],
+            CONTROL[self.control],
+            VF[self.vf],
+            int(self.clip_dt),
+            self.rtol,
+            self.atol,
+            self.dt0,
+            self.eps,
+            self.damp,
+            self.safety,
+            self.factor_min,
+            self.factor_max,
+            self.exponent_integral,
+            self.exponent_proportional,
+            self.base_scale,
+            int(self.correct_asymptotic_underconfidence),
+            self.max_attempts,
+        )
+
+    @property
+    def N(self) -> int:
+        return self.nu + 1
+
+    @property
+    def nblk(self) -> int:
+        return self.d if self.fact == "blockdiag" else 1
+
+    @property
+    def n(self) -> int:
+        return self.N * self.d if self.fact == "dense" else self.N
+
+    @property
+    def mean_size(self) -> int:
+        return self.N * self.d
+
+    @property
+    def chol_size(self) -> int:
+        return self.nblk * self.n * self.n
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++, no GPU needed)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.pdqo_solve_save_at.restype = ctypes.c_int
+        L.pdqo_solve_fixed_grid.restype = ctypes.c_int
+        L.pdqo_lml.restype = ctypes.c_double
+        L.pdqo_lml_from_y0.restype = ctypes.c_double
+        L.pdqo_lml_grad_fd.restype = ctypes.c_double
+        L.pdqo_logpdf.restype = ctypes.c_double
+        L.pdqo_control.restype = ctypes.c_double
+        L.pdqo_solve_save_at_batch.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def qr_r(M):
+    M = _f64(M)
+    m, n = M.shape
+    R = np.zeros((min(m, n), n))
+    lib().pdqo_qr_r(m, n, _p(M), _p(R))
+    return R
+
+
+def revert_conditional(R_X_F, R_X, R_YX, lstsq=False):
+    R_X_F, R_X, R_YX = _f64(R_X_F), _f64(R_X), _f64(R_YX)
+    n, k = R_X_F.shape
+    R_Y, R_XY, G = np.zeros((k, k)), np.zeros((n, n)), np.zeros((n, k))
+    lib().pdqo_revert_conditional(n, k, _p(R_X_F), _p(R_X), _p(R_YX), int(lstsq), _p(R_Y), _p(R_XY), _p(G))
+    return R_Y, (R_XY, G)
+
+
+def system_matrices(nu):
+    A = np.zeros((nu + 1, nu + 1))
+    Q = np.zeros((nu + 1, nu + 1))
+    lib().pdqo_system_matrices(nu, _p(A), _p(Q))
+    return A, Q
+
+
+def transition(cfg: Config, dt, os=None):
+    n = cfg.n
+    os = _f64(np.ones(cfg.nblk) if os is None else np.atleast_1d(os))
+    A, L, tl, to = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros(n)
+    c = cfg.c()
+    lib().pdqo_transition(ctypes.byref(c), ctypes.c_double(dt), _p(os), _p(A), _p(L), _p(tl), _p(to))
+    return A, L, tl, to
+
+
+def logpdf(mean, chol, u, isotropic=False):
+    mean, chol, u = _f64(mean), _f64(chol), _f64(u)
+    if mean.ndim == 1:
+        mean, u = mean[:, None], u[:, None]
+    return lib().pdqo_logpdf(int(isotropic), mean.shape[0], mean.shape[1], _p(mean), _p(chol), _p(u))
+
+
+def control(cfg: Config, dt, state, error_power):
+    s = ctypes.c_double(state)
+    c = cfg.c()
+    dt_new = lib().pdqo_control(ctypes.byref(c), ctypes.c_double(dt), ctypes.byref(s), ctypes.c_double(error_power))
+    return dt_new, s.value
+
+
+def taylor_coefficients(vf, params, y0, t0, num):
+    params, y0 = _f64(params), _f64(y0)
+    d = y0.size
+    out = np.zeros((num + 1, d))
+    lib().pdqo_taylor_coefficients(VF[vf], d, _p(params), _p(y0), ctypes.c_double(t0), num, _p(out))
+    return out
+
+
+def vector_field(vf, params, y, t=0.0):
+    params, y = _f64(params), _f64(y)
+    d = y.size
+    f, J = np.zeros(d), np.zeros((d, d))
+    lib().pdqo_vf(VF[vf], d, _p(params), _p(y), ctypes.c_double(t), _p(f), _p(J))
+    return f, J
+
+
+def _alloc(cfg: Config, K: int):
+    ms, cs, nn, nv = cfg.mean_size, cfg.chol_size, cfg.nblk * cfg.n * cfg.n, cfg.nblk * cfg.n
+    smoother = cfg.strategy != "filter"
+    arrs = {
+        "t": np.zeros(K),
+        "mean": np.zeros((K, ms)),
+        "chol": np.zeros((K, cs)),
+        "output_scale": np.zeros((K, cfg.nblk)),
+        "num_steps": np.zeros(K - 1, dtype=np.int64),
+    }
+    if smoother:
+        arrs.update(
+            filt_mean=np.zeros((K, ms)),
+            filt_chol=np.zeros((K, cs)),
+            cond_A=np.zeros((K - 1, nn)),
+            cond_mean=np.zeros((K - 1, ms)),
+            cond_chol=np.zeros((K - 1, cs)),
+            cond_tl=np.zeros((K - 1, nv)),
+            cond_to=np.zeros((K - 1, nv)),
+            term_mean=np.zeros(ms),
+            term_chol=np.zeros(cs),
+        )
+    out = _Out()
+    out.t, out.mean, out.chol, out.os = _p(arrs["t"]), _p(arrs["mean"]), _p(arrs["chol"]), _p(arrs["output_scale"])
+    out.num_steps = arrs["num_steps"].ctypes.data_as(_PL)
+    if smoother:
+        for k in ["filt_mean", "filt_chol", "cond_A", "cond_mean", "cond_chol", "cond_tl", "cond_to", "term_mean", "term_chol"]:
+            setattr(out, k, _p(arrs[k]))
+    return arrs, out
+
+
+def _finish(cfg, arrs, out):
+    arrs["attempts"] = out.attempts
+    arrs["accepted"] = out.accepted
+    arrs["status"] = out.status
+    if cfg.solver != "dynamic":
+        arrs["output_scale"] = arrs["output_scale"][:-1]
+    return arrs
+
+
+def solve_save_at(cfg: Config, params, tcoeffs, save_at, init_std=0.0):
+    params, tcoeffs, save_at = _f64(params), _f64(tcoeffs), _f64(save_at)
+    K = save_at.size
+    arrs, out = _alloc(cfg, K)
+    c = cfg.c()
+    st = lib().pdqo_solve_save_at(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), _p(save_at), K, ctypes.byref(out))
+    arrs = _finish(cfg, arrs, out)
+    arrs["status"] = st
+    return arrs
+
+
+def solve_fixed_grid(cfg: Config, params, tcoeffs, grid, init_std=0.0):
+    params, tcoeffs, grid = _f64(params), _f64(tcoeffs), _f64(grid)
+    K = grid.size
+    arrs, out = _alloc(cfg, K)
+    c = cfg.c()
+    lib().pdqo_solve_fixed_grid(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), _p(grid), K, ctypes.byref(out))
+    return _finish(cfg, arrs, out)
+
+
+def lml(cfg: Config, params, tcoeffs, grid, every, data, stds, average=True, init_std=0.0):
+    params, tcoeffs, grid, data, stds = map(_f64, (params, tcoeffs, grid, data, stds))
+    stds2 = stds.reshape(data.shape[0], -1)
+    c = cfg.c()
+    return lib().pdqo_lml(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average))
+
+
+def lml_from_y0(cfg: Config, params, y0, grid, every, data, stds, average=True):
+    params, y0, grid, data, stds = map(_f64, (params, y0, grid, data, stds))
+    stds2 = stds.reshape(data.shape[0], -1)
+    c = cfg.c()
+    return lib().pdqo_lml_from_y0(ctypes.byref(c), _p(params), _p(y0), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average))
+
+
+def lml_grad_fd(cfg: Config, params, y0, grid, every, data, stds, h=1e-6, average=True):
+    params, y0, grid, data, stds = map(_f64, (params, y0, grid, data, stds))
+    stds2 = stds.reshape(data.shape[0], -1)
+    grad = np.zeros(params.size + y0.size)
+    c = cfg.c()
+    val = lib().pdqo_lml_grad_fd(ctypes.byref(c), _p(params), params.size, _p(y0), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average), ctypes.c_double(h), _p(grad))
+    return val, grad
+
+
+def solve_save_at_batch(cfg: Config, params, tcoeffs, save_at, n_threads=0):
+    params, tcoeffs, save_at = _f64(params), _f64(tcoeffs), _f64(save_at)
+    n_inst = params.shape[0]
+    K = save_at.shape[1]
+    term = np.zeros((n_inst, cfg.mean_size))
+    att = np.zeros(n_inst, dtype=np.int64)
+    acc = np.zeros(n_inst, dtype=np.int64)
+    c = cfg.c()
+    st = lib().pdqo_solve_save_at_batch(ctypes.byref(c), n_inst, _p(params), params.shape[1], _p(tcoeffs), _p(save_at), K, _p(term), att.ctypes.data_as(_PL), acc.ctypes.data_as(_PL), n_threads)
+    return {"term_mean": term, "attempts": att, "accepted": acc, "status": st}
+
+
+def solve_save_every_step(cfg: Config, params, tcoeffs, t0, t1, init_std=0.0, K_max=20000):
+    params, tcoeffs = _f64(params), _f64(tcoeffs)
+    arrs, out = _alloc(cfg, K_max)
+    c = cfg.c()
+    K = lib().pdqo_solve_save_every_step(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), ctypes.c_double(t0), ctypes.c_double(t1), K_max, ctypes.byref(out))
+    if K < 0:
+        raise RuntimeError("K_max too small")
+    arrs = _finish(cfg, arrs, out)
+    for k, v in list(arrs.items()):
+        if isinstance(v, np.ndarray) and k not in ("term_mean", "term_chol"):
+            rows = K if v.shape[0] == K_max else (K - 1 if v.shape[0] == K_max - 1 else v.shape[0])
+            arrs[k] = v[:rows]
+    return arrs
+
+
+def solve_fixed_grid_checkpoints(cfg: Config, params, tcoeffs, grid, every, init_std=0.0):
+    params, tcoeffs, grid = _f64(params), _f64(tcoeffs), _f64(grid)
+    K = (grid.size - 1) // every + 1
+    arrs, out = _alloc(cfg, K)
+    c = cfg.c()
+    lib().pdqo_solve_fixed_grid_checkpoints(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), _p(grid), grid.size, every, ctypes.byref(out))
+    return _finish(cfg, arrs, out)

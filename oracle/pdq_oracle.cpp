@@ -1,0 +1,1802 @@
+// ===========================================================================
+// probdiffeq CPU FP64 oracle -- TEST INFRASTRUCTURE ONLY.
+//
+// A line-by-line C++ restatement of the reference's per-step probabilistic
+// IVP solver path (JAX code, not runnable here).  Only tests/, the smoke()
+// check and bench.py's CPU-baseline leg may load this library, and only as
+// the checker.  The product (probdiffeq_amd/, HIP kernels) never links it.
+//
+// Every function names the reference file:line it restates (paths are
+// relative to the reference checkout).
+//
+// Parity status of each part (what pins it) is listed in tests/ and in the
+// design notes:
+//   qr_r / sum_of_sqrtm_factors / revert_conditional -- numpy LAPACK geqrf
+//       signs, Kalman closed forms (tests/test_oracle_algebra.py)
+//   IWP system matrices / transition -- closed forms + mpmath Cholesky
+//   logpdf -- scipy.stats.multivariate_normal
+//   controller -- PI(1,0) == I KAT
+//   whole solves -- scipy DOP853 rtol 1e-13, convergence, equivalences
+//       (dense == isotropic, vmap(dense) == blockdiag, fixed-point ==
+//       fixed-interval, fixed grid == adaptive grid, truncated save_at)
+//   LML -- brute-force joint Gaussian density on small problems
+//   Taylor init -- sympy derivatives of the vector fields
+// ===========================================================================
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------- matrices
+struct Mat {
+  int r = 0, c = 0;
+  std::vector<double> a;
+  Mat() = default;
+  Mat(int r_, int c_) : r(r_), c(c_), a((size_t)r_ * c_, 0.0) {}
+  double& operator()(int i, int j) { return a[(size_t)i * c + j]; }
+  double operator()(int i, int j) const { return a[(size_t)i * c + j]; }
+  size_t size() const { return a.size(); }
+};
+using Vec = std::vector<double>;
+
+Mat eye(int n) {
+  Mat m(n, n);
+  for (int i = 0; i < n; ++i) m(i, i) = 1.0;
+  return m;
+}
+
+Mat transpose(const Mat& A) {
+  Mat T(A.c, A.r);
+  for (int i = 0; i < A.r; ++i)
+    for (int j = 0; j < A.c; ++j) T(j, i) = A(i, j);
+  return T;
+}
+
+// jnp matmul: acc = sum_k A(i,k) B(k,j), k ascending.
+Mat matmul(const Mat& A, const Mat& B) {
+  Mat C(A.r, B.c);
+  for (int i = 0; i < A.r; ++i)
+    for (int j = 0; j < B.c; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < A.c; ++k) acc += A(i, k) * B(k, j);
+      C(i, j) = acc;
+    }
+  return C;
+}
+
+Mat add(const Mat& A, const Mat& B) {
+  Mat C = A;
+  for (size_t i = 0; i < C.a.size(); ++i) C.a[i] += B.a[i];
+  return C;
+}
+
+Mat sub(const Mat& A, const Mat& B) {
+  Mat C = A;
+  for (size_t i = 0; i < C.a.size(); ++i) C.a[i] -= B.a[i];
+  return C;
+}
+
+Mat vstack(const Mat& A, const Mat& B) {
+  Mat C(A.r + B.r, A.c);
+  for (int i = 0; i < A.r; ++i)
+    for (int j = 0; j < A.c; ++j) C(i, j) = A(i, j);
+  for (int i = 0; i < B.r; ++i)
+    for (int j = 0; j < B.c; ++j) C(A.r + i, j) = B(i, j);
+  return C;
+}
+
+// Row scaling s[:, None] * A.
+Mat rowscale(const Vec& s, const Mat& A, bool absval = false) {
+  Mat C = A;
+  for (int i = 0; i < A.r; ++i) {
+    double f = absval ? std::fabs(s[i]) : s[i];
+    for (int j = 0; j < A.c; ++j) C(i, j) = f * A(i, j);
+  }
+  return C;
+}
+
+Mat scale(double f, const Mat& A) {
+  Mat C = A;
+  for (auto& x : C.a) x = f * x;
+  return C;
+}
+
+double fro(const Mat& A) {
+  double s = 0.0;
+  for (double x : A.a) s += x * x;
+  return std::sqrt(s);
+}
+
+// backend/linalg.py:8-37 qr_r = jnp.linalg.qr(mode="r"): LAPACK dgeqr2
+// Householder convention (dlarfg: beta = -sign(alpha) * ||(alpha, x)||,
+// tau = 0 and no reflection when the sub-column is zero).
+Mat qr_r(const Mat& M) {
+  Mat A = M;
+  const int m = A.r, n = A.c, k = std::min(m, n);
+  for (int i = 0; i < k; ++i) {
+    if (m - i <= 1) continue;
+    double ss = 0.0;
+    for (int r = i + 1; r < m; ++r) ss += A(r, i) * A(r, i);
+    const double xnorm = std::sqrt(ss);
+    if (xnorm == 0.0) continue;
+    const double alpha = A(i, i);
+    const double beta = -std::copysign(std::hypot(alpha, xnorm), alpha);
+    const double tau = (beta - alpha) / beta;
+    const double scal = 1.0 / (alpha - beta);
+    for (int r = i + 1; r < m; ++r) A(r, i) *= scal;
+    A(i, i) = beta;
+    for (int j = i + 1; j < n; ++j) {
+      double w = A(i, j);
+      for (int r = i + 1; r < m; ++r) w += A(r, i) * A(r, j);
+      w *= tau;
+      A(i, j) -= w;
+      for (int r = i + 1; r < m; ++r) A(r, j) -= A(r, i) * w;
+    }
+  }
+  Mat R(k, n);
+  for (int i = 0; i < k; ++i)
+    for (int j = i; j < n; ++j) R(i, j) = A(i, j);
+  return R;
+}
+
+// util/cholesky_util.py:89-103 sum_of_sqrtm_factors / triu_via_qr.
+Mat sum_of_sqrtm_factors(const Mat& R1, const Mat& R2) { return qr_r(vstack(R1, R2)); }
+
+// backend/linalg.py:48-53 solve_triangular (upper / lower), column by column.
+Mat solve_triu(const Mat& R, const Mat& B) {
+  Mat X = B;
+  const int n = R.r;
+  for (int col = 0; col < B.c; ++col)
+    for (int i = n - 1; i >= 0; --i) {
+      double s = B(i, col);
+      for (int j = i + 1; j < n; ++j) s -= R(i, j) * X(j, col);
+      X(i, col) = s / R(i, i);
+    }
+  return X;
+}
+
+Mat solve_tril(const Mat& L, const Mat& B) {
+  Mat X = B;
+  const int n = L.r;
+  for (int col = 0; col < B.c; ++col)
+    for (int i = 0; i < n; ++i) {
+      double s = B(i, col);
+      for (int j = 0; j < i; ++j) s -= L(i, j) * X(j, col);
+      X(i, col) = s / L(i, i);
+    }
+  return X;
+}
+
+// backend/linalg.py:56-61 lstsq_svd = jnp.linalg.lstsq(...)[0]: minimum-norm
+// least squares with the default cutoff rcond = eps * max(m, n) * s_max.
+// One-sided Jacobi SVD (Hestenes) on the square system.
+Mat lstsq_svd(const Mat& A, const Mat& B) {
+  const int m = A.r, n = A.c;
+  Mat U = A;  // columns get orthogonalised
+  Mat V = eye(n);
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double alpha = 0, beta = 0, gamma = 0;
+        for (int i = 0; i < m; ++i) {
+          alpha += U(i, p) * U(i, p);
+          beta += U(i, q) * U(i, q);
+          gamma += U(i, p) * U(i, q);
+        }
+        if (gamma == 0.0) continue;
+        off = std::max(off, std::fabs(gamma) / std::sqrt(alpha * beta));
+        const double zeta = (beta - alpha) / (2.0 * gamma);
+        const double t = std::copysign(1.0, zeta) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+        for (int i = 0; i < m; ++i) {
+          const double up = U(i, p), uq = U(i, q);
+          U(i, p) = cs * up - sn * uq;
+          U(i, q) = sn * up + cs * uq;
+        }
+        for (int i = 0; i < n; ++i) {
+          const double vp = V(i, p), vq = V(i, q);
+          V(i, p) = cs * vp - sn * vq;
+          V(i, q) = sn * vp + cs * vq;
+        }
+      }
+    if (off < 1e-16) break;
+  }
+  Vec s(n);
+  double smax = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double ss = 0;
+    for (int i = 0; i < m; ++i) ss += U(i, j) * U(i, j);
+    s[j] = std::sqrt(ss);
+    smax = std::max(smax, s[j]);
+  }
+  const double cutoff = 2.220446049250313e-16 * std::max(m, n) * smax;
+  Mat X(n, B.c);
+  for (int j = 0; j < n; ++j) {
+    if (!(s[j] > cutoff)) continue;
+    for (int col = 0; col < B.c; ++col) {
+      double ub = 0;
+      for (int i = 0; i < m; ++i) ub += (U(i, j) / s[j]) * B(i, col);
+      ub /= s[j];
+      for (int i = 0; i < n; ++i) X(i, col) += V(i, j) * ub;
+    }
+  }
+  return X;
+}
+
+enum { SOLVE_TRIU = 0, SOLVE_LSTSQ = 1 };
+
+Mat solve_upper(const Mat& R, const Mat& B, int mode) {
+  return mode == SOLVE_LSTSQ ? lstsq_svd(R, B) : solve_triu(R, B);
+}
+
+// util/cholesky_util.py:27-82 revert_conditional.
+struct Reverted {
+  Mat R_Y, R_XY, G;
+};
+Reverted revert_conditional(const Mat& R_X_F, const Mat& R_X, const Mat& R_YX, int mode) {
+  const int k = R_YX.r, n = R_X.c;
+  Mat R(k + n, k + n);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) R(i, j) = R_YX(i, j);
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < k; ++j) R(k + i, j) = R_X_F(i, j);
+    for (int j = 0; j < n; ++j) R(k + i, k + j) = R_X(i, j);
+  }
+  R = qr_r(R);
+  Reverted out;
+  out.R_Y = Mat(k, k);
+  Mat R12(k, n);
+  out.R_XY = Mat(n, n);
+  for (int i = 0; i < k; ++i) {
+    for (int j = 0; j < k; ++j) out.R_Y(i, j) = R(i, j);
+    for (int j = 0; j < n; ++j) R12(i, j) = R(i, k + j);
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) out.R_XY(i, j) = R(k + i, k + j);
+  out.G = transpose(solve_upper(out.R_Y, R12, mode));
+  return out;
+}
+
+// util/cholesky_util.py:106-176 cholesky_hilbert (Kahan's recurrence).
+Mat cholesky_hilbert(int n) {
+  const double Kf = 0.0;
+  Vec dr(n), f(n, 1.0 + Kf);
+  for (int i = 0; i < n; ++i) dr[i] = std::sqrt((double)(2 * i + 1));
+  for (int idx = 1; idx < n; ++idx) {
+    const double prev = f[idx - 1], idxf = idx;
+    f[idx] = (((prev / idxf) * (Kf + 2.0 * idxf)) / (Kf + idxf)) * (Kf + 2.0 * idxf + 1.0);
+  }
+  for (auto& x : f) x = 1.0 / x;
+  Mat U = eye(n);
+  for (int j = 1; j < n; ++j)
+    for (int kk = 0; kk < j; ++kk) {
+      const int i = j - 1 - kk;
+      const double denom = (double)(j - i), factor = Kf + (double)(i + 1) + (double)(j + 1);
+      U(i, j) = (U(i + 1, j) / denom) * factor;
+    }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) U(i, j) = U(i, j) * (dr[i] * f[j]);
+  Mat L = transpose(U);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) L(i, j) = 0.0;
+  return L;
+}
+
+double factorial_int(int k) {
+  double f = 1.0;
+  for (int i = 2; i <= k; ++i) f *= i;
+  return f;
+}
+
+double binom_int(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  return factorial_int(n) / (factorial_int(n - k) * factorial_int(k));
+}
+
+// _probdiffeq/utilities.py:57-71 system_matrices_1d_iwp.
+void system_matrices_1d_iwp(int nu, Mat& A1, Mat& Q1) {
+  const int N = nu + 1;
+  A1 = Mat(N, N);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) A1(i, j) = binom_int(nu - i, nu - j);
+  Mat L = cholesky_hilbert(N);
+  Mat Lf(N, N);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) Lf(i, j) = L(N - 1 - i, j);
+  Mat Q = transpose(qr_r(transpose(Lf)));
+  for (int j = 0; j < N; ++j) {
+    double s = (Q(j, j) > 0) ? 1.0 : (Q(j, j) < 0 ? -1.0 : 1.0);
+    for (int i = 0; i < N; ++i) Q(i, j) *= s;
+  }
+  Q1 = Q;
+}
+
+// _probdiffeq/utilities.py:74-84 preconditioner_taylor.  Factorials are
+// exact integers here (the reference forms exp(lgamma(k+1))).
+void preconditioner(int nu, double dt, Vec& p, Vec& pinv) {
+  const int N = nu + 1;
+  p.assign(N, 0.0);
+  pinv.assign(N, 0.0);
+  for (int i = 0; i < N; ++i) {
+    const int pw = nu - i;
+    const double s = factorial_int(pw);
+    p[i] = std::pow(dt, (double)pw) / s;
+    pinv[i] = std::pow(dt, -(double)pw) * s;
+  }
+}
+
+// ------------------------------------------------------- vector fields
+enum VfId { VF_LOTKA_VOLTERRA = 0, VF_VAN_DER_POL = 1, VF_BURGERS = 2, VF_LINEAR = 3, VF_LOGISTIC = 4 };
+
+struct Vf {
+  int id = 0, d = 0;
+  const double* prm = nullptr;
+  // f(y, t) -> out (d)
+  void eval(const double* y, double t, double* out) const {
+    switch (id) {
+      case VF_LOTKA_VOLTERRA: {
+        const double a = prm[0], b = prm[1], c = prm[2], dd = prm[3];
+        out[0] = a * y[0] - b * y[0] * y[1];
+        out[1] = -c * y[1] + dd * y[0] * y[1];
+        break;
+      }
+      case VF_VAN_DER_POL: {
+        const double mu = prm[0];
+        out[0] = y[1];
+        out[1] = mu * ((1.0 - y[0] * y[0]) * y[1]) - y[0];
+        break;
+      }
+      case VF_BURGERS: {
+        const double visc = prm[0], dx = prm[1];
+        for (int j = 0; j < d; ++j) {
+          const double um = y[(j + d - 1) % d], u0 = y[j], up = y[(j + 1) % d];
+          out[j] = -u0 * ((up - um) / (2.0 * dx)) + visc * ((up - 2.0 * u0 + um) / (dx * dx));
+        }
+        break;
+      }
+      case VF_LINEAR: {
+        for (int j = 0; j < d; ++j) out[j] = prm[j] * y[j] + t;
+        break;
+      }
+      case VF_LOGISTIC: {
+        const double r = prm[0];
+        for (int j = 0; j < d; ++j) out[j] = r * y[j] * (1.0 - y[j]);
+        break;
+      }
+    }
+  }
+  // Jacobian df/dy (d x d, row-major).
+  void jac(const double* y, double t, double* J) const {
+    (void)t;
+    std::fill(J, J + d * d, 0.0);
+    switch (id) {
+      case VF_LOTKA_VOLTERRA: {
+        const double a = prm[0], b = prm[1], c = prm[2], dd = prm[3];
+        J[0] = a - b * y[1];
+        J[1] = -b * y[0];
+        J[2] = dd * y[1];
+        J[3] = -c + dd * y[0];
+        break;
+      }
+      case VF_VAN_DER_POL: {
+        const double mu = prm[0];
+        J[0] = 0.0;
+        J[1] = 1.0;
+        J[2] = mu * (-2.0 * y[0] * y[1]) - 1.0;
+        J[3] = mu * (1.0 - y[0] * y[0]);
+        break;
+      }
+      case VF_BURGERS: {
+        const double visc = prm[0], dx = prm[1];
+        for (int j = 0; j < d; ++j) {
+          const double um = y[(j + d - 1) % d], u0 = y[j], up = y[(j + 1) % d];
+          const int jm = (j + d - 1) % d, jp = (j + 1) % d;
+          J[j * d + j] += -((up - um) / (2.0 * dx)) - 2.0 * visc / (dx * dx);
+          J[j * d + jp] += -u0 / (2.0 * dx) + visc / (dx * dx);
+          J[j * d + jm] += u0 / (2.0 * dx) + visc / (dx * dx);
+        }
+        break;
+      }
+      case VF_LINEAR: {
+        for (int j = 0; j < d; ++j) J[j * d + j] = prm[j];
+        break;
+      }
+      case VF_LOGISTIC: {
+        const double r = prm[0];
+        for (int j = 0; j < d; ++j) J[j * d + j] = r * (1.0 - 2.0 * y[j]);
+        break;
+      }
+    }
+  }
+};
+
+// Taylor-mode arithmetic for the initial Taylor coefficients (restates
+// jet_expansion_algorithms.py:49-103 / :155-177 for the registered vector
+// fields).  c[k][j] are normalised Taylor coefficients; the returned
+// derivatives are k! c[k].
+void taylor_coefficients(const Vf& vf, const double* y0, double t0, int num, double* out) {
+  const int d = vf.d, K = num + 1;
+  std::vector<Vec> c(K, Vec(d, 0.0));
+  for (int j = 0; j < d; ++j) c[0][j] = y0[j];
+  auto cauchy = [&](const Vec* X, const Vec* Y, int k, int j, int jx, int jy) {
+    double s = 0.0;
+    for (int i = 0; i <= k; ++i) s += X[i][jx] * Y[k - i][jy];
+    (void)j;
+    return s;
+  };
+  for (int k = 0; k < num; ++k) {
+    // coefficient k of f(y(t0 + s))
+    Vec fk(d, 0.0);
+    switch (vf.id) {
+      case VF_LOTKA_VOLTERRA: {
+        const double a = vf.prm[0], b = vf.prm[1], cc = vf.prm[2], dd = vf.prm[3];
+        const double y0y1 = cauchy(c.data(), c.data(), k, 0, 0, 1);
+        fk[0] = a * c[k][0] - b * y0y1;
+        fk[1] = -cc * c[k][1] + dd * y0y1;
+        break;
+      }
+      case VF_VAN_DER_POL: {
+        const double mu = vf.prm[0];
+        // (1 - y0^2) y1 coefficient k
+        std::vector<Vec> sq(k + 1, Vec(1, 0.0));
+        for (int m = 0; m <= k; ++m) {
+          double s = 0.0;
+          for (int i = 0; i <= m; ++i) s += c[i][0] * c[m - i][0];
+          sq[m][0] = (m == 0 ? 1.0 : 0.0) - s;
+        }
+        double s = 0.0;
+        for (int i = 0; i <= k; ++i) s += sq[i][0] * c[k - i][1];
+        fk[0] = c[k][1];
+        fk[1] = mu * s - c[k][0];
+        break;
+      }
+      case VF_BURGERS: {
+        const double visc = vf.prm[0], dx = vf.prm[1];
+        for (int j = 0; j < d; ++j) {
+          const int jm = (j + d - 1) % d, jp = (j + 1) % d;
+          double s = 0.0;
+          for (int i = 0; i <= k; ++i) s += c[i][j] * ((c[k - i][jp] - c[k - i][jm]) / (2.0 * dx));
+          fk[j] = -s + visc * ((c[k][jp] - 2.0 * c[k][j] + c[k][jm]) / (dx * dx));
+        }
+        break;
+      }
+      case VF_LINEAR: {
+        for (int j = 0; j < d; ++j) fk[j] = vf.prm[j] * c[k][j] + (k == 0 ? t0 : (k == 1 ? 1.0 : 0.0));
+        break;
+      }
+      case VF_LOGISTIC: {
+        const double r = vf.prm[0];
+        for (int j = 0; j < d; ++j) {
+          double s = 0.0;
+          for (int i = 0; i <= k; ++i) s += c[i][j] * c[k - i][j];
+          fk[j] = r * (c[k][j] - s);
+        }
+        break;
+      }
+    }
+    for (int j = 0; j < d; ++j) c[k + 1][j] = fk[j] / (double)(k + 1);
+  }
+  for (int k = 0; k < K; ++k)
+    for (int j = 0; j < d; ++j) out[k * d + j] = factorial_int(k) * c[k][j];
+}
+
+// ------------------------------------------------------ configuration
+enum { FACT_DENSE = 0, FACT_ISO = 1, FACT_BLOCKDIAG = 2 };
+enum { CONSTRAINT_TS0 = 0, CONSTRAINT_TS1 = 1 };
+enum { STRAT_FILTER = 0, STRAT_FIXEDPOINT = 1, STRAT_FIXEDINTERVAL = 2 };
+enum { SOLVER_PLAIN = 0, SOLVER_DYNAMIC = 1, SOLVER_MLE = 2 };
+enum { NORM_SCALE_THEN_RMS = 0, NORM_RMS_THEN_SCALE = 1 };
+enum { CONTROL_INTEGRAL = 0, CONTROL_PI = 1 };
+
+}  // namespace
+
+extern "C" {
+struct pdqo_cfg {
+  int fact, nu, d, constraint, strategy, solver, error_norm, control, vf_id, clip_dt;
+  double rtol, atol, dt0, eps, damp;
+  double safety, factor_min, factor_max, exponent_integral, exponent_proportional;
+  double base_scale;  // prior output scale (scalar, broadcast)
+  int correct_asymptotic_underconfidence, max_attempts;
+};
+}
+
+namespace {
+
+// ============================================================== SSM ops
+// Dense (ssm_impl_dense.py) and isotropic (ssm_impl_isotropic.py) share the
+// representation Normal{m (rows x cols), L (rows x rows)}; dense means are
+// columns (n x 1) in derivative-major order, isotropic means are N x d.
+struct Normal {
+  Mat m, L;
+};
+struct Cond {
+  Mat A;
+  Normal noise;
+  Vec tl, to;  // to_latent (A.c), to_observed (A.r)
+};
+
+Vec ones(int n) { return Vec(n, 1.0); }
+
+Vec inv(const Vec& v) {
+  Vec r(v.size());
+  for (size_t i = 0; i < v.size(); ++i) r[i] = 1.0 / v[i];
+  return r;
+}
+
+struct SSMBase {
+  int fact;
+  bool iso() const { return fact == FACT_ISO; }
+
+  // ssm_impl_dense.py:18-22 / ssm_impl_isotropic.py:75-79 apply_flat.
+  Normal apply_flat(const Cond& C, const Mat& x) const {
+    Mat xl = rowscale(C.tl, x);
+    Normal out;
+    if (iso()) {
+      Mat toA = rowscale(C.to, C.A);
+      out.m = add(matmul(toA, xl), C.noise.m);
+    } else {
+      out.m = rowscale(C.to, add(matmul(C.A, xl), C.noise.m));
+    }
+    out.L = rowscale(C.to, C.noise.L, true);
+    return out;
+  }
+
+  // ssm_impl_dense.py:24-33 / ssm_impl_isotropic.py:81-89 (blockdiag:
+  // ssm_impl_blockdiag.py:28-43 takes |to_latent| on the factor).
+  Normal marginalise(const Cond& C, const Normal& rv, bool abs_tl = false) const {
+    Mat mean = rowscale(C.tl, rv.m);
+    Mat chol = rowscale(C.tl, rv.L, abs_tl);
+    Mat R = sum_of_sqrtm_factors(transpose(matmul(C.A, chol)), transpose(C.noise.L));
+    Normal out;
+    out.m = rowscale(C.to, add(matmul(C.A, mean), C.noise.m));
+    out.L = rowscale(C.to, transpose(R), true);
+    return out;
+  }
+
+  // ssm_impl_dense.py:35-49 / ssm_impl_isotropic.py:91-105 merge.
+  Cond merge(const Cond& self, const Cond& other) const {
+    Vec T(self.tl.size());
+    for (size_t i = 0; i < T.size(); ++i) T[i] = self.tl[i] * other.to[i];
+    Cond out;
+    out.A = matmul(self.A, rowscale(T, other.A));
+    out.noise.m = add(matmul(self.A, rowscale(T, other.noise.m)), self.noise.m);
+    Mat R1 = transpose(matmul(self.A, rowscale(T, other.noise.L, true)));
+    Mat R2 = transpose(self.noise.L);
+    out.noise.L = transpose(sum_of_sqrtm_factors(R1, R2));
+    out.tl = other.tl;
+    out.to = self.to;
+    return out;
+  }
+
+  // ssm_impl_dense.py:51-77 / ssm_impl_isotropic.py:107-133 revert.
+  void revert(const Cond& C, const Normal& rv, int mode, Normal& observed, Cond& bw) const {
+    Mat mean = rowscale(C.tl, rv.m);
+    Mat chol = rowscale(C.tl, rv.L, true);
+    Mat R_X_F = transpose(matmul(C.A, chol));
+    Mat R_X = transpose(chol);
+    Mat R_YX = transpose(C.noise.L);
+    Reverted rc = revert_conditional(R_X_F, R_X, R_YX, mode);
+    Mat mean_observed = add(matmul(C.A, mean), C.noise.m);
+    Mat mean_corrected = sub(mean, matmul(rc.G, mean_observed));
+    bw.A = rc.G;
+    bw.noise.m = mean_corrected;
+    bw.noise.L = transpose(rc.R_XY);
+    bw.tl = inv(C.to);
+    bw.to = inv(C.tl);
+    observed.m = rowscale(C.to, mean_observed);
+    observed.L = rowscale(C.to, transpose(rc.R_Y), true);
+  }
+
+  // ssm_impl_dense.py:156-160 / ssm_impl_isotropic.py:203-207.
+  double residual_whitened_rms_zero(const Normal& rv) const {
+    Mat dx = rv.m;
+    if (!iso())
+      for (auto& x : dx.a) x = 0.0 - x;
+    Mat w = solve_tril(rv.L, dx);
+    return fro(w) / std::sqrt((double)rv.m.size());
+  }
+
+  // ssm_impl_dense.py:141-150 (row norms via per-row qr_r) /
+  // ssm_impl_isotropic.py:189-197 (vector_norm of rows).
+  Vec std_rows(const Normal& rv) const {
+    const int n = rv.L.r;
+    Vec s(n);
+    for (int i = 0; i < n; ++i) {
+      if (iso()) {
+        double ss = 0;
+        for (int j = 0; j < rv.L.c; ++j) ss += rv.L(i, j) * rv.L(i, j);
+        s[i] = std::sqrt(ss);
+      } else {
+        Mat col(rv.L.c, 1);
+        for (int j = 0; j < rv.L.c; ++j) col(j, 0) = rv.L(i, j);
+        s[i] = std::fabs(qr_r(col)(0, 0));
+      }
+    }
+    return s;
+  }
+
+  // ssm_impl_dense.py:170-180 / ssm_impl_isotropic.py:217-236 logpdf.
+  double logpdf(const Normal& rv, const Mat& u) const {
+    Mat chol = transpose(qr_r(transpose(rv.L)));
+    double slogdet = 0.0;
+    for (int i = 0; i < chol.r; ++i) slogdet += std::log(std::fabs(chol(i, i)));
+    const double c2pi = std::log(M_PI * 2.0);
+    if (!iso()) {
+      Mat dx = sub(u, rv.m);
+      Mat w = solve_tril(chol, dx);
+      double sq = 0;
+      for (double x : w.a) sq += x * x;
+      return -1.0 / 2.0 * sq - (double)u.size() / 2.0 * c2pi - slogdet;
+    }
+    double total = 0.0;
+    for (int col = 0; col < u.c; ++col) {
+      Mat dx(u.r, 1);
+      for (int i = 0; i < u.r; ++i) dx(i, 0) = u(i, col) - rv.m(i, col);
+      Mat w = solve_tril(chol, dx);
+      double maha = 0;
+      for (double x : w.a) maha += x * x;
+      const double logdet_term = 2.0 * slogdet;
+      total += -0.5 * (logdet_term + maha + (double)u.r * c2pi);
+    }
+    return total;
+  }
+
+  // ssm_impl_dense.py:196-200 / ssm_impl_isotropic.py:260-266.
+  Cond identity_conditional(const Normal& like) const {
+    const int n = like.L.r;
+    Cond C;
+    C.A = eye(n);
+    C.noise.m = Mat(like.m.r, like.m.c);
+    C.noise.L = Mat(n, n);
+    C.tl = ones(n);
+    C.to = ones(n);
+    return C;
+  }
+};
+
+// ========================================================= problem state
+struct Problem {
+  pdqo_cfg cfg;
+  int N, d, n;   // n = covariance dimension (dense: N*d; iso, blockdiag: N)
+  int nblk;      // blockdiag: d blocks; otherwise 1
+  Mat A1, Q1;    // 1-d IWP matrices
+  Mat A, Q;      // dense: kron(A1, I), kron(Q1, I); iso/bd: A1, Q1
+  Vf vf;
+  SSMBase ops;
+
+  void setup(const pdqo_cfg& c, const double* params) {
+    cfg = c;
+    N = c.nu + 1;
+    d = c.d;
+    system_matrices_1d_iwp(c.nu, A1, Q1);
+    vf.id = c.vf_id;
+    vf.d = d;
+    vf.prm = params;
+    ops.fact = c.fact == FACT_ISO ? FACT_ISO : FACT_DENSE;
+    if (c.fact == FACT_DENSE) {
+      n = N * d;
+      nblk = 1;
+      A = Mat(n, n);
+      Q = Mat(n, n);
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+          for (int k = 0; k < d; ++k) {
+            A(i * d + k, j * d + k) = A1(i, j);
+            Q(i * d + k, j * d + k) = Q1(i, j) * c.base_scale;  // kron(Q1, Lambda)
+          }
+    } else {
+      n = N;
+      nblk = c.fact == FACT_BLOCKDIAG ? d : 1;
+      A = A1;
+      Q = Q1;
+    }
+  }
+  bool bd() const { return cfg.fact == FACT_BLOCKDIAG; }
+};
+
+// A "state" bundles per-block normals (one block unless blockdiag).
+struct BNormal {
+  std::vector<Normal> b;
+};
+struct BCond {
+  std::vector<Cond> b;
+};
+
+// *WienerIntegrated.transition (ssm_impl_dense.py:347-363,
+// ssm_impl_isotropic.py:367-378, ssm_impl_blockdiag.py:406-432).
+BCond transition(const Problem& P, double dt, const Vec& os) {
+  Vec p1, pinv1;
+  preconditioner(P.cfg.nu, dt, p1, pinv1);
+  BCond out;
+  out.b.resize(P.nblk);
+  for (int blk = 0; blk < P.nblk; ++blk) {
+    Cond& C = out.b[blk];
+    if (P.cfg.fact == FACT_DENSE) {
+      C.tl.resize(P.n);
+      C.to.resize(P.n);
+      for (int i = 0; i < P.N; ++i)
+        for (int k = 0; k < P.d; ++k) {
+          C.to[i * P.d + k] = p1[i];
+          C.tl[i * P.d + k] = pinv1[i];
+        }
+      C.A = P.A;
+      C.noise.m = Mat(P.n, 1);
+      C.noise.L = scale(std::sqrt(std::fabs(dt)) * os[0], P.Q);
+    } else if (P.cfg.fact == FACT_ISO) {
+      C.to = p1;
+      C.tl = pinv1;
+      C.A = P.A;
+      C.noise.m = Mat(P.N, P.d);
+      const double s = std::sqrt(std::fabs(dt)) * P.cfg.base_scale * os[0];
+      C.noise.L = scale(s, P.Q);
+    } else {
+      C.to = p1;
+      C.tl = pinv1;
+      C.A = P.A;
+      C.noise.m = Mat(P.N, 1);
+      const double s = std::sqrt(std::fabs(dt)) * (P.cfg.base_scale * os[blk]);
+      C.noise.L = scale(s, P.Q);
+    }
+  }
+  return out;
+}
+
+// Block-wise wrappers.
+BNormal apply_flat(const Problem& P, const BCond& C, const BNormal& x) {
+  BNormal out;
+  out.b.resize(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) out.b[i] = P.ops.apply_flat(C.b[i], x.b[i].m);
+  return out;
+}
+BNormal marginalise(const Problem& P, const BCond& C, const BNormal& rv) {
+  BNormal out;
+  out.b.resize(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) out.b[i] = P.ops.marginalise(C.b[i], rv.b[i], P.bd());
+  return out;
+}
+BCond merge(const Problem& P, const BCond& self, const BCond& other) {
+  BCond out;
+  out.b.resize(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) out.b[i] = P.ops.merge(self.b[i], other.b[i]);
+  return out;
+}
+void revert(const Problem& P, const BCond& C, const BNormal& rv, int mode, BNormal& obs, BCond& bw) {
+  obs.b.resize(P.nblk);
+  bw.b.resize(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) P.ops.revert(C.b[i], rv.b[i], mode, obs.b[i], bw.b[i]);
+}
+BCond identity_conditional(const Problem& P, const BNormal& like) {
+  BCond out;
+  out.b.resize(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) out.b[i] = P.ops.identity_conditional(like.b[i]);
+  return out;
+}
+// residual_whitened_rms against zero data: scalar (dense/iso) or per block.
+Vec residual_whitened_rms_zero(const Problem& P, const BNormal& rv) {
+  Vec out(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) out[i] = P.ops.residual_whitened_rms_zero(rv.b[i]);
+  return out;
+}
+void rescale_cholesky(const Problem& P, BNormal& rv, const Vec& f) {
+  for (int i = 0; i < P.nblk; ++i)
+    for (auto& x : rv.b[i].L.a) x = f[P.bd() ? i : 0] * x;
+}
+void rescale_noise(const Problem& P, BCond& c, const Vec& f) {
+  for (int i = 0; i < P.nblk; ++i)
+    for (auto& x : c.b[i].noise.L.a) x = f[P.bd() ? i : 0] * x;
+}
+// The zeroth Taylor coefficient (u(t)) as a flat vector of length d.
+Vec u0_of(const Problem& P, const BNormal& rv) {
+  Vec u(P.d);
+  if (P.cfg.fact == FACT_DENSE)
+    for (int k = 0; k < P.d; ++k) u[k] = rv.b[0].m(k, 0);
+  else if (P.cfg.fact == FACT_ISO)
+    for (int k = 0; k < P.d; ++k) u[k] = rv.b[0].m(0, k);
+  else
+    for (int k = 0; k < P.d; ++k) u[k] = rv.b[k].m(0, 0);
+  return u;
+}
+Vec coeff_of(const Problem& P, const BNormal& rv, int i) {
+  Vec u(P.d);
+  if (P.cfg.fact == FACT_DENSE)
+    for (int k = 0; k < P.d; ++k) u[k] = rv.b[0].m(i * P.d + k, 0);
+  else if (P.cfg.fact == FACT_ISO)
+    for (int k = 0; k < P.d; ++k) u[k] = rv.b[0].m(i, k);
+  else
+    for (int k = 0; k < P.d; ++k) u[k] = rv.b[k].m(i, 0);
+  return u;
+}
+
+// Linearisations.  ts0: *OdeTs0.linearize (ssm_impl_dense.py:237-259,
+// ssm_impl_isotropic.py:298-317, ssm_impl_blockdiag.py:123-144).
+// ts1: DenseResidual.linearize (ssm_impl_dense.py:262-334) with
+// residual_from_ode (problems.py:518-532) and the prior-mean Taylor point.
+BCond linearize(const Problem& P, const BNormal& rv, double t) {
+  const int d = P.d, N = P.N;
+  Vec y0 = u0_of(P, rv);
+  Vec f(d);
+  P.vf.eval(y0.data(), t, f.data());
+  BCond out;
+  out.b.resize(P.nblk);
+  const double damp = P.cfg.damp;
+  if (P.cfg.fact == FACT_DENSE) {
+    Cond& C = out.b[0];
+    const int n = P.n;
+    if (P.cfg.constraint == CONSTRAINT_TS0) {
+      C.A = Mat(d, n);
+      for (int k = 0; k < d; ++k) C.A(k, 1 * d + k) = 1.0;
+      C.noise.m = Mat(d, 1);
+      for (int k = 0; k < d; ++k) C.noise.m(k, 0) = -f[k];
+    } else {
+      // r(m) = m_1 - f(m_0); J = [-J_f, I, 0...]; fx = r(xi) - J xi.
+      Vec Jf(d * d);
+      P.vf.jac(y0.data(), t, Jf.data());
+      C.A = Mat(d, n);
+      for (int i = 0; i < d; ++i) {
+        for (int j = 0; j < d; ++j) C.A(i, j) = -Jf[i * d + j];
+        C.A(i, d + i) = 1.0;
+      }
+      Vec y1 = coeff_of(P, rv, 1);
+      Mat Jxi = matmul(C.A, rv.b[0].m);
+      C.noise.m = Mat(d, 1);
+      for (int k = 0; k < d; ++k) C.noise.m(k, 0) = (y1[k] - f[k]) - Jxi(k, 0);
+    }
+    C.noise.L = Mat(d, d);
+    for (int k = 0; k < d; ++k) C.noise.L(k, k) = damp;
+    C.tl = ones(n);
+    C.to = ones(d);
+  } else if (P.cfg.fact == FACT_ISO) {
+    Cond& C = out.b[0];
+    C.A = Mat(1, N);
+    C.A(0, 1) = 1.0;
+    C.noise.m = Mat(1, d);
+    for (int k = 0; k < d; ++k) C.noise.m(0, k) = -f[k];
+    C.noise.L = Mat(1, 1);
+    C.noise.L(0, 0) = damp;
+    C.tl = ones(N);
+    C.to = ones(1);
+  } else {
+    for (int k = 0; k < d; ++k) {
+      Cond& C = out.b[k];
+      C.A = Mat(1, N);
+      C.A(0, 1) = 1.0;
+      C.noise.m = Mat(1, 1);
+      C.noise.m(0, 0) = -f[k];
+      C.noise.L = Mat(1, 1);
+      C.noise.L(0, 0) = damp;
+      C.tl = ones(N);
+      C.to = ones(1);
+    }
+  }
+  return out;
+}
+
+// Error-estimate standard deviations of the observed marginal (std rows),
+// flattened to the error vector of estimate_error_norm.
+Vec observed_std(const Problem& P, const BNormal& obs) {
+  Vec out;
+  for (int i = 0; i < P.nblk; ++i) {
+    Vec s = P.ops.std_rows(obs.b[i]);
+    if (P.bd()) {
+      // blockdiag std = vector_norm per row (ssm_impl_blockdiag.py:269-274).
+      double ss = 0;
+      for (int j = 0; j < obs.b[i].L.c; ++j) ss += obs.b[i].L(0, j) * obs.b[i].L(0, j);
+      s[0] = std::sqrt(ss);
+    }
+    out.insert(out.end(), s.begin(), s.end());
+  }
+  return out;
+}
+
+// =========================================================== solutions
+struct Post {
+  BNormal marg;
+  BCond cond;  // smoothers only
+};
+struct Sol {
+  double t = 0;
+  BNormal u;
+  Post post;
+  Vec os;
+  long num_steps = 0;
+  BCond fx;             // fun_evals (cached linearisation)
+  Vec mle_running;      // solver_mle auxiliary
+  double mle_num_data = 0;
+};
+
+bool smoother(const Problem& P) { return P.cfg.strategy != STRAT_FILTER; }
+
+// strategy.predict (estimators_and_losses.py:371-373, 526-532, 612-620).
+void predict(const Problem& P, const Post& post, const BCond& tr, BNormal& u, Post& pred) {
+  if (!smoother(P)) {
+    pred.marg = marginalise(P, tr, post.marg);
+    u = pred.marg;
+    return;
+  }
+  BCond cond;
+  revert(P, tr, post.marg, SOLVE_TRIU, pred.marg, cond);
+  if (P.cfg.strategy == STRAT_FIXEDPOINT)
+    pred.cond = merge(P, post.cond, cond);
+  else
+    pred.cond = cond;
+  u = pred.marg;
+}
+
+// bayes_rule_tree(zeros, rv) (ssm_impl_api.py:112-116) and its variants.
+BNormal bayes_rule_zero(const Problem& P, const BCond& fx, const BNormal& rv, BNormal* observed_out) {
+  BNormal obs;
+  BCond rev;
+  revert(P, fx, rv, SOLVE_TRIU, obs, rev);
+  BNormal out;
+  out.b.resize(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) {
+    Mat zeros(fx.b[i].noise.m.r, fx.b[i].noise.m.c);
+    out.b[i] = P.ops.apply_flat(rev.b[i], zeros);
+  }
+  if (observed_out) *observed_out = obs;
+  return out;
+}
+
+Vec prototype_os(const Problem& P) { return Vec(P.nblk, 1.0); }
+
+// solver*.init (solvers.py:515-547, 664-700, 348-393), constraint_init=None.
+Sol solver_init(const Problem& P, double t, const BNormal& init) {
+  Sol s;
+  s.t = t;
+  s.u = init;
+  s.post.marg = init;
+  if (smoother(P)) s.post.cond = identity_conditional(P, init);
+  s.os = prototype_os(P);
+  s.num_steps = 0;
+  // fun_evals = zeros_like(linearisation) (only its structure matters).
+  s.fx = linearize(P, init, t);
+  for (auto& c : s.fx.b) {
+    for (auto& x : c.A.a) x = 0;
+    for (auto& x : c.noise.m.a) x = 0;
+    for (auto& x : c.noise.L.a) x = 0;
+    for (auto& x : c.tl) x = 0;
+    for (auto& x : c.to) x = 0;
+  }
+  s.mle_running = Vec(P.nblk, 0.0);
+  s.mle_num_data = 0.0;
+  return s;
+}
+
+// solver_dynamic.step (solvers.py:549-599), solver.step (:702-733),
+// solver_mle.step (:395-437).
+Sol solver_step(const Problem& P, const Sol& st, double dt) {
+  Sol out;
+  const double tn = st.t + dt;
+  if (P.cfg.solver == SOLVER_DYNAMIC) {
+    BCond tr = transition(P, dt, prototype_os(P));
+    BNormal u = apply_flat(P, tr, st.u);
+    BCond fx = linearize(P, u, tn);
+    BNormal observed = marginalise(P, fx, u);
+    Vec os = residual_whitened_rms_zero(P, observed);
+    BCond tr2 = transition(P, dt, os);
+    BNormal upred;
+    Post pred;
+    predict(P, st.post, tr2, upred, pred);
+    BNormal upd = bayes_rule_zero(P, fx, upred, nullptr);
+    out.u = upd;
+    out.post.marg = upd;
+    if (smoother(P)) out.post.cond = pred.cond;
+    out.os = os;
+    out.fx = fx;
+  } else {
+    Vec os1 = prototype_os(P);
+    BCond tr = transition(P, dt, os1);
+    BNormal upred;
+    Post pred;
+    predict(P, st.post, tr, upred, pred);
+    BCond fx = linearize(P, upred, tn);
+    BNormal observed;
+    BNormal upd = bayes_rule_zero(P, fx, upred, &observed);
+    out.u = upd;
+    out.post.marg = upd;
+    if (smoother(P)) out.post.cond = pred.cond;
+    out.fx = fx;
+    if (P.cfg.solver == SOLVER_MLE) {
+      Vec new_term = residual_whitened_rms_zero(P, observed);
+      out.mle_running.resize(P.nblk);
+      for (int i = 0; i < P.nblk; ++i) {
+        const double x1 = std::sqrt(st.mle_num_data / (st.mle_num_data + 1)) * st.mle_running[i];
+        const double x2 = std::sqrt(1 / (st.mle_num_data + 1)) * new_term[i];
+        out.mle_running[i] = std::hypot(x1, x2);
+      }
+      out.mle_num_data = st.mle_num_data + 1;
+      out.os = st.os;
+    } else {
+      out.os = os1;
+    }
+  }
+  out.t = tn;
+  out.num_steps = st.num_steps + 1;
+  return out;
+}
+
+// error_residual_std.estimate_error_norm (solvers.py:925-996) with
+// error_norm_scale_then_rms / _rms_then_scale (:770-811).
+double estimate_error_power(const Problem& P, const Sol& prev, const Sol& prop, double dt) {
+  BCond tr = transition(P, dt, prototype_os(P));
+  BNormal rv = apply_flat(P, tr, prev.u);
+  BNormal observed = marginalise(P, prop.fx, rv);
+  Vec os = residual_whitened_rms_zero(P, observed);
+  rescale_cholesky(P, observed, os);
+  Vec err = observed_std(P, observed);
+  Vec u0 = u0_of(P, prev.u), u1 = u0_of(P, prop.u);
+  const int d = P.d;
+  Vec ref(d);
+  for (int k = 0; k < d; ++k) ref[k] = std::max(std::fabs(u0[k]), std::fabs(u1[k]));
+  const int nord = 1;  // residual_order - 1 for first-order ODEs
+  Vec err_abs(err.size());
+  for (size_t i = 0; i < err.size(); ++i) err_abs[i] = err[i] * std::pow(dt, (double)nord) / factorial_int(nord);
+  double norm;
+  if (P.cfg.error_norm == NORM_SCALE_THEN_RMS) {
+    double ss = 0;
+    for (int k = 0; k < d; ++k) {
+      const double e = err_abs[err_abs.size() == 1 ? 0 : k];
+      const double r = e / (P.cfg.atol + P.cfg.rtol * std::fabs(ref[k]));
+      ss += r * r;
+    }
+    norm = std::sqrt(ss) / std::sqrt((double)d);
+  } else {
+    double sa = 0, sr = 0;
+    for (double e : err_abs) sa += e * e;
+    for (double r : ref) sr += r * r;
+    const double na = std::sqrt(sa) / std::sqrt((double)err_abs.size());
+    const double nr = std::sqrt(sr) / std::sqrt((double)ref.size());
+    norm = na / (P.cfg.atol + P.cfg.rtol * nr);
+  }
+  return std::pow(norm, -1.0 / (double)P.N);
+}
+
+// controllers.py:66-84 control_integral / :24-63 proportional-integral.
+double control_apply(const Problem& P, double dt, double& cstate, double error_power) {
+  const auto& c = P.cfg;
+  if (c.control == CONTROL_INTEGRAL) {
+    const double ratio = c.safety * error_power;
+    const double clipped = std::min(ratio, c.factor_max);
+    return std::max(c.factor_min, clipped) * dt;
+  }
+  const double gi = std::pow(error_power, c.exponent_integral);
+  const double gp = std::pow(error_power / cstate, c.exponent_proportional);
+  const double ratio = c.safety * gi * gp;
+  const double clipped = std::min(ratio, c.factor_max);
+  const double sf = std::max(c.factor_min, clipped);
+  if (error_power >= 1.0) cstate = error_power;
+  return sf * dt;
+}
+
+// ProbabilisticSolver.interpolate_fwd / _at_t1 (solvers.py:205-315) with
+// the strategies' interpolate_fwd (estimators_and_losses.py:393-421,
+// 534-591, 622-672, 711-717).
+void interpolate_fwd(const Problem& P, double t, const Sol& from, const Sol& to, Sol& interpolated, Sol& step_from,
+                     Sol& interp_from) {
+  BCond tr0 = transition(P, t - from.t, to.os);
+  BCond tr1 = transition(P, to.t - t, to.os);
+  Post interp_post, step_post, from_post;
+  BNormal marg;
+  if (P.cfg.strategy == STRAT_FILTER) {
+    BNormal u;
+    predict(P, from.post, tr0, u, interp_post);
+    marg = interp_post.marg;
+    step_post = to.post;
+    from_post = interp_post;
+  } else if (P.cfg.strategy == STRAT_FIXEDPOINT) {
+    BNormal u;
+    Post ext_t;
+    predict(P, from.post, tr0, u, ext_t);
+    Post previous_new;
+    previous_new.marg = ext_t.marg;
+    previous_new.cond = identity_conditional(P, from.post.marg);
+    Post ext_t1;
+    predict(P, previous_new, tr1, u, ext_t1);
+    interp_post = ext_t;
+    step_post.marg = to.post.marg;
+    step_post.cond = ext_t1.cond;
+    from_post = previous_new;
+    marg = ext_t.marg;
+  } else {
+    BNormal u;
+    Post sol_t, ext_t1;
+    predict(P, from.post, tr0, u, sol_t);
+    predict(P, sol_t, tr1, u, ext_t1);
+    step_post.marg = to.post.marg;
+    step_post.cond = ext_t1.cond;
+    from_post = sol_t;
+    interp_post = sol_t;
+    marg = sol_t.marg;
+  }
+  step_from = to;
+  step_from.post = step_post;
+  interpolated = to;
+  interpolated.t = t;
+  interpolated.post = interp_post;
+  interpolated.u = marg;
+  interp_from = from;
+  interp_from.t = t;
+  interp_from.post = from_post;
+}
+
+void interpolate_fwd_at_t1(const Problem& P, const Sol& from, const Sol& to, Sol& sol, Sol& step_from, Sol& interp_from) {
+  Post interpolated = to.post, step_post = to.post, from_post = to.post;
+  BNormal marg = to.post.marg;
+  if (P.cfg.strategy == STRAT_FIXEDPOINT) {
+    Post resume;
+    resume.marg = to.post.marg;
+    resume.cond = identity_conditional(P, to.post.marg);
+    step_post = resume;
+    from_post = resume;
+  }
+  interp_from = from;
+  interp_from.t = to.t;
+  interp_from.post = from_post;
+  sol = to;
+  sol.post = interpolated;
+  sol.u = marg;
+  step_from = to;
+  step_from.post = step_post;
+}
+
+// ---------------------------------------------------------- outputs
+struct Outputs {
+  std::vector<Sol> sols;  // the per-checkpoint (or per-step) solutions
+  Sol sol0, sol1;
+  long attempts = 0, accepted = 0;
+  int status = 0;
+};
+
+enum { ST_OK = 0, ST_MAX_ATTEMPTS = 1, ST_NONFINITE = 2, ST_DT_UNDERFLOW = 3 };
+
+// solve_adaptive_save_at (solvers_via_adaptive_steps.py:46-148) with the
+// RejectionLoop (:196-375).
+void solve_save_at(const Problem& P, const BNormal& init, const double* save_at, int K, Outputs& out) {
+  const auto& c = P.cfg;
+  Sol sol0 = solver_init(P, save_at[0], init);
+  out.sol0 = sol0;
+  double dt = c.dt0, cstate = 1.0;
+  Sol step_from = sol0, interp_from = sol0;
+  for (int ck = 1; ck < K && out.status == ST_OK; ++ck) {
+    const double t1 = save_at[ck];
+    bool do_continue = true;
+    Sol solution;
+    while (do_continue) {
+      if (step_from.t + c.eps < t1) {
+        // RejectionLoop.step: attempt until error_power >= 1.
+        double ep = 0.9;
+        Sol proposed;
+        int attempts = 0;
+        while (ep < 1.0) {
+          double dtc = dt;
+          if (c.clip_dt) dtc = std::min(dtc, t1 - step_from.t);
+          proposed = solver_step(P, step_from, dtc);
+          ep = estimate_error_power(P, step_from, proposed, dtc);
+          dt = control_apply(P, dtc, cstate, ep);
+          ++attempts;
+          ++out.attempts;
+          if (!std::isfinite(ep) || !std::isfinite(dt)) {
+            out.status = ST_NONFINITE;
+            break;
+          }
+          if (c.max_attempts > 0 && attempts >= c.max_attempts && ep < 1.0) {
+            out.status = ST_MAX_ATTEMPTS;
+            break;
+          }
+        }
+        if (out.status != ST_OK) break;
+        ++out.accepted;
+        interp_from = step_from;
+        step_from = proposed;
+      }
+      const bool before = step_from.t + c.eps < t1;
+      const bool after = step_from.t > t1 + c.eps;
+      if (before) {
+        solution = step_from;
+      } else if (after) {
+        Sol sf, ifr;
+        interpolate_fwd(P, t1, interp_from, step_from, solution, sf, ifr);
+        step_from = sf;
+        interp_from = ifr;
+      } else {
+        Sol sf, ifr;
+        interpolate_fwd_at_t1(P, interp_from, step_from, solution, sf, ifr);
+        step_from = sf;
+        interp_from = ifr;
+      }
+      do_continue = step_from.t + c.eps < t1;
+    }
+    out.sols.push_back(solution);
+  }
+  out.sol1 = step_from;
+}
+
+// solve_fixed_grid (solvers_via_fixed_steps.py:11-34).
+void solve_fixed_grid(const Problem& P, const BNormal& init, const double* grid, int K, Outputs& out) {
+  Sol s = solver_init(P, grid[0], init);
+  out.sol0 = s;
+  for (int k = 1; k < K; ++k) {
+    const double dt = grid[k] - grid[k - 1];
+    s = solver_step(P, s, dt);
+    ++out.attempts;
+    ++out.accepted;
+    out.sols.push_back(s);
+  }
+  out.sol1 = s;
+}
+
+// util/test_util.py:10-66 solve_adaptive_save_every_step: call the
+// rejection loop until step_from.t >= t1 and keep every returned solution.
+void solve_save_every_step(const Problem& P, const BNormal& init, double t0, double t1, Outputs& out) {
+  const auto& c = P.cfg;
+  Sol sol0 = solver_init(P, t0, init);
+  out.sol0 = sol0;
+  double dt = c.dt0, cstate = 1.0;
+  Sol step_from = sol0, interp_from = sol0;
+  while (step_from.t < t1 && out.status == ST_OK) {
+    Sol solution;
+    if (step_from.t + c.eps < t1) {
+      double ep = 0.9;
+      Sol proposed;
+      while (ep < 1.0) {
+        double dtc = dt;
+        if (c.clip_dt) dtc = std::min(dtc, t1 - step_from.t);
+        proposed = solver_step(P, step_from, dtc);
+        ep = estimate_error_power(P, step_from, proposed, dtc);
+        dt = control_apply(P, dtc, cstate, ep);
+        ++out.attempts;
+        if (!std::isfinite(ep)) {
+          out.status = ST_NONFINITE;
+          break;
+        }
+      }
+      if (out.status != ST_OK) break;
+      ++out.accepted;
+      interp_from = step_from;
+      step_from = proposed;
+    }
+    const bool before = step_from.t + c.eps < t1;
+    const bool after = step_from.t > t1 + c.eps;
+    if (before) {
+      solution = step_from;
+    } else if (after) {
+      Sol sf, ifr;
+      interpolate_fwd(P, t1, interp_from, step_from, solution, sf, ifr);
+      step_from = sf;
+      interp_from = ifr;
+    } else {
+      Sol sf, ifr;
+      interpolate_fwd_at_t1(P, interp_from, step_from, solution, sf, ifr);
+      step_from = sf;
+      interp_from = ifr;
+    }
+    out.sols.push_back(solution);
+  }
+  out.sol1 = step_from;
+}
+
+// Forward pass of the LML configuration: the fixed-point smoother on a fixed
+// grid with checkpoints at every `every`-th grid point (interp_fwd_at_t1
+// semantics at each checkpoint).
+void solve_fixed_grid_checkpoints(const Problem& P, const BNormal& init, const double* grid, int K, int every,
+                                  Outputs& out) {
+  Sol s = solver_init(P, grid[0], init);
+  out.sol0 = s;
+  Sol interp_from = s;
+  for (int k = 1; k < K; ++k) {
+    const double dt = grid[k] - grid[k - 1];
+    Sol prop = solver_step(P, s, dt);
+    ++out.attempts;
+    ++out.accepted;
+    Sol prev = s;
+    s = prop;
+    if (k % every == 0) {
+      Sol sol, sf, ifr;
+      interpolate_fwd_at_t1(P, prev, s, sol, sf, ifr);
+      out.sols.push_back(sol);
+      s = sf;
+    }
+  }
+  out.sol1 = s;
+}
+
+// userfriendly_output + strategy.finalize (solvers.py:439-480, 601-633,
+// 735-767; estimators_and_losses.py:380-391, 437-470).
+struct Final {
+  std::vector<double> t;
+  std::vector<BNormal> u;          // K marginals
+  std::vector<BNormal> filtering;  // smoothers: K filtering marginals
+  std::vector<BCond> conds;        // smoothers: K-1 backward conditionals
+  BNormal terminal;                // smoothers: full_posterior.marginal
+  std::vector<Vec> os;             // K output scales
+  std::vector<long> num_steps;     // K-1
+};
+
+void finalize(const Problem& P, const Outputs& o, Final& F) {
+  const int Km1 = (int)o.sols.size();
+  Vec scale = prototype_os(P);
+  if (P.cfg.solver == SOLVER_MLE) {
+    scale = o.sol1.mle_running;
+    if (P.cfg.correct_asymptotic_underconfidence)
+      for (auto& x : scale) x = x / std::sqrt((double)o.sols.back().num_steps);
+  }
+  F.t.push_back(o.sol0.t);
+  for (auto& s : o.sols) F.t.push_back(s.t);
+  if (P.cfg.solver == SOLVER_DYNAMIC) {
+    F.os.push_back(o.sol0.os);
+    for (auto& s : o.sols) F.os.push_back(s.os);
+  } else {
+    // solvers.py:755 / :469: ones_like(solution.output_scale) * scale has
+    // one entry per checkpoint after t0 (K-1 rows, unlike solver_dynamic).
+    for (int i = 0; i < Km1; ++i) F.os.push_back(scale);
+  }
+  for (auto& s : o.sols) F.num_steps.push_back(s.num_steps);
+  if (!smoother(P)) {
+    BNormal p0 = o.sol0.post.marg;
+    rescale_cholesky(P, p0, scale);
+    F.u.push_back(p0);
+    for (auto& s : o.sols) {
+      BNormal p = s.post.marg;
+      rescale_cholesky(P, p, scale);
+      F.u.push_back(p);
+    }
+    return;
+  }
+  Post p0 = o.sol0.post, p1 = o.sol1.post;
+  rescale_cholesky(P, p0.marg, scale);
+  rescale_noise(P, p0.cond, scale);
+  rescale_cholesky(P, p1.marg, scale);
+  rescale_noise(P, p1.cond, scale);
+  std::vector<Post> post(Km1);
+  for (int i = 0; i < Km1; ++i) {
+    post[i] = o.sols[i].post;
+    rescale_cholesky(P, post[i].marg, scale);
+    rescale_noise(P, post[i].cond, scale);
+  }
+  BNormal rv_at_t1 = marginalise(P, p1.cond, p1.marg);
+  F.terminal = rv_at_t1;
+  // MarkovSequence.evaluate_marginals (estimators_and_losses.py:156-178).
+  std::vector<BNormal> margs(Km1);
+  BNormal x = rv_at_t1;
+  for (int i = Km1 - 1; i >= 0; --i) {
+    x = marginalise(P, post[i].cond, x);
+    margs[i] = x;
+  }
+  F.u = margs;
+  F.u.push_back(rv_at_t1);
+  F.filtering.push_back(p0.marg);
+  for (int i = 0; i < Km1; ++i) {
+    F.filtering.push_back(post[i].marg);
+    F.conds.push_back(post[i].cond);
+  }
+}
+
+// to_derivative(0, std) observation model (ssm_impl_dense.py:206-217,
+// ssm_impl_isotropic.py:271-278, ssm_impl_blockdiag.py:367-377).
+BCond observation_model(const Problem& P, const double* std_j, int nstd) {
+  BCond out;
+  out.b.resize(P.nblk);
+  const int d = P.d, N = P.N;
+  if (P.cfg.fact == FACT_DENSE) {
+    Cond& C = out.b[0];
+    C.A = Mat(d, P.n);
+    for (int k = 0; k < d; ++k) C.A(k, k) = 1.0;
+    C.noise.m = Mat(d, 1);
+    C.noise.L = Mat(d, d);
+    for (int k = 0; k < d; ++k) C.noise.L(k, k) = std_j[nstd == 1 ? 0 : k];
+    C.tl = ones(P.n);
+    C.to = ones(d);
+  } else if (P.cfg.fact == FACT_ISO) {
+    Cond& C = out.b[0];
+    C.A = Mat(1, N);
+    C.A(0, 0) = 1.0;
+    C.noise.m = Mat(1, d);
+    C.noise.L = Mat(1, 1);
+    C.noise.L(0, 0) = std_j[0];
+    C.tl = ones(N);
+    C.to = ones(1);
+  } else {
+    for (int k = 0; k < d; ++k) {
+      Cond& C = out.b[k];
+      C.A = Mat(1, N);
+      C.A(0, 0) = 1.0;
+      C.noise.m = Mat(1, 1);
+      C.noise.L = Mat(1, 1);
+      C.noise.L(0, 0) = std_j[nstd == 1 ? 0 : k];
+      C.tl = ones(N);
+      C.to = ones(1);
+    }
+  }
+  return out;
+}
+
+// bayes_rule_and_logpdf_tree (ssm_impl_api.py:118-124) with lstsq gains.
+double bayes_logpdf(const Problem& P, const BCond& model, const double* y, const BNormal& rv, BNormal& updated) {
+  BNormal obs;
+  BCond rev;
+  revert(P, model, rv, SOLVE_LSTSQ, obs, rev);
+  double lp = 0.0;
+  updated.b.resize(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) {
+    Mat data(obs.b[i].m.r, obs.b[i].m.c);
+    if (P.cfg.fact == FACT_BLOCKDIAG)
+      data(0, 0) = y[i];
+    else
+      for (size_t e = 0; e < data.a.size(); ++e) data.a[e] = y[e];
+    if (P.cfg.fact == FACT_BLOCKDIAG) {
+      // logpdf_scalar_flat per block (ssm_impl_blockdiag.py:302-316).
+      SSMBase iso;
+      iso.fact = FACT_ISO;
+      lp += iso.logpdf(obs.b[i], data);
+    } else {
+      lp += P.ops.logpdf(obs.b[i], data);
+    }
+    updated.b[i] = P.ops.apply_flat(rev.b[i], data);
+  }
+  return lp;
+}
+
+// MarkovSequence.evaluate_lml (estimators_and_losses.py:180-218) via
+// loss_lml_timeseries (:53-105): data y (K x d), std (K x nstd).
+double evaluate_lml(const Problem& P, const Final& F, const double* y, const double* stds, int nstd, int average) {
+  const int K = (int)F.conds.size() + 1, d = P.d;
+  BNormal upd;
+  BCond m0 = observation_model(P, stds + (size_t)(K - 1) * nstd, nstd);
+  double logpdf = bayes_logpdf(P, m0, y + (size_t)(K - 1) * d, F.terminal, upd);
+  double num = 1.0;
+  BNormal rv = upd;
+  for (int j = K - 2; j >= 0; --j) {
+    BNormal pred = marginalise(P, F.conds[j], rv);
+    BCond mj = observation_model(P, stds + (size_t)j * nstd, nstd);
+    BNormal corrected;
+    double lpn = bayes_logpdf(P, mj, y + (size_t)j * d, pred, corrected);
+    if (average)
+      logpdf = (logpdf * num + lpn) / (num + 1);
+    else
+      logpdf = logpdf + lpn;
+    num += 1;
+    rv = corrected;
+  }
+  return logpdf;
+}
+
+// Prior init normal from Taylor coefficients (tcoeffs K x d, derivatives)
+// and is_exact (std zero) or inexact std.
+BNormal prior_init(const Problem& P, const double* tc, double init_std) {
+  BNormal out;
+  out.b.resize(P.nblk);
+  const int N = P.N, d = P.d;
+  if (P.cfg.fact == FACT_DENSE) {
+    Normal& x = out.b[0];
+    x.m = Mat(P.n, 1);
+    x.L = Mat(P.n, P.n);
+    for (int i = 0; i < N; ++i)
+      for (int k = 0; k < d; ++k) x.m(i * d + k, 0) = tc[i * d + k];
+    for (int i = 0; i < P.n; ++i) x.L(i, i) = init_std;
+  } else if (P.cfg.fact == FACT_ISO) {
+    Normal& x = out.b[0];
+    x.m = Mat(N, d);
+    x.L = Mat(N, N);
+    for (int i = 0; i < N; ++i)
+      for (int k = 0; k < d; ++k) x.m(i, k) = tc[i * d + k];
+    for (int i = 0; i < N; ++i) x.L(i, i) = init_std;
+  } else {
+    for (int k = 0; k < d; ++k) {
+      Normal& x = out.b[k];
+      x.m = Mat(N, 1);
+      x.L = Mat(N, N);
+      for (int i = 0; i < N; ++i) x.m(i, 0) = tc[i * d + k];
+      for (int i = 0; i < N; ++i) x.L(i, i) = init_std;
+    }
+  }
+  return out;
+}
+
+// Flat layouts of a (blocked) normal: dense mean n, chol n*n; isotropic
+// mean N*d, chol N*N; blockdiag mean d*N, chol d*N*N.
+int mean_size(const Problem& P) { return P.cfg.fact == FACT_BLOCKDIAG ? P.d * P.N : P.n * (P.cfg.fact == FACT_ISO ? P.d : 1); }
+int chol_size(const Problem& P) { return P.nblk * P.n * P.n; }
+
+void write_normal(const Problem& P, const BNormal& x, double* mean, double* chol) {
+  if (P.cfg.fact == FACT_BLOCKDIAG) {
+    for (int k = 0; k < P.d; ++k)
+      for (int i = 0; i < P.N; ++i) {
+        if (mean) mean[k * P.N + i] = x.b[k].m(i, 0);
+        if (chol)
+          for (int j = 0; j < P.N; ++j) chol[(size_t)k * P.N * P.N + i * P.N + j] = x.b[k].L(i, j);
+      }
+    return;
+  }
+  if (mean) std::memcpy(mean, x.b[0].m.a.data(), sizeof(double) * x.b[0].m.a.size());
+  if (chol) std::memcpy(chol, x.b[0].L.a.data(), sizeof(double) * x.b[0].L.a.size());
+}
+
+// A conditional's flat layout: A (nblk*n*n), noise mean (mean_size), noise
+// chol (chol_size), to_latent (nblk*n), to_observed (nblk*n).
+void write_cond(const Problem& P, const BCond& c, double* A, double* nm, double* nc, double* tl, double* to) {
+  BNormal noise;
+  noise.b.resize(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) noise.b[i] = c.b[i].noise;
+  write_normal(P, noise, nm, nc);
+  for (int i = 0; i < P.nblk; ++i) {
+    std::memcpy(A + (size_t)i * P.n * P.n, c.b[i].A.a.data(), sizeof(double) * P.n * P.n);
+    std::memcpy(tl + (size_t)i * P.n, c.b[i].tl.data(), sizeof(double) * P.n);
+    std::memcpy(to + (size_t)i * P.n, c.b[i].to.data(), sizeof(double) * P.n);
+  }
+}
+
+}  // namespace
+
+// =============================================================== C API
+extern "C" {
+
+struct pdqo_out {
+  // Caller-allocated; any pointer may be null.
+  double* t;            // K
+  double* mean;         // K * mean_size
+  double* chol;         // K * chol_size
+  double* os;           // K * nblk
+  long* num_steps;      // K-1
+  double* filt_mean;    // smoothers: K * mean_size
+  double* filt_chol;    // K * chol_size
+  double* cond_A;       // smoothers: (K-1) * nblk*n*n
+  double* cond_mean;    // (K-1) * mean_size
+  double* cond_chol;    // (K-1) * chol_size
+  double* cond_tl;      // (K-1) * nblk*n
+  double* cond_to;      // (K-1) * nblk*n
+  double* term_mean;    // smoothers: mean_size
+  double* term_chol;    // chol_size
+  long attempts, accepted;
+  int status;
+};
+
+int pdqo_version(void) { return 1; }
+
+void pdqo_system_matrices(int nu, double* A, double* Q) {
+  Mat A1, Q1;
+  system_matrices_1d_iwp(nu, A1, Q1);
+  std::memcpy(A, A1.a.data(), sizeof(double) * A1.a.size());
+  std::memcpy(Q, Q1.a.data(), sizeof(double) * Q1.a.size());
+}
+
+void pdqo_qr_r(int m, int n, const double* M, double* R) {
+  Mat A(m, n);
+  std::memcpy(A.a.data(), M, sizeof(double) * m * n);
+  Mat r = qr_r(A);
+  std::memcpy(R, r.a.data(), sizeof(double) * r.a.size());
+}
+
+void pdqo_revert_conditional(int n, int k, const double* R_X_F, const double* R_X, const double* R_YX, int mode,
+                             double* R_Y, double* R_XY, double* G) {
+  Mat a(n, k), b(n, n), c(k, k);
+  std::memcpy(a.a.data(), R_X_F, sizeof(double) * n * k);
+  std::memcpy(b.a.data(), R_X, sizeof(double) * n * n);
+  std::memcpy(c.a.data(), R_YX, sizeof(double) * k * k);
+  Reverted r = revert_conditional(a, b, c, mode);
+  std::memcpy(R_Y, r.R_Y.a.data(), sizeof(double) * k * k);
+  std::memcpy(R_XY, r.R_XY.a.data(), sizeof(double) * n * n);
+  std::memcpy(G, r.G.a.data(), sizeof(double) * n * k);
+}
+
+// IWP transition conditional for one block (dense: full n).
+void pdqo_transition(const pdqo_cfg* cfg, double dt, const double* os, double* A, double* noise_chol, double* tl,
+                     double* to) {
+  Problem P;
+  P.setup(*cfg, nullptr);
+  Vec osv(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) osv[i] = os[i];
+  BCond c = transition(P, dt, osv);
+  std::memcpy(A, c.b[0].A.a.data(), sizeof(double) * P.n * P.n);
+  std::memcpy(noise_chol, c.b[0].noise.L.a.data(), sizeof(double) * P.n * P.n);
+  std::memcpy(tl, c.b[0].tl.data(), sizeof(double) * P.n);
+  std::memcpy(to, c.b[0].to.data(), sizeof(double) * P.n);
+}
+
+double pdqo_logpdf(int fact, int rows, int cols, const double* mean, const double* chol, const double* u) {
+  SSMBase ops;
+  ops.fact = fact == FACT_ISO ? FACT_ISO : FACT_DENSE;
+  Normal x;
+  x.m = Mat(rows, cols);
+  x.L = Mat(rows, rows);
+  std::memcpy(x.m.a.data(), mean, sizeof(double) * rows * cols);
+  std::memcpy(x.L.a.data(), chol, sizeof(double) * rows * rows);
+  Mat uu(rows, cols);
+  std::memcpy(uu.a.data(), u, sizeof(double) * rows * cols);
+  return ops.logpdf(x, uu);
+}
+
+double pdqo_control(const pdqo_cfg* cfg, double dt, double* state, double error_power) {
+  Problem P;
+  P.cfg = *cfg;
+  return control_apply(P, dt, *state, error_power);
+}
+
+void pdqo_taylor_coefficients(int vf_id, int d, const double* params, const double* y0, double t0, int num,
+                              double* out) {
+  Vf vf;
+  vf.id = vf_id;
+  vf.d = d;
+  vf.prm = params;
+  taylor_coefficients(vf, y0, t0, num, out);
+}
+
+void pdqo_vf(int vf_id, int d, const double* params, const double* y, double t, double* f, double* J) {
+  Vf vf;
+  vf.id = vf_id;
+  vf.d = d;
+  vf.prm = params;
+  if (f) vf.eval(y, t, f);
+  if (J) vf.jac(y, t, J);
+}
+
+static void write_outputs(const Problem& P, const Outputs& o, const Final& F, pdqo_out* out) {
+  const int K = (int)F.t.size();
+  const int ms = mean_size(P), cs = chol_size(P), nn = P.nblk * P.n * P.n, nv = P.nblk * P.n;
+  for (int i = 0; i < K; ++i) {
+    if (out->t) out->t[i] = F.t[i];
+    write_normal(P, F.u[i], out->mean ? out->mean + (size_t)i * ms : nullptr,
+                 out->chol ? out->chol + (size_t)i * cs : nullptr);
+    if (out->os && i < (int)F.os.size())
+      for (int b = 0; b < P.nblk; ++b) out->os[(size_t)i * P.nblk + b] = F.os[i][b];
+    if (i < K - 1 && out->num_steps) out->num_steps[i] = F.num_steps[i];
+  }
+  if (smoother(P)) {
+    for (int i = 0; i < K; ++i)
+      write_normal(P, F.filtering[i], out->filt_mean ? out->filt_mean + (size_t)i * ms : nullptr,
+                   out->filt_chol ? out->filt_chol + (size_t)i * cs : nullptr);
+    if (out->cond_A)
+      for (int i = 0; i < K - 1; ++i)
+        write_cond(P, F.conds[i], out->cond_A + (size_t)i * nn, out->cond_mean + (size_t)i * ms,
+                   out->cond_chol + (size_t)i * cs, out->cond_tl + (size_t)i * nv, out->cond_to + (size_t)i * nv);
+    write_normal(P, F.terminal, out->term_mean, out->term_chol);
+  }
+  out->attempts = o.attempts;
+  out->accepted = o.accepted;
+  out->status = o.status;
+}
+
+// One adaptive save_at solve.  tcoeffs: (nu+1) x d derivatives.
+int pdqo_solve_save_at(const pdqo_cfg* cfg, const double* params, const double* tcoeffs, double init_std,
+                       const double* save_at, int K, pdqo_out* out) {
+  Problem P;
+  P.setup(*cfg, params);
+  BNormal init = prior_init(P, tcoeffs, init_std);
+  Outputs o;
+  solve_save_at(P, init, save_at, K, o);
+  if (o.status != ST_OK) {
+    out->attempts = o.attempts;
+    out->accepted = o.accepted;
+    out->status = o.status;
+    return o.status;
+  }
+  Final F;
+  finalize(P, o, F);
+  write_outputs(P, o, F, out);
+  return 0;
+}
+
+int pdqo_solve_fixed_grid(const pdqo_cfg* cfg, const double* params, const double* tcoeffs, double init_std,
+                          const double* grid, int K, pdqo_out* out) {
+  Problem P;
+  P.setup(*cfg, params);
+  BNormal init = prior_init(P, tcoeffs, init_std);
+  Outputs o;
+  solve_fixed_grid(P, init, grid, K, o);
+  Final F;
+  finalize(P, o, F);
+  write_outputs(P, o, F, out);
+  return 0;
+}
+
+// Marginal log-likelihood of data (Kobs x d) at every `every`-th grid point
+// (grid[0] included), fixed-point smoother on the fixed grid.
+double pdqo_lml(const pdqo_cfg* cfg, const double* params, const double* tcoeffs, double init_std, const double* grid,
+                int K, int every, const double* data, const double* stds, int nstd, int average) {
+  Problem P;
+  P.setup(*cfg, params);
+  BNormal init = prior_init(P, tcoeffs, init_std);
+  Outputs o;
+  solve_fixed_grid_checkpoints(P, init, grid, K, every, o);
+  Final F;
+  finalize(P, o, F);
+  return evaluate_lml(P, F, data, stds, nstd, average);
+}
+
+// LML as a function of (vector-field params, y0): Taylor coefficients are
+// recomputed from y0 (jet init, num = nu).
+double pdqo_lml_from_y0(const pdqo_cfg* cfg, const double* params, const double* y0, const double* grid, int K,
+                        int every, const double* data, const double* stds, int nstd, int average) {
+  std::vector<double> tc((size_t)(cfg->nu + 1) * cfg->d);
+  pdqo_taylor_coefficients(cfg->vf_id, cfg->d, params, y0, grid[0], cfg->nu, tc.data());
+  return pdqo_lml(cfg, params, tc.data(), 0.0, grid, K, every, data, stds, nstd, average);
+}
+
+// Central finite-difference gradient of pdqo_lml_from_y0 with respect to
+// (params[0:np], y0[0:d]); the plain-path gradient baseline.
+double pdqo_lml_grad_fd(const pdqo_cfg* cfg, const double* params, int np, const double* y0, const double* grid, int K,
+                        int every, const double* data, const double* stds, int nstd, int average, double h,
+                        double* grad) {
+  std::vector<double> p(params, params + np), y(y0, y0 + cfg->d);
+  const double f0 = pdqo_lml_from_y0(cfg, p.data(), y.data(), grid, K, every, data, stds, nstd, average);
+  for (int i = 0; i < np + cfg->d; ++i) {
+    double* x = i < np ? &p[i] : &y[i - np];
+    const double x0 = *x;
+    *x = x0 + h;
+    const double fp = pdqo_lml_from_y0(cfg, p.data(), y.data(), grid, K, every, data, stds, nstd, average);
+    *x = x0 - h;
+    const double fm = pdqo_lml_from_y0(cfg, p.data(), y.data(), grid, K, every, data, stds, nstd, average);
+    *x = x0;
+    grad[i] = (fp - fm) / (2.0 * h);
+  }
+  return f0;
+}
+
+// Save-every-step harness output; K_max bounds the number of rows written.
+int pdqo_solve_save_every_step(const pdqo_cfg* cfg, const double* params, const double* tcoeffs, double init_std,
+                               double t0, double t1, int K_max, pdqo_out* out) {
+  Problem P;
+  P.setup(*cfg, params);
+  BNormal init = prior_init(P, tcoeffs, init_std);
+  Outputs o;
+  solve_save_every_step(P, init, t0, t1, o);
+  if ((int)o.sols.size() + 1 > K_max) return -(int)o.sols.size() - 1;
+  Final F;
+  finalize(P, o, F);
+  write_outputs(P, o, F, out);
+  return (int)F.t.size();
+}
+
+// The LML configuration's forward pass with its Markov-sequence outputs.
+int pdqo_solve_fixed_grid_checkpoints(const pdqo_cfg* cfg, const double* params, const double* tcoeffs,
+                                      double init_std, const double* grid, int K, int every, pdqo_out* out) {
+  Problem P;
+  P.setup(*cfg, params);
+  BNormal init = prior_init(P, tcoeffs, init_std);
+  Outputs o;
+  solve_fixed_grid_checkpoints(P, init, grid, K, every, o);
+  Final F;
+  finalize(P, o, F);
+  write_outputs(P, o, F, out);
+  return (int)F.t.size();
+}
+
+// Batched adaptive solves for the CPU baseline: instance i has its own
+// params, tcoeffs and save_at row; outputs only the terminal mean and the
+// attempt/accept counts.  OpenMP over instances.
+int pdqo_solve_save_at_batch(const pdqo_cfg* cfg, int n_inst, const double* params, int n_params,
+                             const double* tcoeffs, const double* save_at, int K, double* term_mean, long* attempts,
+                             long* accepted, int n_threads) {
+  int err = 0;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+#endif
+  for (int i = 0; i < n_inst; ++i) {
+    Problem P;
+    P.setup(*cfg, params + (size_t)i * n_params);
+    BNormal init = prior_init(P, tcoeffs + (size_t)i * (cfg->nu + 1) * cfg->d, 0.0);
+    Outputs o;
+    solve_save_at(P, init, save_at + (size_t)i * K, K, o);
+    if (o.status != ST_OK) {
+      err |= 1;
+      continue;
+    }
+    Final F;
+    finalize(P, o, F);
+    write_normal(P, F.u.back(), term_mean + (size_t)i * mean_size(P), nullptr);
+    attempts[i] = o.attempts;
+    accepted[i] = o.accepted;
+  }
+  (void)n_threads;
+  return err;
+}
+
+}  // extern "C"

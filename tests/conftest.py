@@ -1,0 +1,27 @@
+"""Shared pytest configuration.
+
+Tests marked ``gpu`` need a HIP device (MI355X, gfx950); everything else runs
+on the CPU.  The CPU oracle under ``oracle/`` is the checker for every parity
+test.
+"""
+
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import oracle as O
+
+    O.build()
+    return O
