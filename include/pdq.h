@@ -1,0 +1,178 @@
+/*
+ * pdq.h -- C ABI of the MI355X (gfx950) probabilistic IVP solver path.
+ *
+ * Every entry point takes device pointers (hipMalloc'ed or torch-owned),
+ * plain sizes and a hipStream_t passed as void*.  Calls are stream-ordered;
+ * no pointer is retained after the call's kernels complete.  Return value 0
+ * means success; a non-zero PDQ_ERR_* code leaves a message readable with
+ * pdq_last_error().  There is no CPU fallback: without a HIP device every
+ * compute entry point returns PDQ_ERR_NO_DEVICE.
+ *
+ * Flat layouts (row-major) follow the reference's flattened normals:
+ *   dense:      mean n = (nu+1)*d (derivative-major), chol n x n
+ *   isotropic:  mean (nu+1) x d,                        chol (nu+1) x (nu+1)
+ *   blockdiag:  mean d x (nu+1),                        chol d x (nu+1) x (nu+1)
+ * (probdiffeq/_probdiffeq/ssm_impl_dense.py:108-129,
+ *  ssm_impl_isotropic.py:146-178, ssm_impl_blockdiag.py:215-253)
+ */
+#ifndef PDQ_H_
+#define PDQ_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* factorisation: state_space_model_{dense,isotropic,blockdiag}
+ * (ssm_impl_dense.py:455, ssm_impl_isotropic.py:406, ssm_impl_blockdiag.py:460) */
+enum { PDQ_DENSE = 0, PDQ_ISOTROPIC = 1, PDQ_BLOCKDIAG = 2 };
+/* constraint_ode_ts0 / constraint_ode_ts1 (ssm_impl_api.py:520-558) */
+enum { PDQ_TS0 = 0, PDQ_TS1 = 1 };
+/* strategy_filter / _smoother_fixedpoint / _smoother_fixedinterval
+ * (estimators_and_losses.py:347, 473, 594) */
+enum { PDQ_FILTER = 0, PDQ_FIXEDPOINT = 1, PDQ_FIXEDINTERVAL = 2 };
+/* solver / solver_dynamic / solver_mle (solvers.py:636, 483, 318) */
+enum { PDQ_SOLVER = 0, PDQ_SOLVER_DYNAMIC = 1, PDQ_SOLVER_MLE = 2 };
+/* error_norm_scale_then_rms / _rms_then_scale (solvers.py:770-811) */
+enum { PDQ_NORM_SCALE_THEN_RMS = 0, PDQ_NORM_RMS_THEN_SCALE = 1 };
+/* control_integral / control_proportional_integral (controllers.py:24-84) */
+enum { PDQ_CONTROL_INTEGRAL = 0, PDQ_CONTROL_PI = 1 };
+/* device vector-field registry (problems.py:283-295 convention u' = f(u, t)) */
+enum {
+  PDQ_VF_LOTKA_VOLTERRA = 0, /* params (a, b, c, d), d = 2 */
+  PDQ_VF_VAN_DER_POL = 1,    /* params (mu), d = 2 */
+  PDQ_VF_BURGERS = 2,        /* params (viscosity, dx), periodic grid of d points */
+  PDQ_VF_LINEAR = 3,         /* params lambda (d): f = lambda * u + t */
+  PDQ_VF_LOGISTIC = 4        /* params (r): f = r u (1 - u) */
+};
+
+/* Per-instance status (the reference has no runtime error channel). */
+enum { PDQ_STATUS_OK = 0, PDQ_STATUS_MAX_ATTEMPTS = 1, PDQ_STATUS_NONFINITE = 2, PDQ_STATUS_DT_UNDERFLOW = 3 };
+
+/* Return codes. */
+enum {
+  PDQ_OK = 0,
+  PDQ_ERR_INVALID = 1,
+  PDQ_ERR_UNSUPPORTED = 2,
+  PDQ_ERR_NO_DEVICE = 3,
+  PDQ_ERR_HIP = 4,
+  PDQ_ERR_INSTANCE = 5 /* at least one instance ended with a non-OK status */
+};
+
+/* POD mirror of the reference's configuration objects.  Defaults of the
+ * reference: safety 0.95, factor_min 0.2, factor_max 10 (controllers.py:
+ * 478), dt0 0.1, eps 1e-8, damp 0 (solvers_via_adaptive_steps.py:101). */
+typedef struct pdq_config {
+  int32_t factorisation;
+  int32_t num_derivatives; /* nu; the prior has nu+1 Taylor coefficients */
+  int32_t ode_dim;         /* d */
+  int32_t constraint;
+  int32_t strategy;
+  int32_t solver;
+  int32_t error_norm;
+  int32_t control;
+  int32_t vector_field;
+  int32_t n_params;        /* vector-field parameters per instance */
+  int32_t clip_dt;
+  int32_t max_attempts;    /* per accepted step; 0 = unbounded (reference) */
+  double rtol, atol, dt0, eps, damp;
+  double safety, factor_min, factor_max, exponent_integral, exponent_proportional;
+  double base_output_scale;   /* prior_wiener_integrated(output_scale=...) */
+  double init_std;            /* 0: is_exact=True; else inexact_eps */
+  int32_t correct_asymptotic_underconfidence; /* solver_mle */
+  int32_t lanes_per_instance; /* 0: autotuned; else 1..64 (power of two) */
+} pdq_config;
+
+/* Caller-allocated device outputs; NULL members are not written.
+ * K = number of save_at points (or grid points). */
+typedef struct pdq_solution {
+  double* t;            /* n_inst x K */
+  double* mean;         /* n_inst x K x mean_size      (u.mean) */
+  double* chol;         /* n_inst x K x chol_size      (u.cholesky) */
+  double* output_scale; /* n_inst x K x nblk (solver_dynamic; K-1 rows otherwise) */
+  int64_t* num_steps;   /* n_inst x (K-1) accepted steps up to each point */
+  /* smoothers: solution_full (SmoothingSolution, estimators_and_losses.py:277) */
+  double* filt_mean;    /* n_inst x K x mean_size */
+  double* filt_chol;    /* n_inst x K x chol_size */
+  double* cond_A;       /* n_inst x (K-1) x nblk*n*n  backward conditionals */
+  double* cond_mean;    /* n_inst x (K-1) x mean_size */
+  double* cond_chol;    /* n_inst x (K-1) x chol_size */
+  double* cond_tl;      /* n_inst x (K-1) x nblk*n    to_latent */
+  double* cond_to;      /* n_inst x (K-1) x nblk*n    to_observed */
+  double* term_mean;    /* n_inst x mean_size         full_posterior.marginal */
+  double* term_chol;    /* n_inst x chol_size */
+  /* statistics (the reference counts accepted steps only) */
+  int64_t* attempts;    /* n_inst */
+  int64_t* accepted;    /* n_inst */
+  int32_t* status;      /* n_inst, PDQ_STATUS_* */
+} pdq_solution;
+
+int32_t pdq_version(void);
+
+/* Initialise the library on the current device (idempotent). */
+int32_t pdq_init(void);
+
+/* Copies the last error message of the calling thread into buf. */
+size_t pdq_last_error(char* buf, size_t size);
+
+/* Replaces ivpsolve.solve_adaptive_save_at(solver=, error=, control=,
+ * clip_dt=)(prior, save_at, atol, rtol, dt0, eps, damp)
+ * (probdiffeq/_ivpsolve/solvers_via_adaptive_steps.py:46-148), batched over
+ * n_inst instances (the reference batches with jax.vmap).
+ *   params:  n_inst x cfg->n_params
+ *   tcoeffs: n_inst x (nu+1) x d  Taylor coefficients (prior mean)
+ *   save_at: n_inst x K           increasing checkpoints, save_at[0] = t0
+ * Each instance owns its accept/reject loop inside one persistent launch. */
+int32_t pdq_solve_adaptive_save_at(const pdq_config* cfg, int32_t n_inst, const double* params,
+                                   const double* tcoeffs, const double* save_at, int32_t K,
+                                   const pdq_solution* out, void* stream);
+
+/* Replaces ivpsolve.solve_fixed_grid(solver=)(prior, grid=, damp=)
+ * (probdiffeq/_ivpsolve/solvers_via_fixed_steps.py:11-34).
+ *   grid: n_inst x K */
+int32_t pdq_solve_fixed_grid(const pdq_config* cfg, int32_t n_inst, const double* params,
+                             const double* tcoeffs, const double* grid, int32_t K,
+                             const pdq_solution* out, void* stream);
+
+/* Replaces the parameter-to-loss map of examples/E2_parameter_estimation_optax.py
+ * (loss_lml_timeseries, estimators_and_losses.py:53-105, evaluated on the
+ * fixed-point smoother's Markov sequence, :473-591) together with its
+ * gradient (jax.value_and_grad in the example).
+ * The prior mean is the Taylor expansion of y0 (jetexpand, num = nu); the
+ * forward pass steps the fixed grid and checkpoints every `every` points.
+ *   params: n_inst x n_params, y0: n_inst x d, grid: K_grid (shared),
+ *   data:   n_inst x K_obs x d  (K_obs = (K_grid-1)/every + 1)
+ *   stds:   n_inst x K_obs x n_std (n_std = 1 or the per-block count)
+ *   value:  n_inst, grad: n_inst x (n_params + d) (may be NULL) */
+int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const double* params,
+                               const double* y0, const double* grid, int32_t K_grid, int32_t every,
+                               const double* data, const double* stds, int32_t n_std,
+                               int32_t average_pdfs, double* value, double* grad, void* stream);
+
+/* Replaces jetexpand_ode_padded_scan(num=nu)(vf, (y0,), t=t0)
+ * (probdiffeq/_probdiffeq/jet_expansion_algorithms.py:49-103) for the
+ * registered vector fields.  out: n_inst x (num+1) x d derivatives. */
+int32_t pdq_taylor_coefficients(int32_t vector_field, int32_t d, int32_t n_params, int32_t n_inst,
+                                const double* params, const double* y0, double t0, int32_t num,
+                                double* out, void* stream);
+
+/* Sizes of the flat layouts above for a configuration. */
+int32_t pdq_mean_size(const pdq_config* cfg);
+int32_t pdq_chol_size(const pdq_config* cfg);
+
+/* Lanes per instance the autotuner picked for this configuration and batch
+ * size (0 if not yet tuned). */
+int32_t pdq_tuned_lanes(const pdq_config* cfg, int32_t n_inst);
+
+/* Records the lanes-per-instance a timing sweep picked for this
+ * configuration, batch-size bucket and mode (0 save_at, 1 fixed grid,
+ * 2 LML); later calls with lanes_per_instance = 0 use it. */
+int32_t pdq_set_tuned_lanes(const pdq_config* cfg, int32_t n_inst, int32_t mode, int32_t lanes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDQ_H_ */

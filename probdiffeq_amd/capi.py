@@ -1,0 +1,334 @@
+"""ctypes binding of libpdq.so (the C ABI in include/pdq.h).
+
+Device memory and streams come from PyTorch-ROCm: every pointer passed here
+is a ``torch.Tensor.data_ptr()`` of a CUDA(HIP) tensor.  There is no CPU
+path: without the built library or without a HIP device every call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libpdq.so")
+
+FACT = {"dense": 0, "isotropic": 1, "blockdiag": 2}
+CONSTRAINT = {"ts0": 0, "ts1": 1}
+STRATEGY = {"filter": 0, "fixedpoint": 1, "fixedinterval": 2}
+SOLVER = {"solver": 0, "dynamic": 1, "mle": 2}
+ERROR_NORM = {"scale_then_rms": 0, "rms_then_scale": 1}
+CONTROL = {"integral": 0, "proportional_integral": 1}
+VF = {"lotka_volterra": 0, "van_der_pol": 1, "burgers": 2, "linear": 3, "logistic": 4}
+STATUS = {0: "ok", 1: "max_attempts", 2: "nonfinite", 3: "dt_underflow"}
+
+ERR_NO_DEVICE = 3
+
+
+class PdqError(RuntimeError):
+    """A C-ABI call failed (the message comes from pdq_last_error)."""
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("factorisation", ctypes.c_int32),
+        ("num_derivatives", ctypes.c_int32),
+        ("ode_dim", ctypes.c_int32),
+        ("constraint", ctypes.c_int32),
+        ("strategy", ctypes.c_int32),
+        ("solver", ctypes.c_int32),
+        ("error_norm", ctypes.c_int32),
+        ("control", ctypes.c_int32),
+        ("vector_field", ctypes.c_int32),
+        ("n_params", ctypes.c_int32),
+        ("clip_dt", ctypes.c_int32),
+        ("max_attempts", ctypes.c_int32),
+        ("rtol", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("dt0", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("damp", ctypes.c_double),
+        ("safety", ctypes.c_double),
+        ("factor_min", ctypes.c_double),
+        ("factor_max", ctypes.c_double),
+        ("exponent_integral", ctypes.c_double),
+        ("exponent_proportional", ctypes.c_double),
+        ("base_output_scale", ctypes.c_double),
+        ("init_std", ctypes.c_double),
+        ("correct_asymptotic_underconfidence", ctypes.c_int32),
+        ("lanes_per_instance", ctypes.c_int32),
+    ]
+
+
+_VP = ctypes.c_void_p
+
+
+class _Solution(ctypes.Structure):
+    _fields_ = [
+        (name, _VP)
+        for name in [
+            "t",
+            "mean",
+            "chol",
+            "output_scale",
+            "num_steps",
+            "filt_mean",
+            "filt_chol",
+            "cond_A",
+            "cond_mean",
+            "cond_chol",
+            "cond_tl",
+            "cond_to",
+            "term_mean",
+            "term_chol",
+            "attempts",
+            "accepted",
+            "status",
+        ]
+    ]
+
+
+@dataclasses.dataclass
+class Config:
+    """Flat configuration mirroring the reference's solver objects."""
+
+    factorisation: str = "dense"
+    num_derivatives: int = 4
+    ode_dim: int = 2
+    constraint: str = "ts0"
+    strategy: str = "filter"
+    solver: str = "dynamic"
+    error_norm: str = "scale_then_rms"
+    control: str = "integral"
+    vector_field: str = "lotka_volterra"
+    n_params: int = 4
+    clip_dt: bool = False
+    max_attempts: int = 0
+    rtol: float = 1e-6
+    atol: float = 1e-8
+    dt0: float = 0.1
+    eps: float = 1e-8
+    damp: float = 0.0
+    safety: float = 0.95
+    factor_min: float = 0.2
+    factor_max: float = 10.0
+    exponent_integral: float = 0.3
+    exponent_proportional: float = 0.4
+    base_output_scale: float = 1.0
+    init_std: float = 0.0
+    correct_asymptotic_underconfidence: bool = True
+    lanes_per_instance: int = 0
+
+    def c(self) -> _Config:
+        return _Config(
+            FACT[self.factorisation],
+            self.num_derivatives,
+            self.ode_dim,
+            CONSTRAINT[self.constraint],
+            STRATEGY[self.strategy],
+            SOLVER[self.solver],
+            ERROR_NORM[self.error_norm],
+            CONTROL[self.control],
+            VF[self.vector_field],
+            self.n_params,
+            int(self.clip_dt),
+            self.max_attempts,
+            self.rtol,
+            self.atol,
+            self.dt0,
+            self.eps,
+            self.damp,
+            self.safety,
+            self.factor_min,
+            self.factor_max,
+            self.exponent_integral,
+            self.exponent_proportional,
+            self.base_output_scale,
+            self.init_std,
+            int(self.correct_asymptotic_underconfidence),
+            self.lanes_per_instance,
+        )
+
+    @property
+    def N(self) -> int:
+        return self.num_derivatives + 1
+
+    @property
+    def nblk(self) -> int:
+        return self.ode_dim if self.factorisation == "blockdiag" else 1
+
+    @property
+    def n(self) -> int:
+        return self.N * self.ode_dim if self.factorisation == "dense" else self.N
+
+    @property
+    def mean_size(self) -> int:
+        return self.N * self.ode_dim
+
+    @property
+    def chol_size(self) -> int:
+        return self.nblk * self.n * self.n
+
+
+# Symbols the header declares; the CPU test suite checks that all exist.
+EXPORTED = [
+    "pdq_version",
+    "pdq_init",
+    "pdq_last_error",
+    "pdq_solve_adaptive_save_at",
+    "pdq_solve_fixed_grid",
+    "pdq_lml_value_and_grad",
+    "pdq_taylor_coefficients",
+    "pdq_mean_size",
+    "pdq_chol_size",
+    "pdq_tuned_lanes",
+    "pdq_set_tuned_lanes",
+]
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libpdq.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PdqError(f"{LIB_PATH} is missing: build it with probdiffeq_amd.build.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name in EXPORTED:
+            getattr(L, name).restype = ctypes.c_int32
+        L.pdq_last_error.restype = ctypes.c_size_t
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib().pdq_last_error(buf, 1024)
+    return buf.value.decode()
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise PdqError(f"pdq error {rc}: {last_error()}")
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise PdqError("pdq takes device tensors (HIP); got a CPU tensor")
+    if not t.is_contiguous():
+        raise PdqError("pdq takes contiguous tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def require_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise PdqError("no HIP device: the solver path runs on MI355X (gfx950) only; there is no CPU fallback")
+    return torch.device("cuda")
+
+
+def alloc_solution(cfg: Config, n_inst: int, K: int, device=None, *, smoother_storage: bool | None = None):
+    """Allocate the output tensors of one batched solve (see pdq_solution)."""
+    dev = device or require_device()
+    f64 = dict(dtype=torch.float64, device=dev)
+    ms, cs, nn, nv = cfg.mean_size, cfg.chol_size, cfg.nblk * cfg.n * cfg.n, cfg.nblk * cfg.n
+    out = {
+        "t": torch.zeros(n_inst, K, **f64),
+        "mean": torch.zeros(n_inst, K, ms, **f64),
+        "chol": torch.zeros(n_inst, K, cs, **f64),
+        "output_scale": torch.zeros(n_inst, K, cfg.nblk, **f64),
+        "num_steps": torch.zeros(n_inst, K - 1, dtype=torch.int64, device=dev),
+        "attempts": torch.zeros(n_inst, dtype=torch.int64, device=dev),
+        "accepted": torch.zeros(n_inst, dtype=torch.int64, device=dev),
+        "status": torch.zeros(n_inst, dtype=torch.int32, device=dev),
+    }
+    smoother = cfg.strategy != "filter" if smoother_storage is None else smoother_storage
+    if smoother:
+        out.update(
+            filt_mean=torch.zeros(n_inst, K, ms, **f64),
+            filt_chol=torch.zeros(n_inst, K, cs, **f64),
+            cond_A=torch.zeros(n_inst, K - 1, nn, **f64),
+            cond_mean=torch.zeros(n_inst, K - 1, ms, **f64),
+            cond_chol=torch.zeros(n_inst, K - 1, cs, **f64),
+            cond_tl=torch.zeros(n_inst, K - 1, nv, **f64),
+            cond_to=torch.zeros(n_inst, K - 1, nv, **f64),
+            term_mean=torch.zeros(n_inst, ms, **f64),
+            term_chol=torch.zeros(n_inst, cs, **f64),
+        )
+    return out
+
+
+def _solution_struct(out: dict) -> _Solution:
+    s = _Solution()
+    for name, _ in _Solution._fields_:
+        t = out.get(name)
+        setattr(s, name, _ptr(t).value if t is not None else None)
+    return s
+
+
+def solve(cfg: Config, params, tcoeffs, times, *, fixed_grid=False, out=None, stream=None):
+    """Batched solve; params (B, P), tcoeffs (B, N, d), times (B, K) device tensors."""
+    n_inst, K = times.shape
+    if out is None:
+        out = alloc_solution(cfg, n_inst, K, times.device)
+    c = cfg.c()
+    s = _solution_struct(out)
+    fn = lib().pdq_solve_fixed_grid if fixed_grid else lib().pdq_solve_adaptive_save_at
+    check(fn(ctypes.byref(c), n_inst, _ptr(params), _ptr(tcoeffs), _ptr(times), K, ctypes.byref(s), _stream(stream)))
+    return out
+
+
+def lml_value_and_grad(cfg: Config, params, y0, grid, every, data, stds, *, average=True, grad=True, stream=None):
+    """Batched marginal log-likelihood (and its gradient w.r.t. params and y0)."""
+    n_inst = params.shape[0]
+    dev = params.device
+    value = torch.zeros(n_inst, dtype=torch.float64, device=dev)
+    g = torch.zeros(n_inst, cfg.n_params + cfg.ode_dim, dtype=torch.float64, device=dev) if grad else None
+    n_std = stds.shape[-1] if stds.dim() == 3 else 1
+    c = cfg.c()
+    check(
+        lib().pdq_lml_value_and_grad(
+            ctypes.byref(c),
+            n_inst,
+            _ptr(params),
+            _ptr(y0),
+            _ptr(grid),
+            grid.shape[0],
+            every,
+            _ptr(data),
+            _ptr(stds),
+            n_std,
+            int(average),
+            _ptr(value),
+            _ptr(g),
+            _stream(stream),
+        )
+    )
+    return value, g
+
+
+def taylor_coefficients(vector_field: str, params, y0, t0: float, num: int, stream=None):
+    """Initial Taylor coefficients (derivatives) of the registered vector field."""
+    n_inst, d = y0.shape
+    out = torch.zeros(n_inst, num + 1, d, dtype=torch.float64, device=y0.device)
+    check(
+        lib().pdq_taylor_coefficients(
+            VF[vector_field], d, params.shape[1], n_inst, _ptr(params), _ptr(y0), ctypes.c_double(t0), num, _ptr(out), _stream(stream)
+        )
+    )
+    return out
+
+
+def set_tuned_lanes(cfg: Config, n_inst: int, lanes: int, mode: int = 0) -> None:
+    c = cfg.c()
+    check(lib().pdq_set_tuned_lanes(ctypes.byref(c), n_inst, mode, lanes))

@@ -1,0 +1,69 @@
+// Kernel argument blocks and host-side launch entry points shared by the
+// translation units of libpdq.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/pdq.h"
+#include "pdq_layout.h"
+
+namespace pdq {
+
+struct KArgs {
+  pdq_config cfg;
+  Layout lay;
+  int n_inst;
+  int K;              // save_at / grid points
+  int mode;           // MODE_SAVE_AT, MODE_FIXED_GRID, MODE_LML
+  int every;          // MODE_LML: checkpoint stride on the grid
+  int n_tan;          // MODE_LML: tangent directions (0: no gradient)
+  int average_pdfs;
+  int n_std;
+  const double* params;
+  const double* tcoeffs;  // or y0 in MODE_LML
+  const double* times;    // save_at / grid (n_inst x K); MODE_LML: shared grid (K)
+  const double* data;
+  const double* stds;
+  const double* A;        // nc x nc IWP transition (latent)
+  const double* Q;        // nc x nc IWP noise factor (latent)
+  pdq_solution out;
+  double* lml_value;
+  double* lml_grad;
+  double* scratch;        // MODE_LML: per-instance checkpoint storage
+  int* queue;             // work-queue counter (zeroed before launch)
+};
+
+enum { MODE_SAVE_AT = 0, MODE_FIXED_GRID = 1, MODE_LML = 2 };
+
+struct BArgs {
+  pdq_config cfg;
+  int n_inst, K, mode, nblk_threads;
+  const double* params;
+  const double* tcoeffs;
+  const double* times;
+  const double* A1;  // (nu+1)^2 latent transition
+  const double* Q1;  // (nu+1)^2 latent noise factor
+  pdq_solution out;
+  int* queue;
+};
+
+enum { BMODE_SAVE_AT = 0, BMODE_FIXED_GRID = 1 };
+
+// Host helpers (pdq_capi.hip).
+int host_error(int code, const char* fmt, ...);
+int num_cus();
+
+// Launchers (one per translation unit, compiled in parallel).
+int launch_coop(KArgs& a, int G, hipStream_t st);
+int launch_bd(BArgs& a, hipStream_t st);
+int launch_taylor_tangents(int vf, int d, int np, int n_inst, int num, int ntan, const double* params,
+                           const double* y0, const double* t0, double* out, hipStream_t st);
+size_t coop_lds_bytes(const Layout& L, int G);
+
+}  // namespace pdq
+
+#define PDQ_HIP_TRY(expr)                                                                               \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess) return ::pdq::host_error(PDQ_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)

@@ -1,0 +1,475 @@
+// C ABI of the gfx950 solver path (declared in include/pdq.h).
+//
+// Host responsibilities: validate the configuration, build the IWP system
+// matrices once per (nu, d, factorisation, base scale), size the LDS layout,
+// pick lanes-per-instance (autotuned or heuristic), zero the work-queue
+// counter with a kernel (no memset node, so launches stay capturable) and
+// launch the persistent kernels.  No compute ever falls back to the CPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "pdq_args.h"
+#include "pdq_device.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+}  // namespace
+
+namespace pdq {
+int host_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int num_cus() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 256;
+  return v;
+}
+}  // namespace pdq
+
+namespace {
+
+int set_err(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return set_err(PDQ_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------- IWP system matrices
+// utilities.py:57-71: A = flip(Pascal) (binomials C(nu-i, nu-j)), Q = lower
+// Cholesky factor of the flipped Hilbert matrix 1/(2nu+1-i-j) with positive
+// diagonal.  Q is formed as in the reference: Kahan's recurrence for the
+// Hilbert Cholesky factor (cholesky_util.py:106-176), row flip, one
+// Householder triangularisation and a sign fix.
+double binom(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  double r = 1.0;
+  for (int i = 1; i <= k; ++i) r = r * (double)(n - k + i) / (double)i;
+  return r;
+}
+
+void iwp_matrices(int nu, std::vector<double>& A, std::vector<double>& Q) {
+  const int n = nu + 1;
+  A.assign(n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int j = i; j < n; ++j) A[i * n + j] = binom(nu - i, nu - j);
+  // Kahan: U upper with H = U^T U ... columns scaled by sqrt(2i+1) f_j
+  std::vector<double> f(n, 1.0), U(n * n, 0.0);
+  for (int k = 1; k < n; ++k) f[k] = (((f[k - 1] / k) * (2.0 * k)) / k) * (2.0 * k + 1.0);
+  for (int i = 0; i < n; ++i) U[i * n + i] = 1.0;
+  for (int j = 1; j < n; ++j)
+    for (int i = j - 1; i >= 0; --i) U[i * n + j] = (U[(i + 1) * n + j] / (double)(j - i)) * (double)(i + j + 2);
+  std::vector<double> Lh(n * n, 0.0);  // lower factor = U^T scaled
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double u = U[i * n + j] * (std::sqrt(2.0 * i + 1.0) * (1.0 / f[j]));
+      if (j >= i) Lh[j * n + i] = u;
+    }
+  // M = (row-flipped Lh)^T, triangularise, transpose back
+  std::vector<double> M(n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) M[i * n + j] = Lh[(n - 1 - j) * n + i];
+  for (int i = 0; i < n - 1; ++i) {
+    double ss = 0.0;
+    for (int r = i + 1; r < n; ++r) ss += M[r * n + i] * M[r * n + i];
+    if (ss == 0.0) continue;
+    const double alpha = M[i * n + i];
+    const double beta = -std::copysign(std::hypot(alpha, std::sqrt(ss)), alpha);
+    const double tau = (beta - alpha) / beta, scal = 1.0 / (alpha - beta);
+    for (int r = i + 1; r < n; ++r) M[r * n + i] *= scal;
+    for (int j = i + 1; j < n; ++j) {
+      double w = M[i * n + j];
+      for (int r = i + 1; r < n; ++r) w += M[r * n + i] * M[r * n + j];
+      w *= tau;
+      M[i * n + j] -= w;
+      for (int r = i + 1; r < n; ++r) M[r * n + j] -= M[r * n + i] * w;
+    }
+    M[i * n + i] = beta;
+  }
+  Q.assign(n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) Q[i * n + j] = M[j * n + i];
+  for (int j = 0; j < n; ++j) {
+    const double s = Q[j * n + j] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) Q[i * n + j] *= s;
+  }
+}
+
+struct DevMats {
+  double* A = nullptr;
+  double* Q = nullptr;
+};
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, int, int, double, int>, DevMats> g_mats;  // (dev, fact, nu, d, base, kind)
+std::map<std::string, int> g_tuned;
+
+int device_mats(const pdq_config& cfg, bool latent_1d, DevMats& out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, latent_1d ? -1 : cfg.factorisation, cfg.num_derivatives,
+                                   latent_1d ? 0 : cfg.ode_dim, latent_1d ? 0.0 : cfg.base_output_scale, 0);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_mats.find(key);
+  if (it != g_mats.end()) {
+    out = it->second;
+    return PDQ_OK;
+  }
+  std::vector<double> A1, Q1;
+  iwp_matrices(cfg.num_derivatives, A1, Q1);
+  const int N = cfg.num_derivatives + 1, d = cfg.ode_dim;
+  std::vector<double> A, Q;
+  if (!latent_1d && cfg.factorisation == PDQ_DENSE) {
+    const int n = N * d;
+    A.assign(n * n, 0.0);
+    Q.assign(n * n, 0.0);
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j)
+        for (int k = 0; k < d; ++k) {
+          A[(i * d + k) * n + j * d + k] = A1[i * N + j];
+          Q[(i * d + k) * n + j * d + k] = Q1[i * N + j] * cfg.base_output_scale;  // kron(Q1, Lambda)
+        }
+  } else {
+    A = A1;
+    Q = Q1;
+  }
+  DevMats m;
+  HIP_TRY(hipMalloc(&m.A, A.size() * sizeof(double)));
+  HIP_TRY(hipMalloc(&m.Q, Q.size() * sizeof(double)));
+  HIP_TRY(hipMemcpy(m.A, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(m.Q, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice));
+  g_mats[key] = m;
+  out = m;
+  return PDQ_OK;
+}
+
+int check_common(const pdq_config* cfg, int n_inst, int K) {
+  if (!cfg) return set_err(PDQ_ERR_INVALID, "cfg is NULL");
+  if (n_inst < 0) return set_err(PDQ_ERR_INVALID, "n_inst < 0");
+  if (K < 2) return set_err(PDQ_ERR_INVALID, "need at least two time points (got %d)", K);
+  if (cfg->num_derivatives < 1 || cfg->num_derivatives > 7)
+    return set_err(PDQ_ERR_UNSUPPORTED, "num_derivatives must be in [1, 7] (got %d)", cfg->num_derivatives);
+  if (cfg->ode_dim < 1) return set_err(PDQ_ERR_INVALID, "ode_dim must be positive");
+  if (cfg->n_params < 0 || cfg->n_params > pdq::kMaxParams) return set_err(PDQ_ERR_INVALID, "n_params out of range");
+  const int vf = cfg->vector_field;
+  const int d = cfg->ode_dim;
+  if ((vf == PDQ_VF_LOTKA_VOLTERRA || vf == PDQ_VF_VAN_DER_POL) && d != 2)
+    return set_err(PDQ_ERR_INVALID, "this vector field needs ode_dim == 2");
+  if (vf == PDQ_VF_LOTKA_VOLTERRA && cfg->n_params < 4) return set_err(PDQ_ERR_INVALID, "lotka_volterra needs 4 params");
+  if (vf == PDQ_VF_VAN_DER_POL && cfg->n_params < 1) return set_err(PDQ_ERR_INVALID, "van_der_pol needs 1 param");
+  if (vf == PDQ_VF_BURGERS && cfg->n_params < 2) return set_err(PDQ_ERR_INVALID, "burgers needs (viscosity, dx)");
+  if (vf == PDQ_VF_LINEAR && cfg->n_params < d) return set_err(PDQ_ERR_INVALID, "linear needs d params");
+  if (vf == PDQ_VF_LOGISTIC && cfg->n_params < 1) return set_err(PDQ_ERR_INVALID, "logistic needs 1 param");
+  if (vf < 0 || vf > PDQ_VF_LOGISTIC) return set_err(PDQ_ERR_UNSUPPORTED, "unknown vector field %d", vf);
+  if (cfg->strategy < 0 || cfg->strategy > 2 || cfg->solver < 0 || cfg->solver > 2 || cfg->constraint < 0 ||
+      cfg->constraint > 1 || cfg->factorisation < 0 || cfg->factorisation > 2)
+    return set_err(PDQ_ERR_INVALID, "invalid enum value in configuration");
+  if (cfg->constraint == PDQ_TS1 && cfg->factorisation != PDQ_DENSE)
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "ts1 is implemented for the dense factorisation only (isotropic/blockdiag ts1 use Hutchinson "
+                   "estimators, jacobians.py:116-271)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
+  return PDQ_OK;
+}
+
+int lds_limit() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
+  return v;
+}
+
+std::string tune_key(const pdq_config& c, int n_inst, int mode) {
+  int bucket = 1;
+  while (bucket < n_inst) bucket <<= 1;
+  char buf[256];
+  snprintf(buf, sizeof(buf), "%d/%d/%d/%d/%d/%d/%d/%d/%d", mode, c.factorisation, c.num_derivatives, c.ode_dim,
+           c.strategy, c.solver, c.constraint, c.vector_field, bucket);
+  return buf;
+}
+
+// Heuristic lanes per instance: enough lanes that the batch fills the chip
+// (>= 4 wavefronts per CU) without giving a group more lanes than its
+// largest matrix dimension can use.
+int heuristic_lanes(const pdq::Layout& L, int n_inst) {
+  const int cus = pdq::num_cus();
+  int G = L.nc >= 16 ? 64 : (L.nc >= 8 ? 32 : 16);
+  while (G > 4 && (long)n_inst * G / 64 > 4L * cus * 4) G /= 2;
+  while (G < 64 && (long)n_inst * G / 64 < 4L * cus) G *= 2;
+  if (n_inst <= 64 / G) G = 64;
+  const int limit = lds_limit();
+  while (G < 64 && pdq::coop_lds_bytes(L, G) > (size_t)limit) G *= 2;
+  return G;
+}
+
+int pick_lanes(const pdq_config& cfg, const pdq::Layout& L, int n_inst, int mode) {
+  if (cfg.lanes_per_instance > 0) return cfg.lanes_per_instance;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tuned.find(tune_key(cfg, n_inst, mode));
+    if (it != g_tuned.end()) return it->second;
+  }
+  return heuristic_lanes(L, n_inst);
+}
+
+int solve_common(const pdq_config* cfg, int n_inst, const double* params, const double* tcoeffs,
+                 const double* times, int K, const pdq_solution* out, void* stream, int mode) {
+  int rc = check_common(cfg, n_inst, K);
+  if (rc) return rc;
+  if (!out) return set_err(PDQ_ERR_INVALID, "out is NULL");
+  if (n_inst == 0) return PDQ_OK;
+  if (!tcoeffs || !times || (cfg->n_params > 0 && !params))
+    return set_err(PDQ_ERR_INVALID, "tcoeffs, times and params must be device pointers");
+  hipStream_t st = (hipStream_t)stream;
+  int* queue = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&queue, sizeof(int), st));
+  if (cfg->factorisation == PDQ_BLOCKDIAG) {
+    if (cfg->strategy != PDQ_FILTER) {
+      (void)hipFreeAsync(queue, st);
+      return set_err(PDQ_ERR_UNSUPPORTED, "blockdiag is implemented for strategy_filter");
+    }
+    if (cfg->solver == PDQ_SOLVER_MLE || cfg->error_norm != PDQ_NORM_SCALE_THEN_RMS) {
+      (void)hipFreeAsync(queue, st);
+      return set_err(PDQ_ERR_UNSUPPORTED, "blockdiag supports solver/solver_dynamic with error_norm_scale_then_rms");
+    }
+    DevMats m;
+    rc = device_mats(*cfg, true, m);
+    if (rc) return rc;
+    pdq::BArgs a{};
+    a.cfg = *cfg;
+    a.n_inst = n_inst;
+    a.K = K;
+    a.mode = mode == pdq::MODE_FIXED_GRID ? pdq::BMODE_FIXED_GRID : pdq::BMODE_SAVE_AT;
+    a.params = params;
+    a.tcoeffs = tcoeffs;
+    a.times = times;
+    a.A1 = m.A;
+    a.Q1 = m.Q;
+    a.out = *out;
+    a.queue = queue;
+    rc = pdq::launch_bd(a, st);
+    (void)hipFreeAsync(queue, st);
+    return rc;
+  }
+  pdq::Layout L = pdq::make_layout(*cfg, 0);
+  if (L.nc > 32) {
+    (void)hipFreeAsync(queue, st);
+    return set_err(PDQ_ERR_UNSUPPORTED, "dense state dimension %d exceeds 32", L.nc);
+  }
+  if (L.smoother && (!out->filt_mean || !out->filt_chol || !out->cond_A || !out->cond_mean || !out->cond_chol ||
+                     !out->cond_tl || !out->cond_to)) {
+    (void)hipFreeAsync(queue, st);
+    return set_err(PDQ_ERR_INVALID, "smoothers need filt_* and cond_* outputs (backward pass storage)");
+  }
+  if (cfg->solver == PDQ_SOLVER_MLE && !out->num_steps) {
+    (void)hipFreeAsync(queue, st);
+    return set_err(PDQ_ERR_INVALID, "solver_mle needs the num_steps output");
+  }
+  DevMats m;
+  rc = device_mats(*cfg, false, m);
+  if (rc) return rc;
+  pdq::KArgs a{};
+  a.cfg = *cfg;
+  a.lay = L;
+  a.n_inst = n_inst;
+  a.K = K;
+  a.mode = mode;
+  a.params = params;
+  a.tcoeffs = tcoeffs;
+  a.times = times;
+  a.A = m.A;
+  a.Q = m.Q;
+  a.out = *out;
+  a.queue = queue;
+  const int G = pick_lanes(*cfg, L, n_inst, mode);
+  rc = pdq::launch_coop(a, G, st);
+  (void)hipFreeAsync(queue, st);
+  return rc;
+}
+
+__global__ void taylor_kernel(int vf, int d, int np, int n_inst, const double* params, const double* y0, double t0,
+                              int num, double* out) {
+  const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+  if (inst >= n_inst) return;
+  double c[8 * 64];
+  const double* prm = params + (size_t)inst * np;
+  for (int j = 0; j < d; ++j) c[j] = y0[(size_t)inst * d + j];
+  for (int k = 0; k < num; ++k)
+    for (int j = 0; j < d; ++j) c[(k + 1) * d + j] = pdq::taylor_rhs(vf, d, c, prm, t0, k, j) / (double)(k + 1);
+  for (int k = 0; k <= num; ++k)
+    for (int j = 0; j < d; ++j) out[((size_t)inst * (num + 1) + k) * d + j] = pdq::factorial_d(k) * c[k * d + j];
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t pdq_version(void) { return 1; }
+
+int32_t pdq_init(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
+  return PDQ_OK;
+}
+
+size_t pdq_last_error(char* buf, size_t size) {
+  if (buf && size) {
+    std::strncpy(buf, g_err.c_str(), size - 1);
+    buf[size - 1] = 0;
+  }
+  return g_err.size();
+}
+
+int32_t pdq_mean_size(const pdq_config* cfg) { return (cfg->num_derivatives + 1) * cfg->ode_dim; }
+
+int32_t pdq_chol_size(const pdq_config* cfg) {
+  const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
+  if (cfg->factorisation == PDQ_DENSE) return N * d * N * d;
+  if (cfg->factorisation == PDQ_ISOTROPIC) return N * N;
+  return d * N * N;
+}
+
+int32_t pdq_solve_adaptive_save_at(const pdq_config* cfg, int32_t n_inst, const double* params,
+                                   const double* tcoeffs, const double* save_at, int32_t K,
+                                   const pdq_solution* out, void* stream) {
+  return solve_common(cfg, n_inst, params, tcoeffs, save_at, K, out, stream, pdq::MODE_SAVE_AT);
+}
+
+int32_t pdq_solve_fixed_grid(const pdq_config* cfg, int32_t n_inst, const double* params, const double* tcoeffs,
+                             const double* grid, int32_t K, const pdq_solution* out, void* stream) {
+  return solve_common(cfg, n_inst, params, tcoeffs, grid, K, out, stream, pdq::MODE_FIXED_GRID);
+}
+
+int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const double* params, const double* y0,
+                               const double* grid, int32_t K_grid, int32_t every, const double* data,
+                               const double* stds, int32_t n_std, int32_t average_pdfs, double* value, double* grad,
+                               void* stream) {
+  int rc = check_common(cfg, n_inst, K_grid);
+  if (rc) return rc;
+  if (cfg->factorisation == PDQ_BLOCKDIAG)
+    return set_err(PDQ_ERR_UNSUPPORTED, "the LML path supports the dense and isotropic factorisations");
+  if (cfg->strategy != PDQ_FIXEDPOINT)
+    return set_err(PDQ_ERR_UNSUPPORTED, "the LML needs strategy_smoother_fixedpoint (a Markov sequence)");
+  if (cfg->constraint != PDQ_TS0 && grad)
+    return set_err(PDQ_ERR_UNSUPPORTED, "gradients need ts0 (the covariance recursion must not depend on them)");
+  if (cfg->solver == PDQ_SOLVER_MLE) return set_err(PDQ_ERR_UNSUPPORTED, "solver_mle is not supported by the LML path");
+  if (every < 1 || (K_grid - 1) % every != 0)
+    return set_err(PDQ_ERR_INVALID, "every (%d) must divide K_grid - 1 (%d)", every, K_grid - 1);
+  const int n_tan = grad ? cfg->n_params + cfg->ode_dim : 0;
+  if (n_tan > pdq::kMaxTangents) return set_err(PDQ_ERR_UNSUPPORTED, "at most %d gradient directions", pdq::kMaxTangents);
+  if (cfg->factorisation == PDQ_DENSE && cfg->ode_dim > 1 && n_std != 1 && n_std != cfg->ode_dim)
+    return set_err(PDQ_ERR_INVALID, "n_std must be 1 or ode_dim");
+  if (cfg->factorisation == PDQ_ISOTROPIC && n_std != 1)
+    return set_err(PDQ_ERR_INVALID, "isotropic observation noise is one scalar per time point (n_std = 1)");
+  if ((cfg->num_derivatives + 1) * cfg->ode_dim > 64) return set_err(PDQ_ERR_UNSUPPORTED, "Taylor init too large");
+  if (n_inst == 0) return PDQ_OK;
+  if (!params || !y0 || !grid || !data || !stds) return set_err(PDQ_ERR_INVALID, "null input pointer");
+  hipStream_t st = (hipStream_t)stream;
+  pdq::Layout L = pdq::make_layout(*cfg, n_tan);
+  if (L.nc > 32) return set_err(PDQ_ERR_UNSUPPORTED, "dense state dimension %d exceeds 32", L.nc);
+  const int Kobs = (K_grid - 1) / every + 1;
+  const size_t csz = 2 * (size_t)L.nc * L.nc + (size_t)L.nc * L.mc + 2 * (size_t)L.nc;
+  double* scratch = nullptr;
+  int* queue = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&queue, sizeof(int), st));
+  HIP_TRY(hipMallocAsync((void**)&scratch, sizeof(double) * csz * (size_t)std::max(1, Kobs - 1) * n_inst, st));
+  // prior means with tangents: Taylor expansion of y0 (jetexpand, num = nu)
+  double* tinit = nullptr;
+  const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
+  HIP_TRY(hipMallocAsync((void**)&tinit, sizeof(double) * (size_t)n_inst * N * d * (1 + n_tan), st));
+  rc = pdq::launch_taylor_tangents(cfg->vector_field, d, cfg->n_params, n_inst, cfg->num_derivatives, n_tan, params, y0,
+                                   grid, tinit, st);
+  if (rc) return rc;
+  DevMats m;
+  rc = device_mats(*cfg, false, m);
+  if (rc) return rc;
+  pdq::KArgs a{};
+  a.cfg = *cfg;
+  a.lay = L;
+  a.n_inst = n_inst;
+  a.K = K_grid;
+  a.mode = pdq::MODE_LML;
+  a.every = every;
+  a.n_tan = n_tan;
+  a.average_pdfs = average_pdfs;
+  a.n_std = n_std;
+  a.params = params;
+  a.tcoeffs = tinit;
+  a.times = grid;
+  a.data = data;
+  a.stds = stds;
+  a.A = m.A;
+  a.Q = m.Q;
+  a.lml_value = value;
+  a.lml_grad = grad;
+  a.scratch = scratch;
+  a.queue = queue;
+  const int G = pick_lanes(*cfg, L, n_inst, pdq::MODE_LML);
+  rc = pdq::launch_coop(a, G, st);
+  (void)hipFreeAsync(tinit, st);
+  (void)hipFreeAsync(scratch, st);
+  (void)hipFreeAsync(queue, st);
+  return rc;
+}
+
+int32_t pdq_taylor_coefficients(int32_t vector_field, int32_t d, int32_t n_params, int32_t n_inst,
+                                const double* params, const double* y0, double t0, int32_t num, double* out,
+                                void* stream) {
+  if ((num + 1) * d > 8 * 64) return set_err(PDQ_ERR_UNSUPPORTED, "(num+1)*d must not exceed 512");
+  if (num < 0 || d < 1 || n_inst < 0) return set_err(PDQ_ERR_INVALID, "invalid sizes");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
+  if (n_inst == 0) return PDQ_OK;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(taylor_kernel, dim3((n_inst + 63) / 64), dim3(64), 0, st, vector_field, d, n_params, n_inst,
+                     params, y0, t0, num, out);
+  HIP_TRY(hipGetLastError());
+  return PDQ_OK;
+}
+
+int32_t pdq_tuned_lanes(const pdq_config* cfg, int32_t n_inst) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_tuned.find(tune_key(*cfg, n_inst, pdq::MODE_SAVE_AT));
+  return it == g_tuned.end() ? 0 : it->second;
+}
+
+// Records the lanes-per-instance choice of an external timing sweep.
+int32_t pdq_set_tuned_lanes(const pdq_config* cfg, int32_t n_inst, int32_t mode, int32_t lanes) {
+  if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+    return set_err(PDQ_ERR_INVALID, "lanes must be a power of two in [4, 64]");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tuned[tune_key(*cfg, n_inst, mode)] = lanes;
+  return PDQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,555 @@
+"""Strategy/solver API of probdiffeq, backed by the gfx950 C ABI.
+
+Mirrors the names, arguments and error behaviour of the reference's
+``probdiffeq.probdiffeq`` facade (probdiffeq/probdiffeq.py:6-17) for the
+configurations the HIP kernels implement.  Configuration objects only
+collect parameters; every computation happens in one batched launch in
+``probdiffeq_amd.ivpsolve`` (or ``loss_lml_timeseries``).  A configuration
+the kernels do not implement raises ``NotImplementedError``; nothing falls
+back to the CPU.
+
+Vector fields: device code cannot call Python callables, so ODEs come from
+the registry in this module (``lotka_volterra``, ``van_der_pol``,
+``burgers``, ``linear``, ``logistic``), each a ``JetOde`` with
+``num_tcoeffs_in_args=1`` and ``tcoeff_indices_output=[1]`` like
+``problems.ode`` (problems.py:283-295).  Parameters may carry a leading
+batch dimension; a batch is solved in one launch (the reference uses
+jax.vmap).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from typing import Any
+
+import numpy as np
+import torch
+
+from probdiffeq_amd import capi
+
+__all__ = [
+    "JetOde",
+    "MarkovSequence",
+    "Normal",
+    "ProbabilisticSolution",
+    "SmoothingSolution",
+    "burgers",
+    "error_norm_rms_then_scale",
+    "error_norm_scale_then_rms",
+    "error_residual_std",
+    "jacobian_materialize",
+    "jetexpand_ode_padded_scan",
+    "linear",
+    "logistic",
+    "loss_lml_timeseries",
+    "lotka_volterra",
+    "ode",
+    "solver",
+    "solver_dynamic",
+    "solver_mle",
+    "state_space_model_blockdiag",
+    "state_space_model_dense",
+    "state_space_model_isotropic",
+    "strategy_filter",
+    "strategy_smoother_fixedinterval",
+    "strategy_smoother_fixedpoint",
+    "van_der_pol",
+]
+
+
+def _device():
+    return capi.require_device()
+
+
+def _as_f64(x, device=None):
+    if isinstance(x, torch.Tensor):
+        return x.to(dtype=torch.float64, device=device or _device()).contiguous()
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=device or _device()).contiguous()
+
+
+# ----------------------------------------------------------------- problems
+@dataclasses.dataclass
+class JetOde:
+    """A registered vector field u' = f(u, t; params) (problems.py:170-210)."""
+
+    name: str
+    params: Any  # (P,) or (B, P)
+    ode_dim: int
+    num_tcoeffs_in_args: int = 1
+    tcoeff_indices_output: tuple = (1,)
+    jacobian: Any = None
+
+    def params_batched(self, batch: int | None = None) -> torch.Tensor:
+        p = _as_f64(self.params)
+        if p.dim() == 1:
+            p = p[None]
+        if batch is not None and p.shape[0] == 1 and batch > 1:
+            p = p.expand(batch, -1).contiguous()
+        return p
+
+
+def lotka_volterra(a=0.5, b=0.05, c=0.5, d=0.05) -> JetOde:
+    """dy1 = a y1 - b y1 y2, dy2 = -c y2 + d y1 y2 (backend/ode.py:30-51)."""
+    p = torch.stack([_as_f64(v) for v in (a, b, c, d)], dim=-1)
+    return JetOde("lotka_volterra", p, 2)
+
+
+def van_der_pol(mu=1.0) -> JetOde:
+    """y1' = y2, y2' = mu ((1 - y1^2) y2) - y1."""
+    p = _as_f64(mu)
+    return JetOde("van_der_pol", p[..., None] if p.dim() <= 1 else p, 2)
+
+
+def burgers(num_points: int, viscosity=0.01, length=1.0) -> JetOde:
+    """u_t + u u_x = nu u_xx on a periodic grid, central differences."""
+    return JetOde("burgers", _as_f64([viscosity, length / num_points]), num_points)
+
+
+def linear(lam) -> JetOde:
+    """u' = lam * u + t (the reference's calibration-equivalence test ODE)."""
+    lam = _as_f64(lam)
+    return JetOde("linear", lam, lam.shape[-1])
+
+
+def logistic(r=2.0, ode_dim=1) -> JetOde:
+    """u' = r u (1 - u) (examples/A0_quickstart.py)."""
+    p = _as_f64(r)
+    return JetOde("logistic", p[..., None] if p.dim() <= 1 else p, ode_dim)
+
+
+def ode(func, /, *, jacobian=None):
+    """problems.ode (problems.py:283-295): only registered vector fields run on the device."""
+    if isinstance(func, JetOde):
+        if jacobian is not None:
+            func = dataclasses.replace(func, jacobian=jacobian)
+        return func
+    msg = "Arbitrary Python vector fields cannot run inside the gfx950 kernels; use a registered"
+    msg += " vector field (lotka_volterra, van_der_pol, burgers, linear, logistic)."
+    raise NotImplementedError(msg)
+
+
+class jacobian_materialize:
+    """jacobians.py:78-113; the registry provides analytic Jacobians."""
+
+    def __init__(self, *, jacfun=None) -> None:
+        self.jacfun = jacfun
+
+
+def jetexpand_ode_padded_scan(*, num: int):
+    """Taylor coefficients by Taylor-mode recursion (jet_expansion_algorithms.py:49-103)."""
+
+    def expand(vf: JetOde, inits, /, *, t):
+        if not isinstance(vf, JetOde):
+            raise TypeError(vf)
+        (u0,) = inits
+        u0 = _as_f64(u0)
+        y0 = u0[None] if u0.dim() == 1 else u0
+        params = vf.params_batched(y0.shape[0])
+        if params.shape[0] != y0.shape[0]:
+            raise ValueError(f"batch sizes differ: params {params.shape[0]}, inits {y0.shape[0]}")
+        tc = capi.taylor_coefficients(vf.name, params, y0.contiguous(), float(t), num)
+        coeffs = [tc[:, i, :] for i in range(num + 1)]
+        if u0.dim() == 1:
+            coeffs = [c[0] for c in coeffs]
+        return coeffs, {}
+
+    return expand
+
+
+# ------------------------------------------------------------ distributions
+@dataclasses.dataclass
+class Normal:
+    """Flattened normal of one factorisation (ssm_impl_*.py Normal classes).
+
+    ``mean_flat`` / ``cholesky_flat`` carry leading (batch, time) axes:
+    dense (.., n) / (.., n, n); isotropic (.., N, d) / (.., N, N);
+    blockdiag (.., d, N) / (.., d, N, N).
+    """
+
+    mean_flat: torch.Tensor
+    cholesky_flat: torch.Tensor
+    factorisation: str
+    num_coeffs: int
+    ode_dim: int
+
+    @property
+    def mean(self) -> list:
+        """List of Taylor coefficients, each (.., d) (tree_flatten.unflatten_array)."""
+        N, d = self.num_coeffs, self.ode_dim
+        if self.factorisation == "dense":
+            return [self.mean_flat[..., i * d : (i + 1) * d] for i in range(N)]
+        if self.factorisation == "isotropic":
+            return [self.mean_flat[..., i, :] for i in range(N)]
+        return [self.mean_flat[..., :, i] for i in range(N)]
+
+    @property
+    def std(self) -> list:
+        """Row norms of the Cholesky factor (ssm_impl_dense.py:141-150 & co.)."""
+        rn = torch.linalg.vector_norm(self.cholesky_flat, dim=-1)
+        N, d = self.num_coeffs, self.ode_dim
+        if self.factorisation == "dense":
+            return [rn[..., i * d : (i + 1) * d] for i in range(N)]
+        if self.factorisation == "isotropic":
+            return [rn[..., i] for i in range(N)]
+        return [rn[..., :, i] for i in range(N)]
+
+    def to_multivariate_normal(self):
+        """(mean, covariance) in the reference's flattened coordinates."""
+        L = self.cholesky_flat
+        cov = L @ L.transpose(-1, -2)
+        if self.factorisation == "dense":
+            return self.mean_flat, cov
+        if self.factorisation == "isotropic":
+            d = self.ode_dim
+            eye = torch.eye(d, dtype=cov.dtype, device=cov.device)
+            full = torch.einsum("...nm,dt->...ndmt", cov, eye)
+            n = self.mean_flat.shape[-2] * d
+            return self.mean_flat.reshape(*self.mean_flat.shape[:-2], n), full.reshape(*full.shape[:-4], n, n)
+        raise NotImplementedError("blockdiag to_multivariate_normal")
+
+
+@dataclasses.dataclass
+class LatentConditional:
+    """Backward conditionals p(x_k | x_{k+1}) in latent coordinates (ssm_impl_api.py:35-132)."""
+
+    A: torch.Tensor
+    noise: Normal
+    to_latent: torch.Tensor
+    to_observed: torch.Tensor
+
+
+@dataclasses.dataclass
+class MarkovSequence:
+    """estimators_and_losses.py:121-273: a terminal marginal and backward conditionals."""
+
+    marginal: Normal
+    conditional: LatentConditional
+    reverse: bool = True
+
+
+@dataclasses.dataclass
+class SmoothingSolution:
+    posterior: MarkovSequence
+    filtering: Normal
+
+
+@dataclasses.dataclass
+class ProbabilisticSolution:
+    """solvers.py:33-69, batched: every field has a leading instance axis."""
+
+    t: torch.Tensor
+    u: Normal
+    solution_full: Any
+    output_scale: torch.Tensor
+    num_steps: torch.Tensor
+    attempts: torch.Tensor  # attempted steps (accepted + rejected), per instance
+    status: torch.Tensor    # 0 ok, 1 max attempts, 2 non-finite, 3 dt underflow
+
+    def squeeze(self):
+        """Drop the instance axis of a single-instance solve."""
+
+        def sq(x):
+            if isinstance(x, torch.Tensor):
+                return x[0]
+            if dataclasses.is_dataclass(x):
+                return type(x)(**{f.name: sq(getattr(x, f.name)) for f in dataclasses.fields(x)})
+            return x
+
+        return sq(self)
+
+
+# ------------------------------------------------------- state-space models
+@dataclasses.dataclass
+class Prior:
+    """An integrated Wiener process prior (prior_wiener_integrated)."""
+
+    factorisation: str
+    tcoeffs: torch.Tensor  # (B, N, d)
+    init_std: float
+    base_output_scale: float
+    batched: bool
+
+    @property
+    def num_derivatives(self) -> int:
+        return self.tcoeffs.shape[1] - 1
+
+    @property
+    def ode_dim(self) -> int:
+        return self.tcoeffs.shape[2]
+
+
+@dataclasses.dataclass
+class Constraint:
+    kind: str  # "ts0" | "ts1"
+    ode: JetOde
+    residual_order: int = 2
+
+
+class _StateSpaceModel:
+    factorisation = ""
+
+    def __repr__(self) -> str:
+        return f"{self.__class__.__name__}()"
+
+    def prior_wiener_integrated(self, tcoeffs, /, *, is_exact=True, inexact_eps=1e-6, diffuse_derivatives=0, diffuse_eps=1.0, output_scale=None):
+        """ssm_impl_*.py prior_wiener_integrated (dense :458-514, iso :409-482, bd :463-547)."""
+        if isinstance(tcoeffs, torch.Tensor) or isinstance(tcoeffs, np.ndarray):
+            msg = "Mean must be a pytree, not an array."
+            raise TypeError(msg)
+        coeffs = [_as_f64(c) for c in tcoeffs]
+        shape0 = coeffs[0].shape
+        for i, c in enumerate(coeffs):
+            if c.shape != shape0:
+                msg = f"All leaves of the mean must have the same shape. However, leaf {i} has shape {tuple(c.shape)}, while leaf 0 has shape {tuple(shape0)}"
+                raise ValueError(msg)
+        if diffuse_derivatives:
+            raise NotImplementedError("diffuse derivatives are not implemented on the device path")
+        batched = coeffs[0].dim() == 2
+        tc = torch.stack(coeffs, dim=-2)
+        if not batched:
+            tc = tc[None]
+        if not isinstance(is_exact, bool):
+            raise NotImplementedError("element-wise is_exact is not implemented on the device path")
+        init_std = 0.0 if is_exact else float(inexact_eps)
+        base = 1.0
+        if output_scale is not None:
+            s = np.asarray(output_scale.cpu() if isinstance(output_scale, torch.Tensor) else output_scale, dtype=np.float64)
+            expect = () if self.factorisation == "isotropic" else (tc.shape[-1],)
+            if s.shape not in [expect, ()]:
+                msg = f"The base-scale has the wrong shape. Expected: {expect}. Received: {s.shape}."
+                raise ValueError(msg)
+            if s.size > 1 and not np.all(s == s.flat[0]):
+                raise NotImplementedError("non-uniform base output scales are not implemented on the device path")
+            base = float(s.flat[0])
+        return Prior(self.factorisation, tc.contiguous(), init_std, base, batched)
+
+    def constraint_ode_ts0(self, ode, /):
+        """ssm_impl_api.py:520-534."""
+        if not isinstance(ode, JetOde):
+            raise TypeError(ode)
+        return Constraint("ts0", ode)
+
+    def constraint_ode_ts1(self, ode, /, taylor_point=None):
+        """ssm_impl_api.py:536-558 (prior-mean Taylor point, exact Jacobian)."""
+        if not isinstance(ode, JetOde):
+            raise TypeError(ode)
+        if taylor_point is not None:
+            raise NotImplementedError("only the prior-mean Taylor point is implemented")
+        if self.factorisation != "dense":
+            msg = "ts1 for isotropic/blockdiag models uses Hutchinson estimators (jacobians.py:116-271), not implemented on the device path"
+            raise NotImplementedError(msg)
+        return Constraint("ts1", ode)
+
+
+class state_space_model_dense(_StateSpaceModel):
+    factorisation = "dense"
+
+
+class state_space_model_isotropic(_StateSpaceModel):
+    factorisation = "isotropic"
+
+
+class state_space_model_blockdiag(_StateSpaceModel):
+    factorisation = "blockdiag"
+
+
+# -------------------------------------------------------- strategies/solvers
+class _Strategy:
+    name = ""
+    is_suitable_for_save_at = True
+    is_suitable_for_save_every_step = True
+    is_suitable_for_offgrid_marginals = True
+
+    def __repr__(self) -> str:
+        return f"{self.__class__.__name__}()"
+
+
+class strategy_filter(_Strategy):
+    """estimators_and_losses.py:347-421."""
+
+    name = "filter"
+
+
+class strategy_smoother_fixedpoint(_Strategy):
+    """estimators_and_losses.py:473-591."""
+
+    name = "fixedpoint"
+    is_suitable_for_save_every_step = False
+    is_suitable_for_offgrid_marginals = False
+
+
+class strategy_smoother_fixedinterval(_Strategy):
+    """estimators_and_losses.py:594-717."""
+
+    name = "fixedinterval"
+    is_suitable_for_save_at = False
+
+
+class _Solver:
+    kind = ""
+
+    def __init__(self, *, strategy, constraint, constraint_init=None) -> None:
+        if constraint_init is not None:
+            raise NotImplementedError("constraint_init is not implemented on the device path")
+        self.strategy = strategy
+        self.constraint = constraint
+        self.constraint_init = constraint_init
+
+    def __repr__(self) -> str:
+        return f"{self.__class__.__name__}(strategy={self.strategy!r}, constraint={self.constraint!r})"
+
+    @property
+    def is_suitable_for_save_at(self):
+        return self.strategy.is_suitable_for_save_at
+
+    @property
+    def is_suitable_for_save_every_step(self):
+        return self.strategy.is_suitable_for_save_every_step
+
+    @property
+    def is_suitable_for_offgrid_marginals(self):
+        return self.strategy.is_suitable_for_offgrid_marginals
+
+
+class solver(_Solver):
+    """solvers.py:636-767 (no calibration)."""
+
+    kind = "solver"
+
+
+class solver_dynamic(_Solver):
+    """solvers.py:483-633 (dynamic output-scale calibration)."""
+
+    kind = "dynamic"
+
+    def __init__(self, *, strategy, constraint, constraint_init=None, re_linearize_after_calibration=False, stop_gradient_through_calibration=True):
+        super().__init__(strategy=strategy, constraint=constraint, constraint_init=constraint_init)
+        if re_linearize_after_calibration:
+            raise NotImplementedError("re_linearize_after_calibration=True is not implemented on the device path")
+        self.re_linearize_after_calibration = re_linearize_after_calibration
+        self.stop_gradient_through_calibration = stop_gradient_through_calibration
+
+
+class solver_mle(_Solver):
+    """solvers.py:318-480 (running maximum-likelihood calibration)."""
+
+    kind = "mle"
+
+    def __init__(self, *, strategy, constraint, constraint_init=None, correct_asymptotic_underconfidence=True):
+        super().__init__(strategy=strategy, constraint=constraint, constraint_init=constraint_init)
+        self.correct_asymptotic_underconfidence = correct_asymptotic_underconfidence
+
+
+def error_norm_scale_then_rms(*, norm_order=None):
+    """solvers.py:770-791."""
+    if norm_order is not None:
+        raise NotImplementedError("only the 2-norm is implemented")
+    return "scale_then_rms"
+
+
+def error_norm_rms_then_scale(norm_order=None):
+    """solvers.py:794-811."""
+    if norm_order is not None:
+        raise NotImplementedError("only the 2-norm is implemented")
+    return "rms_then_scale"
+
+
+class error_residual_std:
+    """solvers.py:850-996."""
+
+    def __init__(self, *, constraint, error_norm=None, re_linearize_before_error=False, error_per_unit_step=False) -> None:
+        if re_linearize_before_error or error_per_unit_step:
+            raise NotImplementedError("re_linearize_before_error / error_per_unit_step are not implemented")
+        self.constraint = constraint
+        self.error_norm = error_norm or "scale_then_rms"
+
+
+# ------------------------------------------------------------- device call
+def make_config(prior: Prior, solver_obj: _Solver, error=None, control=None, clip_dt=False) -> capi.Config:
+    """Pack the configuration objects into the C ABI's POD struct."""
+    ode_ = solver_obj.constraint.ode
+    if ode_.ode_dim != prior.ode_dim:
+        raise ValueError(f"the vector field has dimension {ode_.ode_dim}, the prior {prior.ode_dim}")
+    cfg = capi.Config(
+        factorisation=prior.factorisation,
+        num_derivatives=prior.num_derivatives,
+        ode_dim=prior.ode_dim,
+        constraint=solver_obj.constraint.kind,
+        strategy=solver_obj.strategy.name,
+        solver=solver_obj.kind,
+        vector_field=ode_.name,
+        n_params=ode_.params_batched().shape[-1],
+        clip_dt=clip_dt,
+        base_output_scale=prior.base_output_scale,
+        init_std=prior.init_std,
+    )
+    if isinstance(solver_obj, solver_mle):
+        cfg.correct_asymptotic_underconfidence = solver_obj.correct_asymptotic_underconfidence
+    if error is not None:
+        cfg.error_norm = error.error_norm
+    if control is not None:
+        cfg.control = control.kind
+        cfg.safety, cfg.factor_min, cfg.factor_max = control.safety, control.factor_min, control.factor_max
+        if control.kind == "proportional_integral":
+            cfg.exponent_integral = control.exponent_integral
+            cfg.exponent_proportional = control.exponent_proportional
+    return cfg
+
+
+def build_solution(cfg: capi.Config, out: dict, *, batched: bool) -> ProbabilisticSolution:
+    """Wrap the C ABI's flat outputs into the reference's solution structure."""
+    B, K = out["t"].shape
+    N, d = cfg.N, cfg.ode_dim
+    fact = cfg.factorisation
+
+    def normal(m, L):
+        lead = m.shape[:-1]
+        if fact == "dense":
+            return Normal(m, L.reshape(*lead, N * d, N * d), fact, N, d)
+        if fact == "isotropic":
+            return Normal(m.reshape(*lead, N, d), L.reshape(*lead, N, N), fact, N, d)
+        return Normal(m.reshape(*lead, d, N), L.reshape(*lead, d, N, N), fact, N, d)
+
+    u = normal(out["mean"], out["chol"])
+    if cfg.strategy == "filter":
+        full = u
+    else:
+        n = cfg.n
+        cond = LatentConditional(
+            out["cond_A"].reshape(B, K - 1, n, n) if fact != "blockdiag" else out["cond_A"].reshape(B, K - 1, d, N, N),
+            normal(out["cond_mean"], out["cond_chol"]),
+            out["cond_tl"],
+            out["cond_to"],
+        )
+        post = MarkovSequence(normal(out["term_mean"], out["term_chol"]), cond, reverse=True)
+        full = SmoothingSolution(post, normal(out["filt_mean"], out["filt_chol"]))
+    os_ = out["output_scale"]
+    if cfg.solver != "dynamic":
+        os_ = os_[:, : K - 1]
+    if fact != "blockdiag":
+        os_ = os_[..., 0]
+    sol = ProbabilisticSolution(out["t"], u, full, os_, out["num_steps"], out["attempts"], out["status"])
+    return sol if batched else sol.squeeze()
+
+
+# ---------------------------------------------------------------- the LML
+def loss_lml_timeseries(*, average_pdfs: bool = True, tcoeff_index=0, solve_triu=None):
+    """estimators_and_losses.py:53-105 on the device, as a differentiable torch op.
+
+    Returns ``loss(u, /, *, problem, std)`` where ``problem`` is an
+    ``LMLProblem`` (prior configuration, solver, grid and checkpoint stride):
+    the reference differentiates through a solve that depends on the
+    parameters; here the solve and the likelihood run in one launch and the
+    gradient with respect to the vector-field parameters and the initial
+    value comes from the same launch (forward-mode tangents).
+    """
+    if tcoeff_index != 0:
+        raise NotImplementedError("observations of the zeroth Taylor coefficient only")
+
+    def loss(u, /, *, problem, params, y0, std):
+        from probdiffeq_amd.torch_ops import lml
+
+        return lml(problem, u, std, params=params, y0=y0, average_pdfs=average_pdfs)
+
+    return loss
+

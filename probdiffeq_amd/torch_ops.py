@@ -1,0 +1,89 @@
+"""torch.autograd wrappers of the C ABI.
+
+``LML`` is the differentiable marginal log-likelihood of the parameter
+estimation workflow (examples/E2_parameter_estimation_optax.py): forward
+runs the fixed-point-smoother solve and ``evaluate_lml`` on the device and
+obtains the gradient with respect to the vector-field parameters and the
+initial value in the same launch; backward scales it by the incoming
+gradient.  There is no CPU path.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import torch
+
+from probdiffeq_amd import capi
+
+
+@dataclasses.dataclass
+class LMLProblem:
+    """The solve behind the likelihood (mirrors the example's solver setup).
+
+    ``grid``: fixed time grid (K_grid,); ``every``: observation stride on the
+    grid (observations at grid[::every]); the solver is ``solver`` /
+    ``solver_dynamic`` with ``strategy_smoother_fixedpoint`` and ts0.
+    """
+
+    vector_field: str
+    num_derivatives: int
+    ode_dim: int
+    grid: torch.Tensor
+    every: int
+    factorisation: str = "isotropic"
+    solver: str = "solver"
+    base_output_scale: float = 1.0
+    init_std: float = 0.0
+    lanes_per_instance: int = 0
+
+    def config(self, n_params: int) -> capi.Config:
+        return capi.Config(
+            factorisation=self.factorisation,
+            num_derivatives=self.num_derivatives,
+            ode_dim=self.ode_dim,
+            strategy="fixedpoint",
+            solver=self.solver,
+            vector_field=self.vector_field,
+            n_params=n_params,
+            base_output_scale=self.base_output_scale,
+            init_std=self.init_std,
+            lanes_per_instance=self.lanes_per_instance,
+        )
+
+
+class LML(torch.autograd.Function):
+    """value = loss_lml_timeseries(data, posterior=solve(params, y0), std)."""
+
+    @staticmethod
+    def forward(ctx, params, y0, data, stds, problem: LMLProblem, average: bool):
+        cfg = problem.config(params.shape[-1])
+        need_grad = params.requires_grad or y0.requires_grad
+        value, grad = capi.lml_value_and_grad(
+            cfg,
+            params.detach().contiguous(),
+            y0.detach().contiguous(),
+            problem.grid.contiguous(),
+            problem.every,
+            data.contiguous(),
+            stds.contiguous(),
+            average=average,
+            grad=need_grad,
+        )
+        ctx.n_params = params.shape[-1]
+        ctx.save_for_backward(grad if grad is not None else value)
+        ctx.has_grad = grad is not None
+        return value
+
+    @staticmethod
+    def backward(ctx, g):
+        if not ctx.has_grad:
+            return None, None, None, None, None, None
+        (grad,) = ctx.saved_tensors
+        full = grad * g[:, None]
+        return full[:, : ctx.n_params], full[:, ctx.n_params :], None, None, None, None
+
+
+def lml(problem: LMLProblem, data, std, *, params, y0, average_pdfs=True):
+    """Batched LML (one value per instance) with autograd support."""
+    return LML.apply(params, y0, data, std, problem, average_pdfs)

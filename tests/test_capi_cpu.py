@@ -1,0 +1,101 @@
+"""CPU-side checks of the C ABI and the host logic (no compute calls)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "pdq.h")).read()
+    return sorted(set(re.findall(r"\b(pdq_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from probdiffeq_amd import capi
+
+    lib = capi.lib()
+    declared = _header_symbols()
+    assert "pdq_solve_adaptive_save_at" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    for name in capi.EXPORTED:
+        assert name in declared, name
+
+
+def test_config_struct_matches_header_layout():
+    """The ctypes mirror of pdq_config has the C layout (12 ints, 12 doubles, 2 ints)."""
+    from probdiffeq_amd import capi
+
+    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 2 * 4
+    assert ctypes.sizeof(capi._Solution) == 17 * 8
+
+
+def test_sizes_match_layouts():
+    from probdiffeq_amd import capi
+
+    lib = capi.lib()
+    for fact, n_chol in [("dense", 100), ("isotropic", 25), ("blockdiag", 2 * 25)]:
+        cfg = capi.Config(factorisation=fact, num_derivatives=4, ode_dim=2)
+        c = cfg.c()
+        assert lib.pdq_mean_size(ctypes.byref(c)) == 10 == cfg.mean_size
+        assert lib.pdq_chol_size(ctypes.byref(c)) == n_chol == cfg.chol_size
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    from probdiffeq_amd import capi
+
+    assert capi.lib().pdq_init() == capi.ERR_NO_DEVICE
+    with pytest.raises(capi.PdqError, match="no HIP device"):
+        capi.require_device()
+
+
+def test_validation_errors_are_reported():
+    """Argument checks run before any device work (controllers and shapes)."""
+    from probdiffeq_amd import capi
+
+    lib = capi.lib()
+    cfg = capi.Config(factorisation="isotropic", constraint="ts1")
+    c = cfg.c()
+    sol = capi._Solution()
+    rc = lib.pdq_solve_adaptive_save_at(ctypes.byref(c), 1, None, None, None, 10, ctypes.byref(sol), None)
+    assert rc == 2  # PDQ_ERR_UNSUPPORTED
+    assert "ts1" in capi.last_error()
+    cfg = capi.Config(num_derivatives=12)
+    c = cfg.c()
+    rc = lib.pdq_solve_adaptive_save_at(ctypes.byref(c), 1, None, None, None, 10, ctypes.byref(sol), None)
+    assert rc == 2 and "num_derivatives" in capi.last_error()
+    rc = lib.pdq_solve_adaptive_save_at(ctypes.byref(capi.Config().c()), 1, None, None, None, 1, ctypes.byref(sol), None)
+    assert rc == 1 and "two time points" in capi.last_error()
+
+
+def test_api_objects_mirror_the_reference():
+    """Configuration objects and their reference error behaviour (no device needed)."""
+    from probdiffeq_amd import probdiffeq as pdq
+
+    assert pdq.strategy_smoother_fixedpoint().is_suitable_for_save_every_step is False
+    assert pdq.strategy_smoother_fixedinterval().is_suitable_for_save_at is False
+    ssm = pdq.state_space_model_isotropic()
+    with pytest.raises(TypeError):
+        ssm.constraint_ode_ts0(lambda y, t: y)
+    with pytest.raises(NotImplementedError):
+        pdq.ode(lambda y, *, t: y)
+    with pytest.raises(NotImplementedError):
+        pdq.solver_dynamic(strategy=pdq.strategy_filter(), constraint=None, re_linearize_after_calibration=True)
+
+
+def test_fixed_grid_warns_for_fixedpoint_smoother():
+    """test_solve_fixed_grid.py:57-71."""
+    from probdiffeq_amd import ivpsolve
+    from probdiffeq_amd import probdiffeq as pdq
+
+    slv = pdq.solver(strategy=pdq.strategy_smoother_fixedpoint(), constraint=None)
+    with pytest.warns(UserWarning, match="fixed-point"):
+        ivpsolve.solve_fixed_grid(solver=slv)

@@ -1,0 +1,189 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Tolerance: FP64 results agree with the oracle to 1e-10 normwise-relative
+per output array (max |gpu - oracle| <= 1e-10 * max |oracle|); covariances
+are compared as L L^T (factor signs are checked separately where the
+LAPACK convention fixes them).  Step counts (attempts and accepted steps)
+must be identical.
+"""
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+THETA = np.array([0.5, 0.05, 0.5, 0.05])
+RTOL = 1e-10
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return np.max(np.abs(a - b)) / scale
+
+
+def _cov(chol, n, nblk=1):
+    L = np.asarray(chol).reshape(-1, nblk, n, n)
+    return np.einsum("kbij,kblj->kbil", L, L)
+
+
+@pytest.fixture(scope="module")
+def pdq():
+    from probdiffeq_amd import capi
+
+    capi.require_device()
+    capi.lib()
+    return capi
+
+
+def _dev(x, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype).cuda()
+
+
+def _oracle_cfg(oracle, cfg):
+    return oracle.Config(
+        fact=cfg.factorisation,
+        nu=cfg.num_derivatives,
+        d=cfg.ode_dim,
+        constraint=cfg.constraint,
+        strategy=cfg.strategy,
+        solver=cfg.solver,
+        error_norm=cfg.error_norm,
+        control=cfg.control,
+        vf=cfg.vector_field,
+        clip_dt=cfg.clip_dt,
+        rtol=cfg.rtol,
+        atol=cfg.atol,
+        dt0=cfg.dt0,
+        eps=cfg.eps,
+        damp=cfg.damp,
+        base_scale=cfg.base_output_scale,
+        max_attempts=cfg.max_attempts,
+    )
+
+
+def _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=False, check_factor_signs=False):
+    """Solve a batch on the GPU and every instance with the oracle; compare."""
+    B, K = times.shape
+    ocfg = _oracle_cfg(oracle, cfg)
+    tcs = np.stack([oracle.taylor_coefficients(cfg.vector_field, params[i], y0s[i], times[i, 0], cfg.num_derivatives) for i in range(B)])
+    out = pdq.solve(cfg, _dev(params), _dev(tcs), _dev(times), fixed_grid=fixed_grid)
+    torch.cuda.synchronize()
+    g = {k: v.cpu().numpy() for k, v in out.items()}
+    n, nblk = cfg.n, cfg.nblk
+    for i in range(B):
+        if fixed_grid:
+            r = oracle.solve_fixed_grid(ocfg, params[i], tcs[i], times[i])
+        else:
+            r = oracle.solve_save_at(ocfg, params[i], tcs[i], times[i])
+        assert r["status"] == 0
+        assert g["status"][i] == 0, f"instance {i} status {g['status'][i]}"
+        assert g["attempts"][i] == r["attempts"], (i, g["attempts"][i], r["attempts"])
+        assert g["accepted"][i] == r["accepted"]
+        assert np.array_equal(g["num_steps"][i], r["num_steps"])
+        assert _rel(g["t"][i], r["t"]) < 1e-14
+        assert _rel(g["mean"][i], r["mean"]) < RTOL, _rel(g["mean"][i], r["mean"])
+        assert _rel(_cov(g["chol"][i], n, nblk), _cov(r["chol"], n, nblk)) < RTOL
+        os_rows = r["output_scale"].shape[0]
+        assert _rel(g["output_scale"][i][:os_rows], r["output_scale"]) < RTOL
+        if check_factor_signs:
+            assert _rel(g["chol"][i], r["chol"]) < RTOL
+        if cfg.strategy != "filter":
+            assert _rel(g["filt_mean"][i], r["filt_mean"]) < RTOL
+            assert _rel(_cov(g["filt_chol"][i], n, nblk), _cov(r["filt_chol"], n, nblk)) < RTOL
+            assert _rel(g["cond_A"][i], r["cond_A"]) < RTOL
+            assert _rel(g["cond_mean"][i], r["cond_mean"]) < RTOL
+            assert _rel(g["term_mean"][i], r["term_mean"]) < RTOL
+    return g
+
+
+def _lv_batch(B, seed, t_lo=5.0, t_hi=15.0, K=20):
+    rng = np.random.default_rng(seed)
+    params = THETA * np.exp(rng.uniform(np.log(0.8), np.log(1.2), size=(B, 4)))
+    y0 = rng.uniform(10, 30, size=(B, 2))
+    t_end = rng.uniform(t_lo, t_hi, size=B)
+    times = np.stack([np.linspace(0.0, te, K) for te in t_end])
+    return params, y0, times
+
+
+def test_c1_dense_filter_single_solve(oracle, pdq):
+    """C1: one LV solve, dense nu=4, ts0, dynamic calibration, filter, 200 save_at."""
+    cfg = pdq.Config(factorisation="dense", strategy="filter", solver="dynamic")
+    times = np.linspace(0.0, 50.0, 200)[None]
+    _compare(oracle, pdq, cfg, THETA[None], np.array([[20.0, 20.0]]), times, check_factor_signs=True)
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+def test_isotropic_fixedpoint_batch_all_group_sizes(oracle, pdq, lanes):
+    """C2 kind: isotropic nu=4, fixed-point smoother, dynamic, uneven lengths."""
+    cfg = pdq.Config(factorisation="isotropic", strategy="fixedpoint", solver="dynamic", lanes_per_instance=lanes)
+    params, y0, times = _lv_batch(12, seed=lanes)
+    _compare(oracle, pdq, cfg, params, y0, times)
+
+
+@pytest.mark.parametrize("fact", ["dense", "isotropic"])
+@pytest.mark.parametrize("strategy", ["filter", "fixedpoint", "fixedinterval"])
+@pytest.mark.parametrize("solver", ["solver", "dynamic", "mle"])
+def test_strategy_solver_grid(oracle, pdq, fact, strategy, solver):
+    cfg = pdq.Config(factorisation=fact, num_derivatives=3, strategy=strategy, solver=solver, rtol=1e-5, atol=1e-7)
+    params, y0, times = _lv_batch(6, seed=7, t_lo=3.0, t_hi=8.0, K=9)
+    _compare(oracle, pdq, cfg, params, y0, times)
+
+
+def test_c3_dense_ts1_van_der_pol(oracle, pdq):
+    """C3 kind: van der Pol, dense nu=5, ts1 with the analytic Jacobian."""
+    rng = np.random.default_rng(3)
+    B = 8
+    mu = np.exp(rng.uniform(np.log(1.0), np.log(100.0), size=(B, 1)))
+    y0 = np.tile([2.0, 0.0], (B, 1))
+    times = np.tile([0.0, 6.3], (B, 1))
+    cfg = pdq.Config(factorisation="dense", num_derivatives=5, constraint="ts1", vector_field="van_der_pol", n_params=1, rtol=1e-5, atol=1e-7)
+    _compare(oracle, pdq, cfg, mu, y0, times)
+
+
+@pytest.mark.parametrize("fact", ["dense", "isotropic"])
+def test_fixed_grid(oracle, pdq, fact):
+    cfg = pdq.Config(factorisation=fact, num_derivatives=3, strategy="fixedinterval", solver="dynamic")
+    params, y0, _ = _lv_batch(4, seed=11)
+    times = np.tile(np.linspace(0.0, 4.0, 41), (4, 1))
+    _compare(oracle, pdq, cfg, params, y0, times, fixed_grid=True)
+
+
+def test_blockdiag_lotka_volterra(oracle, pdq):
+    cfg = pdq.Config(factorisation="blockdiag", num_derivatives=4, strategy="filter", solver="dynamic")
+    params, y0, times = _lv_batch(5, seed=5)
+    _compare(oracle, pdq, cfg, params, y0, times)
+
+
+def test_blockdiag_burgers_small_grid(oracle, pdq):
+    """C4 kind at a small grid: periodic Burgers, blockdiag nu=2, filter."""
+    Ng = 64
+    rng = np.random.default_rng(0)
+    x = np.arange(Ng) / Ng
+    B = 3
+    y0 = np.stack([sum(rng.normal(0, 1 / k) * np.sin(2 * np.pi * k * x) for k in range(1, 9)) for _ in range(B)])
+    params = np.tile([0.01, 1.0 / Ng], (B, 1))
+    times = np.tile([0.0, 0.2], (B, 1))
+    cfg = pdq.Config(factorisation="blockdiag", num_derivatives=2, ode_dim=Ng, vector_field="burgers", n_params=2, rtol=1e-4, atol=1e-6)
+    _compare(oracle, pdq, cfg, params, y0, times)
+
+
+def test_taylor_coefficients_match_oracle(oracle, pdq):
+    params, y0, _ = _lv_batch(5, seed=2)
+    got = pdq.taylor_coefficients("lotka_volterra", _dev(params), _dev(y0), 0.0, 5).cpu().numpy()
+    for i in range(5):
+        exp = oracle.taylor_coefficients("lotka_volterra", params[i], y0[i], 0.0, 5)
+        assert _rel(got[i], exp) < 1e-14
+
+
+def test_status_max_attempts(pdq):
+    cfg = pdq.Config(rtol=1e-300, atol=1e-300, max_attempts=4)
+    params, y0, times = _lv_batch(2, seed=1)
+    from oracle import oracle as O
+
+    tcs = np.stack([O.taylor_coefficients("lotka_volterra", params[i], y0[i], 0.0, 4) for i in range(2)])
+    out = pdq.solve(cfg, _dev(params), _dev(tcs), _dev(times))
+    assert out["status"].cpu().numpy().tolist() == [1, 1]
