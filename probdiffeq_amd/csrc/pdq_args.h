@@ -2,7 +2,11 @@
 // translation units of libpdq.so.
 #pragma once
 
+#ifdef PDQ_HOST_EMU
+#include "pdq_hostemu.h"
+#else
 #include <hip/hip_runtime.h>
+#endif
 
 #include "../../include/pdq.h"
 #include "pdq_layout.h"

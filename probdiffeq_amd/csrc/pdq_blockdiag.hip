@@ -53,6 +53,9 @@ __device__ __forceinline__ void qr_regs(double (&A)[M * C]) {
 }
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
+#ifdef PDQ_HOST_EMU
+  return v;  // one thread per workgroup in the CPU build
+#endif
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
   const int w = threadIdx.x / 64, nw = blockDim.x / 64;
@@ -398,6 +401,7 @@ __device__ void solve_instance_bd(const BArgs& a, int inst, double* y, double* r
   }
 }
 
+#ifndef PDQ_HOST_EMU
 template <int N, int BPT>
 __global__ void __launch_bounds__(1024) blockdiag_kernel(BArgs a) {
   extern __shared__ double lds[];
@@ -464,4 +468,13 @@ int launch_bd(BArgs& a, hipStream_t st) {
 }
 
 
+#else
+// CPU build: one thread owns all d blocks (BPT >= d); y holds d doubles.
+template <int N, int BPT>
+void emu_solve_instance_bd(const BArgs& a, int inst, double* y, double* red) {
+  threadIdx.x = 0;
+  blockDim.x = 1;
+  solve_instance_bd<N, BPT>(a, inst, y, red);
+}
+#endif
 }  // namespace pdq

@@ -20,6 +20,7 @@
 
 #include "pdq_args.h"
 #include "pdq_device.h"
+#include "pdq_iwp.h"
 
 namespace {
 
@@ -64,66 +65,6 @@ int set_err(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return set_err(PDQ_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-// ------------------------------------------------- IWP system matrices
-// utilities.py:57-71: A = flip(Pascal) (binomials C(nu-i, nu-j)), Q = lower
-// Cholesky factor of the flipped Hilbert matrix 1/(2nu+1-i-j) with positive
-// diagonal.  Q is formed as in the reference: Kahan's recurrence for the
-// Hilbert Cholesky factor (cholesky_util.py:106-176), row flip, one
-// Householder triangularisation and a sign fix.
-double binom(int n, int k) {
-  if (k < 0 || k > n) return 0.0;
-  double r = 1.0;
-  for (int i = 1; i <= k; ++i) r = r * (double)(n - k + i) / (double)i;
-  return r;
-}
-
-void iwp_matrices(int nu, std::vector<double>& A, std::vector<double>& Q) {
-  const int n = nu + 1;
-  A.assign(n * n, 0.0);
-  for (int i = 0; i < n; ++i)
-    for (int j = i; j < n; ++j) A[i * n + j] = binom(nu - i, nu - j);
-  // Kahan: U upper with H = U^T U ... columns scaled by sqrt(2i+1) f_j
-  std::vector<double> f(n, 1.0), U(n * n, 0.0);
-  for (int k = 1; k < n; ++k) f[k] = (((f[k - 1] / k) * (2.0 * k)) / k) * (2.0 * k + 1.0);
-  for (int i = 0; i < n; ++i) U[i * n + i] = 1.0;
-  for (int j = 1; j < n; ++j)
-    for (int i = j - 1; i >= 0; --i) U[i * n + j] = (U[(i + 1) * n + j] / (double)(j - i)) * (double)(i + j + 2);
-  std::vector<double> Lh(n * n, 0.0);  // lower factor = U^T scaled
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      const double u = U[i * n + j] * (std::sqrt(2.0 * i + 1.0) * (1.0 / f[j]));
-      if (j >= i) Lh[j * n + i] = u;
-    }
-  // M = (row-flipped Lh)^T, triangularise, transpose back
-  std::vector<double> M(n * n);
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) M[i * n + j] = Lh[(n - 1 - j) * n + i];
-  for (int i = 0; i < n - 1; ++i) {
-    double ss = 0.0;
-    for (int r = i + 1; r < n; ++r) ss += M[r * n + i] * M[r * n + i];
-    if (ss == 0.0) continue;
-    const double alpha = M[i * n + i];
-    const double beta = -std::copysign(std::hypot(alpha, std::sqrt(ss)), alpha);
-    const double tau = (beta - alpha) / beta, scal = 1.0 / (alpha - beta);
-    for (int r = i + 1; r < n; ++r) M[r * n + i] *= scal;
-    for (int j = i + 1; j < n; ++j) {
-      double w = M[i * n + j];
-      for (int r = i + 1; r < n; ++r) w += M[r * n + i] * M[r * n + j];
-      w *= tau;
-      M[i * n + j] -= w;
-      for (int r = i + 1; r < n; ++r) M[r * n + j] -= M[r * n + i] * w;
-    }
-    M[i * n + i] = beta;
-  }
-  Q.assign(n * n, 0.0);
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j <= i; ++j) Q[i * n + j] = M[j * n + i];
-  for (int j = 0; j < n; ++j) {
-    const double s = Q[j * n + j] < 0 ? -1.0 : 1.0;
-    for (int i = 0; i < n; ++i) Q[i * n + j] *= s;
-  }
-}
-
 struct DevMats {
   double* A = nullptr;
   double* Q = nullptr;
@@ -145,7 +86,7 @@ int device_mats(const pdq_config& cfg, bool latent_1d, DevMats& out) {
     return PDQ_OK;
   }
   std::vector<double> A1, Q1;
-  iwp_matrices(cfg.num_derivatives, A1, Q1);
+  pdq::iwp_matrices(cfg.num_derivatives, A1, Q1);
   const int N = cfg.num_derivatives + 1, d = cfg.ode_dim;
   std::vector<double> A, Q;
   if (!latent_1d && cfg.factorisation == PDQ_DENSE) {

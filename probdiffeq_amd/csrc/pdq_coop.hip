@@ -1237,6 +1237,7 @@ __device__ void solve_instance(const Ctx<G>& c, int inst) {
   }
 }
 
+#ifndef PDQ_HOST_EMU
 template <int G>
 __global__ void __launch_bounds__(64) solve_kernel(KArgs args) {
   extern __shared__ double lds[];
@@ -1353,4 +1354,33 @@ int launch_coop(KArgs& a, int G, hipStream_t st) {
 }
 
 
+#else
+// CPU build of one instance (G = 1); ws must hold L.inst_size doubles and
+// shared must hold L.wg_shared doubles.
+inline void emu_solve_instance(const KArgs& a, int inst, double* shared, double* ws) {
+  const Layout& L = a.lay;
+  const int nc = L.nc;
+  for (int i = 0; i < nc * nc; ++i) {
+    shared[i] = a.A[i];
+    shared[nc * nc + i] = a.Q[i];
+  }
+  Ctx<1> c;
+  c.g.l = 0;
+  c.a = &a;
+  c.L = &L;
+  c.nc = L.nc;
+  c.mc = L.mc;
+  c.mcp = L.mcp;
+  c.ko = L.ko;
+  c.N = L.N;
+  c.d = L.d;
+  c.dense = a.cfg.factorisation == PDQ_DENSE;
+  c.sA = shared;
+  c.sQ = shared + nc * nc;
+  c.ws = ws;
+  c.prm = ws + L.w_prm;
+  if (a.mode == MODE_LML) lml_instance(c, inst);
+  else solve_instance(c, inst);
+}
+#endif
 }  // namespace pdq

@@ -5,7 +5,11 @@
 // IVP instance).
 #pragma once
 
+#ifdef PDQ_HOST_EMU
+#include "pdq_hostemu.h"
+#else
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #include "../../include/pdq.h"

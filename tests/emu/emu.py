@@ -1,0 +1,76 @@
+"""Loader of the CPU build of the kernels (test infrastructure)."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(BUILD, "libpdqemu.so")
+
+
+def build(extra=()) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    src = os.path.join(HERE, "pdq_emu.cpp")
+    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-DPDQ_HOST_EMU", *extra, "-o", LIB, src]
+    subprocess.run(cmd, check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+    return _lib
+
+
+def solve(cfg, params, tcoeffs, times, fixed_grid=False):
+    """cfg: probdiffeq_amd.capi.Config; numpy inputs; returns numpy outputs."""
+    from probdiffeq_amd import capi
+
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    tcoeffs = np.ascontiguousarray(tcoeffs, dtype=np.float64)
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    B, K = times.shape
+    ms, cs, nn, nv = cfg.mean_size, cfg.chol_size, cfg.nblk * cfg.n * cfg.n, cfg.nblk * cfg.n
+    out = {
+        "t": np.zeros((B, K)),
+        "mean": np.zeros((B, K, ms)),
+        "chol": np.zeros((B, K, cs)),
+        "output_scale": np.zeros((B, K, cfg.nblk)),
+        "num_steps": np.zeros((B, K - 1), dtype=np.int64),
+        "attempts": np.zeros(B, dtype=np.int64),
+        "accepted": np.zeros(B, dtype=np.int64),
+        "status": np.zeros(B, dtype=np.int32),
+    }
+    if cfg.strategy != "filter":
+        out.update(
+            filt_mean=np.zeros((B, K, ms)),
+            filt_chol=np.zeros((B, K, cs)),
+            cond_A=np.zeros((B, K - 1, nn)),
+            cond_mean=np.zeros((B, K - 1, ms)),
+            cond_chol=np.zeros((B, K - 1, cs)),
+            cond_tl=np.zeros((B, K - 1, nv)),
+            cond_to=np.zeros((B, K - 1, nv)),
+            term_mean=np.zeros((B, ms)),
+            term_chol=np.zeros((B, cs)),
+        )
+    s = capi._Solution()
+    for name, _ in capi._Solution._fields_:
+        a = out.get(name)
+        setattr(s, name, a.ctypes.data if a is not None else None)
+    c = cfg.c()
+    P = ctypes.c_void_p
+    rc = lib().pdqemu_solve(ctypes.byref(c), B, P(params.ctypes.data), P(tcoeffs.ctypes.data), P(times.ctypes.data), K, ctypes.byref(s), int(fixed_grid))
+    if rc:
+        raise RuntimeError(f"emulation rc {rc}")
+    return out
