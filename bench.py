@@ -1,0 +1,258 @@
+"""Benchmark: batched adaptive IVP solves/s (and LML gradients/s) on one MI355X.
+
+Workload (one "step" = one batched launch over the full batch, inputs
+resident in HBM): 4096 Lotka-Volterra instances, isotropic IWP prior with
+nu = 4, ts0, fixed-point smoother, dynamic calibration, rtol 1e-6 /
+atol 1e-8, save_at = linspace(0, t_end_i, 200) with t_end_i ~ U[20, 80],
+parameters theta_0 * exp(U(ln 0.8, ln 1.2)) and y0 ~ U[10, 30]^2 drawn with
+numpy default_rng(0).  Data are synthetic.
+
+Also measured (reported as extra fields): the plain drop-in (one solve per
+launch), the LML-gradient workload (1024 Lotka-Volterra instances,
+isotropic nu = 3, fixed grid of 2000 steps, 101 observations, gradient with
+respect to (a, b, c, d, y0)) against per-instance central finite
+differences of plain launches, and the CPU oracle on a bounded sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "IVP solves/s and log-lik grads/s at rtol 1e-6 FP64: batched/adjoint HIP vs plain"
+THETA0 = np.array([0.5, 0.05, 0.5, 0.05])
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and dense matrix)
+
+
+def c2_inputs(n_inst=4096, K=200, seed=0):
+    rng = np.random.default_rng(seed)
+    params = THETA0 * np.exp(rng.uniform(np.log(0.8), np.log(1.2), size=(n_inst, 4)))
+    y0 = rng.uniform(10.0, 30.0, size=(n_inst, 2))
+    t_end = rng.uniform(20.0, 80.0, size=n_inst)
+    save_at = np.stack([np.linspace(0.0, te, K) for te in t_end])
+    return params, y0, save_at
+
+
+def c5_inputs(n_inst=1024, seed=0):
+    """Parameter-estimation data: 101 noisy observations at t = 0.2 i on [0, 20]."""
+    from scipy.integrate import solve_ivp
+
+    rng = np.random.default_rng(seed)
+    params_true = THETA0 * np.exp(rng.uniform(np.log(0.8), np.log(1.2), size=(n_inst, 4)))
+    y0_true = rng.uniform(10.0, 30.0, size=(n_inst, 2))
+    t_obs = 0.2 * np.arange(101)
+    data = np.zeros((n_inst, 101, 2))
+    n_sim = min(n_inst, 64)
+    for i in range(n_sim):
+        a, b, c, d = params_true[i]
+        s = solve_ivp(lambda t, y: [a * y[0] - b * y[0] * y[1], -c * y[1] + d * y[0] * y[1]], (0, 20), y0_true[i], method="DOP853", rtol=1e-10, atol=1e-10, t_eval=t_obs)
+        data[i] = s.y.T
+    for i in range(n_sim, n_inst):
+        data[i] = data[i % n_sim]
+    noise = rng.normal(size=data.shape)
+    data_noisy = data + 0.1 * np.abs(data) * noise
+    stds = 0.1 * np.sqrt(np.mean(data**2, axis=-1))  # isotropic: one scalar per time point
+    grid = np.linspace(0.0, 20.0, 2001)
+    params = np.tile(THETA0, (n_inst, 1))
+    y0 = np.tile([20.0, 20.0], (n_inst, 1))
+    return params, y0, grid, data_noisy, stds[..., None]
+
+
+def _events():
+    import torch
+
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def bench_c2(args, torch, capi, flops):
+    params, y0, save_at = c2_inputs(args.instances)
+    dev = torch.device("cuda")
+    P = torch.tensor(params, dtype=torch.float64, device=dev)
+    Y0 = torch.tensor(y0, dtype=torch.float64, device=dev)
+    T = torch.tensor(save_at, dtype=torch.float64, device=dev)
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=4, ode_dim=2, strategy="fixedpoint", solver="dynamic", rtol=1e-6, atol=1e-8, lanes_per_instance=args.lanes)
+    tc = capi.taylor_coefficients("lotka_volterra", P, Y0, 0.0, 4)
+    out = capi.alloc_solution(cfg, P.shape[0], T.shape[1], dev)
+    for _ in range(args.warmup):
+        capi.solve(cfg, P, tc, T, out=out)
+    torch.cuda.synchronize()
+    st = out["status"].cpu().numpy()
+    if np.any(st != 0):
+        raise SystemExit(f"solver status != 0 for {(st != 0).sum()} instances")
+    times = []
+    t_wall0 = time.perf_counter()
+    for _ in range(args.steps):
+        e0, e1 = _events()
+        e0.record()
+        capi.solve(cfg, P, tc, T, out=out)
+        e1.record()
+        times.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall0
+    ms = [a.elapsed_time(b) for a, b in times]
+    ms_step = float(np.mean(ms))
+    attempts = out["attempts"].cpu().numpy()
+    fl = flops.solve(cfg, float(attempts.sum()), 0) + P.shape[0] * (T.shape[1] - 1) * (flops.interpolation(cfg) + flops.finalize_per_checkpoint(cfg))
+    res = {
+        "ms_per_step": ms_step,
+        "wall_s": wall,
+        "solves_per_s": P.shape[0] / (ms_step / 1e3),
+        "attempts_total": int(attempts.sum()),
+        "attempts_min_max": [int(attempts.min()), int(attempts.max())],
+        "flops_per_launch": fl,
+        "achieved_tflops": fl / (ms_step / 1e3) / 1e12,
+    }
+    # plain drop-in: one solve per launch, instances one after another
+    n_plain = min(args.plain_sample, P.shape[0])
+    cfg1 = capi.Config(**{**cfg.__dict__, "lanes_per_instance": 64})
+    outs1 = capi.alloc_solution(cfg1, 1, T.shape[1], dev)
+    capi.solve(cfg1, P[:1].contiguous(), tc[:1].contiguous(), T[:1].contiguous(), out=outs1)
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    e0.record()
+    for i in range(n_plain):
+        capi.solve(cfg1, P[i : i + 1], tc[i : i + 1], T[i : i + 1], out=outs1)
+    e1.record()
+    torch.cuda.synchronize()
+    plain_ms = e0.elapsed_time(e1)
+    res["plain_solves_per_s"] = n_plain / (plain_ms / 1e3)
+    res["plain_sample"] = n_plain
+    return res, cfg, params, y0, save_at
+
+
+def bench_c5(args, torch, capi, flops):
+    params, y0, grid, data, stds = c5_inputs(args.lml_instances)
+    dev = torch.device("cuda")
+    P = torch.tensor(params, dtype=torch.float64, device=dev)
+    Y0 = torch.tensor(y0, dtype=torch.float64, device=dev)
+    G = torch.tensor(grid, dtype=torch.float64, device=dev)
+    D = torch.tensor(data, dtype=torch.float64, device=dev)
+    S = torch.tensor(stds, dtype=torch.float64, device=dev)
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver")
+    for _ in range(max(1, args.warmup)):
+        capi.lml_value_and_grad(cfg, P, Y0, G, 20, D, S)
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    reps = max(1, args.steps)
+    e0.record()
+    for _ in range(reps):
+        val, grad = capi.lml_value_and_grad(cfg, P, Y0, G, 20, D, S)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    res = {"lml_grads_per_s": P.shape[0] / (ms / 1e3), "lml_ms_per_batch": ms}
+    # plain: central differences (h = 1e-6) of single-instance value launches, 13 per gradient
+    n_plain = min(args.plain_sample, 8)
+    e0, e1 = _events()
+    e0.record()
+    h = 1e-6
+    for i in range(n_plain):
+        p, y = P[i : i + 1].clone(), Y0[i : i + 1].clone()
+        capi.lml_value_and_grad(cfg, p, y, G, 20, D[i : i + 1], S[i : i + 1], grad=False)
+        for j in range(6):
+            for sgn in (1.0, -1.0):
+                pp, yy = p.clone(), y.clone()
+                if j < 4:
+                    pp[0, j] += sgn * h
+                else:
+                    yy[0, j - 4] += sgn * h
+                capi.lml_value_and_grad(cfg, pp, yy, G, 20, D[i : i + 1], S[i : i + 1], grad=False)
+    e1.record()
+    torch.cuda.synchronize()
+    res["plain_fd_grads_per_s"] = n_plain / (e0.elapsed_time(e1) / 1e3)
+    res["lml_value_sample"] = float(val[0].cpu())
+    res["lml_grad_sample"] = grad[0].cpu().numpy().tolist()
+    return res
+
+
+def cpu_baseline(params, y0, save_at, n_sample, threads):
+    from oracle import oracle
+
+    cfg = oracle.Config(fact="isotropic", nu=4, d=2, strategy="fixedpoint", solver="dynamic", rtol=1e-6, atol=1e-8)
+    idx = np.arange(n_sample)
+    tcs = np.stack([oracle.taylor_coefficients("lotka_volterra", params[i], y0[i], 0.0, 4) for i in idx])
+    oracle.solve_save_at_batch(cfg, params[idx[:8]], tcs[:8], save_at[idx[:8]], n_threads=threads)
+    t0 = time.perf_counter()
+    r = oracle.solve_save_at_batch(cfg, params[idx], tcs, save_at[idx], n_threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n_sample / dt, "unit": "solves/s", "cores": threads, "kind": "port", "sample": f"{n_sample} of the 4096 C2 instances (first by seed), OpenMP over instances, {dt:.2f} s wall", "attempts": int(r["attempts"].sum())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--lml-instances", type=int, default=1024)
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--skip-lml", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; instances shard across GPUs with no exchange (see the design notes), multi-GPU runs are not part of it"}))
+        raise SystemExit(2)
+
+    import torch
+
+    from probdiffeq_amd import capi, flops
+
+    capi.require_device()
+    c2, cfg, params, y0, save_at = bench_c2(args, torch, capi, flops)
+    c5 = {} if args.skip_lml else bench_c5(args, torch, capi, flops)
+    threads = min(16, os.cpu_count() or 1)
+    cpu = None if args.skip_cpu else cpu_baseline(params, y0, save_at, args.cpu_sample, threads)
+    line = {
+        "metric": METRIC,
+        "value": c2["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": c2["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "4096 Lotka-Volterra instances, isotropic nu=4, ts0, fixed-point smoother, dynamic calibration, rtol 1e-6 atol 1e-8, 200 save_at points per instance, t_end ~ U[20,80]",
+            "instances": args.instances,
+            "save_at": 200,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "compute": "FP64 vector ALU (no MFMA; FP64 vector peak equals the FP64 matrix peak, 78.6 TFLOP/s)",
+            "achieved": c2["achieved_tflops"],
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": c2["achieved_tflops"] / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "flops_per_launch": c2["flops_per_launch"],
+        },
+        "cpu_baseline": cpu,
+        "plain_solves_per_s": c2["plain_solves_per_s"],
+        "batched_vs_plain": c2["solves_per_s"] / c2["plain_solves_per_s"],
+        "attempts_total": c2["attempts_total"],
+    }
+    if c5:
+        line.update(
+            lml_grads_per_s=c5["lml_grads_per_s"],
+            plain_fd_grads_per_s=c5["plain_fd_grads_per_s"],
+            adjoint_vs_plain=c5["lml_grads_per_s"] / c5["plain_fd_grads_per_s"],
+        )
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -8,6 +8,7 @@ package never does.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import dataclasses
 import os
@@ -17,6 +18,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libpdq_oracle.so")
+_LIB_PATHS = {"strict": _LIB_PATH, "fma": os.path.join(_HERE, "_build", "libpdq_oracle_fma.so")}
 
 FACT = {"dense": 0, "isotropic": 1, "blockdiag": 2}
 CONSTRAINT = {"ts0": 0, "ts1": 1}
@@ -158,21 +160,41 @@ class Config:
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (g++, no GPU needed)."""
-    if force or not os.path.exists(_LIB_PATH):
+    """Compile both oracle variants with the Makefile (g++, no GPU needed)."""
+    if force or not all(os.path.exists(p) for p in _LIB_PATHS.values()):
         subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
     return _LIB_PATH
 
 
-_lib = None
+_libs = {}
+_variant = "strict"
+
+
+@contextlib.contextmanager
+def rounding(name: str):
+    """Run oracle calls inside the block on the given build ("strict" or "fma").
+
+    The "fma" build is the same restatement compiled with FMA contraction; the
+    difference between the two builds' outputs is the rounding sensitivity of
+    each output, used to scale parity tolerances of ill-conditioned quantities.
+    """
+    global _variant
+    if name not in _LIB_PATHS:
+        raise ValueError(f"unknown oracle build {name!r}")
+    prev, _variant = _variant, name
+    try:
+        yield
+    finally:
+        _variant = prev
 
 
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+    L = _libs.get(_variant)
+    if L is None:
+        path = _LIB_PATHS[_variant]
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         L.pdqo_solve_save_at.restype = ctypes.c_int
         L.pdqo_solve_fixed_grid.restype = ctypes.c_int
         L.pdqo_lml.restype = ctypes.c_double
@@ -181,8 +203,8 @@ def lib():
         L.pdqo_logpdf.restype = ctypes.c_double
         L.pdqo_control.restype = ctypes.c_double
         L.pdqo_solve_save_at_batch.restype = ctypes.c_int
-        _lib = L
-    return _lib
+        _libs[_variant] = L
+    return L
 
 
 def _p(a):

@@ -3,7 +3,9 @@
 #pragma once
 
 #ifdef PDQ_HOST_EMU
+#ifndef PDQ_EMU_SHIM_INCLUDED
 #include "pdq_hostemu.h"
+#endif
 #else
 #include <hip/hip_runtime.h>
 #endif

@@ -53,7 +53,7 @@ __device__ __forceinline__ void qr_regs(double (&A)[M * C]) {
 }
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
-#ifdef PDQ_HOST_EMU
+#if defined(PDQ_HOST_EMU) && !defined(PDQ_NO_EMU_BLOCKSUM)
   return v;  // one thread per workgroup in the CPU build
 #endif
 #pragma unroll

@@ -6,7 +6,9 @@
 #pragma once
 
 #ifdef PDQ_HOST_EMU
+#ifndef PDQ_EMU_SHIM_INCLUDED
 #include "pdq_hostemu.h"
+#endif
 #else
 #include <hip/hip_runtime.h>
 #endif
@@ -209,12 +211,17 @@ struct Grp {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
-  // Butterfly all-reduce; every lane ends with the bitwise same value.
+  // Group sum.  A butterfly leaves each lane with the total in its own
+  // association order (lane-dependent rounding), so lane 0's total is
+  // broadcast: every lane must take the same accept/reject decisions and
+  // build the same Householder reflectors.
   __device__ __forceinline__ double sum(double v) const {
 #pragma unroll
     for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
+    if constexpr (G > 1) return __shfl(v, 0, G);
     return v;
   }
+  // fmax is exact, so the butterfly result is the same in every lane.
   __device__ __forceinline__ double max(double v) const {
 #pragma unroll
     for (int m = G / 2; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, G));

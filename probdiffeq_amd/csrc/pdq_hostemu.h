@@ -25,6 +25,7 @@ inline void __syncthreads() {}
 inline void __threadfence_block() {}
 inline double __shfl_xor(double v, int, int) { return v; }
 inline int __shfl(int v, int, int) { return v; }
+inline double __shfl(double v, int, int) { return v; }
 inline int atomicAdd(int* p, int v) {
   const int old = *p;
   *p += v;

@@ -25,3 +25,7 @@ def oracle():
 
     O.build()
     return O
+
+TESTS = os.path.dirname(os.path.abspath(__file__))
+if TESTS not in sys.path:
+    sys.path.insert(0, TESTS)

@@ -74,3 +74,17 @@ def solve(cfg, params, tcoeffs, times, fixed_grid=False):
     if rc:
         raise RuntimeError(f"emulation rc {rc}")
     return out
+
+
+def lml(cfg, params, y0, grid, every, data, stds, average=True, grad=True):
+    params, y0, grid, data, stds = (np.ascontiguousarray(x, dtype=np.float64) for x in (params, y0, grid, data, stds))
+    B = params.shape[0]
+    value = np.zeros(B)
+    g = np.zeros((B, params.shape[1] + y0.shape[1])) if grad else None
+    n_std = stds.shape[-1] if stds.ndim == 3 else 1
+    c = cfg.c()
+    P = ctypes.c_void_p
+    rc = lib().pdqemu_lml(ctypes.byref(c), B, P(params.ctypes.data), P(y0.ctypes.data), P(grid.ctypes.data), grid.size, every, P(data.ctypes.data), P(stds.ctypes.data), n_std, int(average), P(value.ctypes.data), P(g.ctypes.data) if grad else None)
+    if rc:
+        raise RuntimeError(f"emulation rc {rc}")
+    return value, g

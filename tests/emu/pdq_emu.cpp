@@ -76,4 +76,67 @@ int pdqemu_solve(const pdq_config* cfg, int n_inst, const double* params, const 
   return 0;
 }
 
+
+// LML value and forward-mode gradient (MODE_LML) on the CPU.
+int pdqemu_lml(const pdq_config* cfg, int n_inst, const double* params, const double* y0, const double* grid, int K,
+               int every, const double* data, const double* stds, int n_std, int average, double* value,
+               double* grad) {
+  const int n_tan = grad ? cfg->n_params + cfg->ode_dim : 0;
+  const int N = cfg->num_derivatives + 1, d = cfg->ode_dim, np_ = cfg->n_params;
+  pdq::Layout L = pdq::make_layout(*cfg, n_tan);
+  std::vector<double> A, Q;
+  if (cfg->factorisation == PDQ_DENSE) pdq::iwp_dense(cfg->num_derivatives, d, cfg->base_output_scale, A, Q);
+  else pdq::iwp_matrices(cfg->num_derivatives, A, Q);
+  // initial Taylor coefficients with tangents (as taylor_tangent_kernel)
+  std::vector<double> tinit((size_t)n_inst * N * d * (1 + n_tan));
+  for (int inst = 0; inst < n_inst; ++inst) {
+    pdq::Dual prmD[pdq::kMaxParams], cc[64];
+    for (int q = 0; q < np_; ++q) {
+      prmD[q] = pdq::dconst(params[(size_t)inst * np_ + q]);
+      if (q < n_tan) prmD[q].g[q] = 1.0;
+    }
+    for (int k = 0; k < d; ++k) {
+      cc[k] = pdq::dconst(y0[(size_t)inst * d + k]);
+      if (np_ + k < n_tan) cc[k].g[np_ + k] = 1.0;
+    }
+    for (int k = 0; k + 1 < N; ++k)
+      for (int j = 0; j < d; ++j)
+        cc[(k + 1) * d + j] = pdq::taylor_rhs(cfg->vector_field, d, cc, prmD, grid[0], k, j) / (double)(k + 1);
+    for (int i = 0; i < N; ++i)
+      for (int k = 0; k < d; ++k) {
+        const pdq::Dual v = pdq::factorial_d(i) * cc[i * d + k];
+        for (int q = 0; q <= n_tan; ++q)
+          tinit[(((size_t)inst * N + i) * d + k) * (1 + n_tan) + q] = q == 0 ? v.v : v.g[q - 1];
+      }
+  }
+  const int Kobs = (K - 1) / every + 1;
+  const size_t csz = 2 * (size_t)L.nc * L.nc + (size_t)L.nc * L.mc + 2 * (size_t)L.nc;
+  std::vector<double> scratch(csz * (size_t)(Kobs > 1 ? Kobs - 1 : 1) * n_inst);
+  int queue = 0;
+  pdq::KArgs a{};
+  a.cfg = *cfg;
+  a.lay = L;
+  a.n_inst = n_inst;
+  a.K = K;
+  a.mode = pdq::MODE_LML;
+  a.every = every;
+  a.n_tan = n_tan;
+  a.average_pdfs = average;
+  a.n_std = n_std;
+  a.params = params;
+  a.tcoeffs = tinit.data();
+  a.times = grid;
+  a.data = data;
+  a.stds = stds;
+  a.A = A.data();
+  a.Q = Q.data();
+  a.lml_value = value;
+  a.lml_grad = grad;
+  a.scratch = scratch.data();
+  a.queue = &queue;
+  std::vector<double> shared(L.wg_shared), ws(L.inst_size);
+  for (int i = 0; i < n_inst; ++i) pdq::emu_solve_instance(a, i, shared.data(), ws.data());
+  return 0;
+}
+
 }  // extern "C"

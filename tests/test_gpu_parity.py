@@ -1,17 +1,22 @@
 """GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
 
-Tolerances (normwise relative per output array, max |gpu - oracle| <=
-tol * max |oracle|):
-  * step counts (attempts, accepted steps, num_steps) and times: identical;
-  * the ODE solution u(t) (Taylor coefficient 0) and its covariance block:
-    1e-10;
-  * the full state (all Taylor coefficients, full covariances, backward
-    conditionals): 1e-9.  The top Taylor coefficients are ill-conditioned:
-    the oracle compiled with and without FMA contraction already differs by
-    6.5e-11 there on the C1 solve (coefficient 0: 1.3e-14), and the kernels
-    contract to FMA and reduce in tree order.
-Covariances are compared as L L^T; factor signs (LAPACK convention) are
-checked on the dense filter solve.
+Step counts (attempts, accepted steps, num_steps) and the output times are
+identical.  Every floating-point output array is compared normwise relative
+(max |gpu - oracle| <= tol * max |oracle|) with
+
+    tol = max(1e-10, 10 * spread),
+
+where ``spread`` is the same normwise difference between the oracle compiled
+with strict evaluation order and the oracle compiled with FMA contraction: two
+valid FP64 roundings of the same algorithm.  Where an output is well
+conditioned (the ODE solution, most covariances) the spread is ~1e-14 and the
+bound is 1e-10.  Some outputs are not: the top Taylor coefficients (scaled by
+dt^-k), the coefficient-0 covariance block (variances ~1e-12 inside
+covariances dominated by the top coefficients) and, on stiff van der Pol with
+ts1, a dynamic calibration whose whitened residual is at cancellation level;
+there the two oracle builds already differ by 1e-9 .. 1e-3 and the bound
+follows.  Covariances are compared as L L^T; factor signs (LAPACK convention)
+are checked on the dense filter solve.
 """
 
 import numpy as np
@@ -22,8 +27,8 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 THETA = np.array([0.5, 0.05, 0.5, 0.05])
-RTOL_U0 = 1e-10
-RTOL = 1e-9
+RTOL_FLOOR = 1e-10
+SENSITIVITY_FACTOR = 10.0
 
 
 def _rel(a, b):
@@ -89,6 +94,39 @@ def _oracle_cfg(oracle, cfg):
     )
 
 
+def _fields(cfg, r):
+    """The compared output arrays of one solve (covariances as L L^T)."""
+    n, nblk = cfg.n, cfg.nblk
+    cov = _cov(r["chol"], n, nblk)
+    out = {"u0": _u0(cfg, r["mean"]), "mean": r["mean"], "cov0": _cov0(cfg, cov), "cov": cov, "output_scale": r["output_scale"]}
+    if cfg.strategy != "filter":
+        out.update(
+            filt_mean=r["filt_mean"],
+            filt_cov=_cov(r["filt_chol"], n, nblk),
+            cond_A=r["cond_A"],
+            cond_mean=r["cond_mean"],
+            term_mean=r["term_mean"],
+        )
+    return out
+
+
+def _oracle_solve(oracle, ocfg, params, tcs, times, fixed_grid):
+    f = oracle.solve_fixed_grid if fixed_grid else oracle.solve_save_at
+    strict = f(ocfg, params, tcs, times)
+    with oracle.rounding("fma"):
+        fma = f(ocfg, params, tcs, times)
+    assert strict["status"] == 0 and fma["status"] == 0
+    assert strict["attempts"] == fma["attempts"]
+    return strict, fma
+
+
+def _check_fields(cfg, got, strict, fma, label):
+    for key, exp in _fields(cfg, strict).items():
+        tol = max(RTOL_FLOOR, SENSITIVITY_FACTOR * _rel(_fields(cfg, fma)[key], exp))
+        err = _rel(got[key][: exp.shape[0]] if key == "output_scale" else got[key], exp)
+        assert err <= tol, f"{label}: {key} rel err {err:.3e} > tol {tol:.3e}"
+
+
 def _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=False, check_factor_signs=False):
     """Solve a batch on the GPU and every instance with the oracle; compare."""
     B, K = times.shape
@@ -99,32 +137,17 @@ def _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=False, check_facto
     g = {k: v.cpu().numpy() for k, v in out.items()}
     n, nblk = cfg.n, cfg.nblk
     for i in range(B):
-        if fixed_grid:
-            r = oracle.solve_fixed_grid(ocfg, params[i], tcs[i], times[i])
-        else:
-            r = oracle.solve_save_at(ocfg, params[i], tcs[i], times[i])
-        assert r["status"] == 0
+        r, rf = _oracle_solve(oracle, ocfg, params[i], tcs[i], times[i], fixed_grid)
         assert g["status"][i] == 0, f"instance {i} status {g['status'][i]}"
         assert g["attempts"][i] == r["attempts"], (i, g["attempts"][i], r["attempts"])
         assert g["accepted"][i] == r["accepted"]
         assert np.array_equal(g["num_steps"][i], r["num_steps"])
         assert _rel(g["t"][i], r["t"]) < 1e-14
-        assert _rel(_u0(cfg, g["mean"][i]), _u0(cfg, r["mean"])) < RTOL_U0, _rel(_u0(cfg, g["mean"][i]), _u0(cfg, r["mean"]))
-        assert _rel(g["mean"][i], r["mean"]) < RTOL, _rel(g["mean"][i], r["mean"])
-        cg, co = _cov(g["chol"][i], n, nblk), _cov(r["chol"], n, nblk)
-        assert _rel(_cov0(cfg, cg), _cov0(cfg, co)) < RTOL_U0
-        assert _rel(cg, co) < RTOL
-        os_rows = r["output_scale"].shape[0]
-        assert _rel(g["output_scale"][i][:os_rows], r["output_scale"]) < RTOL
+        got = {k: g[k][i] for k in ("mean", "chol", "output_scale", "filt_mean", "filt_chol", "cond_A", "cond_mean", "term_mean") if k in g}
+        _check_fields(cfg, _fields(cfg, got), r, rf, f"instance {i}")
         if check_factor_signs:
-            assert _rel(g["chol"][i], r["chol"]) < RTOL
+            assert _rel(g["chol"][i], r["chol"]) <= max(RTOL_FLOOR, SENSITIVITY_FACTOR * _rel(rf["chol"], r["chol"]))
             assert np.all(np.sign(np.diagonal(g["chol"][i].reshape(-1, n, n), axis1=1, axis2=2)) == np.sign(np.diagonal(r["chol"].reshape(-1, n, n), axis1=1, axis2=2)))
-        if cfg.strategy != "filter":
-            assert _rel(g["filt_mean"][i], r["filt_mean"]) < RTOL
-            assert _rel(_cov(g["filt_chol"][i], n, nblk), _cov(r["filt_chol"], n, nblk)) < RTOL
-            assert _rel(g["cond_A"][i], r["cond_A"]) < RTOL
-            assert _rel(g["cond_mean"][i], r["cond_mean"]) < RTOL
-            assert _rel(g["term_mean"][i], r["term_mean"]) < RTOL
     return g
 
 

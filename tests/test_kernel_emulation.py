@@ -1,0 +1,113 @@
+"""The kernels' per-instance code, built for the CPU, against the oracle.
+
+``tests/emu/pdq_emu.cpp`` compiles the product's device sources with g++
+(-DPDQ_HOST_EMU, one lane per group, one thread per workgroup).  This pins
+the kernel logic -- loops, interpolation, smoother bookkeeping, status
+codes -- on the CPU; the GPU tests pin the same code executed cooperatively
+on gfx950.
+"""
+
+import numpy as np
+import pytest
+
+import test_gpu_parity as T  # noqa: E402  (shared comparison helpers)
+
+pytestmark = pytest.mark.filterwarnings("ignore")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    from emu import emu as E
+
+    E.build()
+    E._lib = None
+    return E
+
+
+def _run(oracle, emu, cfg, params, y0s, times, fixed_grid=False):
+    from probdiffeq_amd import capi
+
+    B, K = times.shape
+    ocfg = T._oracle_cfg(oracle, cfg)
+    tcs = np.stack([oracle.taylor_coefficients(cfg.vector_field, params[i], y0s[i], times[i, 0], cfg.num_derivatives) for i in range(B)])
+    g = emu.solve(cfg, params, tcs, times, fixed_grid)
+    for i in range(B):
+        r, rf = T._oracle_solve(oracle, ocfg, params[i], tcs[i], times[i], fixed_grid)
+        assert g["status"][i] == 0
+        assert g["attempts"][i] == r["attempts"]
+        assert np.array_equal(g["num_steps"][i], r["num_steps"])
+        assert T._rel(g["t"][i], r["t"]) < 1e-14
+        got = {k: g[k][i] for k in ("mean", "chol", "output_scale", "filt_mean", "filt_chol", "cond_A", "cond_mean", "term_mean") if k in g}
+        T._check_fields(cfg, T._fields(cfg, got), r, rf, f"instance {i}")
+    return g
+
+
+def test_c1(oracle, emu):
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="dense", strategy="filter", solver="dynamic")
+    _run(oracle, emu, cfg, T.THETA[None], np.array([[20.0, 20.0]]), np.linspace(0, 50, 200)[None])
+
+
+@pytest.mark.parametrize("fact", ["dense", "isotropic"])
+@pytest.mark.parametrize("strategy", ["filter", "fixedpoint", "fixedinterval"])
+@pytest.mark.parametrize("solver", ["solver", "dynamic", "mle"])
+def test_strategy_solver_grid(oracle, emu, fact, strategy, solver):
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation=fact, num_derivatives=3, strategy=strategy, solver=solver, rtol=1e-5, atol=1e-7)
+    p, y, t = T._lv_batch(4, seed=7, t_lo=3.0, t_hi=8.0, K=9)
+    _run(oracle, emu, cfg, p, y, t)
+
+
+def test_c3_ts1(oracle, emu):
+    from probdiffeq_amd import capi
+
+    rng = np.random.default_rng(3)
+    mu = np.exp(rng.uniform(np.log(1.0), np.log(100.0), size=(3, 1)))
+    cfg = capi.Config(factorisation="dense", num_derivatives=5, constraint="ts1", vector_field="van_der_pol", n_params=1, rtol=1e-5, atol=1e-7)
+    _run(oracle, emu, cfg, mu, np.tile([2.0, 0.0], (3, 1)), np.tile([0.0, 6.3], (3, 1)))
+
+
+@pytest.mark.parametrize("fact", ["dense", "isotropic"])
+def test_fixed_grid(oracle, emu, fact):
+    from probdiffeq_amd import capi
+
+    p, y, _ = T._lv_batch(3, seed=11)
+    cfg = capi.Config(factorisation=fact, num_derivatives=3, strategy="fixedinterval", solver="dynamic")
+    _run(oracle, emu, cfg, p, y, np.tile(np.linspace(0.0, 4.0, 41), (3, 1)), fixed_grid=True)
+
+
+@pytest.mark.parametrize("solver", ["solver", "dynamic"])
+def test_blockdiag_lotka_volterra(oracle, emu, solver):
+    from probdiffeq_amd import capi
+
+    p, y, t = T._lv_batch(3, seed=5)
+    cfg = capi.Config(factorisation="blockdiag", num_derivatives=4, strategy="filter", solver=solver)
+    _run(oracle, emu, cfg, p, y, t)
+
+
+def test_blockdiag_burgers(oracle, emu):
+    from probdiffeq_amd import capi
+
+    Ng = 64
+    rng = np.random.default_rng(0)
+    x = np.arange(Ng) / Ng
+    y0 = np.stack([sum(rng.normal(0, 1 / k) * np.sin(2 * np.pi * k * x) for k in range(1, 9)) for _ in range(2)])
+    params = np.tile([0.01, 1.0 / Ng], (2, 1))
+    cfg = capi.Config(factorisation="blockdiag", num_derivatives=2, ode_dim=Ng, vector_field="burgers", n_params=2, rtol=1e-4, atol=1e-6)
+    _run(oracle, emu, cfg, params, y0, np.tile([0.0, 0.2], (2, 1)))
+
+
+def test_status_codes(emu):
+    from oracle import oracle as O
+    from probdiffeq_amd import capi
+
+    p, y, t = T._lv_batch(2, seed=1)
+    tcs = np.stack([O.taylor_coefficients("lotka_volterra", p[i], y[i], 0.0, 4) for i in range(2)])
+    g = emu.solve(capi.Config(rtol=1e-300, atol=1e-300, max_attempts=4), p, tcs, t)
+    assert g["status"].tolist() == [1, 1]
+    bad = tcs.copy()
+    bad[0, 1, 0] = np.nan
+    g = emu.solve(capi.Config(), p, bad, t)
+    assert g["status"].tolist() == [2, 0]
