@@ -186,6 +186,16 @@ def cpu_baseline(params, y0, save_at, n_sample, threads):
     return {"value": n_sample / dt, "unit": "solves/s", "cores": threads, "kind": "port", "sample": f"{n_sample} of the 4096 C2 instances (first by seed), OpenMP over instances, {dt:.2f} s wall", "attempts": int(r["attempts"].sum())}
 
 
+def copy_calibration(torch, n_bytes=1 << 30, reps=3):
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters."""
+    src = torch.ones(n_bytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n_bytes, "copy_bytes_written": n_bytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,15 +205,20 @@ def main():
     ap.add_argument("--lml-instances", type=int, default=1024)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=32)
-    ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--skip-lml", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true", help="only run a 1 GiB device-to-device copy (scales the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; instances shard across GPUs with no exchange (see the design notes), multi-GPU runs are not part of it"}))
         raise SystemExit(2)
 
     import torch
+
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
 
     from probdiffeq_amd import capi, flops
 

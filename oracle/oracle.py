@@ -18,7 +18,17 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libpdq_oracle.so")
-_LIB_PATHS = {"strict": _LIB_PATH, "fma": os.path.join(_HERE, "_build", "libpdq_oracle_fma.so")}
+_LIB_PATHS = {
+    "strict": _LIB_PATH,
+    "fma": os.path.join(_HERE, "_build", "libpdq_oracle_fma.so"),
+    "extended": os.path.join(_HERE, "_build", "libpdq_oracle_ext.so"),
+}
+# scalar type of each build: (numpy dtype, ctypes type)
+_REAL = {
+    "strict": (np.float64, ctypes.c_double),
+    "fma": (np.float64, ctypes.c_double),
+    "extended": (np.longdouble, ctypes.c_longdouble),
+}
 
 FACT = {"dense": 0, "isotropic": 1, "blockdiag": 2}
 CONSTRAINT = {"ts0": 0, "ts1": 1}
@@ -29,58 +39,84 @@ CONTROL = {"integral": 0, "proportional_integral": 1}
 VF = {"lotka_volterra": 0, "van_der_pol": 1, "burgers": 2, "linear": 3, "logistic": 4}
 
 
-class _Cfg(ctypes.Structure):
-    _fields_ = [
-        ("fact", ctypes.c_int),
-        ("nu", ctypes.c_int),
-        ("d", ctypes.c_int),
-        ("constraint", ctypes.c_int),
-        ("strategy", ctypes.c_int),
-        ("solver", ctypes.c_int),
-        ("error_norm", ctypes.c_int),
-        ("control", ctypes.c_int),
-        ("vf_id", ctypes.c_int),
-        ("clip_dt", ctypes.c_int),
-        ("rtol", ctypes.c_double),
-        ("atol", ctypes.c_double),
-        ("dt0", ctypes.c_double),
-        ("eps", ctypes.c_double),
-        ("damp", ctypes.c_double),
-        ("safety", ctypes.c_double),
-        ("factor_min", ctypes.c_double),
-        ("factor_max", ctypes.c_double),
-        ("exponent_integral", ctypes.c_double),
-        ("exponent_proportional", ctypes.c_double),
-        ("base_scale", ctypes.c_double),
-        ("correct_asymptotic_underconfidence", ctypes.c_int),
-        ("max_attempts", ctypes.c_int),
-    ]
+def _cfg_struct(ct):
+    class _Cfg(ctypes.Structure):
+        _fields_ = [
+            ("fact", ctypes.c_int),
+            ("nu", ctypes.c_int),
+            ("d", ctypes.c_int),
+            ("constraint", ctypes.c_int),
+            ("strategy", ctypes.c_int),
+            ("solver", ctypes.c_int),
+            ("error_norm", ctypes.c_int),
+            ("control", ctypes.c_int),
+            ("vf_id", ctypes.c_int),
+            ("clip_dt", ctypes.c_int),
+            ("rtol", ct),
+            ("atol", ct),
+            ("dt0", ct),
+            ("eps", ct),
+            ("damp", ct),
+            ("safety", ct),
+            ("factor_min", ct),
+            ("factor_max", ct),
+            ("exponent_integral", ct),
+            ("exponent_proportional", ct),
+            ("base_scale", ct),
+            ("correct_asymptotic_underconfidence", ctypes.c_int),
+            ("max_attempts", ctypes.c_int),
+        ]
+
+    return _Cfg
 
 
-_P = ctypes.POINTER(ctypes.c_double)
+def _out_struct(ct):
+    P = ctypes.POINTER(ct)
+
+    class _Out(ctypes.Structure):
+        _fields_ = [
+            ("t", P),
+            ("mean", P),
+            ("chol", P),
+            ("os", P),
+            ("num_steps", _PL),
+            ("filt_mean", P),
+            ("filt_chol", P),
+            ("cond_A", P),
+            ("cond_mean", P),
+            ("cond_chol", P),
+            ("cond_tl", P),
+            ("cond_to", P),
+            ("term_mean", P),
+            ("term_chol", P),
+            ("attempts", ctypes.c_long),
+            ("accepted", ctypes.c_long),
+            ("status", ctypes.c_int),
+        ]
+
+    return _Out
+
+
 _PL = ctypes.POINTER(ctypes.c_long)
+_CFG = {ct: _cfg_struct(ct) for ct in (ctypes.c_double, ctypes.c_longdouble)}
+_OUT = {ct: _out_struct(ct) for ct in (ctypes.c_double, ctypes.c_longdouble)}
 
 
-class _Out(ctypes.Structure):
-    _fields_ = [
-        ("t", _P),
-        ("mean", _P),
-        ("chol", _P),
-        ("os", _P),
-        ("num_steps", _PL),
-        ("filt_mean", _P),
-        ("filt_chol", _P),
-        ("cond_A", _P),
-        ("cond_mean", _P),
-        ("cond_chol", _P),
-        ("cond_tl", _P),
-        ("cond_to", _P),
-        ("term_mean", _P),
-        ("term_chol", _P),
-        ("attempts", ctypes.c_long),
-        ("accepted", ctypes.c_long),
-        ("status", ctypes.c_int),
-    ]
+def _dtype():
+    return _REAL[_variant][0]
+
+
+def _ct():
+    return _REAL[_variant][1]
+
+
+def _zeros(shape):
+    return np.zeros(shape, dtype=_dtype())
+
+
+def _s(x):
+    """A scalar argument in the active build's type."""
+    return _ct()(x)
 
 
 @dataclasses.dataclass
@@ -111,8 +147,8 @@ class Config:
     correct_asymptotic_underconfidence: bool = True
     max_attempts: int = 0
 
-    def c(self) -> _Cfg:
-        return _Cfg(
+    def c(self):
+        return _CFG[_ct()](
             FACT[self.fact],
             self.nu,
             self.d,
@@ -195,30 +231,37 @@ def lib():
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)
+        ct = _REAL[_variant][1]
+        if L.pdqo_real_size() != ctypes.sizeof(ct):
+            raise RuntimeError(f"oracle build {_variant!r} has a {L.pdqo_real_size()}-byte scalar, expected {ctypes.sizeof(ct)}")
         L.pdqo_solve_save_at.restype = ctypes.c_int
         L.pdqo_solve_fixed_grid.restype = ctypes.c_int
-        L.pdqo_lml.restype = ctypes.c_double
-        L.pdqo_lml_from_y0.restype = ctypes.c_double
-        L.pdqo_lml_grad_fd.restype = ctypes.c_double
-        L.pdqo_logpdf.restype = ctypes.c_double
-        L.pdqo_control.restype = ctypes.c_double
+        L.pdqo_lml.restype = ct
+        L.pdqo_lml_from_y0.restype = ct
+        L.pdqo_lml_grad_fd.restype = ct
+        L.pdqo_logpdf.restype = ct
+        L.pdqo_control.restype = ct
         L.pdqo_solve_save_at_batch.restype = ctypes.c_int
         _libs[_variant] = L
     return L
 
 
 def _p(a):
-    return a.ctypes.data_as(_P) if a is not None else None
+    if a is None:
+        return None
+    assert a.dtype == _dtype(), (a.dtype, _dtype())
+    return a.ctypes.data_as(ctypes.POINTER(_ct()))
 
 
 def _f64(a):
-    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    """Input array in the active build's scalar type (float64 or long double)."""
+    return np.ascontiguousarray(np.asarray(a, dtype=_dtype()))
 
 
 def qr_r(M):
     M = _f64(M)
     m, n = M.shape
-    R = np.zeros((min(m, n), n))
+    R = _zeros((min(m, n), n))
     lib().pdqo_qr_r(m, n, _p(M), _p(R))
     return R
 
@@ -226,14 +269,14 @@ def qr_r(M):
 def revert_conditional(R_X_F, R_X, R_YX, lstsq=False):
     R_X_F, R_X, R_YX = _f64(R_X_F), _f64(R_X), _f64(R_YX)
     n, k = R_X_F.shape
-    R_Y, R_XY, G = np.zeros((k, k)), np.zeros((n, n)), np.zeros((n, k))
+    R_Y, R_XY, G = _zeros((k, k)), _zeros((n, n)), _zeros((n, k))
     lib().pdqo_revert_conditional(n, k, _p(R_X_F), _p(R_X), _p(R_YX), int(lstsq), _p(R_Y), _p(R_XY), _p(G))
     return R_Y, (R_XY, G)
 
 
 def system_matrices(nu):
-    A = np.zeros((nu + 1, nu + 1))
-    Q = np.zeros((nu + 1, nu + 1))
+    A = _zeros((nu + 1, nu + 1))
+    Q = _zeros((nu + 1, nu + 1))
     lib().pdqo_system_matrices(nu, _p(A), _p(Q))
     return A, Q
 
@@ -241,9 +284,9 @@ def system_matrices(nu):
 def transition(cfg: Config, dt, os=None):
     n = cfg.n
     os = _f64(np.ones(cfg.nblk) if os is None else np.atleast_1d(os))
-    A, L, tl, to = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros(n)
+    A, L, tl, to = _zeros((n, n)), _zeros((n, n)), _zeros(n), _zeros(n)
     c = cfg.c()
-    lib().pdqo_transition(ctypes.byref(c), ctypes.c_double(dt), _p(os), _p(A), _p(L), _p(tl), _p(to))
+    lib().pdqo_transition(ctypes.byref(c), _s(dt), _p(os), _p(A), _p(L), _p(tl), _p(to))
     return A, L, tl, to
 
 
@@ -255,25 +298,25 @@ def logpdf(mean, chol, u, isotropic=False):
 
 
 def control(cfg: Config, dt, state, error_power):
-    s = ctypes.c_double(state)
+    s = _s(state)
     c = cfg.c()
-    dt_new = lib().pdqo_control(ctypes.byref(c), ctypes.c_double(dt), ctypes.byref(s), ctypes.c_double(error_power))
+    dt_new = lib().pdqo_control(ctypes.byref(c), _s(dt), ctypes.byref(s), _s(error_power))
     return dt_new, s.value
 
 
 def taylor_coefficients(vf, params, y0, t0, num):
     params, y0 = _f64(params), _f64(y0)
     d = y0.size
-    out = np.zeros((num + 1, d))
-    lib().pdqo_taylor_coefficients(VF[vf], d, _p(params), _p(y0), ctypes.c_double(t0), num, _p(out))
+    out = _zeros((num + 1, d))
+    lib().pdqo_taylor_coefficients(VF[vf], d, _p(params), _p(y0), _s(t0), num, _p(out))
     return out
 
 
 def vector_field(vf, params, y, t=0.0):
     params, y = _f64(params), _f64(y)
     d = y.size
-    f, J = np.zeros(d), np.zeros((d, d))
-    lib().pdqo_vf(VF[vf], d, _p(params), _p(y), ctypes.c_double(t), _p(f), _p(J))
+    f, J = _zeros(d), _zeros((d, d))
+    lib().pdqo_vf(VF[vf], d, _p(params), _p(y), _s(t), _p(f), _p(J))
     return f, J
 
 
@@ -281,25 +324,25 @@ def _alloc(cfg: Config, K: int):
     ms, cs, nn, nv = cfg.mean_size, cfg.chol_size, cfg.nblk * cfg.n * cfg.n, cfg.nblk * cfg.n
     smoother = cfg.strategy != "filter"
     arrs = {
-        "t": np.zeros(K),
-        "mean": np.zeros((K, ms)),
-        "chol": np.zeros((K, cs)),
-        "output_scale": np.zeros((K, cfg.nblk)),
+        "t": _zeros(K),
+        "mean": _zeros((K, ms)),
+        "chol": _zeros((K, cs)),
+        "output_scale": _zeros((K, cfg.nblk)),
         "num_steps": np.zeros(K - 1, dtype=np.int64),
     }
     if smoother:
         arrs.update(
-            filt_mean=np.zeros((K, ms)),
-            filt_chol=np.zeros((K, cs)),
-            cond_A=np.zeros((K - 1, nn)),
-            cond_mean=np.zeros((K - 1, ms)),
-            cond_chol=np.zeros((K - 1, cs)),
-            cond_tl=np.zeros((K - 1, nv)),
-            cond_to=np.zeros((K - 1, nv)),
-            term_mean=np.zeros(ms),
-            term_chol=np.zeros(cs),
+            filt_mean=_zeros((K, ms)),
+            filt_chol=_zeros((K, cs)),
+            cond_A=_zeros((K - 1, nn)),
+            cond_mean=_zeros((K - 1, ms)),
+            cond_chol=_zeros((K - 1, cs)),
+            cond_tl=_zeros((K - 1, nv)),
+            cond_to=_zeros((K - 1, nv)),
+            term_mean=_zeros(ms),
+            term_chol=_zeros(cs),
         )
-    out = _Out()
+    out = _OUT[_ct()]()
     out.t, out.mean, out.chol, out.os = _p(arrs["t"]), _p(arrs["mean"]), _p(arrs["chol"]), _p(arrs["output_scale"])
     out.num_steps = arrs["num_steps"].ctypes.data_as(_PL)
     if smoother:
@@ -322,7 +365,7 @@ def solve_save_at(cfg: Config, params, tcoeffs, save_at, init_std=0.0):
     K = save_at.size
     arrs, out = _alloc(cfg, K)
     c = cfg.c()
-    st = lib().pdqo_solve_save_at(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), _p(save_at), K, ctypes.byref(out))
+    st = lib().pdqo_solve_save_at(ctypes.byref(c), _p(params), _p(tcoeffs), _s(init_std), _p(save_at), K, ctypes.byref(out))
     arrs = _finish(cfg, arrs, out)
     arrs["status"] = st
     return arrs
@@ -333,7 +376,7 @@ def solve_fixed_grid(cfg: Config, params, tcoeffs, grid, init_std=0.0):
     K = grid.size
     arrs, out = _alloc(cfg, K)
     c = cfg.c()
-    lib().pdqo_solve_fixed_grid(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), _p(grid), K, ctypes.byref(out))
+    lib().pdqo_solve_fixed_grid(ctypes.byref(c), _p(params), _p(tcoeffs), _s(init_std), _p(grid), K, ctypes.byref(out))
     return _finish(cfg, arrs, out)
 
 
@@ -341,7 +384,7 @@ def lml(cfg: Config, params, tcoeffs, grid, every, data, stds, average=True, ini
     params, tcoeffs, grid, data, stds = map(_f64, (params, tcoeffs, grid, data, stds))
     stds2 = stds.reshape(data.shape[0], -1)
     c = cfg.c()
-    return lib().pdqo_lml(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average))
+    return lib().pdqo_lml(ctypes.byref(c), _p(params), _p(tcoeffs), _s(init_std), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average))
 
 
 def lml_from_y0(cfg: Config, params, y0, grid, every, data, stds, average=True):
@@ -354,9 +397,9 @@ def lml_from_y0(cfg: Config, params, y0, grid, every, data, stds, average=True):
 def lml_grad_fd(cfg: Config, params, y0, grid, every, data, stds, h=1e-6, average=True):
     params, y0, grid, data, stds = map(_f64, (params, y0, grid, data, stds))
     stds2 = stds.reshape(data.shape[0], -1)
-    grad = np.zeros(params.size + y0.size)
+    grad = _zeros(params.size + y0.size)
     c = cfg.c()
-    val = lib().pdqo_lml_grad_fd(ctypes.byref(c), _p(params), params.size, _p(y0), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average), ctypes.c_double(h), _p(grad))
+    val = lib().pdqo_lml_grad_fd(ctypes.byref(c), _p(params), params.size, _p(y0), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average), _s(h), _p(grad))
     return val, grad
 
 
@@ -364,7 +407,7 @@ def solve_save_at_batch(cfg: Config, params, tcoeffs, save_at, n_threads=0):
     params, tcoeffs, save_at = _f64(params), _f64(tcoeffs), _f64(save_at)
     n_inst = params.shape[0]
     K = save_at.shape[1]
-    term = np.zeros((n_inst, cfg.mean_size))
+    term = _zeros((n_inst, cfg.mean_size))
     att = np.zeros(n_inst, dtype=np.int64)
     acc = np.zeros(n_inst, dtype=np.int64)
     c = cfg.c()
@@ -376,7 +419,7 @@ def solve_save_every_step(cfg: Config, params, tcoeffs, t0, t1, init_std=0.0, K_
     params, tcoeffs = _f64(params), _f64(tcoeffs)
     arrs, out = _alloc(cfg, K_max)
     c = cfg.c()
-    K = lib().pdqo_solve_save_every_step(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), ctypes.c_double(t0), ctypes.c_double(t1), K_max, ctypes.byref(out))
+    K = lib().pdqo_solve_save_every_step(ctypes.byref(c), _p(params), _p(tcoeffs), _s(init_std), _s(t0), _s(t1), K_max, ctypes.byref(out))
     if K < 0:
         raise RuntimeError("K_max too small")
     arrs = _finish(cfg, arrs, out)
@@ -392,5 +435,5 @@ def solve_fixed_grid_checkpoints(cfg: Config, params, tcoeffs, grid, every, init
     K = (grid.size - 1) // every + 1
     arrs, out = _alloc(cfg, K)
     c = cfg.c()
-    lib().pdqo_solve_fixed_grid_checkpoints(ctypes.byref(c), _p(params), _p(tcoeffs), ctypes.c_double(init_std), _p(grid), grid.size, every, ctypes.byref(out))
+    lib().pdqo_solve_fixed_grid_checkpoints(ctypes.byref(c), _p(params), _p(tcoeffs), _s(init_std), _p(grid), grid.size, every, ctypes.byref(out))
     return _finish(cfg, arrs, out)

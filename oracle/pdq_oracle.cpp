@@ -32,19 +32,28 @@
 #include <omp.h>
 #endif
 
+// The scalar type of the whole restatement, including the C API.  The
+// oracle proper is built with double (strict and FMA-contracted); the
+// "extended" build uses long double (64-bit mantissa) as a higher-precision
+// reference against which the FP64 results' own rounding errors are measured.
+#ifndef PDQO_REAL
+#define PDQO_REAL double
+#endif
+using Real = PDQO_REAL;
+
 namespace {
 
 // ---------------------------------------------------------------- matrices
 struct Mat {
   int r = 0, c = 0;
-  std::vector<double> a;
+  std::vector<Real> a;
   Mat() = default;
   Mat(int r_, int c_) : r(r_), c(c_), a((size_t)r_ * c_, 0.0) {}
-  double& operator()(int i, int j) { return a[(size_t)i * c + j]; }
-  double operator()(int i, int j) const { return a[(size_t)i * c + j]; }
+  Real& operator()(int i, int j) { return a[(size_t)i * c + j]; }
+  Real operator()(int i, int j) const { return a[(size_t)i * c + j]; }
   size_t size() const { return a.size(); }
 };
-using Vec = std::vector<double>;
+using Vec = std::vector<Real>;
 
 Mat eye(int n) {
   Mat m(n, n);
@@ -64,7 +73,7 @@ Mat matmul(const Mat& A, const Mat& B) {
   Mat C(A.r, B.c);
   for (int i = 0; i < A.r; ++i)
     for (int j = 0; j < B.c; ++j) {
-      double acc = 0.0;
+      Real acc = 0.0;
       for (int k = 0; k < A.c; ++k) acc += A(i, k) * B(k, j);
       C(i, j) = acc;
     }
@@ -96,21 +105,21 @@ Mat vstack(const Mat& A, const Mat& B) {
 Mat rowscale(const Vec& s, const Mat& A, bool absval = false) {
   Mat C = A;
   for (int i = 0; i < A.r; ++i) {
-    double f = absval ? std::fabs(s[i]) : s[i];
+    Real f = absval ? std::fabs(s[i]) : s[i];
     for (int j = 0; j < A.c; ++j) C(i, j) = f * A(i, j);
   }
   return C;
 }
 
-Mat scale(double f, const Mat& A) {
+Mat scale(Real f, const Mat& A) {
   Mat C = A;
   for (auto& x : C.a) x = f * x;
   return C;
 }
 
-double fro(const Mat& A) {
-  double s = 0.0;
-  for (double x : A.a) s += x * x;
+Real fro(const Mat& A) {
+  Real s = 0.0;
+  for (Real x : A.a) s += x * x;
   return std::sqrt(s);
 }
 
@@ -122,18 +131,18 @@ Mat qr_r(const Mat& M) {
   const int m = A.r, n = A.c, k = std::min(m, n);
   for (int i = 0; i < k; ++i) {
     if (m - i <= 1) continue;
-    double ss = 0.0;
+    Real ss = 0.0;
     for (int r = i + 1; r < m; ++r) ss += A(r, i) * A(r, i);
-    const double xnorm = std::sqrt(ss);
+    const Real xnorm = std::sqrt(ss);
     if (xnorm == 0.0) continue;
-    const double alpha = A(i, i);
-    const double beta = -std::copysign(std::hypot(alpha, xnorm), alpha);
-    const double tau = (beta - alpha) / beta;
-    const double scal = 1.0 / (alpha - beta);
+    const Real alpha = A(i, i);
+    const Real beta = -std::copysign(std::hypot(alpha, xnorm), alpha);
+    const Real tau = (beta - alpha) / beta;
+    const Real scal = 1.0 / (alpha - beta);
     for (int r = i + 1; r < m; ++r) A(r, i) *= scal;
     A(i, i) = beta;
     for (int j = i + 1; j < n; ++j) {
-      double w = A(i, j);
+      Real w = A(i, j);
       for (int r = i + 1; r < m; ++r) w += A(r, i) * A(r, j);
       w *= tau;
       A(i, j) -= w;
@@ -155,7 +164,7 @@ Mat solve_triu(const Mat& R, const Mat& B) {
   const int n = R.r;
   for (int col = 0; col < B.c; ++col)
     for (int i = n - 1; i >= 0; --i) {
-      double s = B(i, col);
+      Real s = B(i, col);
       for (int j = i + 1; j < n; ++j) s -= R(i, j) * X(j, col);
       X(i, col) = s / R(i, i);
     }
@@ -167,7 +176,7 @@ Mat solve_tril(const Mat& L, const Mat& B) {
   const int n = L.r;
   for (int col = 0; col < B.c; ++col)
     for (int i = 0; i < n; ++i) {
-      double s = B(i, col);
+      Real s = B(i, col);
       for (int j = 0; j < i; ++j) s -= L(i, j) * X(j, col);
       X(i, col) = s / L(i, i);
     }
@@ -182,10 +191,10 @@ Mat lstsq_svd(const Mat& A, const Mat& B) {
   Mat U = A;  // columns get orthogonalised
   Mat V = eye(n);
   for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0;
+    Real off = 0.0;
     for (int p = 0; p < n; ++p)
       for (int q = p + 1; q < n; ++q) {
-        double alpha = 0, beta = 0, gamma = 0;
+        Real alpha = 0, beta = 0, gamma = 0;
         for (int i = 0; i < m; ++i) {
           alpha += U(i, p) * U(i, p);
           beta += U(i, q) * U(i, q);
@@ -193,36 +202,36 @@ Mat lstsq_svd(const Mat& A, const Mat& B) {
         }
         if (gamma == 0.0) continue;
         off = std::max(off, std::fabs(gamma) / std::sqrt(alpha * beta));
-        const double zeta = (beta - alpha) / (2.0 * gamma);
-        const double t = std::copysign(1.0, zeta) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
-        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+        const Real zeta = (beta - alpha) / (2.0 * gamma);
+        const Real t = std::copysign(1.0, zeta) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const Real cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
         for (int i = 0; i < m; ++i) {
-          const double up = U(i, p), uq = U(i, q);
+          const Real up = U(i, p), uq = U(i, q);
           U(i, p) = cs * up - sn * uq;
           U(i, q) = sn * up + cs * uq;
         }
         for (int i = 0; i < n; ++i) {
-          const double vp = V(i, p), vq = V(i, q);
+          const Real vp = V(i, p), vq = V(i, q);
           V(i, p) = cs * vp - sn * vq;
           V(i, q) = sn * vp + cs * vq;
         }
       }
-    if (off < 1e-16) break;
+    if (off < (sizeof(Real) > sizeof(double) ? Real(1e-19) : Real(1e-16))) break;
   }
   Vec s(n);
-  double smax = 0.0;
+  Real smax = 0.0;
   for (int j = 0; j < n; ++j) {
-    double ss = 0;
+    Real ss = 0;
     for (int i = 0; i < m; ++i) ss += U(i, j) * U(i, j);
     s[j] = std::sqrt(ss);
     smax = std::max(smax, s[j]);
   }
-  const double cutoff = 2.220446049250313e-16 * std::max(m, n) * smax;
+  const Real cutoff = 2.220446049250313e-16 * std::max(m, n) * smax;
   Mat X(n, B.c);
   for (int j = 0; j < n; ++j) {
     if (!(s[j] > cutoff)) continue;
     for (int col = 0; col < B.c; ++col) {
-      double ub = 0;
+      Real ub = 0;
       for (int i = 0; i < m; ++i) ub += (U(i, j) / s[j]) * B(i, col);
       ub /= s[j];
       for (int i = 0; i < n; ++i) X(i, col) += V(i, j) * ub;
@@ -267,11 +276,11 @@ Reverted revert_conditional(const Mat& R_X_F, const Mat& R_X, const Mat& R_YX, i
 
 // util/cholesky_util.py:106-176 cholesky_hilbert (Kahan's recurrence).
 Mat cholesky_hilbert(int n) {
-  const double Kf = 0.0;
+  const Real Kf = 0.0;
   Vec dr(n), f(n, 1.0 + Kf);
-  for (int i = 0; i < n; ++i) dr[i] = std::sqrt((double)(2 * i + 1));
+  for (int i = 0; i < n; ++i) dr[i] = std::sqrt((Real)(2 * i + 1));
   for (int idx = 1; idx < n; ++idx) {
-    const double prev = f[idx - 1], idxf = idx;
+    const Real prev = f[idx - 1], idxf = idx;
     f[idx] = (((prev / idxf) * (Kf + 2.0 * idxf)) / (Kf + idxf)) * (Kf + 2.0 * idxf + 1.0);
   }
   for (auto& x : f) x = 1.0 / x;
@@ -279,7 +288,7 @@ Mat cholesky_hilbert(int n) {
   for (int j = 1; j < n; ++j)
     for (int kk = 0; kk < j; ++kk) {
       const int i = j - 1 - kk;
-      const double denom = (double)(j - i), factor = Kf + (double)(i + 1) + (double)(j + 1);
+      const Real denom = (Real)(j - i), factor = Kf + (Real)(i + 1) + (Real)(j + 1);
       U(i, j) = (U(i + 1, j) / denom) * factor;
     }
   for (int i = 0; i < n; ++i)
@@ -290,13 +299,13 @@ Mat cholesky_hilbert(int n) {
   return L;
 }
 
-double factorial_int(int k) {
-  double f = 1.0;
+Real factorial_int(int k) {
+  Real f = 1.0;
   for (int i = 2; i <= k; ++i) f *= i;
   return f;
 }
 
-double binom_int(int n, int k) {
+Real binom_int(int n, int k) {
   if (k < 0 || k > n) return 0.0;
   return factorial_int(n) / (factorial_int(n - k) * factorial_int(k));
 }
@@ -313,7 +322,7 @@ void system_matrices_1d_iwp(int nu, Mat& A1, Mat& Q1) {
     for (int j = 0; j < N; ++j) Lf(i, j) = L(N - 1 - i, j);
   Mat Q = transpose(qr_r(transpose(Lf)));
   for (int j = 0; j < N; ++j) {
-    double s = (Q(j, j) > 0) ? 1.0 : (Q(j, j) < 0 ? -1.0 : 1.0);
+    Real s = (Q(j, j) > 0) ? 1.0 : (Q(j, j) < 0 ? -1.0 : 1.0);
     for (int i = 0; i < N; ++i) Q(i, j) *= s;
   }
   Q1 = Q;
@@ -321,15 +330,15 @@ void system_matrices_1d_iwp(int nu, Mat& A1, Mat& Q1) {
 
 // _probdiffeq/utilities.py:74-84 preconditioner_taylor.  Factorials are
 // exact integers here (the reference forms exp(lgamma(k+1))).
-void preconditioner(int nu, double dt, Vec& p, Vec& pinv) {
+void preconditioner(int nu, Real dt, Vec& p, Vec& pinv) {
   const int N = nu + 1;
   p.assign(N, 0.0);
   pinv.assign(N, 0.0);
   for (int i = 0; i < N; ++i) {
     const int pw = nu - i;
-    const double s = factorial_int(pw);
-    p[i] = std::pow(dt, (double)pw) / s;
-    pinv[i] = std::pow(dt, -(double)pw) * s;
+    const Real s = factorial_int(pw);
+    p[i] = std::pow(dt, (Real)pw) / s;
+    pinv[i] = std::pow(dt, -(Real)pw) * s;
   }
 }
 
@@ -338,26 +347,26 @@ enum VfId { VF_LOTKA_VOLTERRA = 0, VF_VAN_DER_POL = 1, VF_BURGERS = 2, VF_LINEAR
 
 struct Vf {
   int id = 0, d = 0;
-  const double* prm = nullptr;
+  const Real* prm = nullptr;
   // f(y, t) -> out (d)
-  void eval(const double* y, double t, double* out) const {
+  void eval(const Real* y, Real t, Real* out) const {
     switch (id) {
       case VF_LOTKA_VOLTERRA: {
-        const double a = prm[0], b = prm[1], c = prm[2], dd = prm[3];
+        const Real a = prm[0], b = prm[1], c = prm[2], dd = prm[3];
         out[0] = a * y[0] - b * y[0] * y[1];
         out[1] = -c * y[1] + dd * y[0] * y[1];
         break;
       }
       case VF_VAN_DER_POL: {
-        const double mu = prm[0];
+        const Real mu = prm[0];
         out[0] = y[1];
         out[1] = mu * ((1.0 - y[0] * y[0]) * y[1]) - y[0];
         break;
       }
       case VF_BURGERS: {
-        const double visc = prm[0], dx = prm[1];
+        const Real visc = prm[0], dx = prm[1];
         for (int j = 0; j < d; ++j) {
-          const double um = y[(j + d - 1) % d], u0 = y[j], up = y[(j + 1) % d];
+          const Real um = y[(j + d - 1) % d], u0 = y[j], up = y[(j + 1) % d];
           out[j] = -u0 * ((up - um) / (2.0 * dx)) + visc * ((up - 2.0 * u0 + um) / (dx * dx));
         }
         break;
@@ -367,19 +376,19 @@ struct Vf {
         break;
       }
       case VF_LOGISTIC: {
-        const double r = prm[0];
+        const Real r = prm[0];
         for (int j = 0; j < d; ++j) out[j] = r * y[j] * (1.0 - y[j]);
         break;
       }
     }
   }
   // Jacobian df/dy (d x d, row-major).
-  void jac(const double* y, double t, double* J) const {
+  void jac(const Real* y, Real t, Real* J) const {
     (void)t;
     std::fill(J, J + d * d, 0.0);
     switch (id) {
       case VF_LOTKA_VOLTERRA: {
-        const double a = prm[0], b = prm[1], c = prm[2], dd = prm[3];
+        const Real a = prm[0], b = prm[1], c = prm[2], dd = prm[3];
         J[0] = a - b * y[1];
         J[1] = -b * y[0];
         J[2] = dd * y[1];
@@ -387,7 +396,7 @@ struct Vf {
         break;
       }
       case VF_VAN_DER_POL: {
-        const double mu = prm[0];
+        const Real mu = prm[0];
         J[0] = 0.0;
         J[1] = 1.0;
         J[2] = mu * (-2.0 * y[0] * y[1]) - 1.0;
@@ -395,9 +404,9 @@ struct Vf {
         break;
       }
       case VF_BURGERS: {
-        const double visc = prm[0], dx = prm[1];
+        const Real visc = prm[0], dx = prm[1];
         for (int j = 0; j < d; ++j) {
-          const double um = y[(j + d - 1) % d], u0 = y[j], up = y[(j + 1) % d];
+          const Real um = y[(j + d - 1) % d], u0 = y[j], up = y[(j + 1) % d];
           const int jm = (j + d - 1) % d, jp = (j + 1) % d;
           J[j * d + j] += -((up - um) / (2.0 * dx)) - 2.0 * visc / (dx * dx);
           J[j * d + jp] += -u0 / (2.0 * dx) + visc / (dx * dx);
@@ -410,7 +419,7 @@ struct Vf {
         break;
       }
       case VF_LOGISTIC: {
-        const double r = prm[0];
+        const Real r = prm[0];
         for (int j = 0; j < d; ++j) J[j * d + j] = r * (1.0 - 2.0 * y[j]);
         break;
       }
@@ -422,12 +431,12 @@ struct Vf {
 // jet_expansion_algorithms.py:49-103 / :155-177 for the registered vector
 // fields).  c[k][j] are normalised Taylor coefficients; the returned
 // derivatives are k! c[k].
-void taylor_coefficients(const Vf& vf, const double* y0, double t0, int num, double* out) {
+void taylor_coefficients(const Vf& vf, const Real* y0, Real t0, int num, Real* out) {
   const int d = vf.d, K = num + 1;
   std::vector<Vec> c(K, Vec(d, 0.0));
   for (int j = 0; j < d; ++j) c[0][j] = y0[j];
   auto cauchy = [&](const Vec* X, const Vec* Y, int k, int j, int jx, int jy) {
-    double s = 0.0;
+    Real s = 0.0;
     for (int i = 0; i <= k; ++i) s += X[i][jx] * Y[k - i][jy];
     (void)j;
     return s;
@@ -437,32 +446,32 @@ void taylor_coefficients(const Vf& vf, const double* y0, double t0, int num, dou
     Vec fk(d, 0.0);
     switch (vf.id) {
       case VF_LOTKA_VOLTERRA: {
-        const double a = vf.prm[0], b = vf.prm[1], cc = vf.prm[2], dd = vf.prm[3];
-        const double y0y1 = cauchy(c.data(), c.data(), k, 0, 0, 1);
+        const Real a = vf.prm[0], b = vf.prm[1], cc = vf.prm[2], dd = vf.prm[3];
+        const Real y0y1 = cauchy(c.data(), c.data(), k, 0, 0, 1);
         fk[0] = a * c[k][0] - b * y0y1;
         fk[1] = -cc * c[k][1] + dd * y0y1;
         break;
       }
       case VF_VAN_DER_POL: {
-        const double mu = vf.prm[0];
+        const Real mu = vf.prm[0];
         // (1 - y0^2) y1 coefficient k
         std::vector<Vec> sq(k + 1, Vec(1, 0.0));
         for (int m = 0; m <= k; ++m) {
-          double s = 0.0;
+          Real s = 0.0;
           for (int i = 0; i <= m; ++i) s += c[i][0] * c[m - i][0];
           sq[m][0] = (m == 0 ? 1.0 : 0.0) - s;
         }
-        double s = 0.0;
+        Real s = 0.0;
         for (int i = 0; i <= k; ++i) s += sq[i][0] * c[k - i][1];
         fk[0] = c[k][1];
         fk[1] = mu * s - c[k][0];
         break;
       }
       case VF_BURGERS: {
-        const double visc = vf.prm[0], dx = vf.prm[1];
+        const Real visc = vf.prm[0], dx = vf.prm[1];
         for (int j = 0; j < d; ++j) {
           const int jm = (j + d - 1) % d, jp = (j + 1) % d;
-          double s = 0.0;
+          Real s = 0.0;
           for (int i = 0; i <= k; ++i) s += c[i][j] * ((c[k - i][jp] - c[k - i][jm]) / (2.0 * dx));
           fk[j] = -s + visc * ((c[k][jp] - 2.0 * c[k][j] + c[k][jm]) / (dx * dx));
         }
@@ -473,16 +482,16 @@ void taylor_coefficients(const Vf& vf, const double* y0, double t0, int num, dou
         break;
       }
       case VF_LOGISTIC: {
-        const double r = vf.prm[0];
+        const Real r = vf.prm[0];
         for (int j = 0; j < d; ++j) {
-          double s = 0.0;
+          Real s = 0.0;
           for (int i = 0; i <= k; ++i) s += c[i][j] * c[k - i][j];
           fk[j] = r * (c[k][j] - s);
         }
         break;
       }
     }
-    for (int j = 0; j < d; ++j) c[k + 1][j] = fk[j] / (double)(k + 1);
+    for (int j = 0; j < d; ++j) c[k + 1][j] = fk[j] / (Real)(k + 1);
   }
   for (int k = 0; k < K; ++k)
     for (int j = 0; j < d; ++j) out[k * d + j] = factorial_int(k) * c[k][j];
@@ -501,9 +510,9 @@ enum { CONTROL_INTEGRAL = 0, CONTROL_PI = 1 };
 extern "C" {
 struct pdqo_cfg {
   int fact, nu, d, constraint, strategy, solver, error_norm, control, vf_id, clip_dt;
-  double rtol, atol, dt0, eps, damp;
-  double safety, factor_min, factor_max, exponent_integral, exponent_proportional;
-  double base_scale;  // prior output scale (scalar, broadcast)
+  Real rtol, atol, dt0, eps, damp;
+  Real safety, factor_min, factor_max, exponent_integral, exponent_proportional;
+  Real base_scale;  // prior output scale (scalar, broadcast)
   int correct_asymptotic_underconfidence, max_attempts;
 };
 }
@@ -596,12 +605,12 @@ struct SSMBase {
   }
 
   // ssm_impl_dense.py:156-160 / ssm_impl_isotropic.py:203-207.
-  double residual_whitened_rms_zero(const Normal& rv) const {
+  Real residual_whitened_rms_zero(const Normal& rv) const {
     Mat dx = rv.m;
     if (!iso())
       for (auto& x : dx.a) x = 0.0 - x;
     Mat w = solve_tril(rv.L, dx);
-    return fro(w) / std::sqrt((double)rv.m.size());
+    return fro(w) / std::sqrt((Real)rv.m.size());
   }
 
   // ssm_impl_dense.py:141-150 (row norms via per-row qr_r) /
@@ -611,7 +620,7 @@ struct SSMBase {
     Vec s(n);
     for (int i = 0; i < n; ++i) {
       if (iso()) {
-        double ss = 0;
+        Real ss = 0;
         for (int j = 0; j < rv.L.c; ++j) ss += rv.L(i, j) * rv.L(i, j);
         s[i] = std::sqrt(ss);
       } else {
@@ -624,27 +633,27 @@ struct SSMBase {
   }
 
   // ssm_impl_dense.py:170-180 / ssm_impl_isotropic.py:217-236 logpdf.
-  double logpdf(const Normal& rv, const Mat& u) const {
+  Real logpdf(const Normal& rv, const Mat& u) const {
     Mat chol = transpose(qr_r(transpose(rv.L)));
-    double slogdet = 0.0;
+    Real slogdet = 0.0;
     for (int i = 0; i < chol.r; ++i) slogdet += std::log(std::fabs(chol(i, i)));
-    const double c2pi = std::log(M_PI * 2.0);
+    const Real c2pi = std::log(Real(2) * Real(3.14159265358979323846264338327950288L));
     if (!iso()) {
       Mat dx = sub(u, rv.m);
       Mat w = solve_tril(chol, dx);
-      double sq = 0;
-      for (double x : w.a) sq += x * x;
-      return -1.0 / 2.0 * sq - (double)u.size() / 2.0 * c2pi - slogdet;
+      Real sq = 0;
+      for (Real x : w.a) sq += x * x;
+      return -1.0 / 2.0 * sq - (Real)u.size() / 2.0 * c2pi - slogdet;
     }
-    double total = 0.0;
+    Real total = 0.0;
     for (int col = 0; col < u.c; ++col) {
       Mat dx(u.r, 1);
       for (int i = 0; i < u.r; ++i) dx(i, 0) = u(i, col) - rv.m(i, col);
       Mat w = solve_tril(chol, dx);
-      double maha = 0;
-      for (double x : w.a) maha += x * x;
-      const double logdet_term = 2.0 * slogdet;
-      total += -0.5 * (logdet_term + maha + (double)u.r * c2pi);
+      Real maha = 0;
+      for (Real x : w.a) maha += x * x;
+      const Real logdet_term = 2.0 * slogdet;
+      total += -0.5 * (logdet_term + maha + (Real)u.r * c2pi);
     }
     return total;
   }
@@ -672,7 +681,7 @@ struct Problem {
   Vf vf;
   SSMBase ops;
 
-  void setup(const pdqo_cfg& c, const double* params) {
+  void setup(const pdqo_cfg& c, const Real* params) {
     cfg = c;
     N = c.nu + 1;
     d = c.d;
@@ -712,7 +721,7 @@ struct BCond {
 
 // *WienerIntegrated.transition (ssm_impl_dense.py:347-363,
 // ssm_impl_isotropic.py:367-378, ssm_impl_blockdiag.py:406-432).
-BCond transition(const Problem& P, double dt, const Vec& os) {
+BCond transition(const Problem& P, Real dt, const Vec& os) {
   Vec p1, pinv1;
   preconditioner(P.cfg.nu, dt, p1, pinv1);
   BCond out;
@@ -735,14 +744,14 @@ BCond transition(const Problem& P, double dt, const Vec& os) {
       C.tl = pinv1;
       C.A = P.A;
       C.noise.m = Mat(P.N, P.d);
-      const double s = std::sqrt(std::fabs(dt)) * P.cfg.base_scale * os[0];
+      const Real s = std::sqrt(std::fabs(dt)) * P.cfg.base_scale * os[0];
       C.noise.L = scale(s, P.Q);
     } else {
       C.to = p1;
       C.tl = pinv1;
       C.A = P.A;
       C.noise.m = Mat(P.N, 1);
-      const double s = std::sqrt(std::fabs(dt)) * (P.cfg.base_scale * os[blk]);
+      const Real s = std::sqrt(std::fabs(dt)) * (P.cfg.base_scale * os[blk]);
       C.noise.L = scale(s, P.Q);
     }
   }
@@ -819,14 +828,14 @@ Vec coeff_of(const Problem& P, const BNormal& rv, int i) {
 // ssm_impl_isotropic.py:298-317, ssm_impl_blockdiag.py:123-144).
 // ts1: DenseResidual.linearize (ssm_impl_dense.py:262-334) with
 // residual_from_ode (problems.py:518-532) and the prior-mean Taylor point.
-BCond linearize(const Problem& P, const BNormal& rv, double t) {
+BCond linearize(const Problem& P, const BNormal& rv, Real t) {
   const int d = P.d, N = P.N;
   Vec y0 = u0_of(P, rv);
   Vec f(d);
   P.vf.eval(y0.data(), t, f.data());
   BCond out;
   out.b.resize(P.nblk);
-  const double damp = P.cfg.damp;
+  const Real damp = P.cfg.damp;
   if (P.cfg.fact == FACT_DENSE) {
     Cond& C = out.b[0];
     const int n = P.n;
@@ -887,7 +896,7 @@ Vec observed_std(const Problem& P, const BNormal& obs) {
     Vec s = P.ops.std_rows(obs.b[i]);
     if (P.bd()) {
       // blockdiag std = vector_norm per row (ssm_impl_blockdiag.py:269-274).
-      double ss = 0;
+      Real ss = 0;
       for (int j = 0; j < obs.b[i].L.c; ++j) ss += obs.b[i].L(0, j) * obs.b[i].L(0, j);
       s[0] = std::sqrt(ss);
     }
@@ -902,14 +911,14 @@ struct Post {
   BCond cond;  // smoothers only
 };
 struct Sol {
-  double t = 0;
+  Real t = 0;
   BNormal u;
   Post post;
   Vec os;
   long num_steps = 0;
   BCond fx;             // fun_evals (cached linearisation)
   Vec mle_running;      // solver_mle auxiliary
-  double mle_num_data = 0;
+  Real mle_num_data = 0;
 };
 
 bool smoother(const Problem& P) { return P.cfg.strategy != STRAT_FILTER; }
@@ -948,7 +957,7 @@ BNormal bayes_rule_zero(const Problem& P, const BCond& fx, const BNormal& rv, BN
 Vec prototype_os(const Problem& P) { return Vec(P.nblk, 1.0); }
 
 // solver*.init (solvers.py:515-547, 664-700, 348-393), constraint_init=None.
-Sol solver_init(const Problem& P, double t, const BNormal& init) {
+Sol solver_init(const Problem& P, Real t, const BNormal& init) {
   Sol s;
   s.t = t;
   s.u = init;
@@ -972,9 +981,9 @@ Sol solver_init(const Problem& P, double t, const BNormal& init) {
 
 // solver_dynamic.step (solvers.py:549-599), solver.step (:702-733),
 // solver_mle.step (:395-437).
-Sol solver_step(const Problem& P, const Sol& st, double dt) {
+Sol solver_step(const Problem& P, const Sol& st, Real dt) {
   Sol out;
-  const double tn = st.t + dt;
+  const Real tn = st.t + dt;
   if (P.cfg.solver == SOLVER_DYNAMIC) {
     BCond tr = transition(P, dt, prototype_os(P));
     BNormal u = apply_flat(P, tr, st.u);
@@ -1008,8 +1017,8 @@ Sol solver_step(const Problem& P, const Sol& st, double dt) {
       Vec new_term = residual_whitened_rms_zero(P, observed);
       out.mle_running.resize(P.nblk);
       for (int i = 0; i < P.nblk; ++i) {
-        const double x1 = std::sqrt(st.mle_num_data / (st.mle_num_data + 1)) * st.mle_running[i];
-        const double x2 = std::sqrt(1 / (st.mle_num_data + 1)) * new_term[i];
+        const Real x1 = std::sqrt(st.mle_num_data / (st.mle_num_data + 1)) * st.mle_running[i];
+        const Real x2 = std::sqrt(1 / (st.mle_num_data + 1)) * new_term[i];
         out.mle_running[i] = std::hypot(x1, x2);
       }
       out.mle_num_data = st.mle_num_data + 1;
@@ -1025,7 +1034,7 @@ Sol solver_step(const Problem& P, const Sol& st, double dt) {
 
 // error_residual_std.estimate_error_norm (solvers.py:925-996) with
 // error_norm_scale_then_rms / _rms_then_scale (:770-811).
-double estimate_error_power(const Problem& P, const Sol& prev, const Sol& prop, double dt) {
+Real estimate_error_power(const Problem& P, const Sol& prev, const Sol& prop, Real dt) {
   BCond tr = transition(P, dt, prototype_os(P));
   BNormal rv = apply_flat(P, tr, prev.u);
   BNormal observed = marginalise(P, prop.fx, rv);
@@ -1038,40 +1047,40 @@ double estimate_error_power(const Problem& P, const Sol& prev, const Sol& prop, 
   for (int k = 0; k < d; ++k) ref[k] = std::max(std::fabs(u0[k]), std::fabs(u1[k]));
   const int nord = 1;  // residual_order - 1 for first-order ODEs
   Vec err_abs(err.size());
-  for (size_t i = 0; i < err.size(); ++i) err_abs[i] = err[i] * std::pow(dt, (double)nord) / factorial_int(nord);
-  double norm;
+  for (size_t i = 0; i < err.size(); ++i) err_abs[i] = err[i] * std::pow(dt, (Real)nord) / factorial_int(nord);
+  Real norm;
   if (P.cfg.error_norm == NORM_SCALE_THEN_RMS) {
-    double ss = 0;
+    Real ss = 0;
     for (int k = 0; k < d; ++k) {
-      const double e = err_abs[err_abs.size() == 1 ? 0 : k];
-      const double r = e / (P.cfg.atol + P.cfg.rtol * std::fabs(ref[k]));
+      const Real e = err_abs[err_abs.size() == 1 ? 0 : k];
+      const Real r = e / (P.cfg.atol + P.cfg.rtol * std::fabs(ref[k]));
       ss += r * r;
     }
-    norm = std::sqrt(ss) / std::sqrt((double)d);
+    norm = std::sqrt(ss) / std::sqrt((Real)d);
   } else {
-    double sa = 0, sr = 0;
-    for (double e : err_abs) sa += e * e;
-    for (double r : ref) sr += r * r;
-    const double na = std::sqrt(sa) / std::sqrt((double)err_abs.size());
-    const double nr = std::sqrt(sr) / std::sqrt((double)ref.size());
+    Real sa = 0, sr = 0;
+    for (Real e : err_abs) sa += e * e;
+    for (Real r : ref) sr += r * r;
+    const Real na = std::sqrt(sa) / std::sqrt((Real)err_abs.size());
+    const Real nr = std::sqrt(sr) / std::sqrt((Real)ref.size());
     norm = na / (P.cfg.atol + P.cfg.rtol * nr);
   }
-  return std::pow(norm, -1.0 / (double)P.N);
+  return std::pow(norm, -1.0 / (Real)P.N);
 }
 
 // controllers.py:66-84 control_integral / :24-63 proportional-integral.
-double control_apply(const Problem& P, double dt, double& cstate, double error_power) {
+Real control_apply(const Problem& P, Real dt, Real& cstate, Real error_power) {
   const auto& c = P.cfg;
   if (c.control == CONTROL_INTEGRAL) {
-    const double ratio = c.safety * error_power;
-    const double clipped = std::min(ratio, c.factor_max);
+    const Real ratio = c.safety * error_power;
+    const Real clipped = std::min(ratio, c.factor_max);
     return std::max(c.factor_min, clipped) * dt;
   }
-  const double gi = std::pow(error_power, c.exponent_integral);
-  const double gp = std::pow(error_power / cstate, c.exponent_proportional);
-  const double ratio = c.safety * gi * gp;
-  const double clipped = std::min(ratio, c.factor_max);
-  const double sf = std::max(c.factor_min, clipped);
+  const Real gi = std::pow(error_power, c.exponent_integral);
+  const Real gp = std::pow(error_power / cstate, c.exponent_proportional);
+  const Real ratio = c.safety * gi * gp;
+  const Real clipped = std::min(ratio, c.factor_max);
+  const Real sf = std::max(c.factor_min, clipped);
   if (error_power >= 1.0) cstate = error_power;
   return sf * dt;
 }
@@ -1079,7 +1088,7 @@ double control_apply(const Problem& P, double dt, double& cstate, double error_p
 // ProbabilisticSolver.interpolate_fwd / _at_t1 (solvers.py:205-315) with
 // the strategies' interpolate_fwd (estimators_and_losses.py:393-421,
 // 534-591, 622-672, 711-717).
-void interpolate_fwd(const Problem& P, double t, const Sol& from, const Sol& to, Sol& interpolated, Sol& step_from,
+void interpolate_fwd(const Problem& P, Real t, const Sol& from, const Sol& to, Sol& interpolated, Sol& step_from,
                      Sol& interp_from) {
   BCond tr0 = transition(P, t - from.t, to.os);
   BCond tr1 = transition(P, to.t - t, to.os);
@@ -1156,27 +1165,30 @@ struct Outputs {
 };
 
 enum { ST_OK = 0, ST_MAX_ATTEMPTS = 1, ST_NONFINITE = 2, ST_DT_UNDERFLOW = 3 };
+// Bound on the attempts of one solve (the kernels use the same bound, so that
+// every persistent wave reaches its exit in bounded time).
+constexpr long kMaxTotalAttempts = 1L << 22;
 
 // solve_adaptive_save_at (solvers_via_adaptive_steps.py:46-148) with the
 // RejectionLoop (:196-375).
-void solve_save_at(const Problem& P, const BNormal& init, const double* save_at, int K, Outputs& out) {
+void solve_save_at(const Problem& P, const BNormal& init, const Real* save_at, int K, Outputs& out) {
   const auto& c = P.cfg;
   Sol sol0 = solver_init(P, save_at[0], init);
   out.sol0 = sol0;
-  double dt = c.dt0, cstate = 1.0;
+  Real dt = c.dt0, cstate = 1.0;
   Sol step_from = sol0, interp_from = sol0;
   for (int ck = 1; ck < K && out.status == ST_OK; ++ck) {
-    const double t1 = save_at[ck];
+    const Real t1 = save_at[ck];
     bool do_continue = true;
     Sol solution;
     while (do_continue) {
       if (step_from.t + c.eps < t1) {
         // RejectionLoop.step: attempt until error_power >= 1.
-        double ep = 0.9;
+        Real ep = 0.9;
         Sol proposed;
         int attempts = 0;
         while (ep < 1.0) {
-          double dtc = dt;
+          Real dtc = dt;
           if (c.clip_dt) dtc = std::min(dtc, t1 - step_from.t);
           proposed = solver_step(P, step_from, dtc);
           ep = estimate_error_power(P, step_from, proposed, dtc);
@@ -1187,7 +1199,19 @@ void solve_save_at(const Problem& P, const BNormal& init, const double* save_at,
             out.status = ST_NONFINITE;
             break;
           }
-          if (c.max_attempts > 0 && attempts >= c.max_attempts && ep < 1.0) {
+          // Failure semantics the reference lacks (it would loop forever):
+          // a rejected attempt whose next step cannot advance t, or an
+          // accepted one that did not, ends the solve with DT_UNDERFLOW;
+          // more than kMaxTotalAttempts attempts end it with MAX_ATTEMPTS.
+          if (ep < 1.0 && (dt == 0.0 || step_from.t + dt == step_from.t)) {
+            out.status = ST_DT_UNDERFLOW;
+            break;
+          }
+          if (ep >= 1.0 && proposed.t == step_from.t) {
+            out.status = ST_DT_UNDERFLOW;
+            break;
+          }
+          if ((c.max_attempts > 0 && attempts >= c.max_attempts && ep < 1.0) || out.attempts >= kMaxTotalAttempts) {
             out.status = ST_MAX_ATTEMPTS;
             break;
           }
@@ -1220,11 +1244,11 @@ void solve_save_at(const Problem& P, const BNormal& init, const double* save_at,
 }
 
 // solve_fixed_grid (solvers_via_fixed_steps.py:11-34).
-void solve_fixed_grid(const Problem& P, const BNormal& init, const double* grid, int K, Outputs& out) {
+void solve_fixed_grid(const Problem& P, const BNormal& init, const Real* grid, int K, Outputs& out) {
   Sol s = solver_init(P, grid[0], init);
   out.sol0 = s;
   for (int k = 1; k < K; ++k) {
-    const double dt = grid[k] - grid[k - 1];
+    const Real dt = grid[k] - grid[k - 1];
     s = solver_step(P, s, dt);
     ++out.attempts;
     ++out.accepted;
@@ -1235,19 +1259,19 @@ void solve_fixed_grid(const Problem& P, const BNormal& init, const double* grid,
 
 // util/test_util.py:10-66 solve_adaptive_save_every_step: call the
 // rejection loop until step_from.t >= t1 and keep every returned solution.
-void solve_save_every_step(const Problem& P, const BNormal& init, double t0, double t1, Outputs& out) {
+void solve_save_every_step(const Problem& P, const BNormal& init, Real t0, Real t1, Outputs& out) {
   const auto& c = P.cfg;
   Sol sol0 = solver_init(P, t0, init);
   out.sol0 = sol0;
-  double dt = c.dt0, cstate = 1.0;
+  Real dt = c.dt0, cstate = 1.0;
   Sol step_from = sol0, interp_from = sol0;
   while (step_from.t < t1 && out.status == ST_OK) {
     Sol solution;
     if (step_from.t + c.eps < t1) {
-      double ep = 0.9;
+      Real ep = 0.9;
       Sol proposed;
       while (ep < 1.0) {
-        double dtc = dt;
+        Real dtc = dt;
         if (c.clip_dt) dtc = std::min(dtc, t1 - step_from.t);
         proposed = solver_step(P, step_from, dtc);
         ep = estimate_error_power(P, step_from, proposed, dtc);
@@ -1255,6 +1279,14 @@ void solve_save_every_step(const Problem& P, const BNormal& init, double t0, dou
         ++out.attempts;
         if (!std::isfinite(ep)) {
           out.status = ST_NONFINITE;
+          break;
+        }
+        if ((ep < 1.0 && (dt == 0.0 || step_from.t + dt == step_from.t)) || (ep >= 1.0 && proposed.t == step_from.t)) {
+          out.status = ST_DT_UNDERFLOW;
+          break;
+        }
+        if (out.attempts >= kMaxTotalAttempts) {
+          out.status = ST_MAX_ATTEMPTS;
           break;
         }
       }
@@ -1286,13 +1318,13 @@ void solve_save_every_step(const Problem& P, const BNormal& init, double t0, dou
 // Forward pass of the LML configuration: the fixed-point smoother on a fixed
 // grid with checkpoints at every `every`-th grid point (interp_fwd_at_t1
 // semantics at each checkpoint).
-void solve_fixed_grid_checkpoints(const Problem& P, const BNormal& init, const double* grid, int K, int every,
+void solve_fixed_grid_checkpoints(const Problem& P, const BNormal& init, const Real* grid, int K, int every,
                                   Outputs& out) {
   Sol s = solver_init(P, grid[0], init);
   out.sol0 = s;
   Sol interp_from = s;
   for (int k = 1; k < K; ++k) {
-    const double dt = grid[k] - grid[k - 1];
+    const Real dt = grid[k] - grid[k - 1];
     Sol prop = solver_step(P, s, dt);
     ++out.attempts;
     ++out.accepted;
@@ -1311,7 +1343,7 @@ void solve_fixed_grid_checkpoints(const Problem& P, const BNormal& init, const d
 // userfriendly_output + strategy.finalize (solvers.py:439-480, 601-633,
 // 735-767; estimators_and_losses.py:380-391, 437-470).
 struct Final {
-  std::vector<double> t;
+  std::vector<Real> t;
   std::vector<BNormal> u;          // K marginals
   std::vector<BNormal> filtering;  // smoothers: K filtering marginals
   std::vector<BCond> conds;        // smoothers: K-1 backward conditionals
@@ -1326,7 +1358,7 @@ void finalize(const Problem& P, const Outputs& o, Final& F) {
   if (P.cfg.solver == SOLVER_MLE) {
     scale = o.sol1.mle_running;
     if (P.cfg.correct_asymptotic_underconfidence)
-      for (auto& x : scale) x = x / std::sqrt((double)o.sols.back().num_steps);
+      for (auto& x : scale) x = x / std::sqrt((Real)o.sols.back().num_steps);
   }
   F.t.push_back(o.sol0.t);
   for (auto& s : o.sols) F.t.push_back(s.t);
@@ -1381,7 +1413,7 @@ void finalize(const Problem& P, const Outputs& o, Final& F) {
 
 // to_derivative(0, std) observation model (ssm_impl_dense.py:206-217,
 // ssm_impl_isotropic.py:271-278, ssm_impl_blockdiag.py:367-377).
-BCond observation_model(const Problem& P, const double* std_j, int nstd) {
+BCond observation_model(const Problem& P, const Real* std_j, int nstd) {
   BCond out;
   out.b.resize(P.nblk);
   const int d = P.d, N = P.N;
@@ -1419,11 +1451,11 @@ BCond observation_model(const Problem& P, const double* std_j, int nstd) {
 }
 
 // bayes_rule_and_logpdf_tree (ssm_impl_api.py:118-124) with lstsq gains.
-double bayes_logpdf(const Problem& P, const BCond& model, const double* y, const BNormal& rv, BNormal& updated) {
+Real bayes_logpdf(const Problem& P, const BCond& model, const Real* y, const BNormal& rv, BNormal& updated) {
   BNormal obs;
   BCond rev;
   revert(P, model, rv, SOLVE_LSTSQ, obs, rev);
-  double lp = 0.0;
+  Real lp = 0.0;
   updated.b.resize(P.nblk);
   for (int i = 0; i < P.nblk; ++i) {
     Mat data(obs.b[i].m.r, obs.b[i].m.c);
@@ -1446,18 +1478,18 @@ double bayes_logpdf(const Problem& P, const BCond& model, const double* y, const
 
 // MarkovSequence.evaluate_lml (estimators_and_losses.py:180-218) via
 // loss_lml_timeseries (:53-105): data y (K x d), std (K x nstd).
-double evaluate_lml(const Problem& P, const Final& F, const double* y, const double* stds, int nstd, int average) {
+Real evaluate_lml(const Problem& P, const Final& F, const Real* y, const Real* stds, int nstd, int average) {
   const int K = (int)F.conds.size() + 1, d = P.d;
   BNormal upd;
   BCond m0 = observation_model(P, stds + (size_t)(K - 1) * nstd, nstd);
-  double logpdf = bayes_logpdf(P, m0, y + (size_t)(K - 1) * d, F.terminal, upd);
-  double num = 1.0;
+  Real logpdf = bayes_logpdf(P, m0, y + (size_t)(K - 1) * d, F.terminal, upd);
+  Real num = 1.0;
   BNormal rv = upd;
   for (int j = K - 2; j >= 0; --j) {
     BNormal pred = marginalise(P, F.conds[j], rv);
     BCond mj = observation_model(P, stds + (size_t)j * nstd, nstd);
     BNormal corrected;
-    double lpn = bayes_logpdf(P, mj, y + (size_t)j * d, pred, corrected);
+    Real lpn = bayes_logpdf(P, mj, y + (size_t)j * d, pred, corrected);
     if (average)
       logpdf = (logpdf * num + lpn) / (num + 1);
     else
@@ -1470,7 +1502,7 @@ double evaluate_lml(const Problem& P, const Final& F, const double* y, const dou
 
 // Prior init normal from Taylor coefficients (tcoeffs K x d, derivatives)
 // and is_exact (std zero) or inexact std.
-BNormal prior_init(const Problem& P, const double* tc, double init_std) {
+BNormal prior_init(const Problem& P, const Real* tc, Real init_std) {
   BNormal out;
   out.b.resize(P.nblk);
   const int N = P.N, d = P.d;
@@ -1505,7 +1537,7 @@ BNormal prior_init(const Problem& P, const double* tc, double init_std) {
 int mean_size(const Problem& P) { return P.cfg.fact == FACT_BLOCKDIAG ? P.d * P.N : P.n * (P.cfg.fact == FACT_ISO ? P.d : 1); }
 int chol_size(const Problem& P) { return P.nblk * P.n * P.n; }
 
-void write_normal(const Problem& P, const BNormal& x, double* mean, double* chol) {
+void write_normal(const Problem& P, const BNormal& x, Real* mean, Real* chol) {
   if (P.cfg.fact == FACT_BLOCKDIAG) {
     for (int k = 0; k < P.d; ++k)
       for (int i = 0; i < P.N; ++i) {
@@ -1515,21 +1547,21 @@ void write_normal(const Problem& P, const BNormal& x, double* mean, double* chol
       }
     return;
   }
-  if (mean) std::memcpy(mean, x.b[0].m.a.data(), sizeof(double) * x.b[0].m.a.size());
-  if (chol) std::memcpy(chol, x.b[0].L.a.data(), sizeof(double) * x.b[0].L.a.size());
+  if (mean) std::memcpy(mean, x.b[0].m.a.data(), sizeof(Real) * x.b[0].m.a.size());
+  if (chol) std::memcpy(chol, x.b[0].L.a.data(), sizeof(Real) * x.b[0].L.a.size());
 }
 
 // A conditional's flat layout: A (nblk*n*n), noise mean (mean_size), noise
 // chol (chol_size), to_latent (nblk*n), to_observed (nblk*n).
-void write_cond(const Problem& P, const BCond& c, double* A, double* nm, double* nc, double* tl, double* to) {
+void write_cond(const Problem& P, const BCond& c, Real* A, Real* nm, Real* nc, Real* tl, Real* to) {
   BNormal noise;
   noise.b.resize(P.nblk);
   for (int i = 0; i < P.nblk; ++i) noise.b[i] = c.b[i].noise;
   write_normal(P, noise, nm, nc);
   for (int i = 0; i < P.nblk; ++i) {
-    std::memcpy(A + (size_t)i * P.n * P.n, c.b[i].A.a.data(), sizeof(double) * P.n * P.n);
-    std::memcpy(tl + (size_t)i * P.n, c.b[i].tl.data(), sizeof(double) * P.n);
-    std::memcpy(to + (size_t)i * P.n, c.b[i].to.data(), sizeof(double) * P.n);
+    std::memcpy(A + (size_t)i * P.n * P.n, c.b[i].A.a.data(), sizeof(Real) * P.n * P.n);
+    std::memcpy(tl + (size_t)i * P.n, c.b[i].tl.data(), sizeof(Real) * P.n);
+    std::memcpy(to + (size_t)i * P.n, c.b[i].to.data(), sizeof(Real) * P.n);
   }
 }
 
@@ -1540,87 +1572,90 @@ extern "C" {
 
 struct pdqo_out {
   // Caller-allocated; any pointer may be null.
-  double* t;            // K
-  double* mean;         // K * mean_size
-  double* chol;         // K * chol_size
-  double* os;           // K * nblk
+  Real* t;            // K
+  Real* mean;         // K * mean_size
+  Real* chol;         // K * chol_size
+  Real* os;           // K * nblk
   long* num_steps;      // K-1
-  double* filt_mean;    // smoothers: K * mean_size
-  double* filt_chol;    // K * chol_size
-  double* cond_A;       // smoothers: (K-1) * nblk*n*n
-  double* cond_mean;    // (K-1) * mean_size
-  double* cond_chol;    // (K-1) * chol_size
-  double* cond_tl;      // (K-1) * nblk*n
-  double* cond_to;      // (K-1) * nblk*n
-  double* term_mean;    // smoothers: mean_size
-  double* term_chol;    // chol_size
+  Real* filt_mean;    // smoothers: K * mean_size
+  Real* filt_chol;    // K * chol_size
+  Real* cond_A;       // smoothers: (K-1) * nblk*n*n
+  Real* cond_mean;    // (K-1) * mean_size
+  Real* cond_chol;    // (K-1) * chol_size
+  Real* cond_tl;      // (K-1) * nblk*n
+  Real* cond_to;      // (K-1) * nblk*n
+  Real* term_mean;    // smoothers: mean_size
+  Real* term_chol;    // chol_size
   long attempts, accepted;
   int status;
 };
 
 int pdqo_version(void) { return 1; }
 
-void pdqo_system_matrices(int nu, double* A, double* Q) {
+// Size of the scalar type every array and the configuration use.
+int pdqo_real_size(void) { return (int)sizeof(Real); }
+
+void pdqo_system_matrices(int nu, Real* A, Real* Q) {
   Mat A1, Q1;
   system_matrices_1d_iwp(nu, A1, Q1);
-  std::memcpy(A, A1.a.data(), sizeof(double) * A1.a.size());
-  std::memcpy(Q, Q1.a.data(), sizeof(double) * Q1.a.size());
+  std::memcpy(A, A1.a.data(), sizeof(Real) * A1.a.size());
+  std::memcpy(Q, Q1.a.data(), sizeof(Real) * Q1.a.size());
 }
 
-void pdqo_qr_r(int m, int n, const double* M, double* R) {
+void pdqo_qr_r(int m, int n, const Real* M, Real* R) {
   Mat A(m, n);
-  std::memcpy(A.a.data(), M, sizeof(double) * m * n);
+  std::memcpy(A.a.data(), M, sizeof(Real) * m * n);
   Mat r = qr_r(A);
-  std::memcpy(R, r.a.data(), sizeof(double) * r.a.size());
+  std::memcpy(R, r.a.data(), sizeof(Real) * r.a.size());
 }
 
-void pdqo_revert_conditional(int n, int k, const double* R_X_F, const double* R_X, const double* R_YX, int mode,
-                             double* R_Y, double* R_XY, double* G) {
+void pdqo_revert_conditional(int n, int k, const Real* R_X_F, const Real* R_X, const Real* R_YX, int mode,
+                             Real* R_Y, Real* R_XY, Real* G) {
   Mat a(n, k), b(n, n), c(k, k);
-  std::memcpy(a.a.data(), R_X_F, sizeof(double) * n * k);
-  std::memcpy(b.a.data(), R_X, sizeof(double) * n * n);
-  std::memcpy(c.a.data(), R_YX, sizeof(double) * k * k);
+  std::memcpy(a.a.data(), R_X_F, sizeof(Real) * n * k);
+  std::memcpy(b.a.data(), R_X, sizeof(Real) * n * n);
+  std::memcpy(c.a.data(), R_YX, sizeof(Real) * k * k);
   Reverted r = revert_conditional(a, b, c, mode);
-  std::memcpy(R_Y, r.R_Y.a.data(), sizeof(double) * k * k);
-  std::memcpy(R_XY, r.R_XY.a.data(), sizeof(double) * n * n);
-  std::memcpy(G, r.G.a.data(), sizeof(double) * n * k);
+  std::memcpy(R_Y, r.R_Y.a.data(), sizeof(Real) * k * k);
+  std::memcpy(R_XY, r.R_XY.a.data(), sizeof(Real) * n * n);
+  std::memcpy(G, r.G.a.data(), sizeof(Real) * n * k);
 }
 
 // IWP transition conditional for one block (dense: full n).
-void pdqo_transition(const pdqo_cfg* cfg, double dt, const double* os, double* A, double* noise_chol, double* tl,
-                     double* to) {
+void pdqo_transition(const pdqo_cfg* cfg, Real dt, const Real* os, Real* A, Real* noise_chol, Real* tl,
+                     Real* to) {
   Problem P;
   P.setup(*cfg, nullptr);
   Vec osv(P.nblk);
   for (int i = 0; i < P.nblk; ++i) osv[i] = os[i];
   BCond c = transition(P, dt, osv);
-  std::memcpy(A, c.b[0].A.a.data(), sizeof(double) * P.n * P.n);
-  std::memcpy(noise_chol, c.b[0].noise.L.a.data(), sizeof(double) * P.n * P.n);
-  std::memcpy(tl, c.b[0].tl.data(), sizeof(double) * P.n);
-  std::memcpy(to, c.b[0].to.data(), sizeof(double) * P.n);
+  std::memcpy(A, c.b[0].A.a.data(), sizeof(Real) * P.n * P.n);
+  std::memcpy(noise_chol, c.b[0].noise.L.a.data(), sizeof(Real) * P.n * P.n);
+  std::memcpy(tl, c.b[0].tl.data(), sizeof(Real) * P.n);
+  std::memcpy(to, c.b[0].to.data(), sizeof(Real) * P.n);
 }
 
-double pdqo_logpdf(int fact, int rows, int cols, const double* mean, const double* chol, const double* u) {
+Real pdqo_logpdf(int fact, int rows, int cols, const Real* mean, const Real* chol, const Real* u) {
   SSMBase ops;
   ops.fact = fact == FACT_ISO ? FACT_ISO : FACT_DENSE;
   Normal x;
   x.m = Mat(rows, cols);
   x.L = Mat(rows, rows);
-  std::memcpy(x.m.a.data(), mean, sizeof(double) * rows * cols);
-  std::memcpy(x.L.a.data(), chol, sizeof(double) * rows * rows);
+  std::memcpy(x.m.a.data(), mean, sizeof(Real) * rows * cols);
+  std::memcpy(x.L.a.data(), chol, sizeof(Real) * rows * rows);
   Mat uu(rows, cols);
-  std::memcpy(uu.a.data(), u, sizeof(double) * rows * cols);
+  std::memcpy(uu.a.data(), u, sizeof(Real) * rows * cols);
   return ops.logpdf(x, uu);
 }
 
-double pdqo_control(const pdqo_cfg* cfg, double dt, double* state, double error_power) {
+Real pdqo_control(const pdqo_cfg* cfg, Real dt, Real* state, Real error_power) {
   Problem P;
   P.cfg = *cfg;
   return control_apply(P, dt, *state, error_power);
 }
 
-void pdqo_taylor_coefficients(int vf_id, int d, const double* params, const double* y0, double t0, int num,
-                              double* out) {
+void pdqo_taylor_coefficients(int vf_id, int d, const Real* params, const Real* y0, Real t0, int num,
+                              Real* out) {
   Vf vf;
   vf.id = vf_id;
   vf.d = d;
@@ -1628,7 +1663,7 @@ void pdqo_taylor_coefficients(int vf_id, int d, const double* params, const doub
   taylor_coefficients(vf, y0, t0, num, out);
 }
 
-void pdqo_vf(int vf_id, int d, const double* params, const double* y, double t, double* f, double* J) {
+void pdqo_vf(int vf_id, int d, const Real* params, const Real* y, Real t, Real* f, Real* J) {
   Vf vf;
   vf.id = vf_id;
   vf.d = d;
@@ -1664,8 +1699,8 @@ static void write_outputs(const Problem& P, const Outputs& o, const Final& F, pd
 }
 
 // One adaptive save_at solve.  tcoeffs: (nu+1) x d derivatives.
-int pdqo_solve_save_at(const pdqo_cfg* cfg, const double* params, const double* tcoeffs, double init_std,
-                       const double* save_at, int K, pdqo_out* out) {
+int pdqo_solve_save_at(const pdqo_cfg* cfg, const Real* params, const Real* tcoeffs, Real init_std,
+                       const Real* save_at, int K, pdqo_out* out) {
   Problem P;
   P.setup(*cfg, params);
   BNormal init = prior_init(P, tcoeffs, init_std);
@@ -1683,8 +1718,8 @@ int pdqo_solve_save_at(const pdqo_cfg* cfg, const double* params, const double* 
   return 0;
 }
 
-int pdqo_solve_fixed_grid(const pdqo_cfg* cfg, const double* params, const double* tcoeffs, double init_std,
-                          const double* grid, int K, pdqo_out* out) {
+int pdqo_solve_fixed_grid(const pdqo_cfg* cfg, const Real* params, const Real* tcoeffs, Real init_std,
+                          const Real* grid, int K, pdqo_out* out) {
   Problem P;
   P.setup(*cfg, params);
   BNormal init = prior_init(P, tcoeffs, init_std);
@@ -1698,8 +1733,8 @@ int pdqo_solve_fixed_grid(const pdqo_cfg* cfg, const double* params, const doubl
 
 // Marginal log-likelihood of data (Kobs x d) at every `every`-th grid point
 // (grid[0] included), fixed-point smoother on the fixed grid.
-double pdqo_lml(const pdqo_cfg* cfg, const double* params, const double* tcoeffs, double init_std, const double* grid,
-                int K, int every, const double* data, const double* stds, int nstd, int average) {
+Real pdqo_lml(const pdqo_cfg* cfg, const Real* params, const Real* tcoeffs, Real init_std, const Real* grid,
+                int K, int every, const Real* data, const Real* stds, int nstd, int average) {
   Problem P;
   P.setup(*cfg, params);
   BNormal init = prior_init(P, tcoeffs, init_std);
@@ -1712,27 +1747,27 @@ double pdqo_lml(const pdqo_cfg* cfg, const double* params, const double* tcoeffs
 
 // LML as a function of (vector-field params, y0): Taylor coefficients are
 // recomputed from y0 (jet init, num = nu).
-double pdqo_lml_from_y0(const pdqo_cfg* cfg, const double* params, const double* y0, const double* grid, int K,
-                        int every, const double* data, const double* stds, int nstd, int average) {
-  std::vector<double> tc((size_t)(cfg->nu + 1) * cfg->d);
+Real pdqo_lml_from_y0(const pdqo_cfg* cfg, const Real* params, const Real* y0, const Real* grid, int K,
+                        int every, const Real* data, const Real* stds, int nstd, int average) {
+  std::vector<Real> tc((size_t)(cfg->nu + 1) * cfg->d);
   pdqo_taylor_coefficients(cfg->vf_id, cfg->d, params, y0, grid[0], cfg->nu, tc.data());
   return pdqo_lml(cfg, params, tc.data(), 0.0, grid, K, every, data, stds, nstd, average);
 }
 
 // Central finite-difference gradient of pdqo_lml_from_y0 with respect to
 // (params[0:np], y0[0:d]); the plain-path gradient baseline.
-double pdqo_lml_grad_fd(const pdqo_cfg* cfg, const double* params, int np, const double* y0, const double* grid, int K,
-                        int every, const double* data, const double* stds, int nstd, int average, double h,
-                        double* grad) {
-  std::vector<double> p(params, params + np), y(y0, y0 + cfg->d);
-  const double f0 = pdqo_lml_from_y0(cfg, p.data(), y.data(), grid, K, every, data, stds, nstd, average);
+Real pdqo_lml_grad_fd(const pdqo_cfg* cfg, const Real* params, int np, const Real* y0, const Real* grid, int K,
+                        int every, const Real* data, const Real* stds, int nstd, int average, Real h,
+                        Real* grad) {
+  std::vector<Real> p(params, params + np), y(y0, y0 + cfg->d);
+  const Real f0 = pdqo_lml_from_y0(cfg, p.data(), y.data(), grid, K, every, data, stds, nstd, average);
   for (int i = 0; i < np + cfg->d; ++i) {
-    double* x = i < np ? &p[i] : &y[i - np];
-    const double x0 = *x;
+    Real* x = i < np ? &p[i] : &y[i - np];
+    const Real x0 = *x;
     *x = x0 + h;
-    const double fp = pdqo_lml_from_y0(cfg, p.data(), y.data(), grid, K, every, data, stds, nstd, average);
+    const Real fp = pdqo_lml_from_y0(cfg, p.data(), y.data(), grid, K, every, data, stds, nstd, average);
     *x = x0 - h;
-    const double fm = pdqo_lml_from_y0(cfg, p.data(), y.data(), grid, K, every, data, stds, nstd, average);
+    const Real fm = pdqo_lml_from_y0(cfg, p.data(), y.data(), grid, K, every, data, stds, nstd, average);
     *x = x0;
     grad[i] = (fp - fm) / (2.0 * h);
   }
@@ -1740,8 +1775,8 @@ double pdqo_lml_grad_fd(const pdqo_cfg* cfg, const double* params, int np, const
 }
 
 // Save-every-step harness output; K_max bounds the number of rows written.
-int pdqo_solve_save_every_step(const pdqo_cfg* cfg, const double* params, const double* tcoeffs, double init_std,
-                               double t0, double t1, int K_max, pdqo_out* out) {
+int pdqo_solve_save_every_step(const pdqo_cfg* cfg, const Real* params, const Real* tcoeffs, Real init_std,
+                               Real t0, Real t1, int K_max, pdqo_out* out) {
   Problem P;
   P.setup(*cfg, params);
   BNormal init = prior_init(P, tcoeffs, init_std);
@@ -1755,8 +1790,8 @@ int pdqo_solve_save_every_step(const pdqo_cfg* cfg, const double* params, const 
 }
 
 // The LML configuration's forward pass with its Markov-sequence outputs.
-int pdqo_solve_fixed_grid_checkpoints(const pdqo_cfg* cfg, const double* params, const double* tcoeffs,
-                                      double init_std, const double* grid, int K, int every, pdqo_out* out) {
+int pdqo_solve_fixed_grid_checkpoints(const pdqo_cfg* cfg, const Real* params, const Real* tcoeffs,
+                                      Real init_std, const Real* grid, int K, int every, pdqo_out* out) {
   Problem P;
   P.setup(*cfg, params);
   BNormal init = prior_init(P, tcoeffs, init_std);
@@ -1771,8 +1806,8 @@ int pdqo_solve_fixed_grid_checkpoints(const pdqo_cfg* cfg, const double* params,
 // Batched adaptive solves for the CPU baseline: instance i has its own
 // params, tcoeffs and save_at row; outputs only the terminal mean and the
 // attempt/accept counts.  OpenMP over instances.
-int pdqo_solve_save_at_batch(const pdqo_cfg* cfg, int n_inst, const double* params, int n_params,
-                             const double* tcoeffs, const double* save_at, int K, double* term_mean, long* attempts,
+int pdqo_solve_save_at_batch(const pdqo_cfg* cfg, int n_inst, const Real* params, int n_params,
+                             const Real* tcoeffs, const Real* save_at, int K, Real* term_mean, long* attempts,
                              long* accepted, int n_threads) {
   int err = 0;
 #ifdef _OPENMP

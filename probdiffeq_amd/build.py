@@ -1,7 +1,10 @@
 """Compile the gfx950 HIP library (libpdq.so) in-tree with hipcc.
 
-The three translation units (C ABI, cooperative dense/isotropic kernel,
-blockdiag kernel) compile in parallel and link into one shared library.
+The translation units (C ABI, cooperative dense/isotropic kernel) compile in
+parallel and link into one shared library.  The per-block blockdiag kernel
+(csrc/pdq_blockdiag.hip) is not part of the device library: it did not finish
+on its first device run and stays off the device until that is explained; its
+code is still compiled for the CPU by the kernel-emulation tests.
 """
 
 from __future__ import annotations
@@ -14,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libpdq.so")
-UNITS = ["pdq_capi.hip", "pdq_coop.hip", "pdq_blockdiag.hip"]
+UNITS = ["pdq_capi.hip", "pdq_coop.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

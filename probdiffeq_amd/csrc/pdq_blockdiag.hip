@@ -344,6 +344,10 @@ __device__ void solve_instance_bd(const BArgs& a, int inst, double* y, double* r
               status = PDQ_STATUS_DT_UNDERFLOW;
               break;
             }
+            if (ep >= 1.0 && t_pr == t_sf) {
+              status = PDQ_STATUS_DT_UNDERFLOW;  // accepted without advancing t
+              break;
+            }
             if ((cfg.max_attempts > 0 && att >= cfg.max_attempts && ep < 1.0) || attempts >= kMaxTotalAttempts) {
               status = PDQ_STATUS_MAX_ATTEMPTS;
               break;

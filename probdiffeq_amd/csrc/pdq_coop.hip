@@ -1197,11 +1197,13 @@ __device__ void solve_instance(const Ctx<G>& c, int inst) {
               status = PDQ_STATUS_NONFINITE;
               break;
             }
-            if (dt == 0.0 || slotv(c, sf).sc[SC_T] + dt == slotv(c, sf).sc[SC_T]) {
-              if (ep < 1.0) {
-                status = PDQ_STATUS_DT_UNDERFLOW;
-                break;
-              }
+            if (ep < 1.0 && (dt == 0.0 || slotv(c, sf).sc[SC_T] + dt == slotv(c, sf).sc[SC_T])) {
+              status = PDQ_STATUS_DT_UNDERFLOW;
+              break;
+            }
+            if (ep >= 1.0 && slotv(c, pr).sc[SC_T] == slotv(c, sf).sc[SC_T]) {
+              status = PDQ_STATUS_DT_UNDERFLOW;  // accepted without advancing t
+              break;
             }
             if ((cfg.max_attempts > 0 && att >= cfg.max_attempts && ep < 1.0) || attempts >= kMaxTotalAttempts) {
               status = PDQ_STATUS_MAX_ATTEMPTS;

@@ -21,7 +21,7 @@ namespace pdq {
 constexpr int kMaxParams = 64;   // vector-field parameters staged in LDS
 constexpr int kMaxTangents = 8;  // forward-mode directions of the LML gradient
 // Safety net that bounds every instance's loop (the reference has none).
-constexpr long kMaxTotalAttempts = 100000000L;
+constexpr long kMaxTotalAttempts = 1L << 22;  // per solve: every persistent wave ends in bounded time
 
 // ------------------------------------------------------------ vector fields
 // The registry restates problems.py:283-295's u' = f(u, t) for the device.

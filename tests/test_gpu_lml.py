@@ -1,0 +1,84 @@
+"""GPU: marginal log-likelihood value and gradient through the C ABI.
+
+Value: against the oracle's loss_lml_timeseries restatement in FP64 and in
+long double, with the accuracy bound of the solve parity tests.  Gradient (forward
+mode through the solve and the reverse LML scan): against Richardson
+finite differences of the oracle, 1e-8 normwise relative.
+"""
+
+import numpy as np
+import pytest
+
+import lml_cases as LC
+import test_gpu_parity as T
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pdq():
+    from probdiffeq_amd import capi
+
+    capi.require_device()
+    capi.lib()
+    return capi
+
+
+def _dev(x):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64).cuda()
+
+
+def _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, average=True, grad=True):
+    ocfg = T._oracle_cfg(oracle, cfg)
+    print(f"lml case: fact={cfg.factorisation} solver={cfg.solver} lanes={cfg.lanes_per_instance} B={params.shape[0]} K={grid.size}", flush=True)
+    v, g = pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds), average=average, grad=grad)
+    torch.cuda.synchronize()
+    v = v.cpu().numpy()
+    g = g.cpu().numpy() if grad else None
+    for i in range(params.shape[0]):
+        vo, vf, vt = LC.oracle_value(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
+        T.check_accuracy(v[i], vo, vf, vt, f"lml {cfg.factorisation}/{cfg.solver} instance {i}")
+        if grad:
+            gr = LC.richardson_grad(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
+            assert LC.rel(g[i], gr) < LC.GRAD_RTOL, (i, g[i], gr)
+    return v, g
+
+
+@pytest.mark.parametrize("lanes", [0, 8, 64])
+def test_lml_isotropic_batch(oracle, pdq, lanes):
+    params, y0, grid, every, data, stds = LC.lml_inputs(6, seed=lanes)
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver", lanes_per_instance=lanes)
+    _check(oracle, pdq, cfg, params, y0, grid, every, data, stds)
+
+
+@pytest.mark.parametrize("fact,solver,per_coordinate_std,average", [("isotropic", "dynamic", False, False), ("dense", "solver", True, True), ("dense", "dynamic", False, True)])
+def test_lml_variants(oracle, pdq, fact, solver, per_coordinate_std, average):
+    params, y0, grid, every, data, stds = LC.lml_inputs(4, seed=5, t_end=10.0, n_grid=1001, every=10, per_coordinate_std=per_coordinate_std)
+    cfg = pdq.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver=solver)
+    _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, average=average)
+
+
+def test_lml_single_instance_value_only(oracle, pdq):
+    params, y0, grid, every, data, stds = LC.lml_inputs(1, seed=9)
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver")
+    v_only, _ = _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=False)
+    v_grad, _ = _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=True)
+    assert v_only[0] == v_grad[0]
+
+
+def test_torch_autograd_function(oracle, pdq):
+    from probdiffeq_amd import torch_ops
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(3, seed=4)
+    prob = torch_ops.LMLProblem(vector_field="lotka_volterra", num_derivatives=3, ode_dim=2, grid=_dev(grid), every=every)
+    P = _dev(params).requires_grad_(True)
+    Y = _dev(y0).requires_grad_(True)
+    value = torch_ops.lml(prob, _dev(data), _dev(stds), params=P, y0=Y)
+    w = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64, device="cuda")
+    (value * w).sum().backward()
+    v_ref, g_ref = pdq.lml_value_and_grad(prob.config(4), _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds))
+    assert torch.equal(value.detach(), v_ref)
+    full = torch.cat([P.grad, Y.grad], dim=1)
+    assert torch.allclose(full, g_ref * w[:, None], rtol=0, atol=0)

@@ -1,22 +1,31 @@
 """GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
 
 Step counts (attempts, accepted steps, num_steps) and the output times are
-identical.  Every floating-point output array is compared normwise relative
-(max |gpu - oracle| <= tol * max |oracle|) with
+identical.  Every floating-point output array X is compared normwise
+(rel(a, b) = max |a - b| / max |b|) against three oracle solves of the same
+float64 inputs: the FP64 restatement compiled with strict evaluation order
+("ref") and with FMA contraction ("fma"), two valid FP64 roundings of the
+reference's algorithm, and the same restatement in long double ("truth",
+64-bit mantissa).  The FP64 reference's own rounding error on X is
 
-    tol = max(1e-10, 10 * spread),
+    e_ref = max(rel(ref, truth), rel(fma, truth)),
 
-where ``spread`` is the same normwise difference between the oracle compiled
-with strict evaluation order and the oracle compiled with FMA contraction: two
-valid FP64 roundings of the same algorithm.  Where an output is well
-conditioned (the ODE solution, most covariances) the spread is ~1e-14 and the
-bound is 1e-10.  Some outputs are not: the top Taylor coefficients (scaled by
-dt^-k), the coefficient-0 covariance block (variances ~1e-12 inside
-covariances dominated by the top coefficients) and, on stiff van der Pol with
-ts1, a dynamic calibration whose whitened residual is at cancellation level;
-there the two oracle builds already differ by 1e-9 .. 1e-3 and the bound
-follows.  Covariances are compared as L L^T; factor signs (LAPACK convention)
-are checked on the dense filter solve.
+and the kernels must meet
+
+    rel(gpu, truth) <= tol  and  rel(gpu, ref) <= tol,
+    tol = max(1e-10, ACCURACY_FACTOR * e_ref).
+
+Where X is well conditioned (the ODE solution u(t): e_ref ~ 1e-15) this is
+1e-10 relative.  Some outputs are not: the top Taylor coefficients (scaled by
+dt^-k), covariances and the dynamic calibration carry FP64 rounding errors
+of 1e-12 .. 1e-3 in the reference itself (the IWP noise factor is the
+Cholesky factor of a Hilbert-type matrix; on stiff van der Pol with ts1 the
+calibration's whitened residual is at cancellation level); there the GPU
+result must be as accurate as the FP64 reference within ACCURACY_FACTOR.
+Measured over the parity suite, the median of rel(gpu, truth) / e_ref is
+about 1.  Covariances are compared as L L^T; factor signs (LAPACK
+convention) are checked on the dense filter solve.  With
+PDQ_PARITY_REPORT=<path>, every comparison is written there as JSON.
 """
 
 import numpy as np
@@ -28,7 +37,8 @@ pytestmark = pytest.mark.gpu
 
 THETA = np.array([0.5, 0.05, 0.5, 0.05])
 RTOL_FLOOR = 1e-10
-SENSITIVITY_FACTOR = 10.0
+ACCURACY_FACTOR = 10.0
+REPORT = []
 
 
 def _rel(a, b):
@@ -111,20 +121,35 @@ def _fields(cfg, r):
 
 
 def _oracle_solve(oracle, ocfg, params, tcs, times, fixed_grid):
+    """(FP64 strict, FP64 FMA, long double) oracle solves of the same float64 inputs."""
     f = oracle.solve_fixed_grid if fixed_grid else oracle.solve_save_at
-    strict = f(ocfg, params, tcs, times)
+    ref = f(ocfg, params, tcs, times)
     with oracle.rounding("fma"):
         fma = f(ocfg, params, tcs, times)
-    assert strict["status"] == 0 and fma["status"] == 0
-    assert strict["attempts"] == fma["attempts"]
-    return strict, fma
+    with oracle.rounding("extended"):
+        truth = f(ocfg, params, tcs, times)
+    for r in (ref, fma, truth):
+        assert r["status"] == 0
+        assert r["attempts"] == ref["attempts"], "the oracle builds took different step sequences"
+    return ref, fma, truth
 
 
-def _check_fields(cfg, got, strict, fma, label):
-    for key, exp in _fields(cfg, strict).items():
-        tol = max(RTOL_FLOOR, SENSITIVITY_FACTOR * _rel(_fields(cfg, fma)[key], exp))
-        err = _rel(got[key][: exp.shape[0]] if key == "output_scale" else got[key], exp)
-        assert err <= tol, f"{label}: {key} rel err {err:.3e} > tol {tol:.3e}"
+def check_accuracy(got, ref, fma, truth, label):
+    """Assert the accuracy bound for one array; records the measured numbers."""
+    got, ref, fma, truth = (np.asarray(x, dtype=np.float64) for x in (got, ref, fma, truth))
+    e_ref = max(_rel(ref, truth), _rel(fma, truth))
+    e_gpu = _rel(got, truth)
+    d = _rel(got, ref)
+    tol = max(RTOL_FLOOR, ACCURACY_FACTOR * e_ref)
+    REPORT.append({"case": label, "e_ref": e_ref, "e_gpu": e_gpu, "gpu_vs_ref": d, "tol": tol})
+    assert e_gpu <= tol and d <= tol, f"{label}: rel(gpu, truth) {e_gpu:.3e}, rel(gpu, ref) {d:.3e} > tol {tol:.3e} (e_ref {e_ref:.3e})"
+
+
+def _check_fields(cfg, got, ref, fma, truth, label):
+    fr, ff, ft = _fields(cfg, ref), _fields(cfg, fma), _fields(cfg, truth)
+    for key, exp in fr.items():
+        g = got[key][: exp.shape[0]] if key == "output_scale" else got[key]
+        check_accuracy(g, exp, ff[key], ft[key], f"{label} {key}")
 
 
 def _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=False, check_factor_signs=False):
@@ -137,16 +162,15 @@ def _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=False, check_facto
     g = {k: v.cpu().numpy() for k, v in out.items()}
     n, nblk = cfg.n, cfg.nblk
     for i in range(B):
-        r, rf = _oracle_solve(oracle, ocfg, params[i], tcs[i], times[i], fixed_grid)
+        r, rf, rt = _oracle_solve(oracle, ocfg, params[i], tcs[i], times[i], fixed_grid)
         assert g["status"][i] == 0, f"instance {i} status {g['status'][i]}"
         assert g["attempts"][i] == r["attempts"], (i, g["attempts"][i], r["attempts"])
         assert g["accepted"][i] == r["accepted"]
         assert np.array_equal(g["num_steps"][i], r["num_steps"])
         assert _rel(g["t"][i], r["t"]) < 1e-14
         got = {k: g[k][i] for k in ("mean", "chol", "output_scale", "filt_mean", "filt_chol", "cond_A", "cond_mean", "term_mean") if k in g}
-        _check_fields(cfg, _fields(cfg, got), r, rf, f"instance {i}")
+        _check_fields(cfg, _fields(cfg, got), r, rf, rt, f"{cfg.factorisation}/{cfg.strategy}/{cfg.solver} instance {i}")
         if check_factor_signs:
-            assert _rel(g["chol"][i], r["chol"]) <= max(RTOL_FLOOR, SENSITIVITY_FACTOR * _rel(rf["chol"], r["chol"]))
             assert np.all(np.sign(np.diagonal(g["chol"][i].reshape(-1, n, n), axis1=1, axis2=2)) == np.sign(np.diagonal(r["chol"].reshape(-1, n, n), axis1=1, axis2=2)))
     return g
 
@@ -203,14 +227,14 @@ def test_fixed_grid(oracle, pdq, fact):
     _compare(oracle, pdq, cfg, params, y0, times, fixed_grid=True)
 
 
-@pytest.mark.skip(reason="blockdiag kernel exceeded the 120 s test limit on its first GPU run; off the GPU until the cause is shown")
+@pytest.mark.skip(reason="blockdiag device kernel did not finish within the 120 s test limit on its first GPU run; the cause is not shown, so the factorisation is disabled on the device (its kernel logic is checked by the CPU emulation tests)")
 def test_blockdiag_lotka_volterra(oracle, pdq):
     cfg = pdq.Config(factorisation="blockdiag", num_derivatives=4, strategy="filter", solver="dynamic")
     params, y0, times = _lv_batch(5, seed=5)
     _compare(oracle, pdq, cfg, params, y0, times)
 
 
-@pytest.mark.skip(reason="blockdiag kernel exceeded the 120 s test limit on its first GPU run; off the GPU until the cause is shown")
+@pytest.mark.skip(reason="blockdiag device kernel did not finish within the 120 s test limit on its first GPU run; the cause is not shown, so the factorisation is disabled on the device (its kernel logic is checked by the CPU emulation tests)")
 def test_blockdiag_burgers_small_grid(oracle, pdq):
     """C4 kind at a small grid: periodic Burgers, blockdiag nu=2, filter."""
     Ng = 64
@@ -222,6 +246,15 @@ def test_blockdiag_burgers_small_grid(oracle, pdq):
     times = np.tile([0.0, 0.2], (B, 1))
     cfg = pdq.Config(factorisation="blockdiag", num_derivatives=2, ode_dim=Ng, vector_field="burgers", n_params=2, rtol=1e-4, atol=1e-6)
     _compare(oracle, pdq, cfg, params, y0, times)
+
+
+def test_blockdiag_is_refused_on_the_device(pdq):
+    """The C ABI refuses blockdiag before any allocation or launch."""
+    cfg = pdq.Config(factorisation="blockdiag", num_derivatives=4, strategy="filter", solver="dynamic")
+    params, y0, times = _lv_batch(1, seed=5)
+    tcs = np.zeros((1, 5, 2))
+    with pytest.raises(pdq.PdqError, match="blockdiag"):
+        pdq.solve(cfg, _dev(params), _dev(tcs), _dev(times))
 
 
 def test_taylor_coefficients_match_oracle(oracle, pdq):

@@ -32,13 +32,13 @@ def _run(oracle, emu, cfg, params, y0s, times, fixed_grid=False):
     tcs = np.stack([oracle.taylor_coefficients(cfg.vector_field, params[i], y0s[i], times[i, 0], cfg.num_derivatives) for i in range(B)])
     g = emu.solve(cfg, params, tcs, times, fixed_grid)
     for i in range(B):
-        r, rf = T._oracle_solve(oracle, ocfg, params[i], tcs[i], times[i], fixed_grid)
+        r, rf, rt = T._oracle_solve(oracle, ocfg, params[i], tcs[i], times[i], fixed_grid)
         assert g["status"][i] == 0
         assert g["attempts"][i] == r["attempts"]
         assert np.array_equal(g["num_steps"][i], r["num_steps"])
         assert T._rel(g["t"][i], r["t"]) < 1e-14
         got = {k: g[k][i] for k in ("mean", "chol", "output_scale", "filt_mean", "filt_chol", "cond_A", "cond_mean", "term_mean") if k in g}
-        T._check_fields(cfg, T._fields(cfg, got), r, rf, f"instance {i}")
+        T._check_fields(cfg, T._fields(cfg, got), r, rf, rt, f"emulation instance {i}")
     return g
 
 
@@ -111,3 +111,23 @@ def test_status_codes(emu):
     bad[0, 1, 0] = np.nan
     g = emu.solve(capi.Config(), p, bad, t)
     assert g["status"].tolist() == [2, 0]
+
+
+@pytest.mark.parametrize(
+    "fact,solver,per_coordinate_std,average",
+    [("isotropic", "solver", False, True), ("isotropic", "dynamic", False, False), ("dense", "solver", True, True)],
+)
+def test_lml_value_and_gradient(oracle, emu, fact, solver, per_coordinate_std, average):
+    """The LML kernel code: value against the oracle, forward-mode gradient against Richardson FD."""
+    import lml_cases as LC
+    from probdiffeq_amd import capi
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(2, seed=3, t_end=10.0, n_grid=501, every=10, per_coordinate_std=per_coordinate_std)
+    cfg = capi.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver=solver)
+    ocfg = T._oracle_cfg(oracle, cfg)
+    v, g = emu.lml(cfg, params, y0, grid, every, data, stds, average=average)
+    for i in range(2):
+        vo, vf, vt = LC.oracle_value(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
+        T.check_accuracy(v[i], vo, vf, vt, f"emulation lml instance {i}")
+        gr = LC.richardson_grad(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
+        assert LC.rel(g[i], gr) < LC.GRAD_RTOL, (g[i], gr)
