@@ -1,0 +1,119 @@
+"""GPU: the reference-shaped Python API (probdiffeq_amd.ivpsolve / .probdiffeq).
+
+Mirrors the reference's solve tests (tests/test_ivpsolve/test_solve_adaptive_save_at.py:
+211-247, test_solve_fixed_grid.py): the API builds the configuration, the C ABI
+runs the kernels, and the result is checked against the oracle (same step
+sequence, the accuracy bound of test_gpu_parity) and against a high-accuracy
+DOP853 reference solution at the solver's tolerance.
+"""
+
+import numpy as np
+import pytest
+
+import test_gpu_parity as T
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    from probdiffeq_amd import capi, ivpsolve
+    from probdiffeq_amd import probdiffeq as pdx
+
+    capi.require_device()
+    return pdx, ivpsolve
+
+
+def _dop853(theta, y0, t):
+    from scipy.integrate import solve_ivp
+
+    a, b, c, d = theta
+    s = solve_ivp(lambda _, y: [a * y[0] - b * y[0] * y[1], -c * y[1] + d * y[0] * y[1]], (t[0], t[-1]), y0, method="DOP853", rtol=1e-13, atol=1e-13, t_eval=t)
+    return s.y.T
+
+
+def _setup(pdx, fact, strategy, solver_name, u0, num=4):
+    vf = pdx.lotka_volterra(*T.THETA)
+    tcoeffs, _ = pdx.jetexpand_ode_padded_scan(num=num)(vf, [u0], t=0.0)
+    ssm = {"dense": pdx.state_space_model_dense, "isotropic": pdx.state_space_model_isotropic}[fact]()
+    init = ssm.prior_wiener_integrated(tcoeffs)
+    constraint = ssm.constraint_ode_ts0(vf)
+    strat = {"filter": pdx.strategy_filter, "fixedpoint": pdx.strategy_smoother_fixedpoint, "fixedinterval": pdx.strategy_smoother_fixedinterval}[strategy]()
+    slv = {"solver": pdx.solver, "dynamic": pdx.solver_dynamic, "mle": pdx.solver_mle}[solver_name](strategy=strat, constraint=constraint)
+    return init, slv, pdx.error_residual_std(constraint=constraint)
+
+
+def test_quickstart_save_at_matches_oracle_and_dop853(oracle, api):
+    """examples/A0_quickstart-like solve: dense filter, dynamic calibration, 200 save_at points."""
+    pdx, ivpsolve = api
+    u0 = torch.tensor([20.0, 20.0], dtype=torch.float64, device="cuda")
+    init, slv, err = _setup(pdx, "dense", "filter", "dynamic", u0)
+    save_at = torch.linspace(0.0, 50.0, 200, dtype=torch.float64, device="cuda")
+    sol = ivpsolve.solve_adaptive_save_at(solver=slv, error=err)(init, save_at=save_at, atol=1e-8, rtol=1e-6)
+    u = sol.u.mean[0].cpu().numpy()
+    assert u.shape == (200, 2) and sol.t.shape == (200,)
+    truth = _dop853(T.THETA, [20.0, 20.0], save_at.cpu().numpy())
+    assert np.max(np.abs(u - truth) / (1e-8 + 1e-6 * np.abs(truth))) < 50.0
+    tc = oracle.taylor_coefficients("lotka_volterra", T.THETA, [20.0, 20.0], 0.0, 4)
+    r = oracle.solve_save_at(oracle.Config(), T.THETA, tc, save_at.cpu().numpy())
+    assert int(sol.attempts) == r["attempts"]
+    assert np.array_equal(sol.num_steps.cpu().numpy(), r["num_steps"])
+    assert T._rel(u, r["mean"][:, :2]) < T.RTOL_FLOOR
+
+
+def test_batched_fixedpoint_api_equals_c_abi(api):
+    """A batch of initial values through the API gives exactly the C ABI's batched result."""
+    pdx, ivpsolve = api
+    from probdiffeq_amd import capi
+
+    rng = np.random.default_rng(0)
+    y0 = torch.tensor(rng.uniform(10, 30, size=(16, 2)), dtype=torch.float64, device="cuda")
+    init, slv, err = _setup(pdx, "isotropic", "fixedpoint", "dynamic", y0)
+    save_at = torch.linspace(0.0, 20.0, 50, dtype=torch.float64, device="cuda")
+    sol = ivpsolve.solve_adaptive_save_at(solver=slv, error=err)(init, save_at=save_at, atol=1e-8, rtol=1e-6)
+    cfg = capi.Config(factorisation="isotropic", strategy="fixedpoint", solver="dynamic")
+    params = torch.tensor(np.tile(T.THETA, (16, 1)), dtype=torch.float64, device="cuda")
+    out = capi.solve(cfg, params, init.tcoeffs, save_at[None].expand(16, -1).contiguous())
+    assert torch.equal(sol.u.mean_flat, out["mean"].reshape(16, 50, 5, 2))
+    assert torch.equal(sol.u.cholesky_flat, out["chol"].reshape(16, 50, 5, 5))
+    assert torch.equal(sol.solution_full.posterior.conditional.A, out["cond_A"].reshape(16, 49, 5, 5))
+    assert torch.equal(sol.attempts, out["attempts"])
+
+
+def test_terminal_values(api):
+    pdx, ivpsolve = api
+    u0 = torch.tensor([20.0, 20.0], dtype=torch.float64, device="cuda")
+    init, slv, err = _setup(pdx, "isotropic", "filter", "dynamic", u0)
+    sol = ivpsolve.solve_adaptive_terminal_values(solver=slv, error=err)(init, t0=0.0, t1=10.0, atol=1e-8, rtol=1e-6)
+    assert float(sol.t) == 10.0
+    truth = _dop853(T.THETA, [20.0, 20.0], np.array([0.0, 10.0]))[-1]
+    u = sol.u.mean[0].cpu().numpy()
+    assert np.max(np.abs(u - truth) / (1e-8 + 1e-6 * np.abs(truth))) < 50.0
+
+
+def test_fixed_grid_api(oracle, api):
+    pdx, ivpsolve = api
+    u0 = torch.tensor([20.0, 20.0], dtype=torch.float64, device="cuda")
+    init, slv, _ = _setup(pdx, "dense", "fixedinterval", "dynamic", u0, num=3)
+    grid = torch.linspace(0.0, 5.0, 101, dtype=torch.float64, device="cuda")
+    sol = ivpsolve.solve_fixed_grid(solver=slv)(init, grid=grid)
+    tc = oracle.taylor_coefficients("lotka_volterra", T.THETA, [20.0, 20.0], 0.0, 3)
+    ocfg = oracle.Config(nu=3, strategy="fixedinterval")
+    r, rf, rt = T._oracle_solve(oracle, ocfg, T.THETA, tc, grid.cpu().numpy(), True)
+    T.check_accuracy(sol.u.mean_flat.cpu().numpy(), r["mean"], rf["mean"], rt["mean"], "api fixed grid mean")
+    assert int(sol.attempts) == 100
+
+
+def test_unsuitable_strategy_warns_and_failed_status_raises(api):
+    pdx, ivpsolve = api
+    u0 = torch.tensor([20.0, 20.0], dtype=torch.float64, device="cuda")
+    init, slv, err = _setup(pdx, "dense", "fixedinterval", "dynamic", u0)
+    with pytest.warns(UserWarning):
+        ivpsolve.solve_adaptive_save_at(solver=slv, error=err)
+    init, slv, err = _setup(pdx, "dense", "filter", "dynamic", u0)
+    solve = ivpsolve.solve_adaptive_save_at(solver=slv, error=err)
+    save_at = torch.linspace(0.0, 5.0, 3, dtype=torch.float64, device="cuda")
+    with pytest.raises(FloatingPointError):
+        solve(init, save_at=save_at, atol=1e-300, rtol=1e-300, max_attempts=3)
