@@ -66,6 +66,11 @@ def c5_inputs(n_inst=1024, seed=0):
     return params, y0, grid, data_noisy, stds[..., None]
 
 
+def _progress(msg: str) -> None:
+    """One line per finished stage on stderr (the JSON result line is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _events():
     import torch
 
@@ -223,10 +228,16 @@ def main():
     from probdiffeq_amd import capi, flops
 
     capi.require_device()
+    _progress("C2 batched + plain solves: start")
     c2, cfg, params, y0, save_at = bench_c2(args, torch, capi, flops)
+    _progress(f"C2 done: {c2['ms_per_step']:.2f} ms per batch, plain {c2['plain_solves_per_s']:.1f} solves/s")
     c5 = {} if args.skip_lml else bench_c5(args, torch, capi, flops)
+    if c5:
+        _progress(f"C5 done: {c5['lml_grads_per_s']:.1f} gradients/s")
     threads = min(16, os.cpu_count() or 1)
     cpu = None if args.skip_cpu else cpu_baseline(params, y0, save_at, args.cpu_sample, threads)
+    if cpu:
+        _progress(f"CPU baseline done: {cpu['value']:.1f} solves/s on {threads} threads")
     line = {
         "metric": METRIC,
         "value": c2["solves_per_s"],

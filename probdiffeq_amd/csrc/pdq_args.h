@@ -12,7 +12,6 @@
 
 #include "../../include/pdq.h"
 #include "pdq_layout.h"
-#include "pdq_shapes.h"
 
 namespace pdq {
 
@@ -62,13 +61,6 @@ int num_cus();
 
 // Launchers (one per translation unit, compiled in parallel).
 int launch_coop(KArgs& a, int G, hipStream_t st);
-// Compile-time-shape instantiations (pdq_coop_fixed.hip); returns
-// kNoFixedShape when no instantiation matches the layout and group size.
-constexpr int kNoFixedShape = -1000;
-int launch_coop_fixed(KArgs& a, int G, hipStream_t st);
-#define PDQ_DECLARE_SHAPE(name, ...) int launch_coop_##name(KArgs& a, int G, hipStream_t st);
-PDQ_FIXED_SHAPES(PDQ_DECLARE_SHAPE)
-#undef PDQ_DECLARE_SHAPE
 int launch_bd(BArgs& a, hipStream_t st);
 int launch_taylor_tangents(int vf, int d, int np, int n_inst, int num, int ntan, const double* params,
                            const double* y0, const double* t0, double* out, hipStream_t st);

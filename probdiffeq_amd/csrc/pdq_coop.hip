@@ -1,9 +1,1285 @@
-// The cooperative kernel for any dense / isotropic shape (dimensions read at
-// run time), the Taylor-tangent kernel of the LML path, and the dispatcher
-// that prefers a compile-time-shape instantiation when one exists.
-#include "pdq_coop.h"
+// Cooperative persistent kernel for the dense and isotropic factorisations.
+//
+// Mapping: a workgroup is one wavefront (64 lanes) split into 64/G groups of
+// G lanes; each group owns one IVP instance at a time and pulls the next one
+// from a device work queue when it finishes (uneven adaptive step counts).
+// Every matrix of an instance lives in LDS; the square-root algebra
+// (Householder QR in LAPACK's sign convention, triangular solves, GEMMs) is
+// executed cooperatively by the group's lanes in FP64.  A "covariance
+// dimension" nc and "mean columns" mc describe both factorisations:
+//   dense:      nc = (nu+1) d, mc = 1, ko = d observation rows
+//   isotropic:  nc = nu+1,     mc = d, ko = 1
+// (ssm_impl_dense.py:15-363, ssm_impl_isotropic.py:72-403).
+#include "pdq_args.h"
+#include "pdq_device.h"
 
 namespace pdq {
+
+// ---------------------------------------------------------------- context
+
+
+// Slot scalar indices.
+enum { SC_T = 0, SC_OS = 1, SC_NSTEPS = 2, SC_MLE_RUN = 3, SC_MLE_N = 4, SC_IDENT = 5 };
+
+template <int G>
+struct Ctx {
+  Grp<G> g;
+  const KArgs* a;
+  const Layout* L;
+  int nc, mc, mcp, ko, N, d, dense;  // mcp: primal mean columns (mc includes tangents)
+  double* sA;   // shared A (nc x nc)
+  double* sQ;   // shared Q (nc x nc)
+  double* ws;   // instance workspace base
+  double* prm;  // vector-field parameters (LDS copy)
+  __device__ double* slot(int s) const { return ws + s * L->s_size; }
+  __device__ double* W(int off) const { return ws + off; }
+};
+
+// ------------------------------------------------------------- primitives
+template <int G>
+__device__ __forceinline__ void g_copy(const Grp<G>& g, double* dst, const double* src, int n) {
+  for (int i = g.l; i < n; i += G) dst[i] = src[i];
+}
+template <int G>
+__device__ __forceinline__ void g_fill(const Grp<G>& g, double* dst, double v, int n) {
+  for (int i = g.l; i < n; i += G) dst[i] = v;
+}
+
+// In-place R-only Householder QR of the m x n matrix A (leading dim ld) in
+// LAPACK dgeqr2 convention (backend/linalg.py:8-37 -> geqrf); on exit the
+// upper triangle of rows 0..min(m,n)-1 holds R and the strict lower part is
+// zeroed.
+template <int G>
+__device__ void g_qr_r(const Grp<G>& g, double* A, int m, int n, int ld) {
+  const int k = m < n ? m : n;
+  for (int i = 0; i < k; ++i) {
+    if (m - i <= 1) break;
+    double ss = 0.0;
+    for (int r = i + 1 + g.l; r < m; r += G) {
+      const double x = A[r * ld + i];
+      ss += x * x;
+    }
+    ss = g.sum(ss);
+    if (ss == 0.0) continue;
+    const double alpha = A[i * ld + i];
+    const double beta = -copysign(hypot(alpha, sqrt(ss)), alpha);
+    const double tau = (beta - alpha) / beta;
+    const double scal = 1.0 / (alpha - beta);
+    g.sync();
+    for (int r = i + 1 + g.l; r < m; r += G) A[r * ld + i] *= scal;
+    g.sync();
+    for (int j = i + 1 + g.l; j < n; j += G) {
+      double w = A[i * ld + j];
+      for (int r = i + 1; r < m; ++r) w += A[r * ld + i] * A[r * ld + j];
+      w *= tau;
+      A[i * ld + j] -= w;
+      for (int r = i + 1; r < m; ++r) A[r * ld + j] -= A[r * ld + i] * w;
+    }
+    if (g.l == 0) A[i * ld + i] = beta;
+    g.sync();
+  }
+  // zero the Householder vectors (strict lower part) of the R rows
+  for (int e = g.l; e < k * n; e += G) {
+    const int i = e / n, j = e % n;
+    if (j < i) A[i * ld + j] = 0.0;
+  }
+  g.sync();
+}
+
+// G := solve_triu(R_Y, R12)^T (cholesky_util.py:77-78): R_Y k x k upper
+// (leading dim ldr), R12 k x n (same array, columns k..k+n-1), out n x k.
+// mode 1 = lstsq_svd semantics for k == 1 (pseudo-inverse of a scalar).
+template <int G>
+__device__ void g_gain(const Grp<G>& g, const double* R, int ldr, int k, int n, double* out, int mode) {
+  for (int j = g.l; j < n; j += G) {
+    // back substitution for column j of R12, written straight into out
+    for (int i = k - 1; i >= 0; --i) {
+      double s = R[i * ldr + k + j];
+      for (int c = i + 1; c < k; ++c) s -= R[i * ldr + c] * out[j * k + c];
+      const double r = R[i * ldr + i];
+      // mode 1: lstsq_svd for a 1 x 1 factor (pseudo-inverse of zero is zero)
+      out[j * k + i] = (mode == 1 && r == 0.0) ? 0.0 : s / r;
+    }
+  }
+}
+
+// ------------------------------------------------ IWP prior and transitions
+// preconditioner_taylor (utilities.py:74-84) expanded to nc entries.
+template <int G>
+__device__ void precon(const Ctx<G>& c, double dt, double* P, double* PI) {
+  for (int i = c.g.l; i < c.nc; i += G) {
+    const int deriv = c.dense ? i / c.d : i;
+    const int pw = c.N - 1 - deriv;
+    const double s = factorial_d(pw);
+    P[i] = pow(dt, (double)pw) / s;
+    PI[i] = pow(dt, -(double)pw) * s;
+  }
+}
+
+// Scalar factor of the transition noise: dense sqrt|dt| * s (Q holds the base
+// scale), isotropic (sqrt|dt| * base) * s (ssm_impl_dense.py:360-362,
+// ssm_impl_isotropic.py:375-376).
+template <int G>
+__device__ __forceinline__ double noise_scale(const Ctx<G>& c, double dt, double s) {
+  if (c.dense) return sqrt(fabs(dt)) * s;
+  return sqrt(fabs(dt)) * c.a->cfg.base_output_scale * s;
+}
+
+// apply_flat of the transition to a mean x (ssm_impl_dense.py:18-22,
+// ssm_impl_isotropic.py:75-79): out = p * (A (p^-1 * x)) (dense) or
+// (p * A) (p^-1 * x) (isotropic); the noise mean is zero.
+template <int G>
+__device__ void apply_flat_mean(const Ctx<G>& c, const double* P, const double* PI, const double* x, double* out) {
+  const int nc = c.nc, mc = c.mc;
+  for (int e = c.g.l; e < nc * mc; e += G) {
+    const int i = e / mc, col = e % mc;
+    double acc = 0.0;
+    if (c.dense) {
+      for (int k = 0; k < nc; ++k) acc += c.sA[i * nc + k] * (PI[k] * x[k * mc + col]);
+      out[e] = P[i] * (acc + 0.0);
+    } else {
+      for (int k = 0; k < nc; ++k) acc += (P[i] * c.sA[i * nc + k]) * (PI[k] * x[k * mc + col]);
+      out[e] = acc + 0.0;
+    }
+  }
+}
+
+// chol of apply_flat: |p| * (s Q)
+template <int G>
+__device__ void apply_flat_chol(const Ctx<G>& c, const double* P, double s, double* out) {
+  const int nc = c.nc;
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int i = e / nc;
+    out[e] = fabs(P[i]) * (s * c.sQ[e]);
+  }
+}
+
+// marginalise of the transition (ssm_impl_dense.py:24-33,
+// ssm_impl_isotropic.py:81-89): (Xm, XL) -> (Ym, YL).  big: 2nc x nc.
+template <int G>
+__device__ void marginalise_prior(const Ctx<G>& c, const double* P, const double* PI, double s, const double* Xm,
+                                  const double* XL, double* Ym, double* YL, double* big) {
+  const int nc = c.nc, mc = c.mc;
+  // rows 0..nc-1: (A (p^-1 * L))^T ; rows nc..2nc-1: (s Q)^T
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int r = e / nc, col = e % nc;  // big[r][col] = (A chol_l)[col][r]
+    double acc = 0.0;
+    for (int k = 0; k < nc; ++k) acc += c.sA[col * nc + k] * (PI[k] * XL[k * nc + r]);
+    big[r * nc + col] = acc;
+    big[(nc + r) * nc + col] = s * c.sQ[col * nc + r];
+  }
+  for (int e = c.g.l; e < nc * mc; e += G) {
+    const int i = e / mc, col = e % mc;
+    double acc = 0.0;
+    for (int k = 0; k < nc; ++k) acc += c.sA[i * nc + k] * (PI[k] * Xm[k * mc + col]);
+    Ym[e] = P[i] * (acc + 0.0);
+  }
+  c.g.sync();
+  g_qr_r(c.g, big, 2 * nc, nc, nc);
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int i = e / nc, j = e % nc;
+    YL[e] = fabs(P[i]) * big[j * nc + i];
+  }
+}
+
+// revert of the transition (ssm_impl_dense.py:51-77, isotropic :107-133,
+// cholesky_util.py:27-82): X -> observed (Ym, YL) and backward conditional.
+template <int G>
+__device__ void revert_prior(const Ctx<G>& c, const double* P, const double* PI, double s, const double* Xm,
+                             const double* XL, double* Ym, double* YL, double* bA, double* bm, double* bL, double* btl,
+                             double* bto, double* big, double* tmp) {
+  const int nc = c.nc, mc = c.mc, n2 = 2 * nc;
+  // [[R_YX, 0], [R_X_F, R_X]]; R_YX = (sQ)^T, R_X_F = (A chol_l)^T, R_X = chol_l^T
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int r = e / nc, col = e % nc;
+    big[r * n2 + col] = s * c.sQ[col * nc + r];
+    big[r * n2 + nc + col] = 0.0;
+    double acc = 0.0;
+    for (int k = 0; k < nc; ++k) acc += c.sA[col * nc + k] * (fabs(PI[k]) * XL[k * nc + r]);
+    big[(nc + r) * n2 + col] = acc;
+    big[(nc + r) * n2 + nc + col] = fabs(PI[col]) * XL[col * nc + r];
+  }
+  // mean_observed = A (p^-1 * m)
+  for (int e = c.g.l; e < nc * mc; e += G) {
+    const int i = e / mc, col = e % mc;
+    double acc = 0.0;
+    for (int k = 0; k < nc; ++k) acc += c.sA[i * nc + k] * (PI[k] * Xm[k * mc + col]);
+    tmp[e] = acc + 0.0;
+  }
+  c.g.sync();
+  g_qr_r(c.g, big, n2, n2, n2);
+  g_gain(c.g, big, n2, nc, nc, bA, 0);
+  c.g.sync();
+  for (int e = c.g.l; e < nc * mc; e += G) {
+    const int i = e / mc, col = e % mc;
+    double acc = 0.0;
+    for (int k = 0; k < nc; ++k) acc += bA[i * nc + k] * tmp[k * mc + col];
+    bm[e] = PI[i] * Xm[e] - acc;
+    Ym[e] = P[i] * tmp[e];
+  }
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int i = e / nc, j = e % nc;
+    bL[e] = big[(nc + j) * n2 + nc + i];
+    YL[e] = fabs(P[i]) * big[j * n2 + i];
+  }
+  for (int i = c.g.l; i < nc; i += G) {
+    btl[i] = 1.0 / P[i];
+    bto[i] = 1.0 / PI[i];
+  }
+}
+
+// merge (ssm_impl_dense.py:35-49, isotropic :91-105): out = self o other.
+// self_ident: self is the identity conditional (exact shortcut, see below).
+template <int G>
+__device__ void merge(const Ctx<G>& c, bool self_ident, const double* sA, const double* sm, const double* sL,
+                      const double* stl, const double* sto, const double* oA, const double* om, const double* oL,
+                      const double* otl, const double* oto, double* rA, double* rm, double* rL, double* rtl,
+                      double* rto, double* big) {
+  const int nc = c.nc, mc = c.mc;
+  if (self_ident) {
+    // I (T o A2) == T o A2 and qr_r of the upper-triangular (|T| o L2)^T
+    // stacked on zeros is that matrix itself (dlarfg skips zero
+    // sub-columns), so the general formula reduces exactly to this.
+    for (int e = c.g.l; e < nc * nc; e += G) {
+      const int i = e / nc;
+      const double T = 1.0 * oto[i];
+      rA[e] = T * oA[e];
+      rL[e] = fabs(T) * oL[e];
+    }
+    for (int e = c.g.l; e < nc * mc; e += G) {
+      const int i = e / mc;
+      rm[e] = (1.0 * oto[i]) * om[e] + 0.0;
+    }
+    for (int i = c.g.l; i < nc; i += G) {
+      rtl[i] = otl[i];
+      rto[i] = 1.0;
+    }
+    c.g.sync();
+    return;
+  }
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int i = e / nc, j = e % nc;
+    double acc = 0.0, acc2 = 0.0;
+    for (int k = 0; k < nc; ++k) {
+      const double T = stl[k] * oto[k];
+      acc += sA[i * nc + k] * (T * oA[k * nc + j]);
+      acc2 += sA[i * nc + k] * (fabs(T) * oL[k * nc + j]);
+    }
+    rA[e] = acc;
+    big[j * nc + i] = acc2;             // R1 = (A1 (|T| o C2))^T
+    big[(nc + j) * nc + i] = sL[e];     // R2 = C1^T
+  }
+  for (int e = c.g.l; e < nc * mc; e += G) {
+    const int i = e / mc, col = e % mc;
+    double acc = 0.0;
+    for (int k = 0; k < nc; ++k) acc += sA[i * nc + k] * ((stl[k] * oto[k]) * om[k * mc + col]);
+    rm[e] = acc + sm[e];
+  }
+  c.g.sync();
+  g_qr_r(c.g, big, 2 * nc, nc, nc);
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int i = e / nc, j = e % nc;
+    rL[e] = big[j * nc + i];
+  }
+  for (int i = c.g.l; i < nc; i += G) {
+    rtl[i] = otl[i];
+    rto[i] = sto[i];
+  }
+  c.g.sync();
+}
+
+// marginalise of a general (backward) conditional; used by the smoother's
+// backward pass (estimators_and_losses.py:156-178).
+template <int G>
+__device__ void marginalise_cond(const Ctx<G>& c, const double* cA, const double* cm, const double* cL,
+                                 const double* ctl, const double* cto, const double* Xm, const double* XL, double* Ym,
+                                 double* YL, double* big) {
+  const int nc = c.nc, mc = c.mc;
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int r = e / nc, col = e % nc;
+    double acc = 0.0;
+    for (int k = 0; k < nc; ++k) acc += cA[col * nc + k] * (ctl[k] * XL[k * nc + r]);
+    big[r * nc + col] = acc;
+    big[(nc + r) * nc + col] = cL[col * nc + r];
+  }
+  for (int e = c.g.l; e < nc * mc; e += G) {
+    const int i = e / mc, col = e % mc;
+    double acc = 0.0;
+    for (int k = 0; k < nc; ++k) acc += cA[i * nc + k] * (ctl[k] * Xm[k * mc + col]);
+    Ym[e] = cto[i] * (acc + cm[e]);
+  }
+  c.g.sync();
+  g_qr_r(c.g, big, 2 * nc, nc, nc);
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int i = e / nc, j = e % nc;
+    YL[e] = fabs(cto[i]) * big[j * nc + i];
+  }
+}
+
+// ------------------------------------------------------- linearisation
+// Row selector of ts0 (derivative 1) as an index into the state.
+template <int G>
+__device__ __forceinline__ int sel_row(const Ctx<G>& c, int obs) {
+  return c.dense ? c.d + obs : 1;
+}
+
+// ts0 / ts1 linearisation at the mean Xm (ssm_impl_dense.py:237-334,
+// ssm_impl_isotropic.py:298-317): writes b (ko x mc) and, for ts1, J (ko x nc).
+// With tangent columns (MODE_LML) b also carries d b / d theta.
+template <int G>
+__device__ void linearize(const Ctx<G>& c, const double* Xm, double t, double* b, double* J, double* y0buf) {
+  const int d = c.d, mc = c.mc;
+  const int id = c.a->cfg.vector_field;
+  // gather u(t) (zeroth Taylor coefficient) and, with tangents, its tangents
+  const int ntan = c.mcp == c.mc ? 0 : c.a->n_tan;
+  for (int k = c.g.l; k < d * (1 + ntan); k += G) {
+    const int q = k / d, j = k % d;
+    y0buf[k] = c.dense ? Xm[j * mc + q] : Xm[q * d + j];
+  }
+  c.g.sync();
+  const double* y = y0buf;
+  if (c.a->cfg.constraint == PDQ_TS0) {
+    for (int k = c.g.l; k < d * (1 + ntan); k += G) {
+      const int q = k / d, j = k % d;
+      double v;
+      if (q == 0) {
+        v = -vf_component(id, d, y, c.prm, t, j);
+      } else {
+        double acc = 0.0;
+        for (int jj = 0; jj < d; ++jj) acc += vf_jac(id, d, y, c.prm, j, jj) * y[q * d + jj];
+        const int qq = q - 1;
+        if (qq < c.a->cfg.n_params) acc += vf_jac_param(id, d, y, c.prm, j, qq);
+        v = -acc;
+      }
+      if (c.dense)
+        b[j * mc + q] = v;
+      else
+        b[q * d + j] = v;
+    }
+  } else {
+    // ts1 (dense): J = [-J_f, I, 0...], b = (m_1 - f(m_0)) - J m
+    const int nc = c.nc;
+    for (int e = c.g.l; e < d * nc; e += G) {
+      const int i = e / nc, j = e % nc;
+      double v = 0.0;
+      if (j < d) v = -vf_jac(id, d, y, c.prm, i, j);
+      else if (j == d + i) v = 1.0;
+      J[e] = v;
+    }
+    c.g.sync();
+    for (int i = c.g.l; i < d; i += G) {
+      const double f = vf_component(id, d, y, c.prm, t, i);
+      double jm = 0.0;
+      for (int k = 0; k < nc; ++k) jm += J[i * nc + k] * Xm[k * mc];
+      b[i * mc] = (Xm[(d + i) * mc] - f) - jm;
+    }
+  }
+  c.g.sync();
+}
+
+// (H L) entry (obs row o, column r) for ts0 / ts1.
+template <int G>
+__device__ __forceinline__ double HL(const Ctx<G>& c, const double* J, const double* XL, int o, int r) {
+  if (c.a->cfg.constraint == PDQ_TS0) return XL[sel_row(c, o) * c.nc + r];
+  double acc = 0.0;
+  for (int k = 0; k < c.nc; ++k) acc += J[o * c.nc + k] * XL[k * c.nc + r];
+  return acc;
+}
+template <int G>
+__device__ __forceinline__ double Hm(const Ctx<G>& c, const double* J, const double* Xm, int o, int col) {
+  if (c.a->cfg.constraint == PDQ_TS0) return Xm[sel_row(c, o) * c.mc + col];
+  double acc = 0.0;
+  for (int k = 0; k < c.nc; ++k) acc += J[o * c.nc + k] * Xm[k * c.mc + col];
+  return acc;
+}
+
+// fx.marginalise(u) -> observed (om ko x mc, oL ko x ko).  big: (nc+ko) x ko.
+template <int G>
+__device__ void marginalise_fx(const Ctx<G>& c, const double* b, const double* J, const double* Xm, const double* XL,
+                               double* om, double* oL, double* big) {
+  const int nc = c.nc, ko = c.ko, mc = c.mc;
+  const double damp = c.a->cfg.damp;
+  for (int e = c.g.l; e < (nc + ko) * ko; e += G) {
+    const int r = e / ko, o = e % ko;
+    big[e] = r < nc ? HL(c, J, XL, o, r) : (r - nc == o ? damp : 0.0);
+  }
+  for (int e = c.g.l; e < ko * mc; e += G) {
+    const int o = e / mc, col = e % mc;
+    om[e] = Hm(c, J, Xm, o, col) + b[e];
+  }
+  c.g.sync();
+  g_qr_r(c.g, big, nc + ko, ko, ko);
+  for (int e = c.g.l; e < ko * ko; e += G) {
+    const int i = e / ko, j = e % ko;
+    oL[e] = big[j * ko + i];
+  }
+  c.g.sync();
+}
+
+// residual_whitened_rms against zero data (ssm_impl_dense.py:156-160,
+// isotropic :203-207), primal columns only.  Every lane returns the value.
+template <int G>
+__device__ double rms_zero(const Ctx<G>& c, const double* om, const double* oL) {
+  const int ko = c.ko, mc = c.mc, mcp = c.mcp;
+  double* w = c.W(c.L->w_tri);
+  double ss = 0.0;
+  for (int col = c.g.l; col < mcp; col += G) {
+    for (int i = 0; i < ko; ++i) {
+      double s = c.dense ? (0.0 - om[i * mc + col]) : om[i * mc + col];
+      for (int j = 0; j < i; ++j) s -= oL[i * ko + j] * w[col * ko + j];
+      const double wi = s / oL[i * ko + i];
+      w[col * ko + i] = wi;
+      ss += wi * wi;
+    }
+  }
+  ss = c.g.sum(ss);
+  c.g.sync();
+  return sqrt(ss) / sqrt((double)(ko * mcp));
+}
+
+// fx.revert(pred) and apply_flat(data) (ssm_impl_api.py:112-132):
+// (Pm, PL) -> (Xm, XL); optionally returns the observed normal.  big:
+// (ko+nc)^2, gain: nc x ko, mo: ko x mc.
+template <int G>
+__device__ void update(const Ctx<G>& c, const double* b, const double* J, const double* Pm, const double* PL,
+                       double* Xm, double* XL, double* big, double* gain, double* mo, double* oL_out) {
+  const int nc = c.nc, ko = c.ko, mc = c.mc, n2 = ko + nc;
+  const double damp = c.a->cfg.damp;
+  for (int e = c.g.l; e < n2 * n2; e += G) {
+    const int r = e / n2, col = e % n2;
+    double v;
+    if (r < ko) v = col < ko ? (r == col ? damp : 0.0) : 0.0;
+    else v = col < ko ? HL(c, J, PL, col, r - ko) : PL[(col - ko) * nc + (r - ko)];
+    big[e] = v;
+  }
+  for (int e = c.g.l; e < ko * mc; e += G) {
+    const int o = e / mc, col = e % mc;
+    mo[e] = Hm(c, J, Pm, o, col) + b[e];
+  }
+  c.g.sync();
+  g_qr_r(c.g, big, n2, n2, n2);
+  g_gain(c.g, big, n2, ko, nc, gain, 0);
+  c.g.sync();
+  for (int e = c.g.l; e < nc * mc; e += G) {
+    const int i = e / mc, col = e % mc;
+    double acc = 0.0;
+    for (int o = 0; o < ko; ++o) acc += gain[i * ko + o] * mo[o * mc + col];
+    Xm[e] = Pm[e] - acc;
+  }
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int i = e / nc, j = e % nc;
+    XL[e] = big[(ko + j) * n2 + ko + i];
+  }
+  if (oL_out)
+    for (int e = c.g.l; e < ko * ko; e += G) {
+      const int i = e / ko, j = e % ko;
+      oL_out[e] = big[j * n2 + i];
+    }
+  c.g.sync();
+}
+
+// ------------------------------------------------------------- error norm
+// error_residual_std.estimate_error_norm (solvers.py:925-996) given the
+// observed marginal of the error model and its local scale sigma.
+template <int G>
+__device__ double error_power(const Ctx<G>& c, const double* oL, double sigma, const double* prev_m,
+                              const double* prop_m, double dt) {
+  const int ko = c.ko, d = c.d;
+  const pdq_config& cfg = c.a->cfg;
+  // std of rescaled observed rows: dense per-row qr_r (|beta| of dlarfg),
+  // isotropic vector_norm
+  auto err = [&](int r) {
+    double ss = 0.0;
+    for (int j = 1; j < ko; ++j) {
+      const double x = sigma * oL[r * ko + j];
+      ss += x * x;
+    }
+    const double alpha = sigma * oL[r * ko];
+    if (c.dense) return ss == 0.0 ? fabs(alpha) : fabs(hypot(alpha, sqrt(ss)));
+    return sqrt(alpha * alpha + ss);
+  };
+  const double fact = 1.0;  // factorial(n) for n = residual_order - 1 = 1
+  double norm;
+  if (cfg.error_norm == PDQ_NORM_SCALE_THEN_RMS) {
+    double ss = 0.0;
+    for (int k = 0; k < d; ++k) {
+      const double u0 = c.dense ? prev_m[k * c.mc] : prev_m[k];
+      const double u1 = c.dense ? prop_m[k * c.mc] : prop_m[k];
+      const double ref = fmax(fabs(u0), fabs(u1));
+      const double e = err(ko == 1 ? 0 : k) * pow(dt, 1.0) / fact;
+      const double rr = e / (cfg.atol + cfg.rtol * fabs(ref));
+      ss += rr * rr;
+    }
+    norm = sqrt(ss) / sqrt((double)d);
+  } else {
+    double sa = 0.0, sr = 0.0;
+    for (int r = 0; r < ko; ++r) {
+      const double e = err(r) * pow(dt, 1.0) / fact;
+      sa += e * e;
+    }
+    for (int k = 0; k < d; ++k) {
+      const double u0 = c.dense ? prev_m[k * c.mc] : prev_m[k];
+      const double u1 = c.dense ? prop_m[k * c.mc] : prop_m[k];
+      const double ref = fmax(fabs(u0), fabs(u1));
+      sr += ref * ref;
+    }
+    norm = (sqrt(sa) / sqrt((double)ko)) / (cfg.atol + cfg.rtol * (sqrt(sr) / sqrt((double)d)));
+  }
+  return pow(norm, -1.0 / (double)c.N);
+}
+
+// controllers.py:66-84 / :24-63
+__device__ __forceinline__ double control_apply(const pdq_config& cfg, double dt, double& cstate, double ep) {
+  if (cfg.control == PDQ_CONTROL_INTEGRAL) {
+    const double ratio = cfg.safety * ep;
+    return fmax(cfg.factor_min, fmin(ratio, cfg.factor_max)) * dt;
+  }
+  const double gi = pow(ep, cfg.exponent_integral);
+  const double gp = pow(ep / cstate, cfg.exponent_proportional);
+  const double ratio = cfg.safety * gi * gp;
+  const double sf = fmax(cfg.factor_min, fmin(ratio, cfg.factor_max));
+  if (ep >= 1.0) cstate = ep;
+  return sf * dt;
+}
+
+// ------------------------------------------------------------ slot access
+struct SlotV {
+  double *sc, *m, *L, *cA, *cm, *cL, *ctl, *cto, *b, *J;
+};
+template <int G>
+__device__ __forceinline__ SlotV slotv(const Ctx<G>& c, int s) {
+  double* base = c.slot(s);
+  const Layout& L = *c.L;
+  return SlotV{base + L.s_sc, base + L.s_m, base + L.s_L, base + L.s_cA, base + L.s_cm, base + L.s_cL,
+               base + L.s_ctl, base + L.s_cto, base + L.s_b, base + L.s_J};
+}
+
+template <int G>
+__device__ void copy_slot(const Ctx<G>& c, int dst, int src) {
+  g_copy(c.g, c.slot(dst), c.slot(src), c.L->s_size);
+  c.g.sync();
+}
+
+// ------------------------------------------------------------- the step
+// solver_dynamic.step (solvers.py:549-599), solver.step (:702-733) and
+// solver_mle.step (:395-437), followed by the residual error estimate.
+// Returns error_power (only meaningful with `estimate`).
+template <int G>
+__device__ double step(const Ctx<G>& c, int from, int to, double dt, bool estimate) {
+  const Layout& L = *c.L;
+  const pdq_config& cfg = c.a->cfg;
+  SlotV S = slotv(c, from), R = slotv(c, to);
+  double* P = c.W(L.w_P);
+  double* PI = c.W(L.w_PI);
+  double* big = c.W(L.w_big);
+  double* Um = c.W(L.w_Um);
+  double* UL = c.W(L.w_UL);
+  double* Pm = c.W(L.w_Pm);
+  double* PL = c.W(L.w_PL);
+  double* om = c.W(L.w_om);
+  double* oL = c.W(L.w_oL);
+  double* gain = c.W(L.w_G);
+  double* mo = c.W(L.w_mo);
+  double* tmp = c.W(L.w_T1);
+  double* y0buf = c.W(L.w_y0);
+  const double t = S.sc[SC_T];
+  const double tn = t + dt;
+  const bool smoother = cfg.strategy != PDQ_FILTER;
+  const bool ident = S.sc[SC_IDENT] != 0.0;
+  precon(c, dt, P, PI);
+  c.g.sync();
+  double sigma = 1.0, ep = 0.0;
+  if (cfg.solver == PDQ_SOLVER_DYNAMIC) {
+    const double s1 = noise_scale(c, dt, 1.0);
+    apply_flat_mean(c, P, PI, S.m, Um);
+    apply_flat_chol(c, P, s1, UL);
+    c.g.sync();
+    linearize(c, Um, tn, R.b, R.J, y0buf);
+    marginalise_fx(c, R.b, R.J, Um, UL, om, oL, big);
+    sigma = rms_zero(c, om, oL);
+    const double s = noise_scale(c, dt, sigma);
+    if (!smoother) {
+      marginalise_prior(c, P, PI, s, S.m, S.L, Pm, PL, big);
+    } else {
+      double* bA = c.W(L.w_bA);
+      double* bm = c.W(L.w_bm);
+      double* bL = c.W(L.w_bL);
+      double* btl = c.W(L.w_btl);
+      double* bto = c.W(L.w_bto);
+      revert_prior(c, P, PI, s, S.m, S.L, Pm, PL, bA, bm, bL, btl, bto, big, tmp);
+      c.g.sync();
+      if (cfg.strategy == PDQ_FIXEDPOINT) {
+        merge(c, ident, S.cA, S.cm, S.cL, S.ctl, S.cto, bA, bm, bL, btl, bto, R.cA, R.cm, R.cL, R.ctl, R.cto, big);
+      } else {
+        g_copy(c.g, R.cA, bA, c.nc * c.nc);
+        g_copy(c.g, R.cm, bm, c.nc * c.mc);
+        g_copy(c.g, R.cL, bL, c.nc * c.nc);
+        g_copy(c.g, R.ctl, btl, c.nc);
+        g_copy(c.g, R.cto, bto, c.nc);
+      }
+    }
+    c.g.sync();
+    update(c, R.b, R.J, Pm, PL, R.m, R.L, big, gain, mo, nullptr);
+    if (estimate) ep = error_power(c, oL, sigma, S.m, R.m, dt);
+  } else {
+    const double s1 = noise_scale(c, dt, 1.0);
+    if (!smoother) {
+      marginalise_prior(c, P, PI, s1, S.m, S.L, Pm, PL, big);
+    } else {
+      double* bA = c.W(L.w_bA);
+      double* bm = c.W(L.w_bm);
+      double* bL = c.W(L.w_bL);
+      double* btl = c.W(L.w_btl);
+      double* bto = c.W(L.w_bto);
+      revert_prior(c, P, PI, s1, S.m, S.L, Pm, PL, bA, bm, bL, btl, bto, big, tmp);
+      c.g.sync();
+      if (cfg.strategy == PDQ_FIXEDPOINT) {
+        merge(c, ident, S.cA, S.cm, S.cL, S.ctl, S.cto, bA, bm, bL, btl, bto, R.cA, R.cm, R.cL, R.ctl, R.cto, big);
+      } else {
+        g_copy(c.g, R.cA, bA, c.nc * c.nc);
+        g_copy(c.g, R.cm, bm, c.nc * c.mc);
+        g_copy(c.g, R.cL, bL, c.nc * c.nc);
+        g_copy(c.g, R.ctl, btl, c.nc);
+        g_copy(c.g, R.cto, bto, c.nc);
+      }
+    }
+    c.g.sync();
+    linearize(c, Pm, tn, R.b, R.J, y0buf);
+    const bool mle = cfg.solver == PDQ_SOLVER_MLE;
+    update(c, R.b, R.J, Pm, PL, R.m, R.L, big, gain, mo, mle ? oL : nullptr);
+    if (mle) {
+      // bayes_rule_and_residual_whitened_rms_tree: observed = (mo, R_Y^T)
+      const double nt = rms_zero(c, mo, oL);
+      const double nd = S.sc[SC_MLE_N];
+      const double x1 = sqrt(nd / (nd + 1)) * S.sc[SC_MLE_RUN];
+      const double x2 = sqrt(1 / (nd + 1)) * nt;
+      c.g.sync();
+      if (c.g.l == 0) {
+        R.sc[SC_MLE_RUN] = hypot(x1, x2);
+        R.sc[SC_MLE_N] = nd + 1;
+      }
+    }
+    if (estimate) {
+      apply_flat_mean(c, P, PI, S.m, Um);
+      apply_flat_chol(c, P, s1, UL);
+      c.g.sync();
+      marginalise_fx(c, R.b, R.J, Um, UL, om, oL, big);
+      const double se = rms_zero(c, om, oL);
+      ep = error_power(c, oL, se, S.m, R.m, dt);
+    }
+    sigma = mle ? S.sc[SC_OS] : 1.0;
+  }
+  c.g.sync();
+  if (c.g.l == 0) {
+    R.sc[SC_T] = tn;
+    R.sc[SC_OS] = sigma;
+    R.sc[SC_NSTEPS] = S.sc[SC_NSTEPS] + 1.0;
+    R.sc[SC_IDENT] = 0.0;
+    if (cfg.solver != PDQ_SOLVER_MLE) {
+      R.sc[SC_MLE_RUN] = 0.0;
+      R.sc[SC_MLE_N] = 0.0;
+    }
+  }
+  c.g.sync();
+  return ep;
+}
+
+// ---------------------------------------------------------------- outputs
+template <int G>
+__device__ void write_normal(const Ctx<G>& c, const double* m, const double* Lc, double* gm, double* gL) {
+  if (gm) {
+    const int mcp = c.mcp, mc = c.mc;
+    for (int e = c.g.l; e < c.nc * mcp; e += G) {
+      const int i = e / mcp, col = e % mcp;
+      gm[e] = m[i * mc + col];
+    }
+  }
+  if (gL) g_copy(c.g, gL, Lc, c.nc * c.nc);
+}
+
+template <int G>
+__device__ void write_cond(const Ctx<G>& c, const SlotV& S, int inst, int j) {
+  const pdq_solution& o = c.a->out;
+  const int nc = c.nc, Km1 = c.a->K - 1, ms = c.nc * c.mcp;
+  const size_t idx = (size_t)inst * Km1 + j;
+  const bool ident = S.sc[SC_IDENT] != 0.0;
+  for (int e = c.g.l; e < nc * nc; e += G) {
+    const int i = e / nc, k = e % nc;
+    o.cond_A[idx * nc * nc + e] = ident ? (i == k ? 1.0 : 0.0) : S.cA[e];
+    o.cond_chol[idx * nc * nc + e] = ident ? 0.0 : S.cL[e];
+  }
+  for (int e = c.g.l; e < ms; e += G) {
+    const int i = e / c.mcp, col = e % c.mcp;
+    o.cond_mean[idx * ms + e] = ident ? 0.0 : S.cm[i * c.mc + col];
+  }
+  for (int i = c.g.l; i < nc; i += G) {
+    o.cond_tl[idx * nc + i] = ident ? 1.0 : S.ctl[i];
+    o.cond_to[idx * nc + i] = ident ? 1.0 : S.cto[i];
+  }
+}
+
+// Writes checkpoint k of instance `inst` from a solution view; `u` is the
+// marginal written as u (filter) or as the filtering marginal (smoothers).
+template <int G>
+__device__ void write_checkpoint(const Ctx<G>& c, int inst, int k, double t, const double* m, const double* Lc,
+                                 double os, double nsteps, const SlotV* condsrc) {
+  const pdq_solution& o = c.a->out;
+  const int K = c.a->K, ms = c.nc * c.mcp, cs = c.nc * c.nc;
+  const bool smoother = c.a->cfg.strategy != PDQ_FILTER;
+  const size_t row = (size_t)inst * K + k;
+  if (c.g.l == 0) {
+    if (o.t) o.t[row] = t;
+    const bool dyn = c.a->cfg.solver == PDQ_SOLVER_DYNAMIC;
+    if (o.output_scale && dyn) o.output_scale[row] = os;
+    if (o.num_steps && k > 0) o.num_steps[(size_t)inst * (K - 1) + k - 1] = (int64_t)nsteps;
+  }
+  if (smoother) {
+    write_normal(c, m, Lc, o.filt_mean ? o.filt_mean + row * ms : nullptr, o.filt_chol ? o.filt_chol + row * cs : nullptr);
+    if (k > 0 && condsrc) write_cond(c, *condsrc, inst, k - 1);
+  } else {
+    write_normal(c, m, Lc, o.mean ? o.mean + row * ms : nullptr, o.chol ? o.chol + row * cs : nullptr);
+  }
+}
+
+// ------------------------------------------------------------ finalize
+// userfriendly_output + strategy.finalize (solvers.py:439-480, 601-633,
+// 735-767; estimators_and_losses.py:380-391, 437-470).
+template <int G>
+__device__ void finalize(const Ctx<G>& c, int inst, int sf) {
+  const Layout& L = *c.L;
+  const pdq_config& cfg = c.a->cfg;
+  const pdq_solution& o = c.a->out;
+  const int K = c.a->K, nc = c.nc, ms = nc * c.mcp, cs = nc * nc, Km1 = K - 1;
+  SlotV S = slotv(c, sf);
+  __threadfence_block();
+  c.g.sync();
+  double scale = 1.0;
+  const bool mle = cfg.solver == PDQ_SOLVER_MLE;
+  if (mle) {
+    scale = S.sc[SC_MLE_RUN];
+    if (cfg.correct_asymptotic_underconfidence) {
+      const double ns = (double)o.num_steps[(size_t)inst * Km1 + Km1 - 1];
+      scale = scale / sqrt(ns);
+    }
+    if (o.output_scale && c.g.l == 0)
+      for (int k = 0; k < Km1; ++k) o.output_scale[(size_t)inst * K + k] = scale;
+  } else if (cfg.solver == PDQ_SOLVER && o.output_scale && c.g.l == 0) {
+    for (int k = 0; k < Km1; ++k) o.output_scale[(size_t)inst * K + k] = 1.0;
+  }
+  if (cfg.strategy == PDQ_FILTER) {
+    if (mle && o.chol)
+      for (int e = c.g.l; e < K * cs; e += G) o.chol[(size_t)inst * K * cs + e] = scale * o.chol[(size_t)inst * K * cs + e];
+    return;
+  }
+  if (mle) {
+    for (int e = c.g.l; e < K * cs; e += G) o.filt_chol[(size_t)inst * K * cs + e] = scale * o.filt_chol[(size_t)inst * K * cs + e];
+    for (int e = c.g.l; e < Km1 * cs; e += G)
+      o.cond_chol[(size_t)inst * Km1 * cs + e] = scale * o.cond_chol[(size_t)inst * Km1 * cs + e];
+    for (int e = c.g.l; e < nc * nc; e += G) {
+      S.L[e] = scale * S.L[e];
+      S.cL[e] = scale * S.cL[e];
+    }
+    __threadfence_block();
+    c.g.sync();
+  }
+  double* big = c.W(L.w_big);
+  double* Xm = c.W(L.w_Pm);
+  double* XL = c.W(L.w_PL);
+  double* Ym = c.W(L.w_Um);
+  double* YL = c.W(L.w_UL);
+  // rv_at_t1 = posterior1.conditional.marginalise(posterior1.marginal)
+  if (S.sc[SC_IDENT] != 0.0) {
+    g_copy(c.g, Xm, S.m, nc * c.mc);
+    g_copy(c.g, XL, S.L, nc * nc);
+  } else {
+    marginalise_cond(c, S.cA, S.cm, S.cL, S.ctl, S.cto, S.m, S.L, Xm, XL, big);
+  }
+  c.g.sync();
+  const size_t rowK = (size_t)inst * K + (K - 1);
+  write_normal(c, Xm, XL, o.term_mean ? o.term_mean + (size_t)inst * ms : nullptr,
+               o.term_chol ? o.term_chol + (size_t)inst * cs : nullptr);
+  write_normal(c, Xm, XL, o.mean ? o.mean + rowK * ms : nullptr, o.chol ? o.chol + rowK * cs : nullptr);
+  // backward pass: x_j = cond_j.marginalise(x_{j+1}), j = K-2 .. 0
+  double* cA = c.W(L.w_bA);
+  double* cm = c.W(L.w_bm);
+  double* cL = c.W(L.w_bL);
+  double* ctl = c.W(L.w_btl);
+  double* cto = c.W(L.w_bto);
+  for (int j = Km1 - 1; j >= 0; --j) {
+    const size_t idx = (size_t)inst * Km1 + j;
+    __threadfence_block();
+    g_copy(c.g, cA, o.cond_A + idx * nc * nc, nc * nc);
+    g_copy(c.g, cL, o.cond_chol + idx * nc * nc, nc * nc);
+    for (int e = c.g.l; e < nc * c.mc; e += G) {
+      const int i = e / c.mc, col = e % c.mc;
+      cm[e] = col < c.mcp ? o.cond_mean[idx * ms + i * c.mcp + col] : 0.0;
+    }
+    g_copy(c.g, ctl, o.cond_tl + idx * nc, nc);
+    g_copy(c.g, cto, o.cond_to + idx * nc, nc);
+    c.g.sync();
+    marginalise_cond(c, cA, cm, cL, ctl, cto, Xm, XL, Ym, YL, big);
+    c.g.sync();
+    const size_t row = (size_t)inst * K + j;
+    write_normal(c, Ym, YL, o.mean ? o.mean + row * ms : nullptr, o.chol ? o.chol + row * cs : nullptr);
+    g_copy(c.g, Xm, Ym, nc * c.mc);
+    g_copy(c.g, XL, YL, nc * nc);
+    c.g.sync();
+  }
+}
+
+// ------------------------------------------------------ interpolations
+// ProbabilisticSolver.interpolate_fwd (solvers.py:205-269) for the three
+// strategies.  from = interp_from slot, to = step_from slot; writes the
+// checkpoint output and updates both slots.
+template <int G>
+__device__ void interp_beyond(const Ctx<G>& c, int inst, int k, double t1, int from, int to) {
+  const Layout& L = *c.L;
+  const pdq_config& cfg = c.a->cfg;
+  SlotV F = slotv(c, from), T = slotv(c, to);
+  double* P = c.W(L.w_P);
+  double* PI = c.W(L.w_PI);
+  double* big = c.W(L.w_big);
+  double* Pm = c.W(L.w_Pm);
+  double* PL = c.W(L.w_PL);
+  double* tmp = c.W(L.w_T1);
+  const double os = T.sc[SC_OS];
+  const double dt0 = t1 - F.sc[SC_T];
+  const double dt1 = T.sc[SC_T] - t1;
+  precon(c, dt0, P, PI);
+  c.g.sync();
+  const double s0 = noise_scale(c, dt0, os);
+  if (cfg.strategy == PDQ_FILTER) {
+    marginalise_prior(c, P, PI, s0, F.m, F.L, Pm, PL, big);
+    c.g.sync();
+    write_checkpoint(c, inst, k, t1, Pm, PL, os, T.sc[SC_NSTEPS], nullptr);
+    g_copy(c.g, F.m, Pm, c.nc * c.mc);
+    g_copy(c.g, F.L, PL, c.nc * c.nc);
+    if (c.g.l == 0) F.sc[SC_T] = t1;
+    c.g.sync();
+    return;
+  }
+  double* bA = c.W(L.w_bA);
+  double* bm = c.W(L.w_bm);
+  double* bL = c.W(L.w_bL);
+  double* btl = c.W(L.w_btl);
+  double* bto = c.W(L.w_bto);
+  double* eA = c.W(L.w_eA);
+  double* em = c.W(L.w_em);
+  double* eL = c.W(L.w_eL);
+  double* etl = c.W(L.w_etl);
+  double* eto = c.W(L.w_eto);
+  // extrapolate t0 -> t: ext_t (marginal Pm/PL, conditional e*)
+  revert_prior(c, P, PI, s0, F.m, F.L, Pm, PL, bA, bm, bL, btl, bto, big, tmp);
+  c.g.sync();
+  if (cfg.strategy == PDQ_FIXEDPOINT) {
+    merge(c, F.sc[SC_IDENT] != 0.0, F.cA, F.cm, F.cL, F.ctl, F.cto, bA, bm, bL, btl, bto, eA, em, eL, etl, eto, big);
+  } else {
+    g_copy(c.g, eA, bA, c.nc * c.nc);
+    g_copy(c.g, em, bm, c.nc * c.mc);
+    g_copy(c.g, eL, bL, c.nc * c.nc);
+    g_copy(c.g, etl, btl, c.nc);
+    g_copy(c.g, eto, bto, c.nc);
+  }
+  c.g.sync();
+  // extrapolate t -> t1 from (ext_t.marginal, identity) (fixed point) or
+  // from ext_t (fixed interval); only the backward conditional is kept.
+  precon(c, dt1, P, PI);
+  c.g.sync();
+  const double s1 = noise_scale(c, dt1, os);
+  double* Ym = c.W(L.w_Um);
+  double* YL = c.W(L.w_UL);
+  revert_prior(c, P, PI, s1, Pm, PL, Ym, YL, bA, bm, bL, btl, bto, big, tmp);
+  c.g.sync();
+  // new step_from.cond = (fixed point) identity o bw, (fixed interval) bw
+  if (cfg.strategy == PDQ_FIXEDPOINT) {
+    merge(c, true, nullptr, nullptr, nullptr, nullptr, nullptr, bA, bm, bL, btl, bto, T.cA, T.cm, T.cL, T.ctl, T.cto, big);
+  } else {
+    g_copy(c.g, T.cA, bA, c.nc * c.nc);
+    g_copy(c.g, T.cm, bm, c.nc * c.mc);
+    g_copy(c.g, T.cL, bL, c.nc * c.nc);
+    g_copy(c.g, T.ctl, btl, c.nc);
+    g_copy(c.g, T.cto, bto, c.nc);
+  }
+  // output: interpolated = (ext_t.marginal, ext_t.conditional)
+  SlotV E{nullptr, nullptr, nullptr, eA, em, eL, etl, eto, nullptr, nullptr};
+  double esc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  E.sc = esc;
+  c.g.sync();
+  write_checkpoint(c, inst, k, t1, Pm, PL, os, T.sc[SC_NSTEPS], &E);
+  // new interp_from: fixed point (ext_t.marginal, identity); fixed interval ext_t
+  g_copy(c.g, F.m, Pm, c.nc * c.mc);
+  g_copy(c.g, F.L, PL, c.nc * c.nc);
+  if (cfg.strategy == PDQ_FIXEDINTERVAL) {
+    g_copy(c.g, F.cA, eA, c.nc * c.nc);
+    g_copy(c.g, F.cm, em, c.nc * c.mc);
+    g_copy(c.g, F.cL, eL, c.nc * c.nc);
+    g_copy(c.g, F.ctl, etl, c.nc);
+    g_copy(c.g, F.cto, eto, c.nc);
+  }
+  c.g.sync();
+  if (c.g.l == 0) {
+    F.sc[SC_T] = t1;
+    F.sc[SC_IDENT] = cfg.strategy == PDQ_FIXEDPOINT ? 1.0 : 0.0;
+    T.sc[SC_IDENT] = 0.0;
+  }
+  c.g.sync();
+}
+
+// ProbabilisticSolver.interpolate_fwd_at_t1 (solvers.py:271-315).
+template <int G>
+__device__ void interp_at(const Ctx<G>& c, int inst, int k, int from, int to) {
+  const pdq_config& cfg = c.a->cfg;
+  SlotV T = slotv(c, to);
+  write_checkpoint(c, inst, k, T.sc[SC_T], T.m, T.L, T.sc[SC_OS], T.sc[SC_NSTEPS], &T);
+  c.g.sync();
+  // new interp_from: posterior_t1 (filter / fixed interval) or the reset one
+  copy_slot(c, from, to);
+  if (cfg.strategy == PDQ_FIXEDPOINT) {
+    SlotV F = slotv(c, from);
+    if (c.g.l == 0) {
+      T.sc[SC_IDENT] = 1.0;
+      F.sc[SC_IDENT] = 1.0;
+    }
+  }
+  c.g.sync();
+}
+
+
+// ------------------------------------------------------------ LML + grad
+// Marginal log-likelihood of loss_lml_timeseries (estimators_and_losses.py:
+// 53-105) on the fixed-point smoother's Markov sequence: forward pass on a
+// fixed grid with a checkpoint (interpolate_fwd_at_t1 reset, :534-547) every
+// `every` grid points, then MarkovSequence.evaluate_lml's reverse scan
+// (:180-218) with lstsq gains.  Forward-mode tangents (d/d params, d/d y0)
+// ride along as extra mean columns: with ts0 the covariance recursion does
+// not depend on them, so every square-root factor is shared and only the
+// mean recursions and the log-densities carry derivatives.
+template <int G>
+__device__ void lml_instance(const Ctx<G>& c, int inst) {
+  const pdq_config& cfg = c.a->cfg;
+  const KArgs& a = *c.a;
+  const Layout& L = *c.L;
+  const int N = c.N, d = c.d, nc = c.nc, mc = c.mc, mcp = c.mcp, ko = c.ko;
+  const int ntan = mc / mcp - 1;
+  const int np = cfg.n_params;
+  const int K = a.K, every = a.every, Kobs = (K - 1) / every + 1;
+  for (int i = c.g.l; i < np; i += G) c.prm[i] = a.params[(size_t)inst * np + i];
+  int sf = 0, pr = 2;
+  g_fill(c.g, c.slot(sf), 0.0, L.s_size);
+  c.g.sync();
+  SlotV S0 = slotv(c, sf);
+  // prior mean: Taylor coefficients of y0 with their tangents, computed by
+  // taylor_tangent_kernel (layout n_inst x (N d) x (1 + ntan))
+  const double* tc = a.tcoeffs + (size_t)inst * N * d * (1 + ntan);
+  for (int e = c.g.l; e < N * d * (1 + ntan); e += G) {
+    const int ik = e / (1 + ntan), q = e % (1 + ntan);
+    const int i = ik / d, k = ik % d;
+    if (c.dense) S0.m[(i * d + k) * mc + q] = tc[e];
+    else S0.m[i * mc + q * d + k] = tc[e];
+  }
+  for (int i = c.g.l; i < nc; i += G) S0.L[i * nc + i] = cfg.init_std;
+  if (c.g.l == 0) {
+    S0.sc[SC_T] = a.times[0];
+    S0.sc[SC_OS] = 1.0;
+    S0.sc[SC_IDENT] = 1.0;
+  }
+  c.g.sync();
+  const int csz = 2 * nc * nc + nc * mc + 2 * nc;
+  double* store = a.scratch + (size_t)inst * (Kobs - 1) * csz;
+  for (int k = 1; k < K; ++k) {
+    step(c, sf, pr, a.times[k] - a.times[k - 1], false);
+    const int t = sf;
+    sf = pr;
+    pr = t;
+    if (k % every == 0) {
+      SlotV S = slotv(c, sf);
+      double* dst = store + (size_t)(k / every - 1) * csz;
+      g_copy(c.g, dst, S.cA, nc * nc);
+      g_copy(c.g, dst + nc * nc, S.cm, nc * mc);
+      g_copy(c.g, dst + nc * nc + nc * mc, S.cL, nc * nc);
+      g_copy(c.g, dst + 2 * nc * nc + nc * mc, S.ctl, nc);
+      g_copy(c.g, dst + 2 * nc * nc + nc * mc + nc, S.cto, nc);
+      c.g.sync();
+      if (c.g.l == 0) S.sc[SC_IDENT] = 1.0;
+      c.g.sync();
+    }
+  }
+  __threadfence_block();
+  // terminal marginal: posterior1.conditional.marginalise(posterior1.marginal)
+  double* big = c.W(L.w_big);
+  double* Xm = c.W(L.w_Pm);
+  double* XL = c.W(L.w_PL);
+  double* Ym = c.W(L.w_Um);
+  double* YL = c.W(L.w_UL);
+  SlotV S = slotv(c, sf);
+  if (S.sc[SC_IDENT] != 0.0) {
+    g_copy(c.g, Xm, S.m, nc * mc);
+    g_copy(c.g, XL, S.L, nc * nc);
+  } else {
+    marginalise_cond(c, S.cA, S.cm, S.cL, S.ctl, S.cto, S.m, S.L, Xm, XL, big);
+  }
+  c.g.sync();
+  double* gain = c.W(L.w_G);
+  double* mo = c.W(L.w_mo);
+  double* lp = c.W(L.w_lml);          // 1 + ntan running log-densities
+  double* lpb = lp + (1 + ntan);       // this node's terms
+  double* w = lpb + (1 + ntan);        // ko whitened residuals
+  double* wt = w + ko;                 // ko tangents of them
+  if (c.g.l == 0)
+    for (int q = 0; q <= ntan; ++q) lp[q] = 0.0;
+  double num = 0.0;
+  const double c2pi = log(M_PI * 2.0);
+  for (int j = Kobs - 1; j >= 0; --j) {
+    if (j < Kobs - 1) {
+      const double* src = store + (size_t)j * csz;
+      marginalise_cond(c, src, src + nc * nc, src + nc * nc + nc * mc, src + 2 * nc * nc + nc * mc,
+                       src + 2 * nc * nc + nc * mc + nc, Xm, XL, Ym, YL, big);
+      c.g.sync();
+      g_copy(c.g, Xm, Ym, nc * mc);
+      g_copy(c.g, XL, YL, nc * nc);
+      c.g.sync();
+    }
+    // model_j = marginal.to_derivative(0, std_j); bayes_rule_and_logpdf_tree
+    const double* y = a.data + ((size_t)inst * Kobs + j) * d;
+    const double* sd = a.stds + ((size_t)inst * Kobs + j) * a.n_std;
+    const int n2 = ko + nc;
+    for (int e = c.g.l; e < n2 * n2; e += G) {
+      const int r = e / n2, col = e % n2;
+      double v;
+      if (r < ko) v = col < ko ? (r == col ? sd[a.n_std == 1 ? 0 : r] : 0.0) : 0.0;
+      else v = col < ko ? XL[col * nc + (r - ko)] : XL[(col - ko) * nc + (r - ko)];
+      big[e] = v;
+    }
+    for (int e = c.g.l; e < ko * mc; e += G) {
+      const int o = e / mc, col = e % mc;
+      mo[e] = Xm[o * mc + col] + 0.0;
+    }
+    c.g.sync();
+    g_qr_r(c.g, big, n2, n2, n2);
+    g_gain(c.g, big, n2, ko, nc, gain, 1);
+    c.g.sync();
+    if (c.g.l == 0) {
+      // observed.logpdf (ssm_impl_dense.py:170-180 / isotropic :217-236);
+      // qr_r of the already-triangular R_Y leaves it unchanged.
+      for (int q = 0; q <= ntan; ++q) lpb[q] = 0.0;
+      double slogdet = 0.0;
+      for (int i = 0; i < ko; ++i) slogdet += log(fabs(big[i * n2 + i]));
+      for (int col = 0; col < mcp; ++col) {
+        for (int i = 0; i < ko; ++i) {
+          const double yi = c.dense ? y[i] : y[col];
+          double s = yi - mo[i * mc + col];
+          for (int k = 0; k < i; ++k) s -= big[k * n2 + i] * w[k];
+          w[i] = s / big[i * n2 + i];
+        }
+        double sq = 0.0;
+        for (int i = 0; i < ko; ++i) sq += w[i] * w[i];
+        if (c.dense) lpb[0] += -1.0 / 2.0 * sq - (double)ko / 2.0 * c2pi - slogdet;
+        else lpb[0] += -0.5 * (2.0 * slogdet + sq + (double)ko * c2pi);
+        for (int q = 1; q <= ntan; ++q) {
+          double dot = 0.0;
+          for (int i = 0; i < ko; ++i) {
+            double s = -mo[i * mc + q * mcp + col];
+            for (int k = 0; k < i; ++k) s -= big[k * n2 + i] * wt[k];
+            wt[i] = s / big[i * n2 + i];
+            dot += w[i] * wt[i];
+          }
+          lpb[q] += -dot;
+        }
+      }
+      for (int q = 0; q <= ntan; ++q) {
+        const double v = lpb[q];
+        if (num == 0.0) lp[q] = v;
+        else if (a.average_pdfs) lp[q] = (lp[q] * num + v) / (num + 1);
+        else lp[q] = lp[q] + v;
+      }
+    }
+    c.g.sync();
+    num += 1.0;
+    // updated = reverted.apply_flat(data)
+    for (int e = c.g.l; e < nc * mc; e += G) {
+      const int i = e / mc, col = e % mc;
+      double gy = 0.0, gm = 0.0;
+      for (int o = 0; o < ko; ++o) {
+        const double yo = col < mcp ? (c.dense ? y[o] : y[col]) : 0.0;
+        gy += gain[i * ko + o] * yo;
+        gm += gain[i * ko + o] * mo[o * mc + col];
+      }
+      Ym[e] = c.dense ? 1.0 * (gy + (Xm[e] - gm)) : gy + (Xm[e] - gm);
+    }
+    for (int e = c.g.l; e < nc * nc; e += G) {
+      const int i = e / nc, k = e % nc;
+      YL[e] = big[(ko + k) * n2 + ko + i];
+    }
+    c.g.sync();
+    g_copy(c.g, Xm, Ym, nc * mc);
+    g_copy(c.g, XL, YL, nc * nc);
+    c.g.sync();
+  }
+  if (c.g.l == 0) {
+    if (a.lml_value) a.lml_value[inst] = lp[0];
+    if (a.lml_grad)
+      for (int q = 0; q < ntan; ++q) a.lml_grad[(size_t)inst * ntan + q] = lp[1 + q];
+  }
+}
+
+// --------------------------------------------------------------- kernel
+template <int G>
+__device__ void init_instance(const Ctx<G>& c, int inst, int slot0) {
+  const pdq_config& cfg = c.a->cfg;
+  SlotV S = slotv(c, slot0);
+  const int N = c.N, d = c.d, nc = c.nc, mc = c.mc;
+  g_fill(c.g, c.slot(slot0), 0.0, c.L->s_size);
+  c.g.sync();
+  // prior init: mean = tcoeffs, chol = init_std * I (from_mean_and_std)
+  const double* tc = c.a->tcoeffs + (size_t)inst * N * d;
+  for (int e = c.g.l; e < N * d; e += G) {
+    const int i = e / d, k = e % d;
+    if (c.dense) S.m[(i * d + k) * mc] = tc[e];
+    else S.m[i * mc + k] = tc[e];
+  }
+  for (int i = c.g.l; i < nc; i += G) S.L[i * nc + i] = cfg.init_std;
+  if (c.g.l == 0) {
+    S.sc[SC_T] = c.a->times[(size_t)inst * c.a->K];
+    S.sc[SC_OS] = 1.0;
+    S.sc[SC_NSTEPS] = 0.0;
+    S.sc[SC_IDENT] = cfg.strategy != PDQ_FILTER ? 1.0 : 0.0;
+  }
+  c.g.sync();
+}
+
+template <int G>
+__device__ void solve_instance(const Ctx<G>& c, int inst) {
+  const pdq_config& cfg = c.a->cfg;
+  const pdq_solution& o = c.a->out;
+  const int K = c.a->K;
+  const double* ts = c.a->times + (size_t)inst * K;
+  // vector-field parameters
+  for (int i = c.g.l; i < cfg.n_params; i += G) c.prm[i] = c.a->params[(size_t)inst * cfg.n_params + i];
+  int sf = 0, ifr = 1, pr = 2;
+  init_instance(c, inst, sf);
+  copy_slot(c, ifr, sf);
+  {
+    SlotV S = slotv(c, sf);
+    write_checkpoint(c, inst, 0, S.sc[SC_T], S.m, S.L, 1.0, 0.0, nullptr);
+  }
+  long attempts = 0, accepted = 0;
+  int status = PDQ_STATUS_OK;
+  if (c.a->mode == MODE_FIXED_GRID) {
+    for (int k = 1; k < K && status == PDQ_STATUS_OK; ++k) {
+      const double dt = ts[k] - ts[k - 1];
+      step(c, sf, pr, dt, false);
+      ++attempts;
+      ++accepted;
+      SlotV R = slotv(c, pr);
+      write_checkpoint(c, inst, k, R.sc[SC_T], R.m, R.L, R.sc[SC_OS], R.sc[SC_NSTEPS], &R);
+      c.g.sync();
+      const int tmp = sf;
+      sf = pr;
+      pr = tmp;
+    }
+  } else {
+    double dt = cfg.dt0, cstate = 1.0;
+    for (int k = 1; k < K && status == PDQ_STATUS_OK; ++k) {
+      const double t1 = ts[k];
+      bool cont = true;
+      while (cont) {
+        if (slotv(c, sf).sc[SC_T] + cfg.eps < t1) {
+          double ep = 0.9;
+          int att = 0;
+          while (ep < 1.0) {
+            double dtc = dt;
+            if (cfg.clip_dt) dtc = fmin(dtc, t1 - slotv(c, sf).sc[SC_T]);
+            ep = step(c, sf, pr, dtc, true);
+            dt = control_apply(cfg, dtc, cstate, ep);
+            ++att;
+            ++attempts;
+            if (!isfinite(ep) || !isfinite(dt)) {
+              status = PDQ_STATUS_NONFINITE;
+              break;
+            }
+            if (ep < 1.0 && (dt == 0.0 || slotv(c, sf).sc[SC_T] + dt == slotv(c, sf).sc[SC_T])) {
+              status = PDQ_STATUS_DT_UNDERFLOW;
+              break;
+            }
+            if (ep >= 1.0 && slotv(c, pr).sc[SC_T] == slotv(c, sf).sc[SC_T]) {
+              status = PDQ_STATUS_DT_UNDERFLOW;  // accepted without advancing t
+              break;
+            }
+            if ((cfg.max_attempts > 0 && att >= cfg.max_attempts && ep < 1.0) || attempts >= kMaxTotalAttempts) {
+              status = PDQ_STATUS_MAX_ATTEMPTS;
+              break;
+            }
+          }
+          if (status != PDQ_STATUS_OK) break;
+          ++accepted;
+          // interp_from <- step_from, step_from <- proposed
+          const int old_if = ifr;
+          ifr = sf;
+          sf = pr;
+          pr = old_if;
+        }
+        const double tsf = slotv(c, sf).sc[SC_T];
+        if (tsf + cfg.eps < t1) {
+          cont = true;  // interp_skip
+        } else if (tsf > t1 + cfg.eps) {
+          interp_beyond(c, inst, k, t1, ifr, sf);
+          cont = slotv(c, sf).sc[SC_T] + cfg.eps < t1;
+        } else {
+          interp_at(c, inst, k, ifr, sf);
+          cont = false;
+        }
+      }
+    }
+  }
+  if (status == PDQ_STATUS_OK) finalize(c, inst, sf);
+  if (c.g.l == 0) {
+    if (o.attempts) o.attempts[inst] = attempts;
+    if (o.accepted) o.accepted[inst] = accepted;
+    if (o.status) o.status[inst] = status;
+  }
+}
+
+#ifndef PDQ_HOST_EMU
+template <int G>
+__global__ void __launch_bounds__(64) solve_kernel(KArgs args) {
+  extern __shared__ double lds[];
+  const KArgs& a = args;
+  const Layout& L = a.lay;
+  const int nc = L.nc;
+  // shared (per workgroup) IWP matrices
+  double* sA = lds;
+  double* sQ = lds + nc * nc;
+  for (int i = threadIdx.x; i < nc * nc; i += blockDim.x) {
+    sA[i] = a.A[i];
+    sQ[i] = a.Q[i];
+  }
+  __syncthreads();
+  const int grp = threadIdx.x / G;
+  Ctx<G> c;
+  c.g.l = threadIdx.x % G;
+  c.a = &a;
+  c.L = &L;
+  c.nc = L.nc;
+  c.mc = L.mc;
+  c.mcp = L.mcp;
+  c.ko = L.ko;
+  c.N = L.N;
+  c.d = L.d;
+  c.dense = a.cfg.factorisation == PDQ_DENSE;
+  c.sA = sA;
+  c.sQ = sQ;
+  c.ws = lds + L.wg_shared + grp * L.inst_size;
+  c.prm = c.ws + L.w_prm;
+  for (;;) {
+    int inst = 0;
+    if (c.g.l == 0) inst = atomicAdd(a.queue, 1);
+    inst = c.g.bcast(inst);
+    if (inst >= a.n_inst) break;
+    if (a.mode == MODE_LML) lml_instance(c, inst);
+    else solve_instance(c, inst);
+  }
+}
+
 
 // Initial Taylor coefficients with forward-mode tangents d/d params and
 // d/d y0 (jet_expansion_algorithms.py:49-103 for the registry); one thread
@@ -36,6 +1312,10 @@ __global__ void taylor_tangent_kernel(int vf, int d, int np, int n_inst, int num
   }
 }
 
+__global__ void zero_queue(int* p) {
+  if (threadIdx.x == 0) p[0] = 0;
+}
+
 int launch_taylor_tangents(int vf, int d, int np, int n_inst, int num, int ntan, const double* params,
                            const double* y0, const double* t0, double* out, hipStream_t st) {
   hipLaunchKernelGGL(taylor_tangent_kernel, dim3((n_inst + 63) / 64), dim3(64), 0, st, vf, d, np, n_inst, num, ntan,
@@ -44,30 +1324,65 @@ int launch_taylor_tangents(int vf, int d, int np, int n_inst, int num, int ntan,
   return PDQ_OK;
 }
 
-int launch_coop_fixed(KArgs& a, int G, hipStream_t st) {
-  const Layout& L = a.lay;
-  const int dense = a.cfg.factorisation == PDQ_DENSE;
-#define PDQ_MATCH_SHAPE(name, NC, MC, MCP, KO, N_, D_, DENSE_)                                          \
-  if (L.nc == NC && L.mc == MC && L.mcp == MCP && L.ko == KO && L.N == N_ && L.d == D_ && dense == DENSE_) \
-    return launch_coop_##name(a, G, st);
-  PDQ_FIXED_SHAPES(PDQ_MATCH_SHAPE)
-#undef PDQ_MATCH_SHAPE
-  return kNoFixedShape;
-}
-
 size_t coop_lds_bytes(const Layout& L, int G) { return sizeof(double) * (size_t)(L.wg_shared + (64 / G) * L.inst_size); }
 
+template <int G>
+int launch_coop_g(KArgs& a, hipStream_t st) {
+  const size_t lds = coop_lds_bytes(a.lay, G);
+  auto kern = solve_kernel<G>;
+  PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per_cu = 0;
+  PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
+  if (per_cu < 1) return host_error(PDQ_ERR_UNSUPPORTED, "instance does not fit into LDS (%zu bytes)", lds);
+  const int ipw = 64 / G;
+  const long need = ((long)a.n_inst + ipw - 1) / ipw;
+  const long cap = (long)per_cu * num_cus();
+  const int grid = (int)std::max<long>(1, std::min(need, cap));
+  hipLaunchKernelGGL(zero_queue, dim3(1), dim3(64), 0, st, a.queue);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, a);
+  PDQ_HIP_TRY(hipGetLastError());
+  return PDQ_OK;
+}
+
 int launch_coop(KArgs& a, int G, hipStream_t st) {
-  const int rc = launch_coop_fixed(a, G, st);
-  if (rc != kNoFixedShape) return rc;
   switch (G) {
-    case 4: return launch_coop_g<4, DynDims>(a, st);
-    case 8: return launch_coop_g<8, DynDims>(a, st);
-    case 16: return launch_coop_g<16, DynDims>(a, st);
-    case 32: return launch_coop_g<32, DynDims>(a, st);
-    case 64: return launch_coop_g<64, DynDims>(a, st);
+    case 4: return launch_coop_g<4>(a, st);
+    case 8: return launch_coop_g<8>(a, st);
+    case 16: return launch_coop_g<16>(a, st);
+    case 32: return launch_coop_g<32>(a, st);
+    case 64: return launch_coop_g<64>(a, st);
   }
   return host_error(PDQ_ERR_INVALID, "lanes_per_instance must be one of 4, 8, 16, 32, 64 (got %d)", G);
 }
 
+
+#else
+// CPU build of one instance (G = 1); ws must hold L.inst_size doubles and
+// shared must hold L.wg_shared doubles.
+inline void emu_solve_instance(const KArgs& a, int inst, double* shared, double* ws) {
+  const Layout& L = a.lay;
+  const int nc = L.nc;
+  for (int i = 0; i < nc * nc; ++i) {
+    shared[i] = a.A[i];
+    shared[nc * nc + i] = a.Q[i];
+  }
+  Ctx<1> c;
+  c.g.l = 0;
+  c.a = &a;
+  c.L = &L;
+  c.nc = L.nc;
+  c.mc = L.mc;
+  c.mcp = L.mcp;
+  c.ko = L.ko;
+  c.N = L.N;
+  c.d = L.d;
+  c.dense = a.cfg.factorisation == PDQ_DENSE;
+  c.sA = shared;
+  c.sQ = shared + nc * nc;
+  c.ws = ws;
+  c.prm = ws + L.w_prm;
+  if (a.mode == MODE_LML) lml_instance(c, inst);
+  else solve_instance(c, inst);
+}
+#endif
 }  // namespace pdq

@@ -6,30 +6,13 @@
 #include <vector>
 
 #include "../../probdiffeq_amd/csrc/pdq_blockdiag.hip"
-#include "../../probdiffeq_amd/csrc/pdq_coop.h"
+#include "../../probdiffeq_amd/csrc/pdq_coop.hip"
 #include "../../probdiffeq_amd/csrc/pdq_iwp.h"
 
 namespace pdq {
 int host_error(int code, const char*, ...) { return code; }
 int num_cus() { return 1; }
 }  // namespace pdq
-
-// Runs every instance with the compile-time-shape code when the layout is one
-// of the shapes the device library instantiates (pdq_shapes.h), otherwise
-// with the run-time-shape code: the same selection as the device dispatcher.
-static void run_instances(const pdq::KArgs& a, int n_inst, double* shared, double* ws) {
-  const pdq::Layout& L = a.lay;
-  const int dense = a.cfg.factorisation == PDQ_DENSE;
-#define PDQ_EMU_MATCH(name, NC, MC, MCP, KO, N_, D_, DENSE_)                                             \
-  if (L.nc == NC && L.mc == MC && L.mcp == MCP && L.ko == KO && L.N == N_ && L.d == D_ && dense == DENSE_) { \
-    for (int i = 0; i < n_inst; ++i)                                                                      \
-      pdq::emu_solve_instance<pdq::FixDims<NC, MC, MCP, KO, N_, D_, DENSE_>>(a, i, shared, ws);          \
-    return;                                                                                               \
-  }
-  PDQ_FIXED_SHAPES(PDQ_EMU_MATCH)
-#undef PDQ_EMU_MATCH
-  for (int i = 0; i < n_inst; ++i) pdq::emu_solve_instance(a, i, shared, ws);
-}
 
 extern "C" {
 
@@ -89,7 +72,7 @@ int pdqemu_solve(const pdq_config* cfg, int n_inst, const double* params, const 
   a.out = *out;
   a.queue = &queue;
   std::vector<double> shared(L.wg_shared), ws(L.inst_size);
-  run_instances(a, n_inst, shared.data(), ws.data());
+  for (int i = 0; i < n_inst; ++i) pdq::emu_solve_instance(a, i, shared.data(), ws.data());
   return 0;
 }
 
@@ -152,7 +135,7 @@ int pdqemu_lml(const pdq_config* cfg, int n_inst, const double* params, const do
   a.scratch = scratch.data();
   a.queue = &queue;
   std::vector<double> shared(L.wg_shared), ws(L.inst_size);
-  run_instances(a, n_inst, shared.data(), ws.data());
+  for (int i = 0; i < n_inst; ++i) pdq::emu_solve_instance(a, i, shared.data(), ws.data());
   return 0;
 }
 
