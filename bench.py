@@ -30,6 +30,12 @@ sys.path.insert(0, ROOT)
 METRIC = "IVP solves/s and log-lik grads/s at rtol 1e-6 FP64: batched/adjoint HIP vs plain"
 THETA0 = np.array([0.5, 0.05, 0.5, 0.05])
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and dense matrix)
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# HBM bytes per C2 launch of the solve kernel, from rocprofv3 PMC runs
+# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, scales from a 1 GiB copy); see
+# profiles/solve_kernel_c2_isotropic_nu4_4096.md.  Re-measure when the kernel changes.
+C2_MEASURED_TRAFFIC_BYTES = 3_740_516 * 1024
+C2_TRAFFIC_SOURCE = "profiles/solve_kernel_c2_isotropic_nu4_4096.md"
 
 
 def c2_inputs(n_inst=4096, K=200, seed=0):
@@ -114,6 +120,7 @@ def bench_c2(args, torch, capi, flops):
         "attempts_min_max": [int(attempts.min()), int(attempts.max())],
         "flops_per_launch": fl,
         "achieved_tflops": fl / (ms_step / 1e3) / 1e12,
+        "algorithmic_bytes": flops.solve_bytes(cfg, P.shape[0], T.shape[1]),
     }
     # plain drop-in: one solve per launch, instances one after another
     n_plain = min(args.plain_sample, P.shape[0])
@@ -263,8 +270,15 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": c2["achieved_tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": C2_MEASURED_TRAFFIC_BYTES if args.instances == 4096 else None,
+            "traffic_source": C2_TRAFFIC_SOURCE,
             "flops_per_launch": c2["flops_per_launch"],
+            "hbm": {
+                "algorithmic_bytes": c2["algorithmic_bytes"],
+                "achieved_gbs": c2["algorithmic_bytes"] / (c2["ms_per_step"] / 1e3) / 1e9,
+                "peak_gbs": HBM_PEAK_GBS,
+                "frac": c2["algorithmic_bytes"] / (c2["ms_per_step"] / 1e3) / 1e9 / HBM_PEAK_GBS,
+            },
         },
         "cpu_baseline": cpu,
         "plain_solves_per_s": c2["plain_solves_per_s"],

@@ -106,3 +106,20 @@ def lml_forward(cfg, grid_steps: int, observations: int) -> float:
     step = attempt(cfg) - (gemm(nc, nc, mc) + _marginalise(nc, mc, ko) + trsm(ko, mc))  # no error estimate
     lml = observations * (_marginalise(nc, mc, nc) + _revert(nc, mc, ko) + qr(ko, ko) + trsm(ko, mc))
     return nb * (grid_steps * step) + nb * lml
+
+
+def solve_bytes(cfg, n_inst: int, K: int) -> float:
+    """Algorithmic HBM bytes of one batched save_at solve: inputs, outputs written
+    once, and the stored backward conditionals read back by the smoother's backward pass."""
+    nc, mc, _, nb = dims(cfg)
+    N, d = cfg.num_derivatives + 1, cfg.ode_dim
+    ms = N * d                       # mean entries
+    cs = nb * nc * nc                # factor entries
+    inputs = cfg.n_params + N * d + K
+    per_ckpt = 1 + ms + cs + nb + 1  # t, mean, chol, output_scale, num_steps
+    reread = 0
+    if cfg.strategy != "filter":
+        cond = nb * nc * nc + ms + cs + 2 * nb * nc  # A, noise mean, noise chol, to_latent, to_observed
+        per_ckpt += ms + cs + cond                   # filtering marginal + backward conditional
+        reread = (K - 1) * cond
+    return 8.0 * n_inst * (inputs + K * per_ckpt + reread)

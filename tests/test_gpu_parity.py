@@ -23,8 +23,8 @@ Cholesky factor of a Hilbert-type matrix; on stiff van der Pol with ts1 the
 calibration's whitened residual is at cancellation level); there the GPU
 result must be as accurate as the FP64 reference within ACCURACY_FACTOR.
 Measured over the parity suite, the median of rel(gpu, truth) / e_ref is
-about 1.  Covariances are compared as L L^T; factor signs (LAPACK
-convention) are checked on the dense filter solve.  With
+about 0.8.  Covariances are compared as L L^T, and the dense filter solve
+also compares the factors themselves, up to column signs.  With
 PDQ_PARITY_REPORT=<path>, every comparison is written there as JSON.
 """
 
@@ -171,7 +171,17 @@ def _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=False, check_facto
         got = {k: g[k][i] for k in ("mean", "chol", "output_scale", "filt_mean", "filt_chol", "cond_A", "cond_mean", "term_mean") if k in g}
         _check_fields(cfg, _fields(cfg, got), r, rf, rt, f"{cfg.factorisation}/{cfg.strategy}/{cfg.solver} instance {i}")
         if check_factor_signs:
-            assert np.all(np.sign(np.diagonal(g["chol"][i].reshape(-1, n, n), axis1=1, axis2=2)) == np.sign(np.diagonal(r["chol"].reshape(-1, n, n), axis1=1, axis2=2)))
+            # The R-only QR's sign convention ties each column's sign to an
+            # element that can be at cancellation level: the strict and FMA
+            # oracle builds already disagree on 10% of C1's diagonal signs.
+            # Factors are compared up to column signs (positive diagonal).
+            def unsigned(ch):
+                L = np.asarray(ch, dtype=np.float64).reshape(-1, n, n)
+                s = np.sign(np.diagonal(L, axis1=1, axis2=2))
+                s[s == 0] = 1.0
+                return L * s[:, None, :]
+
+            check_accuracy(unsigned(g["chol"][i]), unsigned(r["chol"]), unsigned(rf["chol"]), unsigned(rt["chol"]), "factor up to column signs")
     return g
 
 
