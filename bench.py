@@ -124,7 +124,7 @@ def bench_c2(args, torch, capi, flops):
     }
     # plain drop-in: one solve per launch, instances one after another
     n_plain = min(args.plain_sample, P.shape[0])
-    cfg1 = capi.Config(**{**cfg.__dict__, "lanes_per_instance": 64})
+    cfg1 = capi.Config(**{**cfg.__dict__, "lanes_per_instance": args.plain_lanes})
     outs1 = capi.alloc_solution(cfg1, 1, T.shape[1], dev)
     capi.solve(cfg1, P[:1].contiguous(), tc[:1].contiguous(), T[:1].contiguous(), out=outs1)
     torch.cuda.synchronize()
@@ -136,6 +136,7 @@ def bench_c2(args, torch, capi, flops):
     torch.cuda.synchronize()
     plain_ms = e0.elapsed_time(e1)
     res["plain_solves_per_s"] = n_plain / (plain_ms / 1e3)
+    res["plain_lanes"] = args.plain_lanes
     res["plain_sample"] = n_plain
     return res, cfg, params, y0, save_at
 
@@ -217,6 +218,7 @@ def main():
     ap.add_argument("--lml-instances", type=int, default=1024)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--plain-lanes", type=int, default=0, help="lanes per instance of the plain path (0: the library's choice)")
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--skip-lml", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")

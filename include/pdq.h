@@ -82,7 +82,8 @@ typedef struct pdq_config {
   double base_output_scale;   /* prior_wiener_integrated(output_scale=...) */
   double init_std;            /* 0: is_exact=True; else inexact_eps */
   int32_t correct_asymptotic_underconfidence; /* solver_mle */
-  int32_t lanes_per_instance; /* 0: autotuned; else 1..64 (power of two) */
+  int32_t lanes_per_instance; /* 0: automatic; 1: one lane per instance (isotropic, nu = 3 or 4, d = 2);
+                                 4, 8, 16, 32, 64: cooperative kernel with that many lanes per instance */
 } pdq_config;
 
 /* Caller-allocated device outputs; NULL members are not written.

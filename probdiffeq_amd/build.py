@@ -1,7 +1,8 @@
 """Compile the gfx950 HIP library (libpdq.so) in-tree with hipcc.
 
-The translation units (C ABI, cooperative dense/isotropic kernel) compile in
-parallel and link into one shared library.  The per-block blockdiag kernel
+The translation units (C ABI, cooperative dense/isotropic kernel,
+lane-per-instance isotropic kernel) compile in parallel and link into one
+shared library.  The per-block blockdiag kernel
 (csrc/pdq_blockdiag.hip) is not part of the device library: it did not finish
 on its first device run and stays off the device until that is explained; its
 code is still compiled for the CPU by the kernel-emulation tests.
@@ -17,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libpdq.so")
-UNITS = ["pdq_capi.hip", "pdq_coop.hip"]
+UNITS = ["pdq_capi.hip", "pdq_coop.hip", "pdq_lane.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

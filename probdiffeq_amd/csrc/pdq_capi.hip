@@ -173,6 +173,9 @@ int heuristic_lanes(const pdq::Layout& L, int n_inst) {
   return G;
 }
 
+// Lanes per instance: 1 = the lane-per-instance isotropic kernel (the
+// default where it is instantiated, save_at and fixed-grid solves), 4..64 =
+// the cooperative kernel with that group size.
 int pick_lanes(const pdq_config& cfg, const pdq::Layout& L, int n_inst, int mode) {
   if (cfg.lanes_per_instance > 0) return cfg.lanes_per_instance;
   {
@@ -180,6 +183,8 @@ int pick_lanes(const pdq_config& cfg, const pdq::Layout& L, int n_inst, int mode
     auto it = g_tuned.find(tune_key(cfg, n_inst, mode));
     if (it != g_tuned.end()) return it->second;
   }
+  const bool lane_ok = cfg.factorisation == PDQ_ISOTROPIC && mode != pdq::MODE_LML && pdq::lane_supported(L.N, L.d);
+  if (lane_ok) return 1;
   return heuristic_lanes(L, n_inst);
 }
 
@@ -198,25 +203,43 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
   if (!tcoeffs || !times || (cfg->n_params > 0 && !params))
     return set_err(PDQ_ERR_INVALID, "tcoeffs, times and params must be device pointers");
   hipStream_t st = (hipStream_t)stream;
-  int* queue = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&queue, sizeof(int), st));
   pdq::Layout L = pdq::make_layout(*cfg, 0);
-  if (L.nc > 32) {
-    (void)hipFreeAsync(queue, st);
-    return set_err(PDQ_ERR_UNSUPPORTED, "dense state dimension %d exceeds 32", L.nc);
-  }
+  if (L.nc > 32) return set_err(PDQ_ERR_UNSUPPORTED, "dense state dimension %d exceeds 32", L.nc);
   if (L.smoother && (!out->filt_mean || !out->filt_chol || !out->cond_A || !out->cond_mean || !out->cond_chol ||
-                     !out->cond_tl || !out->cond_to)) {
-    (void)hipFreeAsync(queue, st);
+                     !out->cond_tl || !out->cond_to))
     return set_err(PDQ_ERR_INVALID, "smoothers need filt_* and cond_* outputs (backward pass storage)");
-  }
-  if (cfg->solver == PDQ_SOLVER_MLE && !out->num_steps) {
-    (void)hipFreeAsync(queue, st);
+  if (cfg->solver == PDQ_SOLVER_MLE && !out->num_steps)
     return set_err(PDQ_ERR_INVALID, "solver_mle needs the num_steps output");
-  }
   DevMats m;
   rc = device_mats(*cfg, false, m);
   if (rc) return rc;
+  const int G = pick_lanes(*cfg, L, n_inst, mode);
+  if (G == 1) {
+    if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(L.N, L.d))
+      return set_err(PDQ_ERR_UNSUPPORTED,
+                     "lanes_per_instance = 1 (lane-per-instance kernel) needs the isotropic factorisation with "
+                     "nu = 3 or 4 and d = 2 (got nu = %d, d = %d)",
+                     L.N - 1, L.d);
+    double* ws = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&ws, sizeof(double) * pdq::lane_workspace_doubles(L.N, L.d, n_inst), st));
+    pdq::LArgs la{};
+    la.cfg = *cfg;
+    la.n_inst = n_inst;
+    la.K = K;
+    la.mode = mode == pdq::MODE_FIXED_GRID ? pdq::LMODE_FIXED_GRID : pdq::LMODE_SAVE_AT;
+    la.params = params;
+    la.tcoeffs = tcoeffs;
+    la.times = times;
+    la.A = m.A;
+    la.Q = m.Q;
+    la.ws = ws;
+    la.out = *out;
+    rc = pdq::launch_lane(la, L.N, L.d, st);
+    (void)hipFreeAsync(ws, st);
+    return rc;
+  }
+  int* queue = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&queue, sizeof(int), st));
   pdq::KArgs a{};
   a.cfg = *cfg;
   a.lay = L;
@@ -230,7 +253,6 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
   a.Q = m.Q;
   a.out = *out;
   a.queue = queue;
-  const int G = pick_lanes(*cfg, L, n_inst, mode);
   rc = pdq::launch_coop(a, G, st);
   (void)hipFreeAsync(queue, st);
   return rc;

@@ -1,0 +1,921 @@
+// Lane-per-instance kernel for the isotropic factorisation.
+//
+// Every lane owns one IVP instance for the whole solve: the accept/reject
+// loop, checkpoint interpolation and the smoother's backward pass.  The
+// per-step matrices of an instance (N x N factors, the 2N x 2N revert block,
+// N x D means) are register arrays with compile-time sizes; every loop over
+// them is unrolled, so they never touch memory.  The three solution slots
+// that persist between steps (step_from, interp_from, proposed: scalars,
+// mean, factor, backward conditional) live in a per-call HBM workspace laid
+// out element-major across instances, [element][instance], so that the
+// lanes of a wavefront (consecutive instances) access consecutive words.
+// No lane ever reads data another lane wrote: the code's single-thread
+// semantics are the whole story, and the same source built for the CPU
+// (tests/emu) runs exactly this per-instance program.
+//
+// The algebra restates ssm_impl_isotropic.py:72-403 (marginalise, revert,
+// merge, apply_flat, ts0 linearisation), util/cholesky_util.py:27-103
+// (revert_conditional, triu_via_qr) with backend/linalg.py:8-61's R-only
+// Householder QR in LAPACK's dgeqr2 sign convention, and the solver loop of
+// _ivpsolve/solvers_via_adaptive_steps.py:46-375 and
+// _probdiffeq/solvers.py:205-767, in the same order of operations as the
+// cooperative kernel (pdq_coop.hip) and the oracle.
+#pragma once
+
+#include "pdq_args.h"
+#include "pdq_device.h"
+
+namespace pdq {
+
+// Slot scalars.
+enum { LS_T = 0, LS_OS = 1, LS_NSTEPS = 2, LS_MLE_RUN = 3, LS_MLE_N = 4, LS_IDENT = 5, LS_NSC = 8 };
+
+// Element offsets inside one slot.
+template <int N, int D>
+struct SlotLayout {
+  static constexpr int sc = 0;
+  static constexpr int m = LS_NSC;
+  static constexpr int L = m + N * D;
+  static constexpr int cA = L + N * N;
+  static constexpr int cm = cA + N * N;
+  static constexpr int cL = cm + N * D;
+  static constexpr int ctl = cL + N * N;
+  static constexpr int cto = ctl + N;
+  static constexpr int size = cto + N;
+};
+
+// Doubles of the workspace for n_inst instances (three slots each); the
+// same sum as SlotLayout<N, D>::size.
+inline size_t lane_ws_doubles(int N, int D, int n_inst) {
+  const int slot = LS_NSC + 2 * N * D + 3 * N * N + 2 * N;
+  return (size_t)3 * slot * (size_t)n_inst;
+}
+
+// A backward conditional in latent coordinates (ssm_impl_api.py:35-132).
+template <int N, int D>
+struct LCond {
+  double A[N * N], m[N * D], L[N * N], tl[N], to[N];
+};
+
+// View of one instance's slots in the element-major workspace.
+template <int N, int D>
+struct LaneWs {
+  using SL = SlotLayout<N, D>;
+  double* base;  // ws + inst
+  size_t stride;  // n_inst
+  __device__ __forceinline__ double& at(int slot, int e) const {
+    return base[((size_t)slot * SL::size + e) * stride];
+  }
+  __device__ __forceinline__ void load_mL(int s, double* m, double* L) const {
+#pragma unroll
+    for (int i = 0; i < N * D; ++i) m[i] = at(s, SL::m + i);
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) L[i] = at(s, SL::L + i);
+  }
+  __device__ __forceinline__ void store_mL(int s, const double* m, const double* L) const {
+#pragma unroll
+    for (int i = 0; i < N * D; ++i) at(s, SL::m + i) = m[i];
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) at(s, SL::L + i) = L[i];
+  }
+  __device__ __forceinline__ void load_cond(int s, LCond<N, D>& c) const {
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) c.A[i] = at(s, SL::cA + i);
+#pragma unroll
+    for (int i = 0; i < N * D; ++i) c.m[i] = at(s, SL::cm + i);
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) c.L[i] = at(s, SL::cL + i);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      c.tl[i] = at(s, SL::ctl + i);
+      c.to[i] = at(s, SL::cto + i);
+    }
+  }
+  __device__ __forceinline__ void store_cond(int s, const LCond<N, D>& c) const {
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) at(s, SL::cA + i) = c.A[i];
+#pragma unroll
+    for (int i = 0; i < N * D; ++i) at(s, SL::cm + i) = c.m[i];
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) at(s, SL::cL + i) = c.L[i];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      at(s, SL::ctl + i) = c.tl[i];
+      at(s, SL::cto + i) = c.to[i];
+    }
+  }
+  __device__ __forceinline__ void copy_slot(int dst, int src) const {
+#pragma unroll
+    for (int e = 0; e < SL::size; ++e) at(dst, e) = at(src, e);
+  }
+};
+
+// ------------------------------------------------------------- algebra
+// In-place R-only Householder QR of the M x C register matrix A (row-major)
+// in LAPACK dgeqr2 convention (backend/linalg.py:8-37 -> geqrf): on exit the
+// upper triangle of rows 0..min(M,C)-1 holds R, the strict lower part of
+// those rows is zero.
+template <int M, int C>
+__device__ __forceinline__ void lane_qr(double* A) {
+  constexpr int k = M < C ? M : C;
+#pragma unroll
+  for (int i = 0; i < k; ++i) {
+    if (M - i <= 1) break;
+    double ss = 0.0;
+#pragma unroll
+    for (int r = i + 1; r < M; ++r) ss += A[r * C + i] * A[r * C + i];
+    if (ss != 0.0) {
+      const double alpha = A[i * C + i];
+      const double beta = -copysign(hypot(alpha, sqrt(ss)), alpha);
+      const double tau = (beta - alpha) / beta;
+      const double scal = 1.0 / (alpha - beta);
+#pragma unroll
+      for (int r = i + 1; r < M; ++r) A[r * C + i] *= scal;
+#pragma unroll
+      for (int j = i + 1; j < C; ++j) {
+        double w = A[i * C + j];
+#pragma unroll
+        for (int r = i + 1; r < M; ++r) w += A[r * C + i] * A[r * C + j];
+        w *= tau;
+        A[i * C + j] -= w;
+#pragma unroll
+        for (int r = i + 1; r < M; ++r) A[r * C + j] -= A[r * C + i] * w;
+      }
+      A[i * C + i] = beta;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < k; ++i)
+#pragma unroll
+    for (int j = 0; j < i; ++j) A[i * C + j] = 0.0;
+}
+
+// preconditioner_taylor (utilities.py:74-84).
+template <int N>
+__device__ __forceinline__ void lane_precon(double dt, double* P, double* PI) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int pw = N - 1 - i;
+    const double s = factorial_d(pw);
+    P[i] = pow(dt, (double)pw) / s;
+    PI[i] = pow(dt, -(double)pw) * s;
+  }
+}
+
+// Isotropic noise scale (ssm_impl_isotropic.py:375-376): (sqrt|dt| * base) * s.
+__device__ __forceinline__ double lane_noise_scale(const pdq_config& cfg, double dt, double s) {
+  return sqrt(fabs(dt)) * cfg.base_output_scale * s;
+}
+
+// apply_flat of the transition to a mean (ssm_impl_isotropic.py:75-79).
+template <int N, int D>
+__device__ __forceinline__ void lane_apply_flat_mean(const double* __restrict__ A, const double* P, const double* PI,
+                                                     const double* x, double* out) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += (P[i] * A[i * N + k]) * (PI[k] * x[k * D + col]);
+      out[i * D + col] = acc + 0.0;
+    }
+}
+
+// marginalise of the transition (ssm_impl_isotropic.py:81-89).
+template <int N, int D>
+__device__ __forceinline__ void lane_marginalise_prior(const double* __restrict__ A, const double* __restrict__ Q,
+                                                       const double* P, const double* PI, double s, const double* Xm,
+                                                       const double* XL, double* Ym, double* YL) {
+  double big[2 * N * N];
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int col = 0; col < N; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += A[col * N + k] * (PI[k] * XL[k * N + r]);
+      big[r * N + col] = acc;
+      big[(N + r) * N + col] = s * Q[col * N + r];
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += A[i * N + k] * (PI[k] * Xm[k * D + col]);
+      Ym[i * D + col] = P[i] * (acc + 0.0);
+    }
+  lane_qr<2 * N, N>(big);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) YL[i * N + j] = fabs(P[i]) * big[j * N + i];
+}
+
+// revert of the transition (ssm_impl_isotropic.py:107-133 with
+// cholesky_util.py:27-82): observed (Ym, YL) and backward conditional bw.
+template <int N, int D>
+__device__ __forceinline__ void lane_revert_prior(const double* __restrict__ A, const double* __restrict__ Q,
+                                                  const double* P, const double* PI, double s, const double* Xm,
+                                                  const double* XL, double* Ym, double* YL, LCond<N, D>& bw) {
+  constexpr int n2 = 2 * N;
+  double big[n2 * n2];
+  double tmp[N * D];
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int col = 0; col < N; ++col) {
+      big[r * n2 + col] = s * Q[col * N + r];
+      big[r * n2 + N + col] = 0.0;
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += A[col * N + k] * (fabs(PI[k]) * XL[k * N + r]);
+      big[(N + r) * n2 + col] = acc;
+      big[(N + r) * n2 + N + col] = fabs(PI[col]) * XL[col * N + r];
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += A[i * N + k] * (PI[k] * Xm[k * D + col]);
+      tmp[i * D + col] = acc + 0.0;
+    }
+  lane_qr<n2, n2>(big);
+  // gain = solve_triu(R_Y, R12)^T, written as bw.A (N x N)
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+      double sacc = big[i * n2 + N + j];
+#pragma unroll
+      for (int c = i + 1; c < N; ++c) sacc -= big[i * n2 + c] * bw.A[j * N + c];
+      bw.A[j * N + i] = sacc / big[i * n2 + i];
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += bw.A[i * N + k] * tmp[k * D + col];
+      bw.m[i * D + col] = PI[i] * Xm[i * D + col] - acc;
+      Ym[i * D + col] = P[i] * tmp[i * D + col];
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      bw.L[i * N + j] = big[(N + j) * n2 + N + i];
+      YL[i * N + j] = fabs(P[i]) * big[j * n2 + i];
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    bw.tl[i] = 1.0 / P[i];
+    bw.to[i] = 1.0 / PI[i];
+  }
+}
+
+// merge (ssm_impl_isotropic.py:91-105): out = self o other.  With an
+// identity `self` the general formula reduces exactly to a rescaling (the
+// QR of an upper-triangular factor stacked on zeros is that factor).
+template <int N, int D>
+__device__ __forceinline__ void lane_merge(bool self_ident, const LCond<N, D>& sf, const LCond<N, D>& ot,
+                                           LCond<N, D>& r) {
+  if (self_ident) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double T = 1.0 * ot.to[i];
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        r.A[i * N + j] = T * ot.A[i * N + j];
+        r.L[i * N + j] = fabs(T) * ot.L[i * N + j];
+      }
+#pragma unroll
+      for (int col = 0; col < D; ++col) r.m[i * D + col] = (1.0 * ot.to[i]) * ot.m[i * D + col] + 0.0;
+      r.tl[i] = ot.tl[i];
+      r.to[i] = 1.0;
+    }
+    return;
+  }
+  double big[2 * N * N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double acc = 0.0, acc2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const double T = sf.tl[k] * ot.to[k];
+        acc += sf.A[i * N + k] * (T * ot.A[k * N + j]);
+        acc2 += sf.A[i * N + k] * (fabs(T) * ot.L[k * N + j]);
+      }
+      r.A[i * N + j] = acc;
+      big[j * N + i] = acc2;
+      big[(N + j) * N + i] = sf.L[i * N + j];
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += sf.A[i * N + k] * ((sf.tl[k] * ot.to[k]) * ot.m[k * D + col]);
+      r.m[i * D + col] = acc + sf.m[i * D + col];
+    }
+  lane_qr<2 * N, N>(big);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.L[i * N + j] = big[j * N + i];
+    r.tl[i] = ot.tl[i];
+    r.to[i] = sf.to[i];
+  }
+}
+
+// marginalise of a backward conditional (estimators_and_losses.py:156-178).
+template <int N, int D>
+__device__ __forceinline__ void lane_marginalise_cond(const LCond<N, D>& c, const double* Xm, const double* XL,
+                                                      double* Ym, double* YL) {
+  double big[2 * N * N];
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int col = 0; col < N; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += c.A[col * N + k] * (c.tl[k] * XL[k * N + r]);
+      big[r * N + col] = acc;
+      big[(N + r) * N + col] = c.L[col * N + r];
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += c.A[i * N + k] * (c.tl[k] * Xm[k * D + col]);
+      Ym[i * D + col] = c.to[i] * (acc + c.m[i * D + col]);
+    }
+  lane_qr<2 * N, N>(big);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) YL[i * N + j] = fabs(c.to[i]) * big[j * N + i];
+}
+
+// ts0 linearisation (ssm_impl_isotropic.py:298-317): b = -f(u(t)).
+template <int N, int D>
+__device__ __forceinline__ void lane_linearize(const pdq_config& cfg, const double* __restrict__ prm, const double* Xm,
+                                               double t, double* b) {
+  double y[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) y[j] = Xm[j];
+#pragma unroll
+  for (int j = 0; j < D; ++j) b[j] = -vf_component(cfg.vector_field, D, y, prm, t, j);
+}
+
+// fx.marginalise(u) for ts0: observed mean om (D) and factor oL (1 x 1).
+template <int N, int D>
+__device__ __forceinline__ void lane_marginalise_fx(const pdq_config& cfg, const double* b, const double* Xm,
+                                                    const double* XL, double* om, double& oL) {
+  double col[N + 1];
+#pragma unroll
+  for (int r = 0; r < N; ++r) col[r] = XL[1 * N + r];
+  col[N] = cfg.damp;
+#pragma unroll
+  for (int c = 0; c < D; ++c) om[c] = Xm[1 * D + c] + b[c];
+  lane_qr<N + 1, 1>(col);
+  oL = col[0];
+}
+
+// residual_whitened_rms against zero data (ssm_impl_isotropic.py:203-207).
+template <int D>
+__device__ __forceinline__ double lane_rms_zero(const double* om, double oL) {
+  double ss = 0.0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const double w = om[c] / oL;
+    ss += w * w;
+  }
+  return sqrt(ss) / sqrt((double)D);
+}
+
+// fx.revert(pred) + apply_flat(0) (ssm_impl_api.py:112-132): the update.
+template <int N, int D>
+__device__ __forceinline__ void lane_update(const pdq_config& cfg, const double* b, const double* Pm, const double* PL,
+                                            double* Xm, double* XL, double* mo, double& oL_out) {
+  constexpr int n2 = N + 1;
+  double big[n2 * n2];
+#pragma unroll
+  for (int r = 0; r < n2; ++r)
+#pragma unroll
+    for (int col = 0; col < n2; ++col) {
+      double v;
+      if (r < 1) v = col < 1 ? cfg.damp : 0.0;
+      else v = col < 1 ? PL[1 * N + (r - 1)] : PL[(col - 1) * N + (r - 1)];
+      big[r * n2 + col] = v;
+    }
+#pragma unroll
+  for (int c = 0; c < D; ++c) mo[c] = Pm[1 * D + c] + b[c];
+  lane_qr<n2, n2>(big);
+  double gain[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) gain[j] = big[1 + j] / big[0];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const double acc = 0.0 + gain[i] * mo[c];
+      Xm[i * D + c] = Pm[i * D + c] - acc;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) XL[i * N + j] = big[(1 + j) * n2 + 1 + i];
+  oL_out = big[0];
+}
+
+// error_residual_std.estimate_error_norm (solvers.py:925-996), isotropic.
+template <int N, int D>
+__device__ __forceinline__ double lane_error_power(const pdq_config& cfg, double oL, double sigma,
+                                                   const double* prev_m, const double* prop_m, double dt) {
+  const double alpha = sigma * oL;
+  const double err = sqrt(alpha * alpha + 0.0);
+  double norm;
+  if (cfg.error_norm == PDQ_NORM_SCALE_THEN_RMS) {
+    double ss = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double ref = fmax(fabs(prev_m[k]), fabs(prop_m[k]));
+      const double e = err * pow(dt, 1.0) / 1.0;
+      const double rr = e / (cfg.atol + cfg.rtol * fabs(ref));
+      ss += rr * rr;
+    }
+    norm = sqrt(ss) / sqrt((double)D);
+  } else {
+    const double e = err * pow(dt, 1.0) / 1.0;
+    const double sa = 0.0 + e * e;
+    double sr = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double ref = fmax(fabs(prev_m[k]), fabs(prop_m[k]));
+      sr += ref * ref;
+    }
+    norm = (sqrt(sa) / sqrt(1.0)) / (cfg.atol + cfg.rtol * (sqrt(sr) / sqrt((double)D)));
+  }
+  return pow(norm, -1.0 / (double)N);
+}
+
+// controllers.py:66-84 / :24-63
+__device__ __forceinline__ double lane_control(const pdq_config& cfg, double dt, double& cstate, double ep) {
+  if (cfg.control == PDQ_CONTROL_INTEGRAL) {
+    const double ratio = cfg.safety * ep;
+    return fmax(cfg.factor_min, fmin(ratio, cfg.factor_max)) * dt;
+  }
+  const double gi = pow(ep, cfg.exponent_integral);
+  const double gp = pow(ep / cstate, cfg.exponent_proportional);
+  const double ratio = cfg.safety * gi * gp;
+  const double sf = fmax(cfg.factor_min, fmin(ratio, cfg.factor_max));
+  if (ep >= 1.0) cstate = ep;
+  return sf * dt;
+}
+
+// ------------------------------------------------------------ instance
+template <int N, int D>
+struct LaneCtx {
+  const LArgs* a;
+  const double* prm;
+  LaneWs<N, D> w;
+  int inst;
+};
+
+// solver_dynamic.step (solvers.py:549-599), solver.step (:702-733) and
+// solver_mle.step (:395-437) with the residual error estimate; slot `from`
+// -> slot `to`.  Returns error_power (meaningful with `estimate`).
+template <int N, int D>
+__device__ __forceinline__ double lane_step(const LaneCtx<N, D>& c, int from, int to, double dt, bool estimate) {
+  const pdq_config& cfg = c.a->cfg;
+  const double* __restrict__ A = c.a->A;
+  const double* __restrict__ Q = c.a->Q;
+  const LaneWs<N, D>& w = c.w;
+  double Sm[N * D], SLf[N * N];
+  w.load_mL(from, Sm, SLf);
+  const double t = w.at(from, LS_T);
+  const double tn = t + dt;
+  const bool smoother = cfg.strategy != PDQ_FILTER;
+  const bool ident = w.at(from, LS_IDENT) != 0.0;
+  double P[N], PI[N];
+  lane_precon<N>(dt, P, PI);
+  double sigma = 1.0, ep = 0.0;
+  double Pm[N * D], PL[N * N], Rm[N * D], RL[N * N], b[D];
+  auto predict = [&](double s) {
+    if (!smoother) {
+      lane_marginalise_prior<N, D>(A, Q, P, PI, s, Sm, SLf, Pm, PL);
+      return;
+    }
+    LCond<N, D> bw, rc;
+    lane_revert_prior<N, D>(A, Q, P, PI, s, Sm, SLf, Pm, PL, bw);
+    if (cfg.strategy == PDQ_FIXEDPOINT) {
+      LCond<N, D> sc;
+      if (!ident) w.load_cond(from, sc);
+      lane_merge<N, D>(ident, sc, bw, rc);
+      w.store_cond(to, rc);
+    } else {
+      w.store_cond(to, bw);
+    }
+  };
+  if (cfg.solver == PDQ_SOLVER_DYNAMIC) {
+    const double s1 = lane_noise_scale(cfg, dt, 1.0);
+    double Um[N * D], UL[N * N];
+    lane_apply_flat_mean<N, D>(A, P, PI, Sm, Um);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * Q[i * N + j]);
+    lane_linearize<N, D>(cfg, c.prm, Um, tn, b);
+    double om[D], oL;
+    lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oL);
+    sigma = lane_rms_zero<D>(om, oL);
+    predict(lane_noise_scale(cfg, dt, sigma));
+    double mo[D], oLu;
+    lane_update<N, D>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
+    if (estimate) ep = lane_error_power<N, D>(cfg, oL, sigma, Sm, Rm, dt);
+  } else {
+    const double s1 = lane_noise_scale(cfg, dt, 1.0);
+    predict(s1);
+    lane_linearize<N, D>(cfg, c.prm, Pm, tn, b);
+    const bool mle = cfg.solver == PDQ_SOLVER_MLE;
+    double mo[D], oLu;
+    lane_update<N, D>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
+    if (mle) {
+      // bayes_rule_and_residual_whitened_rms_tree: observed = (mo, R_Y^T)
+      const double nt = lane_rms_zero<D>(mo, oLu);
+      const double nd = w.at(from, LS_MLE_N);
+      const double x1 = sqrt(nd / (nd + 1)) * w.at(from, LS_MLE_RUN);
+      const double x2 = sqrt(1 / (nd + 1)) * nt;
+      w.at(to, LS_MLE_RUN) = hypot(x1, x2);
+      w.at(to, LS_MLE_N) = nd + 1;
+    }
+    if (estimate) {
+      double Um[N * D], UL[N * N];
+      lane_apply_flat_mean<N, D>(A, P, PI, Sm, Um);
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * Q[i * N + j]);
+      double om[D], oL;
+      lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oL);
+      const double se = lane_rms_zero<D>(om, oL);
+      ep = lane_error_power<N, D>(cfg, oL, se, Sm, Rm, dt);
+    }
+    sigma = mle ? w.at(from, LS_OS) : 1.0;
+  }
+  w.store_mL(to, Rm, RL);
+  w.at(to, LS_T) = tn;
+  w.at(to, LS_OS) = sigma;
+  w.at(to, LS_NSTEPS) = w.at(from, LS_NSTEPS) + 1.0;
+  w.at(to, LS_IDENT) = 0.0;
+  if (cfg.solver != PDQ_SOLVER_MLE) {
+    w.at(to, LS_MLE_RUN) = 0.0;
+    w.at(to, LS_MLE_N) = 0.0;
+  }
+  return ep;
+}
+
+// ---------------------------------------------------------------- outputs
+template <int N, int D>
+__device__ __forceinline__ void lane_write_normal(const double* m, const double* L, double* gm, double* gL) {
+  if (gm)
+#pragma unroll
+    for (int i = 0; i < N * D; ++i) gm[i] = m[i];
+  if (gL)
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) gL[i] = L[i];
+}
+
+template <int N, int D>
+__device__ __forceinline__ void lane_write_cond(const LaneCtx<N, D>& c, bool ident, const LCond<N, D>& s, int j) {
+  const pdq_solution& o = c.a->out;
+  const size_t idx = (size_t)c.inst * (c.a->K - 1) + j;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      o.cond_A[idx * N * N + i * N + k] = ident ? (i == k ? 1.0 : 0.0) : s.A[i * N + k];
+      o.cond_chol[idx * N * N + i * N + k] = ident ? 0.0 : s.L[i * N + k];
+    }
+#pragma unroll
+  for (int e = 0; e < N * D; ++e) o.cond_mean[idx * N * D + e] = ident ? 0.0 : s.m[e];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    o.cond_tl[idx * N + i] = ident ? 1.0 : s.tl[i];
+    o.cond_to[idx * N + i] = ident ? 1.0 : s.to[i];
+  }
+}
+
+// Writes checkpoint k (the filter's marginal, or the smoothers' filtering
+// marginal and the backward conditional that leads to it).
+template <int N, int D>
+__device__ __forceinline__ void lane_write_checkpoint(const LaneCtx<N, D>& c, int k, double t, const double* m,
+                                                      const double* L, double os, double nsteps, bool has_cond,
+                                                      bool cond_ident, const LCond<N, D>& cond) {
+  const pdq_solution& o = c.a->out;
+  const int K = c.a->K;
+  const size_t row = (size_t)c.inst * K + k;
+  const pdq_config& cfg = c.a->cfg;
+  if (o.t) o.t[row] = t;
+  if (o.output_scale && cfg.solver == PDQ_SOLVER_DYNAMIC) o.output_scale[row] = os;
+  if (o.num_steps && k > 0) o.num_steps[(size_t)c.inst * (K - 1) + k - 1] = (int64_t)nsteps;
+  if (cfg.strategy != PDQ_FILTER) {
+    lane_write_normal<N, D>(m, L, o.filt_mean ? o.filt_mean + row * N * D : nullptr,
+                            o.filt_chol ? o.filt_chol + row * N * N : nullptr);
+    if (k > 0 && has_cond) lane_write_cond<N, D>(c, cond_ident, cond, k - 1);
+  } else {
+    lane_write_normal<N, D>(m, L, o.mean ? o.mean + row * N * D : nullptr, o.chol ? o.chol + row * N * N : nullptr);
+  }
+}
+
+// Checkpoint k from slot s (its own marginal and backward conditional).
+template <int N, int D>
+__device__ __forceinline__ void lane_write_slot(const LaneCtx<N, D>& c, int k, int s, bool with_cond) {
+  double m[N * D], L[N * N];
+  c.w.load_mL(s, m, L);
+  LCond<N, D> cond;
+  const bool smoother = c.a->cfg.strategy != PDQ_FILTER;
+  const bool ident = c.w.at(s, LS_IDENT) != 0.0;
+  if (smoother && with_cond && k > 0 && !ident) c.w.load_cond(s, cond);
+  lane_write_checkpoint<N, D>(c, k, c.w.at(s, LS_T), m, L, c.w.at(s, LS_OS), c.w.at(s, LS_NSTEPS), with_cond, ident,
+                              cond);
+}
+
+// ProbabilisticSolver.interpolate_fwd (solvers.py:205-269): from = interp_from,
+// to = step_from; writes checkpoint k and updates both slots.
+template <int N, int D>
+__device__ __forceinline__ void lane_interp_beyond(const LaneCtx<N, D>& c, int k, double t1, int from, int to) {
+  const pdq_config& cfg = c.a->cfg;
+  const double* __restrict__ A = c.a->A;
+  const double* __restrict__ Q = c.a->Q;
+  const LaneWs<N, D>& w = c.w;
+  const double os = w.at(to, LS_OS);
+  const double dt0 = t1 - w.at(from, LS_T);
+  const double dt1 = w.at(to, LS_T) - t1;
+  const double nsteps = w.at(to, LS_NSTEPS);
+  double Fm[N * D], FL[N * N], P[N], PI[N], Pm[N * D], PL[N * N];
+  w.load_mL(from, Fm, FL);
+  lane_precon<N>(dt0, P, PI);
+  const double s0 = lane_noise_scale(cfg, dt0, os);
+  if (cfg.strategy == PDQ_FILTER) {
+    lane_marginalise_prior<N, D>(A, Q, P, PI, s0, Fm, FL, Pm, PL);
+    LCond<N, D> none;
+    lane_write_checkpoint<N, D>(c, k, t1, Pm, PL, os, nsteps, false, false, none);
+    w.store_mL(from, Pm, PL);
+    w.at(from, LS_T) = t1;
+    return;
+  }
+  LCond<N, D> bw, ext;
+  lane_revert_prior<N, D>(A, Q, P, PI, s0, Fm, FL, Pm, PL, bw);
+  const bool from_ident = w.at(from, LS_IDENT) != 0.0;
+  if (cfg.strategy == PDQ_FIXEDPOINT) {
+    LCond<N, D> fc;
+    if (!from_ident) w.load_cond(from, fc);
+    lane_merge<N, D>(from_ident, fc, bw, ext);
+  } else {
+    ext = bw;
+  }
+  // extrapolate t -> t1 from ext_t's marginal; only the backward conditional is kept
+  lane_precon<N>(dt1, P, PI);
+  const double s1 = lane_noise_scale(cfg, dt1, os);
+  double Ym[N * D], YL[N * N];
+  LCond<N, D> bw2;
+  lane_revert_prior<N, D>(A, Q, P, PI, s1, Pm, PL, Ym, YL, bw2);
+  if (cfg.strategy == PDQ_FIXEDPOINT) {
+    LCond<N, D> unused, rc;
+    lane_merge<N, D>(true, unused, bw2, rc);
+    w.store_cond(to, rc);
+  } else {
+    w.store_cond(to, bw2);
+  }
+  lane_write_checkpoint<N, D>(c, k, t1, Pm, PL, os, nsteps, true, false, ext);
+  w.store_mL(from, Pm, PL);
+  if (cfg.strategy == PDQ_FIXEDINTERVAL) w.store_cond(from, ext);
+  w.at(from, LS_T) = t1;
+  w.at(from, LS_IDENT) = cfg.strategy == PDQ_FIXEDPOINT ? 1.0 : 0.0;
+  w.at(to, LS_IDENT) = 0.0;
+}
+
+// ProbabilisticSolver.interpolate_fwd_at_t1 (solvers.py:271-315).
+template <int N, int D>
+__device__ __forceinline__ void lane_interp_at(const LaneCtx<N, D>& c, int k, int from, int to) {
+  lane_write_slot<N, D>(c, k, to, true);
+  c.w.copy_slot(from, to);
+  if (c.a->cfg.strategy == PDQ_FIXEDPOINT) {
+    c.w.at(to, LS_IDENT) = 1.0;
+    c.w.at(from, LS_IDENT) = 1.0;
+  }
+}
+
+// userfriendly_output + strategy.finalize (solvers.py:439-480, 601-633,
+// 735-767; estimators_and_losses.py:380-391, 437-470).
+template <int N, int D>
+__device__ __forceinline__ void lane_finalize(const LaneCtx<N, D>& c, int sf) {
+  const pdq_config& cfg = c.a->cfg;
+  const pdq_solution& o = c.a->out;
+  const int K = c.a->K, Km1 = K - 1;
+  const size_t inst = (size_t)c.inst;
+  double scale = 1.0;
+  const bool mle = cfg.solver == PDQ_SOLVER_MLE;
+  if (mle) {
+    scale = c.w.at(sf, LS_MLE_RUN);
+    if (cfg.correct_asymptotic_underconfidence) {
+      const double ns = (double)o.num_steps[inst * Km1 + Km1 - 1];
+      scale = scale / sqrt(ns);
+    }
+    if (o.output_scale)
+      for (int k = 0; k < Km1; ++k) o.output_scale[inst * K + k] = scale;
+  } else if (cfg.solver == PDQ_SOLVER && o.output_scale) {
+    for (int k = 0; k < Km1; ++k) o.output_scale[inst * K + k] = 1.0;
+  }
+  if (cfg.strategy == PDQ_FILTER) {
+    if (mle && o.chol)
+      for (size_t e = 0; e < (size_t)K * N * N; ++e) o.chol[inst * K * N * N + e] = scale * o.chol[inst * K * N * N + e];
+    return;
+  }
+  double Sm[N * D], SLf[N * N];
+  c.w.load_mL(sf, Sm, SLf);
+  LCond<N, D> cond;
+  const bool ident = c.w.at(sf, LS_IDENT) != 0.0;
+  if (!ident) c.w.load_cond(sf, cond);
+  if (mle) {
+    for (size_t e = 0; e < (size_t)K * N * N; ++e) o.filt_chol[inst * K * N * N + e] = scale * o.filt_chol[inst * K * N * N + e];
+    for (size_t e = 0; e < (size_t)Km1 * N * N; ++e)
+      o.cond_chol[inst * Km1 * N * N + e] = scale * o.cond_chol[inst * Km1 * N * N + e];
+#pragma unroll
+    for (int e = 0; e < N * N; ++e) {
+      SLf[e] = scale * SLf[e];
+      cond.L[e] = scale * cond.L[e];
+    }
+  }
+  // rv_at_t1 = posterior1.conditional.marginalise(posterior1.marginal)
+  double Xm[N * D], XL[N * N];
+  if (ident) {
+#pragma unroll
+    for (int e = 0; e < N * D; ++e) Xm[e] = Sm[e];
+#pragma unroll
+    for (int e = 0; e < N * N; ++e) XL[e] = SLf[e];
+  } else {
+    lane_marginalise_cond<N, D>(cond, Sm, SLf, Xm, XL);
+  }
+  const size_t rowK = inst * K + (K - 1);
+  lane_write_normal<N, D>(Xm, XL, o.term_mean ? o.term_mean + inst * N * D : nullptr,
+                          o.term_chol ? o.term_chol + inst * N * N : nullptr);
+  lane_write_normal<N, D>(Xm, XL, o.mean ? o.mean + rowK * N * D : nullptr, o.chol ? o.chol + rowK * N * N : nullptr);
+  // backward pass: x_j = cond_j.marginalise(x_{j+1}), j = K-2 .. 0
+  for (int j = Km1 - 1; j >= 0; --j) {
+    const size_t idx = inst * Km1 + j;
+    LCond<N, D> cj;
+#pragma unroll
+    for (int e = 0; e < N * N; ++e) {
+      cj.A[e] = o.cond_A[idx * N * N + e];
+      cj.L[e] = o.cond_chol[idx * N * N + e];
+    }
+#pragma unroll
+    for (int e = 0; e < N * D; ++e) cj.m[e] = o.cond_mean[idx * N * D + e];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      cj.tl[i] = o.cond_tl[idx * N + i];
+      cj.to[i] = o.cond_to[idx * N + i];
+    }
+    double Ym[N * D], YL[N * N];
+    lane_marginalise_cond<N, D>(cj, Xm, XL, Ym, YL);
+    const size_t row = inst * K + j;
+    lane_write_normal<N, D>(Ym, YL, o.mean ? o.mean + row * N * D : nullptr, o.chol ? o.chol + row * N * N : nullptr);
+#pragma unroll
+    for (int e = 0; e < N * D; ++e) Xm[e] = Ym[e];
+#pragma unroll
+    for (int e = 0; e < N * N; ++e) XL[e] = YL[e];
+  }
+}
+
+// One instance: init, the save_at / fixed-grid loop, finalize, counters.
+template <int N, int D>
+__device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst) {
+  const pdq_config& cfg = a.cfg;
+  LaneCtx<N, D> c;
+  c.a = &a;
+  c.inst = inst;
+  c.prm = a.params + (size_t)inst * cfg.n_params;
+  c.w.base = a.ws + inst;
+  c.w.stride = (size_t)a.n_inst;
+  const LaneWs<N, D>& w = c.w;
+  const int K = a.K;
+  const double* ts = a.times + (size_t)inst * K;
+  int sf = 0, ifr = 1, pr = 2;
+  // prior init (from_mean_and_std): mean = Taylor coefficients, chol = init_std I
+  {
+    const double* tc = a.tcoeffs + (size_t)inst * N * D;
+    double m[N * D], L[N * N];
+#pragma unroll
+    for (int e = 0; e < N * D; ++e) m[e] = tc[e];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) L[i * N + j] = i == j ? cfg.init_std : 0.0;
+#pragma unroll
+    for (int e = 0; e < SlotLayout<N, D>::size; ++e) w.at(sf, e) = 0.0;
+    w.store_mL(sf, m, L);
+    w.at(sf, LS_T) = ts[0];
+    w.at(sf, LS_OS) = 1.0;
+    w.at(sf, LS_NSTEPS) = 0.0;
+    w.at(sf, LS_IDENT) = cfg.strategy != PDQ_FILTER ? 1.0 : 0.0;
+    w.copy_slot(ifr, sf);
+    LCond<N, D> none;
+    lane_write_checkpoint<N, D>(c, 0, ts[0], m, L, 1.0, 0.0, false, false, none);
+  }
+  long attempts = 0, accepted = 0;
+  int status = PDQ_STATUS_OK;
+  if (a.mode == LMODE_FIXED_GRID) {
+    for (int k = 1; k < K; ++k) {
+      lane_step<N, D>(c, sf, pr, ts[k] - ts[k - 1], false);
+      ++attempts;
+      ++accepted;
+      lane_write_slot<N, D>(c, k, pr, true);
+      const int tmp = sf;
+      sf = pr;
+      pr = tmp;
+    }
+  } else {
+    double dt = cfg.dt0, cstate = 1.0;
+    for (int k = 1; k < K && status == PDQ_STATUS_OK; ++k) {
+      const double t1 = ts[k];
+      bool cont = true;
+      while (cont) {
+        if (w.at(sf, LS_T) + cfg.eps < t1) {
+          double ep = 0.9;
+          int att = 0;
+          while (ep < 1.0) {
+            double dtc = dt;
+            const double tsf = w.at(sf, LS_T);
+            if (cfg.clip_dt) dtc = fmin(dtc, t1 - tsf);
+            ep = lane_step<N, D>(c, sf, pr, dtc, true);
+            dt = lane_control(cfg, dtc, cstate, ep);
+            ++att;
+            ++attempts;
+            if (!isfinite(ep) || !isfinite(dt)) {
+              status = PDQ_STATUS_NONFINITE;
+              break;
+            }
+            if (ep < 1.0 && (dt == 0.0 || tsf + dt == tsf)) {
+              status = PDQ_STATUS_DT_UNDERFLOW;
+              break;
+            }
+            if (ep >= 1.0 && w.at(pr, LS_T) == tsf) {
+              status = PDQ_STATUS_DT_UNDERFLOW;  // accepted without advancing t
+              break;
+            }
+            if ((cfg.max_attempts > 0 && att >= cfg.max_attempts && ep < 1.0) || attempts >= kMaxTotalAttempts) {
+              status = PDQ_STATUS_MAX_ATTEMPTS;
+              break;
+            }
+          }
+          if (status != PDQ_STATUS_OK) break;
+          ++accepted;
+          const int old_if = ifr;  // interp_from <- step_from, step_from <- proposed
+          ifr = sf;
+          sf = pr;
+          pr = old_if;
+        }
+        const double tsf = w.at(sf, LS_T);
+        if (tsf + cfg.eps < t1) {
+          cont = true;
+        } else if (tsf > t1 + cfg.eps) {
+          lane_interp_beyond<N, D>(c, k, t1, ifr, sf);
+          cont = w.at(sf, LS_T) + cfg.eps < t1;
+        } else {
+          lane_interp_at<N, D>(c, k, ifr, sf);
+          cont = false;
+        }
+      }
+    }
+  }
+  if (status == PDQ_STATUS_OK) lane_finalize<N, D>(c, sf);
+  const pdq_solution& o = a.out;
+  if (o.attempts) o.attempts[inst] = attempts;
+  if (o.accepted) o.accepted[inst] = accepted;
+  if (o.status) o.status[inst] = status;
+}
+
+#ifndef PDQ_HOST_EMU
+// One lane per instance, grid-stride over instances.
+template <int N, int D>
+__global__ void __launch_bounds__(64) lane_kernel(LArgs args) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int inst = blockIdx.x * blockDim.x + threadIdx.x; inst < args.n_inst; inst += stride)
+    lane_solve_instance<N, D>(args, inst);
+}
+#endif
+
+}  // namespace pdq

@@ -33,8 +33,11 @@ def lib():
     return _lib
 
 
-def solve(cfg, params, tcoeffs, times, fixed_grid=False):
-    """cfg: probdiffeq_amd.capi.Config; numpy inputs; returns numpy outputs."""
+def solve(cfg, params, tcoeffs, times, fixed_grid=False, kernel="coop"):
+    """cfg: probdiffeq_amd.capi.Config; numpy inputs; returns numpy outputs.
+
+    kernel: "coop" (the cooperative kernel's code) or "lane" (the
+    lane-per-instance isotropic kernel's code)."""
     from probdiffeq_amd import capi
 
     params = np.ascontiguousarray(params, dtype=np.float64)
@@ -70,7 +73,8 @@ def solve(cfg, params, tcoeffs, times, fixed_grid=False):
         setattr(s, name, a.ctypes.data if a is not None else None)
     c = cfg.c()
     P = ctypes.c_void_p
-    rc = lib().pdqemu_solve(ctypes.byref(c), B, P(params.ctypes.data), P(tcoeffs.ctypes.data), P(times.ctypes.data), K, ctypes.byref(s), int(fixed_grid))
+    fn = lib().pdqemu_solve_lane if kernel == "lane" else lib().pdqemu_solve
+    rc = fn(ctypes.byref(c), B, P(params.ctypes.data), P(tcoeffs.ctypes.data), P(times.ctypes.data), K, ctypes.byref(s), int(fixed_grid))
     if rc:
         raise RuntimeError(f"emulation rc {rc}")
     return out

@@ -8,6 +8,7 @@
 #include "../../probdiffeq_amd/csrc/pdq_blockdiag.hip"
 #include "../../probdiffeq_amd/csrc/pdq_coop.hip"
 #include "../../probdiffeq_amd/csrc/pdq_iwp.h"
+#include "../../probdiffeq_amd/csrc/pdq_lane.h"
 
 namespace pdq {
 int host_error(int code, const char*, ...) { return code; }
@@ -15,6 +16,36 @@ int num_cus() { return 1; }
 }  // namespace pdq
 
 extern "C" {
+
+// The lane-per-instance isotropic kernel's per-instance program (pdq_lane.h)
+// on the CPU, one instance after another.  Returns 2 for a shape the device
+// library does not instantiate.
+int pdqemu_solve_lane(const pdq_config* cfg, int n_inst, const double* params, const double* tcoeffs,
+                      const double* times, int K, const pdq_solution* out, int mode) {
+  const int N = cfg->num_derivatives + 1, D = cfg->ode_dim;
+  if (cfg->factorisation != PDQ_ISOTROPIC) return 2;
+  std::vector<double> A, Q;
+  pdq::iwp_matrices(cfg->num_derivatives, A, Q);
+  std::vector<double> ws(pdq::lane_ws_doubles(N, D, n_inst));
+  pdq::LArgs a{};
+  a.cfg = *cfg;
+  a.n_inst = n_inst;
+  a.K = K;
+  a.mode = mode;
+  a.params = params;
+  a.tcoeffs = tcoeffs;
+  a.times = times;
+  a.A = A.data();
+  a.Q = Q.data();
+  a.ws = ws.data();
+  a.out = *out;
+  for (int i = 0; i < n_inst; ++i) {
+    if (N == 4 && D == 2) pdq::lane_solve_instance<4, 2>(a, i);
+    else if (N == 5 && D == 2) pdq::lane_solve_instance<5, 2>(a, i);
+    else return 2;
+  }
+  return 0;
+}
 
 // mode: 0 save_at, 1 fixed grid.  Output pointers as in pdq_solution (host).
 int pdqemu_solve(const pdq_config* cfg, int n_inst, const double* params, const double* tcoeffs, const double* times,

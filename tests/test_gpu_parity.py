@@ -201,19 +201,23 @@ def test_c1_dense_filter_single_solve(oracle, pdq):
     _compare(oracle, pdq, cfg, THETA[None], np.array([[20.0, 20.0]]), times, check_factor_signs=True)
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("lanes", [0, 1, 4, 8, 16, 32, 64])
 def test_isotropic_fixedpoint_batch_all_group_sizes(oracle, pdq, lanes):
-    """C2 kind: isotropic nu=4, fixed-point smoother, dynamic, uneven lengths."""
+    """C2 kind: isotropic nu=4, fixed-point smoother, dynamic, uneven lengths.
+
+    lanes 0 (automatic) and 1 run the lane-per-instance kernel, 4..64 the
+    cooperative kernel with that group size."""
     cfg = pdq.Config(factorisation="isotropic", strategy="fixedpoint", solver="dynamic", lanes_per_instance=lanes)
     params, y0, times = _lv_batch(12, seed=lanes)
     _compare(oracle, pdq, cfg, params, y0, times)
 
 
-@pytest.mark.parametrize("fact", ["dense", "isotropic"])
+@pytest.mark.parametrize("fact,lanes", [("dense", 0), ("isotropic", 0), ("isotropic", 16)])
 @pytest.mark.parametrize("strategy", ["filter", "fixedpoint", "fixedinterval"])
 @pytest.mark.parametrize("solver", ["solver", "dynamic", "mle"])
-def test_strategy_solver_grid(oracle, pdq, fact, strategy, solver):
-    cfg = pdq.Config(factorisation=fact, num_derivatives=3, strategy=strategy, solver=solver, rtol=1e-5, atol=1e-7)
+def test_strategy_solver_grid(oracle, pdq, fact, lanes, strategy, solver):
+    """isotropic with lanes 0: lane-per-instance kernel; lanes 16: cooperative kernel."""
+    cfg = pdq.Config(factorisation=fact, num_derivatives=3, strategy=strategy, solver=solver, rtol=1e-5, atol=1e-7, lanes_per_instance=lanes)
     params, y0, times = _lv_batch(6, seed=7, t_lo=3.0, t_hi=8.0, K=9)
     _compare(oracle, pdq, cfg, params, y0, times)
 
@@ -229,9 +233,9 @@ def test_c3_dense_ts1_van_der_pol(oracle, pdq):
     _compare(oracle, pdq, cfg, mu, y0, times)
 
 
-@pytest.mark.parametrize("fact", ["dense", "isotropic"])
-def test_fixed_grid(oracle, pdq, fact):
-    cfg = pdq.Config(factorisation=fact, num_derivatives=3, strategy="fixedinterval", solver="dynamic")
+@pytest.mark.parametrize("fact,lanes", [("dense", 0), ("isotropic", 0), ("isotropic", 16)])
+def test_fixed_grid(oracle, pdq, fact, lanes):
+    cfg = pdq.Config(factorisation=fact, num_derivatives=3, strategy="fixedinterval", solver="dynamic", lanes_per_instance=lanes)
     params, y0, _ = _lv_batch(4, seed=11)
     times = np.tile(np.linspace(0.0, 4.0, 41), (4, 1))
     _compare(oracle, pdq, cfg, params, y0, times, fixed_grid=True)
@@ -265,6 +269,28 @@ def test_blockdiag_is_refused_on_the_device(pdq):
     tcs = np.zeros((1, 5, 2))
     with pytest.raises(pdq.PdqError, match="blockdiag"):
         pdq.solve(cfg, _dev(params), _dev(tcs), _dev(times))
+
+
+@pytest.mark.parametrize(
+    "kw",
+    [
+        dict(strategy="filter", solver="dynamic", clip_dt=True),
+        dict(strategy="filter", solver="dynamic", error_norm="rms_then_scale"),
+        dict(strategy="fixedpoint", solver="dynamic", control="proportional_integral"),
+    ],
+)
+def test_lane_kernel_configuration_switches(oracle, pdq, kw):
+    """clip_dt, the rms-then-scale error norm and the PI controller on the lane kernel (C2 shape)."""
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=4, **kw)
+    params, y0, times = _lv_batch(8, seed=21)
+    _compare(oracle, pdq, cfg, params, y0, times)
+
+
+def test_lane_kernel_refuses_other_shapes(pdq):
+    cfg = pdq.Config(factorisation="dense", lanes_per_instance=1)
+    params, y0, times = _lv_batch(1, seed=5)
+    with pytest.raises(pdq.PdqError, match="lane-per-instance"):
+        pdq.solve(cfg, _dev(params), _dev(np.zeros((1, 5, 2))), _dev(times))
 
 
 def test_taylor_coefficients_match_oracle(oracle, pdq):

@@ -24,13 +24,13 @@ def emu():
     return E
 
 
-def _run(oracle, emu, cfg, params, y0s, times, fixed_grid=False):
+def _run(oracle, emu, cfg, params, y0s, times, fixed_grid=False, kernel="coop"):
     from probdiffeq_amd import capi
 
     B, K = times.shape
     ocfg = T._oracle_cfg(oracle, cfg)
     tcs = np.stack([oracle.taylor_coefficients(cfg.vector_field, params[i], y0s[i], times[i, 0], cfg.num_derivatives) for i in range(B)])
-    g = emu.solve(cfg, params, tcs, times, fixed_grid)
+    g = emu.solve(cfg, params, tcs, times, fixed_grid, kernel=kernel)
     for i in range(B):
         r, rf, rt = T._oracle_solve(oracle, ocfg, params[i], tcs[i], times[i], fixed_grid)
         assert g["status"][i] == 0
@@ -131,3 +131,54 @@ def test_lml_value_and_gradient(oracle, emu, fact, solver, per_coordinate_std, a
         T.check_accuracy(v[i], vo, vf, vt, f"emulation lml instance {i}")
         gr = LC.richardson_grad(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
         assert LC.rel(g[i], gr) < LC.GRAD_RTOL, (g[i], gr)
+
+
+# ----------------------------------------------- lane-per-instance kernel
+@pytest.mark.parametrize("strategy", ["filter", "fixedpoint", "fixedinterval"])
+@pytest.mark.parametrize("solver", ["solver", "dynamic", "mle"])
+def test_lane_kernel_strategy_solver_grid(oracle, emu, strategy, solver):
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=3, strategy=strategy, solver=solver, rtol=1e-5, atol=1e-7)
+    p, y, t = T._lv_batch(4, seed=7, t_lo=3.0, t_hi=8.0, K=9)
+    _run(oracle, emu, cfg, p, y, t, kernel="lane")
+
+
+@pytest.mark.parametrize(
+    "kw",
+    [
+        dict(strategy="fixedpoint", solver="dynamic"),
+        dict(strategy="filter", solver="dynamic", clip_dt=True),
+        dict(strategy="filter", solver="dynamic", error_norm="rms_then_scale"),
+        dict(strategy="fixedpoint", solver="dynamic", control="proportional_integral"),
+    ],
+)
+def test_lane_kernel_c2_shape(oracle, emu, kw):
+    """The C2 shape (isotropic nu=4, d=2) and the configuration switches."""
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=4, **kw)
+    p, y, t = T._lv_batch(3, seed=1, t_lo=5.0, t_hi=15.0, K=20)
+    _run(oracle, emu, cfg, p, y, t, kernel="lane")
+
+
+@pytest.mark.parametrize("strategy", ["filter", "fixedinterval"])
+def test_lane_kernel_fixed_grid(oracle, emu, strategy):
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=3, strategy=strategy, solver="dynamic")
+    p, y, _ = T._lv_batch(3, seed=11)
+    times = np.tile(np.linspace(0.0, 4.0, 41), (3, 1))
+    _run(oracle, emu, cfg, p, y, times, fixed_grid=True, kernel="lane")
+
+
+def test_lane_kernel_status_codes(emu):
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=4, rtol=1e-300, atol=1e-300, max_attempts=4)
+    p, y, t = T._lv_batch(2, seed=1)
+    from oracle import oracle as O
+
+    tcs = np.stack([O.taylor_coefficients("lotka_volterra", p[i], y[i], 0.0, 4) for i in range(2)])
+    g = emu.solve(cfg, p, tcs, t, kernel="lane")
+    assert g["status"].tolist() == [1, 1]
