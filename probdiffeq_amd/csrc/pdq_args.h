@@ -66,7 +66,6 @@ struct LArgs {
   const double* times;    // n_inst x K
   const double* A;        // N x N latent IWP transition
   const double* Q;        // N x N latent IWP noise factor
-  double* ws;             // 3 slots x slot size x n_inst, element-major
   pdq_solution out;
 };
 enum { LMODE_SAVE_AT = 0, LMODE_FIXED_GRID = 1 };
@@ -79,9 +78,8 @@ int num_cus();
 int launch_coop(KArgs& a, int G, hipStream_t st);
 int launch_bd(BArgs& a, hipStream_t st);
 // Lane-per-instance isotropic kernel (pdq_lane.hip): shapes (N = nu + 1, d)
-// it is instantiated for, its workspace size and its launcher.
+// it is instantiated for and its launcher.
 bool lane_supported(int N, int D);
-size_t lane_workspace_doubles(int N, int D, int n_inst);
 int launch_lane(LArgs& a, int N, int D, hipStream_t st);
 int launch_taylor_tangents(int vf, int d, int np, int n_inst, int num, int ntan, const double* params,
                            const double* y0, const double* t0, double* out, hipStream_t st);

@@ -220,8 +220,6 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
                      "lanes_per_instance = 1 (lane-per-instance kernel) needs the isotropic factorisation with "
                      "nu = 3 or 4 and d = 2 (got nu = %d, d = %d)",
                      L.N - 1, L.d);
-    double* ws = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&ws, sizeof(double) * pdq::lane_workspace_doubles(L.N, L.d, n_inst), st));
     pdq::LArgs la{};
     la.cfg = *cfg;
     la.n_inst = n_inst;
@@ -232,10 +230,8 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
     la.times = times;
     la.A = m.A;
     la.Q = m.Q;
-    la.ws = ws;
     la.out = *out;
     rc = pdq::launch_lane(la, L.N, L.d, st);
-    (void)hipFreeAsync(ws, st);
     return rc;
   }
   int* queue = nullptr;

@@ -4,11 +4,11 @@
 // loop, checkpoint interpolation and the smoother's backward pass.  The
 // per-step matrices of an instance (N x N factors, the 2N x 2N revert block,
 // N x D means) are register arrays with compile-time sizes; every loop over
-// them is unrolled, so they never touch memory.  The three solution slots
-// that persist between steps (step_from, interp_from, proposed: scalars,
-// mean, factor, backward conditional) live in a per-call HBM workspace laid
-// out element-major across instances, [element][instance], so that the
-// lanes of a wavefront (consecutive instances) access consecutive words.
+// them is unrolled.  The three solution slots that persist between steps
+// (step_from, interp_from, proposed: scalars, mean, packed factor, backward
+// conditional) live in LDS, element-major across the workgroup's 64 lanes,
+// so that consecutive lanes access consecutive words; a workgroup is one
+// wavefront and takes 104 KB (nu = 3) or 140 KB (nu = 4) of LDS.
 // No lane ever reads data another lane wrote: the code's single-thread
 // semantics are the whole story, and the same source built for the CPU
 // (tests/emu) runs exactly this per-instance program.
@@ -27,82 +27,117 @@
 
 namespace pdq {
 
+// Scheduling barrier between the phases of a step: the compiler may not move
+// instructions across it, which keeps one phase's loads from being hoisted
+// into the previous one (long live ranges, register spills).  No code.
+#ifdef PDQ_HOST_EMU
+#define PDQ_PHASE_BOUNDARY() ((void)0)
+#else
+#define PDQ_PHASE_BOUNDARY() __builtin_amdgcn_sched_barrier(0)
+#endif
+
 // Slot scalars.
 enum { LS_T = 0, LS_OS = 1, LS_NSTEPS = 2, LS_MLE_RUN = 3, LS_MLE_N = 4, LS_IDENT = 5, LS_NSC = 8 };
 
-// Element offsets inside one slot.
+// Lower-triangular factors are stored packed, row by row: (i, j <= i) at
+// i (i + 1) / 2 + j.
+__device__ __forceinline__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+
+// Element offsets inside one slot: scalars, mean (N x D), factor (packed),
+// then the backward conditional (CondLayout order).
 template <int N, int D>
 struct SlotLayout {
+  static constexpr int T = N * (N + 1) / 2;
   static constexpr int sc = 0;
   static constexpr int m = LS_NSC;
   static constexpr int L = m + N * D;
-  static constexpr int cA = L + N * N;
+  static constexpr int cA = L + T;
   static constexpr int cm = cA + N * N;
   static constexpr int cL = cm + N * D;
-  static constexpr int ctl = cL + N * N;
+  static constexpr int ctl = cL + T;
   static constexpr int cto = ctl + N;
   static constexpr int size = cto + N;
 };
 
-// Doubles of the workspace for n_inst instances (three slots each); the
-// same sum as SlotLayout<N, D>::size.
-inline size_t lane_ws_doubles(int N, int D, int n_inst) {
-  const int slot = LS_NSC + 2 * N * D + 3 * N * N + 2 * N;
-  return (size_t)3 * slot * (size_t)n_inst;
-}
+// Each lane keeps its instance's three slots in LDS, element-major across the
+// workgroup's 64 lanes (element e of the lane's slot s at
+// lds[(s * size + e) * 64 + lane]): consecutive lanes touch consecutive words
+// and the element stride is a compile-time 512 bytes.
+constexpr int kLaneBlock = 64;
 
-// A backward conditional in latent coordinates (ssm_impl_api.py:35-132).
+// A backward conditional in latent coordinates (ssm_impl_api.py:35-132):
+// A (N x N), noise mean m (N x D), noise factor L (N x N), to_latent tl (N)
+// and to_observed to (N).  LCond holds one in registers; CondRef views one
+// stored element-major (element e at p[e * 64]) in the blocked HBM workspace
+// or in the lane's LDS staging area.  Both expose the same accessors, so the
+// algebra below reads and writes either.
 template <int N, int D>
-struct LCond {
-  double A[N * N], m[N * D], L[N * N], tl[N], to[N];
+struct CondLayout {
+  static constexpr int T = N * (N + 1) / 2;
+  static constexpr int A = 0, m = N * N, L = N * N + N * D, tl = L + T, to = tl + N, size = to + N;
 };
 
-// View of one instance's slots in the element-major workspace.
+template <int N, int D>
+struct LCond {
+  using CL = CondLayout<N, D>;
+  double v[CL::size];
+  __device__ __forceinline__ double& at(int e) { return v[e]; }
+  __device__ __forceinline__ double at(int e) const { return v[e]; }
+  __device__ __forceinline__ double& A(int i, int j) { return v[CL::A + i * N + j]; }
+  __device__ __forceinline__ double& m(int i, int c) { return v[CL::m + i * D + c]; }
+  __device__ __forceinline__ double& tl(int i) { return v[CL::tl + i]; }
+  __device__ __forceinline__ double& to(int i) { return v[CL::to + i]; }
+  __device__ __forceinline__ double A(int i, int j) const { return v[CL::A + i * N + j]; }
+  __device__ __forceinline__ double m(int i, int c) const { return v[CL::m + i * D + c]; }
+  __device__ __forceinline__ double tl(int i) const { return v[CL::tl + i]; }
+  __device__ __forceinline__ double to(int i) const { return v[CL::to + i]; }
+  __device__ __forceinline__ double L(int i, int j) const { return j <= i ? v[CL::L + tri(i, j)] : 0.0; }
+  __device__ __forceinline__ void setL(int i, int j, double x) {
+    if (j <= i) v[CL::L + tri(i, j)] = x;
+  }
+};
+
+template <int N, int D>
+struct CondRef {
+  using CL = CondLayout<N, D>;
+  double* p;
+  __device__ __forceinline__ double& at(int e) const { return p[e * kLaneBlock]; }
+  __device__ __forceinline__ double& A(int i, int j) const { return at(CL::A + i * N + j); }
+  __device__ __forceinline__ double& m(int i, int c) const { return at(CL::m + i * D + c); }
+  __device__ __forceinline__ double& tl(int i) const { return at(CL::tl + i); }
+  __device__ __forceinline__ double& to(int i) const { return at(CL::to + i); }
+  __device__ __forceinline__ double L(int i, int j) const { return j <= i ? at(CL::L + tri(i, j)) : 0.0; }
+  __device__ __forceinline__ void setL(int i, int j, double x) const {
+    if (j <= i) at(CL::L + tri(i, j)) = x;
+  }
+};
+
+// View of one instance's slots in the blocked workspace.
 template <int N, int D>
 struct LaneWs {
   using SL = SlotLayout<N, D>;
-  double* base;  // ws + inst
-  size_t stride;  // n_inst
+  double* base;  // lds + lane
   __device__ __forceinline__ double& at(int slot, int e) const {
-    return base[((size_t)slot * SL::size + e) * stride];
+    return base[(slot * SL::size + e) * kLaneBlock];
+  }
+  __device__ __forceinline__ CondRef<N, D> cond(int slot) const {
+    return CondRef<N, D>{base + (slot * SL::size + SL::cA) * kLaneBlock};
   }
   __device__ __forceinline__ void load_mL(int s, double* m, double* L) const {
 #pragma unroll
     for (int i = 0; i < N * D; ++i) m[i] = at(s, SL::m + i);
 #pragma unroll
-    for (int i = 0; i < N * N; ++i) L[i] = at(s, SL::L + i);
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) L[i * N + j] = j <= i ? at(s, SL::L + tri(i, j)) : 0.0;
   }
   __device__ __forceinline__ void store_mL(int s, const double* m, const double* L) const {
 #pragma unroll
     for (int i = 0; i < N * D; ++i) at(s, SL::m + i) = m[i];
 #pragma unroll
-    for (int i = 0; i < N * N; ++i) at(s, SL::L + i) = L[i];
-  }
-  __device__ __forceinline__ void load_cond(int s, LCond<N, D>& c) const {
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int i = 0; i < N * N; ++i) c.A[i] = at(s, SL::cA + i);
-#pragma unroll
-    for (int i = 0; i < N * D; ++i) c.m[i] = at(s, SL::cm + i);
-#pragma unroll
-    for (int i = 0; i < N * N; ++i) c.L[i] = at(s, SL::cL + i);
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      c.tl[i] = at(s, SL::ctl + i);
-      c.to[i] = at(s, SL::cto + i);
-    }
-  }
-  __device__ __forceinline__ void store_cond(int s, const LCond<N, D>& c) const {
-#pragma unroll
-    for (int i = 0; i < N * N; ++i) at(s, SL::cA + i) = c.A[i];
-#pragma unroll
-    for (int i = 0; i < N * D; ++i) at(s, SL::cm + i) = c.m[i];
-#pragma unroll
-    for (int i = 0; i < N * N; ++i) at(s, SL::cL + i) = c.L[i];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      at(s, SL::ctl + i) = c.tl[i];
-      at(s, SL::cto + i) = c.to[i];
-    }
+      for (int j = 0; j <= i; ++j) at(s, SL::L + tri(i, j)) = L[i * N + j];
   }
   __device__ __forceinline__ void copy_slot(int dst, int src) const {
 #pragma unroll
@@ -110,12 +145,30 @@ struct LaneWs {
   }
 };
 
+template <int N, int D, class Dst, class Src>
+__device__ __forceinline__ void cond_copy(const Dst& d, const Src& s) {
+  using CL = CondLayout<N, D>;
+#pragma unroll
+  for (int e = 0; e < CL::size; ++e) d.at(e) = s.at(e);
+}
+
 // ------------------------------------------------------------- algebra
 // In-place R-only Householder QR of the M x C register matrix A (row-major)
 // in LAPACK dgeqr2 convention (backend/linalg.py:8-37 -> geqrf): on exit the
 // upper triangle of rows 0..min(M,C)-1 holds R, the strict lower part of
 // those rows is zero.
-template <int M, int C>
+//
+// S::nz(i, r) tells, at compile time, whether row r > i can be non-zero in
+// column i when reflection i is formed.  A row that cannot contributes an
+// exact zero to the norm, its Householder coefficient is an exact zero and
+// its update subtracts an exact zero, so skipping it leaves every result
+// bit-identical to the dense dgeqr2 (for finite data) while the unrolled
+// code keeps fewer values live and does fewer FLOPs.
+struct DenseRows {
+  static constexpr bool nz(int, int) { return true; }
+};
+
+template <int M, int C, class S = DenseRows>
 __device__ __forceinline__ void lane_qr(double* A) {
   constexpr int k = M < C ? M : C;
 #pragma unroll
@@ -123,23 +176,27 @@ __device__ __forceinline__ void lane_qr(double* A) {
     if (M - i <= 1) break;
     double ss = 0.0;
 #pragma unroll
-    for (int r = i + 1; r < M; ++r) ss += A[r * C + i] * A[r * C + i];
+    for (int r = i + 1; r < M; ++r)
+      if (S::nz(i, r)) ss += A[r * C + i] * A[r * C + i];
     if (ss != 0.0) {
       const double alpha = A[i * C + i];
       const double beta = -copysign(hypot(alpha, sqrt(ss)), alpha);
       const double tau = (beta - alpha) / beta;
       const double scal = 1.0 / (alpha - beta);
 #pragma unroll
-      for (int r = i + 1; r < M; ++r) A[r * C + i] *= scal;
+      for (int r = i + 1; r < M; ++r)
+        if (S::nz(i, r)) A[r * C + i] *= scal;
 #pragma unroll
       for (int j = i + 1; j < C; ++j) {
         double w = A[i * C + j];
 #pragma unroll
-        for (int r = i + 1; r < M; ++r) w += A[r * C + i] * A[r * C + j];
+        for (int r = i + 1; r < M; ++r)
+          if (S::nz(i, r)) w += A[r * C + i] * A[r * C + j];
         w *= tau;
         A[i * C + j] -= w;
 #pragma unroll
-        for (int r = i + 1; r < M; ++r) A[r * C + j] -= A[r * C + i] * w;
+        for (int r = i + 1; r < M; ++r)
+          if (S::nz(i, r)) A[r * C + j] -= A[r * C + i] * w;
       }
       A[i * C + i] = beta;
     }
@@ -149,6 +206,36 @@ __device__ __forceinline__ void lane_qr(double* A) {
 #pragma unroll
     for (int j = 0; j < i; ++j) A[i * C + j] = 0.0;
 }
+
+// [dense N x N; upper-triangular N x N] stacks (sum_of_sqrtm_factors of a
+// product factor and a Cholesky factor): bottom row N + q is zero in the
+// columns before q, and reflections never fill it in.
+template <int N>
+struct TriStackRows {
+  static constexpr bool nz(int i, int r) { return r < N || r - N <= i; }
+};
+
+// The revert block [[R_YX, 0], [R_X_F, R_X]] with R_YX upper triangular:
+// below the pivot, the left block's top rows stay zero; after the left
+// block, the bottom-right block is dense.
+template <int N>
+struct RevertRows {
+  static constexpr bool nz(int i, int r) { return i >= N || r >= N; }
+};
+
+// The update block [[damp, 0], [(H L)^T, L^T]] of ts0 (H selects derivative
+// 1): column 0 has non-zeros in rows 0..2 only, and after two reflections
+// the block is upper triangular.
+struct UpdateRows {
+  static constexpr bool nz(int i, int r) { return i <= 1 && r <= 2; }
+};
+
+// fx.marginalise's stacked column [(H L)^T; damp] for a prior factor L =
+// |p| (s Q) with Q lower triangular: rows 0, 1 and the damping row.
+template <int N>
+struct FxColumnRows {
+  static constexpr bool nz(int, int r) { return r <= 1 || r == N; }
+};
 
 // preconditioner_taylor (utilities.py:74-84).
 template <int N>
@@ -207,7 +294,7 @@ __device__ __forceinline__ void lane_marginalise_prior(const double* __restrict_
       for (int k = 0; k < N; ++k) acc += A[i * N + k] * (PI[k] * Xm[k * D + col]);
       Ym[i * D + col] = P[i] * (acc + 0.0);
     }
-  lane_qr<2 * N, N>(big);
+  lane_qr<2 * N, N, TriStackRows<N>>(big);
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -215,11 +302,12 @@ __device__ __forceinline__ void lane_marginalise_prior(const double* __restrict_
 }
 
 // revert of the transition (ssm_impl_isotropic.py:107-133 with
-// cholesky_util.py:27-82): observed (Ym, YL) and backward conditional bw.
-template <int N, int D>
+// cholesky_util.py:27-82): observed (Ym, YL) and backward conditional bw
+// (written to any conditional storage).
+template <int N, int D, class Cond>
 __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ A, const double* __restrict__ Q,
                                                   const double* P, const double* PI, double s, const double* Xm,
-                                                  const double* XL, double* Ym, double* YL, LCond<N, D>& bw) {
+                                                  const double* XL, double* Ym, double* YL, const Cond& bw) {
   constexpr int n2 = 2 * N;
   double big[n2 * n2];
   double tmp[N * D];
@@ -244,16 +332,17 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ A, 
       for (int k = 0; k < N; ++k) acc += A[i * N + k] * (PI[k] * Xm[k * D + col]);
       tmp[i * D + col] = acc + 0.0;
     }
-  lane_qr<n2, n2>(big);
-  // gain = solve_triu(R_Y, R12)^T, written as bw.A (N x N)
+  lane_qr<n2, n2, RevertRows<N>>(big);
+  // gain = solve_triu(R_Y, R12)^T
+  double G[N * N];
 #pragma unroll
   for (int j = 0; j < N; ++j)
 #pragma unroll
     for (int i = N - 1; i >= 0; --i) {
       double sacc = big[i * n2 + N + j];
 #pragma unroll
-      for (int c = i + 1; c < N; ++c) sacc -= big[i * n2 + c] * bw.A[j * N + c];
-      bw.A[j * N + i] = sacc / big[i * n2 + i];
+      for (int c = i + 1; c < N; ++c) sacc -= big[i * n2 + c] * G[j * N + c];
+      G[j * N + i] = sacc / big[i * n2 + i];
     }
 #pragma unroll
   for (int i = 0; i < N; ++i)
@@ -261,95 +350,126 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ A, 
     for (int col = 0; col < D; ++col) {
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc += bw.A[i * N + k] * tmp[k * D + col];
-      bw.m[i * D + col] = PI[i] * Xm[i * D + col] - acc;
+      for (int k = 0; k < N; ++k) acc += G[i * N + k] * tmp[k * D + col];
+      bw.m(i, col) = PI[i] * Xm[i * D + col] - acc;
       Ym[i * D + col] = P[i] * tmp[i * D + col];
     }
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      bw.L[i * N + j] = big[(N + j) * n2 + N + i];
+      bw.A(i, j) = G[i * N + j];
+      bw.setL(i, j, big[(N + j) * n2 + N + i]);
       YL[i * N + j] = fabs(P[i]) * big[j * n2 + i];
     }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    bw.tl[i] = 1.0 / P[i];
-    bw.to[i] = 1.0 / PI[i];
+    bw.tl(i) = 1.0 / P[i];
+    bw.to(i) = 1.0 / PI[i];
   }
 }
 
-// merge (ssm_impl_isotropic.py:91-105): out = self o other.  With an
-// identity `self` the general formula reduces exactly to a rescaling (the
-// QR of an upper-triangular factor stacked on zeros is that factor).
-template <int N, int D>
-__device__ __forceinline__ void lane_merge(bool self_ident, const LCond<N, D>& sf, const LCond<N, D>& ot,
-                                           LCond<N, D>& r) {
+// merge (ssm_impl_isotropic.py:91-105): r = self o other.  r may be the same
+// storage as `other` (every part of r is written after the parts of other it
+// depends on have been read).  With an identity `self` the general formula
+// reduces exactly to a rescaling (the QR of an upper-triangular factor
+// stacked on zeros is that factor).
+template <int N, int D, class CS, class CO, class CR>
+__device__ __forceinline__ void lane_merge(bool self_ident, const CS& sf, const CO& ot, const CR& r) {
   if (self_ident) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      const double T = 1.0 * ot.to[i];
+      const double T = 1.0 * ot.to(i);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        r.A[i * N + j] = T * ot.A[i * N + j];
-        r.L[i * N + j] = fabs(T) * ot.L[i * N + j];
+        r.A(i, j) = T * ot.A(i, j);
+        r.setL(i, j, fabs(T) * ot.L(i, j));
       }
 #pragma unroll
-      for (int col = 0; col < D; ++col) r.m[i * D + col] = (1.0 * ot.to[i]) * ot.m[i * D + col] + 0.0;
-      r.tl[i] = ot.tl[i];
-      r.to[i] = 1.0;
+      for (int col = 0; col < D; ++col) r.m(i, col) = (1.0 * ot.to(i)) * ot.m(i, col) + 0.0;
+      r.tl(i) = ot.tl(i);
+      r.to(i) = 1.0;
     }
     return;
   }
-  double big[2 * N * N];
+  double T[N], sA[N * N];
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int k = 0; k < N; ++k) T[k] = sf.tl(k) * ot.to(k);
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double acc = 0.0, acc2 = 0.0;
+  for (int e = 0; e < N * N; ++e) sA[e] = sf.A(e / N, e % N);
+  {
+    double oA[N * N], rA[N * N];
 #pragma unroll
-      for (int k = 0; k < N; ++k) {
-        const double T = sf.tl[k] * ot.to[k];
-        acc += sf.A[i * N + k] * (T * ot.A[k * N + j]);
-        acc2 += sf.A[i * N + k] * (fabs(T) * ot.L[k * N + j]);
+    for (int e = 0; e < N * N; ++e) oA[e] = ot.A(e / N, e % N);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) acc += sA[i * N + k] * (T[k] * oA[k * N + j]);
+        rA[i * N + j] = acc;
       }
-      r.A[i * N + j] = acc;
-      big[j * N + i] = acc2;
-      big[(N + j) * N + i] = sf.L[i * N + j];
-    }
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+    for (int e = 0; e < N * N; ++e) r.A(e / N, e % N) = rA[e];
+  }
+  {
+    double om[N * D], rm[N * D];
 #pragma unroll
-    for (int col = 0; col < D; ++col) {
-      double acc = 0.0;
+    for (int e = 0; e < N * D; ++e) om[e] = ot.m(e / D, e % D);
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc += sf.A[i * N + k] * ((sf.tl[k] * ot.to[k]) * ot.m[k * D + col]);
-      r.m[i * D + col] = acc + sf.m[i * D + col];
-    }
-  lane_qr<2 * N, N>(big);
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int col = 0; col < D; ++col) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) acc += sA[i * N + k] * (T[k] * om[k * D + col]);
+        rm[i * D + col] = acc + sf.m(i, col);
+      }
+#pragma unroll
+    for (int e = 0; e < N * D; ++e) r.m(e / D, e % D) = rm[e];
+  }
+  {
+    double big[2 * N * N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double acc2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) acc2 += sA[i * N + k] * (fabs(T[k]) * ot.L(k, j));
+        big[j * N + i] = acc2;
+        big[(N + j) * N + i] = sf.L(i, j);
+      }
+    lane_qr<2 * N, N, TriStackRows<N>>(big);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) r.setL(i, j, big[j * N + i]);
+  }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-#pragma unroll
-    for (int j = 0; j < N; ++j) r.L[i * N + j] = big[j * N + i];
-    r.tl[i] = ot.tl[i];
-    r.to[i] = sf.to[i];
+    r.tl(i) = ot.tl(i);
+    r.to(i) = sf.to(i);
   }
 }
 
 // marginalise of a backward conditional (estimators_and_losses.py:156-178).
-template <int N, int D>
-__device__ __forceinline__ void lane_marginalise_cond(const LCond<N, D>& c, const double* Xm, const double* XL,
-                                                      double* Ym, double* YL) {
-  double big[2 * N * N];
+template <int N, int D, class Cond>
+__device__ __forceinline__ void lane_marginalise_cond(const Cond& c, const double* Xm, const double* XL, double* Ym,
+                                                      double* YL) {
+  double big[2 * N * N], tl[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) tl[k] = c.tl(k);
 #pragma unroll
   for (int r = 0; r < N; ++r)
 #pragma unroll
     for (int col = 0; col < N; ++col) {
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc += c.A[col * N + k] * (c.tl[k] * XL[k * N + r]);
+      for (int k = 0; k < N; ++k) acc += c.A(col, k) * (tl[k] * XL[k * N + r]);
       big[r * N + col] = acc;
-      big[(N + r) * N + col] = c.L[col * N + r];
+      big[(N + r) * N + col] = c.L(col, r);
     }
 #pragma unroll
   for (int i = 0; i < N; ++i)
@@ -357,14 +477,14 @@ __device__ __forceinline__ void lane_marginalise_cond(const LCond<N, D>& c, cons
     for (int col = 0; col < D; ++col) {
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc += c.A[i * N + k] * (c.tl[k] * Xm[k * D + col]);
-      Ym[i * D + col] = c.to[i] * (acc + c.m[i * D + col]);
+      for (int k = 0; k < N; ++k) acc += c.A(i, k) * (tl[k] * Xm[k * D + col]);
+      Ym[i * D + col] = c.to(i) * (acc + c.m(i, col));
     }
-  lane_qr<2 * N, N>(big);
+  lane_qr<2 * N, N, TriStackRows<N>>(big);
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < N; ++j) YL[i * N + j] = fabs(c.to[i]) * big[j * N + i];
+    for (int j = 0; j < N; ++j) YL[i * N + j] = fabs(c.to(i)) * big[j * N + i];
 }
 
 // ts0 linearisation (ssm_impl_isotropic.py:298-317): b = -f(u(t)).
@@ -388,7 +508,7 @@ __device__ __forceinline__ void lane_marginalise_fx(const pdq_config& cfg, const
   col[N] = cfg.damp;
 #pragma unroll
   for (int c = 0; c < D; ++c) om[c] = Xm[1 * D + c] + b[c];
-  lane_qr<N + 1, 1>(col);
+  lane_qr<N + 1, 1, FxColumnRows<N>>(col);
   oL = col[0];
 }
 
@@ -421,7 +541,7 @@ __device__ __forceinline__ void lane_update(const pdq_config& cfg, const double*
     }
 #pragma unroll
   for (int c = 0; c < D; ++c) mo[c] = Pm[1 * D + c] + b[c];
-  lane_qr<n2, n2>(big);
+  lane_qr<n2, n2, UpdateRows>(big);
   double gain[N];
 #pragma unroll
   for (int j = 0; j < N; ++j) gain[j] = big[1 + j] / big[0];
@@ -515,17 +635,15 @@ __device__ __forceinline__ double lane_step(const LaneCtx<N, D>& c, int from, in
   auto predict = [&](double s) {
     if (!smoother) {
       lane_marginalise_prior<N, D>(A, Q, P, PI, s, Sm, SLf, Pm, PL);
-      return;
-    }
-    LCond<N, D> bw, rc;
-    lane_revert_prior<N, D>(A, Q, P, PI, s, Sm, SLf, Pm, PL, bw);
-    if (cfg.strategy == PDQ_FIXEDPOINT) {
-      LCond<N, D> sc;
-      if (!ident) w.load_cond(from, sc);
-      lane_merge<N, D>(ident, sc, bw, rc);
-      w.store_cond(to, rc);
+    } else if (cfg.strategy == PDQ_FIXEDPOINT) {
+      // revert into proposed's conditional, then merge step_from's in front of
+      // it, in place
+      lane_revert_prior<N, D>(A, Q, P, PI, s, Sm, SLf, Pm, PL, w.cond(to));
+      PDQ_PHASE_BOUNDARY();
+      lane_merge<N, D>(ident, w.cond(from), w.cond(to), w.cond(to));
+      PDQ_PHASE_BOUNDARY();
     } else {
-      w.store_cond(to, bw);
+      lane_revert_prior<N, D>(A, Q, P, PI, s, Sm, SLf, Pm, PL, w.cond(to));
     }
   };
   if (cfg.solver == PDQ_SOLVER_DYNAMIC) {
@@ -540,6 +658,7 @@ __device__ __forceinline__ double lane_step(const LaneCtx<N, D>& c, int from, in
     double om[D], oL;
     lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oL);
     sigma = lane_rms_zero<D>(om, oL);
+    PDQ_PHASE_BOUNDARY();
     predict(lane_noise_scale(cfg, dt, sigma));
     double mo[D], oLu;
     lane_update<N, D>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
@@ -597,32 +716,35 @@ __device__ __forceinline__ void lane_write_normal(const double* m, const double*
     for (int i = 0; i < N * N; ++i) gL[i] = L[i];
 }
 
-template <int N, int D>
-__device__ __forceinline__ void lane_write_cond(const LaneCtx<N, D>& c, bool ident, const LCond<N, D>& s, int j) {
+// Backward conditional j of the instance's output (identity when `ident`).
+template <int N, int D, class Cond>
+__device__ __forceinline__ void lane_write_cond(const LaneCtx<N, D>& c, bool ident, const Cond& s, int j) {
   const pdq_solution& o = c.a->out;
   const size_t idx = (size_t)c.inst * (c.a->K - 1) + j;
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      o.cond_A[idx * N * N + i * N + k] = ident ? (i == k ? 1.0 : 0.0) : s.A[i * N + k];
-      o.cond_chol[idx * N * N + i * N + k] = ident ? 0.0 : s.L[i * N + k];
+      o.cond_A[idx * N * N + i * N + k] = ident ? (i == k ? 1.0 : 0.0) : s.A(i, k);
+      o.cond_chol[idx * N * N + i * N + k] = ident ? 0.0 : s.L(i, k);
     }
 #pragma unroll
-  for (int e = 0; e < N * D; ++e) o.cond_mean[idx * N * D + e] = ident ? 0.0 : s.m[e];
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) o.cond_mean[idx * N * D + i * D + col] = ident ? 0.0 : s.m(i, col);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    o.cond_tl[idx * N + i] = ident ? 1.0 : s.tl[i];
-    o.cond_to[idx * N + i] = ident ? 1.0 : s.to[i];
+    o.cond_tl[idx * N + i] = ident ? 1.0 : s.tl(i);
+    o.cond_to[idx * N + i] = ident ? 1.0 : s.to(i);
   }
 }
 
-// Writes checkpoint k (the filter's marginal, or the smoothers' filtering
-// marginal and the backward conditional that leads to it).
-template <int N, int D>
+// Writes checkpoint k: the filter's marginal, or the smoothers' filtering
+// marginal (and, with `cond`, the backward conditional that leads to it).
+template <int N, int D, class Cond>
 __device__ __forceinline__ void lane_write_checkpoint(const LaneCtx<N, D>& c, int k, double t, const double* m,
-                                                      const double* L, double os, double nsteps, bool has_cond,
-                                                      bool cond_ident, const LCond<N, D>& cond) {
+                                                      const double* L, double os, double nsteps, const Cond* cond,
+                                                      bool cond_ident) {
   const pdq_solution& o = c.a->out;
   const int K = c.a->K;
   const size_t row = (size_t)c.inst * K + k;
@@ -633,7 +755,7 @@ __device__ __forceinline__ void lane_write_checkpoint(const LaneCtx<N, D>& c, in
   if (cfg.strategy != PDQ_FILTER) {
     lane_write_normal<N, D>(m, L, o.filt_mean ? o.filt_mean + row * N * D : nullptr,
                             o.filt_chol ? o.filt_chol + row * N * N : nullptr);
-    if (k > 0 && has_cond) lane_write_cond<N, D>(c, cond_ident, cond, k - 1);
+    if (k > 0 && cond) lane_write_cond<N, D>(c, cond_ident, *cond, k - 1);
   } else {
     lane_write_normal<N, D>(m, L, o.mean ? o.mean + row * N * D : nullptr, o.chol ? o.chol + row * N * N : nullptr);
   }
@@ -641,15 +763,12 @@ __device__ __forceinline__ void lane_write_checkpoint(const LaneCtx<N, D>& c, in
 
 // Checkpoint k from slot s (its own marginal and backward conditional).
 template <int N, int D>
-__device__ __forceinline__ void lane_write_slot(const LaneCtx<N, D>& c, int k, int s, bool with_cond) {
+__device__ __forceinline__ void lane_write_slot(const LaneCtx<N, D>& c, int k, int s) {
   double m[N * D], L[N * N];
   c.w.load_mL(s, m, L);
-  LCond<N, D> cond;
-  const bool smoother = c.a->cfg.strategy != PDQ_FILTER;
-  const bool ident = c.w.at(s, LS_IDENT) != 0.0;
-  if (smoother && with_cond && k > 0 && !ident) c.w.load_cond(s, cond);
-  lane_write_checkpoint<N, D>(c, k, c.w.at(s, LS_T), m, L, c.w.at(s, LS_OS), c.w.at(s, LS_NSTEPS), with_cond, ident,
-                              cond);
+  const CondRef<N, D> cond = c.w.cond(s);
+  lane_write_checkpoint<N, D>(c, k, c.w.at(s, LS_T), m, L, c.w.at(s, LS_OS), c.w.at(s, LS_NSTEPS), &cond,
+                              c.w.at(s, LS_IDENT) != 0.0);
 }
 
 // ProbabilisticSolver.interpolate_fwd (solvers.py:205-269): from = interp_from,
@@ -670,38 +789,34 @@ __device__ __forceinline__ void lane_interp_beyond(const LaneCtx<N, D>& c, int k
   const double s0 = lane_noise_scale(cfg, dt0, os);
   if (cfg.strategy == PDQ_FILTER) {
     lane_marginalise_prior<N, D>(A, Q, P, PI, s0, Fm, FL, Pm, PL);
-    LCond<N, D> none;
-    lane_write_checkpoint<N, D>(c, k, t1, Pm, PL, os, nsteps, false, false, none);
+    lane_write_checkpoint<N, D, CondRef<N, D>>(c, k, t1, Pm, PL, os, nsteps, nullptr, false);
     w.store_mL(from, Pm, PL);
     w.at(from, LS_T) = t1;
     return;
   }
-  LCond<N, D> bw, ext;
-  lane_revert_prior<N, D>(A, Q, P, PI, s0, Fm, FL, Pm, PL, bw);
-  const bool from_ident = w.at(from, LS_IDENT) != 0.0;
+  // extrapolate t0 -> t: ext_t = (marginal Pm/PL, conditional ext).  ext is
+  // built in interp_from's conditional, which it replaces: fixed point
+  // ext = interp_from.cond o bw (bw staged in step_from's conditional, which
+  // is rewritten below); fixed interval ext = bw.
   if (cfg.strategy == PDQ_FIXEDPOINT) {
-    LCond<N, D> fc;
-    if (!from_ident) w.load_cond(from, fc);
-    lane_merge<N, D>(from_ident, fc, bw, ext);
+    lane_revert_prior<N, D>(A, Q, P, PI, s0, Fm, FL, Pm, PL, w.cond(to));
+    PDQ_PHASE_BOUNDARY();
+    lane_merge<N, D>(w.at(from, LS_IDENT) != 0.0, w.cond(from), w.cond(to), w.cond(from));
   } else {
-    ext = bw;
+    lane_revert_prior<N, D>(A, Q, P, PI, s0, Fm, FL, Pm, PL, w.cond(from));
   }
-  // extrapolate t -> t1 from ext_t's marginal; only the backward conditional is kept
+  PDQ_PHASE_BOUNDARY();
+  // extrapolate t -> t1 from ext_t's marginal; only the backward conditional
+  // is kept, as step_from's new conditional (fixed point: identity o bw)
   lane_precon<N>(dt1, P, PI);
   const double s1 = lane_noise_scale(cfg, dt1, os);
   double Ym[N * D], YL[N * N];
-  LCond<N, D> bw2;
-  lane_revert_prior<N, D>(A, Q, P, PI, s1, Pm, PL, Ym, YL, bw2);
-  if (cfg.strategy == PDQ_FIXEDPOINT) {
-    LCond<N, D> unused, rc;
-    lane_merge<N, D>(true, unused, bw2, rc);
-    w.store_cond(to, rc);
-  } else {
-    w.store_cond(to, bw2);
-  }
-  lane_write_checkpoint<N, D>(c, k, t1, Pm, PL, os, nsteps, true, false, ext);
+  lane_revert_prior<N, D>(A, Q, P, PI, s1, Pm, PL, Ym, YL, w.cond(to));
+  if (cfg.strategy == PDQ_FIXEDPOINT) lane_merge<N, D>(true, w.cond(to), w.cond(to), w.cond(to));
+  const CondRef<N, D> ext = w.cond(from);
+  lane_write_checkpoint<N, D>(c, k, t1, Pm, PL, os, nsteps, &ext, false);
+  // new interp_from: fixed point (ext_t.marginal, identity); fixed interval ext_t
   w.store_mL(from, Pm, PL);
-  if (cfg.strategy == PDQ_FIXEDINTERVAL) w.store_cond(from, ext);
   w.at(from, LS_T) = t1;
   w.at(from, LS_IDENT) = cfg.strategy == PDQ_FIXEDPOINT ? 1.0 : 0.0;
   w.at(to, LS_IDENT) = 0.0;
@@ -710,7 +825,7 @@ __device__ __forceinline__ void lane_interp_beyond(const LaneCtx<N, D>& c, int k
 // ProbabilisticSolver.interpolate_fwd_at_t1 (solvers.py:271-315).
 template <int N, int D>
 __device__ __forceinline__ void lane_interp_at(const LaneCtx<N, D>& c, int k, int from, int to) {
-  lane_write_slot<N, D>(c, k, to, true);
+  lane_write_slot<N, D>(c, k, to);
   c.w.copy_slot(from, to);
   if (c.a->cfg.strategy == PDQ_FIXEDPOINT) {
     c.w.at(to, LS_IDENT) = 1.0;
@@ -746,18 +861,18 @@ __device__ __forceinline__ void lane_finalize(const LaneCtx<N, D>& c, int sf) {
   }
   double Sm[N * D], SLf[N * N];
   c.w.load_mL(sf, Sm, SLf);
-  LCond<N, D> cond;
+  const CondRef<N, D> cond = c.w.cond(sf);
   const bool ident = c.w.at(sf, LS_IDENT) != 0.0;
-  if (!ident) c.w.load_cond(sf, cond);
   if (mle) {
     for (size_t e = 0; e < (size_t)K * N * N; ++e) o.filt_chol[inst * K * N * N + e] = scale * o.filt_chol[inst * K * N * N + e];
     for (size_t e = 0; e < (size_t)Km1 * N * N; ++e)
       o.cond_chol[inst * Km1 * N * N + e] = scale * o.cond_chol[inst * Km1 * N * N + e];
 #pragma unroll
-    for (int e = 0; e < N * N; ++e) {
-      SLf[e] = scale * SLf[e];
-      cond.L[e] = scale * cond.L[e];
-    }
+    for (int e = 0; e < N * N; ++e) SLf[e] = scale * SLf[e];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) cond.setL(i, j, scale * cond.L(i, j));
   }
   // rv_at_t1 = posterior1.conditional.marginalise(posterior1.marginal)
   double Xm[N * D], XL[N * N];
@@ -778,16 +893,20 @@ __device__ __forceinline__ void lane_finalize(const LaneCtx<N, D>& c, int sf) {
     const size_t idx = inst * Km1 + j;
     LCond<N, D> cj;
 #pragma unroll
-    for (int e = 0; e < N * N; ++e) {
-      cj.A[e] = o.cond_A[idx * N * N + e];
-      cj.L[e] = o.cond_chol[idx * N * N + e];
-    }
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int e = 0; e < N * D; ++e) cj.m[e] = o.cond_mean[idx * N * D + e];
+      for (int k = 0; k < N; ++k) {
+        cj.A(i, k) = o.cond_A[idx * N * N + i * N + k];
+        cj.setL(i, k, o.cond_chol[idx * N * N + i * N + k]);
+      }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int col = 0; col < D; ++col) cj.m(i, col) = o.cond_mean[idx * N * D + i * D + col];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      cj.tl[i] = o.cond_tl[idx * N + i];
-      cj.to[i] = o.cond_to[idx * N + i];
+      cj.tl(i) = o.cond_tl[idx * N + i];
+      cj.to(i) = o.cond_to[idx * N + i];
     }
     double Ym[N * D], YL[N * N];
     lane_marginalise_cond<N, D>(cj, Xm, XL, Ym, YL);
@@ -801,15 +920,15 @@ __device__ __forceinline__ void lane_finalize(const LaneCtx<N, D>& c, int sf) {
 }
 
 // One instance: init, the save_at / fixed-grid loop, finalize, counters.
+// `slots` is the lane's slot storage (its LDS column on the device).
 template <int N, int D>
-__device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst) {
+__device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, double* slots) {
   const pdq_config& cfg = a.cfg;
   LaneCtx<N, D> c;
   c.a = &a;
   c.inst = inst;
   c.prm = a.params + (size_t)inst * cfg.n_params;
-  c.w.base = a.ws + inst;
-  c.w.stride = (size_t)a.n_inst;
+  c.w.base = slots;
   const LaneWs<N, D>& w = c.w;
   const int K = a.K;
   const double* ts = a.times + (size_t)inst * K;
@@ -832,8 +951,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst) {
     w.at(sf, LS_NSTEPS) = 0.0;
     w.at(sf, LS_IDENT) = cfg.strategy != PDQ_FILTER ? 1.0 : 0.0;
     w.copy_slot(ifr, sf);
-    LCond<N, D> none;
-    lane_write_checkpoint<N, D>(c, 0, ts[0], m, L, 1.0, 0.0, false, false, none);
+    lane_write_checkpoint<N, D, CondRef<N, D>>(c, 0, ts[0], m, L, 1.0, 0.0, nullptr, false);
   }
   long attempts = 0, accepted = 0;
   int status = PDQ_STATUS_OK;
@@ -842,7 +960,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst) {
       lane_step<N, D>(c, sf, pr, ts[k] - ts[k - 1], false);
       ++attempts;
       ++accepted;
-      lane_write_slot<N, D>(c, k, pr, true);
+      lane_write_slot<N, D>(c, k, pr);
       const int tmp = sf;
       sf = pr;
       pr = tmp;
@@ -860,7 +978,9 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst) {
             double dtc = dt;
             const double tsf = w.at(sf, LS_T);
             if (cfg.clip_dt) dtc = fmin(dtc, t1 - tsf);
+            PDQ_PHASE_BOUNDARY();
             ep = lane_step<N, D>(c, sf, pr, dtc, true);
+            PDQ_PHASE_BOUNDARY();
             dt = lane_control(cfg, dtc, cstate, ep);
             ++att;
             ++attempts;
@@ -908,13 +1028,23 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst) {
   if (o.status) o.status[inst] = status;
 }
 
+// LDS bytes of one 64-lane workgroup: three slots per lane.
+template <int N, int D>
+constexpr size_t lane_lds_bytes() {
+  return sizeof(double) * kLaneBlock * 3 * SlotLayout<N, D>::size;
+}
+
 #ifndef PDQ_HOST_EMU
-// One lane per instance, grid-stride over instances.
+// One lane per instance, grid-stride over instances.  Each lane's slots live
+// in LDS, element-major across the workgroup's lanes, read and written by
+// that lane only.
 template <int N, int D>
 __global__ void __launch_bounds__(64) lane_kernel(LArgs args) {
+  extern __shared__ double lds[];
+  double* const slots = lds + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   for (int inst = blockIdx.x * blockDim.x + threadIdx.x; inst < args.n_inst; inst += stride)
-    lane_solve_instance<N, D>(args, inst);
+    lane_solve_instance<N, D>(args, inst, slots);
 }
 #endif
 

@@ -8,20 +8,20 @@ namespace pdq {
 template <int N, int D>
 int launch_lane_nd(LArgs& a, hipStream_t st) {
   auto kern = lane_kernel<N, D>;
+  const size_t lds = lane_lds_bytes<N, D>();
+  PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = 0;
-  PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
-  if (per_cu < 1) per_cu = 1;
+  PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
+  if (per_cu < 1) return host_error(PDQ_ERR_UNSUPPORTED, "lane kernel does not fit (%zu bytes of LDS)", lds);
   const long need = ((long)a.n_inst + 63) / 64;
   const long cap = (long)per_cu * num_cus();
   const int grid = (int)std::max<long>(1, std::min(need, cap));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, st, a);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, a);
   PDQ_HIP_TRY(hipGetLastError());
   return PDQ_OK;
 }
 
 bool lane_supported(int N, int D) { return D == 2 && (N == 4 || N == 5); }
-
-size_t lane_workspace_doubles(int N, int D, int n_inst) { return lane_ws_doubles(N, D, n_inst); }
 
 int launch_lane(LArgs& a, int N, int D, hipStream_t st) {
   if (D == 2) {

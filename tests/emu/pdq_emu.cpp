@@ -26,7 +26,6 @@ int pdqemu_solve_lane(const pdq_config* cfg, int n_inst, const double* params, c
   if (cfg->factorisation != PDQ_ISOTROPIC) return 2;
   std::vector<double> A, Q;
   pdq::iwp_matrices(cfg->num_derivatives, A, Q);
-  std::vector<double> ws(pdq::lane_ws_doubles(N, D, n_inst));
   pdq::LArgs a{};
   a.cfg = *cfg;
   a.n_inst = n_inst;
@@ -37,11 +36,12 @@ int pdqemu_solve_lane(const pdq_config* cfg, int n_inst, const double* params, c
   a.times = times;
   a.A = A.data();
   a.Q = Q.data();
-  a.ws = ws.data();
   a.out = *out;
+  // the lane's slots (its LDS column on the device), element stride 64
+  std::vector<double> slots(64 * 3 * 128);
   for (int i = 0; i < n_inst; ++i) {
-    if (N == 4 && D == 2) pdq::lane_solve_instance<4, 2>(a, i);
-    else if (N == 5 && D == 2) pdq::lane_solve_instance<5, 2>(a, i);
+    if (N == 4 && D == 2) pdq::lane_solve_instance<4, 2>(a, i, slots.data());
+    else if (N == 5 && D == 2) pdq::lane_solve_instance<5, 2>(a, i, slots.data());
     else return 2;
   }
   return 0;
