@@ -613,62 +613,74 @@ struct LaneCtx {
   int inst;
 };
 
-// solver_dynamic.step (solvers.py:549-599), solver.step (:702-733) and
-// solver_mle.step (:395-437) with the residual error estimate; slot `from`
-// -> slot `to`.  Returns error_power (meaningful with `estimate`).
+// The three parts of a step attempt: solver_dynamic.step (solvers.py:549-599),
+// solver.step (:702-733) and solver_mle.step (:395-437) with the residual
+// error estimate, from slot `from` to slot `to`.
+//
+// (1) Calibration (solver_dynamic only): extrapolate the mean with output
+// scale 1, linearise there, sigma = the whitened residual's RMS.
 template <int N, int D>
-__device__ __forceinline__ double lane_step(const LaneCtx<N, D>& c, int from, int to, double dt, bool estimate) {
+__device__ __forceinline__ void lane_calibrate(const LaneCtx<N, D>& c, const double* Sm, const double* P,
+                                               const double* PI, double h, double tn, double* b, double& sigma,
+                                               double& oL) {
+  const pdq_config& cfg = c.a->cfg;
+  const double* __restrict__ A = c.a->A;
+  const double* __restrict__ Q = c.a->Q;
+  const double s1 = lane_noise_scale(cfg, h, 1.0);
+  double Um[N * D], UL[N * N];
+  lane_apply_flat_mean<N, D>(A, P, PI, Sm, Um);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * Q[i * N + j]);
+  lane_linearize<N, D>(cfg, c.prm, Um, tn, b);
+  double om[D];
+  lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oL);
+  sigma = lane_rms_zero<D>(om, oL);
+}
+
+// (2) The prediction, shared by step attempts and checkpoint interpolation:
+// the filter marginalises the transition; the smoothers revert it into the
+// conditional of slot `bws`, and the fixed-point smoother then merges the
+// conditional of slot `mself` (identity when `mident`) in front of it,
+// writing slot `mres` (which may be `bws` or `mself`).
+template <int N, int D>
+__device__ __forceinline__ void lane_predict(const LaneCtx<N, D>& c, const double* P, const double* PI, double s,
+                                             const double* Xm, const double* XL, double* Pm, double* PL, int bws,
+                                             bool merge, bool mident, int mself, int mres) {
+  const pdq_config& cfg = c.a->cfg;
+  const double* __restrict__ A = c.a->A;
+  const double* __restrict__ Q = c.a->Q;
+  if (cfg.strategy == PDQ_FILTER) {
+    lane_marginalise_prior<N, D>(A, Q, P, PI, s, Xm, XL, Pm, PL);
+    return;
+  }
+  lane_revert_prior<N, D>(A, Q, P, PI, s, Xm, XL, Pm, PL, c.w.cond(bws));
+  PDQ_PHASE_BOUNDARY();
+  if (merge) lane_merge<N, D>(mident, c.w.cond(mself), c.w.cond(bws), c.w.cond(mres));
+  PDQ_PHASE_BOUNDARY();
+}
+
+// (3) The update, the MLE running scale, the error estimate and the proposed
+// solution's slot.  b, sigma and oL come from (1) for solver_dynamic.
+// Returns error_power (meaningful with `estimate`).
+template <int N, int D>
+__device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int from, int to, double h, double tn,
+                                                   const double* P, const double* PI, const double* Sm,
+                                                   const double* Pm, const double* PL, double* b, double sigma,
+                                                   double oL, bool estimate) {
   const pdq_config& cfg = c.a->cfg;
   const double* __restrict__ A = c.a->A;
   const double* __restrict__ Q = c.a->Q;
   const LaneWs<N, D>& w = c.w;
-  double Sm[N * D], SLf[N * N];
-  w.load_mL(from, Sm, SLf);
-  const double t = w.at(from, LS_T);
-  const double tn = t + dt;
-  const bool smoother = cfg.strategy != PDQ_FILTER;
-  const bool ident = w.at(from, LS_IDENT) != 0.0;
-  double P[N], PI[N];
-  lane_precon<N>(dt, P, PI);
-  double sigma = 1.0, ep = 0.0;
-  double Pm[N * D], PL[N * N], Rm[N * D], RL[N * N], b[D];
-  auto predict = [&](double s) {
-    if (!smoother) {
-      lane_marginalise_prior<N, D>(A, Q, P, PI, s, Sm, SLf, Pm, PL);
-    } else if (cfg.strategy == PDQ_FIXEDPOINT) {
-      // revert into proposed's conditional, then merge step_from's in front of
-      // it, in place
-      lane_revert_prior<N, D>(A, Q, P, PI, s, Sm, SLf, Pm, PL, w.cond(to));
-      PDQ_PHASE_BOUNDARY();
-      lane_merge<N, D>(ident, w.cond(from), w.cond(to), w.cond(to));
-      PDQ_PHASE_BOUNDARY();
-    } else {
-      lane_revert_prior<N, D>(A, Q, P, PI, s, Sm, SLf, Pm, PL, w.cond(to));
-    }
-  };
+  double Rm[N * D], RL[N * N], mo[D], oLu, ep = 0.0;
   if (cfg.solver == PDQ_SOLVER_DYNAMIC) {
-    const double s1 = lane_noise_scale(cfg, dt, 1.0);
-    double Um[N * D], UL[N * N];
-    lane_apply_flat_mean<N, D>(A, P, PI, Sm, Um);
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * Q[i * N + j]);
-    lane_linearize<N, D>(cfg, c.prm, Um, tn, b);
-    double om[D], oL;
-    lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oL);
-    sigma = lane_rms_zero<D>(om, oL);
-    PDQ_PHASE_BOUNDARY();
-    predict(lane_noise_scale(cfg, dt, sigma));
-    double mo[D], oLu;
     lane_update<N, D>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
-    if (estimate) ep = lane_error_power<N, D>(cfg, oL, sigma, Sm, Rm, dt);
+    if (estimate) ep = lane_error_power<N, D>(cfg, oL, sigma, Sm, Rm, h);
   } else {
-    const double s1 = lane_noise_scale(cfg, dt, 1.0);
-    predict(s1);
+    const double s1 = lane_noise_scale(cfg, h, 1.0);
     lane_linearize<N, D>(cfg, c.prm, Pm, tn, b);
     const bool mle = cfg.solver == PDQ_SOLVER_MLE;
-    double mo[D], oLu;
     lane_update<N, D>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
     if (mle) {
       // bayes_rule_and_residual_whitened_rms_tree: observed = (mo, R_Y^T)
@@ -686,10 +698,10 @@ __device__ __forceinline__ double lane_step(const LaneCtx<N, D>& c, int from, in
       for (int i = 0; i < N; ++i)
 #pragma unroll
         for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * Q[i * N + j]);
-      double om[D], oL;
-      lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oL);
-      const double se = lane_rms_zero<D>(om, oL);
-      ep = lane_error_power<N, D>(cfg, oL, se, Sm, Rm, dt);
+      double om[D], oLe;
+      lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oLe);
+      const double se = lane_rms_zero<D>(om, oLe);
+      ep = lane_error_power<N, D>(cfg, oLe, se, Sm, Rm, h);
     }
     sigma = mle ? w.at(from, LS_OS) : 1.0;
   }
@@ -703,6 +715,23 @@ __device__ __forceinline__ double lane_step(const LaneCtx<N, D>& c, int from, in
     w.at(to, LS_MLE_N) = 0.0;
   }
   return ep;
+}
+
+// One step attempt from slot `from` to slot `to` (the three parts in order).
+template <int N, int D>
+__device__ __forceinline__ double lane_step(const LaneCtx<N, D>& c, int from, int to, double h, bool estimate) {
+  const pdq_config& cfg = c.a->cfg;
+  const LaneWs<N, D>& w = c.w;
+  double Sm[N * D], SLf[N * N], P[N], PI[N], b[D], Pm[N * D], PL[N * N];
+  w.load_mL(from, Sm, SLf);
+  const double tn = w.at(from, LS_T) + h;
+  lane_precon<N>(h, P, PI);
+  double sigma = 1.0, oL = 0.0;
+  if (cfg.solver == PDQ_SOLVER_DYNAMIC) lane_calibrate<N, D>(c, Sm, P, PI, h, tn, b, sigma, oL);
+  PDQ_PHASE_BOUNDARY();
+  lane_predict<N, D>(c, P, PI, lane_noise_scale(cfg, h, sigma), Sm, SLf, Pm, PL, to,
+                     cfg.strategy == PDQ_FIXEDPOINT, w.at(from, LS_IDENT) != 0.0, from, to);
+  return lane_finish_step<N, D>(c, from, to, h, tn, P, PI, Sm, Pm, PL, b, sigma, oL, estimate);
 }
 
 // ---------------------------------------------------------------- outputs
@@ -966,58 +995,118 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
       pr = tmp;
     }
   } else {
+    // The adaptive loop (solvers_via_adaptive_steps.py:46-375) as a state
+    // machine: every iteration performs one operation per lane -- a step
+    // attempt, one half of interpolate_fwd beyond t1, or
+    // interpolate_fwd_at_t1 -- in the reference's order.  Step attempts and
+    // both halves of the interpolation share one prediction (revert + merge),
+    // so lanes of a wavefront that are at different points of their solves
+    // execute it together instead of one after the other.
+    enum { OP_STEP = 0, OP_BEYOND1 = 1, OP_BEYOND2 = 2, OP_AT = 3 };
+    const bool smoother = cfg.strategy != PDQ_FILTER;
     double dt = cfg.dt0, cstate = 1.0;
-    for (int k = 1; k < K && status == PDQ_STATUS_OK; ++k) {
+    int k = 1, att = 0;
+    bool second_half = false;
+    while (k < K && status == PDQ_STATUS_OK) {
       const double t1 = ts[k];
-      bool cont = true;
-      while (cont) {
-        if (w.at(sf, LS_T) + cfg.eps < t1) {
-          double ep = 0.9;
-          int att = 0;
-          while (ep < 1.0) {
-            double dtc = dt;
-            const double tsf = w.at(sf, LS_T);
-            if (cfg.clip_dt) dtc = fmin(dtc, t1 - tsf);
-            PDQ_PHASE_BOUNDARY();
-            ep = lane_step<N, D>(c, sf, pr, dtc, true);
-            PDQ_PHASE_BOUNDARY();
-            dt = lane_control(cfg, dtc, cstate, ep);
-            ++att;
-            ++attempts;
-            if (!isfinite(ep) || !isfinite(dt)) {
-              status = PDQ_STATUS_NONFINITE;
-              break;
-            }
-            if (ep < 1.0 && (dt == 0.0 || tsf + dt == tsf)) {
-              status = PDQ_STATUS_DT_UNDERFLOW;
-              break;
-            }
-            if (ep >= 1.0 && w.at(pr, LS_T) == tsf) {
-              status = PDQ_STATUS_DT_UNDERFLOW;  // accepted without advancing t
-              break;
-            }
-            if ((cfg.max_attempts > 0 && att >= cfg.max_attempts && ep < 1.0) || attempts >= kMaxTotalAttempts) {
-              status = PDQ_STATUS_MAX_ATTEMPTS;
-              break;
-            }
-          }
-          if (status != PDQ_STATUS_OK) break;
+      const double tsf = w.at(sf, LS_T);
+      int op;
+      if (second_half) op = OP_BEYOND2;
+      else if (tsf + cfg.eps < t1) op = OP_STEP;
+      else if (tsf > t1 + cfg.eps) op = OP_BEYOND1;
+      else op = OP_AT;
+      if (op == OP_AT) {
+        lane_interp_at<N, D>(c, k, ifr, sf);
+        ++k;
+        continue;
+      }
+      // inputs of the shared prediction
+      const int xs = op == OP_STEP ? sf : ifr;
+      double Xm[N * D], XL[N * N], P[N], PI[N], b[D], Pm[N * D], PL[N * N];
+      w.load_mL(xs, Xm, XL);
+      double h, tn = 0.0, sigma = 1.0, oL = 0.0;
+      if (op == OP_STEP) {
+        h = dt;
+        if (cfg.clip_dt) h = fmin(h, t1 - tsf);
+        tn = tsf + h;
+      } else if (op == OP_BEYOND1) {
+        h = t1 - w.at(ifr, LS_T);  // interp_from -> t1
+      } else {
+        h = tsf - t1;  // t1 -> step_from
+      }
+      lane_precon<N>(h, P, PI);
+      if (op == OP_STEP && cfg.solver == PDQ_SOLVER_DYNAMIC) lane_calibrate<N, D>(c, Xm, P, PI, h, tn, b, sigma, oL);
+      const double os = w.at(sf, LS_OS);
+      const double scale = op == OP_STEP ? sigma : os;
+      // step: revert into proposed, merge step_from's conditional in front;
+      // first half: revert into step_from's conditional (fixed point; it is
+      // rewritten by the second half) and merge interp_from's in front, in
+      // place (fixed interval: revert straight into interp_from's);
+      // second half: revert into step_from's conditional behind an identity.
+      int bws, mself, mres;
+      bool mident;
+      if (op == OP_STEP) {
+        bws = pr;
+        mself = sf;
+        mres = pr;
+        mident = w.at(sf, LS_IDENT) != 0.0;
+      } else if (op == OP_BEYOND1) {
+        bws = cfg.strategy == PDQ_FIXEDPOINT ? sf : ifr;
+        mself = ifr;
+        mres = ifr;
+        mident = w.at(ifr, LS_IDENT) != 0.0;
+      } else {
+        bws = sf;
+        mself = sf;
+        mres = sf;
+        mident = true;
+      }
+      PDQ_PHASE_BOUNDARY();
+      lane_predict<N, D>(c, P, PI, lane_noise_scale(cfg, h, scale), Xm, XL, Pm, PL, bws,
+                         cfg.strategy == PDQ_FIXEDPOINT, mident, mself, mres);
+      if (op == OP_STEP) {
+        const double ep = lane_finish_step<N, D>(c, sf, pr, h, tn, P, PI, Xm, Pm, PL, b, sigma, oL, true);
+        dt = lane_control(cfg, h, cstate, ep);
+        ++att;
+        ++attempts;
+        if (!isfinite(ep) || !isfinite(dt)) {
+          status = PDQ_STATUS_NONFINITE;
+        } else if (ep < 1.0 && (dt == 0.0 || tsf + dt == tsf)) {
+          status = PDQ_STATUS_DT_UNDERFLOW;
+        } else if (ep >= 1.0 && w.at(pr, LS_T) == tsf) {
+          status = PDQ_STATUS_DT_UNDERFLOW;  // accepted without advancing t
+        } else if ((cfg.max_attempts > 0 && att >= cfg.max_attempts && ep < 1.0) || attempts >= kMaxTotalAttempts) {
+          status = PDQ_STATUS_MAX_ATTEMPTS;
+        } else if (ep >= 1.0) {
           ++accepted;
+          att = 0;
           const int old_if = ifr;  // interp_from <- step_from, step_from <- proposed
           ifr = sf;
           sf = pr;
           pr = old_if;
         }
-        const double tsf = w.at(sf, LS_T);
-        if (tsf + cfg.eps < t1) {
-          cont = true;
-        } else if (tsf > t1 + cfg.eps) {
-          lane_interp_beyond<N, D>(c, k, t1, ifr, sf);
-          cont = w.at(sf, LS_T) + cfg.eps < t1;
+      } else if (op == OP_BEYOND1) {
+        const double nsteps = w.at(sf, LS_NSTEPS);
+        if (!smoother) {
+          lane_write_checkpoint<N, D, CondRef<N, D>>(c, k, t1, Pm, PL, os, nsteps, nullptr, false);
+          w.store_mL(ifr, Pm, PL);
+          w.at(ifr, LS_T) = t1;
+          ++k;
         } else {
-          lane_interp_at<N, D>(c, k, ifr, sf);
-          cont = false;
+          w.store_mL(ifr, Pm, PL);  // ext_t's marginal: the new interp_from's
+          second_half = true;
         }
+      } else {
+        // interpolated = (ext_t.marginal, ext_t.conditional), the checkpoint
+        double Em[N * D], EL[N * N];
+        w.load_mL(ifr, Em, EL);
+        const CondRef<N, D> ext = w.cond(ifr);
+        lane_write_checkpoint<N, D>(c, k, t1, Em, EL, os, w.at(sf, LS_NSTEPS), &ext, false);
+        w.at(ifr, LS_T) = t1;
+        w.at(ifr, LS_IDENT) = cfg.strategy == PDQ_FIXEDPOINT ? 1.0 : 0.0;
+        w.at(sf, LS_IDENT) = 0.0;
+        second_half = false;
+        ++k;
       }
     }
   }
