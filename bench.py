@@ -31,11 +31,11 @@ METRIC = "IVP solves/s and log-lik grads/s at rtol 1e-6 FP64: batched/adjoint HI
 THETA0 = np.array([0.5, 0.05, 0.5, 0.05])
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and dense matrix)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# HBM bytes per C2 launch of the solve kernel, from rocprofv3 PMC runs
+# HBM bytes per C2 launch of the lane kernel, from rocprofv3 PMC runs
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, scales from a 1 GiB copy); see
-# profiles/solve_kernel_c2_isotropic_nu4_4096.md.  Re-measure when the kernel changes.
-C2_MEASURED_TRAFFIC_BYTES = 3_740_516 * 1024
-C2_TRAFFIC_SOURCE = "profiles/solve_kernel_c2_isotropic_nu4_4096.md"
+# profiles/lane_kernel_c2_isotropic_nu4_4096.md.  Re-measure when the kernel changes.
+C2_MEASURED_TRAFFIC_BYTES = 1_398_735 * 1024
+C2_TRAFFIC_SOURCE = "profiles/lane_kernel_c2_isotropic_nu4_4096.md"
 
 
 def c2_inputs(n_inst=4096, K=200, seed=0):
