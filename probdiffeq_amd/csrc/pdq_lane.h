@@ -238,14 +238,26 @@ struct FxColumnRows {
 };
 
 // preconditioner_taylor (utilities.py:74-84).
-template <int N>
+// dt^k for an exponent known at compile time.  k = 0, 1, 2, -1 are single
+// correctly rounded operations and equal pow's correctly rounded result;
+// other exponents call pow.
+template <int k>
+__device__ __forceinline__ double lane_ipow(double dt) {
+  if constexpr (k == 0) return 1.0;
+  else if constexpr (k == 1) return dt;
+  else if constexpr (k == 2) return dt * dt;
+  else if constexpr (k == -1) return 1.0 / dt;
+  else return pow(dt, (double)k);
+}
+
+template <int N, int i = 0>
 __device__ __forceinline__ void lane_precon(double dt, double* P, double* PI) {
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const int pw = N - 1 - i;
+  if constexpr (i < N) {
+    constexpr int pw = N - 1 - i;
     const double s = factorial_d(pw);
-    P[i] = pow(dt, (double)pw) / s;
-    PI[i] = pow(dt, -(double)pw) * s;
+    P[i] = lane_ipow<pw>(dt) / s;
+    PI[i] = lane_ipow<-pw>(dt) * s;
+    lane_precon<N, i + 1>(dt, P, PI);
   }
 }
 
@@ -571,13 +583,13 @@ __device__ __forceinline__ double lane_error_power(const pdq_config& cfg, double
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       const double ref = fmax(fabs(prev_m[k]), fabs(prop_m[k]));
-      const double e = err * pow(dt, 1.0) / 1.0;
+      const double e = err * dt / 1.0;
       const double rr = e / (cfg.atol + cfg.rtol * fabs(ref));
       ss += rr * rr;
     }
     norm = sqrt(ss) / sqrt((double)D);
   } else {
-    const double e = err * pow(dt, 1.0) / 1.0;
+    const double e = err * dt / 1.0;
     const double sa = 0.0 + e * e;
     double sr = 0.0;
 #pragma unroll
