@@ -168,8 +168,11 @@ int32_t pdq_chol_size(const pdq_config* cfg);
 int32_t pdq_tuned_lanes(const pdq_config* cfg, int32_t n_inst);
 
 /* Records the lanes-per-instance a timing sweep picked for this
- * configuration, batch-size bucket and mode (0 save_at, 1 fixed grid,
- * 2 LML); later calls with lanes_per_instance = 0 use it. */
+ * configuration, batch-size bucket (next power of two) and mode (0 save_at,
+ * 1 fixed grid, 2 LML); later calls with lanes_per_instance = 0 use it.
+ * lanes: 1 (lane-per-instance kernel) or 4, 8, 16, 32, 64 (cooperative).
+ * No counterpart in the reference (XLA picks its own launch shapes); the
+ * sweep itself is probdiffeq_amd/autotune.py. */
 int32_t pdq_set_tuned_lanes(const pdq_config* cfg, int32_t n_inst, int32_t mode, int32_t lanes);
 
 #ifdef __cplusplus

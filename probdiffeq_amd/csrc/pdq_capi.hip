@@ -402,8 +402,8 @@ int32_t pdq_tuned_lanes(const pdq_config* cfg, int32_t n_inst) {
 
 // Records the lanes-per-instance choice of an external timing sweep.
 int32_t pdq_set_tuned_lanes(const pdq_config* cfg, int32_t n_inst, int32_t mode, int32_t lanes) {
-  if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
-    return set_err(PDQ_ERR_INVALID, "lanes must be a power of two in [4, 64]");
+  if (lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+    return set_err(PDQ_ERR_INVALID, "lanes must be 1 (lane kernel) or 4, 8, 16, 32, 64 (cooperative kernel)");
   std::lock_guard<std::mutex> lk(g_mu);
   g_tuned[tune_key(*cfg, n_inst, mode)] = lanes;
   return PDQ_OK;

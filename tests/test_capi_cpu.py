@@ -99,3 +99,18 @@ def test_fixed_grid_warns_for_fixedpoint_smoother():
     slv = pdq.solver(strategy=pdq.strategy_smoother_fixedpoint(), constraint=None)
     with pytest.warns(UserWarning, match="fixed-point"):
         ivpsolve.solve_fixed_grid(solver=slv)
+
+
+def test_tuning_table_accepts_kernel_choices_and_rejects_others():
+    """pdq_set_tuned_lanes needs no device: 1 (lane kernel) and 4..64 are recorded, 3 is refused."""
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=4)
+    c = cfg.c()
+    assert capi.lib().pdq_set_tuned_lanes(ctypes.byref(c), 100, 0, 1) == 0
+    assert capi.lib().pdq_tuned_lanes(ctypes.byref(c), 100) == 1
+    assert capi.lib().pdq_set_tuned_lanes(ctypes.byref(c), 100, 0, 32) == 0
+    assert capi.lib().pdq_tuned_lanes(ctypes.byref(c), 128) == 32  # same batch-size bucket
+    assert capi.lib().pdq_tuned_lanes(ctypes.byref(c), 129) == 0
+    assert capi.lib().pdq_set_tuned_lanes(ctypes.byref(c), 100, 0, 3) == 1  # PDQ_ERR_INVALID
+    assert "lane kernel" in capi.last_error()
