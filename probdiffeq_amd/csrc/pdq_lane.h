@@ -237,17 +237,52 @@ struct FxColumnRows {
   static constexpr bool nz(int, int r) { return r <= 1 || r == N; }
 };
 
-// preconditioner_taylor (utilities.py:74-84).
-// dt^k for an exponent known at compile time.  k = 0, 1, 2, -1 are single
-// correctly rounded operations and equal pow's correctly rounded result;
-// other exponents call pow.
+// preconditioner_taylor (utilities.py:74-84): dt^k with libm pow semantics.
+// dt^k for an exponent known at compile time, computed as an unevaluated sum
+// hi + lo of exact FMA products (double-double), then rounded once: the
+// result is within about half an ulp of dt^k, as a correctly rounded pow is,
+// at a few instructions instead of a general pow.  k = 0, 1, 2, -1 are single
+// correctly rounded operations.  A negative exponent is the reciprocal of the
+// double-double power: q = 1/hi refined by one FMA step against hi + lo.
+struct LaneDD {
+  double hi, lo;
+};
+
+template <int k>
+__device__ __forceinline__ LaneDD lane_dd_pow(double x) {
+  static_assert(k >= 1 && k <= 8, "lane_dd_pow covers the exponents of num_derivatives <= 7");
+  if constexpr (k == 1) {
+    return {x, 0.0};
+  } else if constexpr (k % 2 == 0) {
+    const LaneDD h = lane_dd_pow<k / 2>(x);
+    const double p = h.hi * h.hi;
+    return {p, fma(h.hi, h.hi, -p) + 2.0 * h.hi * h.lo};
+  } else {
+    const LaneDD h = lane_dd_pow<k - 1>(x);
+    const double p = h.hi * x;
+    return {p, fma(h.hi, x, -p) + h.lo * x};
+  }
+}
+
 template <int k>
 __device__ __forceinline__ double lane_ipow(double dt) {
-  if constexpr (k == 0) return 1.0;
-  else if constexpr (k == 1) return dt;
-  else if constexpr (k == 2) return dt * dt;
-  else if constexpr (k == -1) return 1.0 / dt;
-  else return pow(dt, (double)k);
+  if constexpr (k == 0) {
+    return 1.0;
+  } else if constexpr (k == 1) {
+    return dt;
+  } else if constexpr (k == 2) {
+    return dt * dt;
+  } else if constexpr (k == -1) {
+    return 1.0 / dt;
+  } else if constexpr (k > 0) {
+    const LaneDD p = lane_dd_pow<k>(dt);
+    return p.hi + p.lo;
+  } else {
+    const LaneDD p = lane_dd_pow<-k>(dt);
+    const double q = 1.0 / p.hi;
+    const double r = fma(-p.hi, q, 1.0) - p.lo * q;
+    return isfinite(q) ? fma(q, r, q) : q;  // dt^k = 0 or overflowed: 1/hi as pow gives it
+  }
 }
 
 template <int N, int i = 0>
@@ -266,24 +301,51 @@ __device__ __forceinline__ double lane_noise_scale(const pdq_config& cfg, double
   return sqrt(fabs(dt)) * cfg.base_output_scale * s;
 }
 
+// The IWP transition in preconditioned coordinates (utilities.py:57-71):
+// A[i][j] = C(nu - i, nu - j) for j >= i, else 0.  These are small exact
+// integers (the values pdq_iwp.h computes, by the same recurrence), known here
+// at compile time: in the unrolled products a zero entry's term is dropped
+// (it adds an exact zero) and a unit entry's product is the other factor, so
+// for finite data the results are bit-identical to the generic product.
+constexpr double lane_binom(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  double r = 1.0;
+  for (int i = 1; i <= k; ++i) r = r * (double)(n - k + i) / (double)i;
+  return r;
+}
+
+template <int N>
+constexpr double lane_iwp_a(int i, int j) {
+  return j < i ? 0.0 : lane_binom(N - 1 - i, N - 1 - j);
+}
+
+// The process-noise factor Q is lower triangular; its upper entries are
+// (signed) zeros that every consumer treats as structural zeros.
+template <int N>
+__device__ __forceinline__ double lane_iwp_q(const double* __restrict__ Q, int i, int j) {
+  return j <= i ? Q[i * N + j] : 0.0;
+}
+
 // apply_flat of the transition to a mean (ssm_impl_isotropic.py:75-79).
 template <int N, int D>
-__device__ __forceinline__ void lane_apply_flat_mean(const double* __restrict__ A, const double* P, const double* PI,
-                                                     const double* x, double* out) {
+__device__ __forceinline__ void lane_apply_flat_mean(const double* P, const double* PI, const double* x, double* out) {
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int col = 0; col < D; ++col) {
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc += (P[i] * A[i * N + k]) * (PI[k] * x[k * D + col]);
+      for (int k = 0; k < N; ++k) {
+        const double a = lane_iwp_a<N>(i, k);
+        if (a != 0.0) acc += (P[i] * a) * (PI[k] * x[k * D + col]);
+      }
       out[i * D + col] = acc + 0.0;
     }
 }
 
 // marginalise of the transition (ssm_impl_isotropic.py:81-89).
 template <int N, int D>
-__device__ __forceinline__ void lane_marginalise_prior(const double* __restrict__ A, const double* __restrict__ Q,
+__device__ __forceinline__ void lane_marginalise_prior(const double* __restrict__ Q,
                                                        const double* P, const double* PI, double s, const double* Xm,
                                                        const double* XL, double* Ym, double* YL) {
   double big[2 * N * N];
@@ -293,9 +355,12 @@ __device__ __forceinline__ void lane_marginalise_prior(const double* __restrict_
     for (int col = 0; col < N; ++col) {
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc += A[col * N + k] * (PI[k] * XL[k * N + r]);
+      for (int k = 0; k < N; ++k) {
+        const double a = lane_iwp_a<N>(col, k);
+        if (a != 0.0) acc += a * (PI[k] * XL[k * N + r]);
+      }
       big[r * N + col] = acc;
-      big[(N + r) * N + col] = s * Q[col * N + r];
+      big[(N + r) * N + col] = s * lane_iwp_q<N>(Q, col, r);
     }
 #pragma unroll
   for (int i = 0; i < N; ++i)
@@ -303,7 +368,10 @@ __device__ __forceinline__ void lane_marginalise_prior(const double* __restrict_
     for (int col = 0; col < D; ++col) {
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc += A[i * N + k] * (PI[k] * Xm[k * D + col]);
+      for (int k = 0; k < N; ++k) {
+        const double a = lane_iwp_a<N>(i, k);
+        if (a != 0.0) acc += a * (PI[k] * Xm[k * D + col]);
+      }
       Ym[i * D + col] = P[i] * (acc + 0.0);
     }
   lane_qr<2 * N, N, TriStackRows<N>>(big);
@@ -317,7 +385,7 @@ __device__ __forceinline__ void lane_marginalise_prior(const double* __restrict_
 // cholesky_util.py:27-82): observed (Ym, YL) and backward conditional bw
 // (written to any conditional storage).
 template <int N, int D, class Cond>
-__device__ __forceinline__ void lane_revert_prior(const double* __restrict__ A, const double* __restrict__ Q,
+__device__ __forceinline__ void lane_revert_prior(const double* __restrict__ Q,
                                                   const double* P, const double* PI, double s, const double* Xm,
                                                   const double* XL, double* Ym, double* YL, const Cond& bw) {
   constexpr int n2 = 2 * N;
@@ -327,11 +395,14 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ A, 
   for (int r = 0; r < N; ++r)
 #pragma unroll
     for (int col = 0; col < N; ++col) {
-      big[r * n2 + col] = s * Q[col * N + r];
+      big[r * n2 + col] = s * lane_iwp_q<N>(Q, col, r);
       big[r * n2 + N + col] = 0.0;
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc += A[col * N + k] * (fabs(PI[k]) * XL[k * N + r]);
+      for (int k = 0; k < N; ++k) {
+        const double a = lane_iwp_a<N>(col, k);
+        if (a != 0.0) acc += a * (fabs(PI[k]) * XL[k * N + r]);
+      }
       big[(N + r) * n2 + col] = acc;
       big[(N + r) * n2 + N + col] = fabs(PI[col]) * XL[col * N + r];
     }
@@ -341,7 +412,10 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ A, 
     for (int col = 0; col < D; ++col) {
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc += A[i * N + k] * (PI[k] * Xm[k * D + col]);
+      for (int k = 0; k < N; ++k) {
+        const double a = lane_iwp_a<N>(i, k);
+        if (a != 0.0) acc += a * (PI[k] * Xm[k * D + col]);
+      }
       tmp[i * D + col] = acc + 0.0;
     }
   lane_qr<n2, n2, RevertRows<N>>(big);
@@ -636,15 +710,14 @@ __device__ __forceinline__ void lane_calibrate(const LaneCtx<N, D>& c, const dou
                                                const double* PI, double h, double tn, double* b, double& sigma,
                                                double& oL) {
   const pdq_config& cfg = c.a->cfg;
-  const double* __restrict__ A = c.a->A;
   const double* __restrict__ Q = c.a->Q;
   const double s1 = lane_noise_scale(cfg, h, 1.0);
   double Um[N * D], UL[N * N];
-  lane_apply_flat_mean<N, D>(A, P, PI, Sm, Um);
+  lane_apply_flat_mean<N, D>(P, PI, Sm, Um);
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * Q[i * N + j]);
+    for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * lane_iwp_q<N>(Q, i, j));
   lane_linearize<N, D>(cfg, c.prm, Um, tn, b);
   double om[D];
   lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oL);
@@ -661,13 +734,12 @@ __device__ __forceinline__ void lane_predict(const LaneCtx<N, D>& c, const doubl
                                              const double* Xm, const double* XL, double* Pm, double* PL, int bws,
                                              bool merge, bool mident, int mself, int mres) {
   const pdq_config& cfg = c.a->cfg;
-  const double* __restrict__ A = c.a->A;
   const double* __restrict__ Q = c.a->Q;
   if (cfg.strategy == PDQ_FILTER) {
-    lane_marginalise_prior<N, D>(A, Q, P, PI, s, Xm, XL, Pm, PL);
+    lane_marginalise_prior<N, D>(Q, P, PI, s, Xm, XL, Pm, PL);
     return;
   }
-  lane_revert_prior<N, D>(A, Q, P, PI, s, Xm, XL, Pm, PL, c.w.cond(bws));
+  lane_revert_prior<N, D>(Q, P, PI, s, Xm, XL, Pm, PL, c.w.cond(bws));
   PDQ_PHASE_BOUNDARY();
   if (merge) lane_merge<N, D>(mident, c.w.cond(mself), c.w.cond(bws), c.w.cond(mres));
   PDQ_PHASE_BOUNDARY();
@@ -682,7 +754,6 @@ __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int f
                                                    const double* Pm, const double* PL, double* b, double sigma,
                                                    double oL, bool estimate) {
   const pdq_config& cfg = c.a->cfg;
-  const double* __restrict__ A = c.a->A;
   const double* __restrict__ Q = c.a->Q;
   const LaneWs<N, D>& w = c.w;
   double Rm[N * D], RL[N * N], mo[D], oLu, ep = 0.0;
@@ -705,11 +776,11 @@ __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int f
     }
     if (estimate) {
       double Um[N * D], UL[N * N];
-      lane_apply_flat_mean<N, D>(A, P, PI, Sm, Um);
+      lane_apply_flat_mean<N, D>(P, PI, Sm, Um);
 #pragma unroll
       for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * Q[i * N + j]);
+        for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * lane_iwp_q<N>(Q, i, j));
       double om[D], oLe;
       lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oLe);
       const double se = lane_rms_zero<D>(om, oLe);
@@ -817,7 +888,6 @@ __device__ __forceinline__ void lane_write_slot(const LaneCtx<N, D>& c, int k, i
 template <int N, int D>
 __device__ __forceinline__ void lane_interp_beyond(const LaneCtx<N, D>& c, int k, double t1, int from, int to) {
   const pdq_config& cfg = c.a->cfg;
-  const double* __restrict__ A = c.a->A;
   const double* __restrict__ Q = c.a->Q;
   const LaneWs<N, D>& w = c.w;
   const double os = w.at(to, LS_OS);
@@ -829,7 +899,7 @@ __device__ __forceinline__ void lane_interp_beyond(const LaneCtx<N, D>& c, int k
   lane_precon<N>(dt0, P, PI);
   const double s0 = lane_noise_scale(cfg, dt0, os);
   if (cfg.strategy == PDQ_FILTER) {
-    lane_marginalise_prior<N, D>(A, Q, P, PI, s0, Fm, FL, Pm, PL);
+    lane_marginalise_prior<N, D>(Q, P, PI, s0, Fm, FL, Pm, PL);
     lane_write_checkpoint<N, D, CondRef<N, D>>(c, k, t1, Pm, PL, os, nsteps, nullptr, false);
     w.store_mL(from, Pm, PL);
     w.at(from, LS_T) = t1;
@@ -840,11 +910,11 @@ __device__ __forceinline__ void lane_interp_beyond(const LaneCtx<N, D>& c, int k
   // ext = interp_from.cond o bw (bw staged in step_from's conditional, which
   // is rewritten below); fixed interval ext = bw.
   if (cfg.strategy == PDQ_FIXEDPOINT) {
-    lane_revert_prior<N, D>(A, Q, P, PI, s0, Fm, FL, Pm, PL, w.cond(to));
+    lane_revert_prior<N, D>(Q, P, PI, s0, Fm, FL, Pm, PL, w.cond(to));
     PDQ_PHASE_BOUNDARY();
     lane_merge<N, D>(w.at(from, LS_IDENT) != 0.0, w.cond(from), w.cond(to), w.cond(from));
   } else {
-    lane_revert_prior<N, D>(A, Q, P, PI, s0, Fm, FL, Pm, PL, w.cond(from));
+    lane_revert_prior<N, D>(Q, P, PI, s0, Fm, FL, Pm, PL, w.cond(from));
   }
   PDQ_PHASE_BOUNDARY();
   // extrapolate t -> t1 from ext_t's marginal; only the backward conditional
@@ -852,7 +922,7 @@ __device__ __forceinline__ void lane_interp_beyond(const LaneCtx<N, D>& c, int k
   lane_precon<N>(dt1, P, PI);
   const double s1 = lane_noise_scale(cfg, dt1, os);
   double Ym[N * D], YL[N * N];
-  lane_revert_prior<N, D>(A, Q, P, PI, s1, Pm, PL, Ym, YL, w.cond(to));
+  lane_revert_prior<N, D>(Q, P, PI, s1, Pm, PL, Ym, YL, w.cond(to));
   if (cfg.strategy == PDQ_FIXEDPOINT) lane_merge<N, D>(true, w.cond(to), w.cond(to), w.cond(to));
   const CondRef<N, D> ext = w.cond(from);
   lane_write_checkpoint<N, D>(c, k, t1, Pm, PL, os, nsteps, &ext, false);

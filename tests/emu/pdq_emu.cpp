@@ -3,6 +3,7 @@
 // every instance with one lane (G = 1) so that the kernel logic can be
 // compared with the oracle on the CPU.
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "../../probdiffeq_amd/csrc/pdq_blockdiag.hip"
@@ -16,6 +17,51 @@ int num_cus() { return 1; }
 }  // namespace pdq
 
 extern "C" {
+
+// Entries where the lane kernel's compile-time IWP transition (pdq_lane.h:
+// lane_iwp_a) differs from the host's iwp_matrices, for nu = 1..7.
+int pdqemu_iwp_mismatches(void) {
+  int bad = 0;
+  auto check = [&](auto n_tag) {
+    constexpr int N = decltype(n_tag)::value;
+    std::vector<double> A, Q;
+    pdq::iwp_matrices(N - 1, A, Q);
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) bad += pdq::lane_iwp_a<N>(i, j) != A[i * N + j];
+    for (int i = 0; i < N; ++i)
+      for (int j = i + 1; j < N; ++j) bad += Q[i * N + j] != 0.0;  // Q lower triangular
+  };
+  check(std::integral_constant<int, 2>{});
+  check(std::integral_constant<int, 3>{});
+  check(std::integral_constant<int, 4>{});
+  check(std::integral_constant<int, 5>{});
+  check(std::integral_constant<int, 6>{});
+  check(std::integral_constant<int, 7>{});
+  check(std::integral_constant<int, 8>{});
+  return bad;
+}
+
+// The lane kernel's dt^k (pdq_lane.h: lane_ipow) for k in [-7, 7].
+double pdqemu_ipow(int k, double x) {
+  switch (k) {
+    case -7: return pdq::lane_ipow<-7>(x);
+    case -6: return pdq::lane_ipow<-6>(x);
+    case -5: return pdq::lane_ipow<-5>(x);
+    case -4: return pdq::lane_ipow<-4>(x);
+    case -3: return pdq::lane_ipow<-3>(x);
+    case -2: return pdq::lane_ipow<-2>(x);
+    case -1: return pdq::lane_ipow<-1>(x);
+    case 0: return pdq::lane_ipow<0>(x);
+    case 1: return pdq::lane_ipow<1>(x);
+    case 2: return pdq::lane_ipow<2>(x);
+    case 3: return pdq::lane_ipow<3>(x);
+    case 4: return pdq::lane_ipow<4>(x);
+    case 5: return pdq::lane_ipow<5>(x);
+    case 6: return pdq::lane_ipow<6>(x);
+    case 7: return pdq::lane_ipow<7>(x);
+    default: return 0.0;
+  }
+}
 
 // The lane-per-instance isotropic kernel's per-instance program (pdq_lane.h)
 // on the CPU, one instance after another.  Returns 2 for a shape the device

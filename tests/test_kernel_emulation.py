@@ -182,3 +182,36 @@ def test_lane_kernel_status_codes(emu):
     tcs = np.stack([O.taylor_coefficients("lotka_volterra", p[i], y[i], 0.0, 4) for i in range(2)])
     g = emu.solve(cfg, p, tcs, t, kernel="lane")
     assert g["status"].tolist() == [1, 1]
+
+
+def test_lane_powers_of_dt_are_within_half_an_ulp(emu):
+    """The lane kernel's dt^k (preconditioner_taylor, utilities.py:74-84, which calls pow):
+    against a long-double pow, every result is within 0.5 ulp plus a small margin, as a
+    correctly rounded pow is; special values follow pow."""
+    import ctypes
+
+    f = emu.lib().pdqemu_ipow
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_int, ctypes.c_double]
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([np.exp(rng.uniform(np.log(1e-12), np.log(1e3), 4000)), [1e-3, 0.1, 0.3, 1.0, 2.0, 7.5]])
+    xs = np.concatenate([xs, -xs[:500]])
+    worst = 0.0
+    for k in range(-7, 8):
+        for x in xs:
+            got = f(k, float(x))
+            exact = np.longdouble(x) ** k
+            if not np.isfinite(float(exact)) or float(exact) == 0.0:
+                continue
+            ulp = np.spacing(abs(float(exact)))
+            err = abs(np.longdouble(got) - exact) / np.longdouble(ulp)
+            worst = max(worst, float(err))
+    assert worst < 0.52, worst
+    assert f(-4, 0.0) == np.inf and f(-3, -0.0) == -np.inf
+    assert f(4, 0.0) == 0.0
+
+
+def test_lane_compile_time_transition_matches_the_host_matrices(emu):
+    """The lane kernel's compile-time IWP transition equals the host's iwp_matrices (utilities.py:57-71)
+    entry for entry, and the host's Q factor is lower triangular, for nu = 1..7."""
+    assert emu.lib().pdqemu_iwp_mismatches() == 0
