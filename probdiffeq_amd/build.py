@@ -18,7 +18,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libpdq.so")
-UNITS = ["pdq_capi.hip", "pdq_coop.hip", "pdq_lane.hip"]
+UNITS = [
+    "pdq_capi.hip",
+    "pdq_coop.hip",
+    "pdq_lane.hip",
+    "pdq_lane_filter.hip",
+    "pdq_lane_fixedpoint.hip",
+    "pdq_lane_fixedinterval.hip",
+]
 
 HIPCC_FLAGS = [
     "-O3",

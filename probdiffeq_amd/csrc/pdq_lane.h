@@ -1208,14 +1208,20 @@ constexpr size_t lane_lds_bytes() {
 #ifndef PDQ_HOST_EMU
 // One lane per instance, grid-stride over instances.  Each lane's slots live
 // in LDS, element-major across the workgroup's lanes, read and written by
-// that lane only.
-template <int N, int D>
+// that lane only.  The strategy and the solver are compile-time constants of
+// the instantiation: the kernel overwrites them in its own copy of the
+// arguments, so every test of them folds and the other strategies' and
+// solvers' code (with the registers it would hold) is not compiled in.
+template <int N, int D, int STRATEGY, int SOLVER>
 __global__ void __launch_bounds__(64) lane_kernel(LArgs args) {
   extern __shared__ double lds[];
+  LArgs a = args;
+  a.cfg.strategy = STRATEGY;
+  a.cfg.solver = SOLVER;
   double* const slots = lds + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
-  for (int inst = blockIdx.x * blockDim.x + threadIdx.x; inst < args.n_inst; inst += stride)
-    lane_solve_instance<N, D>(args, inst, slots);
+  for (int inst = blockIdx.x * blockDim.x + threadIdx.x; inst < a.n_inst; inst += stride)
+    lane_solve_instance<N, D>(a, inst, slots);
 }
 #endif
 
