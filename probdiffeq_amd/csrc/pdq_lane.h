@@ -158,6 +158,12 @@ __device__ __forceinline__ void cond_copy(const Dst& d, const Src& s) {
 // upper triangle of rows 0..min(M,C)-1 holds R, the strict lower part of
 // those rows is zero.
 //
+// The reflector's norm is sqrt(alpha^2 + ss) with one FMA and one square
+// root.  dgeqr2 forms it as dlapy2(alpha, dnrm2(x)) = hypot(alpha, sqrt(ss));
+// both are within about an ulp of the exact norm, and the single square root
+// is a much shorter dependency chain (ss itself is the same unscaled sum of
+// squares in both, so the range of inputs that avoid underflow is the same).
+//
 // S::nz(i, r) tells, at compile time, whether row r > i can be non-zero in
 // column i when reflection i is formed.  A row that cannot contributes an
 // exact zero to the norm, its Householder coefficient is an exact zero and
@@ -180,7 +186,7 @@ __device__ __forceinline__ void lane_qr(double* A) {
       if (S::nz(i, r)) ss += A[r * C + i] * A[r * C + i];
     if (ss != 0.0) {
       const double alpha = A[i * C + i];
-      const double beta = -copysign(hypot(alpha, sqrt(ss)), alpha);
+      const double beta = -copysign(sqrt(fma(alpha, alpha, ss)), alpha);
       const double tau = (beta - alpha) / beta;
       const double scal = 1.0 / (alpha - beta);
 #pragma unroll
@@ -205,6 +211,21 @@ __device__ __forceinline__ void lane_qr(double* A) {
   for (int i = 0; i < k; ++i)
 #pragma unroll
     for (int j = 0; j < i; ++j) A[i * C + j] = 0.0;
+}
+
+// a / b given y = 1 / b (correctly rounded): q = a y, then one FMA
+// correction step against the exact residual a - b q (Markstein).  For a
+// correctly rounded, normal y and a finite quotient the result is the
+// correctly rounded quotient, as a division gives it, for three instructions
+// once y is known; used where many quotients share one divisor.  Any other
+// case (b = 0, infinities, NaN, y outside the normal range) takes the
+// division itself, so special values behave exactly as IEEE division.
+__device__ __forceinline__ double lane_div(double a, double b, double y) {
+  const double q = a * y;
+  const double r = fma(-q, b, a);
+  const double res = r == 0.0 ? q : fma(r, y, q);  // exact q keeps the sign of a zero quotient
+  if (__builtin_expect(!(__builtin_isnormal(y) && isfinite(res)), 0)) return a / b;
+  return res;
 }
 
 // [dense N x N; upper-triangular N x N] stacks (sum_of_sqrtm_factors of a
@@ -420,7 +441,9 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ Q,
     }
   lane_qr<n2, n2, RevertRows<N>>(big);
   // gain = solve_triu(R_Y, R12)^T
-  double G[N * N];
+  double G[N * N], rdiag[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) rdiag[i] = 1.0 / big[i * n2 + i];
 #pragma unroll
   for (int j = 0; j < N; ++j)
 #pragma unroll
@@ -428,7 +451,7 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ Q,
       double sacc = big[i * n2 + N + j];
 #pragma unroll
       for (int c = i + 1; c < N; ++c) sacc -= big[i * n2 + c] * G[j * N + c];
-      G[j * N + i] = sacc / big[i * n2 + i];
+      G[j * N + i] = lane_div(sacc, big[i * n2 + i], rdiag[i]);
     }
 #pragma unroll
   for (int i = 0; i < N; ++i)
@@ -629,8 +652,9 @@ __device__ __forceinline__ void lane_update(const pdq_config& cfg, const double*
   for (int c = 0; c < D; ++c) mo[c] = Pm[1 * D + c] + b[c];
   lane_qr<n2, n2, UpdateRows>(big);
   double gain[N];
+  const double r0 = 1.0 / big[0];
 #pragma unroll
-  for (int j = 0; j < N; ++j) gain[j] = big[1 + j] / big[0];
+  for (int j = 0; j < N; ++j) gain[j] = lane_div(big[1 + j], big[0], r0);
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll

@@ -41,6 +41,9 @@ int pdqemu_iwp_mismatches(void) {
   return bad;
 }
 
+// The lane kernel's shared-divisor quotient (pdq_lane.h: lane_div).
+double pdqemu_lane_div(double a, double b) { return pdq::lane_div(a, b, 1.0 / b); }
+
 // The lane kernel's dt^k (pdq_lane.h: lane_ipow) for k in [-7, 7].
 double pdqemu_ipow(int k, double x) {
   switch (k) {

@@ -215,3 +215,30 @@ def test_lane_compile_time_transition_matches_the_host_matrices(emu):
     """The lane kernel's compile-time IWP transition equals the host's iwp_matrices (utilities.py:57-71)
     entry for entry, and the host's Q factor is lower triangular, for nu = 1..7."""
     assert emu.lib().pdqemu_iwp_mismatches() == 0
+
+
+def test_lane_shared_divisor_quotient_is_the_correctly_rounded_division(emu):
+    """lane_div (one reciprocal per divisor, then an FMA correction) returns exactly a / b:
+    random operands over a wide exponent range, hard cases near ties, and IEEE special values."""
+    import ctypes
+
+    f = emu.lib().pdqemu_lane_div
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double, ctypes.c_double]
+    rng = np.random.default_rng(3)
+    n = 200_000
+    a = rng.uniform(1, 2, n) * 2.0 ** rng.integers(-300, 300, n) * rng.choice([-1, 1], n)
+    b = rng.uniform(1, 2, n) * 2.0 ** rng.integers(-300, 300, n) * rng.choice([-1, 1], n)
+    # operands whose quotient lies close to a rounding boundary: a = b * q(1 + tiny)
+    q = rng.uniform(1, 2, 20_000)
+    bb = rng.uniform(1, 2, 20_000)
+    a = np.concatenate([a, bb * q, np.nextafter(bb * q, np.inf), np.nextafter(bb * q, -np.inf)])
+    b = np.concatenate([b, bb, bb, bb])
+    got = np.array([f(x, y) for x, y in zip(a.tolist(), b.tolist())])
+    assert np.array_equal(got, a / b)
+    specials = [(1.0, 0.0), (-1.0, 0.0), (0.0, 0.0), (np.inf, 2.0), (2.0, np.inf), (np.nan, 1.0), (1.0, 1e-310), (1e-310, 3.0), (0.0, 5.0), (-0.0, 5.0)]
+    for x, y in specials:
+        with np.errstate(all="ignore"):
+            e = np.float64(x) / np.float64(y)
+        g = f(x, y)
+        assert (np.isnan(e) and np.isnan(g)) or (g == e and np.signbit(g) == np.signbit(e)), (x, y, g, e)
