@@ -237,13 +237,27 @@ def lib():
         L.pdqo_solve_save_at.restype = ctypes.c_int
         L.pdqo_solve_fixed_grid.restype = ctypes.c_int
         L.pdqo_lml.restype = ct
-        L.pdqo_lml_from_y0.restype = ct
+        # full-precision return: ctypes converts a plain c_longdouble result to a
+        # Python float, but not an instance of a subclass (read back by _full)
+        L.pdqo_lml_from_y0.restype = _LongDoubleResult if ct is ctypes.c_longdouble else ct
+        L.pdqo_lml_from_y0_tape.restype = _LongDoubleResult if ct is ctypes.c_longdouble else ct
         L.pdqo_lml_grad_fd.restype = ct
         L.pdqo_logpdf.restype = ct
         L.pdqo_control.restype = ct
         L.pdqo_solve_save_at_batch.restype = ctypes.c_int
         _libs[_variant] = L
     return L
+
+
+class _LongDoubleResult(ctypes.c_longdouble):
+    """A long double return value that ctypes leaves unconverted."""
+
+
+def _full(v):
+    """A scalar oracle result at the build's precision (np.longdouble for the extended build)."""
+    if isinstance(v, ctypes.c_longdouble):
+        return np.frombuffer(bytes(v), dtype=np.longdouble)[0]
+    return v
 
 
 def _p(a):
@@ -391,7 +405,21 @@ def lml_from_y0(cfg: Config, params, y0, grid, every, data, stds, average=True):
     params, y0, grid, data, stds = map(_f64, (params, y0, grid, data, stds))
     stds2 = stds.reshape(data.shape[0], -1)
     c = cfg.c()
-    return lib().pdqo_lml_from_y0(ctypes.byref(c), _p(params), _p(y0), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average))
+    return _full(lib().pdqo_lml_from_y0(ctypes.byref(c), _p(params), _p(y0), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average)))
+
+
+def lml_from_y0_tape(cfg: Config, params, y0, grid, every, data, stds, tape, record: bool, average=True):
+    """lml_from_y0 with the calibration tape: record=True stores the output scale of every
+    solver_dynamic step into `tape` (an array of the active build's dtype), record=False replays
+    them, so that the scales are constants (the reference's stop_gradient on the calibration)."""
+    params, y0, grid, data, stds = map(_f64, (params, y0, grid, data, stds))
+    stds2 = stds.reshape(data.shape[0], -1)
+    assert tape.dtype == _dtype() and tape.flags.c_contiguous
+    c = cfg.c()
+    v = _full(lib().pdqo_lml_from_y0_tape(ctypes.byref(c), _p(params), _p(y0), _p(grid), grid.size, every, _p(data), _p(stds2), stds2.shape[1], int(average), _p(tape), tape.size, 1 if record else 2))
+    if np.isnan(v):
+        raise ValueError("calibration tape too short")
+    return v
 
 
 def lml_grad_fd(cfg: Config, params, y0, grid, every, data, stds, h=1e-6, average=True):

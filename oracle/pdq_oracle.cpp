@@ -26,6 +26,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <stdexcept>
+#include <limits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -979,6 +981,16 @@ Sol solver_init(const Problem& P, Real t, const BNormal& init) {
   return s;
 }
 
+// Calibration tape (test infrastructure for gradients): in record mode every
+// solver_dynamic step appends its output scale, in replay mode the steps use
+// the recorded scales instead of recomputing them.  Differentiating the LML
+// with a replayed tape is the reference's stop_gradient rule for the
+// calibration (solvers.py:568-569): the scales are constants.
+enum { TAPE_OFF = 0, TAPE_RECORD = 1, TAPE_REPLAY = 2 };
+static thread_local int g_tape_mode = TAPE_OFF;
+static thread_local Real* g_tape = nullptr;
+static thread_local size_t g_tape_len = 0, g_tape_pos = 0;
+
 // solver_dynamic.step (solvers.py:549-599), solver.step (:702-733),
 // solver_mle.step (:395-437).
 Sol solver_step(const Problem& P, const Sol& st, Real dt) {
@@ -990,6 +1002,14 @@ Sol solver_step(const Problem& P, const Sol& st, Real dt) {
     BCond fx = linearize(P, u, tn);
     BNormal observed = marginalise(P, fx, u);
     Vec os = residual_whitened_rms_zero(P, observed);
+    if (g_tape_mode != TAPE_OFF) {
+      for (auto& x : os) {
+        if (g_tape_pos >= g_tape_len) throw std::runtime_error("calibration tape too short");
+        if (g_tape_mode == TAPE_RECORD) g_tape[g_tape_pos] = x;
+        else x = g_tape[g_tape_pos];
+        ++g_tape_pos;
+      }
+    }
     BCond tr2 = transition(P, dt, os);
     BNormal upred;
     Post pred;
@@ -1752,6 +1772,27 @@ Real pdqo_lml_from_y0(const pdqo_cfg* cfg, const Real* params, const Real* y0, c
   std::vector<Real> tc((size_t)(cfg->nu + 1) * cfg->d);
   pdqo_taylor_coefficients(cfg->vf_id, cfg->d, params, y0, grid[0], cfg->nu, tc.data());
   return pdqo_lml(cfg, params, tc.data(), 0.0, grid, K, every, data, stds, nstd, average);
+}
+
+// pdqo_lml_from_y0 with the calibration tape: mode 1 records the output
+// scales of every solver_dynamic step into tape[0:tape_len], mode 2 replays
+// them.  Returns NaN if the tape is too short.
+Real pdqo_lml_from_y0_tape(const pdqo_cfg* cfg, const Real* params, const Real* y0, const Real* grid, int K, int every,
+                           const Real* data, const Real* stds, int nstd, int average, Real* tape, int tape_len,
+                           int mode) {
+  g_tape_mode = mode == 1 ? TAPE_RECORD : TAPE_REPLAY;
+  g_tape = tape;
+  g_tape_len = tape_len < 0 ? 0 : (size_t)tape_len;
+  g_tape_pos = 0;
+  Real v;
+  try {
+    v = pdqo_lml_from_y0(cfg, params, y0, grid, K, every, data, stds, nstd, average);
+  } catch (const std::runtime_error&) {
+    v = std::numeric_limits<Real>::quiet_NaN();
+  }
+  g_tape_mode = TAPE_OFF;
+  g_tape = nullptr;
+  return v;
 }
 
 // Central finite-difference gradient of pdqo_lml_from_y0 with respect to

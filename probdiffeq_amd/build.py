@@ -25,6 +25,7 @@ UNITS = [
     "pdq_lane_filter.hip",
     "pdq_lane_fixedpoint.hip",
     "pdq_lane_fixedinterval.hip",
+    "pdq_lane_lml.hip",
 ]
 
 HIPCC_FLAGS = [

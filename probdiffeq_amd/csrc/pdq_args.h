@@ -81,6 +81,27 @@ int launch_bd(BArgs& a, hipStream_t st);
 // it is instantiated for and its launcher.
 bool lane_supported(int N, int D);
 int launch_lane(LArgs& a, int N, int D, hipStream_t st);
+
+// Lane-per-instance LML (pdq_lane_lml.hip).
+// Arguments of the lane-per-instance LML kernel (pdq_lane_lml.h).
+struct LmlArgs {
+  pdq_config cfg;
+  int n_inst, K, every, n_tan, average_pdfs, G;
+  const double* params;
+  const double* tcoeffs;  // n_inst x (N d) x (1 + n_tan): Taylor coefficients and their tangents
+  const double* times;    // K grid points, shared by every instance
+  const double* data;     // n_inst x Kobs x d
+  const double* stds;     // n_inst x Kobs (one scalar per node)
+  const double* Q;
+  double* value;    // n_inst, or null
+  double* grad;     // n_inst x n_tan, or null
+  double* scratch;  // (Kobs - 1) x CondLayout<N, MC>::size x (n_inst G)
+};
+
+int lane_lml_tangents_per_lane(int n_tan);
+bool lane_lml_supported(int N, int D, int solver);
+size_t lane_lml_cond_size(int N, int D, int tpl);
+int launch_lane_lml(LmlArgs& a, int N, int D, int tpl, hipStream_t st);
 int launch_taylor_tangents(int vf, int d, int np, int n_inst, int num, int ntan, const double* params,
                            const double* y0, const double* t0, double* out, hipStream_t st);
 size_t coop_lds_bytes(const Layout& L, int G);

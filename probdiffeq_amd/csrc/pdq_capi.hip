@@ -329,12 +329,62 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
   if (cfg->factorisation == PDQ_ISOTROPIC && n_std != 1)
     return set_err(PDQ_ERR_INVALID, "isotropic observation noise is one scalar per time point (n_std = 1)");
   if ((cfg->num_derivatives + 1) * cfg->ode_dim > 64) return set_err(PDQ_ERR_UNSUPPORTED, "Taylor init too large");
+  const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
+  const bool lane_ok = cfg->factorisation == PDQ_ISOTROPIC && pdq::lane_lml_supported(N, d, cfg->solver);
+  if (cfg->lanes_per_instance == 1 && !lane_ok)
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "lanes_per_instance = 1 (lane-per-instance LML) needs the isotropic factorisation with nu = 3 or "
+                   "4, d = 2 and solver or solver_dynamic (got nu = %d, d = %d)",
+                   N - 1, d);
   if (n_inst == 0) return PDQ_OK;
   if (!params || !y0 || !grid || !data || !stds) return set_err(PDQ_ERR_INVALID, "null input pointer");
   hipStream_t st = (hipStream_t)stream;
+  const int Kobs = (K_grid - 1) / every + 1;
+  if (lane_ok && cfg->lanes_per_instance <= 1) {
+    // lane-per-instance LML: G lanes per instance, `tpl` tangent columns each
+    const int tpl = pdq::lane_lml_tangents_per_lane(n_tan);
+    const int G = tpl ? n_tan / tpl : 1;
+    const size_t n_lanes = (size_t)n_inst * G;
+    const size_t csz = pdq::lane_lml_cond_size(N, d, tpl);
+    DevMats m;
+    rc = device_mats(*cfg, false, m);
+    if (rc) return rc;
+    double* tinit = nullptr;
+    double* scratch = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&tinit, sizeof(double) * (size_t)n_inst * N * d * (1 + n_tan), st));
+    if (hipMallocAsync((void**)&scratch, sizeof(double) * csz * (size_t)std::max(1, Kobs - 1) * n_lanes, st) !=
+        hipSuccess) {
+      (void)hipFreeAsync(tinit, st);
+      return set_err(PDQ_ERR_HIP, "hipMallocAsync of the LML scratch buffer failed");
+    }
+    rc = pdq::launch_taylor_tangents(cfg->vector_field, d, cfg->n_params, n_inst, cfg->num_derivatives, n_tan, params,
+                                     y0, grid, tinit, st);
+    if (!rc) {
+      pdq::LmlArgs la{};
+      la.cfg = *cfg;
+      la.n_inst = n_inst;
+      la.K = K_grid;
+      la.every = every;
+      la.n_tan = n_tan;
+      la.average_pdfs = average_pdfs;
+      la.G = G;
+      la.params = params;
+      la.tcoeffs = tinit;
+      la.times = grid;
+      la.data = data;
+      la.stds = stds;
+      la.Q = m.Q;
+      la.value = value;
+      la.grad = grad;
+      la.scratch = scratch;
+      rc = pdq::launch_lane_lml(la, N, d, tpl, st);
+    }
+    (void)hipFreeAsync(scratch, st);
+    (void)hipFreeAsync(tinit, st);
+    return rc;
+  }
   pdq::Layout L = pdq::make_layout(*cfg, n_tan);
   if (L.nc > 32) return set_err(PDQ_ERR_UNSUPPORTED, "dense state dimension %d exceeds 32", L.nc);
-  const int Kobs = (K_grid - 1) / every + 1;
   const size_t csz = 2 * (size_t)L.nc * L.nc + (size_t)L.nc * L.mc + 2 * (size_t)L.nc;
   double* scratch = nullptr;
   int* queue = nullptr;
@@ -342,7 +392,6 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
   HIP_TRY(hipMallocAsync((void**)&scratch, sizeof(double) * csz * (size_t)std::max(1, Kobs - 1) * n_inst, st));
   // prior means with tangents: Taylor expansion of y0 (jetexpand, num = nu)
   double* tinit = nullptr;
-  const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
   HIP_TRY(hipMallocAsync((void**)&tinit, sizeof(double) * (size_t)n_inst * N * d * (1 + n_tan), st));
   rc = pdq::launch_taylor_tangents(cfg->vector_field, d, cfg->n_params, n_inst, cfg->num_derivatives, n_tan, params, y0,
                                    grid, tinit, st);

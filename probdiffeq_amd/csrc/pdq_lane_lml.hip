@@ -1,0 +1,75 @@
+// Instantiations and launcher of the lane-per-instance LML kernel.
+#include <algorithm>
+#include <atomic>
+
+#include "pdq_lane_lml.h"
+
+namespace pdq {
+
+namespace {
+
+constexpr int kLmlMaxDevices = 64;
+
+template <int N, int D, int TPL, int S>
+int launch_lml_nd(LmlArgs& a, hipStream_t st) {
+  constexpr int MC = D * (1 + TPL);
+  auto kern = lane_lml_kernel<N, D, TPL, S>;
+  const size_t lds = lane_lml_lds_bytes<N, MC>();
+  // LDS attribute and occupancy once per device and instantiation
+  static std::atomic<int> cache[kLmlMaxDevices];
+  int dev = 0;
+  PDQ_HIP_TRY(hipGetDevice(&dev));
+  int per_cu = (dev >= 0 && dev < kLmlMaxDevices) ? cache[dev].load() : 0;
+  if (per_cu <= 0) {
+    PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
+    if (per_cu < 1) return host_error(PDQ_ERR_UNSUPPORTED, "lane LML kernel does not fit (%zu bytes of LDS)", lds);
+    if (dev >= 0 && dev < kLmlMaxDevices) cache[dev].store(per_cu);
+  }
+  const long lanes = (long)a.n_inst * a.G;
+  const long need = (lanes + 63) / 64;
+  const long cap = (long)per_cu * num_cus();
+  const int grid = (int)std::max<long>(1, std::min(need, cap));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, a);
+  PDQ_HIP_TRY(hipGetLastError());
+  return PDQ_OK;
+}
+
+template <int N, int D, int TPL>
+int launch_lml_tpl(LmlArgs& a, hipStream_t st) {
+  if (a.cfg.solver == PDQ_SOLVER_DYNAMIC) return launch_lml_nd<N, D, TPL, PDQ_SOLVER_DYNAMIC>(a, st);
+  return launch_lml_nd<N, D, TPL, PDQ_SOLVER>(a, st);
+}
+
+template <int N, int D>
+int launch_lml_shape(LmlArgs& a, int tpl, hipStream_t st) {
+  switch (tpl) {
+    case 0: return launch_lml_tpl<N, D, 0>(a, st);
+    case 1: return launch_lml_tpl<N, D, 1>(a, st);
+    case 3: return launch_lml_tpl<N, D, 3>(a, st);
+  }
+  return host_error(PDQ_ERR_UNSUPPORTED, "no lane LML kernel with %d tangents per lane", tpl);
+}
+
+}  // namespace
+
+int lane_lml_tangents_per_lane(int n_tan) { return n_tan == 0 ? 0 : (n_tan % 3 == 0 ? 3 : 1); }
+
+bool lane_lml_supported(int N, int D, int solver) {
+  return D == 2 && (N == 4 || N == 5) && (solver == PDQ_SOLVER || solver == PDQ_SOLVER_DYNAMIC);
+}
+
+size_t lane_lml_cond_size(int N, int D, int tpl) {
+  const size_t mc = (size_t)D * (1 + tpl);
+  return (size_t)N * N + (size_t)N * mc + (size_t)N * (N + 1) / 2 + 2 * (size_t)N;
+}
+
+int launch_lane_lml(LmlArgs& a, int N, int D, int tpl, hipStream_t st) {
+  if (D == 2) {
+    if (N == 4) return launch_lml_shape<4, 2>(a, tpl, st);
+    if (N == 5) return launch_lml_shape<5, 2>(a, tpl, st);
+  }
+  return host_error(PDQ_ERR_UNSUPPORTED, "no lane LML kernel for nu + 1 = %d, d = %d", N, D);
+}
+
+}  // namespace pdq

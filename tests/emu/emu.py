@@ -80,7 +80,9 @@ def solve(cfg, params, tcoeffs, times, fixed_grid=False, kernel="coop"):
     return out
 
 
-def lml(cfg, params, y0, grid, every, data, stds, average=True, grad=True):
+def lml(cfg, params, y0, grid, every, data, stds, average=True, grad=True, kernel="coop", tangents_per_lane=None):
+    """kernel: "coop" (the cooperative kernel's LML) or "lane" (the lane-per-instance LML with
+    `tangents_per_lane` tangent columns per lane; default 3 when it divides the direction count, else 1)."""
     params, y0, grid, data, stds = (np.ascontiguousarray(x, dtype=np.float64) for x in (params, y0, grid, data, stds))
     B = params.shape[0]
     value = np.zeros(B)
@@ -88,7 +90,12 @@ def lml(cfg, params, y0, grid, every, data, stds, average=True, grad=True):
     n_std = stds.shape[-1] if stds.ndim == 3 else 1
     c = cfg.c()
     P = ctypes.c_void_p
-    rc = lib().pdqemu_lml(ctypes.byref(c), B, P(params.ctypes.data), P(y0.ctypes.data), P(grid.ctypes.data), grid.size, every, P(data.ctypes.data), P(stds.ctypes.data), n_std, int(average), P(value.ctypes.data), P(g.ctypes.data) if grad else None)
+    if kernel == "lane":
+        n_tan = params.shape[1] + y0.shape[1] if grad else 0
+        tpl = tangents_per_lane if tangents_per_lane is not None else (0 if n_tan == 0 else (3 if n_tan % 3 == 0 else 1))
+        rc = lib().pdqemu_lml_lane(ctypes.byref(c), B, P(params.ctypes.data), P(y0.ctypes.data), P(grid.ctypes.data), grid.size, every, P(data.ctypes.data), P(stds.ctypes.data), int(average), P(value.ctypes.data), P(g.ctypes.data) if grad else None, tpl)
+    else:
+        rc = lib().pdqemu_lml(ctypes.byref(c), B, P(params.ctypes.data), P(y0.ctypes.data), P(grid.ctypes.data), grid.size, every, P(data.ctypes.data), P(stds.ctypes.data), n_std, int(average), P(value.ctypes.data), P(g.ctypes.data) if grad else None)
     if rc:
         raise RuntimeError(f"emulation rc {rc}")
     return value, g

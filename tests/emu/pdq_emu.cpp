@@ -10,6 +10,7 @@
 #include "../../probdiffeq_amd/csrc/pdq_coop.hip"
 #include "../../probdiffeq_amd/csrc/pdq_iwp.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane.h"
+#include "../../probdiffeq_amd/csrc/pdq_lane_lml.h"
 
 namespace pdq {
 int host_error(int code, const char*, ...) { return code; }
@@ -158,16 +159,12 @@ int pdqemu_solve(const pdq_config* cfg, int n_inst, const double* params, const 
 
 
 // LML value and forward-mode gradient (MODE_LML) on the CPU.
-int pdqemu_lml(const pdq_config* cfg, int n_inst, const double* params, const double* y0, const double* grid, int K,
-               int every, const double* data, const double* stds, int n_std, int average, double* value,
-               double* grad) {
-  const int n_tan = grad ? cfg->n_params + cfg->ode_dim : 0;
+}  // extern "C"
+
+// Initial Taylor coefficients with tangents, as taylor_tangent_kernel computes them.
+static std::vector<double> emu_taylor_tangents(const pdq_config* cfg, int n_inst, const double* params, const double* y0,
+                                               const double* grid, int n_tan) {
   const int N = cfg->num_derivatives + 1, d = cfg->ode_dim, np_ = cfg->n_params;
-  pdq::Layout L = pdq::make_layout(*cfg, n_tan);
-  std::vector<double> A, Q;
-  if (cfg->factorisation == PDQ_DENSE) pdq::iwp_dense(cfg->num_derivatives, d, cfg->base_output_scale, A, Q);
-  else pdq::iwp_matrices(cfg->num_derivatives, A, Q);
-  // initial Taylor coefficients with tangents (as taylor_tangent_kernel)
   std::vector<double> tinit((size_t)n_inst * N * d * (1 + n_tan));
   for (int inst = 0; inst < n_inst; ++inst) {
     pdq::Dual prmD[pdq::kMaxParams], cc[64];
@@ -189,6 +186,21 @@ int pdqemu_lml(const pdq_config* cfg, int n_inst, const double* params, const do
           tinit[(((size_t)inst * N + i) * d + k) * (1 + n_tan) + q] = q == 0 ? v.v : v.g[q - 1];
       }
   }
+  return tinit;
+}
+
+extern "C" {
+
+int pdqemu_lml(const pdq_config* cfg, int n_inst, const double* params, const double* y0, const double* grid, int K,
+               int every, const double* data, const double* stds, int n_std, int average, double* value,
+               double* grad) {
+  const int n_tan = grad ? cfg->n_params + cfg->ode_dim : 0;
+  const int d = cfg->ode_dim;
+  pdq::Layout L = pdq::make_layout(*cfg, n_tan);
+  std::vector<double> A, Q;
+  if (cfg->factorisation == PDQ_DENSE) pdq::iwp_dense(cfg->num_derivatives, d, cfg->base_output_scale, A, Q);
+  else pdq::iwp_matrices(cfg->num_derivatives, A, Q);
+  std::vector<double> tinit = emu_taylor_tangents(cfg, n_inst, params, y0, grid, n_tan);
   const int Kobs = (K - 1) / every + 1;
   const size_t csz = 2 * (size_t)L.nc * L.nc + (size_t)L.nc * L.mc + 2 * (size_t)L.nc;
   std::vector<double> scratch(csz * (size_t)(Kobs > 1 ? Kobs - 1 : 1) * n_inst);
@@ -216,6 +228,67 @@ int pdqemu_lml(const pdq_config* cfg, int n_inst, const double* params, const do
   a.queue = &queue;
   std::vector<double> shared(L.wg_shared), ws(L.inst_size);
   for (int i = 0; i < n_inst; ++i) pdq::emu_solve_instance(a, i, shared.data(), ws.data());
+  return 0;
+}
+
+// The lane-per-instance LML (pdq_lane_lml.h) on the CPU: every (instance,
+// tangent group) lane in turn, with the device's slot and scratch layouts.
+// tpl: tangents per lane (0 without gradient).  Returns 2 for a shape the
+// device library does not instantiate.
+int pdqemu_lml_lane(const pdq_config* cfg, int n_inst, const double* params, const double* y0, const double* grid,
+                    int K, int every, const double* data, const double* stds, int average, double* value, double* grad,
+                    int tpl) {
+  const int n_tan = grad ? cfg->n_params + cfg->ode_dim : 0;
+  const int N = cfg->num_derivatives + 1, D = cfg->ode_dim;
+  if (cfg->factorisation != PDQ_ISOTROPIC || D != 2 || (N != 4 && N != 5)) return 2;
+  if ((n_tan == 0) != (tpl == 0) || (tpl > 0 && n_tan % tpl != 0)) return 2;
+  std::vector<double> A, Q;
+  pdq::iwp_matrices(cfg->num_derivatives, A, Q);
+  std::vector<double> tinit = emu_taylor_tangents(cfg, n_inst, params, y0, grid, n_tan);
+  const int G = tpl ? n_tan / tpl : 1;
+  const int Kobs = (K - 1) / every + 1;
+  const int MC = D * (1 + tpl);
+  const size_t csz = (size_t)N * N + (size_t)N * MC + (size_t)N * (N + 1) / 2 + 2 * (size_t)N;
+  const size_t n_lanes = (size_t)n_inst * G;
+  std::vector<double> scratch(csz * (size_t)(Kobs > 1 ? Kobs - 1 : 1) * n_lanes);
+  pdq::LmlArgs a{};
+  a.cfg = *cfg;
+  a.cfg.strategy = PDQ_FIXEDPOINT;
+  a.n_inst = n_inst;
+  a.K = K;
+  a.every = every;
+  a.n_tan = n_tan;
+  a.average_pdfs = average;
+  a.G = G;
+  a.params = params;
+  a.tcoeffs = tinit.data();
+  a.times = grid;
+  a.data = data;
+  a.stds = stds;
+  a.Q = Q.data();
+  a.value = value;
+  a.grad = grad;
+  a.scratch = scratch.data();
+  pdq::LArgs la{};
+  la.cfg = a.cfg;
+  la.n_inst = n_inst;
+  la.K = K;
+  la.Q = Q.data();
+  std::vector<double> slots(64 * 2 * 256);
+  for (size_t gl = 0; gl < n_lanes; ++gl) {
+    const int inst = (int)(gl / G), g = (int)(gl % G);
+#define PDQEMU_LML(NN, TT) pdq::lane_lml_instance<NN, 2, TT>(la, a, inst, g, slots.data(), gl, n_lanes)
+    if (N == 4 && tpl == 0) PDQEMU_LML(4, 0);
+    else if (N == 4 && tpl == 1) PDQEMU_LML(4, 1);
+    else if (N == 4 && tpl == 2) PDQEMU_LML(4, 2);
+    else if (N == 4 && tpl == 3) PDQEMU_LML(4, 3);
+    else if (N == 5 && tpl == 0) PDQEMU_LML(5, 0);
+    else if (N == 5 && tpl == 1) PDQEMU_LML(5, 1);
+    else if (N == 5 && tpl == 2) PDQEMU_LML(5, 2);
+    else if (N == 5 && tpl == 3) PDQEMU_LML(5, 3);
+    else return 2;
+#undef PDQEMU_LML
+  }
   return 0;
 }
 

@@ -2,14 +2,17 @@
 
 The gradient is pinned by Richardson-extrapolated central differences of the
 oracle's value, (4 D(h) - D(2h)) / 3 with D(h) = (f(x+h) - f(x-h)) / 2h and
-h = 1e-3 |x|: truncation O(h^4), rounding ~1e-13 relative.  On these inputs
-the kernel code (emulated on the CPU) agrees with it to ~1e-10 normwise.
+h = 1e-4 |x|, evaluated with the long double build of the oracle: truncation
+O(h^4) and rounding both stay near 1e-13 relative.  On these inputs the kernel
+code (emulated on the CPU) agrees with it to 1e-12 normwise or better; the
+tolerance is 1e-10.  (With FP64 function values the step must be ~10x larger
+to keep rounding down, and the truncation error then reaches ~1e-8.)
 """
 
 import numpy as np
 
 THETA = np.array([0.5, 0.05, 0.5, 0.05])
-GRAD_RTOL = 1e-8
+GRAD_RTOL = 1e-10
 
 
 def lml_inputs(n_inst, seed=0, t_end=20.0, n_grid=2001, every=20, per_coordinate_std=False):
@@ -47,21 +50,37 @@ def oracle_value(oracle, ocfg, params, y0, grid, every, data, stds, average=True
 
 
 def richardson_grad(oracle, ocfg, params, y0, grid, every, data, stds, average=True):
-    x = np.concatenate([params, y0])
+    """Richardson-extrapolated central differences (steps h and 2h) of the oracle's LML, evaluated
+    in the long double build: its values carry about 19 digits, so a step of 1e-4 |x| keeps both the
+    rounding error and the O(h^4) truncation error far below the 1e-10 tolerance the kernels are held to.
+
+    For solver_dynamic the output scales are recorded at x and replayed at every perturbed point,
+    which differentiates with the calibration held constant: the reference's stop_gradient
+    (solvers.py:568-569), the gradient the kernels compute."""
+    x = np.concatenate([params, y0]).astype(np.longdouble)
     n_p = params.size
+    dynamic = ocfg.solver == "dynamic"
+
+    with oracle.rounding("extended"):
+        tape = np.zeros((np.asarray(grid).size - 1) * max(1, y0.size), dtype=np.longdouble)
+        if dynamic:
+            oracle.lml_from_y0_tape(ocfg, x[:n_p], x[n_p:], grid, every, data, stds, tape, True, average)
 
     def f(z):
-        return oracle.lml_from_y0(ocfg, z[:n_p], z[n_p:], grid, every, data, stds, average)
+        if dynamic:
+            return np.longdouble(oracle.lml_from_y0_tape(ocfg, z[:n_p], z[n_p:], grid, every, data, stds, tape, False, average))
+        return np.longdouble(oracle.lml_from_y0(ocfg, z[:n_p], z[n_p:], grid, every, data, stds, average))
 
-    g = np.zeros(x.size)
-    for j in range(x.size):
-        h = 1e-3 * max(abs(x[j]), 1e-3)
-        e = np.zeros(x.size)
-        e[j] = h
-        d1 = (f(x + e) - f(x - e)) / (2 * h)
-        d2 = (f(x + 2 * e) - f(x - 2 * e)) / (4 * h)
-        g[j] = (4 * d1 - d2) / 3
-    return g
+    g = np.zeros(x.size, dtype=np.longdouble)
+    with oracle.rounding("extended"):
+        for j in range(x.size):
+            h = np.longdouble(1e-4) * max(abs(x[j]), np.longdouble(1e-3))
+            e = np.zeros(x.size, dtype=np.longdouble)
+            e[j] = h
+            d1 = (f(x + e) - f(x - e)) / (2 * h)
+            d2 = (f(x + 2 * e) - f(x - 2 * e)) / (4 * h)
+            g[j] = (4 * d1 - d2) / 3
+    return g.astype(np.float64)
 
 
 def rel(a, b):

@@ -242,3 +242,37 @@ def test_lane_shared_divisor_quotient_is_the_correctly_rounded_division(emu):
             e = np.float64(x) / np.float64(y)
         g = f(x, y)
         assert (np.isnan(e) and np.isnan(g)) or (g == e and np.signbit(g) == np.signbit(e)), (x, y, g, e)
+
+
+# ------------------------------------------------ lane-per-instance LML
+@pytest.mark.parametrize(
+    "nu,solver,average,tpl",
+    [(3, "solver", True, 3), (3, "solver", True, 1), (3, "dynamic", False, 3), (4, "solver", True, 2), (4, "dynamic", True, 1)],
+)
+def test_lane_lml_value_and_gradient(oracle, emu, nu, solver, average, tpl):
+    """The lane LML code (each lane: the shared factors, the primal and `tpl` tangent columns):
+    value against the oracle, forward-mode gradient against Richardson FD of the oracle."""
+    import lml_cases as LC
+    from probdiffeq_amd import capi
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(2, seed=3, t_end=10.0, n_grid=501, every=10)
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=nu, ode_dim=2, strategy="fixedpoint", solver=solver)
+    ocfg = T._oracle_cfg(oracle, cfg)
+    v, g = emu.lml(cfg, params, y0, grid, every, data, stds, average=average, kernel="lane", tangents_per_lane=tpl)
+    for i in range(2):
+        vo, vf, vt = LC.oracle_value(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
+        T.check_accuracy(v[i], vo, vf, vt, f"emulation lane lml instance {i}")
+        gr = LC.richardson_grad(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
+        assert LC.rel(g[i], gr) < LC.GRAD_RTOL, (g[i], gr)
+
+
+def test_lane_lml_value_only_matches_the_gradient_run(oracle, emu):
+    """Without tangents (tpl = 0) the lane LML gives the same value as with them."""
+    import lml_cases as LC
+    from probdiffeq_amd import capi
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(3, seed=8, t_end=10.0, n_grid=501, every=10)
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver")
+    v0, _ = emu.lml(cfg, params, y0, grid, every, data, stds, grad=False, kernel="lane")
+    v1, _ = emu.lml(cfg, params, y0, grid, every, data, stds, kernel="lane")
+    assert np.array_equal(v0, v1)

@@ -295,3 +295,32 @@ def test_lml_equals_bruteforce_joint_density(oracle, fact):
     brute = _lml_bruteforce(oracle, cfg, r, data, stds)
     assert np.isclose(lml_sum, brute, rtol=1e-8)
     assert np.isclose(lml_avg * K, lml_sum, rtol=1e-12)
+
+
+def test_calibration_tape_replays_and_freezes_the_output_scales(oracle):
+    """The oracle's calibration tape (stop_gradient of the dynamic calibration, solvers.py:568-569):
+    replaying the recorded scales reproduces the value bitwise; a wrong tape changes it; and the
+    derivative with the scales frozen differs from the one through them (the calibration does
+    depend on the parameters), by the small amount that sensitivity explains."""
+    import lml_cases as LC
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(1, seed=3, t_end=5.0, n_grid=251, every=10)
+    cfg = oracle.Config(fact="isotropic", nu=3, d=2, strategy="fixedpoint", solver="dynamic")
+    p, y, dat, sd = params[0], y0[0], data[0], stds[0]
+    tape = np.zeros(grid.size - 1)
+    v_rec = oracle.lml_from_y0_tape(cfg, p, y, grid, every, dat, sd, tape, True)
+    assert v_rec == oracle.lml_from_y0(cfg, p, y, grid, every, dat, sd)
+    assert np.all(tape > 0)
+    assert oracle.lml_from_y0_tape(cfg, p, y, grid, every, dat, sd, tape, False) == v_rec
+    wrong = 2.0 * tape
+    assert oracle.lml_from_y0_tape(cfg, p, y, grid, every, dat, sd, wrong, False) != v_rec
+    with pytest.raises(ValueError, match="too short"):
+        oracle.lml_from_y0_tape(cfg, p, y, grid, every, dat, sd, np.zeros(3), True)
+    # d/d a with the scales free vs frozen (central differences, h = 1e-5)
+    h = 1e-5
+    e = np.zeros(4)
+    e[0] = h
+    free = (oracle.lml_from_y0(cfg, p + e, y, grid, every, dat, sd) - oracle.lml_from_y0(cfg, p - e, y, grid, every, dat, sd)) / (2 * h)
+    frozen = (oracle.lml_from_y0_tape(cfg, p + e, y, grid, every, dat, sd, tape, False) - oracle.lml_from_y0_tape(cfg, p - e, y, grid, every, dat, sd, tape, False)) / (2 * h)
+    assert free != frozen
+    assert abs(free - frozen) < 1e-4 * abs(frozen)
