@@ -14,7 +14,9 @@
 // factors and the primal mean (redundantly: they need no data from each
 // other) and TPL of the n_tan = n_params + d tangent columns (lane g owns
 // tangents g TPL .. g TPL + TPL - 1), so a lane's mean has
-// MC = D (1 + TPL) columns: the primal, then its tangents, each D wide.
+// MC = D (1 + TPL) columns: the primal, then its tangents, each D wide.  The
+// device library instantiates TPL = 1 (G = n_tan lanes per instance) and
+// TPL = 0 (the value alone, one lane); the code is generic in TPL.
 //
 // The per-step algebra is the lane solve's (pdq_lane.h) instantiated with MC
 // mean columns.  Each lane keeps two solution slots in LDS (step_from and

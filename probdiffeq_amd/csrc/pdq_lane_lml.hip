@@ -46,14 +46,17 @@ int launch_lml_shape(LmlArgs& a, int tpl, hipStream_t st) {
   switch (tpl) {
     case 0: return launch_lml_tpl<N, D, 0>(a, st);
     case 1: return launch_lml_tpl<N, D, 1>(a, st);
-    case 3: return launch_lml_tpl<N, D, 3>(a, st);
   }
   return host_error(PDQ_ERR_UNSUPPORTED, "no lane LML kernel with %d tangents per lane", tpl);
 }
 
 }  // namespace
 
-int lane_lml_tangents_per_lane(int n_tan) { return n_tan == 0 ? 0 : (n_tan % 3 == 0 ? 3 : 1); }
+// One tangent column per lane: G = n_tan lanes per instance.  Against three
+// per lane (G = 2) this is 1.44x faster on the C5 batch (13.9 vs 20.0 ms):
+// the extra lanes are free (the batch occupies a fraction of the CUs) and each
+// lane's mean recursions shrink from 8 to 4 columns.
+int lane_lml_tangents_per_lane(int n_tan) { return n_tan == 0 ? 0 : 1; }
 
 bool lane_lml_supported(int N, int D, int solver) {
   return D == 2 && (N == 4 || N == 5) && (solver == PDQ_SOLVER || solver == PDQ_SOLVER_DYNAMIC);

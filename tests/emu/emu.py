@@ -82,7 +82,7 @@ def solve(cfg, params, tcoeffs, times, fixed_grid=False, kernel="coop"):
 
 def lml(cfg, params, y0, grid, every, data, stds, average=True, grad=True, kernel="coop", tangents_per_lane=None):
     """kernel: "coop" (the cooperative kernel's LML) or "lane" (the lane-per-instance LML with
-    `tangents_per_lane` tangent columns per lane; default 3 when it divides the direction count, else 1)."""
+    `tangents_per_lane` tangent columns per lane; default 1, the device library's choice)."""
     params, y0, grid, data, stds = (np.ascontiguousarray(x, dtype=np.float64) for x in (params, y0, grid, data, stds))
     B = params.shape[0]
     value = np.zeros(B)
@@ -92,7 +92,7 @@ def lml(cfg, params, y0, grid, every, data, stds, average=True, grad=True, kerne
     P = ctypes.c_void_p
     if kernel == "lane":
         n_tan = params.shape[1] + y0.shape[1] if grad else 0
-        tpl = tangents_per_lane if tangents_per_lane is not None else (0 if n_tan == 0 else (3 if n_tan % 3 == 0 else 1))
+        tpl = tangents_per_lane if tangents_per_lane is not None else (0 if n_tan == 0 else 1)
         rc = lib().pdqemu_lml_lane(ctypes.byref(c), B, P(params.ctypes.data), P(y0.ctypes.data), P(grid.ctypes.data), grid.size, every, P(data.ctypes.data), P(stds.ctypes.data), int(average), P(value.ctypes.data), P(g.ctypes.data) if grad else None, tpl)
     else:
         rc = lib().pdqemu_lml(ctypes.byref(c), B, P(params.ctypes.data), P(y0.ctypes.data), P(grid.ctypes.data), grid.size, every, P(data.ctypes.data), P(stds.ctypes.data), n_std, int(average), P(value.ctypes.data), P(g.ctypes.data) if grad else None)
