@@ -34,8 +34,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per C2 launch of the lane kernel, from rocprofv3 PMC runs
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, scales from a 1 GiB copy); see
 # profiles/lane_kernel_c2_isotropic_nu4_4096.md.  Re-measure when the kernel changes.
-C2_MEASURED_TRAFFIC_BYTES = 1_398_735 * 1024
+C2_MEASURED_TRAFFIC_BYTES = round((228_475.7 * 2 + 940_951.2) * 1024)
 C2_TRAFFIC_SOURCE = "profiles/lane_kernel_c2_isotropic_nu4_4096.md"
+C5_MEASURED_TRAFFIC_BYTES = round((121_126.1 * 2 + 240_058.0) * 1024)
+C5_TRAFFIC_SOURCE = "profiles/lane_lml_kernel_c5_isotropic_nu3_1024.md"
 
 
 def c2_inputs(n_inst=4096, K=200, seed=0):
@@ -162,6 +164,22 @@ def bench_c5(args, torch, capi, flops):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     res = {"lml_grads_per_s": P.shape[0] / (ms / 1e3), "lml_ms_per_batch": ms}
+    n_tan = params.shape[1] + y0.shape[1]
+    n_obs = data.shape[1]
+    fl = P.shape[0] * flops.lml_forward(cfg, grid.size - 1, n_obs, n_tan)
+    by = flops.lml_bytes(cfg, P.shape[0], n_obs, grid.size, n_tan)
+    res["lml_roofline"] = {
+        "bound": "mfma",
+        "achieved": fl / (ms / 1e3) / 1e12,
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": fl / (ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+        "flops_per_launch": fl,
+        "algorithmic_bytes": by,
+        "traffic": C5_MEASURED_TRAFFIC_BYTES if P.shape[0] == 1024 else None,
+        "traffic_source": C5_TRAFFIC_SOURCE,
+        "ms_per_launch": ms,
+    }
     # plain: central differences (h = 1e-6) of single-instance value launches, 13 per gradient
     n_plain = min(args.plain_sample, 8)
     e0, e1 = _events()
@@ -292,6 +310,7 @@ def main():
             lml_grads_per_s=c5["lml_grads_per_s"],
             plain_fd_grads_per_s=c5["plain_fd_grads_per_s"],
             adjoint_vs_plain=c5["lml_grads_per_s"] / c5["plain_fd_grads_per_s"],
+            lml_roofline=c5["lml_roofline"],
         )
     print(json.dumps(line), flush=True)
 

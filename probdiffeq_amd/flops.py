@@ -31,13 +31,15 @@ def trsm(k: int, r: int) -> float:
     return float(k * k * r)
 
 
-def dims(cfg):
+def dims(cfg, n_tan: int = 0):
+    """(nc, mc, ko, nb); with n_tan forward-mode directions every mean column carries n_tan
+    tangent columns beside it (mc grows by the factor 1 + n_tan, the factors do not)."""
     N, d = cfg.num_derivatives + 1, cfg.ode_dim
     if cfg.factorisation == "dense":
-        return N * d, 1, d, 1
+        return N * d, 1 + n_tan, d, 1
     if cfg.factorisation == "isotropic":
-        return N, d, 1, 1
-    return N, 1, 1, d
+        return N, d * (1 + n_tan), 1, 1
+    return N, 1 + n_tan, 1, d
 
 
 def _marginalise(nc, mc, rows_out):
@@ -54,9 +56,9 @@ def _merge(nc, mc):
     return gemm(nc, nc, nc) + gemm(nc, nc, mc) + gemm(nc, nc, nc) + qr(2 * nc, nc)
 
 
-def attempt(cfg) -> float:
+def attempt(cfg, n_tan: int = 0) -> float:
     """FLOPs of one attempted step (solver step + residual error estimate)."""
-    nc, mc, ko, nb = dims(cfg)
+    nc, mc, ko, nb = dims(cfg, n_tan)
     f = 0.0
     lin = _marginalise(nc, mc, ko) + trsm(ko, mc)  # fx.marginalise(u) + whitened residual
     if cfg.constraint == "ts1":
@@ -100,10 +102,11 @@ def solve(cfg, attempts: float, checkpoints: int) -> float:
     return attempts * attempt(cfg) + checkpoints * (interpolation(cfg) + finalize_per_checkpoint(cfg))
 
 
-def lml_forward(cfg, grid_steps: int, observations: int) -> float:
-    """Forward solve on the grid plus evaluate_lml's reverse scan (no tangents)."""
-    nc, mc, ko, nb = dims(cfg)
-    step = attempt(cfg) - (gemm(nc, nc, mc) + _marginalise(nc, mc, ko) + trsm(ko, mc))  # no error estimate
+def lml_forward(cfg, grid_steps: int, observations: int, n_tan: int = 0) -> float:
+    """Forward solve on the grid plus evaluate_lml's reverse scan; with n_tan > 0 the mean
+    recursions carry the tangent columns of a forward-mode gradient (the factors are shared)."""
+    nc, mc, ko, nb = dims(cfg, n_tan)
+    step = attempt(cfg, n_tan) - (gemm(nc, nc, mc) + _marginalise(nc, mc, ko) + trsm(ko, mc))  # no error estimate
     lml = observations * (_marginalise(nc, mc, nc) + _revert(nc, mc, ko) + qr(ko, ko) + trsm(ko, mc))
     return nb * (grid_steps * step) + nb * lml
 
@@ -123,3 +126,16 @@ def solve_bytes(cfg, n_inst: int, K: int) -> float:
         per_ckpt += ms + cs + cond                   # filtering marginal + backward conditional
         reread = (K - 1) * cond
     return 8.0 * n_inst * (inputs + K * per_ckpt + reread)
+
+
+def lml_bytes(cfg, n_inst: int, observations: int, grid_points: int, n_tan: int = 0) -> float:
+    """Algorithmic HBM bytes of one batched LML (+ gradient): inputs (params, y0, the grid, data
+    and one std per observation), outputs (value, gradient), and the backward conditional of every
+    interval between observations (factors once, mean columns with their tangents) written by the
+    forward pass and read back by the reverse scan."""
+    nc, mc, _, nb = dims(cfg, n_tan)
+    d = cfg.ode_dim
+    cond = nb * (2 * nc * nc + 2 * nc) + nc * mc  # A, noise factor, to_latent, to_observed; noise mean
+    inputs = n_inst * (cfg.n_params + d + observations * (d + 1)) + grid_points
+    outputs = n_inst * (1 + n_tan)
+    return 8.0 * (inputs + outputs + 2 * n_inst * (observations - 1) * cond)
