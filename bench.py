@@ -204,6 +204,47 @@ def bench_c5(args, torch, capi, flops):
     return res
 
 
+def bench_offgrid(args, torch, capi, flops, params, y0, save_at, n_query=256):
+    """Off-grid marginals of a fixed-interval smoother: a fixed-grid solve of the C2 instances on
+    their save_at grids, then n_query uniformly drawn times per instance, timed as one launch."""
+    dev = torch.device("cuda")
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=4, ode_dim=2, strategy="fixedinterval", solver="dynamic")
+    P = torch.tensor(params, dtype=torch.float64, device=dev)
+    Y0 = torch.tensor(y0, dtype=torch.float64, device=dev)
+    S = torch.tensor(save_at, dtype=torch.float64, device=dev)
+    tc = capi.taylor_coefficients("lotka_volterra", P, Y0, 0.0, 4)
+    out = capi.solve(cfg, P, tc, S, fixed_grid=True)
+    rng = np.random.default_rng(7)
+    tq = np.sort(rng.uniform(0.0, 1.0, size=(S.shape[0], n_query)), axis=1) * save_at[:, -1:]
+    TQ = torch.tensor(tq, dtype=torch.float64, device=dev)
+    for _ in range(max(1, args.warmup)):
+        capi.offgrid_marginals(cfg, out, S, TQ)
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    reps = max(1, args.steps)
+    e0.record()
+    for _ in range(reps):
+        capi.offgrid_marginals(cfg, out, S, TQ)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nq = S.shape[0] * n_query
+    fl = nq * flops.offgrid(cfg)
+    by = flops.offgrid_bytes(cfg, nq)
+    return {
+        "offgrid_marginals_per_s": nq / (ms / 1e3),
+        "offgrid_roofline": {
+            "workload": f"{S.shape[0]} fixed-interval solutions x {n_query} query times",
+            "ms_per_launch": ms,
+            "achieved_tflops": fl / (ms / 1e3) / 1e12,
+            "flops_frac": fl / (ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+            "algorithmic_bytes": by,
+            "achieved_gbs": by / (ms / 1e3) / 1e9,
+            "hbm_frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        },
+    }
+
+
 def cpu_baseline(params, y0, save_at, n_sample, threads):
     from oracle import oracle
 
@@ -240,6 +281,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--skip-lml", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-offgrid", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true", help="only run a 1 GiB device-to-device copy (scales the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -261,6 +303,9 @@ def main():
     c5 = {} if args.skip_lml else bench_c5(args, torch, capi, flops)
     if c5:
         _progress(f"C5 done: {c5['lml_grads_per_s']:.1f} gradients/s")
+    og = bench_offgrid(args, torch, capi, flops, params, y0, save_at) if not args.skip_offgrid else {}
+    if og:
+        _progress(f"off-grid marginals done: {og['offgrid_marginals_per_s']:.3e} per s")
     threads = min(16, os.cpu_count() or 1)
     cpu = None if args.skip_cpu else cpu_baseline(params, y0, save_at, args.cpu_sample, threads)
     if cpu:
@@ -312,6 +357,7 @@ def main():
             adjoint_vs_plain=c5["lml_grads_per_s"] / c5["plain_fd_grads_per_s"],
             lml_roofline=c5["lml_roofline"],
         )
+    line.update(og)
     print(json.dumps(line), flush=True)
 
 

@@ -159,6 +159,21 @@ int32_t pdq_taylor_coefficients(int32_t vector_field, int32_t d, int32_t n_param
                                 const double* params, const double* y0, double t0, int32_t num,
                                 double* out, void* stream);
 
+/* Replaces ProbabilisticSolver.offgrid_marginals(t, solution=)
+ * (probdiffeq/_probdiffeq/solvers.py:149-203; the strategies' parts are
+ * estimators_and_losses.py:403-414 and 674-707) for a batch of solutions
+ * and n_query times per instance.  The solution is what a solve with the
+ * same configuration wrote: t (as `times`), mean, chol, output_scale
+ * (K rows) and, for the fixed-interval smoother, filt_mean / filt_chol.
+ *   t_query:  n_inst x n_query, each strictly inside (times[0], times[K-1])
+ *             and off the grid points (as the reference requires)
+ *   mean_out: n_inst x n_query x mean_size, chol_out: ... x chol_size
+ * Filter and fixed-interval smoother; the fixed-point smoother is refused
+ * (the reference raises NotImplementedError).  Isotropic nu = 3, 4, d = 2. */
+int32_t pdq_offgrid_marginals(const pdq_config* cfg, int32_t n_inst, int32_t K, const double* times,
+                              const pdq_solution* solution, int32_t n_query, const double* t_query,
+                              double* mean_out, double* chol_out, void* stream);
+
 /* Sizes of the flat layouts above for a configuration. */
 int32_t pdq_mean_size(const pdq_config* cfg);
 int32_t pdq_chol_size(const pdq_config* cfg);

@@ -422,6 +422,24 @@ def lml_from_y0_tape(cfg: Config, params, y0, grid, every, data, stds, tape, rec
     return v
 
 
+def offgrid_marginal(cfg: Config, params, t0, t1, t, output_scale, m0, L0, m1=None, L1=None):
+    """offgrid_marginals (solvers.py:149-203) at one time t in (t0, t1): the filter predicts (m0, L0)
+    from t0; the fixed-interval smoother reverts the filtering marginal (m0, L0) at t0 to t and t1 and
+    marginalises the smoothed marginal (m1, L1) at t1 back to t.  Returns (mean, chol) in the
+    layouts of solve()'s outputs."""
+    params = _f64(params)
+    m0, L0 = _f64(m0), _f64(L0)
+    m1 = _f64(m0 if m1 is None else m1)
+    L1 = _f64(L0 if L1 is None else L1)
+    os_ = _f64(np.atleast_1d(output_scale))
+    mo, Lo = _zeros(m0.shape), _zeros(L0.shape)
+    c = cfg.c()
+    rc = lib().pdqo_offgrid_marginal(ctypes.byref(c), _p(params), _s(t0), _s(t1), _s(t), _p(os_), _p(m0), _p(L0), _p(m1), _p(L1), _p(mo), _p(Lo))
+    if rc == 2:
+        raise NotImplementedError("offgrid marginals need a filter or a fixed-interval smoother")
+    return mo, Lo
+
+
 def lml_grad_fd(cfg: Config, params, y0, grid, every, data, stds, h=1e-6, average=True):
     params, y0, grid, data, stds = map(_f64, (params, y0, grid, data, stds))
     stds2 = stds.reshape(data.shape[0], -1)

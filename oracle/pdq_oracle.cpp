@@ -1571,6 +1571,28 @@ void write_normal(const Problem& P, const BNormal& x, Real* mean, Real* chol) {
   if (chol) std::memcpy(chol, x.b[0].L.a.data(), sizeof(Real) * x.b[0].L.a.size());
 }
 
+BNormal read_normal(const Problem& P, const Real* mean, const Real* chol) {
+  BNormal x;
+  x.b.resize(P.nblk);
+  if (P.cfg.fact == FACT_BLOCKDIAG) {
+    for (int k = 0; k < P.d; ++k) {
+      x.b[k].m = Mat(P.N, 1);
+      x.b[k].L = Mat(P.N, P.N);
+      for (int i = 0; i < P.N; ++i) {
+        x.b[k].m(i, 0) = mean[k * P.N + i];
+        for (int j = 0; j < P.N; ++j) x.b[k].L(i, j) = chol[(size_t)k * P.N * P.N + i * P.N + j];
+      }
+    }
+    return x;
+  }
+  const int cols = P.cfg.fact == FACT_ISO ? P.d : 1;
+  x.b[0].m = Mat(P.n, cols);
+  x.b[0].L = Mat(P.n, P.n);
+  std::memcpy(x.b[0].m.a.data(), mean, sizeof(Real) * x.b[0].m.a.size());
+  std::memcpy(x.b[0].L.a.data(), chol, sizeof(Real) * x.b[0].L.a.size());
+  return x;
+}
+
 // A conditional's flat layout: A (nblk*n*n), noise mean (mean_size), noise
 // chol (chol_size), to_latent (nblk*n), to_observed (nblk*n).
 void write_cond(const Problem& P, const BCond& c, Real* A, Real* nm, Real* nc, Real* tl, Real* to) {
@@ -1772,6 +1794,39 @@ Real pdqo_lml_from_y0(const pdqo_cfg* cfg, const Real* params, const Real* y0, c
   std::vector<Real> tc((size_t)(cfg->nu + 1) * cfg->d);
   pdqo_taylor_coefficients(cfg->vf_id, cfg->d, params, y0, grid[0], cfg->nu, tc.data());
   return pdqo_lml(cfg, params, tc.data(), 0.0, grid, K, every, data, stds, nstd, average);
+}
+
+// ProbabilisticSolver.offgrid_marginals (solvers.py:149-203) for one query
+// time t in (t0, t1) of a solution grid, given the output scale of the
+// right-hand checkpoint (`os`, nblk entries):
+// * filter (estimators_and_losses.py:403-414): predict the marginal at t0
+//   (m0, L0) to t;
+// * fixed-interval smoother (:674-707): revert the filtering marginal at t0
+//   (m0, L0) to t and on to t1, then marginalise the smoothed marginal at t1
+//   (m1, L1) through the backward conditional t1 -> t.
+// The fixed-point smoother is not suitable (the reference raises).  Returns
+// 0, or 2 for an unsuitable strategy.
+int pdqo_offgrid_marginal(const pdqo_cfg* cfg, const Real* params, Real t0, Real t1, Real t, const Real* os,
+                          const Real* m0, const Real* L0, const Real* m1, const Real* L1, Real* m_out, Real* L_out) {
+  Problem P;
+  P.setup(*cfg, params);
+  if (P.cfg.strategy == STRAT_FIXEDPOINT) return 2;
+  const Vec osv(os, os + P.nblk);
+  const BNormal x0 = read_normal(P, m0, L0);
+  const BCond tr0 = transition(P, t - t0, osv);
+  BNormal out;
+  if (P.cfg.strategy == STRAT_FILTER) {
+    out = marginalise(P, tr0, x0);
+  } else {
+    BNormal xt, x1p;
+    BCond c_t_t0, c_t1_t;
+    revert(P, tr0, x0, SOLVE_TRIU, xt, c_t_t0);
+    const BCond tr1 = transition(P, t1 - t, osv);
+    revert(P, tr1, xt, SOLVE_TRIU, x1p, c_t1_t);
+    out = marginalise(P, c_t1_t, read_normal(P, m1, L1));
+  }
+  write_normal(P, out, m_out, L_out);
+  return 0;
 }
 
 // pdqo_lml_from_y0 with the calibration tape: mode 1 records the output

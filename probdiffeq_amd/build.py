@@ -26,6 +26,7 @@ UNITS = [
     "pdq_lane_fixedpoint.hip",
     "pdq_lane_fixedinterval.hip",
     "pdq_lane_lml.hip",
+    "pdq_lane_offgrid.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -186,6 +186,7 @@ EXPORTED = [
     "pdq_chol_size",
     "pdq_tuned_lanes",
     "pdq_set_tuned_lanes",
+    "pdq_offgrid_marginals",
 ]
 
 _lib = None
@@ -315,6 +316,20 @@ def lml_value_and_grad(cfg: Config, params, y0, grid, every, data, stds, *, aver
         )
     )
     return value, g
+
+
+def offgrid_marginals(cfg: Config, out: dict, times, t_query, stream=None):
+    """Marginals at off-grid times t_query (B, Q) of the solution `out` that a solve with `cfg`
+    wrote on `times` (B, K) (solvers.py:149-203).  Returns (mean (B, Q, mean_size),
+    chol (B, Q, chol_size))."""
+    B, K = times.shape
+    Qn = t_query.shape[1]
+    mean = torch.empty(B, Qn, cfg.mean_size, dtype=torch.float64, device=times.device)
+    chol = torch.empty(B, Qn, cfg.chol_size, dtype=torch.float64, device=times.device)
+    c = cfg.c()
+    s = _solution_struct(out)
+    check(lib().pdq_offgrid_marginals(ctypes.byref(c), B, K, _ptr(times), ctypes.byref(s), Qn, _ptr(t_query), _ptr(mean), _ptr(chol), _stream(stream)))
+    return mean, chol
 
 
 def taylor_coefficients(vector_field: str, params, y0, t0: float, num: int, stream=None):

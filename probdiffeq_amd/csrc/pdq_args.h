@@ -102,6 +102,24 @@ int lane_lml_tangents_per_lane(int n_tan);
 bool lane_lml_supported(int N, int D, int solver);
 size_t lane_lml_cond_size(int N, int D, int tpl);
 int launch_lane_lml(LmlArgs& a, int N, int D, int tpl, hipStream_t st);
+// Off-grid marginals (pdq_lane_offgrid.hip), isotropic shapes of the lane
+// kernel, filter and fixed-interval smoother.
+struct OffgridArgs {
+  pdq_config cfg;
+  int n_inst, K, n_query;
+  const double* times;         // n_inst x K
+  const double* t_query;       // n_inst x n_query
+  const double* mean;          // the solution's marginals: n_inst x K x (N d)
+  const double* chol;          // n_inst x K x N x N
+  const double* filt_mean;     // fixed-interval smoother: filtering marginals, same layouts
+  const double* filt_chol;
+  const double* output_scale;  // n_inst x K
+  const double* Q;
+  double* mean_out;  // n_inst x n_query x (N d)
+  double* chol_out;  // n_inst x n_query x N x N
+};
+int launch_offgrid(OffgridArgs& a, int N, int D, hipStream_t st);
+
 int launch_taylor_tangents(int vf, int d, int np, int n_inst, int num, int ntan, const double* params,
                            const double* y0, const double* t0, double* out, hipStream_t st);
 size_t coop_lds_bytes(const Layout& L, int G);

@@ -428,6 +428,50 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
   return rc;
 }
 
+int32_t pdq_offgrid_marginals(const pdq_config* cfg, int32_t n_inst, int32_t K, const double* times,
+                              const pdq_solution* solution, int32_t n_query, const double* t_query,
+                              double* mean_out, double* chol_out, void* stream) {
+  int rc = check_common(cfg, n_inst, K);
+  if (rc) return rc;
+  if (!solution) return set_err(PDQ_ERR_INVALID, "solution is NULL");
+  if (n_query < 0) return set_err(PDQ_ERR_INVALID, "n_query < 0");
+  if (cfg->strategy == PDQ_FIXEDPOINT)
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "offgrid marginals need a filter or a fixed-interval smoother (the fixed-point smoother is not "
+                   "suitable, solvers.py:163-164)");
+  const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
+  if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(N, d))
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "offgrid marginals are implemented for the isotropic factorisation with nu = 3 or 4 and d = 2 "
+                   "(got nu = %d, d = %d)",
+                   N - 1, d);
+  if (n_inst == 0 || n_query == 0) return PDQ_OK;
+  if (!times || !t_query || !mean_out || !chol_out || !solution->mean || !solution->chol || !solution->output_scale)
+    return set_err(PDQ_ERR_INVALID, "times, t_query, the outputs and the solution's mean, chol and output_scale "
+                                    "must be device pointers");
+  if (cfg->strategy == PDQ_FIXEDINTERVAL && (!solution->filt_mean || !solution->filt_chol))
+    return set_err(PDQ_ERR_INVALID, "the fixed-interval smoother needs the solution's filtering marginals");
+  DevMats m;
+  rc = device_mats(*cfg, false, m);
+  if (rc) return rc;
+  pdq::OffgridArgs a{};
+  a.cfg = *cfg;
+  a.n_inst = n_inst;
+  a.K = K;
+  a.n_query = n_query;
+  a.times = times;
+  a.t_query = t_query;
+  a.mean = solution->mean;
+  a.chol = solution->chol;
+  a.filt_mean = solution->filt_mean;
+  a.filt_chol = solution->filt_chol;
+  a.output_scale = solution->output_scale;
+  a.Q = m.Q;
+  a.mean_out = mean_out;
+  a.chol_out = chol_out;
+  return pdq::launch_offgrid(a, N, d, (hipStream_t)stream);
+}
+
 int32_t pdq_taylor_coefficients(int32_t vector_field, int32_t d, int32_t n_params, int32_t n_inst,
                                 const double* params, const double* y0, double t0, int32_t num, double* out,
                                 void* stream) {

@@ -408,7 +408,7 @@ __device__ __forceinline__ void lane_marginalise_prior(const double* __restrict_
 template <int N, int D, class Cond>
 __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ Q,
                                                   const double* P, const double* PI, double s, const double* Xm,
-                                                  const double* XL, double* Ym, double* YL, const Cond& bw) {
+                                                  const double* XL, double* Ym, double* YL, Cond&& bw) {
   constexpr int n2 = 2 * N;
   double big[n2 * n2];
   double tmp[N * D];

@@ -139,3 +139,22 @@ def lml_bytes(cfg, n_inst: int, observations: int, grid_points: int, n_tan: int 
     inputs = n_inst * (cfg.n_params + d + observations * (d + 1)) + grid_points
     outputs = n_inst * (1 + n_tan)
     return 8.0 * (inputs + outputs + 2 * n_inst * (observations - 1) * cond)
+
+
+def offgrid(cfg) -> float:
+    """FLOPs of one off-grid marginal (solvers.py:149-203): the filter marginalises the transition;
+    the fixed-interval smoother reverts it twice and marginalises one backward conditional."""
+    nc, mc, _, nb = dims(cfg)
+    if cfg.strategy == "filter":
+        return nb * _marginalise(nc, mc, nc)
+    return nb * (2 * _revert(nc, mc, nc) + _marginalise(nc, mc, nc))
+
+
+def offgrid_bytes(cfg, n_query: int) -> float:
+    """Algorithmic HBM bytes per batch of off-grid queries: each query reads its time, the two
+    bracketing marginals (the filter needs one) and an output scale, and writes one marginal.  The
+    binary search over the grid is not counted (it hits the cache)."""
+    nc, mc, _, nb = dims(cfg)
+    normal = nc * mc + nb * nc * nc
+    reads = 1 + 1 + normal * (1 if cfg.strategy == "filter" else 2)
+    return 8.0 * n_query * (reads + normal)

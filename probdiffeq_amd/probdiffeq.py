@@ -244,6 +244,7 @@ class ProbabilisticSolution:
     num_steps: torch.Tensor
     attempts: torch.Tensor  # attempted steps (accepted + rejected), per instance
     status: torch.Tensor    # 0 ok, 1 max attempts, 2 non-finite, 3 dt underflow
+    config: Any = None      # the capi.Config of the solve (the reference keeps solution.prior)
 
     def squeeze(self):
         """Drop the instance axis of a single-instance solve."""
@@ -410,6 +411,48 @@ class _Solver:
     def is_suitable_for_offgrid_marginals(self):
         return self.strategy.is_suitable_for_offgrid_marginals
 
+    def offgrid_marginals(self, t, *, solution):
+        """solvers.py:149-203: the marginal at t, strictly inside the solution's time interval and off
+        its grid points (the reference does not check this either).  `t` may be a scalar or hold
+        several times (a trailing axis; with a batched solution, a leading instance axis too): the
+        device evaluates them all in one launch (the reference vmaps the scalar call)."""
+        if not self.is_suitable_for_offgrid_marginals:
+            raise NotImplementedError
+        cfg = solution.config
+        if cfg is None:
+            raise ValueError("the solution does not carry the configuration of the solve that made it")
+        batched = solution.t.dim() == 2
+        t_all = solution.t if batched else solution.t[None]
+        B, K = t_all.shape
+        tq = _as_f64(t, t_all.device)
+        scalar = tq.dim() == 0
+        if tq.dim() <= 1:
+            tq = tq.reshape(1, -1).expand(B, -1)
+        tq = tq.contiguous()
+
+        def flat(x, width):
+            x = x if batched else x[None]
+            return x.reshape(B, K, width).contiguous()
+
+        N, d = cfg.N, cfg.ode_dim
+        out = {"mean": flat(solution.u.mean_flat, N * d), "chol": flat(solution.u.cholesky_flat, N * N)}
+        os_ = solution.output_scale if batched else solution.output_scale[None]
+        os_ = os_.reshape(B, -1)
+        if os_.shape[1] == K - 1:  # non-dynamic solvers: K - 1 rows; the reference's index K - 1 clamps
+            os_ = torch.cat([os_, os_[:, -1:]], dim=1)
+        out["output_scale"] = os_.contiguous()
+        if self.strategy.name == "fixedinterval":
+            filt = solution.solution_full.filtering
+            out["filt_mean"] = flat(filt.mean_flat, N * d)
+            out["filt_chol"] = flat(filt.cholesky_flat, N * N)
+        mean, chol = capi.offgrid_marginals(cfg, out, t_all.contiguous(), tq)
+        lead = (B, tq.shape[1]) if batched else (tq.shape[1],)
+        m = mean.reshape(*lead, N, d) if batched else mean[0].reshape(*lead, N, d)
+        L = chol.reshape(*lead, N, N) if batched else chol[0].reshape(*lead, N, N)
+        if scalar:
+            m, L = m[..., 0, :, :], L[..., 0, :, :]
+        return Normal(m, L, cfg.factorisation, N, d)
+
 
 class solver(_Solver):
     """solvers.py:636-767 (no calibration)."""
@@ -528,7 +571,7 @@ def build_solution(cfg: capi.Config, out: dict, *, batched: bool) -> Probabilist
         os_ = os_[:, : K - 1]
     if fact != "blockdiag":
         os_ = os_[..., 0]
-    sol = ProbabilisticSolution(out["t"], u, full, os_, out["num_steps"], out["attempts"], out["status"])
+    sol = ProbabilisticSolution(out["t"], u, full, os_, out["num_steps"], out["attempts"], out["status"], cfg)
     return sol if batched else sol.squeeze()
 
 

@@ -99,3 +99,26 @@ def lml(cfg, params, y0, grid, every, data, stds, average=True, grad=True, kerne
     if rc:
         raise RuntimeError(f"emulation rc {rc}")
     return value, g
+
+
+def offgrid(cfg, times, sol, t_query):
+    """The off-grid marginal kernel's code on the CPU: times (B, K), sol = dict with mean, chol,
+    output_scale (B, K, 1) and, for the fixed-interval smoother, filt_mean / filt_chol, in
+    solve()'s layouts; t_query (B, Q).  Returns (mean (B, Q, N d), chol (B, Q, N N))."""
+    arr = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    times, t_query = arr(times), arr(t_query)
+    B, K = times.shape
+    Qn = t_query.shape[1]
+    N, d = cfg.num_derivatives + 1, cfg.ode_dim
+    mean_out = np.zeros((B, Qn, N * d))
+    chol_out = np.zeros((B, Qn, N * N))
+    m, L, os_ = arr(sol["mean"]), arr(sol["chol"]), arr(sol["output_scale"])
+    fm = arr(sol["filt_mean"]) if "filt_mean" in sol else None
+    fL = arr(sol["filt_chol"]) if "filt_chol" in sol else None
+    P = ctypes.c_void_p
+    ptr = lambda x: P(x.ctypes.data) if x is not None else None
+    c = cfg.c()
+    rc = lib().pdqemu_offgrid(ctypes.byref(c), B, K, ptr(times), ptr(m), ptr(L), ptr(fm), ptr(fL), ptr(os_), Qn, ptr(t_query), ptr(mean_out), ptr(chol_out))
+    if rc:
+        raise RuntimeError(f"emulation rc {rc}")
+    return mean_out, chol_out

@@ -11,6 +11,7 @@
 #include "../../probdiffeq_amd/csrc/pdq_iwp.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane_lml.h"
+#include "../../probdiffeq_amd/csrc/pdq_lane_offgrid.h"
 
 namespace pdq {
 int host_error(int code, const char*, ...) { return code; }
@@ -288,6 +289,42 @@ int pdqemu_lml_lane(const pdq_config* cfg, int n_inst, const double* params, con
     else if (N == 5 && tpl == 3) PDQEMU_LML(5, 3);
     else return 2;
 #undef PDQEMU_LML
+  }
+  return 0;
+}
+
+
+// The off-grid marginal kernel (pdq_lane_offgrid.h) on the CPU, one query
+// after another.  Returns 2 for a shape or strategy it does not cover.
+int pdqemu_offgrid(const pdq_config* cfg, int n_inst, int K, const double* times, const double* mean,
+                   const double* chol, const double* filt_mean, const double* filt_chol, const double* output_scale,
+                   int n_query, const double* t_query, double* mean_out, double* chol_out) {
+  const int N = cfg->num_derivatives + 1, D = cfg->ode_dim;
+  if (cfg->factorisation != PDQ_ISOTROPIC || D != 2 || (N != 4 && N != 5) || cfg->strategy == PDQ_FIXEDPOINT)
+    return 2;
+  std::vector<double> A, Q;
+  pdq::iwp_matrices(cfg->num_derivatives, A, Q);
+  pdq::OffgridArgs a{};
+  a.cfg = *cfg;
+  a.n_inst = n_inst;
+  a.K = K;
+  a.n_query = n_query;
+  a.times = times;
+  a.t_query = t_query;
+  a.mean = mean;
+  a.chol = chol;
+  a.filt_mean = filt_mean;
+  a.filt_chol = filt_chol;
+  a.output_scale = output_scale;
+  a.Q = Q.data();
+  a.mean_out = mean_out;
+  a.chol_out = chol_out;
+  const size_t total = (size_t)n_inst * n_query;
+  for (size_t q = 0; q < total; ++q) {
+    if (N == 4 && cfg->strategy == PDQ_FILTER) pdq::lane_offgrid<4, 2, PDQ_FILTER>(a, q);
+    else if (N == 4) pdq::lane_offgrid<4, 2, PDQ_FIXEDINTERVAL>(a, q);
+    else if (cfg->strategy == PDQ_FILTER) pdq::lane_offgrid<5, 2, PDQ_FILTER>(a, q);
+    else pdq::lane_offgrid<5, 2, PDQ_FIXEDINTERVAL>(a, q);
   }
   return 0;
 }

@@ -1,0 +1,90 @@
+"""GPU: off-grid marginals (solvers.py:149-203) through the C ABI and the mirror API.
+
+The device kernel is held to the oracle's offgrid_marginal in its three builds (the accuracy bound
+of the solve parity tests); the oracle itself is pinned to the Gaussian prediction and the RTS step
+(tests/test_oracle_offgrid.py).
+"""
+
+import numpy as np
+import pytest
+
+import offgrid_cases as OC
+import test_gpu_parity as T
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(x):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64).cuda()
+
+
+@pytest.mark.parametrize("nu,strategy,solver", [(4, "filter", "dynamic"), (3, "filter", "solver"), (4, "fixedinterval", "dynamic"), (3, "fixedinterval", "mle")])
+def test_offgrid_marginals_match_the_oracle(oracle, nu, strategy, solver):
+    from probdiffeq_amd import capi
+
+    capi.require_device()
+    cfg, times, sol, tq, refs = OC.make_case(oracle, nu, strategy, solver)
+    print(f"offgrid case: nu={nu} {strategy}/{solver} K={times.shape[1]} Q={tq.shape[1]}", flush=True)
+    out = {k: _dev(v[None]) for k, v in sol.items()}
+    mean, chol = capi.offgrid_marginals(cfg, out, _dev(times), _dev(tq))
+    torch.cuda.synchronize()
+    OC.check(mean[0].cpu().numpy(), chol[0].cpu().numpy(), refs, f"gpu offgrid {strategy}/{solver} nu={nu}")
+
+
+def test_offgrid_refuses_the_fixedpoint_smoother_and_other_shapes():
+    from probdiffeq_amd import capi
+
+    capi.require_device()
+    times = _dev(np.linspace(0, 1, 5)[None])
+    tq = _dev([[0.3]])
+    out = {"mean": _dev(np.zeros((1, 5, 10))), "chol": _dev(np.zeros((1, 5, 25))), "output_scale": _dev(np.ones((1, 5, 1)))}
+    with pytest.raises(capi.PdqError, match="fixed-point smoother"):
+        capi.offgrid_marginals(capi.Config(factorisation="isotropic", strategy="fixedpoint"), out, times, tq)
+    with pytest.raises(capi.PdqError, match="isotropic"):
+        capi.offgrid_marginals(capi.Config(factorisation="dense", strategy="filter"), out, times, tq)
+
+
+@pytest.mark.parametrize("strategy", ["filter", "fixedinterval"])
+def test_api_offgrid_marginals_of_a_device_solve(oracle, strategy):
+    """solver.offgrid_marginals(t, solution=) on a batched fixed-grid solve made on the device,
+    against the oracle evaluated on that same solution's arrays."""
+    import test_gpu_api as A
+
+    from probdiffeq_amd import ivpsolve
+    from probdiffeq_amd import probdiffeq as pdx
+
+    rng = np.random.default_rng(4)
+    y0 = torch.tensor(rng.uniform(10, 30, size=(3, 2)), dtype=torch.float64, device="cuda")
+    init, slv, _ = A._setup(pdx, "isotropic", strategy, "dynamic", y0)
+    grid = torch.linspace(0.0, 4.0, 41, dtype=torch.float64, device="cuda")
+    sol = ivpsolve.solve_fixed_grid(solver=slv)(init, grid=grid)
+    tq = torch.tensor([0.37, 1.55, 3.93], dtype=torch.float64, device="cuda")
+    print(f"offgrid api case: {strategy} B=3 K=41 Q=3", flush=True)
+    rv = slv.offgrid_marginals(tq, solution=sol)
+    torch.cuda.synchronize()
+    assert rv.mean_flat.shape == (3, 3, 5, 2) and rv.cholesky_flat.shape == (3, 3, 5, 5)
+    one = slv.offgrid_marginals(1.55, solution=sol)
+    assert torch.equal(one.mean_flat, rv.mean_flat[:, 1])
+    ocfg = T._oracle_cfg(oracle, sol.config)
+    g = grid.cpu().numpy()
+    for b in range(3):
+        m = sol.u.mean_flat[b].cpu().numpy()
+        L = sol.u.cholesky_flat[b].cpu().numpy()
+        f = sol.solution_full.filtering if strategy == "fixedinterval" else sol.u
+        fm, fL = f.mean_flat[b].cpu().numpy(), f.cholesky_flat[b].cpu().numpy()
+        os_ = sol.output_scale[b].cpu().numpy().reshape(-1)
+        for q, t in enumerate(tq.cpu().numpy()):
+            i = int(np.searchsorted(g, t))
+            per = []
+            for build in ("strict", "fma", "extended"):
+                with oracle.rounding(build):
+                    mo, Lo = oracle.offgrid_marginal(ocfg, T.THETA, g[i - 1], g[i], t, os_[min(i, os_.size - 1)], fm[i - 1].ravel(), fL[i - 1].ravel(), m[i].ravel(), L[i].ravel())
+                per.append((np.asarray(mo, dtype=np.float64), np.asarray(Lo, dtype=np.float64)))
+            (mr, Lr), (mf, Lf), (mt, Lt) = per
+            T.check_accuracy(rv.mean_flat[b, q].cpu().numpy().ravel(), mr, mf, mt, f"api offgrid {strategy} {b} {q} mean")
+            T.check_accuracy(rv.cholesky_flat[b, q].cpu().numpy().ravel(), Lr, Lf, Lt, f"api offgrid {strategy} {b} {q} chol")
+    fp = pdx.solver_dynamic(strategy=pdx.strategy_smoother_fixedpoint(), constraint=slv.constraint)
+    with pytest.raises(NotImplementedError):
+        fp.offgrid_marginals(tq, solution=sol)

@@ -276,3 +276,16 @@ def test_lane_lml_value_only_matches_the_gradient_run(oracle, emu):
     v0, _ = emu.lml(cfg, params, y0, grid, every, data, stds, grad=False, kernel="lane")
     v1, _ = emu.lml(cfg, params, y0, grid, every, data, stds, kernel="lane")
     assert np.array_equal(v0, v1)
+
+
+# ------------------------------------------------ off-grid marginals
+@pytest.mark.parametrize("nu,strategy,solver", [(4, "filter", "dynamic"), (3, "filter", "solver"), (4, "fixedinterval", "dynamic"), (3, "fixedinterval", "mle")])
+def test_offgrid_marginals_kernel_code(oracle, emu, nu, strategy, solver):
+    """The off-grid marginal kernel's code (pdq_lane_offgrid.h) on an oracle solution, against the
+    oracle's offgrid_marginal (itself pinned to the Gaussian closed forms) in its three builds."""
+    import offgrid_cases as OC
+
+    cfg, times, sol, tq, refs = OC.make_case(oracle, nu, strategy, solver)
+    batch = {k: v[None] for k, v in sol.items()}
+    m, L = emu.offgrid(cfg, times, batch, tq)
+    OC.check(m[0], L[0], refs, f"emulation offgrid {strategy}/{solver} nu={nu}")
