@@ -174,6 +174,17 @@ int32_t pdq_offgrid_marginals(const pdq_config* cfg, int32_t n_inst, int32_t K, 
                               const pdq_solution* solution, int32_t n_query, const double* t_query,
                               double* mean_out, double* chol_out, void* stream);
 
+/* Replaces loss_lml_terminal_values(tcoeff_index=idx)(u, marginals=, std=)
+ * (probdiffeq/_probdiffeq/estimators_and_losses.py:20-50) for a batch of
+ * isotropic marginals (e.g. the terminal values of a solve):
+ *   mean: n_inst x (nu+1) x d, chol: n_inst x (nu+1) x (nu+1),
+ *   u: n_inst x d data, std: n_inst observation noise (one scalar each, as
+ *   marginals.std[idx] is for the isotropic factorisation),
+ *   value: n_inst log-densities. */
+int32_t pdq_lml_terminal_values(const pdq_config* cfg, int32_t n_inst, const double* mean, const double* chol,
+                                const double* u, const double* std, int32_t tcoeff_index, double* value,
+                                void* stream);
+
 /* Sizes of the flat layouts above for a configuration. */
 int32_t pdq_mean_size(const pdq_config* cfg);
 int32_t pdq_chol_size(const pdq_config* cfg);

@@ -242,6 +242,7 @@ def lib():
         L.pdqo_lml_from_y0.restype = _LongDoubleResult if ct is ctypes.c_longdouble else ct
         L.pdqo_lml_from_y0_tape.restype = _LongDoubleResult if ct is ctypes.c_longdouble else ct
         L.pdqo_lml_grad_fd.restype = ct
+        L.pdqo_lml_terminal.restype = ct
         L.pdqo_logpdf.restype = ct
         L.pdqo_control.restype = ct
         L.pdqo_solve_save_at_batch.restype = ctypes.c_int
@@ -420,6 +421,16 @@ def lml_from_y0_tape(cfg: Config, params, y0, grid, every, data, stds, tape, rec
     if np.isnan(v):
         raise ValueError("calibration tape too short")
     return v
+
+
+def lml_terminal(cfg: Config, params, mean, chol, u, std, tcoeff_index=0):
+    """loss_lml_terminal_values (estimators_and_losses.py:20-50) of one marginal (mean, chol in
+    solve()'s layouts) and data u (d,) with observation noise std (one entry, or one per coordinate
+    for dense / blockdiag)."""
+    params, mean, chol, u = _f64(params), _f64(mean), _f64(chol), _f64(u)
+    std = _f64(np.atleast_1d(std))
+    c = cfg.c()
+    return lib().pdqo_lml_terminal(ctypes.byref(c), _p(params), _p(mean), _p(chol), _p(u), _p(std), std.size, int(tcoeff_index))
 
 
 def offgrid_marginal(cfg: Config, params, t0, t1, t, output_scale, m0, L0, m1=None, L1=None):

@@ -1829,6 +1829,43 @@ int pdqo_offgrid_marginal(const pdqo_cfg* cfg, const Real* params, Real t0, Real
   return 0;
 }
 
+// loss_lml_terminal_values (estimators_and_losses.py:20-50): the log-density
+// of the data u under marginals.to_derivative(idx, std).marginalise(
+// marginals), i.e. derivative idx of the marginal observed with noise std
+// (std: 1 entry, or one per block / coordinate as the factorisation's
+// observation model takes them).
+Real pdqo_lml_terminal(const pdqo_cfg* cfg, const Real* params, const Real* mean, const Real* chol, const Real* u,
+                       const Real* std, int nstd, int idx) {
+  Problem P;
+  P.setup(*cfg, params);
+  const BNormal x = read_normal(P, mean, chol);
+  BCond model = observation_model(P, std, nstd);  // selects derivative 0
+  for (auto& C : model.b) {
+    if (P.cfg.fact == FACT_DENSE) {
+      C.A = Mat(P.d, P.n);
+      for (int k = 0; k < P.d; ++k) C.A(k, idx * P.d + k) = 1.0;
+    } else {
+      C.A = Mat(1, P.N);
+      C.A(0, idx) = 1.0;
+    }
+  }
+  const BNormal obs = marginalise(P, model, x);
+  Real lp = 0.0;
+  for (int i = 0; i < P.nblk; ++i) {
+    Mat data(obs.b[i].m.r, obs.b[i].m.c);
+    if (P.cfg.fact == FACT_BLOCKDIAG) {
+      data(0, 0) = u[i];
+      SSMBase iso;
+      iso.fact = FACT_ISO;
+      lp += iso.logpdf(obs.b[i], data);
+    } else {
+      for (size_t e = 0; e < data.a.size(); ++e) data.a[e] = u[e];
+      lp += P.ops.logpdf(obs.b[i], data);
+    }
+  }
+  return lp;
+}
+
 // pdqo_lml_from_y0 with the calibration tape: mode 1 records the output
 // scales of every solver_dynamic step into tape[0:tape_len], mode 2 replays
 // them.  Returns NaN if the tape is too short.

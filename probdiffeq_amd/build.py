@@ -27,6 +27,7 @@ UNITS = [
     "pdq_lane_fixedinterval.hip",
     "pdq_lane_lml.hip",
     "pdq_lane_offgrid.hip",
+    "pdq_post.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -187,6 +187,7 @@ EXPORTED = [
     "pdq_tuned_lanes",
     "pdq_set_tuned_lanes",
     "pdq_offgrid_marginals",
+    "pdq_lml_terminal_values",
 ]
 
 _lib = None
@@ -330,6 +331,16 @@ def offgrid_marginals(cfg: Config, out: dict, times, t_query, stream=None):
     s = _solution_struct(out)
     check(lib().pdq_offgrid_marginals(ctypes.byref(c), B, K, _ptr(times), ctypes.byref(s), Qn, _ptr(t_query), _ptr(mean), _ptr(chol), _stream(stream)))
     return mean, chol
+
+
+def lml_terminal_values(cfg: Config, mean, chol, u, std, tcoeff_index=0, stream=None):
+    """loss_lml_terminal_values (estimators_and_losses.py:20-50) for a batch of isotropic marginals:
+    mean (B, N, d), chol (B, N, N), data u (B, d), std (B,).  Returns (B,) log-densities."""
+    B = mean.shape[0]
+    value = torch.empty(B, dtype=torch.float64, device=mean.device)
+    c = cfg.c()
+    check(lib().pdq_lml_terminal_values(ctypes.byref(c), B, _ptr(mean), _ptr(chol), _ptr(u), _ptr(std), int(tcoeff_index), _ptr(value), _stream(stream)))
+    return value
 
 
 def taylor_coefficients(vector_field: str, params, y0, t0: float, num: int, stream=None):

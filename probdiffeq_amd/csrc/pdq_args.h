@@ -120,6 +120,10 @@ struct OffgridArgs {
 };
 int launch_offgrid(OffgridArgs& a, int N, int D, hipStream_t st);
 
+// loss_lml_terminal_values on isotropic marginals (pdq_post.hip).
+int launch_terminal_lml(const pdq_config& cfg, int n_inst, int idx, const double* mean, const double* chol,
+                        const double* u, const double* std, double* value, hipStream_t st);
+
 int launch_taylor_tangents(int vf, int d, int np, int n_inst, int num, int ntan, const double* params,
                            const double* y0, const double* t0, double* out, hipStream_t st);
 size_t coop_lds_bytes(const Layout& L, int G);

@@ -472,6 +472,24 @@ int32_t pdq_offgrid_marginals(const pdq_config* cfg, int32_t n_inst, int32_t K, 
   return pdq::launch_offgrid(a, N, d, (hipStream_t)stream);
 }
 
+int32_t pdq_lml_terminal_values(const pdq_config* cfg, int32_t n_inst, const double* mean, const double* chol,
+                                const double* u, const double* std, int32_t tcoeff_index, double* value,
+                                void* stream) {
+  if (!cfg) return set_err(PDQ_ERR_INVALID, "cfg is NULL");
+  if (n_inst < 0) return set_err(PDQ_ERR_INVALID, "n_inst < 0");
+  if (cfg->factorisation != PDQ_ISOTROPIC)
+    return set_err(PDQ_ERR_UNSUPPORTED, "loss_lml_terminal_values is implemented for the isotropic factorisation");
+  if (cfg->num_derivatives < 0 || cfg->num_derivatives > 7 || cfg->ode_dim < 1)
+    return set_err(PDQ_ERR_INVALID, "invalid num_derivatives or ode_dim");
+  if (tcoeff_index < 0 || tcoeff_index > cfg->num_derivatives)
+    return set_err(PDQ_ERR_INVALID, "tcoeff_index %d outside [0, %d]", tcoeff_index, cfg->num_derivatives);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
+  if (n_inst == 0) return PDQ_OK;
+  if (!mean || !chol || !u || !std || !value) return set_err(PDQ_ERR_INVALID, "null device pointer");
+  return pdq::launch_terminal_lml(*cfg, n_inst, tcoeff_index, mean, chol, u, std, value, (hipStream_t)stream);
+}
+
 int32_t pdq_taylor_coefficients(int32_t vector_field, int32_t d, int32_t n_params, int32_t n_inst,
                                 const double* params, const double* y0, double t0, int32_t num, double* out,
                                 void* stream) {

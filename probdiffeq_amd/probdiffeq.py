@@ -41,6 +41,7 @@ __all__ = [
     "jetexpand_ode_padded_scan",
     "linear",
     "logistic",
+    "loss_lml_terminal_values",
     "loss_lml_timeseries",
     "lotka_volterra",
     "ode",
@@ -576,6 +577,38 @@ def build_solution(cfg: capi.Config, out: dict, *, batched: bool) -> Probabilist
 
 
 # ---------------------------------------------------------------- the LML
+def loss_lml_terminal_values(*, tcoeff_index=0):
+    """estimators_and_losses.py:20-50 on the device: the log-density of the data u under the
+    marginals' derivative `tcoeff_index` observed with noise `std`.  ``marginals`` is an isotropic
+    Normal (e.g. a terminal-value solution's ``u``), single or batched; ``u`` has shape (d,) or (B, d)."""
+
+    def loss(u, /, *, marginals, std):
+        if marginals.factorisation != "isotropic":
+            raise NotImplementedError("loss_lml_terminal_values is implemented for the isotropic factorisation")
+        dev = marginals.mean_flat.device
+        u_t = _as_f64(u, dev)
+        std_t = _as_f64(std, dev)
+        std_expected = marginals.std[tcoeff_index]
+        if tuple(std_t.shape) != tuple(std_expected.shape):
+            msg = "The standard deviation container differs from what was expected."
+            msg += f" Expected: shape={tuple(std_expected.shape)}."
+            msg += f" Received: shape={tuple(std_t.shape)}."
+            msg += f" For reference: data-shape={tuple(u_t.shape)}."
+            raise ValueError(msg)
+        N, d = marginals.num_coeffs, marginals.ode_dim
+        batched = marginals.mean_flat.dim() == 3
+        m = marginals.mean_flat if batched else marginals.mean_flat[None]
+        L = marginals.cholesky_flat if batched else marginals.cholesky_flat[None]
+        B = m.shape[0]
+        cfg = capi.Config(factorisation="isotropic", num_derivatives=N - 1, ode_dim=d)
+        value = capi.lml_terminal_values(
+            cfg, m.contiguous(), L.contiguous(), u_t.reshape(B, d).contiguous(), std_t.reshape(B).contiguous(), tcoeff_index
+        )
+        return value if batched else value[0]
+
+    return loss
+
+
 def loss_lml_timeseries(*, average_pdfs: bool = True, tcoeff_index=0, solve_triu=None):
     """estimators_and_losses.py:53-105 on the device, as a differentiable torch op.
 

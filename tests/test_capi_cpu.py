@@ -114,3 +114,16 @@ def test_tuning_table_accepts_kernel_choices_and_rejects_others():
     assert capi.lib().pdq_tuned_lanes(ctypes.byref(c), 129) == 0
     assert capi.lib().pdq_set_tuned_lanes(ctypes.byref(c), 100, 0, 3) == 1  # PDQ_ERR_INVALID
     assert "lane kernel" in capi.last_error()
+
+
+def test_lml_terminal_values_checks_the_std_shape_like_the_reference():
+    """estimators_and_losses.py:24-42: a std container of the wrong shape is a ValueError (raised
+    before any device call)."""
+    import torch
+
+    from probdiffeq_amd import probdiffeq as pdq
+
+    marg = pdq.Normal(torch.zeros(5, 2, dtype=torch.float64), torch.eye(5, dtype=torch.float64), "isotropic", 5, 2)
+    loss = pdq.loss_lml_terminal_values()
+    with pytest.raises(ValueError, match="standard deviation container"):
+        loss(torch.zeros(2, dtype=torch.float64), marginals=marg, std=torch.ones(2, dtype=torch.float64))

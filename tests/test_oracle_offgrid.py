@@ -93,3 +93,36 @@ def test_offgrid_is_refused_for_the_fixedpoint_smoother(oracle):
     m0, L0 = np.zeros((4, 2)), np.eye(4)
     with pytest.raises(NotImplementedError):
         oracle.offgrid_marginal(cfg, THETA, 0.0, 1.0, 0.5, 1.0, m0.ravel(), L0.ravel())
+
+
+@pytest.mark.parametrize("fact,idx", [("isotropic", 0), ("isotropic", 2), ("dense", 0), ("dense", 1), ("blockdiag", 0)])
+def test_lml_terminal_values_is_the_gaussian_log_density(oracle, fact, idx):
+    """loss_lml_terminal_values (estimators_and_losses.py:20-50): the log-density of u under
+    derivative idx of the marginal plus independent noise, against scipy."""
+    from scipy.stats import multivariate_normal, norm
+
+    rng = np.random.default_rng(5 + idx)
+    nu, d = 3, 2
+    N = nu + 1
+    cfg = oracle.Config(fact=fact, nu=nu, d=d, strategy="filter")
+    u = rng.normal(size=d)
+    if fact == "isotropic":
+        m, L = _random_normal(rng, N, d)
+        std = 0.3
+        var = L[idx] @ L[idx] + std**2
+        expected = sum(norm.logpdf(u[k], m[idx, k], np.sqrt(var)) for k in range(d))
+        got = oracle.lml_terminal(cfg, THETA, m.ravel(), L.ravel(), u, std, idx)
+    elif fact == "dense":
+        m, L = _random_normal(rng, N * d, 1)
+        std = np.array([0.3, 0.5])
+        rows = L[idx * d : (idx + 1) * d]
+        cov = rows @ rows.T + np.diag(std**2)
+        expected = multivariate_normal(m[idx * d : (idx + 1) * d, 0], cov).logpdf(u)
+        got = oracle.lml_terminal(cfg, THETA, m.ravel(), L.ravel(), u, std, idx)
+    else:
+        ms = [rng.normal(size=N) for _ in range(d)]
+        Ls = [np.tril(rng.normal(size=(N, N))) + 2 * np.eye(N) for _ in range(d)]
+        std = np.array([0.3, 0.5])
+        expected = sum(norm.logpdf(u[k], ms[k][idx], np.sqrt(Ls[k][idx] @ Ls[k][idx] + std[k] ** 2)) for k in range(d))
+        got = oracle.lml_terminal(cfg, THETA, np.concatenate(ms), np.concatenate([x.ravel() for x in Ls]), u, std, idx)
+    assert np.isclose(got, expected, rtol=1e-13, atol=1e-13)
