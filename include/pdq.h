@@ -84,6 +84,8 @@ typedef struct pdq_config {
   int32_t correct_asymptotic_underconfidence; /* solver_mle */
   int32_t lanes_per_instance; /* 0: automatic; 1: one lane per instance (isotropic, nu = 3 or 4, d = 2);
                                  4, 8, 16, 32, 64: cooperative kernel with that many lanes per instance */
+  int32_t re_linearize_after_calibration; /* solver_dynamic(re_linearize_after_calibration=True),
+                                             solvers.py:576-582 */
 } pdq_config;
 
 /* Caller-allocated device outputs; NULL members are not written.

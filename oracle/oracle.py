@@ -65,6 +65,7 @@ def _cfg_struct(ct):
             ("base_scale", ct),
             ("correct_asymptotic_underconfidence", ctypes.c_int),
             ("max_attempts", ctypes.c_int),
+            ("re_linearize", ctypes.c_int),
         ]
 
     return _Cfg
@@ -146,6 +147,7 @@ class Config:
     base_scale: float = 1.0
     correct_asymptotic_underconfidence: bool = True
     max_attempts: int = 0
+    re_linearize_after_calibration: bool = False
 
     def c(self):
         return _CFG[_ct()](
@@ -172,6 +174,7 @@ class Config:
             self.base_scale,
             int(self.correct_asymptotic_underconfidence),
             self.max_attempts,
+            int(self.re_linearize_after_calibration),
         )
 
     @property

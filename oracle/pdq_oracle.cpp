@@ -516,6 +516,7 @@ struct pdqo_cfg {
   Real safety, factor_min, factor_max, exponent_integral, exponent_proportional;
   Real base_scale;  // prior output scale (scalar, broadcast)
   int correct_asymptotic_underconfidence, max_attempts;
+  int re_linearize;  // solver_dynamic(re_linearize_after_calibration=True)
 };
 }
 
@@ -1014,7 +1015,10 @@ Sol solver_step(const Problem& P, const Sol& st, Real dt) {
     BNormal upred;
     Post pred;
     predict(P, st.post, tr2, upred, pred);
-    BNormal upd = bayes_rule_zero(P, fx, upred, nullptr);
+    // solvers.py:576-582: optionally linearise again at the calibrated
+    // prediction; the step still reports the initial linearisation (:597)
+    const BCond fx_upd = P.cfg.re_linearize ? linearize(P, upred, tn) : fx;
+    BNormal upd = bayes_rule_zero(P, fx_upd, upred, nullptr);
     out.u = upd;
     out.post.marg = upd;
     if (smoother(P)) out.post.cond = pred.cond;

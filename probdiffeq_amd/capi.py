@@ -60,6 +60,7 @@ class _Config(ctypes.Structure):
         ("init_std", ctypes.c_double),
         ("correct_asymptotic_underconfidence", ctypes.c_int32),
         ("lanes_per_instance", ctypes.c_int32),
+        ("re_linearize_after_calibration", ctypes.c_int32),
     ]
 
 
@@ -121,6 +122,7 @@ class Config:
     init_std: float = 0.0
     correct_asymptotic_underconfidence: bool = True
     lanes_per_instance: int = 0
+    re_linearize_after_calibration: bool = False
 
     def c(self) -> _Config:
         return _Config(
@@ -150,6 +152,7 @@ class Config:
             self.init_std,
             int(self.correct_asymptotic_underconfidence),
             self.lanes_per_instance,
+            int(self.re_linearize_after_calibration),
         )
 
     @property

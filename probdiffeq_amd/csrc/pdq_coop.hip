@@ -619,6 +619,8 @@ __device__ double step(const Ctx<G>& c, int from, int to, double dt, bool estima
       }
     }
     c.g.sync();
+    // solvers.py:576-582: linearise again at the calibrated prediction
+    if (cfg.re_linearize_after_calibration) linearize(c, Pm, tn, R.b, R.J, y0buf);
     update(c, R.b, R.J, Pm, PL, R.m, R.L, big, gain, mo, nullptr);
     if (estimate) ep = error_power(c, oL, sigma, S.m, R.m, dt);
   } else {

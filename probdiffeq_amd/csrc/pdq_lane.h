@@ -782,6 +782,9 @@ __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int f
   const LaneWs<N, D>& w = c.w;
   double Rm[N * D], RL[N * N], mo[D], oLu, ep = 0.0;
   if (cfg.solver == PDQ_SOLVER_DYNAMIC) {
+    // solvers.py:576-582: linearise again at the calibrated prediction (the
+    // error estimate keeps the calibration's linearisation, :597)
+    if (cfg.re_linearize_after_calibration) lane_linearize<N, D>(cfg, c.prm, Pm, tn, b);
     lane_update<N, D>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
     if (estimate) ep = lane_error_power<N, D>(cfg, oL, sigma, Sm, Rm, h);
   } else {

@@ -113,7 +113,8 @@ __device__ __forceinline__ void lane_lml_step(const LaneCtx<N, D * (1 + TPL)>& c
   PDQ_PHASE_BOUNDARY();
   lane_predict<N, MC>(c, P, PI, lane_noise_scale(cfg, h, sigma), Sm, SLf, Pm, PL, to, true,
                       w.at(from, LS_IDENT) != 0.0, from, to);
-  if (cfg.solver != PDQ_SOLVER_DYNAMIC) lane_lml_linearize<N, D, TPL>(cfg, c.prm, q0, Pm, tn, b);
+  if (cfg.solver != PDQ_SOLVER_DYNAMIC || cfg.re_linearize_after_calibration)
+    lane_lml_linearize<N, D, TPL>(cfg, c.prm, q0, Pm, tn, b);
   double Rm[N * MC], RL[N * N], mo[MC], oLu;
   lane_update<N, MC>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
   w.store_mL(to, Rm, RL);

@@ -468,9 +468,7 @@ class solver_dynamic(_Solver):
 
     def __init__(self, *, strategy, constraint, constraint_init=None, re_linearize_after_calibration=False, stop_gradient_through_calibration=True):
         super().__init__(strategy=strategy, constraint=constraint, constraint_init=constraint_init)
-        if re_linearize_after_calibration:
-            raise NotImplementedError("re_linearize_after_calibration=True is not implemented on the device path")
-        self.re_linearize_after_calibration = re_linearize_after_calibration
+        self.re_linearize_after_calibration = bool(re_linearize_after_calibration)
         self.stop_gradient_through_calibration = stop_gradient_through_calibration
 
 
@@ -529,6 +527,8 @@ def make_config(prior: Prior, solver_obj: _Solver, error=None, control=None, cli
     )
     if isinstance(solver_obj, solver_mle):
         cfg.correct_asymptotic_underconfidence = solver_obj.correct_asymptotic_underconfidence
+    if isinstance(solver_obj, solver_dynamic):
+        cfg.re_linearize_after_calibration = solver_obj.re_linearize_after_calibration
     if error is not None:
         cfg.error_norm = error.error_norm
     if control is not None:

@@ -104,6 +104,7 @@ def _oracle_cfg(oracle, cfg):
         damp=cfg.damp,
         base_scale=cfg.base_output_scale,
         max_attempts=cfg.max_attempts,
+        re_linearize_after_calibration=cfg.re_linearize_after_calibration,
     )
 
 
@@ -312,3 +313,11 @@ def test_status_max_attempts(pdq):
     tcs = np.stack([O.taylor_coefficients("lotka_volterra", params[i], y0[i], 0.0, 4) for i in range(2)])
     out = pdq.solve(cfg, _dev(params), _dev(tcs), _dev(times))
     assert out["status"].cpu().numpy().tolist() == [1, 1]
+
+
+@pytest.mark.parametrize("fact,strategy,lanes", [("isotropic", "fixedpoint", 0), ("isotropic", "filter", 16), ("dense", "filter", 0)])
+def test_re_linearize_after_calibration(oracle, pdq, fact, strategy, lanes):
+    """solver_dynamic(re_linearize_after_calibration=True) (solvers.py:576-582): both kernels."""
+    cfg = pdq.Config(factorisation=fact, num_derivatives=4, strategy=strategy, solver="dynamic", lanes_per_instance=lanes, re_linearize_after_calibration=True)
+    params, y0s, times = _lv_batch(4, seed=31, t_lo=5.0, t_hi=10.0, K=15)
+    _compare(oracle, pdq, cfg, params, y0s, times)

@@ -289,3 +289,13 @@ def test_offgrid_marginals_kernel_code(oracle, emu, nu, strategy, solver):
     batch = {k: v[None] for k, v in sol.items()}
     m, L = emu.offgrid(cfg, times, batch, tq)
     OC.check(m[0], L[0], refs, f"emulation offgrid {strategy}/{solver} nu={nu}")
+
+
+@pytest.mark.parametrize("strategy", ["filter", "fixedpoint"])
+def test_lane_kernel_re_linearize_after_calibration(oracle, emu, strategy):
+    """solver_dynamic(re_linearize_after_calibration=True) (solvers.py:576-582) in the lane kernel's code."""
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver="dynamic", re_linearize_after_calibration=True)
+    p, y, t = T._lv_batch(3, seed=2, t_lo=5.0, t_hi=10.0, K=15)
+    _run(oracle, emu, cfg, p, y, t, kernel="lane")

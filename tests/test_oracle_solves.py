@@ -324,3 +324,33 @@ def test_calibration_tape_replays_and_freezes_the_output_scales(oracle):
     frozen = (oracle.lml_from_y0_tape(cfg, p + e, y, grid, every, dat, sd, tape, False) - oracle.lml_from_y0_tape(cfg, p - e, y, grid, every, dat, sd, tape, False)) / (2 * h)
     assert free != frozen
     assert abs(free - frozen) < 1e-4 * abs(frozen)
+
+
+@pytest.mark.parametrize("fact,strategy", [("dense", "filter"), ("isotropic", "fixedpoint")])
+def test_re_linearize_after_calibration_changes_only_the_rounding(oracle, fact, strategy):
+    """solvers.py:576-582: re-linearising at the calibrated prediction uses the same mean as the
+    calibration's extrapolation (the mean does not depend on the output scale), so the solve is the
+    same up to rounding: identical step counts, and outputs that differ from the plain solve by no
+    more than rounding does, i.e. within 32x the spread of the two FP64 builds around the long
+    double solve (the parity criterion of the device tests)."""
+    import dataclasses
+
+    cfg = oracle.Config(fact=fact, nu=4, d=2, strategy=strategy, solver="dynamic")
+    tc = oracle.taylor_coefficients("lotka_volterra", THETA, [20.0, 20.0], 0.0, 4)
+    save_at = np.linspace(0.0, 30.0, 40)
+    runs = {}
+    for build in ("strict", "fma", "extended"):
+        with oracle.rounding(build):
+            runs[build] = oracle.solve_save_at(cfg, THETA, tc, save_at)
+    r1 = oracle.solve_save_at(dataclasses.replace(cfg, re_linearize_after_calibration=True), THETA, tc, save_at)
+    assert runs["strict"]["attempts"] == r1["attempts"]
+    assert np.array_equal(runs["strict"]["num_steps"], r1["num_steps"])
+
+    def rel(a, b):
+        a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+        return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+    for key in ("mean", "chol", "output_scale"):
+        truth = runs["extended"][key]
+        e_ref = max(rel(runs["strict"][key], truth), rel(runs["fma"][key], truth))
+        assert rel(r1[key], truth) <= max(1e-10, 32 * e_ref), key
