@@ -245,6 +245,44 @@ def bench_offgrid(args, torch, capi, flops, params, y0, save_at, n_query=256):
     }
 
 
+def bench_sample(args, torch, capi, params, y0, save_at, n_samples=16):
+    """Posterior samples of the C2 fixed-point solutions: n_samples draws per instance, one launch."""
+    dev = torch.device("cuda")
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=4, ode_dim=2, strategy="fixedpoint", solver="dynamic", rtol=1e-6, atol=1e-8)
+    P = torch.tensor(params, dtype=torch.float64, device=dev)
+    Y0 = torch.tensor(y0, dtype=torch.float64, device=dev)
+    S = torch.tensor(save_at, dtype=torch.float64, device=dev)
+    tc = capi.taylor_coefficients("lotka_volterra", P, Y0, 0.0, 4)
+    out = capi.solve(cfg, P, tc, S)
+    B, K = S.shape
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(11)
+    base = torch.randn(B, n_samples, K, 5, generator=gen, dtype=torch.float64, device=dev)
+    for _ in range(max(1, args.warmup)):
+        capi.sample(cfg, out, K, base)
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    reps = max(1, args.steps)
+    e0.record()
+    for _ in range(reps):
+        capi.sample(cfg, out, K, base)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    # compulsory bytes: the draws and the samples once, the conditionals and terminal marginals once
+    by = 8.0 * (base.numel() + B * n_samples * K * 10 + B * ((K - 1) * (25 + 10 + 25 + 5 + 5) + 10 + 25))
+    return {
+        "posterior_samples_per_s": B * n_samples / (ms / 1e3),
+        "sample_roofline": {
+            "workload": f"{B} fixed-point solutions x {n_samples} samples of {K} points",
+            "ms_per_launch": ms,
+            "algorithmic_bytes": by,
+            "achieved_gbs": by / (ms / 1e3) / 1e9,
+            "hbm_frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        },
+    }
+
+
 def cpu_baseline(params, y0, save_at, n_sample, threads):
     from oracle import oracle
 
@@ -281,7 +319,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--skip-lml", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--skip-offgrid", action="store_true")
+    ap.add_argument("--skip-offgrid", action="store_true", help="skip the off-grid marginal and posterior-sample figures")
     ap.add_argument("--copy-calibration", action="store_true", help="only run a 1 GiB device-to-device copy (scales the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -306,6 +344,10 @@ def main():
     og = bench_offgrid(args, torch, capi, flops, params, y0, save_at) if not args.skip_offgrid else {}
     if og:
         _progress(f"off-grid marginals done: {og['offgrid_marginals_per_s']:.3e} per s")
+    smp = bench_sample(args, torch, capi, params, y0, save_at) if not args.skip_offgrid else {}
+    if smp:
+        _progress(f"posterior samples done: {smp['posterior_samples_per_s']:.3e} per s")
+    og.update(smp)
     threads = min(16, os.cpu_count() or 1)
     cpu = None if args.skip_cpu else cpu_baseline(params, y0, save_at, args.cpu_sample, threads)
     if cpu:

@@ -187,6 +187,20 @@ int32_t pdq_lml_terminal_values(const pdq_config* cfg, int32_t n_inst, const dou
                                 const double* u, const double* std, int32_t tcoeff_index, double* value,
                                 void* stream);
 
+/* Replaces MarkovSequence.sample(key, shape=(n_samples,))
+ * (probdiffeq/_probdiffeq/estimators_and_losses.py:233-273) on a smoother's
+ * posterior (solution.posterior: term_mean / term_chol and the K-1 backward
+ * conditionals cond_*, as a solve with this configuration wrote them).  The
+ * standard normal draws are an input instead of a JAX key:
+ *   base:    n_inst x n_samples x K x (nu+1); row K-1 samples the terminal
+ *            marginal, row k the conditional k (the isotropic sample_flat
+ *            draws one (nu+1)-vector per row for all d columns)
+ *   samples: n_inst x n_samples x K x mean_size
+ * Smoothers only (a filter's solution is no Markov sequence).  Isotropic
+ * nu = 3, 4, d = 2. */
+int32_t pdq_sample(const pdq_config* cfg, int32_t n_inst, int32_t K, const pdq_solution* solution,
+                   int32_t n_samples, const double* base, double* samples, void* stream);
+
 /* Sizes of the flat layouts above for a configuration. */
 int32_t pdq_mean_size(const pdq_config* cfg);
 int32_t pdq_chol_size(const pdq_config* cfg);

@@ -426,6 +426,21 @@ def lml_from_y0_tape(cfg: Config, params, y0, grid, every, data, stds, tape, rec
     return v
 
 
+def sample(cfg: Config, params, sol: dict, base):
+    """MarkovSequence.sample (estimators_and_losses.py:233-273) of a smoother solution `sol` (solve()'s
+    outputs: term_mean/term_chol and the backward conditionals) with the given standard normal draws
+    base (K, n).  Returns the K samples (K, mean_size)."""
+    params, base = _f64(params), _f64(base)
+    K = base.shape[0]
+    arrs = [_f64(sol[k]) for k in ("term_mean", "term_chol", "cond_A", "cond_mean", "cond_chol", "cond_tl", "cond_to")]
+    out = _zeros((K, arrs[0].size))
+    c = cfg.c()
+    rc = lib().pdqo_sample(ctypes.byref(c), _p(params), K, *[_p(a) for a in arrs], _p(base), _p(out))
+    if rc:
+        raise NotImplementedError("sampling is implemented for the dense and isotropic factorisations")
+    return out
+
+
 def lml_terminal(cfg: Config, params, mean, chol, u, std, tcoeff_index=0):
     """loss_lml_terminal_values (estimators_and_losses.py:20-50) of one marginal (mean, chol in
     solve()'s layouts) and data u (d,) with observation noise std (one entry, or one per coordinate

@@ -1611,6 +1611,34 @@ void write_cond(const Problem& P, const BCond& c, Real* A, Real* nm, Real* nc, R
   }
 }
 
+BCond read_cond(const Problem& P, const Real* A, const Real* nm, const Real* nc, const Real* tl, const Real* to) {
+  BCond c;
+  c.b.resize(P.nblk);
+  const BNormal noise = read_normal(P, nm, nc);
+  for (int i = 0; i < P.nblk; ++i) {
+    c.b[i].noise = noise.b[i];
+    c.b[i].A = Mat(P.n, P.n);
+    std::memcpy(c.b[i].A.a.data(), A + (size_t)i * P.n * P.n, sizeof(Real) * P.n * P.n);
+    c.b[i].tl = Vec(tl + (size_t)i * P.n, tl + (size_t)(i + 1) * P.n);
+    c.b[i].to = Vec(to + (size_t)i * P.n, to + (size_t)(i + 1) * P.n);
+  }
+  return c;
+}
+
+// Normal.sample_flat with given standard normal draws z: dense
+// (ssm_impl_dense.py:192-194) m + L z with z of length n; isotropic
+// (ssm_impl_isotropic.py:255-258) m + (L z)[:, None] with z of length N, the
+// same draw for every one of the d columns.
+Normal sample_with(const Normal& x, const Real* z) {
+  Normal out = x;
+  for (int i = 0; i < x.L.r; ++i) {
+    Real acc = 0.0;
+    for (int j = 0; j < x.L.c; ++j) acc += x.L(i, j) * z[j];
+    for (int k = 0; k < x.m.c; ++k) out.m(i, k) = x.m(i, k) + acc;
+  }
+  return out;
+}
+
 }  // namespace
 
 // =============================================================== C API
@@ -1868,6 +1896,33 @@ Real pdqo_lml_terminal(const pdqo_cfg* cfg, const Real* params, const Real* mean
     }
   }
   return lp;
+}
+
+// MarkovSequence.sample (estimators_and_losses.py:233-273) of a smoother's
+// posterior (the terminal marginal and K-1 backward conditionals, in solve()'s
+// layouts) for one sample, with its standard normal draws given (the
+// reference draws them from JAX keys): base (K x n), row K-1 for the
+// terminal marginal, row k for the conditional k.  The reverse scan
+// propagates the previous sample with cond.apply_flat and samples the result.
+// Writes the K samples (K x mean_size).  Dense and isotropic; returns 2
+// otherwise.
+int pdqo_sample(const pdqo_cfg* cfg, const Real* params, int K, const Real* term_mean, const Real* term_chol,
+                const Real* cA, const Real* cm, const Real* cL, const Real* ctl, const Real* cto, const Real* base,
+                Real* out) {
+  Problem P;
+  P.setup(*cfg, params);
+  if (P.cfg.fact == FACT_BLOCKDIAG) return 2;
+  const int n = P.n, ms = mean_size(P), cs = chol_size(P);
+  Normal x = sample_with(read_normal(P, term_mean, term_chol).b[0], base + (size_t)(K - 1) * n);
+  std::memcpy(out + (size_t)(K - 1) * ms, x.m.a.data(), sizeof(Real) * ms);
+  for (int k = K - 2; k >= 0; --k) {
+    const BCond c = read_cond(P, cA + (size_t)k * n * n, cm + (size_t)k * ms, cL + (size_t)k * cs, ctl + (size_t)k * n,
+                              cto + (size_t)k * n);
+    const Normal pred = P.ops.apply_flat(c.b[0], x.m);
+    x = sample_with(pred, base + (size_t)k * n);
+    std::memcpy(out + (size_t)k * ms, x.m.a.data(), sizeof(Real) * ms);
+  }
+  return 0;
 }
 
 // pdqo_lml_from_y0 with the calibration tape: mode 1 records the output

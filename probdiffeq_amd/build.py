@@ -28,6 +28,7 @@ UNITS = [
     "pdq_lane_lml.hip",
     "pdq_lane_offgrid.hip",
     "pdq_post.hip",
+    "pdq_lane_sample.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -191,6 +191,7 @@ EXPORTED = [
     "pdq_set_tuned_lanes",
     "pdq_offgrid_marginals",
     "pdq_lml_terminal_values",
+    "pdq_sample",
 ]
 
 _lib = None
@@ -334,6 +335,18 @@ def offgrid_marginals(cfg: Config, out: dict, times, t_query, stream=None):
     s = _solution_struct(out)
     check(lib().pdq_offgrid_marginals(ctypes.byref(c), B, K, _ptr(times), ctypes.byref(s), Qn, _ptr(t_query), _ptr(mean), _ptr(chol), _stream(stream)))
     return mean, chol
+
+
+def sample(cfg: Config, out: dict, K: int, base, stream=None):
+    """Posterior samples (estimators_and_losses.py:233-273) of the smoother solution `out` that a solve
+    with `cfg` wrote on K points, from the standard normal draws base (B, S, K, N).  Returns
+    samples (B, S, K, mean_size)."""
+    B, S = base.shape[0], base.shape[1]
+    samples = torch.empty(B, S, K, cfg.mean_size, dtype=torch.float64, device=base.device)
+    c = cfg.c()
+    s = _solution_struct(out)
+    check(lib().pdq_sample(ctypes.byref(c), B, K, ctypes.byref(s), S, _ptr(base), _ptr(samples), _stream(stream)))
+    return samples
 
 
 def lml_terminal_values(cfg: Config, mean, chol, u, std, tcoeff_index=0, stream=None):

@@ -120,6 +120,22 @@ struct OffgridArgs {
 };
 int launch_offgrid(OffgridArgs& a, int N, int D, hipStream_t st);
 
+// Posterior samples of a smoother (pdq_lane_sample.hip), isotropic.
+struct SampleArgs {
+  pdq_config cfg;
+  int n_inst, K, n_samples;
+  const double* term_mean;  // n_inst x (N d)
+  const double* term_chol;  // n_inst x N x N
+  const double* cond_A;     // n_inst x (K - 1) x N x N
+  const double* cond_mean;  // n_inst x (K - 1) x (N d)
+  const double* cond_chol;  // n_inst x (K - 1) x N x N
+  const double* cond_tl;    // n_inst x (K - 1) x N
+  const double* cond_to;    // n_inst x (K - 1) x N
+  const double* base;       // n_inst x n_samples x K x N standard normal draws
+  double* samples;          // n_inst x n_samples x K x (N d)
+};
+int launch_sample(SampleArgs& a, int N, int D, hipStream_t st);
+
 // loss_lml_terminal_values on isotropic marginals (pdq_post.hip).
 int launch_terminal_lml(const pdq_config& cfg, int n_inst, int idx, const double* mean, const double* chol,
                         const double* u, const double* std, double* value, hipStream_t st);

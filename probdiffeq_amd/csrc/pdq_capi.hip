@@ -472,6 +472,42 @@ int32_t pdq_offgrid_marginals(const pdq_config* cfg, int32_t n_inst, int32_t K, 
   return pdq::launch_offgrid(a, N, d, (hipStream_t)stream);
 }
 
+int32_t pdq_sample(const pdq_config* cfg, int32_t n_inst, int32_t K, const pdq_solution* solution,
+                   int32_t n_samples, const double* base, double* samples, void* stream) {
+  int rc = check_common(cfg, n_inst, K);
+  if (rc) return rc;
+  if (!solution) return set_err(PDQ_ERR_INVALID, "solution is NULL");
+  if (n_samples < 0) return set_err(PDQ_ERR_INVALID, "n_samples < 0");
+  if (cfg->strategy == PDQ_FILTER)
+    return set_err(PDQ_ERR_UNSUPPORTED, "sampling needs a smoother's posterior (a filter's solution is no Markov sequence)");
+  const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
+  if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(N, d))
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "sampling is implemented for the isotropic factorisation with nu = 3 or 4 and d = 2 (got nu = %d, "
+                   "d = %d)",
+                   N - 1, d);
+  if (n_inst == 0 || n_samples == 0) return PDQ_OK;
+  const pdq_solution& s = *solution;
+  if (!base || !samples || !s.term_mean || !s.term_chol || !s.cond_A || !s.cond_mean || !s.cond_chol || !s.cond_tl ||
+      !s.cond_to)
+    return set_err(PDQ_ERR_INVALID, "base, samples and the solution's term_* and cond_* must be device pointers");
+  pdq::SampleArgs a{};
+  a.cfg = *cfg;
+  a.n_inst = n_inst;
+  a.K = K;
+  a.n_samples = n_samples;
+  a.term_mean = s.term_mean;
+  a.term_chol = s.term_chol;
+  a.cond_A = s.cond_A;
+  a.cond_mean = s.cond_mean;
+  a.cond_chol = s.cond_chol;
+  a.cond_tl = s.cond_tl;
+  a.cond_to = s.cond_to;
+  a.base = base;
+  a.samples = samples;
+  return pdq::launch_sample(a, N, d, (hipStream_t)stream);
+}
+
 int32_t pdq_lml_terminal_values(const pdq_config* cfg, int32_t n_inst, const double* mean, const double* chol,
                                 const double* u, const double* std, int32_t tcoeff_index, double* value,
                                 void* stream) {

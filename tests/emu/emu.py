@@ -122,3 +122,21 @@ def offgrid(cfg, times, sol, t_query):
     if rc:
         raise RuntimeError(f"emulation rc {rc}")
     return mean_out, chol_out
+
+
+def sample(cfg, sol, base):
+    """The posterior sampler's code on the CPU: sol = one instance's term_*/cond_* (solve()'s layouts),
+    base (S, K, N).  Returns (S, K, N d)."""
+    arr = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    base = arr(base)
+    S, K, N = base.shape
+    d = cfg.ode_dim
+    out = np.zeros((S, K, N * d))
+    keys = ("term_mean", "term_chol", "cond_A", "cond_mean", "cond_chol", "cond_tl", "cond_to")
+    arrs = [arr(sol[k]) for k in keys]
+    P = ctypes.c_void_p
+    c = cfg.c()
+    rc = lib().pdqemu_sample(ctypes.byref(c), 1, K, *[P(a.ctypes.data) for a in arrs], S, P(base.ctypes.data), P(out.ctypes.data))
+    if rc:
+        raise RuntimeError(f"emulation rc {rc}")
+    return out

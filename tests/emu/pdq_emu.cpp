@@ -12,6 +12,7 @@
 #include "../../probdiffeq_amd/csrc/pdq_lane.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane_lml.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane_offgrid.h"
+#include "../../probdiffeq_amd/csrc/pdq_lane_sample.h"
 
 namespace pdq {
 int host_error(int code, const char*, ...) { return code; }
@@ -325,6 +326,35 @@ int pdqemu_offgrid(const pdq_config* cfg, int n_inst, int K, const double* times
     else if (N == 4) pdq::lane_offgrid<4, 2, PDQ_FIXEDINTERVAL>(a, q);
     else if (cfg->strategy == PDQ_FILTER) pdq::lane_offgrid<5, 2, PDQ_FILTER>(a, q);
     else pdq::lane_offgrid<5, 2, PDQ_FIXEDINTERVAL>(a, q);
+  }
+  return 0;
+}
+
+
+// The posterior sampler (pdq_lane_sample.h) on the CPU, one (instance,
+// sample) lane after another.
+int pdqemu_sample(const pdq_config* cfg, int n_inst, int K, const double* term_mean, const double* term_chol,
+                  const double* cA, const double* cm, const double* cL, const double* ctl, const double* cto,
+                  int n_samples, const double* base, double* samples) {
+  const int N = cfg->num_derivatives + 1, D = cfg->ode_dim;
+  if (cfg->factorisation != PDQ_ISOTROPIC || D != 2 || (N != 4 && N != 5)) return 2;
+  pdq::SampleArgs a{};
+  a.cfg = *cfg;
+  a.n_inst = n_inst;
+  a.K = K;
+  a.n_samples = n_samples;
+  a.term_mean = term_mean;
+  a.term_chol = term_chol;
+  a.cond_A = cA;
+  a.cond_mean = cm;
+  a.cond_chol = cL;
+  a.cond_tl = ctl;
+  a.cond_to = cto;
+  a.base = base;
+  a.samples = samples;
+  for (size_t g = 0; g < (size_t)n_inst * n_samples; ++g) {
+    if (N == 4) pdq::lane_sample<4, 2>(a, g);
+    else pdq::lane_sample<5, 2>(a, g);
   }
   return 0;
 }

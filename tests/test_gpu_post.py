@@ -61,3 +61,61 @@ def test_api_lml_terminal_values_of_a_terminal_value_solve(oracle):
             with oracle.rounding(build):
                 vals.append(float(oracle.lml_terminal(ocfg, T.THETA, m, L, data[b].cpu().numpy(), 0.5, 0)))
         T.check_accuracy(value[b].item(), *vals, f"terminal lml api instance {b}")
+
+
+@pytest.mark.parametrize("nu", [3, 4])
+def test_posterior_samples_match_the_oracle(oracle, nu):
+    """MarkovSequence.sample (estimators_and_losses.py:233-273) through the C ABI, with the oracle's
+    draws, against the oracle in its three builds."""
+    import sample_cases as SC
+
+    from probdiffeq_amd import capi
+
+    capi.require_device()
+    cfg, sol, base, refs = SC.make_case(oracle, nu)
+    K = base.shape[1]
+    print(f"sample case: nu={nu} S={base.shape[0]} K={K}", flush=True)
+    out = {k: _dev(v[None]) for k, v in sol.items()}
+    samples = capi.sample(cfg, out, K, _dev(base[None]))
+    torch.cuda.synchronize()
+    SC.check(samples[0].cpu().numpy(), refs, f"gpu sample nu={nu}")
+
+
+def test_api_posterior_samples():
+    """The mirror's MarkovSequence.sample: shapes, a seed's reproducibility, and the draws path equal
+    to the C ABI's; a filter's solution has no posterior to sample."""
+    import test_gpu_api as A
+
+    from probdiffeq_amd import ivpsolve
+    from probdiffeq_amd import probdiffeq as pdx
+
+    y0 = torch.tensor([[20.0, 20.0], [15.0, 25.0], [25.0, 12.0]], dtype=torch.float64, device="cuda")
+    init, slv, err = A._setup(pdx, "isotropic", "fixedpoint", "dynamic", y0)
+    save_at = torch.linspace(0.0, 10.0, 30, dtype=torch.float64, device="cuda")
+    sol = ivpsolve.solve_adaptive_save_at(solver=slv, error=err)(init, save_at=save_at, atol=1e-8, rtol=1e-6)
+    post = sol.solution_full.posterior
+    print("sample api case: B=3 K=30", flush=True)
+    s1 = post.sample(7, shape=(4,))
+    s2 = post.sample(7, shape=(4,))
+    assert s1.shape == (3, 4, 30, 5, 2)
+    assert torch.equal(s1, s2)
+    assert post.sample(3).shape == (3, 30, 5, 2)
+    base = torch.randn(3, 2, 30, 5, dtype=torch.float64, device="cuda")
+    via_api = post.sample_from_draws(base)
+    from probdiffeq_amd import capi
+
+    out = {
+        "term_mean": post.marginal.mean_flat.reshape(3, 10).contiguous(),
+        "term_chol": post.marginal.cholesky_flat.reshape(3, 25).contiguous(),
+        "cond_A": post.conditional.A.reshape(3, 29, 25).contiguous(),
+        "cond_mean": post.conditional.noise.mean_flat.reshape(3, 29, 10).contiguous(),
+        "cond_chol": post.conditional.noise.cholesky_flat.reshape(3, 29, 25).contiguous(),
+        "cond_tl": post.conditional.to_latent.reshape(3, 29, 5).contiguous(),
+        "cond_to": post.conditional.to_observed.reshape(3, 29, 5).contiguous(),
+    }
+    direct = capi.sample(sol.config, out, 30, base)
+    assert torch.equal(via_api.reshape(3, 2, 30, 10), direct)
+    # the terminal sample row is the terminal marginal's mean plus its factor times the draw
+    lz = torch.matmul(post.marginal.cholesky_flat[:, None].contiguous(), base[:, :, -1, :, None].contiguous())[..., 0]
+    exp = post.marginal.mean_flat[:, None] + lz[..., None]
+    assert torch.allclose(via_api[:, :, -1], exp, rtol=1e-14, atol=1e-12)

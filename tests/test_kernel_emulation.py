@@ -299,3 +299,13 @@ def test_lane_kernel_re_linearize_after_calibration(oracle, emu, strategy):
     cfg = capi.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver="dynamic", re_linearize_after_calibration=True)
     p, y, t = T._lv_batch(3, seed=2, t_lo=5.0, t_hi=10.0, K=15)
     _run(oracle, emu, cfg, p, y, t, kernel="lane")
+
+
+@pytest.mark.parametrize("nu", [3, 4])
+def test_posterior_sampler_kernel_code(oracle, emu, nu):
+    """The sampler's code (pdq_lane_sample.h) against the oracle's MarkovSequence.sample with the
+    same draws (the oracle pinned in tests/test_oracle_sample.py)."""
+    import sample_cases as SC
+
+    cfg, sol, base, refs = SC.make_case(oracle, nu)
+    SC.check(emu.sample(cfg, sol, base), refs, f"emulation sample nu={nu}")
