@@ -39,6 +39,8 @@ enum { PDQ_SOLVER = 0, PDQ_SOLVER_DYNAMIC = 1, PDQ_SOLVER_MLE = 2 };
 enum { PDQ_NORM_SCALE_THEN_RMS = 0, PDQ_NORM_RMS_THEN_SCALE = 1 };
 /* control_integral / control_proportional_integral (controllers.py:24-84) */
 enum { PDQ_CONTROL_INTEGRAL = 0, PDQ_CONTROL_PI = 1 };
+/* error_residual_std (solvers.py:850-996), error_state_std (:999-1098) */
+enum { PDQ_ERROR_RESIDUAL_STD = 0, PDQ_ERROR_STATE_STD = 1 };
 /* device vector-field registry (problems.py:283-295 convention u' = f(u, t)) */
 enum {
   PDQ_VF_LOTKA_VOLTERRA = 0, /* params (a, b, c, d), d = 2 */
@@ -86,6 +88,13 @@ typedef struct pdq_config {
                                  4, 8, 16, 32, 64: cooperative kernel with that many lanes per instance */
   int32_t re_linearize_after_calibration; /* solver_dynamic(re_linearize_after_calibration=True),
                                              solvers.py:576-582 */
+  /* Error estimator (solvers.py:850-1098): PDQ_ERROR_RESIDUAL_STD or
+   * PDQ_ERROR_STATE_STD, and their options re_linearize_before_error,
+   * derivative_idx (state std only) and error_per_unit_step. */
+  int32_t error_estimator;
+  int32_t error_re_linearize;
+  int32_t error_derivative_idx;
+  int32_t error_per_unit_step;
 } pdq_config;
 
 /* Caller-allocated device outputs; NULL members are not written.

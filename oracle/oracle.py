@@ -66,6 +66,10 @@ def _cfg_struct(ct):
             ("correct_asymptotic_underconfidence", ctypes.c_int),
             ("max_attempts", ctypes.c_int),
             ("re_linearize", ctypes.c_int),
+            ("error_estimator", ctypes.c_int),
+            ("error_re_linearize", ctypes.c_int),
+            ("error_derivative_idx", ctypes.c_int),
+            ("error_per_unit_step", ctypes.c_int),
         ]
 
     return _Cfg
@@ -148,6 +152,10 @@ class Config:
     correct_asymptotic_underconfidence: bool = True
     max_attempts: int = 0
     re_linearize_after_calibration: bool = False
+    error_estimator: str = "residual_std"
+    error_re_linearize: bool = False
+    error_derivative_idx: int = 0
+    error_per_unit_step: bool = False
 
     def c(self):
         return _CFG[_ct()](
@@ -175,6 +183,10 @@ class Config:
             int(self.correct_asymptotic_underconfidence),
             self.max_attempts,
             int(self.re_linearize_after_calibration),
+            {"residual_std": 0, "state_std": 1}[self.error_estimator],
+            int(self.error_re_linearize),
+            int(self.error_derivative_idx),
+            int(self.error_per_unit_step),
         )
 
     @property
@@ -246,6 +258,7 @@ def lib():
         L.pdqo_lml_from_y0_tape.restype = _LongDoubleResult if ct is ctypes.c_longdouble else ct
         L.pdqo_lml_grad_fd.restype = ct
         L.pdqo_lml_terminal.restype = ct
+        L.pdqo_step_error_power.restype = ct
         L.pdqo_logpdf.restype = ct
         L.pdqo_control.restype = ct
         L.pdqo_solve_save_at_batch.restype = ctypes.c_int
@@ -439,6 +452,16 @@ def sample(cfg: Config, params, sol: dict, base):
     if rc:
         raise NotImplementedError("sampling is implemented for the dense and isotropic factorisations")
     return out
+
+
+def step_error_power(cfg: Config, params, mean, chol, t, dt):
+    """One step attempt from (mean, chol) at t and its error power (the configured error estimator).
+    Returns (error_power, proposed mean)."""
+    params, mean, chol = _f64(params), _f64(mean), _f64(chol)
+    prop = _zeros(mean.shape)
+    c = cfg.c()
+    ep = lib().pdqo_step_error_power(ctypes.byref(c), _p(params), _p(mean), _p(chol), _s(t), _s(dt), _p(prop))
+    return ep, prop
 
 
 def lml_terminal(cfg: Config, params, mean, chol, u, std, tcoeff_index=0):

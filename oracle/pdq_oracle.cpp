@@ -517,6 +517,8 @@ struct pdqo_cfg {
   Real base_scale;  // prior output scale (scalar, broadcast)
   int correct_asymptotic_underconfidence, max_attempts;
   int re_linearize;  // solver_dynamic(re_linearize_after_calibration=True)
+  // error estimator: 0 error_residual_std, 1 error_state_std; options
+  int error_estimator, error_re_linearize, error_derivative_idx, error_per_unit_step;
 };
 }
 
@@ -1058,18 +1060,61 @@ Sol solver_step(const Problem& P, const Sol& st, Real dt) {
 
 // error_residual_std.estimate_error_norm (solvers.py:925-996) with
 // error_norm_scale_then_rms / _rms_then_scale (:770-811).
+// The std of a state's derivative idx: dense rows idx d .. idx d + d - 1 (per
+// row qr_r, ssm_impl_dense.py:141-150), isotropic row idx and blockdiag row
+// idx of every block (vector norms, ssm_impl_isotropic.py:190-197 and
+// ssm_impl_blockdiag.py:269-274).
+Vec state_std_at(const Problem& P, const BNormal& x, int idx) {
+  if (P.cfg.fact == FACT_DENSE) {
+    const Vec s = P.ops.std_rows(x.b[0]);
+    return Vec(s.begin() + idx * P.d, s.begin() + (idx + 1) * P.d);
+  }
+  Vec out(P.nblk);
+  for (int i = 0; i < P.nblk; ++i) {
+    Real ss = 0;
+    for (int j = 0; j < x.b[i].L.c; ++j) ss += x.b[i].L(idx, j) * x.b[i].L(idx, j);
+    out[i] = std::sqrt(ss);
+  }
+  return out;
+}
+
+// error_residual_std.estimate_error_norm (solvers.py:925-996) and
+// error_state_std.estimate_error_norm (:1041-1098).  Both extrapolate the
+// previous mean with output scale 1 (rv) and take the step's linearisation,
+// or re-linearise at rv (re_linearize_before_error).  The residual estimator
+// measures the calibrated std of the observed residual (order n = 1); the
+// state estimator conditions rv on the residual and measures the calibrated
+// std of derivative idx of the result (order n = idx).  error_per_unit_step
+// raises n by one.
 Real estimate_error_power(const Problem& P, const Sol& prev, const Sol& prop, Real dt) {
   BCond tr = transition(P, dt, prototype_os(P));
   BNormal rv = apply_flat(P, tr, prev.u);
-  BNormal observed = marginalise(P, prop.fx, rv);
-  Vec os = residual_whitened_rms_zero(P, observed);
-  rescale_cholesky(P, observed, os);
-  Vec err = observed_std(P, observed);
-  Vec u0 = u0_of(P, prev.u), u1 = u0_of(P, prop.u);
+  const BCond fx = P.cfg.error_re_linearize ? linearize(P, rv, prop.t) : prop.fx;
+  Vec err, u0, u1;
+  int nord;
+  if (P.cfg.error_estimator == 1) {
+    BNormal observed;
+    const BNormal updated = bayes_rule_zero(P, fx, rv, &observed);
+    const Vec os = residual_whitened_rms_zero(P, observed);
+    const int idx = P.cfg.error_derivative_idx;
+    err = state_std_at(P, updated, idx);
+    for (size_t i = 0; i < err.size(); ++i) err[i] = os[os.size() == 1 ? 0 : i] * err[i];
+    u0 = coeff_of(P, prev.u, idx);
+    u1 = coeff_of(P, prop.u, idx);
+    nord = idx;
+  } else {
+    BNormal observed = marginalise(P, fx, rv);
+    Vec os = residual_whitened_rms_zero(P, observed);
+    rescale_cholesky(P, observed, os);
+    err = observed_std(P, observed);
+    u0 = u0_of(P, prev.u);
+    u1 = u0_of(P, prop.u);
+    nord = 1;  // residual_order - 1 for first-order ODEs
+  }
+  if (P.cfg.error_per_unit_step) nord += 1;
   const int d = P.d;
   Vec ref(d);
   for (int k = 0; k < d; ++k) ref[k] = std::max(std::fabs(u0[k]), std::fabs(u1[k]));
-  const int nord = 1;  // residual_order - 1 for first-order ODEs
   Vec err_abs(err.size());
   for (size_t i = 0; i < err.size(); ++i) err_abs[i] = err[i] * std::pow(dt, (Real)nord) / factorial_int(nord);
   Real norm;
@@ -1923,6 +1968,19 @@ int pdqo_sample(const pdqo_cfg* cfg, const Real* params, int K, const Real* term
     std::memcpy(out + (size_t)k * ms, x.m.a.data(), sizeof(Real) * ms);
   }
   return 0;
+}
+
+// One step attempt from the state (mean, chol) at t with step dt and its
+// error power (test hook for the error estimators): writes the proposed
+// mean (mean_size) and returns error_power.
+Real pdqo_step_error_power(const pdqo_cfg* cfg, const Real* params, const Real* mean, const Real* chol, Real t, Real dt,
+                           Real* prop_mean) {
+  Problem P;
+  P.setup(*cfg, params);
+  Sol prev = solver_init(P, t, read_normal(P, mean, chol));
+  const Sol prop = solver_step(P, prev, dt);
+  write_normal(P, prop.u, prop_mean, nullptr);
+  return estimate_error_power(P, prev, prop, dt);
 }
 
 // pdqo_lml_from_y0 with the calibration tape: mode 1 records the output

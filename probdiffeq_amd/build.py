@@ -25,6 +25,9 @@ UNITS = [
     "pdq_lane_filter.hip",
     "pdq_lane_fixedpoint.hip",
     "pdq_lane_fixedinterval.hip",
+    "pdq_lane_filter_opts.hip",
+    "pdq_lane_fixedpoint_opts.hip",
+    "pdq_lane_fixedinterval_opts.hip",
     "pdq_lane_lml.hip",
     "pdq_lane_offgrid.hip",
     "pdq_post.hip",

@@ -22,6 +22,7 @@ STRATEGY = {"filter": 0, "fixedpoint": 1, "fixedinterval": 2}
 SOLVER = {"solver": 0, "dynamic": 1, "mle": 2}
 ERROR_NORM = {"scale_then_rms": 0, "rms_then_scale": 1}
 CONTROL = {"integral": 0, "proportional_integral": 1}
+ERROR_ESTIMATOR = {"residual_std": 0, "state_std": 1}
 VF = {"lotka_volterra": 0, "van_der_pol": 1, "burgers": 2, "linear": 3, "logistic": 4}
 STATUS = {0: "ok", 1: "max_attempts", 2: "nonfinite", 3: "dt_underflow"}
 
@@ -61,6 +62,10 @@ class _Config(ctypes.Structure):
         ("correct_asymptotic_underconfidence", ctypes.c_int32),
         ("lanes_per_instance", ctypes.c_int32),
         ("re_linearize_after_calibration", ctypes.c_int32),
+        ("error_estimator", ctypes.c_int32),
+        ("error_re_linearize", ctypes.c_int32),
+        ("error_derivative_idx", ctypes.c_int32),
+        ("error_per_unit_step", ctypes.c_int32),
     ]
 
 
@@ -123,6 +128,10 @@ class Config:
     correct_asymptotic_underconfidence: bool = True
     lanes_per_instance: int = 0
     re_linearize_after_calibration: bool = False
+    error_estimator: str = "residual_std"
+    error_re_linearize: bool = False
+    error_derivative_idx: int = 0
+    error_per_unit_step: bool = False
 
     def c(self) -> _Config:
         return _Config(
@@ -153,6 +162,10 @@ class Config:
             int(self.correct_asymptotic_underconfidence),
             self.lanes_per_instance,
             int(self.re_linearize_after_calibration),
+            ERROR_ESTIMATOR[self.error_estimator],
+            int(self.error_re_linearize),
+            self.error_derivative_idx,
+            int(self.error_per_unit_step),
         )
 
     @property

@@ -134,6 +134,11 @@ int check_common(const pdq_config* cfg, int n_inst, int K) {
   if (cfg->strategy < 0 || cfg->strategy > 2 || cfg->solver < 0 || cfg->solver > 2 || cfg->constraint < 0 ||
       cfg->constraint > 1 || cfg->factorisation < 0 || cfg->factorisation > 2)
     return set_err(PDQ_ERR_INVALID, "invalid enum value in configuration");
+  if (cfg->error_estimator != PDQ_ERROR_RESIDUAL_STD && cfg->error_estimator != PDQ_ERROR_STATE_STD)
+    return set_err(PDQ_ERR_INVALID, "unknown error estimator %d", cfg->error_estimator);
+  if (cfg->error_derivative_idx < 0 || cfg->error_derivative_idx > cfg->num_derivatives)
+    return set_err(PDQ_ERR_INVALID, "error derivative_idx %d outside [0, %d]", cfg->error_derivative_idx,
+                   cfg->num_derivatives);
   if (cfg->constraint == PDQ_TS1 && cfg->factorisation != PDQ_DENSE)
     return set_err(PDQ_ERR_UNSUPPORTED,
                    "ts1 is implemented for the dense factorisation only (isotropic/blockdiag ts1 use Hutchinson "
@@ -214,6 +219,9 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
   rc = device_mats(*cfg, false, m);
   if (rc) return rc;
   const int G = pick_lanes(*cfg, L, n_inst, mode);
+  if (G != 1 && mode != pdq::MODE_FIXED_GRID && cfg->error_estimator == PDQ_ERROR_STATE_STD)
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "error_state_std is implemented in the lane-per-instance kernel (isotropic, nu = 3 or 4, d = 2)");
   if (G == 1) {
     if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(L.N, L.d))
       return set_err(PDQ_ERR_UNSUPPORTED,

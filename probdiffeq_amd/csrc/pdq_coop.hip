@@ -498,7 +498,9 @@ __device__ double error_power(const Ctx<G>& c, const double* oL, double sigma, c
     if (c.dense) return ss == 0.0 ? fabs(alpha) : fabs(hypot(alpha, sqrt(ss)));
     return sqrt(alpha * alpha + ss);
   };
-  const double fact = 1.0;  // factorial(n) for n = residual_order - 1 = 1
+  // n = residual_order - 1 = 1, one more for error_per_unit_step
+  const int nord = cfg.error_per_unit_step ? 2 : 1;
+  const double fact = nord == 1 ? 1.0 : 2.0;
   double norm;
   if (cfg.error_norm == PDQ_NORM_SCALE_THEN_RMS) {
     double ss = 0.0;
@@ -506,7 +508,7 @@ __device__ double error_power(const Ctx<G>& c, const double* oL, double sigma, c
       const double u0 = c.dense ? prev_m[k * c.mc] : prev_m[k];
       const double u1 = c.dense ? prop_m[k * c.mc] : prop_m[k];
       const double ref = fmax(fabs(u0), fabs(u1));
-      const double e = err(ko == 1 ? 0 : k) * pow(dt, 1.0) / fact;
+      const double e = err(ko == 1 ? 0 : k) * pow(dt, (double)nord) / fact;
       const double rr = e / (cfg.atol + cfg.rtol * fabs(ref));
       ss += rr * rr;
     }
@@ -514,7 +516,7 @@ __device__ double error_power(const Ctx<G>& c, const double* oL, double sigma, c
   } else {
     double sa = 0.0, sr = 0.0;
     for (int r = 0; r < ko; ++r) {
-      const double e = err(r) * pow(dt, 1.0) / fact;
+      const double e = err(r) * pow(dt, (double)nord) / fact;
       sa += e * e;
     }
     for (int k = 0; k < d; ++k) {
@@ -665,6 +667,9 @@ __device__ double step(const Ctx<G>& c, int from, int to, double dt, bool estima
       apply_flat_mean(c, P, PI, S.m, Um);
       apply_flat_chol(c, P, s1, UL);
       c.g.sync();
+      // re_linearize_before_error (solvers.py:951-956): linearise at the
+      // extrapolation instead of reusing the step's linearisation
+      if (cfg.error_re_linearize) linearize(c, Um, tn, R.b, R.J, y0buf);
       marginalise_fx(c, R.b, R.J, Um, UL, om, oL, big);
       const double se = rms_zero(c, om, oL);
       ep = error_power(c, oL, se, S.m, R.m, dt);

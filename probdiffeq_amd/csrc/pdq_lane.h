@@ -669,35 +669,68 @@ __device__ __forceinline__ void lane_update(const pdq_config& cfg, const double*
   oL_out = big[0];
 }
 
-// error_residual_std.estimate_error_norm (solvers.py:925-996), isotropic.
+// The error power of an isotropic error estimate (estimate_error_norm of
+// error_residual_std, solvers.py:925-996, and error_state_std, :1041-1098):
+// err is the calibrated std (one scalar for the isotropic factorisation),
+// `row` the derivative whose means give the reference, and nord the order n
+// in error_abs = err dt^n / n!.
 template <int N, int D>
-__device__ __forceinline__ double lane_error_power(const pdq_config& cfg, double oL, double sigma,
-                                                   const double* prev_m, const double* prop_m, double dt) {
-  const double alpha = sigma * oL;
-  const double err = sqrt(alpha * alpha + 0.0);
+__device__ __forceinline__ double lane_error_power(const pdq_config& cfg, double err, int row, const double* prev_m,
+                                                   const double* prop_m, double dt, int nord) {
+  const double e = nord == 1 ? err * dt / 1.0 : err * pow(dt, (double)nord) / factorial_d(nord);
+  double ref[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) ref[k] = 0.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r)  // the reference row, selected without indexing registers by a run-time row
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      if (r == row) ref[k] = fmax(fabs(prev_m[r * D + k]), fabs(prop_m[r * D + k]));
   double norm;
   if (cfg.error_norm == PDQ_NORM_SCALE_THEN_RMS) {
     double ss = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      const double ref = fmax(fabs(prev_m[k]), fabs(prop_m[k]));
-      const double e = err * dt / 1.0;
-      const double rr = e / (cfg.atol + cfg.rtol * fabs(ref));
+      const double rr = e / (cfg.atol + cfg.rtol * fabs(ref[k]));
       ss += rr * rr;
     }
     norm = sqrt(ss) / sqrt((double)D);
   } else {
-    const double e = err * dt / 1.0;
     const double sa = 0.0 + e * e;
     double sr = 0.0;
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const double ref = fmax(fabs(prev_m[k]), fabs(prop_m[k]));
-      sr += ref * ref;
-    }
+    for (int k = 0; k < D; ++k) sr += ref[k] * ref[k];
     norm = (sqrt(sa) / sqrt(1.0)) / (cfg.atol + cfg.rtol * (sqrt(sr) / sqrt((double)D)));
   }
   return pow(norm, -1.0 / (double)N);
+}
+
+// error_residual_std's calibrated std of the observed residual: the
+// isotropic std (vector norm) of the rescaled 1 x 1 factor sigma oL.
+__device__ __forceinline__ double lane_residual_err(double sigma, double oL) {
+  const double alpha = sigma * oL;
+  return sqrt(alpha * alpha + 0.0);
+}
+
+// error_state_std's calibrated std (solvers.py:1070-1084): condition the
+// output-scale-1 extrapolation (Um, UL) on the linearisation b
+// (bayes_rule_and_residual_whitened_rms_tree), then scale the std of
+// derivative idx of the result by the residual's output scale.
+template <int N, int D>
+__device__ __forceinline__ double lane_state_err(const pdq_config& cfg, const double* b, const double* Um,
+                                                 const double* UL, int idx) {
+  double Xm[N * D], XL[N * N], mo[D], oL;
+  lane_update<N, D>(cfg, b, Um, UL, Xm, XL, mo, oL);
+  const double os = lane_rms_zero<D>(mo, oL);
+  double std_idx = 0.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    double ss = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) ss += XL[r * N + j] * XL[r * N + j];
+    if (r == idx) std_idx = sqrt(ss);
+  }
+  return os * std_idx;
 }
 
 // controllers.py:66-84 / :24-63
@@ -728,11 +761,14 @@ struct LaneCtx {
 // error estimate, from slot `from` to slot `to`.
 //
 // (1) Calibration (solver_dynamic only): extrapolate the mean with output
-// scale 1, linearise there, sigma = the whitened residual's RMS.
+// scale 1, linearise there, sigma = the whitened residual's RMS.  `err` is
+// the error estimate's calibrated std, taken here where its operands are
+// live: the error's own re-linearisation (re_linearize_before_error) would
+// linearise at this same extrapolation, so it reproduces b exactly.
 template <int N, int D>
 __device__ __forceinline__ void lane_calibrate(const LaneCtx<N, D>& c, const double* Sm, const double* P,
                                                const double* PI, double h, double tn, double* b, double& sigma,
-                                               double& oL) {
+                                               double& err, bool estimate) {
   const pdq_config& cfg = c.a->cfg;
   const double* __restrict__ Q = c.a->Q;
   const double s1 = lane_noise_scale(cfg, h, 1.0);
@@ -743,9 +779,14 @@ __device__ __forceinline__ void lane_calibrate(const LaneCtx<N, D>& c, const dou
 #pragma unroll
     for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * lane_iwp_q<N>(Q, i, j));
   lane_linearize<N, D>(cfg, c.prm, Um, tn, b);
-  double om[D];
+  double om[D], oL;
   lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oL);
   sigma = lane_rms_zero<D>(om, oL);
+  err = 0.0;
+  if (estimate)
+    err = cfg.error_estimator == PDQ_ERROR_STATE_STD
+              ? lane_state_err<N, D>(cfg, b, Um, UL, cfg.error_derivative_idx)
+              : lane_residual_err(sigma, oL);
 }
 
 // (2) The prediction, shared by step attempts and checkpoint interpolation:
@@ -770,26 +811,51 @@ __device__ __forceinline__ void lane_predict(const LaneCtx<N, D>& c, const doubl
 }
 
 // (3) The update, the MLE running scale, the error estimate and the proposed
-// solution's slot.  b, sigma and oL come from (1) for solver_dynamic.
+// solution's slot.  b, sigma and err come from (1) for solver_dynamic.
 // Returns error_power (meaningful with `estimate`).
 template <int N, int D>
 __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int from, int to, double h, double tn,
                                                    const double* P, const double* PI, const double* Sm,
                                                    const double* Pm, const double* PL, double* b, double sigma,
-                                                   double oL, bool estimate) {
+                                                   double err, bool estimate) {
   const pdq_config& cfg = c.a->cfg;
   const double* __restrict__ Q = c.a->Q;
   const LaneWs<N, D>& w = c.w;
   double Rm[N * D], RL[N * N], mo[D], oLu, ep = 0.0;
+  const bool state_std = cfg.error_estimator == PDQ_ERROR_STATE_STD;
   if (cfg.solver == PDQ_SOLVER_DYNAMIC) {
-    // solvers.py:576-582: linearise again at the calibrated prediction (the
-    // error estimate keeps the calibration's linearisation, :597)
-    if (cfg.re_linearize_after_calibration) lane_linearize<N, D>(cfg, c.prm, Pm, tn, b);
-    lane_update<N, D>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
-    if (estimate) ep = lane_error_power<N, D>(cfg, oL, sigma, Sm, Rm, h);
+    // solvers.py:576-582: linearise again at the calibrated prediction; the
+    // step reports its first linearisation (:597), which the error uses
+    double bu[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) bu[k] = b[k];
+    if (cfg.re_linearize_after_calibration) lane_linearize<N, D>(cfg, c.prm, Pm, tn, bu);
+    lane_update<N, D>(cfg, bu, Pm, PL, Rm, RL, mo, oLu);
   } else {
     const double s1 = lane_noise_scale(cfg, h, 1.0);
     lane_linearize<N, D>(cfg, c.prm, Pm, tn, b);
+    if (estimate) {
+      // the error's calibrated std, before the update so that its operands
+      // and the posterior are not live together
+      double Um[N * D], UL[N * N], be[D];
+      lane_apply_flat_mean<N, D>(P, PI, Sm, Um);
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * lane_iwp_q<N>(Q, i, j));
+      // the step's linearisation, or a new one at the extrapolation
+      // (re_linearize_before_error, solvers.py:951-956)
+#pragma unroll
+      for (int k = 0; k < D; ++k) be[k] = b[k];
+      if (cfg.error_re_linearize) lane_linearize<N, D>(cfg, c.prm, Um, tn, be);
+      if (state_std) {
+        err = lane_state_err<N, D>(cfg, be, Um, UL, cfg.error_derivative_idx);
+      } else {
+        double om[D], oLe;
+        lane_marginalise_fx<N, D>(cfg, be, Um, UL, om, oLe);
+        err = lane_residual_err(lane_rms_zero<D>(om, oLe), oLe);
+      }
+    }
     const bool mle = cfg.solver == PDQ_SOLVER_MLE;
     lane_update<N, D>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
     if (mle) {
@@ -801,19 +867,12 @@ __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int f
       w.at(to, LS_MLE_RUN) = hypot(x1, x2);
       w.at(to, LS_MLE_N) = nd + 1;
     }
-    if (estimate) {
-      double Um[N * D], UL[N * N];
-      lane_apply_flat_mean<N, D>(P, PI, Sm, Um);
-#pragma unroll
-      for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * lane_iwp_q<N>(Q, i, j));
-      double om[D], oLe;
-      lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oLe);
-      const double se = lane_rms_zero<D>(om, oLe);
-      ep = lane_error_power<N, D>(cfg, oLe, se, Sm, Rm, h);
-    }
     sigma = mle ? w.at(from, LS_OS) : 1.0;
+  }
+  if (estimate) {
+    const int err_row = state_std ? cfg.error_derivative_idx : 0;
+    const int nord = (state_std ? cfg.error_derivative_idx : 1) + (cfg.error_per_unit_step ? 1 : 0);
+    ep = lane_error_power<N, D>(cfg, err, err_row, Sm, Rm, h, nord);
   }
   w.store_mL(to, Rm, RL);
   w.at(to, LS_T) = tn;
@@ -836,12 +895,12 @@ __device__ __forceinline__ double lane_step(const LaneCtx<N, D>& c, int from, in
   w.load_mL(from, Sm, SLf);
   const double tn = w.at(from, LS_T) + h;
   lane_precon<N>(h, P, PI);
-  double sigma = 1.0, oL = 0.0;
-  if (cfg.solver == PDQ_SOLVER_DYNAMIC) lane_calibrate<N, D>(c, Sm, P, PI, h, tn, b, sigma, oL);
+  double sigma = 1.0, err = 0.0;
+  if (cfg.solver == PDQ_SOLVER_DYNAMIC) lane_calibrate<N, D>(c, Sm, P, PI, h, tn, b, sigma, err, estimate);
   PDQ_PHASE_BOUNDARY();
   lane_predict<N, D>(c, P, PI, lane_noise_scale(cfg, h, sigma), Sm, SLf, Pm, PL, to,
                      cfg.strategy == PDQ_FIXEDPOINT, w.at(from, LS_IDENT) != 0.0, from, to);
-  return lane_finish_step<N, D>(c, from, to, h, tn, P, PI, Sm, Pm, PL, b, sigma, oL, estimate);
+  return lane_finish_step<N, D>(c, from, to, h, tn, P, PI, Sm, Pm, PL, b, sigma, err, estimate);
 }
 
 // ---------------------------------------------------------------- outputs
@@ -1133,7 +1192,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
       const int xs = op == OP_STEP ? sf : ifr;
       double Xm[N * D], XL[N * N], P[N], PI[N], b[D], Pm[N * D], PL[N * N];
       w.load_mL(xs, Xm, XL);
-      double h, tn = 0.0, sigma = 1.0, oL = 0.0;
+      double h, tn = 0.0, sigma = 1.0, err = 0.0;
       if (op == OP_STEP) {
         h = dt;
         if (cfg.clip_dt) h = fmin(h, t1 - tsf);
@@ -1144,7 +1203,8 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
         h = tsf - t1;  // t1 -> step_from
       }
       lane_precon<N>(h, P, PI);
-      if (op == OP_STEP && cfg.solver == PDQ_SOLVER_DYNAMIC) lane_calibrate<N, D>(c, Xm, P, PI, h, tn, b, sigma, oL);
+      if (op == OP_STEP && cfg.solver == PDQ_SOLVER_DYNAMIC)
+        lane_calibrate<N, D>(c, Xm, P, PI, h, tn, b, sigma, err, true);
       const double os = w.at(sf, LS_OS);
       const double scale = op == OP_STEP ? sigma : os;
       // step: revert into proposed, merge step_from's conditional in front;
@@ -1174,7 +1234,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
       lane_predict<N, D>(c, P, PI, lane_noise_scale(cfg, h, scale), Xm, XL, Pm, PL, bws,
                          cfg.strategy == PDQ_FIXEDPOINT, mident, mself, mres);
       if (op == OP_STEP) {
-        const double ep = lane_finish_step<N, D>(c, sf, pr, h, tn, P, PI, Xm, Pm, PL, b, sigma, oL, true);
+        const double ep = lane_finish_step<N, D>(c, sf, pr, h, tn, P, PI, Xm, Pm, PL, b, sigma, err, true);
         dt = lane_control(cfg, h, cstate, ep);
         ++att;
         ++attempts;
@@ -1239,12 +1299,23 @@ constexpr size_t lane_lds_bytes() {
 // the instantiation: the kernel overwrites them in its own copy of the
 // arguments, so every test of them folds and the other strategies' and
 // solvers' code (with the registers it would hold) is not compiled in.
-template <int N, int D, int STRATEGY, int SOLVER>
+// Without OPTS the error estimator's and the calibration's options take
+// their defaults the same way (error_residual_std without re-linearisation
+// or per-unit-step scaling, no re-linearisation after the calibration), and
+// the launcher picks this instantiation only for configurations that hold
+// them; with OPTS they are read at run time.
+template <int N, int D, int STRATEGY, int SOLVER, bool OPTS>
 __global__ void __launch_bounds__(64) lane_kernel(LArgs args) {
   extern __shared__ double lds[];
   LArgs a = args;
   a.cfg.strategy = STRATEGY;
   a.cfg.solver = SOLVER;
+  if (!OPTS) {
+    a.cfg.error_estimator = PDQ_ERROR_RESIDUAL_STD;
+    a.cfg.error_re_linearize = 0;
+    a.cfg.error_per_unit_step = 0;
+    a.cfg.re_linearize_after_calibration = 0;
+  }
   double* const slots = lds + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   for (int inst = blockIdx.x * blockDim.x + threadIdx.x; inst < a.n_inst; inst += stride)

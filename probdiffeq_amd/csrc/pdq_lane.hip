@@ -1,6 +1,6 @@
 // Dispatch of the lane-per-instance isotropic kernel to its instantiations
-// (compiled in pdq_lane_filter.hip, pdq_lane_fixedpoint.hip and
-// pdq_lane_fixedinterval.hip).
+// (compiled in pdq_lane_<strategy>.hip with the option defaults folded in,
+// and in pdq_lane_<strategy>_opts.hip with the options read at run time).
 #include "pdq_lane_launch.h"
 
 namespace pdq {
@@ -9,14 +9,19 @@ bool lane_supported(int N, int D) { return D == 2 && (N == 4 || N == 5); }
 
 namespace {
 
-template <int N, int D, int S>
-int launch_lane_s(LArgs& a, hipStream_t st) {
+template <int N, int D, int S, bool O>
+int launch_lane_o(LArgs& a, hipStream_t st) {
   switch (a.cfg.solver) {
-    case PDQ_SOLVER: return launch_lane_nd<N, D, S, PDQ_SOLVER>(a, st);
-    case PDQ_SOLVER_DYNAMIC: return launch_lane_nd<N, D, S, PDQ_SOLVER_DYNAMIC>(a, st);
-    case PDQ_SOLVER_MLE: return launch_lane_nd<N, D, S, PDQ_SOLVER_MLE>(a, st);
+    case PDQ_SOLVER: return launch_lane_nd<N, D, S, PDQ_SOLVER, O>(a, st);
+    case PDQ_SOLVER_DYNAMIC: return launch_lane_nd<N, D, S, PDQ_SOLVER_DYNAMIC, O>(a, st);
+    case PDQ_SOLVER_MLE: return launch_lane_nd<N, D, S, PDQ_SOLVER_MLE, O>(a, st);
     default: return host_error(PDQ_ERR_INVALID, "unknown solver %d", a.cfg.solver);
   }
+}
+
+template <int N, int D, int S>
+int launch_lane_s(LArgs& a, hipStream_t st) {
+  return lane_default_options(a.cfg) ? launch_lane_o<N, D, S, false>(a, st) : launch_lane_o<N, D, S, true>(a, st);
 }
 
 template <int N, int D>

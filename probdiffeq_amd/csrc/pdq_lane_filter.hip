@@ -1,7 +1,8 @@
-// Lane-kernel instantiations of the filter strategy.
+// Lane-kernel instantiations of the filter strategy, with the error
+// estimator's and the calibration's options at their defaults.
 #define PDQ_LANE_DEFINE_LAUNCH
 #include "pdq_lane_launch.h"
 
 namespace pdq {
-PDQ_LANE_INSTANTIATE(PDQ_FILTER)
+PDQ_LANE_INSTANTIATE(PDQ_FILTER, false)
 }  // namespace pdq

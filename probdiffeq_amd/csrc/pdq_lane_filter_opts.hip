@@ -1,0 +1,8 @@
+// Lane-kernel instantiations of the filter strategy, with the error
+// estimator's and the calibration's options read at run time.
+#define PDQ_LANE_DEFINE_LAUNCH
+#include "pdq_lane_launch.h"
+
+namespace pdq {
+PDQ_LANE_INSTANTIATE(PDQ_FILTER, true)
+}  // namespace pdq

@@ -1,7 +1,8 @@
-// Lane-kernel instantiations of the fixedinterval strategy.
+// Lane-kernel instantiations of the fixedinterval strategy, with the error
+// estimator's and the calibration's options at their defaults.
 #define PDQ_LANE_DEFINE_LAUNCH
 #include "pdq_lane_launch.h"
 
 namespace pdq {
-PDQ_LANE_INSTANTIATE(PDQ_FIXEDINTERVAL)
+PDQ_LANE_INSTANTIATE(PDQ_FIXEDINTERVAL, false)
 }  // namespace pdq

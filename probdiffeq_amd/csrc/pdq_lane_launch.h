@@ -12,20 +12,27 @@ namespace pdq {
 
 constexpr int kLaneMaxDevices = 64;
 
-template <int N, int D, int S, int V>
+template <int N, int D, int S, int V, bool O>
 int launch_lane_nd(LArgs& a, hipStream_t st);
+
+// Whether a configuration holds the option defaults that the instantiations
+// without OPTS fold in.
+inline bool lane_default_options(const pdq_config& c) {
+  return c.error_estimator == PDQ_ERROR_RESIDUAL_STD && !c.error_re_linearize && !c.error_per_unit_step &&
+         !c.re_linearize_after_calibration;
+}
 
 #ifdef PDQ_LANE_DEFINE_LAUNCH
 // The LDS attribute and the occupancy are set up once per device and
 // instantiation, so that later launches (including one issued under stream
 // capture) make no other runtime call than the launch itself.
-template <int N, int D, int S, int V>
+template <int N, int D, int S, int V, bool O>
 int lane_blocks_per_cu(int& per_cu) {
   static std::atomic<int> cache[kLaneMaxDevices];
   int dev = 0;
   PDQ_HIP_TRY(hipGetDevice(&dev));
   if (dev >= 0 && dev < kLaneMaxDevices && (per_cu = cache[dev].load()) > 0) return PDQ_OK;
-  auto kern = lane_kernel<N, D, S, V>;
+  auto kern = lane_kernel<N, D, S, V, O>;
   const size_t lds = lane_lds_bytes<N, D>();
   PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   per_cu = 0;
@@ -35,12 +42,12 @@ int lane_blocks_per_cu(int& per_cu) {
   return PDQ_OK;
 }
 
-template <int N, int D, int S, int V>
+template <int N, int D, int S, int V, bool O>
 int launch_lane_nd(LArgs& a, hipStream_t st) {
-  auto kern = lane_kernel<N, D, S, V>;
+  auto kern = lane_kernel<N, D, S, V, O>;
   const size_t lds = lane_lds_bytes<N, D>();
   int per_cu = 0;
-  const int rc = lane_blocks_per_cu<N, D, S, V>(per_cu);
+  const int rc = lane_blocks_per_cu<N, D, S, V, O>(per_cu);
   if (rc) return rc;
   const long need = ((long)a.n_inst + 63) / 64;
   const long cap = (long)per_cu * num_cus();
@@ -51,13 +58,13 @@ int launch_lane_nd(LArgs& a, hipStream_t st) {
 }
 
 // The shapes the device library instantiates, for one strategy.
-#define PDQ_LANE_INSTANTIATE(S)                                              \
-  template int launch_lane_nd<4, 2, S, PDQ_SOLVER>(LArgs&, hipStream_t);         \
-  template int launch_lane_nd<4, 2, S, PDQ_SOLVER_DYNAMIC>(LArgs&, hipStream_t); \
-  template int launch_lane_nd<4, 2, S, PDQ_SOLVER_MLE>(LArgs&, hipStream_t);     \
-  template int launch_lane_nd<5, 2, S, PDQ_SOLVER>(LArgs&, hipStream_t);         \
-  template int launch_lane_nd<5, 2, S, PDQ_SOLVER_DYNAMIC>(LArgs&, hipStream_t); \
-  template int launch_lane_nd<5, 2, S, PDQ_SOLVER_MLE>(LArgs&, hipStream_t);
+#define PDQ_LANE_INSTANTIATE(S, O)                                                   \
+  template int launch_lane_nd<4, 2, S, PDQ_SOLVER, O>(LArgs&, hipStream_t);         \
+  template int launch_lane_nd<4, 2, S, PDQ_SOLVER_DYNAMIC, O>(LArgs&, hipStream_t); \
+  template int launch_lane_nd<4, 2, S, PDQ_SOLVER_MLE, O>(LArgs&, hipStream_t);     \
+  template int launch_lane_nd<5, 2, S, PDQ_SOLVER, O>(LArgs&, hipStream_t);         \
+  template int launch_lane_nd<5, 2, S, PDQ_SOLVER_DYNAMIC, O>(LArgs&, hipStream_t); \
+  template int launch_lane_nd<5, 2, S, PDQ_SOLVER_MLE, O>(LArgs&, hipStream_t);
 #endif
 
 }  // namespace pdq

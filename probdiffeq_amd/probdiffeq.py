@@ -37,6 +37,7 @@ __all__ = [
     "error_norm_rms_then_scale",
     "error_norm_scale_then_rms",
     "error_residual_std",
+    "error_state_std",
     "jacobian_materialize",
     "jetexpand_ode_padded_scan",
     "linear",
@@ -545,11 +546,29 @@ def error_norm_rms_then_scale(norm_order=None):
 class error_residual_std:
     """solvers.py:850-996."""
 
+    kind = "residual_std"
+    derivative_idx = 0
+
     def __init__(self, *, constraint, error_norm=None, re_linearize_before_error=False, error_per_unit_step=False) -> None:
-        if re_linearize_before_error or error_per_unit_step:
-            raise NotImplementedError("re_linearize_before_error / error_per_unit_step are not implemented")
         self.constraint = constraint
         self.error_norm = error_norm or "scale_then_rms"
+        self.re_linearize_before_error = bool(re_linearize_before_error)
+        self.error_per_unit_step = bool(error_per_unit_step)
+
+
+class error_state_std:
+    """solvers.py:999-1098 (marked highly experimental in the reference): the calibrated std of
+    derivative ``derivative_idx`` of the extrapolation conditioned on the step's linearisation.
+    On the device it runs in the lane-per-instance kernel (isotropic, nu = 3 or 4, d = 2)."""
+
+    kind = "state_std"
+
+    def __init__(self, *, constraint, error_norm=None, re_linearize_before_error=False, derivative_idx=0, error_per_unit_step=False) -> None:
+        self.constraint = constraint
+        self.error_norm = error_norm or "scale_then_rms"
+        self.re_linearize_before_error = bool(re_linearize_before_error)
+        self.derivative_idx = int(derivative_idx)
+        self.error_per_unit_step = bool(error_per_unit_step)
 
 
 # ------------------------------------------------------------- device call
@@ -577,6 +596,10 @@ def make_config(prior: Prior, solver_obj: _Solver, error=None, control=None, cli
         cfg.re_linearize_after_calibration = solver_obj.re_linearize_after_calibration
     if error is not None:
         cfg.error_norm = error.error_norm
+        cfg.error_estimator = error.kind
+        cfg.error_re_linearize = error.re_linearize_before_error
+        cfg.error_derivative_idx = error.derivative_idx
+        cfg.error_per_unit_step = error.error_per_unit_step
     if control is not None:
         cfg.control = control.kind
         cfg.safety, cfg.factor_min, cfg.factor_max = control.safety, control.factor_min, control.factor_max

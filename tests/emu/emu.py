@@ -16,8 +16,12 @@ LIB = os.path.join(BUILD, "libpdqemu.so")
 def build(extra=()) -> str:
     os.makedirs(BUILD, exist_ok=True)
     src = os.path.join(HERE, "pdq_emu.cpp")
-    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-DPDQ_HOST_EMU", *extra, "-o", LIB, src]
+    # compiled to a private name and renamed into place, so that concurrent
+    # test processes never load a half-written library
+    tmp = f"{LIB}.{os.getpid()}.tmp"
+    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-DPDQ_HOST_EMU", *extra, "-o", tmp, src]
     subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
     return LIB
 
 

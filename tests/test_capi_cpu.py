@@ -27,10 +27,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_matches_header_layout():
-    """The ctypes mirror of pdq_config has the C layout (12 ints, 12 doubles, 3 ints, padded to 8)."""
+    """The ctypes mirror of pdq_config has the C layout (12 ints, 12 doubles, 7 ints, padded to 8)."""
     from probdiffeq_amd import capi
 
-    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 3 * 4 + 4
+    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 7 * 4 + 4
     assert ctypes.sizeof(capi._Solution) == 17 * 8
 
 

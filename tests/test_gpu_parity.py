@@ -105,6 +105,10 @@ def _oracle_cfg(oracle, cfg):
         base_scale=cfg.base_output_scale,
         max_attempts=cfg.max_attempts,
         re_linearize_after_calibration=cfg.re_linearize_after_calibration,
+        error_estimator=cfg.error_estimator,
+        error_re_linearize=cfg.error_re_linearize,
+        error_derivative_idx=cfg.error_derivative_idx,
+        error_per_unit_step=cfg.error_per_unit_step,
     )
 
 
@@ -321,3 +325,29 @@ def test_re_linearize_after_calibration(oracle, pdq, fact, strategy, lanes):
     cfg = pdq.Config(factorisation=fact, num_derivatives=4, strategy=strategy, solver="dynamic", lanes_per_instance=lanes, re_linearize_after_calibration=True)
     params, y0s, times = _lv_batch(4, seed=31, t_lo=5.0, t_hi=10.0, K=15)
     _compare(oracle, pdq, cfg, params, y0s, times)
+
+
+@pytest.mark.parametrize(
+    "fact,lanes,solver,kw",
+    [
+        ("isotropic", 0, "dynamic", dict(error_estimator="state_std")),
+        ("isotropic", 0, "solver", dict(error_estimator="state_std", error_derivative_idx=2, error_re_linearize=True)),
+        ("isotropic", 16, "mle", dict(error_re_linearize=True, error_per_unit_step=True)),
+        ("dense", 0, "solver", dict(error_re_linearize=True)),
+        ("dense", 0, "dynamic", dict(error_per_unit_step=True)),
+    ],
+)
+def test_error_estimator_variants(oracle, pdq, fact, lanes, solver, kw):
+    """error_state_std (solvers.py:999-1098; lane kernel) and the estimators' options
+    re_linearize_before_error / error_per_unit_step (both kernels)."""
+    cfg = pdq.Config(factorisation=fact, num_derivatives=4, strategy="fixedpoint", solver=solver, rtol=1e-5, atol=1e-7, lanes_per_instance=lanes, **kw)
+    params, y0s, times = _lv_batch(4, seed=41, t_lo=5.0, t_hi=10.0, K=12)
+    _compare(oracle, pdq, cfg, params, y0s, times)
+
+
+def test_error_state_std_is_refused_by_the_cooperative_kernel(pdq):
+    cfg = pdq.Config(factorisation="dense", num_derivatives=4, strategy="filter", error_estimator="state_std")
+    params, y0s, times = _lv_batch(1, seed=1, K=5)
+    tc = pdq.taylor_coefficients("lotka_volterra", _dev(params), _dev(y0s), 0.0, 4)
+    with pytest.raises(pdq.PdqError, match="error_state_std"):
+        pdq.solve(cfg, _dev(params), tc, _dev(times))

@@ -309,3 +309,22 @@ def test_posterior_sampler_kernel_code(oracle, emu, nu):
 
     cfg, sol, base, refs = SC.make_case(oracle, nu)
     SC.check(emu.sample(cfg, sol, base), refs, f"emulation sample nu={nu}")
+
+
+@pytest.mark.parametrize(
+    "solver,kw",
+    [
+        ("dynamic", dict(error_estimator="state_std")),
+        ("dynamic", dict(error_estimator="state_std", error_derivative_idx=2, error_per_unit_step=True)),
+        ("solver", dict(error_estimator="state_std", error_re_linearize=True)),
+        ("mle", dict(error_estimator="residual_std", error_re_linearize=True, error_per_unit_step=True)),
+        ("dynamic", dict(error_estimator="residual_std", error_per_unit_step=True)),
+    ],
+)
+def test_lane_kernel_error_estimators(oracle, emu, solver, kw):
+    """error_state_std (solvers.py:999-1098) and the estimators' options in the lane kernel's code."""
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=4, strategy="fixedpoint", solver=solver, rtol=1e-5, atol=1e-7, **kw)
+    p, y, t = T._lv_batch(3, seed=5, t_lo=4.0, t_hi=8.0, K=10)
+    _run(oracle, emu, cfg, p, y, t, kernel="lane")
