@@ -872,7 +872,26 @@ BCond linearize(const Problem& P, const BNormal& rv, Real t) {
     C.A = Mat(1, N);
     C.A(0, 1) = 1.0;
     C.noise.m = Mat(1, d);
-    for (int k = 0; k < d; ++k) C.noise.m(0, k) = -f[k];
+    if (P.cfg.constraint == CONSTRAINT_TS0) {
+      for (int k = 0; k < d; ++k) C.noise.m(0, k) = -f[k];
+    } else {
+      // IsotropicResidual.linearize (ssm_impl_isotropic.py:321-355) with the
+      // exact trace of jacobian_materialize (jacobians.py:100-106): for
+      // r = m_1 - f(m_0) the trace over d of dr/dm is [-tr J_f, d, 0...],
+      // divided by d; fx = r(m) - linop m.
+      Vec Jf(d * d);
+      P.vf.jac(y0.data(), t, Jf.data());
+      Real tr = 0.0;
+      for (int k = 0; k < d; ++k) tr += -Jf[k * d + k];
+      C.A(0, 0) = tr / (Real)d;
+      C.A(0, 1) = (Real)d / (Real)d;
+      const Mat& m = rv.b[0].m;
+      for (int k = 0; k < d; ++k) {
+        Real lin = 0.0;
+        for (int n = 0; n < N; ++n) lin += C.A(0, n) * m(n, k);
+        C.noise.m(0, k) = (m(1, k) - f[k]) - lin;
+      }
+    }
     C.noise.L = Mat(1, 1);
     C.noise.L(0, 0) = damp;
     C.tl = ones(N);

@@ -139,10 +139,10 @@ int check_common(const pdq_config* cfg, int n_inst, int K) {
   if (cfg->error_derivative_idx < 0 || cfg->error_derivative_idx > cfg->num_derivatives)
     return set_err(PDQ_ERR_INVALID, "error derivative_idx %d outside [0, %d]", cfg->error_derivative_idx,
                    cfg->num_derivatives);
-  if (cfg->constraint == PDQ_TS1 && cfg->factorisation != PDQ_DENSE)
+  if (cfg->constraint == PDQ_TS1 && cfg->factorisation == PDQ_BLOCKDIAG)
     return set_err(PDQ_ERR_UNSUPPORTED,
-                   "ts1 is implemented for the dense factorisation only (isotropic/blockdiag ts1 use Hutchinson "
-                   "estimators, jacobians.py:116-271)");
+                   "ts1 is implemented for the dense and isotropic factorisations (blockdiag ts1 takes the "
+                   "Jacobian's diagonal, jacobians.py:108-113, 171-193)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
   return PDQ_OK;
@@ -222,6 +222,9 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
   if (G != 1 && mode != pdq::MODE_FIXED_GRID && cfg->error_estimator == PDQ_ERROR_STATE_STD)
     return set_err(PDQ_ERR_UNSUPPORTED,
                    "error_state_std is implemented in the lane-per-instance kernel (isotropic, nu = 3 or 4, d = 2)");
+  if (G != 1 && cfg->factorisation == PDQ_ISOTROPIC && cfg->constraint == PDQ_TS1)
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "isotropic ts1 is implemented in the lane-per-instance kernel (nu = 3 or 4, d = 2)");
   if (G == 1) {
     if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(L.N, L.d))
       return set_err(PDQ_ERR_UNSUPPORTED,
@@ -327,6 +330,8 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
     return set_err(PDQ_ERR_UNSUPPORTED, "the LML needs strategy_smoother_fixedpoint (a Markov sequence)");
   if (cfg->constraint != PDQ_TS0 && grad)
     return set_err(PDQ_ERR_UNSUPPORTED, "gradients need ts0 (the covariance recursion must not depend on them)");
+  if (cfg->constraint != PDQ_TS0 && cfg->factorisation == PDQ_ISOTROPIC)
+    return set_err(PDQ_ERR_UNSUPPORTED, "the isotropic LML is implemented for ts0");
   if (cfg->solver == PDQ_SOLVER_MLE) return set_err(PDQ_ERR_UNSUPPORTED, "solver_mle is not supported by the LML path");
   if (every < 1 || (K_grid - 1) % every != 0)
     return set_err(PDQ_ERR_INVALID, "every (%d) must divide K_grid - 1 (%d)", every, K_grid - 1);

@@ -596,27 +596,55 @@ __device__ __forceinline__ void lane_marginalise_cond(const Cond& c, const doubl
     for (int j = 0; j < N; ++j) YL[i * N + j] = fabs(c.to(i)) * big[j * N + i];
 }
 
-// ts0 linearisation (ssm_impl_isotropic.py:298-317): b = -f(u(t)).
+// The linearisation at the mean Xm: the observed residual is
+// h0 x_0 + x_1 + b (the 1 x N linear operator [h0, 1, 0...]).
+// * ts0 (ssm_impl_isotropic.py:298-317): b = -f(u(t)) and the operator
+//   selects derivative 1 (h0 is not applied).
+// * ts1 (IsotropicResidual.linearize, ssm_impl_isotropic.py:321-355) with
+//   jacobian_materialize's exact trace (jacobians.py:100-106): the trace over
+//   d of d r / d x for r = x_1 - f(x_0) is [-tr J_f, d, 0...], divided by d,
+//   and b = r(Xm) - [h0, 1] Xm.
 template <int N, int D>
 __device__ __forceinline__ void lane_linearize(const pdq_config& cfg, const double* __restrict__ prm, const double* Xm,
-                                               double t, double* b) {
+                                               double t, double* b, double& h0) {
   double y[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) y[j] = Xm[j];
+  if (cfg.constraint != PDQ_TS1) {
+    h0 = 0.0;
 #pragma unroll
-  for (int j = 0; j < D; ++j) b[j] = -vf_component(cfg.vector_field, D, y, prm, t, j);
+    for (int j = 0; j < D; ++j) b[j] = -vf_component(cfg.vector_field, D, y, prm, t, j);
+    return;
+  }
+  double tr = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) tr += -vf_jac(cfg.vector_field, D, y, prm, j, j);
+  h0 = tr / (double)D;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const double r = Xm[1 * D + j] - vf_component(cfg.vector_field, D, y, prm, t, j);
+    b[j] = r - (h0 * Xm[j] + Xm[1 * D + j]);
+  }
 }
 
-// fx.marginalise(u) for ts0: observed mean om (D) and factor oL (1 x 1).
+// The linear operator applied to row-major N x C rows: h0 X[0, c] + X[1, c]
+// (ts1), or X[1, c] (ts0).
+template <int C>
+__device__ __forceinline__ double lane_obs_row(const pdq_config& cfg, double h0, const double* X, int c) {
+  return cfg.constraint == PDQ_TS1 ? h0 * X[c] + X[1 * C + c] : X[1 * C + c];
+}
+
+// fx.marginalise(u): observed mean om (D) and factor oL (1 x 1).  The
+// column (H L)^T has the sparsity of row 1 of L for both linearisations.
 template <int N, int D>
-__device__ __forceinline__ void lane_marginalise_fx(const pdq_config& cfg, const double* b, const double* Xm,
-                                                    const double* XL, double* om, double& oL) {
+__device__ __forceinline__ void lane_marginalise_fx(const pdq_config& cfg, const double* b, double h0,
+                                                    const double* Xm, const double* XL, double* om, double& oL) {
   double col[N + 1];
 #pragma unroll
-  for (int r = 0; r < N; ++r) col[r] = XL[1 * N + r];
+  for (int r = 0; r < N; ++r) col[r] = lane_obs_row<N>(cfg, h0, XL, r);
   col[N] = cfg.damp;
 #pragma unroll
-  for (int c = 0; c < D; ++c) om[c] = Xm[1 * D + c] + b[c];
+  for (int c = 0; c < D; ++c) om[c] = lane_obs_row<D>(cfg, h0, Xm, c) + b[c];
   lane_qr<N + 1, 1, FxColumnRows<N>>(col);
   oL = col[0];
 }
@@ -635,8 +663,8 @@ __device__ __forceinline__ double lane_rms_zero(const double* om, double oL) {
 
 // fx.revert(pred) + apply_flat(0) (ssm_impl_api.py:112-132): the update.
 template <int N, int D>
-__device__ __forceinline__ void lane_update(const pdq_config& cfg, const double* b, const double* Pm, const double* PL,
-                                            double* Xm, double* XL, double* mo, double& oL_out) {
+__device__ __forceinline__ void lane_update(const pdq_config& cfg, const double* b, double h0, const double* Pm,
+                                            const double* PL, double* Xm, double* XL, double* mo, double& oL_out) {
   constexpr int n2 = N + 1;
   double big[n2 * n2];
 #pragma unroll
@@ -645,11 +673,11 @@ __device__ __forceinline__ void lane_update(const pdq_config& cfg, const double*
     for (int col = 0; col < n2; ++col) {
       double v;
       if (r < 1) v = col < 1 ? cfg.damp : 0.0;
-      else v = col < 1 ? PL[1 * N + (r - 1)] : PL[(col - 1) * N + (r - 1)];
+      else v = col < 1 ? lane_obs_row<N>(cfg, h0, PL, r - 1) : PL[(col - 1) * N + (r - 1)];
       big[r * n2 + col] = v;
     }
 #pragma unroll
-  for (int c = 0; c < D; ++c) mo[c] = Pm[1 * D + c] + b[c];
+  for (int c = 0; c < D; ++c) mo[c] = lane_obs_row<D>(cfg, h0, Pm, c) + b[c];
   lane_qr<n2, n2, UpdateRows>(big);
   double gain[N];
   const double r0 = 1.0 / big[0];
@@ -717,10 +745,10 @@ __device__ __forceinline__ double lane_residual_err(double sigma, double oL) {
 // (bayes_rule_and_residual_whitened_rms_tree), then scale the std of
 // derivative idx of the result by the residual's output scale.
 template <int N, int D>
-__device__ __forceinline__ double lane_state_err(const pdq_config& cfg, const double* b, const double* Um,
+__device__ __forceinline__ double lane_state_err(const pdq_config& cfg, const double* b, double h0, const double* Um,
                                                  const double* UL, int idx) {
   double Xm[N * D], XL[N * N], mo[D], oL;
-  lane_update<N, D>(cfg, b, Um, UL, Xm, XL, mo, oL);
+  lane_update<N, D>(cfg, b, h0, Um, UL, Xm, XL, mo, oL);
   const double os = lane_rms_zero<D>(mo, oL);
   double std_idx = 0.0;
 #pragma unroll
@@ -767,8 +795,8 @@ struct LaneCtx {
 // linearise at this same extrapolation, so it reproduces b exactly.
 template <int N, int D>
 __device__ __forceinline__ void lane_calibrate(const LaneCtx<N, D>& c, const double* Sm, const double* P,
-                                               const double* PI, double h, double tn, double* b, double& sigma,
-                                               double& err, bool estimate) {
+                                               const double* PI, double h, double tn, double* b, double& h0,
+                                               double& sigma, double& err, bool estimate) {
   const pdq_config& cfg = c.a->cfg;
   const double* __restrict__ Q = c.a->Q;
   const double s1 = lane_noise_scale(cfg, h, 1.0);
@@ -778,14 +806,14 @@ __device__ __forceinline__ void lane_calibrate(const LaneCtx<N, D>& c, const dou
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * lane_iwp_q<N>(Q, i, j));
-  lane_linearize<N, D>(cfg, c.prm, Um, tn, b);
+  lane_linearize<N, D>(cfg, c.prm, Um, tn, b, h0);
   double om[D], oL;
-  lane_marginalise_fx<N, D>(cfg, b, Um, UL, om, oL);
+  lane_marginalise_fx<N, D>(cfg, b, h0, Um, UL, om, oL);
   sigma = lane_rms_zero<D>(om, oL);
   err = 0.0;
   if (estimate)
     err = cfg.error_estimator == PDQ_ERROR_STATE_STD
-              ? lane_state_err<N, D>(cfg, b, Um, UL, cfg.error_derivative_idx)
+              ? lane_state_err<N, D>(cfg, b, h0, Um, UL, cfg.error_derivative_idx)
               : lane_residual_err(sigma, oL);
 }
 
@@ -811,13 +839,13 @@ __device__ __forceinline__ void lane_predict(const LaneCtx<N, D>& c, const doubl
 }
 
 // (3) The update, the MLE running scale, the error estimate and the proposed
-// solution's slot.  b, sigma and err come from (1) for solver_dynamic.
+// solution's slot.  b, h0, sigma and err come from (1) for solver_dynamic.
 // Returns error_power (meaningful with `estimate`).
 template <int N, int D>
 __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int from, int to, double h, double tn,
                                                    const double* P, const double* PI, const double* Sm,
-                                                   const double* Pm, const double* PL, double* b, double sigma,
-                                                   double err, bool estimate) {
+                                                   const double* Pm, const double* PL, double* b, double h0,
+                                                   double sigma, double err, bool estimate) {
   const pdq_config& cfg = c.a->cfg;
   const double* __restrict__ Q = c.a->Q;
   const LaneWs<N, D>& w = c.w;
@@ -826,18 +854,18 @@ __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int f
   if (cfg.solver == PDQ_SOLVER_DYNAMIC) {
     // solvers.py:576-582: linearise again at the calibrated prediction; the
     // step reports its first linearisation (:597), which the error uses
-    double bu[D];
+    double bu[D], hu = h0;
 #pragma unroll
     for (int k = 0; k < D; ++k) bu[k] = b[k];
-    if (cfg.re_linearize_after_calibration) lane_linearize<N, D>(cfg, c.prm, Pm, tn, bu);
-    lane_update<N, D>(cfg, bu, Pm, PL, Rm, RL, mo, oLu);
+    if (cfg.re_linearize_after_calibration) lane_linearize<N, D>(cfg, c.prm, Pm, tn, bu, hu);
+    lane_update<N, D>(cfg, bu, hu, Pm, PL, Rm, RL, mo, oLu);
   } else {
     const double s1 = lane_noise_scale(cfg, h, 1.0);
-    lane_linearize<N, D>(cfg, c.prm, Pm, tn, b);
+    lane_linearize<N, D>(cfg, c.prm, Pm, tn, b, h0);
     if (estimate) {
       // the error's calibrated std, before the update so that its operands
       // and the posterior are not live together
-      double Um[N * D], UL[N * N], be[D];
+      double Um[N * D], UL[N * N], be[D], he = h0;
       lane_apply_flat_mean<N, D>(P, PI, Sm, Um);
 #pragma unroll
       for (int i = 0; i < N; ++i)
@@ -847,17 +875,17 @@ __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int f
       // (re_linearize_before_error, solvers.py:951-956)
 #pragma unroll
       for (int k = 0; k < D; ++k) be[k] = b[k];
-      if (cfg.error_re_linearize) lane_linearize<N, D>(cfg, c.prm, Um, tn, be);
+      if (cfg.error_re_linearize) lane_linearize<N, D>(cfg, c.prm, Um, tn, be, he);
       if (state_std) {
-        err = lane_state_err<N, D>(cfg, be, Um, UL, cfg.error_derivative_idx);
+        err = lane_state_err<N, D>(cfg, be, he, Um, UL, cfg.error_derivative_idx);
       } else {
         double om[D], oLe;
-        lane_marginalise_fx<N, D>(cfg, be, Um, UL, om, oLe);
+        lane_marginalise_fx<N, D>(cfg, be, he, Um, UL, om, oLe);
         err = lane_residual_err(lane_rms_zero<D>(om, oLe), oLe);
       }
     }
     const bool mle = cfg.solver == PDQ_SOLVER_MLE;
-    lane_update<N, D>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
+    lane_update<N, D>(cfg, b, h0, Pm, PL, Rm, RL, mo, oLu);
     if (mle) {
       // bayes_rule_and_residual_whitened_rms_tree: observed = (mo, R_Y^T)
       const double nt = lane_rms_zero<D>(mo, oLu);
@@ -895,12 +923,12 @@ __device__ __forceinline__ double lane_step(const LaneCtx<N, D>& c, int from, in
   w.load_mL(from, Sm, SLf);
   const double tn = w.at(from, LS_T) + h;
   lane_precon<N>(h, P, PI);
-  double sigma = 1.0, err = 0.0;
-  if (cfg.solver == PDQ_SOLVER_DYNAMIC) lane_calibrate<N, D>(c, Sm, P, PI, h, tn, b, sigma, err, estimate);
+  double sigma = 1.0, err = 0.0, h0 = 0.0;
+  if (cfg.solver == PDQ_SOLVER_DYNAMIC) lane_calibrate<N, D>(c, Sm, P, PI, h, tn, b, h0, sigma, err, estimate);
   PDQ_PHASE_BOUNDARY();
   lane_predict<N, D>(c, P, PI, lane_noise_scale(cfg, h, sigma), Sm, SLf, Pm, PL, to,
                      cfg.strategy == PDQ_FIXEDPOINT, w.at(from, LS_IDENT) != 0.0, from, to);
-  return lane_finish_step<N, D>(c, from, to, h, tn, P, PI, Sm, Pm, PL, b, sigma, err, estimate);
+  return lane_finish_step<N, D>(c, from, to, h, tn, P, PI, Sm, Pm, PL, b, h0, sigma, err, estimate);
 }
 
 // ---------------------------------------------------------------- outputs
@@ -1192,7 +1220,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
       const int xs = op == OP_STEP ? sf : ifr;
       double Xm[N * D], XL[N * N], P[N], PI[N], b[D], Pm[N * D], PL[N * N];
       w.load_mL(xs, Xm, XL);
-      double h, tn = 0.0, sigma = 1.0, err = 0.0;
+      double h, tn = 0.0, sigma = 1.0, err = 0.0, h0 = 0.0;
       if (op == OP_STEP) {
         h = dt;
         if (cfg.clip_dt) h = fmin(h, t1 - tsf);
@@ -1204,7 +1232,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
       }
       lane_precon<N>(h, P, PI);
       if (op == OP_STEP && cfg.solver == PDQ_SOLVER_DYNAMIC)
-        lane_calibrate<N, D>(c, Xm, P, PI, h, tn, b, sigma, err, true);
+        lane_calibrate<N, D>(c, Xm, P, PI, h, tn, b, h0, sigma, err, true);
       const double os = w.at(sf, LS_OS);
       const double scale = op == OP_STEP ? sigma : os;
       // step: revert into proposed, merge step_from's conditional in front;
@@ -1234,7 +1262,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
       lane_predict<N, D>(c, P, PI, lane_noise_scale(cfg, h, scale), Xm, XL, Pm, PL, bws,
                          cfg.strategy == PDQ_FIXEDPOINT, mident, mself, mres);
       if (op == OP_STEP) {
-        const double ep = lane_finish_step<N, D>(c, sf, pr, h, tn, P, PI, Xm, Pm, PL, b, sigma, err, true);
+        const double ep = lane_finish_step<N, D>(c, sf, pr, h, tn, P, PI, Xm, Pm, PL, b, h0, sigma, err, true);
         dt = lane_control(cfg, h, cstate, ep);
         ++att;
         ++attempts;
@@ -1299,11 +1327,12 @@ constexpr size_t lane_lds_bytes() {
 // the instantiation: the kernel overwrites them in its own copy of the
 // arguments, so every test of them folds and the other strategies' and
 // solvers' code (with the registers it would hold) is not compiled in.
-// Without OPTS the error estimator's and the calibration's options take
-// their defaults the same way (error_residual_std without re-linearisation
-// or per-unit-step scaling, no re-linearisation after the calibration), and
-// the launcher picks this instantiation only for configurations that hold
-// them; with OPTS they are read at run time.
+// Without OPTS the linearisation, the error estimator's and the
+// calibration's options take their defaults the same way (ts0,
+// error_residual_std without re-linearisation or per-unit-step scaling, no
+// re-linearisation after the calibration), and the launcher picks this
+// instantiation only for configurations that hold them; with OPTS they are
+// read at run time.
 template <int N, int D, int STRATEGY, int SOLVER, bool OPTS>
 __global__ void __launch_bounds__(64) lane_kernel(LArgs args) {
   extern __shared__ double lds[];
@@ -1311,6 +1340,7 @@ __global__ void __launch_bounds__(64) lane_kernel(LArgs args) {
   a.cfg.strategy = STRATEGY;
   a.cfg.solver = SOLVER;
   if (!OPTS) {
+    a.cfg.constraint = PDQ_TS0;
     a.cfg.error_estimator = PDQ_ERROR_RESIDUAL_STD;
     a.cfg.error_re_linearize = 0;
     a.cfg.error_per_unit_step = 0;

@@ -15,11 +15,11 @@ constexpr int kLaneMaxDevices = 64;
 template <int N, int D, int S, int V, bool O>
 int launch_lane_nd(LArgs& a, hipStream_t st);
 
-// Whether a configuration holds the option defaults that the instantiations
+// Whether a configuration holds the defaults that the instantiations
 // without OPTS fold in.
 inline bool lane_default_options(const pdq_config& c) {
-  return c.error_estimator == PDQ_ERROR_RESIDUAL_STD && !c.error_re_linearize && !c.error_per_unit_step &&
-         !c.re_linearize_after_calibration;
+  return c.constraint == PDQ_TS0 && c.error_estimator == PDQ_ERROR_RESIDUAL_STD && !c.error_re_linearize &&
+         !c.error_per_unit_step && !c.re_linearize_after_calibration;
 }
 
 #ifdef PDQ_LANE_DEFINE_LAUNCH

@@ -116,7 +116,7 @@ __device__ __forceinline__ void lane_lml_step(const LaneCtx<N, D * (1 + TPL)>& c
   if (cfg.solver != PDQ_SOLVER_DYNAMIC || cfg.re_linearize_after_calibration)
     lane_lml_linearize<N, D, TPL>(cfg, c.prm, q0, Pm, tn, b);
   double Rm[N * MC], RL[N * N], mo[MC], oLu;
-  lane_update<N, MC>(cfg, b, Pm, PL, Rm, RL, mo, oLu);
+  lane_update<N, MC>(cfg, b, 0.0, Pm, PL, Rm, RL, mo, oLu);
   w.store_mL(to, Rm, RL);
   w.at(to, LS_T) = tn;
   w.at(to, LS_OS) = sigma;
@@ -287,13 +287,15 @@ constexpr size_t lane_lml_lds_bytes() {
 
 #ifndef PDQ_HOST_EMU
 // One lane per (instance, tangent group), grid-stride over the n_inst G
-// lanes.  The solver is a compile-time constant of the instantiation.
+// lanes.  The solver is a compile-time constant of the instantiation; the
+// linearisation is ts0 (the C ABI refuses ts1 for the LML).
 template <int N, int D, int TPL, int SOLVER>
 __global__ void __launch_bounds__(64) lane_lml_kernel(LmlArgs args) {
   extern __shared__ double lds[];
   LmlArgs a = args;
   a.cfg.strategy = PDQ_FIXEDPOINT;
   a.cfg.solver = SOLVER;
+  a.cfg.constraint = PDQ_TS0;
   LArgs la{};
   la.cfg = a.cfg;
   la.n_inst = a.n_inst;

@@ -39,6 +39,8 @@ __all__ = [
     "error_residual_std",
     "error_state_std",
     "jacobian_materialize",
+    "jacobian_monte_carlo_fwd",
+    "jacobian_monte_carlo_rev",
     "jetexpand_ode_padded_scan",
     "linear",
     "logistic",
@@ -131,10 +133,32 @@ def ode(func, /, *, jacobian=None):
 
 
 class jacobian_materialize:
-    """jacobians.py:78-113; the registry provides analytic Jacobians."""
+    """jacobians.py:78-113; the registry provides analytic Jacobians, so traces are exact."""
 
     def __init__(self, *, jacfun=None) -> None:
         self.jacfun = jacfun
+
+    def __repr__(self) -> str:
+        return f"{self.__class__.__name__}(jacfun={self.jacfun})"
+
+
+class jacobian_monte_carlo_fwd:
+    """jacobians.py:116-192: an exact dense Jacobian; Hutchinson estimates of traces and diagonals.
+
+    The device path uses the dense Jacobian (dense ts1).  The estimates draw Rademacher probes from
+    JAX's random stream, which the device path does not reproduce: isotropic ts1 with this handler
+    raises NotImplementedError (pass jacobian_materialize() for the exact trace)."""
+
+    def __init__(self, *, seed=1, num_probes=10) -> None:
+        self.seed = seed
+        self.num_probes = num_probes
+
+    def __repr__(self) -> str:
+        return f"{self.__class__.__name__}(seed={self.seed}, num_probes={self.num_probes})"
+
+
+class jacobian_monte_carlo_rev(jacobian_monte_carlo_fwd):
+    """jacobians.py:195-271 (reverse mode; the reference's default handler of ode())."""
 
 
 def jetexpand_ode_padded_scan(*, num: int):
@@ -379,13 +403,22 @@ class _StateSpaceModel:
         return Constraint("ts0", ode)
 
     def constraint_ode_ts1(self, ode, /, taylor_point=None):
-        """ssm_impl_api.py:536-558 (prior-mean Taylor point, exact Jacobian)."""
+        """ssm_impl_api.py:536-558 (prior-mean Taylor point).
+
+        Dense: the exact Jacobian (every handler materialises it).  Isotropic
+        (ssm_impl_isotropic.py:321-355): the trace over d of the Jacobian, exact with
+        jacobian_materialize (jacobians.py:100-106); the reference's default handler estimates
+        it from JAX random probes, which the device path does not reproduce."""
         if not isinstance(ode, JetOde):
             raise TypeError(ode)
         if taylor_point is not None:
             raise NotImplementedError("only the prior-mean Taylor point is implemented")
-        if self.factorisation != "dense":
-            msg = "ts1 for isotropic/blockdiag models uses Hutchinson estimators (jacobians.py:116-271), not implemented on the device path"
+        if self.factorisation == "blockdiag":
+            msg = "ts1 for blockdiag models (Jacobian diagonals, jacobians.py:108-113, 171-193) is not implemented on the device path"
+            raise NotImplementedError(msg)
+        if self.factorisation == "isotropic" and not isinstance(ode.jacobian, jacobian_materialize):
+            msg = "isotropic ts1 runs with the exact trace: construct the ODE with jacobian=jacobian_materialize()"
+            msg += f" (got {ode.jacobian!r}; Hutchinson estimates from JAX's random stream are not reproduced)"
             raise NotImplementedError(msg)
         return Constraint("ts1", ode)
 

@@ -345,6 +345,46 @@ def test_error_estimator_variants(oracle, pdq, fact, lanes, solver, kw):
     _compare(oracle, pdq, cfg, params, y0s, times)
 
 
+@pytest.mark.parametrize(
+    "strategy,solver,kw",
+    [
+        ("fixedpoint", "dynamic", {}),
+        ("filter", "solver", {}),
+        ("fixedinterval", "mle", {}),
+        ("fixedpoint", "dynamic", dict(error_estimator="state_std", error_derivative_idx=1)),
+        ("fixedpoint", "dynamic", dict(re_linearize_after_calibration=True)),
+        ("filter", "solver", dict(error_re_linearize=True)),
+    ],
+)
+def test_isotropic_ts1(oracle, pdq, strategy, solver, kw):
+    """Isotropic ts1 with the exact trace (ssm_impl_isotropic.py:321-355, jacobians.py:100-106) on the
+    lane kernel, against the oracle pinned by the dense == isotropic equivalence."""
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver=solver, constraint="ts1", rtol=1e-5, atol=1e-7, **kw)
+    params, y0s, times = _lv_batch(6, seed=43, t_lo=5.0, t_hi=10.0, K=12)
+    _compare(oracle, pdq, cfg, params, y0s, times)
+
+
+def test_isotropic_ts1_van_der_pol(oracle, pdq):
+    rng = np.random.default_rng(44)
+    mu = np.exp(rng.uniform(np.log(1.0), np.log(10.0), size=(6, 1)))
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, strategy="filter", solver="dynamic", constraint="ts1",
+                     vector_field="van_der_pol", n_params=1, rtol=1e-5, atol=1e-7)
+    _compare(oracle, pdq, cfg, mu, np.tile([2.0, 0.0], (6, 1)), np.tile(np.linspace(0.0, 6.3, 9), (6, 1)))
+
+
+def test_isotropic_ts1_is_refused_off_the_lane_kernel(pdq):
+    """The cooperative kernel and the LML implement isotropic ts0 only."""
+    params, y0s, times = _lv_batch(1, seed=1, K=5)
+    tc = pdq.taylor_coefficients("lotka_volterra", _dev(params), _dev(y0s), 0.0, 4)
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=4, strategy="filter", constraint="ts1", lanes_per_instance=16)
+    with pytest.raises(pdq.PdqError, match="isotropic ts1"):
+        pdq.solve(cfg, _dev(params), tc, _dev(times))
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, strategy="fixedpoint", constraint="ts1")
+    grid = np.linspace(0.0, 2.0, 11)
+    with pytest.raises(pdq.PdqError, match="ts0"):
+        pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0s), _dev(grid), 5, _dev(np.ones((1, 3, 2))), _dev(np.ones((1, 3))), grad=False)
+
+
 def test_error_state_std_is_refused_by_the_cooperative_kernel(pdq):
     cfg = pdq.Config(factorisation="dense", num_derivatives=4, strategy="filter", error_estimator="state_std")
     params, y0s, times = _lv_batch(1, seed=1, K=5)

@@ -328,3 +328,33 @@ def test_lane_kernel_error_estimators(oracle, emu, solver, kw):
     cfg = capi.Config(factorisation="isotropic", num_derivatives=4, strategy="fixedpoint", solver=solver, rtol=1e-5, atol=1e-7, **kw)
     p, y, t = T._lv_batch(3, seed=5, t_lo=4.0, t_hi=8.0, K=10)
     _run(oracle, emu, cfg, p, y, t, kernel="lane")
+
+
+@pytest.mark.parametrize(
+    "strategy,solver,vf,kw",
+    [
+        ("fixedpoint", "dynamic", "lotka_volterra", {}),
+        ("filter", "solver", "lotka_volterra", {}),
+        ("fixedinterval", "mle", "lotka_volterra", {}),
+        ("fixedpoint", "dynamic", "lotka_volterra", dict(error_estimator="state_std", error_derivative_idx=1)),
+        ("fixedpoint", "dynamic", "lotka_volterra", dict(re_linearize_after_calibration=True)),
+        ("filter", "solver", "lotka_volterra", dict(error_re_linearize=True)),
+        ("filter", "dynamic", "van_der_pol", {}),
+    ],
+)
+def test_lane_kernel_isotropic_ts1(oracle, emu, strategy, solver, vf, kw):
+    """Isotropic ts1 with the exact trace (ssm_impl_isotropic.py:321-355, jacobians.py:100-106) in
+    the lane kernel's code (the oracle pinned by the dense == isotropic equivalence)."""
+    from probdiffeq_amd import capi
+
+    if vf == "van_der_pol":
+        rng = np.random.default_rng(4)
+        p = np.exp(rng.uniform(np.log(1.0), np.log(10.0), size=(3, 1)))
+        y, t = np.tile([2.0, 0.0], (3, 1)), np.tile(np.linspace(0.0, 6.3, 8), (3, 1))
+        cfg = capi.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver=solver, constraint="ts1",
+                          vector_field=vf, n_params=1, rtol=1e-5, atol=1e-7, **kw)
+    else:
+        p, y, t = T._lv_batch(3, seed=6, t_lo=4.0, t_hi=8.0, K=10)
+        cfg = capi.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver=solver, constraint="ts1",
+                          rtol=1e-5, atol=1e-7, **kw)
+    _run(oracle, emu, cfg, p, y, t, kernel="lane")

@@ -85,14 +85,19 @@ def test_burgers_blockdiag_matches_dop853(oracle):
     assert np.max(np.abs(u - s.y[:, -1])) < 1e-4
 
 
-def test_dense_equals_isotropic_for_isotropic_ode(oracle):
-    """test_dynamic_across_factorisations.py:12-65 (ts0 variant; isotropic ts1 is out of scope)."""
+@pytest.mark.parametrize("constraint,estimator", [("ts0", "residual_std"), ("ts1", "residual_std"), ("ts1", "state_std")])
+def test_dense_equals_isotropic_for_isotropic_ode(oracle, constraint, estimator):
+    """test_dynamic_across_factorisations.py:12-65: for f(u) = lam u + t the dense ts1 Jacobian row
+    [-lam I, I, 0] is kron([-tr(lam I)/d, 1, 0], I), so the isotropic ts1 linearisation (exact trace,
+    jacobian_materialize) and the dense one describe the same model.  The reference's test observes
+    the jet-lifted ODE (all derivatives); this one the plain first-order residual."""
     lam = [-0.7] * 5
     u0 = np.linspace(-1, 1, 5)
     save_at = np.linspace(0.0, 10.0, 7)
     res = []
     for fact in ["dense", "isotropic"]:
-        cfg = oracle.Config(fact=fact, nu=2, d=5, vf="linear", strategy="fixedpoint", rtol=1e-3, atol=1e-3)
+        cfg = oracle.Config(fact=fact, nu=2, d=5, vf="linear", strategy="fixedpoint", rtol=1e-3, atol=1e-3,
+                            constraint=constraint, error_estimator=estimator, error_derivative_idx=1)
         tc = oracle.taylor_coefficients("linear", lam, u0, 0.0, 2)
         res.append(oracle.solve_save_at(cfg, lam, tc, save_at))
     rd, ri = res
@@ -104,6 +109,22 @@ def test_dense_equals_isotropic_for_isotropic_ode(oracle):
         Ld = rd["chol"][k].reshape(15, 15)
         Li = ri["chol"][k].reshape(3, 3)
         assert np.allclose(Ld @ Ld.T, np.kron(Li @ Li.T, np.eye(5)), rtol=1e-9, atol=1e-12)
+
+
+def test_isotropic_ts1_lotka_volterra_matches_dop853(oracle):
+    """Isotropic ts1 (exact trace) meets the tolerance like ts0 (test_solve_adaptive_save_at.py:202-247),
+    and differs from ts0 (the Jacobian term is live)."""
+    tc = oracle.taylor_coefficients("lotka_volterra", THETA, [20.0, 20.0], 0.0, 4)
+    save_at = np.linspace(0.0, 50.0, 200)
+    out = {}
+    for constraint in ["ts0", "ts1"]:
+        cfg = oracle.Config(fact="isotropic", strategy="fixedpoint", nu=4, rtol=1e-6, atol=1e-8, constraint=constraint)
+        out[constraint] = r = oracle.solve_save_at(cfg, THETA, tc, save_at)
+        assert r["status"] == 0
+        u = r["mean"].reshape(200, 5, 2)[:, 0, :]
+        ref = lv_reference(save_at)
+        assert np.max(np.abs(u - ref) / np.abs(ref)) < 1e-4
+    assert out["ts0"]["attempts"] != out["ts1"]["attempts"] or not np.array_equal(out["ts0"]["mean"], out["ts1"]["mean"])
 
 
 def test_vmap_dense_equals_blockdiag_for_diagonal_ode(oracle):
