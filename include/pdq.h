@@ -41,6 +41,9 @@ enum { PDQ_NORM_SCALE_THEN_RMS = 0, PDQ_NORM_RMS_THEN_SCALE = 1 };
 enum { PDQ_CONTROL_INTEGRAL = 0, PDQ_CONTROL_PI = 1 };
 /* error_residual_std (solvers.py:850-996), error_state_std (:999-1098) */
 enum { PDQ_ERROR_RESIDUAL_STD = 0, PDQ_ERROR_STATE_STD = 1 };
+/* Priors: integrated Wiener process (ssm_impl_*.py prior_wiener_integrated)
+ * and the dense exponential-integrator prior (ssm_impl_dense.py:392-629). */
+enum { PDQ_PRIOR_IWP = 0, PDQ_PRIOR_EXPONENTIAL = 1 };
 /* device vector-field registry (problems.py:283-295 convention u' = f(u, t)) */
 enum {
   PDQ_VF_LOTKA_VOLTERRA = 0, /* params (a, b, c, d), d = 2 */
@@ -95,6 +98,17 @@ typedef struct pdq_config {
   int32_t error_re_linearize;
   int32_t error_derivative_idx;
   int32_t error_per_unit_step;
+  /* Prior.  PDQ_PRIOR_EXPONENTIAL (dense factorisation, (nu+1) d <= 16,
+   * solves only): prior_exponential / prior_ornstein_uhlenbeck_integrated /
+   * prior_matern (ssm_impl_dense.py:516-629, ssm_impl_api.py:381-510), the
+   * linear SDE dx = A x dt + B dW with A the IWP's shift matrix whose last d
+   * rows are prior_drift and B = kron(e_last, base_output_scale I); each
+   * transition is exp_gram_cholesky(pade_and_legendre_9) of the
+   * preconditioned A dt and B (util/gram_util.py:17-77).  prior_drift is a
+   * device pointer to d x (nu+1) d doubles (row-major, derivative-major
+   * columns), shared by the batch. */
+  int32_t prior;
+  const double* prior_drift;
 } pdq_config;
 
 /* Caller-allocated device outputs; NULL members are not written.

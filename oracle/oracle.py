@@ -70,6 +70,8 @@ def _cfg_struct(ct):
             ("error_re_linearize", ctypes.c_int),
             ("error_derivative_idx", ctypes.c_int),
             ("error_per_unit_step", ctypes.c_int),
+            ("prior", ctypes.c_int),
+            ("prior_drift", ctypes.POINTER(ct)),
         ]
 
     return _Cfg
@@ -156,8 +158,27 @@ class Config:
     error_re_linearize: bool = False
     error_derivative_idx: int = 0
     error_per_unit_step: bool = False
+    # "iwp" or "exponential" (dense): prior_drift is the drift's bottom block,
+    # d x (nu+1) d, derivative-major columns (ssm_impl_dense.py:541-629)
+    prior: str = "iwp"
+    prior_drift: object = None
 
     def c(self):
+        if self.prior not in ("iwp", "exponential"):
+            raise ValueError(f"unknown prior {self.prior!r}")
+        drift = None
+        if self.prior == "exponential":
+            if self.fact != "dense":
+                raise ValueError("exponential priors are dense only (ssm_impl_isotropic.py:484-520)")
+            drift = np.ascontiguousarray(np.asarray(self.prior_drift, dtype=np.float64).reshape(self.d, self.N * self.d), dtype=_dtype())
+        c = self._c()
+        c.prior = 1 if drift is not None else 0
+        if drift is not None:
+            c.prior_drift = drift.ctypes.data_as(ctypes.POINTER(_ct()))
+            c._keep = drift  # the array lives as long as the struct
+        return c
+
+    def _c(self):
         return _CFG[_ct()](
             FACT[self.fact],
             self.nu,
@@ -310,6 +331,15 @@ def system_matrices(nu):
     Q = _zeros((nu + 1, nu + 1))
     lib().pdqo_system_matrices(nu, _p(A), _p(Q))
     return A, Q
+
+
+def exp_gram(A, B):
+    """gram_util.exp_gram_cholesky(pade_and_legendre_9): (e^A, U) with U U^T the Gramian."""
+    A, B = _f64(A), _f64(B)
+    n, m = B.shape
+    eA, U = _zeros((n, n)), _zeros((n, n))
+    lib().pdqo_exp_gram(n, m, _p(A), _p(B), _p(eA), _p(U))
+    return eA, U
 
 
 def transition(cfg: Config, dt, os=None):

@@ -276,6 +276,138 @@ Reverted revert_conditional(const Mat& R_X_F, const Mat& R_X, const Mat& R_YX, i
   return out;
 }
 
+// backend/linalg.py solve_lu = jnp.linalg.solve: LU factorisation with
+// partial pivoting (the largest |entry| of the column below the diagonal,
+// first index on ties, as LAPACK's idamax), then forward and back
+// substitution of every right-hand side.
+Mat solve_lu(const Mat& M, const Mat& B) {
+  const int n = M.r;
+  Mat A = M, X = B;
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    for (int r = k + 1; r < n; ++r)
+      if (std::fabs(A(r, k)) > std::fabs(A(piv, k))) piv = r;
+    if (piv != k) {
+      for (int j = 0; j < n; ++j) std::swap(A(k, j), A(piv, j));
+      for (int j = 0; j < X.c; ++j) std::swap(X(k, j), X(piv, j));
+    }
+    const Real inv_p = 1.0 / A(k, k);
+    for (int r = k + 1; r < n; ++r) {
+      const Real l = A(r, k) * inv_p;
+      A(r, k) = l;
+      for (int j = k + 1; j < n; ++j) A(r, j) -= l * A(k, j);
+    }
+  }
+  for (int j = 0; j < X.c; ++j) {
+    for (int i = 1; i < n; ++i) {
+      Real s = X(i, j);
+      for (int c = 0; c < i; ++c) s -= A(i, c) * X(c, j);
+      X(i, j) = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      Real s = X(i, j);
+      for (int c = i + 1; c < n; ++c) s -= A(i, c) * X(c, j);
+      X(i, j) = s / A(i, i);
+    }
+  }
+  return X;
+}
+
+// util/gram_util.py:17-77 exp_gram_cholesky with pade_and_legendre_9
+// (:261-331), the float64 choice of ssm_impl_dense.py:611-617: e^A and a
+// lower-triangular U with U U^T = int_0^1 e^{sA} B B^T e^{sA^T} ds, by
+// scaling A (and B) by 2^-s, a degree-9 Pade approximant of e^A with a
+// Legendre-weighted approximation of the Gramian's square root, and s
+// doublings, each a QR of [U, e^A U]; columns of U are signed so that its
+// diagonal is non-negative.
+void exp_gram(const Mat& A_in, const Mat& B_in, Mat& eA, Mat& U) {
+  static const Real pade[10] = {17643225600.0, 8821612800.0, 2075673600.0, 302702400.0, 30270240.0,
+                                2162160.0,     110880.0,     3960.0,       90.0,        1.0};
+  static const Real leg[10][10] = {
+      {17643225600.0, 0, 605404800.0, 0, 4324320.0, 0, 7920.0, 0, 2.0, 0},
+      {0, 8821612800.0, 0, 155675520.0, 0, 617760.0, 0, 528.0, 0, 0},
+      {0, 0, 1470268800.0, 0, 15444000.0, 0, 34320.0, 0, 10.0, 0},
+      {0, 0, 0, 147026880.0, 0, 960960.0, 0, 1092.0, 0, 0},
+      {0, 0, 0, 0, 10501920.0, 0, 42120.0, 0, 18.0, 0},
+      {0, 0, 0, 0, 0, 583440.0, 0, 1320.0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 26520.0, 0, 26.0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1020.0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 0, 34.0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 1.0}};
+  const Real eta = 0.41098379928173995, q = 9.0;
+  const int n = A_in.r, m = B_in.c;
+  // _exp_gram_cholesky_init: the number of doublings
+  Real a_l1 = 0.0;
+  for (int j = 0; j < n; ++j) {
+    Real s = 0.0;
+    for (int i = 0; i < n; ++i) s += std::fabs(A_in(i, j));
+    a_l1 = std::max(a_l1, s);
+  }
+  const Real s1 = std::log2(a_l1 / eta), s2 = std::log2((Real)(n - 1) / q);
+  Real num = std::max((Real)0.0, std::ceil(std::max(s1, s2)));
+  if (!(num <= 64)) num = 64;  // bounded (as the device): a larger norm gives a non-finite transition
+  const Real two_num = std::pow((Real)2.0, num);
+  const Mat A = scale(1.0 / two_num, A_in);
+  Mat B = B_in;
+  const Real sq = std::sqrt(two_num);
+  for (auto& x : B.a) x = x / sq;
+  // pade_and_legendre_9.init
+  const Mat I = eye(n);
+  const Mat A2 = matmul(A, A), A4 = matmul(A2, A2), A6 = matmul(A4, A2), A8 = matmul(A6, A2);
+  const Mat Up = matmul(A, add(add(add(add(scale(pade[9], A8), scale(pade[7], A6)), scale(pade[5], A4)),
+                                   scale(pade[3], A2)),
+                               scale(pade[1], I)));
+  const Mat V = add(add(add(add(scale(pade[8], A8), scale(pade[6], A6)), scale(pade[4], A4)), scale(pade[2], A2)),
+                    scale(pade[0], I));
+  const Mat VmU = sub(V, Up);
+  eA = solve_lu(VmU, add(V, Up));
+  Mat P = A2;
+  Mat PB = matmul(P, B);
+  std::vector<Mat> Le(5, Mat(n, m)), Lo(5, Mat(n, m));
+  Le[0] = add(scale(leg[0][2], PB), scale(leg[0][0], B));
+  Le[1] = scale(leg[2][2], PB);
+  Lo[0] = add(scale(leg[1][3], PB), scale(leg[1][1], B));
+  Lo[1] = scale(leg[3][3], PB);
+  for (int k = 2; k <= 4; ++k) {
+    P = matmul(A2, P);
+    PB = matmul(P, B);
+    for (int i = 0; i < 5; ++i) {
+      Le[i] = add(Le[i], scale(leg[2 * i][2 * k], PB));
+      Lo[i] = add(Lo[i], scale(leg[2 * i + 1][2 * k + 1], PB));
+    }
+  }
+  for (int i = 0; i < 5; ++i) Lo[i] = matmul(A, Lo[i]);
+  Mat rhs(n, 10 * m);
+  for (int i = 0; i < 10; ++i) {
+    const Mat& ell = (i % 2 == 0) ? Le[i / 2] : Lo[i / 2];
+    const Real nrm = std::sqrt((Real)(2 * i + 1));
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < m; ++c) rhs(r, i * m + c) = ell(r, c) / nrm;
+  }
+  const Mat Ls = solve_lu(VmU, rhs);
+  const Mat Rt = qr_r(transpose(Ls));  // min(10m, n) x n
+  U = Mat(n, n);
+  for (int i = 0; i < Rt.r; ++i)
+    for (int j = 0; j < n; ++j) U(j, i) = Rt(i, j);
+  // _exp_gram_cholesky_double
+  for (int it = 0; it < (int)num; ++it) {
+    const Mat eU = matmul(eA, U);
+    Mat stack(n, 2 * n);
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c) {
+        stack(r, c) = U(r, c);
+        stack(r, n + c) = eU(r, c);
+      }
+    U = transpose(qr_r(transpose(stack)));
+    eA = matmul(eA, eA);
+  }
+  for (int j = 0; j < n; ++j) {
+    Real sgn = U(j, j) > 0 ? 1.0 : (U(j, j) < 0 ? -1.0 : 0.0);
+    if (sgn == 0.0) sgn = 1.0;
+    for (int i = 0; i < n; ++i) U(i, j) = U(i, j) * sgn;
+  }
+}
+
 // util/cholesky_util.py:106-176 cholesky_hilbert (Kahan's recurrence).
 Mat cholesky_hilbert(int n) {
   const Real Kf = 0.0;
@@ -519,6 +651,11 @@ struct pdqo_cfg {
   int re_linearize;  // solver_dynamic(re_linearize_after_calibration=True)
   // error estimator: 0 error_residual_std, 1 error_state_std; options
   int error_estimator, error_re_linearize, error_derivative_idx, error_per_unit_step;
+  // prior: 0 integrated Wiener process, 1 exponential (dense only) whose
+  // linear drift's bottom block (d x (nu+1)d, row-major, derivative-major
+  // columns) is prior_drift
+  int prior;
+  const Real* prior_drift;
 };
 }
 
@@ -683,6 +820,7 @@ struct Problem {
   int nblk;      // blockdiag: d blocks; otherwise 1
   Mat A1, Q1;    // 1-d IWP matrices
   Mat A, Q;      // dense: kron(A1, I), kron(Q1, I); iso/bd: A1, Q1
+  Mat Aexp, Bexp;  // exponential prior: drift (n x n) and diffusion (n x d)
   Vf vf;
   SSMBase ops;
 
@@ -706,6 +844,18 @@ struct Problem {
             A(i * d + k, j * d + k) = A1(i, j);
             Q(i * d + k, j * d + k) = Q1(i, j) * c.base_scale;  // kron(Q1, Lambda)
           }
+      if (c.prior == 1) {
+        // prior_exponential_diffuse (ssm_impl_dense.py:541-629): A = kron(a, I)
+        // with a the shift matrix and the drift as its last d rows;
+        // B = kron(e_last, Lambda)
+        Aexp = Mat(n, n);
+        for (int i = 0; i + 1 < N; ++i)
+          for (int k = 0; k < d; ++k) Aexp(i * d + k, (i + 1) * d + k) = 1.0;
+        for (int r = 0; r < d; ++r)
+          for (int j = 0; j < n; ++j) Aexp((N - 1) * d + r, j) = c.prior_drift[r * n + j];
+        Bexp = Mat(n, d);
+        for (int k = 0; k < d; ++k) Bexp((N - 1) * d + k, k) = c.base_scale;
+      }
     } else {
       n = N;
       nblk = c.fact == FACT_BLOCKDIAG ? d : 1;
@@ -741,9 +891,22 @@ BCond transition(const Problem& P, Real dt, const Vec& os) {
           C.to[i * P.d + k] = p1[i];
           C.tl[i * P.d + k] = pinv1[i];
         }
-      C.A = P.A;
       C.noise.m = Mat(P.n, 1);
-      C.noise.L = scale(std::sqrt(std::fabs(dt)) * os[0], P.Q);
+      if (P.cfg.prior == 1) {
+        // DenseExponential.transition (ssm_impl_dense.py:405-423)
+        Mat Ap(P.n, P.n), Bp(P.n, P.d);
+        for (int i = 0; i < P.n; ++i)
+          for (int j = 0; j < P.n; ++j) Ap(i, j) = ((dt * C.tl[i]) * P.Aexp(i, j)) * C.to[j];
+        for (int i = 0; i < P.n; ++i)
+          for (int k = 0; k < P.d; ++k) Bp(i, k) = (std::sqrt(std::fabs(dt)) * std::fabs(C.tl[i])) * P.Bexp(i, k);
+        Mat eA, U;
+        exp_gram(Ap, Bp, eA, U);
+        C.A = eA;
+        C.noise.L = scale(os[0], U);
+      } else {
+        C.A = P.A;
+        C.noise.L = scale(std::sqrt(std::fabs(dt)) * os[0], P.Q);
+      }
     } else if (P.cfg.fact == FACT_ISO) {
       C.to = p1;
       C.tl = pinv1;
@@ -1771,6 +1934,16 @@ void pdqo_transition(const pdqo_cfg* cfg, Real dt, const Real* os, Real* A, Real
   std::memcpy(noise_chol, c.b[0].noise.L.a.data(), sizeof(Real) * P.n * P.n);
   std::memcpy(tl, c.b[0].tl.data(), sizeof(Real) * P.n);
   std::memcpy(to, c.b[0].to.data(), sizeof(Real) * P.n);
+}
+
+// gram_util.exp_gram_cholesky(pade_and_legendre_9) of A (n x n) and B (n x m).
+void pdqo_exp_gram(int n, int m, const Real* A, const Real* B, Real* eA, Real* U) {
+  Mat a(n, n), b(n, m), e, u;
+  std::memcpy(a.a.data(), A, sizeof(Real) * n * n);
+  std::memcpy(b.a.data(), B, sizeof(Real) * n * m);
+  exp_gram(a, b, e, u);
+  std::memcpy(eA, e.a.data(), sizeof(Real) * n * n);
+  std::memcpy(U, u.a.data(), sizeof(Real) * n * n);
 }
 
 Real pdqo_logpdf(int fact, int rows, int cols, const Real* mean, const Real* chol, const Real* u) {

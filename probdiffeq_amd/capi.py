@@ -10,7 +10,9 @@ from __future__ import annotations
 import ctypes
 import dataclasses
 import os
+from typing import Any
 
+import numpy as np
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -23,6 +25,7 @@ SOLVER = {"solver": 0, "dynamic": 1, "mle": 2}
 ERROR_NORM = {"scale_then_rms": 0, "rms_then_scale": 1}
 CONTROL = {"integral": 0, "proportional_integral": 1}
 ERROR_ESTIMATOR = {"residual_std": 0, "state_std": 1}
+PRIOR = {"iwp": 0, "exponential": 1}
 VF = {"lotka_volterra": 0, "van_der_pol": 1, "burgers": 2, "linear": 3, "logistic": 4}
 STATUS = {0: "ok", 1: "max_attempts", 2: "nonfinite", 3: "dt_underflow"}
 
@@ -66,6 +69,8 @@ class _Config(ctypes.Structure):
         ("error_re_linearize", ctypes.c_int32),
         ("error_derivative_idx", ctypes.c_int32),
         ("error_per_unit_step", ctypes.c_int32),
+        ("prior", ctypes.c_int32),
+        ("prior_drift", ctypes.c_void_p),
     ]
 
 
@@ -132,8 +137,29 @@ class Config:
     error_re_linearize: bool = False
     error_derivative_idx: int = 0
     error_per_unit_step: bool = False
+    # "iwp", or "exponential" (dense): prior_drift holds the drift's last d rows, d x (nu+1) d
+    # float64 (a device tensor for the library; host memory for the CPU build of the kernels)
+    prior: str = "iwp"
+    prior_drift: Any = None
 
     def c(self) -> _Config:
+        c = self._c()
+        if self.prior not in PRIOR:
+            raise ValueError(f"unknown prior {self.prior!r}")
+        c.prior = PRIOR[self.prior]
+        if self.prior_drift is not None:
+            drift = self.prior_drift
+            if isinstance(drift, np.ndarray):
+                drift = np.ascontiguousarray(drift, dtype=np.float64)
+                c.prior_drift = drift.ctypes.data
+            else:
+                if drift.dtype != torch.float64 or not drift.is_contiguous():
+                    raise ValueError("prior_drift must be a contiguous float64 tensor")
+                c.prior_drift = drift.data_ptr()
+            c._keep = drift  # the memory lives as long as the struct
+        return c
+
+    def _c(self) -> _Config:
         return _Config(
             FACT[self.factorisation],
             self.num_derivatives,

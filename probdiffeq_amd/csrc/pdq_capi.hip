@@ -139,6 +139,18 @@ int check_common(const pdq_config* cfg, int n_inst, int K) {
   if (cfg->error_derivative_idx < 0 || cfg->error_derivative_idx > cfg->num_derivatives)
     return set_err(PDQ_ERR_INVALID, "error derivative_idx %d outside [0, %d]", cfg->error_derivative_idx,
                    cfg->num_derivatives);
+  if (cfg->prior != PDQ_PRIOR_IWP && cfg->prior != PDQ_PRIOR_EXPONENTIAL)
+    return set_err(PDQ_ERR_INVALID, "unknown prior %d", cfg->prior);
+  if (cfg->prior == PDQ_PRIOR_EXPONENTIAL) {
+    if (cfg->factorisation != PDQ_DENSE)
+      return set_err(PDQ_ERR_UNSUPPORTED,
+                     "exponential priors are implemented for the dense factorisation (as in the reference, "
+                     "ssm_impl_isotropic.py:484-520, ssm_impl_blockdiag.py:549-585)");
+    if (!cfg->prior_drift) return set_err(PDQ_ERR_INVALID, "the exponential prior needs prior_drift");
+    if ((cfg->num_derivatives + 1) * cfg->ode_dim > 16)
+      return set_err(PDQ_ERR_UNSUPPORTED, "the exponential prior supports (nu + 1) d <= 16 (got %d)",
+                     (cfg->num_derivatives + 1) * cfg->ode_dim);
+  }
   if (cfg->constraint == PDQ_TS1 && cfg->factorisation == PDQ_BLOCKDIAG)
     return set_err(PDQ_ERR_UNSUPPORTED,
                    "ts1 is implemented for the dense and isotropic factorisations (blockdiag ts1 takes the "
@@ -159,8 +171,8 @@ std::string tune_key(const pdq_config& c, int n_inst, int mode) {
   int bucket = 1;
   while (bucket < n_inst) bucket <<= 1;
   char buf[256];
-  snprintf(buf, sizeof(buf), "%d/%d/%d/%d/%d/%d/%d/%d/%d", mode, c.factorisation, c.num_derivatives, c.ode_dim,
-           c.strategy, c.solver, c.constraint, c.vector_field, bucket);
+  snprintf(buf, sizeof(buf), "%d/%d/%d/%d/%d/%d/%d/%d/%d/%d", mode, c.factorisation, c.num_derivatives, c.ode_dim,
+           c.strategy, c.solver, c.constraint, c.vector_field, c.prior, bucket);
   return buf;
 }
 
@@ -332,6 +344,8 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
     return set_err(PDQ_ERR_UNSUPPORTED, "gradients need ts0 (the covariance recursion must not depend on them)");
   if (cfg->constraint != PDQ_TS0 && cfg->factorisation == PDQ_ISOTROPIC)
     return set_err(PDQ_ERR_UNSUPPORTED, "the isotropic LML is implemented for ts0");
+  if (cfg->prior != PDQ_PRIOR_IWP)
+    return set_err(PDQ_ERR_UNSUPPORTED, "the LML is implemented for the integrated Wiener process prior");
   if (cfg->solver == PDQ_SOLVER_MLE) return set_err(PDQ_ERR_UNSUPPORTED, "solver_mle is not supported by the LML path");
   if (every < 1 || (K_grid - 1) % every != 0)
     return set_err(PDQ_ERR_INVALID, "every (%d) must divide K_grid - 1 (%d)", every, K_grid - 1);

@@ -27,8 +27,10 @@ struct Ctx {
   const KArgs* a;
   const Layout* L;
   int nc, mc, mcp, ko, N, d, dense;  // mcp: primal mean columns (mc includes tangents)
-  double* sA;   // shared A (nc x nc)
-  double* sQ;   // shared Q (nc x nc)
+  double* sA;   // shared A (nc x nc); the exponential prior's e^A of the current step
+  double* sQ;   // shared Q (nc x nc); the exponential prior's Gramian factor
+  const double* sD;  // exponential prior: the drift's last d rows (d x nc, shared)
+  int expo;          // exponential prior (dense)
   double* ws;   // instance workspace base
   double* prm;  // vector-field parameters (LDS copy)
   __device__ double* slot(int s) const { return ws + s * L->s_size; }
@@ -103,8 +105,269 @@ __device__ void g_gain(const Grp<G>& g, const double* R, int ldr, int k, int n, 
   }
 }
 
+// ------------------------------------------------ exponential prior
+// Z = X Y for row-major X (n x k) and Y (k x m), k ascending (jnp matmul);
+// Z must not alias X or Y.
+template <int G>
+__device__ void g_gemm(const Grp<G>& g, const double* X, const double* Y, double* Z, int n, int k, int m) {
+  for (int e = g.l; e < n * m; e += G) {
+    const int i = e / m, j = e % m;
+    double acc = 0.0;
+    for (int q = 0; q < k; ++q) acc += X[i * k + q] * Y[q * m + j];
+    Z[e] = acc;
+  }
+}
+
+// jnp.linalg.solve (backend/linalg.py solve_lu): LU with partial pivoting of
+// the n x n matrix M in place (the first largest |entry| of each column
+// pivots, as LAPACK's idamax), the row swaps applied to the right-hand sides
+// X1 (n x m1) and X2 (n x m2), then forward and back substitution.
+template <int G>
+__device__ void g_lu_solve(const Grp<G>& g, double* M, int n, double* X1, int m1, double* X2, int m2) {
+  for (int k = 0; k < n; ++k) {
+    double best = -1.0;
+    int bi = k;
+    for (int r = k + g.l; r < n; r += G) {
+      const double v = fabs(M[r * n + k]);
+      if (v > best) {
+        best = v;
+        bi = r;
+      }
+    }
+    const double gbest = g.max(best);
+    // the smallest row index among the lanes holding the maximum
+    const double cand = g.max(best == gbest ? -(double)bi : -(double)n);
+    int piv = (int)(-cand);
+    piv = piv < k ? k : (piv >= n ? k : piv);
+    if (piv != k) {
+      for (int j = g.l; j < n; j += G) {
+        const double t = M[k * n + j];
+        M[k * n + j] = M[piv * n + j];
+        M[piv * n + j] = t;
+      }
+      for (int j = g.l; j < m1; j += G) {
+        const double t = X1[k * m1 + j];
+        X1[k * m1 + j] = X1[piv * m1 + j];
+        X1[piv * m1 + j] = t;
+      }
+      for (int j = g.l; j < m2; j += G) {
+        const double t = X2[k * m2 + j];
+        X2[k * m2 + j] = X2[piv * m2 + j];
+        X2[piv * m2 + j] = t;
+      }
+    }
+    g.sync();
+    const double inv_p = 1.0 / M[k * n + k];
+    for (int r = k + 1 + g.l; r < n; r += G) {
+      const double l = M[r * n + k] * inv_p;
+      M[r * n + k] = l;
+      for (int j = k + 1; j < n; ++j) M[r * n + j] -= l * M[k * n + j];
+    }
+    g.sync();
+  }
+  for (int e = g.l; e < m1 + m2; e += G) {
+    double* X = e < m1 ? X1 : X2;
+    const int m = e < m1 ? m1 : m2, j = e < m1 ? e : e - m1;
+    for (int i = 1; i < n; ++i) {
+      double s = X[i * m + j];
+      for (int q = 0; q < i; ++q) s -= M[i * n + q] * X[q * m + j];
+      X[i * m + j] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = X[i * m + j];
+      for (int q = i + 1; q < n; ++q) s -= M[i * n + q] * X[q * m + j];
+      X[i * m + j] = s / M[i * n + i];
+    }
+  }
+  g.sync();
+}
+
+// pade_and_legendre_9 (util/gram_util.py:261-331): Pade coefficients and the
+// Legendre coefficient matrix.
+__device__ __forceinline__ double pade9(int i) {
+  constexpr double c[10] = {17643225600.0, 8821612800.0, 2075673600.0, 302702400.0, 30270240.0,
+                            2162160.0,     110880.0,     3960.0,       90.0,        1.0};
+  return c[i];
+}
+__device__ __forceinline__ double legendre9(int i, int j) {
+  constexpr double c[10][10] = {{17643225600.0, 0, 605404800.0, 0, 4324320.0, 0, 7920.0, 0, 2.0, 0},
+                                {0, 8821612800.0, 0, 155675520.0, 0, 617760.0, 0, 528.0, 0, 0},
+                                {0, 0, 1470268800.0, 0, 15444000.0, 0, 34320.0, 0, 10.0, 0},
+                                {0, 0, 0, 147026880.0, 0, 960960.0, 0, 1092.0, 0, 0},
+                                {0, 0, 0, 0, 10501920.0, 0, 42120.0, 0, 18.0, 0},
+                                {0, 0, 0, 0, 0, 583440.0, 0, 1320.0, 0, 0},
+                                {0, 0, 0, 0, 0, 0, 26520.0, 0, 26.0, 0},
+                                {0, 0, 0, 0, 0, 0, 0, 1020.0, 0, 0},
+                                {0, 0, 0, 0, 0, 0, 0, 0, 34.0, 0},
+                                {0, 0, 0, 0, 0, 0, 0, 0, 0, 1.0}};
+  return c[i][j];
+}
+
+// DenseExponential.transition (ssm_impl_dense.py:405-423) for the step dt,
+// given the preconditioner P, PI: e^{A_p} into sA and the lower-triangular
+// Gramian factor U into sQ, with A_p = dt p^-1 A p and B_p = sqrt|dt| |p^-1| B
+// (exp_gram_cholesky, util/gram_util.py:17-77: scaling by 2^-s, the
+// degree-9 Pade / Legendre approximation, s doublings, non-negative
+// diagonal).  The noise factor of the transition is then output_scale * U.
+template <int G>
+__device__ void exp_transition(const Ctx<G>& c, double dt, const double* P, const double* PI) {
+  const Layout& L = *c.L;
+  const int n = c.nc, d = c.d, N = c.N, m10 = 10 * d;
+  double* eA = c.sA;
+  double* U = c.sQ;
+  double* Ap = c.W(L.w_xa);
+  double* Bp = c.W(L.w_xB);
+  double* A2 = c.W(L.w_xA2);
+  double* A4 = c.W(L.w_xA4);
+  double* A6 = c.W(L.w_xA6);
+  double* A8 = c.W(L.w_xA8);
+  double* T = c.W(L.w_xT);
+  double* V = c.W(L.w_xV);
+  double* VmU = c.W(L.w_xVmU);
+  double* Pw = c.W(L.w_xP);
+  double* PB = c.W(L.w_xPB);
+  double* rhs = c.W(L.w_xrhs);
+  double* tmp = c.W(L.w_xtmp);
+  double* buf = c.W(L.w_xbuf);
+  const double base = c.a->cfg.base_output_scale;
+  for (int e = c.g.l; e < n * n; e += G) {
+    const int i = e / n, j = e % n;
+    const double a = i < n - d ? (j == i + d ? 1.0 : 0.0) : c.sD[(i - (n - d)) * n + j];
+    Ap[e] = ((dt * PI[i]) * a) * P[j];
+  }
+  for (int e = c.g.l; e < n * d; e += G) {
+    const int i = e / d, k = e % d;
+    Bp[e] = (sqrt(fabs(dt)) * fabs(PI[i])) * (i == (N - 1) * d + k ? base : 0.0);
+  }
+  c.g.sync();
+  // _exp_gram_cholesky_init: the number of doublings from the 1-norm
+  double colmax = 0.0;
+  for (int j = c.g.l; j < n; j += G) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += fabs(Ap[i * n + j]);
+    colmax = fmax(colmax, s);
+  }
+  const double a_l1 = c.g.max(colmax);
+  const double s1 = log2(a_l1 / 0.41098379928173995), s2 = log2((double)(n - 1) / 9.0);
+  // at most 64 doublings (a 1-norm up to 0.41 2^64): a larger or non-finite
+  // norm gives a non-finite transition either way, and the loop stays bounded
+  double num = fmax(0.0, ceil(fmax(s1, s2)));
+  if (!(num <= 64.0)) num = 64.0;
+  const int nd = (int)num;
+  const double two = ldexp(1.0, nd), sq = sqrt(two);
+  for (int e = c.g.l; e < n * n; e += G) Ap[e] = (1.0 / two) * Ap[e];
+  for (int e = c.g.l; e < n * d; e += G) Bp[e] = Bp[e] / sq;
+  c.g.sync();
+  g_gemm(c.g, Ap, Ap, A2, n, n, n);
+  c.g.sync();
+  g_gemm(c.g, A2, A2, A4, n, n, n);
+  c.g.sync();
+  g_gemm(c.g, A4, A2, A6, n, n, n);
+  c.g.sync();
+  g_gemm(c.g, A6, A2, A8, n, n, n);
+  c.g.sync();
+  for (int e = c.g.l; e < n * n; e += G) {
+    const double id = (e / n == e % n) ? 1.0 : 0.0;
+    T[e] = (((pade9(9) * A8[e] + pade9(7) * A6[e]) + pade9(5) * A4[e]) + pade9(3) * A2[e]) + pade9(1) * id;
+    V[e] = (((pade9(8) * A8[e] + pade9(6) * A6[e]) + pade9(4) * A4[e]) + pade9(2) * A2[e]) + pade9(0) * id;
+  }
+  c.g.sync();
+  g_gemm(c.g, Ap, T, Pw, n, n, n);  // U of the Pade approximant
+  c.g.sync();
+  for (int e = c.g.l; e < n * n; e += G) {
+    VmU[e] = V[e] - Pw[e];
+    eA[e] = V[e] + Pw[e];
+  }
+  // Legendre terms: blocks 2i (even, L_even[i]) and 2i+1 (odd, L_odd[i]) of rhs
+  g_gemm(c.g, A2, Bp, PB, n, n, d);
+  c.g.sync();
+  for (int e = c.g.l; e < n * m10; e += G) {
+    const int r = e / m10, col = e % m10, blk = col / d, k = col % d;
+    const double pb = PB[r * d + k], b = Bp[r * d + k];
+    double v = 0.0;
+    if (blk == 0) v = legendre9(0, 2) * pb + legendre9(0, 0) * b;
+    else if (blk == 1) v = legendre9(1, 3) * pb + legendre9(1, 1) * b;
+    else if (blk == 2) v = legendre9(2, 2) * pb;
+    else if (blk == 3) v = legendre9(3, 3) * pb;
+    rhs[e] = v;
+  }
+  for (int e = c.g.l; e < n * n; e += G) Pw[e] = A2[e];
+  c.g.sync();
+  for (int kk = 2; kk <= 4; ++kk) {
+    g_gemm(c.g, A2, Pw, T, n, n, n);
+    c.g.sync();
+    for (int e = c.g.l; e < n * n; e += G) Pw[e] = T[e];
+    c.g.sync();
+    g_gemm(c.g, Pw, Bp, PB, n, n, d);
+    c.g.sync();
+    for (int e = c.g.l; e < n * m10; e += G) {
+      const int r = e / m10, col = e % m10, blk = col / d, k = col % d;
+      const int i = blk / 2;
+      const double coef = (blk % 2 == 0) ? legendre9(2 * i, 2 * kk) : legendre9(2 * i + 1, 2 * kk + 1);
+      rhs[e] = rhs[e] + coef * PB[r * d + k];
+    }
+    c.g.sync();
+  }
+  // L_odd = A L_odd, then every block / sqrt(2 b + 1)
+  for (int e = c.g.l; e < n * 5 * d; e += G) {
+    const int r = e / (5 * d), col = e % (5 * d), i = col / d, k = col % d;
+    double acc = 0.0;
+    for (int q = 0; q < n; ++q) acc += Ap[r * n + q] * rhs[q * m10 + (2 * i + 1) * d + k];
+    tmp[e] = acc;
+  }
+  c.g.sync();
+  for (int e = c.g.l; e < n * m10; e += G) {
+    const int r = e / m10, col = e % m10, blk = col / d, k = col % d;
+    const double v = (blk % 2 == 1) ? tmp[r * 5 * d + (blk / 2) * d + k] : rhs[e];
+    rhs[e] = v / sqrt((double)(2 * blk + 1));
+  }
+  c.g.sync();
+  g_lu_solve(c.g, VmU, n, eA, n, rhs, m10);
+  // qr_r(L^T)^T: the Gramian factor of the scaled problem
+  for (int e = c.g.l; e < m10 * n; e += G) {
+    const int i = e / n, j = e % n;
+    buf[e] = rhs[j * m10 + i];
+  }
+  c.g.sync();
+  g_qr_r(c.g, buf, m10, n, n);
+  const int kr = m10 < n ? m10 : n;
+  for (int e = c.g.l; e < n * n; e += G) {
+    const int j = e / n, i = e % n;  // U[j][i] = R[i][j]
+    U[e] = i < kr ? buf[i * n + j] : 0.0;
+  }
+  c.g.sync();
+  // _exp_gram_cholesky_double
+  for (int it = 0; it < nd; ++it) {
+    g_gemm(c.g, eA, U, T, n, n, n);
+    c.g.sync();
+    for (int e = c.g.l; e < n * n; e += G) {
+      const int r = e / n, col = e % n;
+      buf[r * n + col] = U[col * n + r];
+      buf[(n + r) * n + col] = T[col * n + r];
+    }
+    c.g.sync();
+    g_qr_r(c.g, buf, 2 * n, n, n);
+    for (int e = c.g.l; e < n * n; e += G) {
+      const int j = e / n, i = e % n;
+      U[e] = buf[i * n + j];
+    }
+    g_gemm(c.g, eA, eA, T, n, n, n);
+    c.g.sync();
+    for (int e = c.g.l; e < n * n; e += G) eA[e] = T[e];
+    c.g.sync();
+  }
+  // non-negative diagonal (column signs)
+  for (int j = c.g.l; j < n; j += G) {
+    const double djj = U[j * n + j];
+    const double sgn = djj < 0.0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) U[i * n + j] = U[i * n + j] * sgn;
+  }
+  c.g.sync();
+}
+
 // ------------------------------------------------ IWP prior and transitions
-// preconditioner_taylor (utilities.py:74-84) expanded to nc entries.
+// preconditioner_taylor (utilities.py:74-84) expanded to nc entries; with the
+// exponential prior also that step's transition (exp_transition).
 template <int G>
 __device__ void precon(const Ctx<G>& c, double dt, double* P, double* PI) {
   for (int i = c.g.l; i < c.nc; i += G) {
@@ -114,13 +377,19 @@ __device__ void precon(const Ctx<G>& c, double dt, double* P, double* PI) {
     P[i] = pow(dt, (double)pw) / s;
     PI[i] = pow(dt, -(double)pw) * s;
   }
+  if (c.expo) {
+    c.g.sync();
+    exp_transition(c, dt, P, PI);
+  }
 }
 
 // Scalar factor of the transition noise: dense sqrt|dt| * s (Q holds the base
 // scale), isotropic (sqrt|dt| * base) * s (ssm_impl_dense.py:360-362,
-// ssm_impl_isotropic.py:375-376).
+// ssm_impl_isotropic.py:375-376); the exponential prior's factor already
+// holds sqrt|dt| and the base scale (ssm_impl_dense.py:421-423).
 template <int G>
 __device__ __forceinline__ double noise_scale(const Ctx<G>& c, double dt, double s) {
+  if (c.expo) return s;
   if (c.dense) return sqrt(fabs(dt)) * s;
   return sqrt(fabs(dt)) * c.a->cfg.base_output_scale * s;
 }
@@ -1253,13 +1522,16 @@ __global__ void __launch_bounds__(64) solve_kernel(KArgs args) {
   const KArgs& a = args;
   const Layout& L = a.lay;
   const int nc = L.nc;
-  // shared (per workgroup) IWP matrices
+  // shared (per workgroup) IWP matrices and the exponential prior's drift rows
   double* sA = lds;
   double* sQ = lds + nc * nc;
+  double* sD = lds + 2 * nc * nc;
   for (int i = threadIdx.x; i < nc * nc; i += blockDim.x) {
     sA[i] = a.A[i];
     sQ[i] = a.Q[i];
   }
+  if (L.expo)
+    for (int i = threadIdx.x; i < L.d * nc; i += blockDim.x) sD[i] = a.cfg.prior_drift[i];
   __syncthreads();
   const int grp = threadIdx.x / G;
   Ctx<G> c;
@@ -1273,10 +1545,13 @@ __global__ void __launch_bounds__(64) solve_kernel(KArgs args) {
   c.N = L.N;
   c.d = L.d;
   c.dense = a.cfg.factorisation == PDQ_DENSE;
-  c.sA = sA;
-  c.sQ = sQ;
   c.ws = lds + L.wg_shared + grp * L.inst_size;
   c.prm = c.ws + L.w_prm;
+  c.expo = L.expo;
+  c.sD = sD;
+  // the exponential prior's transition lives in the instance's workspace
+  c.sA = L.expo ? c.ws + L.w_xA : sA;
+  c.sQ = L.expo ? c.ws + L.w_xL : sQ;
   for (;;) {
     int inst = 0;
     if (c.g.l == 0) inst = atomicAdd(a.queue, 1);
@@ -1373,6 +1648,8 @@ inline void emu_solve_instance(const KArgs& a, int inst, double* shared, double*
     shared[i] = a.A[i];
     shared[nc * nc + i] = a.Q[i];
   }
+  if (L.expo)
+    for (int i = 0; i < L.d * nc; ++i) shared[2 * nc * nc + i] = a.cfg.prior_drift[i];
   Ctx<1> c;
   c.g.l = 0;
   c.a = &a;
@@ -1384,10 +1661,12 @@ inline void emu_solve_instance(const KArgs& a, int inst, double* shared, double*
   c.N = L.N;
   c.d = L.d;
   c.dense = a.cfg.factorisation == PDQ_DENSE;
-  c.sA = shared;
-  c.sQ = shared + nc * nc;
   c.ws = ws;
   c.prm = ws + L.w_prm;
+  c.expo = L.expo;
+  c.sD = shared + 2 * nc * nc;
+  c.sA = L.expo ? ws + L.w_xA : shared;
+  c.sQ = L.expo ? ws + L.w_xL : shared + nc * nc;
   if (a.mode == MODE_LML) lml_instance(c, inst);
   else solve_instance(c, inst);
 }

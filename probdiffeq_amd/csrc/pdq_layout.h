@@ -8,15 +8,19 @@ namespace pdq {
 
 struct Layout {
   int nc, mc, mcp, ko, N, d;
-  int smoother, ts1;
+  int smoother, ts1, expo;
   // one solution slot
   int s_sc, s_m, s_L, s_cA, s_cm, s_cL, s_ctl, s_cto, s_b, s_J, s_size;
   // instance workspace (after the three slots)
   int w_P, w_PI, w_big, w_T1, w_Um, w_UL, w_Pm, w_PL, w_om, w_oL, w_G, w_mo, w_y0;
   int w_bA, w_bm, w_bL, w_btl, w_btl_pad, w_bto;
   int w_eA, w_em, w_eL, w_etl, w_eto, w_prm, w_tri, w_lml;
+  // exponential prior: the transition of the current step (e^A, Gramian
+  // factor) and the scratch of its computation
+  int w_xA, w_xL, w_xa, w_xB, w_xA2, w_xA4, w_xA6, w_xA8, w_xT, w_xV, w_xVmU, w_xP, w_xPB, w_xrhs, w_xtmp,
+      w_xbuf, w_xpiv;
   int inst_size;   // doubles per instance
-  int wg_shared;   // doubles shared by the workgroup (A, Q)
+  int wg_shared;   // doubles shared by the workgroup (A, Q; the exponential prior's drift rows)
 };
 
 #if defined(__HIPCC__)
@@ -37,6 +41,7 @@ PDQ_HD inline Layout make_layout(const pdq_config& cfg, int n_tan) {
   L.ko = dense ? L.d : 1;
   L.smoother = cfg.strategy != PDQ_FILTER;
   L.ts1 = cfg.constraint == PDQ_TS1;
+  L.expo = dense && cfg.prior == PDQ_PRIOR_EXPONENTIAL;
   const int nc = L.nc, mc = L.mc, ko = L.ko;
   int o = 0;
   L.s_sc = o; o += 8;
@@ -83,9 +88,32 @@ PDQ_HD inline Layout make_layout(const pdq_config& cfg, int n_tan) {
   L.w_prm = o; o += cfg.n_params > 0 ? cfg.n_params : 1;
   L.w_tri = o; o += ko * mc;
   L.w_lml = o; o += 4 * (1 + n_tan) + 4 * ko;
+  if (L.expo) {
+    const int nn = nc * nc, rows = 10 * L.d > 2 * nc ? 10 * L.d : 2 * nc;
+    L.w_xA = o; o += nn;
+    L.w_xL = o; o += nn;
+    L.w_xa = o; o += nn;
+    L.w_xB = o; o += nc * L.d;
+    L.w_xA2 = o; o += nn;
+    L.w_xA4 = o; o += nn;
+    L.w_xA6 = o; o += nn;
+    L.w_xA8 = o; o += nn;
+    L.w_xT = o; o += nn;
+    L.w_xV = o; o += nn;
+    L.w_xVmU = o; o += nn;
+    L.w_xP = o; o += nn;
+    L.w_xPB = o; o += nc * L.d;
+    L.w_xrhs = o; o += nc * 10 * L.d;
+    L.w_xtmp = o; o += nc * (nc > 5 * L.d ? nc : 5 * L.d);
+    L.w_xbuf = o; o += rows * nc;
+    L.w_xpiv = o; o += nc;
+  } else {
+    L.w_xA = L.w_xL = L.w_xa = L.w_xB = L.w_xA2 = L.w_xA4 = L.w_xA6 = L.w_xA8 = L.w_xT = L.w_xV = L.w_xVmU =
+        L.w_xP = L.w_xPB = L.w_xrhs = L.w_xtmp = L.w_xbuf = L.w_xpiv = 0;
+  }
   o += 16;  // scalars / padding
   L.inst_size = o;
-  L.wg_shared = 2 * nc * nc;
+  L.wg_shared = 2 * nc * nc + (L.expo ? L.d * nc : 0);
   return L;
 }
 

@@ -41,7 +41,9 @@ __all__ = [
     "jacobian_materialize",
     "jacobian_monte_carlo_fwd",
     "jacobian_monte_carlo_rev",
+    "JetOdeAutonomous",
     "jetexpand_ode_padded_scan",
+    "ode_autonomous_order_arbitrary",
     "linear",
     "logistic",
     "loss_lml_terminal_values",
@@ -159,6 +161,62 @@ class jacobian_monte_carlo_fwd:
 
 class jacobian_monte_carlo_rev(jacobian_monte_carlo_fwd):
     """jacobians.py:195-271 (reverse mode; the reference's default handler of ode())."""
+
+
+class JetOdeAutonomous:
+    """problems.JetOdeAutonomous (problems.py:254-281): an autonomous u^(k) = f(u, u', ...).
+
+    Only the exponential priors take one: its drift is a torch-traceable Python function of the
+    Taylor coefficients, differentiated on the host when the prior is built (it never runs in a
+    kernel)."""
+
+    def __init__(self, autonomous, *, num_tcoeffs_in_args, jacobian=None, tcoeff_indices_output=None):
+        self.autonomous = autonomous
+        self.num_tcoeffs_in_args = num_tcoeffs_in_args
+        self.jacobian = jacobian
+        self.tcoeff_indices_output = [num_tcoeffs_in_args] if tcoeff_indices_output is None else tcoeff_indices_output
+
+
+def ode_autonomous_order_arbitrary(func, /, *, num_tcoeffs_in_args, jacobian=None):
+    """problems.ode_autonomous_order_arbitrary (problems.py:385-401): func(u, du, ..., /) -> f."""
+    return JetOdeAutonomous(lambda *, jet_coords: func(*jet_coords[:num_tcoeffs_in_args]), num_tcoeffs_in_args=num_tcoeffs_in_args,
+                            jacobian=jacobian)
+
+
+def _ioup_ode(linop, N):
+    """prior_ornstein_uhlenbeck_integrated's drift (ssm_impl_api.py:381-410): f = linop(u^(N-1))."""
+    return JetOdeAutonomous(lambda *, jet_coords: [linop(jet_coords[-1])], num_tcoeffs_in_args=N)
+
+
+def _matern_ode(length_scale, N):
+    """prior_matern's drift (ssm_impl_api.py:443-477): f = -sum_i C(D, i) z^(D-i) u^(i) with
+    z = sqrt(2 (D - 1/2)) / length_scale and D = N Taylor coefficients."""
+    from math import comb, sqrt
+
+    def autonomous(*, jet_coords):
+        D = len(jet_coords)
+        z = sqrt(2 * (D - 0.5)) / length_scale
+        return [-sum(comb(D, i) * z ** (D - i) * c for i, c in enumerate(jet_coords))]
+
+    return JetOdeAutonomous(autonomous, num_tcoeffs_in_args=N)
+
+
+def _linear_drift(ode, tc):
+    """The Jacobian of the flattened f (d,) with respect to the flattened Taylor coefficients (N d,),
+    derivative-major (ssm_impl_dense.py:587-596), at the coefficients tc (N, d)."""
+    N, d = tc.shape
+
+    def vf_flat(x):
+        coeffs = [x[i * d : (i + 1) * d] for i in range(N)]
+        out = ode.autonomous(jet_coords=coeffs)
+        if isinstance(out, (list, tuple)):
+            out = torch.cat([torch.as_tensor(o, dtype=torch.float64).reshape(-1) for o in out])
+        return out.reshape(-1)
+
+    J = torch.autograd.functional.jacobian(vf_flat, tc.reshape(-1).to(torch.float64))
+    if J.shape != (d, N * d):
+        raise ValueError(f"the prior's drift maps (N d,) = ({N * d},) to shape {tuple(J.shape)}, expected ({d}, {N * d})")
+    return J.contiguous()
 
 
 def jetexpand_ode_padded_scan(*, num: int):
@@ -334,13 +392,16 @@ class ProbabilisticSolution:
 # ------------------------------------------------------- state-space models
 @dataclasses.dataclass
 class Prior:
-    """An integrated Wiener process prior (prior_wiener_integrated)."""
+    """An integrated Wiener process prior (prior_wiener_integrated), or (kind "exponential",
+    dense) an exponential-integrator prior whose drift's last d rows are `drift` (d, N d)."""
 
     factorisation: str
     tcoeffs: torch.Tensor  # (B, N, d)
     init_std: float
     base_output_scale: float
     batched: bool
+    kind: str = "iwp"
+    drift: torch.Tensor | None = None
 
     @property
     def num_derivatives(self) -> int:
@@ -395,6 +456,39 @@ class _StateSpaceModel:
                 raise NotImplementedError("non-uniform base output scales are not implemented on the device path")
             base = float(s.flat[0])
         return Prior(self.factorisation, tc.contiguous(), init_std, base, batched)
+
+    def prior_exponential(self, ode, tcoeffs, /, *, is_exact=True, inexact_eps=1e-6, diffuse_derivatives=0, diffuse_eps=1.0, output_scale=None):
+        """ssm_impl_dense.py:516-629: the prior of the linear SDE whose drift is the IWP's shift with
+        the Jacobian of the linear autonomous `ode` (jet_coords -> f, one argument per Taylor
+        coefficient) as its last block row; every transition is exp_gram_cholesky (the device computes
+        it per step).  The Jacobian is taken on the host (FP64 autograd) at the Taylor coefficients and
+        is shared by the batch, so it must be the same for every instance (a linear drift)."""
+        if self.factorisation != "dense":
+            name = {"isotropic": "Isotropic", "blockdiag": "Block-diagonal"}[self.factorisation]
+            msg = f"{name} exponential priors have not been implemented (yet.). If you need them, reach out."
+            raise NotImplementedError(msg)
+        base = self.prior_wiener_integrated(tcoeffs, is_exact=is_exact, inexact_eps=inexact_eps, diffuse_derivatives=diffuse_derivatives,
+                                            diffuse_eps=diffuse_eps, output_scale=output_scale)
+        if ode.num_tcoeffs_in_args != len(tcoeffs):
+            msg = "The exponential prior does not match the Taylor coefficients in the SSM."
+            msg += f" The passed ODE has order {ode.num_tcoeffs_in_args}, whereas the state-space model"
+            msg += f" includes {len(tcoeffs)} Taylor coefficients."
+            raise TypeError(msg)
+        tc = base.tcoeffs.detach().to("cpu")
+        drifts = [_linear_drift(ode, tc[i]) for i in sorted({0, tc.shape[0] - 1})]
+        if not torch.equal(drifts[0], drifts[-1]):
+            raise ValueError("the exponential prior's drift must be linear (the same Jacobian for every instance)")
+        return dataclasses.replace(base, kind="exponential", drift=drifts[0].to(base.tcoeffs.device).contiguous())
+
+    def prior_ornstein_uhlenbeck_integrated(self, linop, tcoeffs, /, **kwargs):
+        """ssm_impl_api.py:381-410: the exponential prior with f(u, ..., u^(nu)) = linop(u^(nu))."""
+        N = len(tcoeffs) + kwargs.get("diffuse_derivatives", 0)
+        return self.prior_exponential(_ioup_ode(linop, N), tcoeffs, **kwargs)
+
+    def prior_matern(self, length_scale, tcoeffs, /, **kwargs):
+        """ssm_impl_api.py:443-477: the exponential prior of the Matern drift (_matern_ode)."""
+        N = len(tcoeffs) + kwargs.get("diffuse_derivatives", 0)
+        return self.prior_exponential(_matern_ode(length_scale, N), tcoeffs, **kwargs)
 
     def constraint_ode_ts0(self, ode, /):
         """ssm_impl_api.py:520-534."""
@@ -623,6 +717,8 @@ def make_config(prior: Prior, solver_obj: _Solver, error=None, control=None, cli
         base_output_scale=prior.base_output_scale,
         init_std=prior.init_std,
     )
+    if prior.kind == "exponential":
+        cfg.prior, cfg.prior_drift = "exponential", prior.drift
     if isinstance(solver_obj, solver_mle):
         cfg.correct_asymptotic_underconfidence = solver_obj.correct_asymptotic_underconfidence
     if isinstance(solver_obj, solver_dynamic):

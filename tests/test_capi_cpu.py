@@ -27,10 +27,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_matches_header_layout():
-    """The ctypes mirror of pdq_config has the C layout (12 ints, 12 doubles, 7 ints, padded to 8)."""
+    """The ctypes mirror of pdq_config has the C layout (12 ints, 12 doubles, 8 ints, a pointer)."""
     from probdiffeq_amd import capi
 
-    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 7 * 4 + 4
+    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 8 * 4 + 8
     assert ctypes.sizeof(capi._Solution) == 17 * 8
 
 
@@ -137,3 +137,27 @@ def test_lml_terminal_values_checks_the_std_shape_like_the_reference():
     loss = pdq.loss_lml_terminal_values()
     with pytest.raises(ValueError, match="standard deviation container"):
         loss(torch.zeros(2, dtype=torch.float64), marginals=marg, std=torch.ones(2, dtype=torch.float64))
+
+
+def test_exponential_prior_drifts_match_the_reference_closed_forms():
+    """The mirror's drift matrices (the Jacobian of the prior's linear ODE, ssm_impl_dense.py:587-596):
+    Matern (test_matern.py:7-27: f = -l^3 u - 3 l^2 du - 3 l ddu, l = sqrt(5) / length_scale),
+    IOUP (test_ornstein_uhlenbeck_integrated.py: f = M u^(nu)) and an arbitrary-order linear ODE."""
+    import numpy as np
+    import torch
+
+    from probdiffeq_amd import probdiffeq as pdq
+
+    tc = torch.ones(3, 4, dtype=torch.float64)
+    ls = 0.567
+    ell = np.sqrt(2 * 2.5) / ls
+    expected = np.concatenate([-(ell**3) * np.eye(4), -3 * ell**2 * np.eye(4), -3 * ell * np.eye(4)], axis=1)
+    assert np.allclose(pdq._linear_drift(pdq._matern_ode(ls, 3), tc).numpy(), expected, rtol=1e-15, atol=0)
+    M = torch.as_tensor(np.random.default_rng(1).normal(size=(4, 4)))
+    J = pdq._linear_drift(pdq._ioup_ode(lambda x: M @ x, 3), tc).numpy()
+    assert np.array_equal(J[:, :8], np.zeros((4, 8))) and np.allclose(J[:, 8:], M.numpy(), rtol=1e-15, atol=0)
+    ode = pdq.ode_autonomous_order_arbitrary(lambda u, du, ddu, /: M @ ddu - 2.0 * u, num_tcoeffs_in_args=3)
+    J = pdq._linear_drift(ode, tc).numpy()
+    assert np.allclose(J, np.concatenate([-2.0 * np.eye(4), np.zeros((4, 4)), M.numpy()], axis=1), rtol=1e-15, atol=0)
+    with pytest.raises(NotImplementedError, match="exponential priors"):
+        pdq.state_space_model_isotropic().prior_exponential(ode, [tc[0]] * 3)

@@ -117,3 +117,40 @@ def test_unsuitable_strategy_warns_and_failed_status_raises(api):
     save_at = torch.linspace(0.0, 5.0, 3, dtype=torch.float64, device="cuda")
     with pytest.raises(FloatingPointError):
         solve(init, save_at=save_at, atol=1e-300, rtol=1e-300, max_attempts=3)
+
+
+def test_exponential_priors_through_the_api(oracle, api):
+    """prior_matern / prior_ornstein_uhlenbeck_integrated / prior_exponential (ssm_impl_dense.py:516-629,
+    ssm_impl_api.py:381-510): the reference's equivalences (test_matern.py, test_ornstein_uhlenbeck_integrated.py)
+    hold for the drifts the API builds, the order check raises TypeError (test_exponential.py:33-55), and a
+    Matern-prior solve matches the oracle and DOP853."""
+    pdx, ivpsolve = api
+    u0 = torch.tensor([20.0, 20.0], dtype=torch.float64, device="cuda")
+    vf = pdx.lotka_volterra(*T.THETA)
+    tcoeffs, _ = pdx.jetexpand_ode_padded_scan(num=4)(vf, [u0], t=0.0)
+    ssm = pdx.state_space_model_dense()
+    ls = 8.0
+    matern = ssm.prior_matern(ls, tcoeffs)
+    ell = np.sqrt(2 * 4.5) / ls
+    from math import comb
+
+    expo = ssm.prior_exponential(
+        pdx.ode_autonomous_order_arbitrary(lambda *c: -sum(comb(5, i) * ell ** (5 - i) * x for i, x in enumerate(c)), num_tcoeffs_in_args=5), tcoeffs
+    )
+    assert torch.allclose(matern.drift, expo.drift, rtol=1e-14, atol=0)
+    M = torch.tensor([[-0.3, 0.1], [0.05, -0.2]], dtype=torch.float64)
+    ioup = ssm.prior_ornstein_uhlenbeck_integrated(lambda x: M @ x, tcoeffs)
+    assert torch.equal(ioup.drift[:, 8:].cpu(), M) and not ioup.drift[:, :8].any()
+    with pytest.raises(TypeError, match="Taylor coefficients"):
+        ssm.prior_exponential(pdx.ode_autonomous_order_arbitrary(lambda u, du, /: du, num_tcoeffs_in_args=2), tcoeffs)
+    constraint = ssm.constraint_ode_ts0(vf)
+    slv = pdx.solver_dynamic(strategy=pdx.strategy_filter(), constraint=constraint)
+    save_at = torch.linspace(0.0, 20.0, 50, dtype=torch.float64, device="cuda")
+    sol = ivpsolve.solve_adaptive_save_at(solver=slv, error=pdx.error_residual_std(constraint=constraint))(matern, save_at=save_at, atol=1e-8, rtol=1e-6)
+    u = sol.u.mean[0].cpu().numpy()
+    truth = _dop853(T.THETA, [20.0, 20.0], save_at.cpu().numpy())
+    assert np.max(np.abs(u - truth) / np.abs(truth)) < 1e-4
+    tc = oracle.taylor_coefficients("lotka_volterra", T.THETA, [20.0, 20.0], 0.0, 4)
+    r = oracle.solve_save_at(oracle.Config(prior="exponential", prior_drift=matern.drift.cpu().numpy()), T.THETA, tc, save_at.cpu().numpy())
+    assert int(sol.num_steps[-1]) == r["accepted"]
+    assert np.max(np.abs(u - r["mean"].reshape(50, -1)[:, :2])) <= 1e-9 * np.max(np.abs(u))

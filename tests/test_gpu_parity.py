@@ -109,6 +109,8 @@ def _oracle_cfg(oracle, cfg):
         error_re_linearize=cfg.error_re_linearize,
         error_derivative_idx=cfg.error_derivative_idx,
         error_per_unit_step=cfg.error_per_unit_step,
+        prior=cfg.prior,
+        prior_drift=None if cfg.prior_drift is None else np.asarray(cfg.prior_drift.cpu() if hasattr(cfg.prior_drift, "cpu") else cfg.prior_drift),
     )
 
 
@@ -382,6 +384,62 @@ def test_isotropic_ts1_is_refused_off_the_lane_kernel(pdq):
     cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, strategy="fixedpoint", constraint="ts1")
     grid = np.linspace(0.0, 2.0, 11)
     with pytest.raises(pdq.PdqError, match="ts0"):
+        pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0s), _dev(grid), 5, _dev(np.ones((1, 3, 2))), _dev(np.ones((1, 3))), grad=False)
+
+
+def _matern_drift(length_scale, N, d):
+    """prior_matern's drift (ssm_impl_api.py:443-477), pinned in tests/test_oracle_prior.py."""
+    from math import comb
+
+    z = np.sqrt(2 * (N - 0.5)) / length_scale
+    return np.concatenate([-comb(N, i) * z ** (N - i) * np.eye(d) for i in range(N)], axis=1)
+
+
+@pytest.mark.parametrize(
+    "nu,prior,strategy,solver,lanes",
+    [
+        (3, "zero", "filter", "dynamic", 0),
+        (4, "matern", "filter", "dynamic", 0),
+        (3, "ioup", "fixedpoint", "dynamic", 0),
+        (3, "matern", "fixedinterval", "solver", 16),
+        (4, "ioup", "filter", "mle", 64),
+        (7, "matern", "filter", "dynamic", 0),
+    ],
+)
+def test_exponential_prior(oracle, pdq, nu, prior, strategy, solver, lanes):
+    """The dense exponential prior (ssm_impl_dense.py:392-629): every transition is
+    exp_gram_cholesky(pade_and_legendre_9), computed per step on the device, against the oracle
+    pinned by the matrix-fraction baseline and the zero-drift == IWP equivalence."""
+    d = 2
+    N, n = nu + 1, (nu + 1) * d
+    if prior == "zero":
+        drift = np.zeros((d, n))
+    elif prior == "matern":
+        drift = _matern_drift(8.0, N, d)
+    else:
+        drift = np.zeros((d, n))
+        drift[:, nu * d :] = np.array([[-0.3, 0.1], [0.05, -0.2]])
+    cfg = pdq.Config(factorisation="dense", num_derivatives=nu, strategy=strategy, solver=solver, rtol=1e-5, atol=1e-7,
+                     lanes_per_instance=lanes, prior="exponential", prior_drift=_dev(drift))
+    params, y0s, times = _lv_batch(6, seed=45, t_lo=4.0, t_hi=8.0, K=10)
+    _compare(oracle, pdq, cfg, params, y0s, times)
+
+
+def test_exponential_prior_refusals(pdq):
+    """Dense only (as the reference), (nu + 1) d <= 16, a drift pointer, and no LML."""
+    params, y0s, times = _lv_batch(1, seed=1, K=5)
+    tc = pdq.taylor_coefficients("lotka_volterra", _dev(params), _dev(y0s), 0.0, 3)
+    drift = _dev(np.zeros((2, 8)))
+    for kw, match in [
+        (dict(factorisation="isotropic", prior_drift=drift), "dense factorisation"),
+        (dict(factorisation="dense"), "prior_drift"),
+    ]:
+        cfg = pdq.Config(num_derivatives=3, prior="exponential", **kw)
+        with pytest.raises(pdq.PdqError, match=match):
+            pdq.solve(cfg, _dev(params), tc, _dev(times))
+    cfg = pdq.Config(factorisation="dense", num_derivatives=3, strategy="fixedpoint", prior="exponential", prior_drift=drift)
+    grid = np.linspace(0.0, 2.0, 11)
+    with pytest.raises(pdq.PdqError, match="Wiener"):
         pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0s), _dev(grid), 5, _dev(np.ones((1, 3, 2))), _dev(np.ones((1, 3))), grad=False)
 
 

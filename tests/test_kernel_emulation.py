@@ -358,3 +358,41 @@ def test_lane_kernel_isotropic_ts1(oracle, emu, strategy, solver, vf, kw):
         cfg = capi.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver=solver, constraint="ts1",
                           rtol=1e-5, atol=1e-7, **kw)
     _run(oracle, emu, cfg, p, y, t, kernel="lane")
+
+
+def _matern_drift(length_scale, N, d):
+    """prior_matern's drift (ssm_impl_api.py:443-477), pinned in tests/test_oracle_prior.py."""
+    from math import comb
+
+    z = np.sqrt(2 * (N - 0.5)) / length_scale
+    return np.concatenate([-comb(N, i) * z ** (N - i) * np.eye(d) for i in range(N)], axis=1)
+
+
+@pytest.mark.parametrize(
+    "prior,strategy,solver",
+    [
+        ("zero", "filter", "dynamic"),
+        ("matern", "filter", "dynamic"),
+        ("ioup", "fixedpoint", "dynamic"),
+        ("matern", "fixedinterval", "solver"),
+        ("ioup", "filter", "mle"),
+    ],
+)
+def test_coop_kernel_exponential_prior(oracle, emu, prior, strategy, solver):
+    """The exponential prior (ssm_impl_dense.py:392-629, exp_gram_cholesky) in the cooperative
+    kernel's code, against the oracle pinned in tests/test_oracle_prior.py."""
+    from probdiffeq_amd import capi
+
+    nu, d = 3, 2
+    N, n = nu + 1, (nu + 1) * 2
+    if prior == "zero":
+        drift = np.zeros((d, n))
+    elif prior == "matern":
+        drift = _matern_drift(8.0, N, d)
+    else:
+        drift = np.zeros((d, n))
+        drift[:, nu * d :] = np.array([[-0.3, 0.1], [0.05, -0.2]])
+    cfg = capi.Config(factorisation="dense", num_derivatives=nu, strategy=strategy, solver=solver, rtol=1e-5, atol=1e-7,
+                      prior="exponential", prior_drift=drift)
+    p, y, t = T._lv_batch(3, seed=8, t_lo=4.0, t_hi=8.0, K=10)
+    _run(oracle, emu, cfg, p, y, t)
