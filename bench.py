@@ -283,6 +283,63 @@ def bench_sample(args, torch, capi, params, y0, save_at, n_samples=16):
     }
 
 
+def _time_solves(args, torch, capi, cfg, P, tc, T):
+    """Mean ms per batched solve (HIP events around the C ABI call) and the attempt total."""
+    out = capi.alloc_solution(cfg, P.shape[0], T.shape[1], P.device)
+    for _ in range(max(1, args.warmup)):
+        capi.solve(cfg, P, tc, T, out=out)
+    torch.cuda.synchronize()
+    st = out["status"].cpu().numpy()
+    if np.any(st != 0):
+        raise SystemExit(f"{cfg.factorisation}/{cfg.constraint}/{cfg.prior}: status != 0 for {(st != 0).sum()} instances")
+    e0, e1 = _events()
+    reps = max(1, args.steps)
+    e0.record()
+    for _ in range(reps):
+        capi.solve(cfg, P, tc, T, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"ms_per_batch": ms, "solves_per_s": P.shape[0] / (ms / 1e3), "attempts_total": int(out["attempts"].sum())}
+
+
+def matern_drift(length_scale, N, d):
+    """prior_matern's drift (ssm_impl_api.py:443-477): -C(N, i) z^(N-i) I per coefficient i."""
+    from math import comb
+
+    z = np.sqrt(2 * (N - 0.5)) / length_scale
+    return np.concatenate([-comb(N, i) * z ** (N - i) * np.eye(d) for i in range(N)], axis=1)
+
+
+def bench_variants(args, torch, capi, params, y0, save_at):
+    """Throughput of the configurations beside the headline one, on the same kind of inputs:
+    the C2 batch with isotropic ts1 (exact trace), 1024 dense Lotka-Volterra solves (the C1
+    configuration, batched) with the IWP and with a Matern prior (per-step exp_gram_cholesky),
+    and 1024 stiff van der Pol solves (dense nu = 5, ts1)."""
+    dev = torch.device("cuda")
+    P = torch.tensor(params, dtype=torch.float64, device=dev)
+    Y0 = torch.tensor(y0, dtype=torch.float64, device=dev)
+    T = torch.tensor(save_at, dtype=torch.float64, device=dev)
+    res = {}
+    tc4 = capi.taylor_coefficients("lotka_volterra", P, Y0, 0.0, 4)
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=4, ode_dim=2, strategy="fixedpoint", solver="dynamic", rtol=1e-6, atol=1e-8, constraint="ts1")
+    res["c2_isotropic_ts1"] = _time_solves(args, torch, capi, cfg, P, tc4, T)
+    n = min(1024, P.shape[0])
+    Pd, Td, tcd = P[:n].contiguous(), T[:n].contiguous(), tc4[:n].contiguous()
+    base = dict(factorisation="dense", num_derivatives=4, ode_dim=2, strategy="filter", solver="dynamic", rtol=1e-6, atol=1e-8)
+    res["dense_lv_1024_iwp"] = _time_solves(args, torch, capi, capi.Config(**base), Pd, tcd, Td)
+    drift = torch.tensor(matern_drift(10.0, 5, 2), dtype=torch.float64, device=dev)
+    res["dense_lv_1024_matern"] = _time_solves(args, torch, capi, capi.Config(**base, prior="exponential", prior_drift=drift), Pd, tcd, Td)
+    rng = np.random.default_rng(0)
+    mu = torch.tensor(np.exp(rng.uniform(np.log(1.0), np.log(100.0), size=(1024, 1))), dtype=torch.float64, device=dev)
+    yv = torch.tensor(np.tile([2.0, 0.0], (1024, 1)), dtype=torch.float64, device=dev)
+    tv = torch.tensor(np.tile([0.0, 6.3], (1024, 1)), dtype=torch.float64, device=dev)
+    cfg3 = capi.Config(factorisation="dense", num_derivatives=5, ode_dim=2, constraint="ts1", vector_field="van_der_pol", n_params=1, strategy="filter",
+                       solver="dynamic", rtol=1e-5, atol=1e-7)
+    res["c3_vdp_1024_dense_ts1"] = _time_solves(args, torch, capi, cfg3, mu, capi.taylor_coefficients("van_der_pol", mu, yv, 0.0, 5), tv)
+    return {"variants": res}
+
+
 def cpu_baseline(params, y0, save_at, n_sample, threads):
     from oracle import oracle
 
@@ -320,6 +377,7 @@ def main():
     ap.add_argument("--skip-lml", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-offgrid", action="store_true", help="skip the off-grid marginal and posterior-sample figures")
+    ap.add_argument("--skip-variants", action="store_true", help="skip the ts1, exponential-prior and van der Pol figures")
     ap.add_argument("--copy-calibration", action="store_true", help="only run a 1 GiB device-to-device copy (scales the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -348,6 +406,11 @@ def main():
     if smp:
         _progress(f"posterior samples done: {smp['posterior_samples_per_s']:.3e} per s")
     og.update(smp)
+    var = {} if args.skip_variants else bench_variants(args, torch, capi, params, y0, save_at)
+    if var:
+        v = var["variants"]
+        _progress("variants done: " + ", ".join(f"{k} {x['ms_per_batch']:.1f} ms" for k, x in v.items()))
+    og.update(var)
     threads = min(16, os.cpu_count() or 1)
     cpu = None if args.skip_cpu else cpu_baseline(params, y0, save_at, args.cpu_sample, threads)
     if cpu:
