@@ -109,6 +109,11 @@ typedef struct pdq_config {
    * columns), shared by the batch. */
   int32_t prior;
   const double* prior_drift;
+  /* Optional initial step per instance (device, n_inst doubles), as a vmap of
+   * the reference's solve over per-instance dt0 (ivpsolve.dt0 /
+   * dt0_adaptive, stepsize_initialisers.py:7-67); NULL: dt0 for every
+   * instance. */
+  const double* dt0_per_instance;
 } pdq_config;
 
 /* Caller-allocated device outputs; NULL members are not written.

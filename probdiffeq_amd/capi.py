@@ -71,6 +71,7 @@ class _Config(ctypes.Structure):
         ("error_per_unit_step", ctypes.c_int32),
         ("prior", ctypes.c_int32),
         ("prior_drift", ctypes.c_void_p),
+        ("dt0_per_instance", ctypes.c_void_p),
     ]
 
 
@@ -141,6 +142,9 @@ class Config:
     # float64 (a device tensor for the library; host memory for the CPU build of the kernels)
     prior: str = "iwp"
     prior_drift: Any = None
+    # optional initial step per instance: an (n_inst,) float64 device tensor (host array for the
+    # CPU build of the kernels); None: dt0 for every instance
+    dt0_per_instance: Any = None
 
     def c(self) -> _Config:
         c = self._c()
@@ -157,6 +161,16 @@ class Config:
                     raise ValueError("prior_drift must be a contiguous float64 tensor")
                 c.prior_drift = drift.data_ptr()
             c._keep = drift  # the memory lives as long as the struct
+        if self.dt0_per_instance is not None:
+            d0 = self.dt0_per_instance
+            if isinstance(d0, np.ndarray):
+                d0 = np.ascontiguousarray(d0, dtype=np.float64)
+                c.dt0_per_instance = d0.ctypes.data
+            else:
+                if d0.dtype != torch.float64 or not d0.is_contiguous():
+                    raise ValueError("dt0_per_instance must be a contiguous float64 tensor")
+                c.dt0_per_instance = d0.data_ptr()
+            c._keep_dt0 = d0
         return c
 
     def _c(self) -> _Config:

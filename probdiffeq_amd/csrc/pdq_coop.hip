@@ -1454,7 +1454,7 @@ __device__ void solve_instance(const Ctx<G>& c, int inst) {
       pr = tmp;
     }
   } else {
-    double dt = cfg.dt0, cstate = 1.0;
+    double dt = cfg.dt0_per_instance ? cfg.dt0_per_instance[inst] : cfg.dt0, cstate = 1.0;
     for (int k = 1; k < K && status == PDQ_STATUS_OK; ++k) {
       const double t1 = ts[k];
       bool cont = true;

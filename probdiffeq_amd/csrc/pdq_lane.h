@@ -1200,7 +1200,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
     // execute it together instead of one after the other.
     enum { OP_STEP = 0, OP_BEYOND1 = 1, OP_BEYOND2 = 2, OP_AT = 3 };
     const bool smoother = cfg.strategy != PDQ_FILTER;
-    double dt = cfg.dt0, cstate = 1.0;
+    double dt = cfg.dt0_per_instance ? cfg.dt0_per_instance[c.inst] : cfg.dt0, cstate = 1.0;
     int k = 1, att = 0;
     bool second_half = false;
     while (k < K && status == PDQ_STATUS_OK) {

@@ -22,6 +22,8 @@ from probdiffeq_amd import probdiffeq as pdq
 __all__ = [
     "control_integral",
     "control_proportional_integral",
+    "dt0",
+    "dt0_adaptive",
     "solve_adaptive_save_at",
     "solve_adaptive_terminal_values",
     "solve_fixed_grid",
@@ -70,6 +72,72 @@ def _raise_on_status(sol_status: torch.Tensor) -> None:
         raise FloatingPointError(msg)
 
 
+def _set_dt0(cfg, dt0, batch, device):
+    """A scalar dt0, or one per instance (what vmap over the reference's solve computes with the
+    per-instance proposals of dt0 / dt0_adaptive)."""
+    if isinstance(dt0, torch.Tensor) and dt0.numel() > 1:
+        if dt0.numel() != batch:
+            raise ValueError(f"dt0 has {dt0.numel()} entries for a batch of {batch}")
+        cfg.dt0 = float(dt0.reshape(-1)[0])
+        cfg.dt0_per_instance = dt0.reshape(-1).to(device=device, dtype=torch.float64).contiguous()
+    else:
+        cfg.dt0 = float(dt0)
+
+
+_TIME_DEPENDENT = {"linear"}
+
+
+def _vf_values(vf, y, t):
+    """f(y, t) of a registered vector field, through the device's Taylor-coefficient entry point
+    (its coefficient 1 is f(y0, t0)); y (B, d) -> (B, d)."""
+    B = y.shape[0]
+    params = vf.params_batched(B).to(y.device)
+    return capi.taylor_coefficients(vf.name, params, y.contiguous(), float(t), 1)[:, 1, :]
+
+
+def _initial_values(vf, initial_values):
+    if getattr(vf, "is_jet_lifted", False):
+        raise ValueError
+    u0 = pdq._as_f64(initial_values[0])
+    batched = u0.dim() == 2
+    return (u0 if batched else u0[None]), batched
+
+
+def dt0(vf, initial_values, /, scale=0.01, nugget=1e-5, t=0.0):
+    """stepsize_initialisers.py:7-21: scale * |u0| / (|f(u0)| + nugget) (2-norms), per instance for
+    batched initial values."""
+    u0, batched = _initial_values(vf, initial_values)
+    f0 = _vf_values(vf, u0, t)
+    out = scale * torch.linalg.vector_norm(u0, dim=-1) / (torch.linalg.vector_norm(f0, dim=-1) + nugget)
+    return out if batched else out[0]
+
+
+def dt0_adaptive(vf, initial_values, /, t0, *, error_contraction_rate, rtol, atol):
+    """stepsize_initialisers.py:24-67 (Hairer, Norsett, Wanner, Sec. II.4), per instance for batched
+    initial values."""
+    if len(initial_values) > 1:
+        raise ValueError
+    y0, batched = _initial_values(vf, initial_values)
+    f0 = _vf_values(vf, y0, t0)
+    scale = atol + torch.abs(y0) * rtol
+    d0, d1 = torch.linalg.vector_norm(y0, dim=-1), torch.linalg.vector_norm(f0, dim=-1)
+    h0 = torch.where((d0 < 1e-5) | (d1 < 1e-5), torch.full_like(d0, 1e-6), 0.01 * d0 / d1)
+    y1 = y0 + h0[:, None] * f0
+    # f at t0 + h0 per instance; of the registry's fields only "linear" depends on t
+    if vf.name in _TIME_DEPENDENT:
+        f1 = torch.stack([_vf_values(vf, y1[i : i + 1], float(t0) + float(h0[i]))[0] for i in range(y0.shape[0])])
+    else:
+        f1 = _vf_values(vf, y1, t0)
+    d2 = torch.linalg.vector_norm((f1 - f0) / scale, dim=-1) / h0
+    h1 = torch.where(
+        (d1 <= 1e-15) & (d2 <= 1e-15),
+        torch.clamp(h0 * 1e-3, min=1e-6),
+        (0.01 / torch.maximum(d1, d2)) ** (1.0 / (error_contraction_rate + 1.0)),
+    )
+    out = torch.minimum(100.0 * h0, h1)
+    return out if batched else out[0]
+
+
 def solve_adaptive_save_at(*, solver, error, control=None, clip_dt=False, while_loop=None, warn=True, lanes_per_instance=0, check_status=True):
     """solvers_via_adaptive_steps.py:46-148."""
     if not solver.is_suitable_for_save_at and warn:
@@ -84,10 +152,11 @@ def solve_adaptive_save_at(*, solver, error, control=None, clip_dt=False, while_
 
     def solve(u, save_at, atol, rtol, dt0=0.1, eps=1e-8, damp=0.0, max_attempts=0):
         cfg = pdq.make_config(u, solver, error, control, clip_dt)
-        cfg.rtol, cfg.atol, cfg.dt0, cfg.eps, cfg.damp = float(rtol), float(atol), float(dt0), float(eps), float(damp)
+        B = u.tcoeffs.shape[0]
+        cfg.rtol, cfg.atol, cfg.eps, cfg.damp = float(rtol), float(atol), float(eps), float(damp)
+        _set_dt0(cfg, dt0, B, u.tcoeffs.device)
         cfg.max_attempts = max_attempts
         cfg.lanes_per_instance = lanes_per_instance
-        B = u.tcoeffs.shape[0]
         params = solver.constraint.ode.params_batched(B)
         times = _times(save_at, B, u.tcoeffs.device)
         out = capi.solve(cfg, params, u.tcoeffs, times)

@@ -154,3 +154,56 @@ def test_exponential_priors_through_the_api(oracle, api):
     r = oracle.solve_save_at(oracle.Config(prior="exponential", prior_drift=matern.drift.cpu().numpy()), T.THETA, tc, save_at.cpu().numpy())
     assert int(sol.num_steps[-1]) == r["accepted"]
     assert np.max(np.abs(u - r["mean"].reshape(50, -1)[:, :2])) <= 1e-9 * np.max(np.abs(u))
+
+
+@pytest.mark.parametrize("fact", ["dense", "isotropic"])
+def test_terminal_values_with_dt0_adaptive(api, fact):
+    """test_solve_adaptive_terminal_values.py:7-52: dt0_adaptive (stepsize_initialisers.py:24-67) and the
+    terminal-values solve on Lotka-Volterra over [0, 2] at rtol = atol = 1e-3, against a tight DOP853
+    solution; dt0 also equals the reference's formula evaluated with numpy."""
+    pdx, ivpsolve = api
+    u0 = torch.tensor([20.0, 20.0], dtype=torch.float64, device="cuda")
+    vf = pdx.lotka_volterra(*T.THETA)
+    tcoeffs, _ = pdx.jetexpand_ode_padded_scan(num=4)(vf, [u0], t=0.0)
+    ssm = {"dense": pdx.state_space_model_dense, "isotropic": pdx.state_space_model_isotropic}[fact]()
+    init = ssm.prior_wiener_integrated(tcoeffs)
+    ts0 = ssm.constraint_ode_ts0(vf)
+    slv = pdx.solver_dynamic(strategy=pdx.strategy_smoother_fixedpoint(), constraint=ts0)
+    dt0 = ivpsolve.dt0_adaptive(vf, [u0], 0.0, error_contraction_rate=5, rtol=1e-3, atol=1e-3)
+    # the reference's formula with numpy
+    f = lambda y: np.array([T.THETA[0] * y[0] - T.THETA[1] * y[0] * y[1], -T.THETA[2] * y[1] + T.THETA[3] * y[0] * y[1]])
+    y0 = np.array([20.0, 20.0])
+    f0 = f(y0)
+    h0 = 0.01 * np.linalg.norm(y0) / np.linalg.norm(f0)
+    d2 = np.linalg.norm((f(y0 + h0 * f0) - f0) / (1e-3 + np.abs(y0) * 1e-3)) / h0
+    expected = min(100 * h0, (0.01 / max(np.linalg.norm(f0), d2)) ** (1 / 6))
+    assert abs(float(dt0) - expected) <= 1e-14 * expected
+    sol = ivpsolve.solve_adaptive_terminal_values(solver=slv, error=pdx.error_residual_std(constraint=ts0))(init, t0=0.0, t1=2.0, dt0=dt0, atol=1e-3, rtol=1e-3)
+    truth = _dop853(T.THETA, [20.0, 20.0], np.array([0.0, 2.0]))[-1]
+    # the reference's testing.allclose in its (float32) CI: atol = sqrt(eps32), rtol = 10 sqrt(eps32)
+    tol = float(np.sqrt(np.finfo(np.float32).eps))
+    assert np.allclose(sol.u.mean[0].cpu().numpy(), truth, rtol=10 * tol, atol=tol)
+    assert float(ivpsolve.dt0(vf, [u0], t=0.0)) == pytest.approx(0.01 * np.linalg.norm(y0) / (np.linalg.norm(f0) + 1e-5), rel=1e-14)
+
+
+@pytest.mark.parametrize("fact", ["dense", "isotropic"])
+def test_per_instance_dt0_equals_single_solves(api, fact):
+    """A batch with one dt0 per instance (the vmap of the reference's solve over per-instance dt0)
+    reproduces each instance solved alone with its dt0, bit for bit (instances share nothing)."""
+    from probdiffeq_amd import capi
+
+    params, y0s, times = T._lv_batch(6, seed=12, t_lo=3.0, t_hi=8.0, K=9)
+    P, Y, S = T._dev(params), T._dev(y0s), T._dev(times)
+    tc = capi.taylor_coefficients("lotka_volterra", P, Y, 0.0, 4)
+    dt0s = T._dev(np.geomspace(1e-3, 0.5, 6))
+    # a fixed group size for the cooperative kernel: its sums' association order depends on it
+    lanes = 16 if fact == "dense" else 0
+    cfg = capi.Config(factorisation=fact, num_derivatives=4, strategy="fixedpoint", solver="dynamic", rtol=1e-5, atol=1e-7, dt0_per_instance=dt0s,
+                      lanes_per_instance=lanes)
+    out = capi.solve(cfg, P, tc, S)
+    for i in range(6):
+        cfg1 = capi.Config(**{**cfg.__dict__, "dt0": float(dt0s[i]), "dt0_per_instance": None})
+        o1 = capi.solve(cfg1, P[i : i + 1].contiguous(), tc[i : i + 1].contiguous(), S[i : i + 1].contiguous())
+        for k in ("t", "mean", "chol", "attempts", "num_steps"):
+            assert torch.equal(out[k][i], o1[k][0]), (fact, i, k)
+    assert len(set(out["attempts"].tolist())) > 1

@@ -396,3 +396,26 @@ def test_coop_kernel_exponential_prior(oracle, emu, prior, strategy, solver):
                       prior="exponential", prior_drift=drift)
     p, y, t = T._lv_batch(3, seed=8, t_lo=4.0, t_hi=8.0, K=10)
     _run(oracle, emu, cfg, p, y, t)
+
+
+@pytest.mark.parametrize("fact,kernel", [("isotropic", "lane"), ("dense", "coop")])
+def test_per_instance_dt0(oracle, emu, fact, kernel):
+    """dt0_per_instance: each instance starts from its own dt0 (a vmap of the reference's solve over
+    per-instance dt0), exactly as the oracle run with that instance's dt0."""
+    from probdiffeq_amd import capi
+
+    p, y, t = T._lv_batch(3, seed=9, t_lo=3.0, t_hi=6.0, K=6)
+    dt0s = np.array([0.01, 0.3, 0.07])
+    cfg = capi.Config(factorisation=fact, num_derivatives=4, strategy="fixedpoint", solver="dynamic", rtol=1e-5, atol=1e-7, dt0=0.5,
+                      dt0_per_instance=dt0s)
+    tcs = np.stack([oracle.taylor_coefficients("lotka_volterra", p[i], y[i], 0.0, 4) for i in range(3)])
+    g = emu.solve(cfg, p, tcs, t, kernel=kernel)
+    for i in range(3):
+        cfg_i = capi.Config(**{**cfg.__dict__, "dt0": float(dt0s[i]), "dt0_per_instance": None})
+        r, rf, rt = T._oracle_solve(oracle, T._oracle_cfg(oracle, cfg_i), p[i], tcs[i], t[i], False)
+        assert g["attempts"][i] == r["attempts"]
+        assert np.array_equal(g["num_steps"][i], r["num_steps"])
+        got = {k: g[k][i] for k in ("mean", "chol", "output_scale", "filt_mean", "filt_chol", "cond_A", "cond_mean", "term_mean") if k in g}
+        T._check_fields(cfg_i, T._fields(cfg_i, got), r, rf, rt, f"per-instance dt0 {kernel} instance {i}")
+    r0 = oracle.solve_save_at(T._oracle_cfg(oracle, capi.Config(**{**cfg.__dict__, "dt0_per_instance": None})), p[1], tcs[1], t[1])
+    assert r0["attempts"] != g["attempts"][1] or not np.allclose(r0["mean"], g["mean"][1], rtol=1e-12)
