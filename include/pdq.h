@@ -148,6 +148,12 @@ int32_t pdq_init(void);
 /* Copies the last error message of the calling thread into buf. */
 size_t pdq_last_error(char* buf, size_t size);
 
+/* utilities.system_matrices_1d_iwp (utilities.py:57-71): the latent IWP
+ * transition A (flipped Pascal) and the lower Cholesky factor Q of the
+ * flipped Hilbert matrix, (nu+1) x (nu+1) row-major each, into host memory.
+ * Host only: needs no device. */
+int32_t pdq_system_matrices_1d_iwp(int32_t num_derivatives, double* A, double* Q);
+
 /* Replaces ivpsolve.solve_adaptive_save_at(solver=, error=, control=,
  * clip_dt=)(prior, save_at, atol, rtol, dt0, eps, damp)
  * (probdiffeq/_ivpsolve/solvers_via_adaptive_steps.py:46-148), batched over

@@ -232,6 +232,7 @@ class Config:
 # Symbols the header declares; the CPU test suite checks that all exist.
 EXPORTED = [
     "pdq_version",
+    "pdq_system_matrices_1d_iwp",
     "pdq_init",
     "pdq_last_error",
     "pdq_solve_adaptive_save_at",
@@ -410,6 +411,14 @@ def lml_terminal_values(cfg: Config, mean, chol, u, std, tcoeff_index=0, stream=
     c = cfg.c()
     check(lib().pdq_lml_terminal_values(ctypes.byref(c), B, _ptr(mean), _ptr(chol), _ptr(u), _ptr(std), int(tcoeff_index), _ptr(value), _stream(stream)))
     return value
+
+
+def system_matrices_1d_iwp(num_derivatives: int):
+    """The library's latent IWP matrices (A, Q), (nu+1) x (nu+1) numpy arrays (host only)."""
+    n = num_derivatives + 1
+    A, Q = np.zeros((n, n)), np.zeros((n, n))
+    check(lib().pdq_system_matrices_1d_iwp(num_derivatives, A.ctypes.data_as(ctypes.c_void_p), Q.ctypes.data_as(ctypes.c_void_p)))
+    return A, Q
 
 
 def taylor_coefficients(vector_field: str, params, y0, t0: float, num: int, stream=None):

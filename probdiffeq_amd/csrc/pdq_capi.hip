@@ -296,6 +296,17 @@ extern "C" {
 
 int32_t pdq_version(void) { return 1; }
 
+int32_t pdq_system_matrices_1d_iwp(int32_t num_derivatives, double* A, double* Q) {
+  if (num_derivatives < 0 || num_derivatives > 7)
+    return set_err(PDQ_ERR_UNSUPPORTED, "num_derivatives must be in [0, 7] (got %d)", num_derivatives);
+  if (!A || !Q) return set_err(PDQ_ERR_INVALID, "A and Q must be host pointers");
+  std::vector<double> a, q;
+  pdq::iwp_matrices(num_derivatives, a, q);
+  std::memcpy(A, a.data(), a.size() * sizeof(double));
+  std::memcpy(Q, q.data(), q.size() * sizeof(double));
+  return PDQ_OK;
+}
+
 int32_t pdq_init(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");

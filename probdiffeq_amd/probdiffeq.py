@@ -42,7 +42,13 @@ __all__ = [
     "jacobian_monte_carlo_fwd",
     "jacobian_monte_carlo_rev",
     "JetOdeAutonomous",
+    "jetexpand_ode_doubling_unroll",
     "jetexpand_ode_padded_scan",
+    "jetexpand_ode_unroll",
+    "jetexpand_ode_via_jvp",
+    "preconditioner_taylor",
+    "system_matrices_1d_iwp",
+    "taylor_point_prior",
     "ode_autonomous_order_arbitrary",
     "linear",
     "logistic",
@@ -238,6 +244,54 @@ def jetexpand_ode_padded_scan(*, num: int):
         return coeffs, {}
 
     return expand
+
+
+def jetexpand_ode_unroll(*, num: int):
+    """jet_expansion_algorithms.py:110-152.  The same Taylor coefficients as
+    jetexpand_ode_padded_scan; unrolling versus scanning is a JAX compilation choice, and the device
+    runs the same recursion."""
+    return jetexpand_ode_padded_scan(num=num)
+
+
+def jetexpand_ode_via_jvp(*, num: int):
+    """jet_expansion_algorithms.py:180-212 (recursive forward mode): the same coefficients, by the
+    device's Taylor-mode recursion."""
+    return jetexpand_ode_padded_scan(num=num)
+
+
+def jetexpand_ode_doubling_unroll(*, num_doublings: int):
+    """jet_expansion_algorithms.py:228-262 (Newton's doubling): num_doublings doublings give
+    2^(num_doublings + 1) - 1 Taylor coefficients; the device's recursion computes the same ones."""
+    return jetexpand_ode_padded_scan(num=2 ** (num_doublings + 1) - 2)
+
+
+def system_matrices_1d_iwp(num_derivatives: int):
+    """utilities.py:57-71: the latent IWP transition A and noise factor Q ((nu+1) x (nu+1)), as the
+    library computes them (host only)."""
+    A, Q = capi.system_matrices_1d_iwp(num_derivatives)
+    return torch.as_tensor(A), torch.as_tensor(Q)
+
+
+def preconditioner_taylor(num_derivatives: int):
+    """utilities.py:74-84: dt -> (p, p^-1) with p_i = dt^(nu-i) / (nu-i)!."""
+    from math import factorial
+
+    powers = torch.arange(num_derivatives, -1, -1, dtype=torch.float64)
+    scales = torch.tensor([float(factorial(int(k))) for k in powers], dtype=torch.float64)
+
+    def precon(dt):
+        dt = torch.as_tensor(dt, dtype=torch.float64)
+        return dt**powers / scales, dt ** (-powers) * scales
+
+    return precon
+
+
+class taylor_point_prior:
+    """taylor_points.py:150-156: linearise ts1 at the prior (predicted) mean, the device path's
+    (and the reference's) default Taylor point."""
+
+    def __repr__(self) -> str:
+        return f"{self.__class__.__name__}()"
 
 
 # ------------------------------------------------------------ distributions
@@ -505,8 +559,8 @@ class _StateSpaceModel:
         it from JAX random probes, which the device path does not reproduce."""
         if not isinstance(ode, JetOde):
             raise TypeError(ode)
-        if taylor_point is not None:
-            raise NotImplementedError("only the prior-mean Taylor point is implemented")
+        if taylor_point is not None and not isinstance(taylor_point, taylor_point_prior):
+            raise NotImplementedError("only the prior-mean Taylor point (taylor_point_prior) is implemented")
         if self.factorisation == "blockdiag":
             msg = "ts1 for blockdiag models (Jacobian diagonals, jacobians.py:108-113, 171-193) is not implemented on the device path"
             raise NotImplementedError(msg)

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _header_symbols():
     text = open(os.path.join(ROOT, "include", "pdq.h")).read()
-    return sorted(set(re.findall(r"\b(pdq_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(pdq_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -161,3 +161,22 @@ def test_exponential_prior_drifts_match_the_reference_closed_forms():
     assert np.allclose(J, np.concatenate([-2.0 * np.eye(4), np.zeros((4, 4)), M.numpy()], axis=1), rtol=1e-15, atol=0)
     with pytest.raises(NotImplementedError, match="exponential priors"):
         pdq.state_space_model_isotropic().prior_exponential(ode, [tc[0]] * 3)
+
+
+def test_system_matrices_and_preconditioner_match_the_oracle(oracle):
+    """utilities.system_matrices_1d_iwp / preconditioner_taylor of the mirror (the library's host
+    matrices; no device) against the oracle, which is pinned to mpmath and the reference's KAT."""
+    import numpy as np
+
+    from probdiffeq_amd import probdiffeq as pdq
+
+    for nu in range(0, 8):
+        A, Q = pdq.system_matrices_1d_iwp(nu)
+        Ao, Qo = oracle.system_matrices(nu)
+        assert np.array_equal(A.numpy(), Ao)
+        assert np.allclose(Q.numpy(), Qo, rtol=0, atol=1e-15 * np.max(np.abs(Qo)))
+    for dt in (1.234, -1.234, 0.01):
+        p, pinv = pdq.preconditioner_taylor(4)(dt)
+        assert np.allclose(p.numpy(), [dt**4 / 24, dt**3 / 6, dt**2 / 2, dt, 1.0], rtol=1e-15, atol=0)
+        assert np.allclose((p * pinv).numpy(), 1.0, rtol=1e-15, atol=0)
+    assert pdq.state_space_model_dense().constraint_ode_ts1(pdq.JetOde("lotka_volterra", None, 2), taylor_point=pdq.taylor_point_prior()).kind == "ts1"

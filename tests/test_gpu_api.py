@@ -207,3 +207,15 @@ def test_per_instance_dt0_equals_single_solves(api, fact):
         for k in ("t", "mean", "chol", "attempts", "num_steps"):
             assert torch.equal(out[k][i], o1[k][0]), (fact, i, k)
     assert len(set(out["attempts"].tolist())) > 1
+
+
+def test_jet_expansion_algorithms_agree(api):
+    """jetexpand_ode_unroll / _via_jvp / _doubling_unroll (jet_expansion_algorithms.py:110-262) give the
+    Taylor coefficients of jetexpand_ode_padded_scan; num_doublings = 2 gives 7 coefficients."""
+    pdx, _ = api
+    u0 = torch.tensor([20.0, 20.0], dtype=torch.float64, device="cuda")
+    vf = pdx.lotka_volterra(*T.THETA)
+    ref, _ = pdx.jetexpand_ode_padded_scan(num=6)(vf, [u0], t=0.0)
+    for alg in (pdx.jetexpand_ode_unroll(num=6), pdx.jetexpand_ode_via_jvp(num=6), pdx.jetexpand_ode_doubling_unroll(num_doublings=2)):
+        tc, _ = alg(vf, [u0], t=0.0)
+        assert len(tc) == 7 and all(torch.equal(a, b) for a, b in zip(tc, ref))
