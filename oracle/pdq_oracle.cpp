@@ -21,6 +21,17 @@
 //       fixed-interval, fixed grid == adaptive grid, truncated save_at)
 //   LML -- brute-force joint Gaussian density on small problems
 //   Taylor init -- sympy derivatives of the vector fields
+//   error estimators (error_state_std, re_linearize_before_error,
+//       error_per_unit_step) -- one-step closed forms, DOP853 accuracy
+//       (tests/test_oracle_error.py)
+//   isotropic ts1 (exact trace) -- dense == isotropic for f = lambda u + t
+//       (tests/test_oracle_solves.py)
+//   exp_gram / exponential prior -- scipy expm matrix fraction, OU closed
+//       form, zero drift == IWP, expm(M dt) block (tests/test_oracle_prior.py)
+//   off-grid marginals, terminal-value LML -- Gaussian prediction / RTS
+//       closed forms, Gaussian log-density (tests/test_oracle_offgrid.py)
+//   posterior samples -- smoothing means and covariances through the
+//       sampler's affine dependence on its draws (tests/test_oracle_sample.py)
 // ===========================================================================
 #include <algorithm>
 #include <cmath>
