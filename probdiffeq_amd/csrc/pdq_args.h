@@ -120,6 +120,24 @@ struct OffgridArgs {
 };
 int launch_offgrid(OffgridArgs& a, int N, int D, hipStream_t st);
 
+// Off-grid marginals on the cooperative kernel's primitives (pdq_coop.hip):
+// the dense factorisation, and isotropic shapes without a lane kernel.  k
+// carries the solve's configuration, layout, grid (times) and transition
+// matrices; the solution's arrays are in pdq_solution's layouts.
+struct OffgridCoopArgs {
+  KArgs k;
+  int n_query;
+  const double* t_query;       // n_inst x n_query
+  const double* mean;          // n_inst x K x mean_size
+  const double* chol;          // n_inst x K x chol_size
+  const double* filt_mean;     // fixed-interval smoother: filtering marginals
+  const double* filt_chol;
+  const double* output_scale;  // n_inst x K
+  double* mean_out;            // n_inst x n_query x mean_size
+  double* chol_out;            // n_inst x n_query x chol_size
+};
+int launch_offgrid_coop(OffgridCoopArgs& a, int G, hipStream_t st);
+
 // Posterior samples of a smoother (pdq_lane_sample.hip), isotropic.
 struct SampleArgs {
   pdq_config cfg;

@@ -478,11 +478,11 @@ int32_t pdq_offgrid_marginals(const pdq_config* cfg, int32_t n_inst, int32_t K, 
                    "offgrid marginals need a filter or a fixed-interval smoother (the fixed-point smoother is not "
                    "suitable, solvers.py:163-164)");
   const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
-  if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(N, d))
-    return set_err(PDQ_ERR_UNSUPPORTED,
-                   "offgrid marginals are implemented for the isotropic factorisation with nu = 3 or 4 and d = 2 "
-                   "(got nu = %d, d = %d)",
-                   N - 1, d);
+  if (cfg->factorisation == PDQ_BLOCKDIAG)
+    return set_err(PDQ_ERR_UNSUPPORTED, "offgrid marginals are implemented for the dense and isotropic factorisations");
+  const bool lane = cfg->factorisation == PDQ_ISOTROPIC && pdq::lane_supported(N, d) && cfg->lanes_per_instance <= 1;
+  pdq::Layout L = pdq::make_layout(*cfg, 0);
+  if (!lane && L.nc > 32) return set_err(PDQ_ERR_UNSUPPORTED, "dense state dimension %d exceeds 32", L.nc);
   if (n_inst == 0 || n_query == 0) return PDQ_OK;
   if (!times || !t_query || !mean_out || !chol_out || !solution->mean || !solution->chol || !solution->output_scale)
     return set_err(PDQ_ERR_INVALID, "times, t_query, the outputs and the solution's mean, chol and output_scale "
@@ -492,6 +492,30 @@ int32_t pdq_offgrid_marginals(const pdq_config* cfg, int32_t n_inst, int32_t K, 
   DevMats m;
   rc = device_mats(*cfg, false, m);
   if (rc) return rc;
+  if (!lane) {
+    // the cooperative kernel's primitives, one G-lane group per query
+    pdq::OffgridCoopArgs o{};
+    o.k.cfg = *cfg;
+    o.k.lay = L;
+    o.k.n_inst = n_inst;
+    o.k.K = K;
+    o.k.times = times;
+    o.k.A = m.A;
+    o.k.Q = m.Q;
+    o.n_query = n_query;
+    o.t_query = t_query;
+    o.mean = solution->mean;
+    o.chol = solution->chol;
+    o.filt_mean = solution->filt_mean;
+    o.filt_chol = solution->filt_chol;
+    o.output_scale = solution->output_scale;
+    o.mean_out = mean_out;
+    o.chol_out = chol_out;
+    int G = cfg->lanes_per_instance > 1 ? cfg->lanes_per_instance : (L.nc <= 8 ? 8 : (L.nc <= 16 ? 16 : 32));
+    const int limit = lds_limit();
+    while (G < 64 && pdq::coop_lds_bytes(L, G) > (size_t)limit) G *= 2;
+    return pdq::launch_offgrid_coop(o, G, (hipStream_t)stream);
+  }
   pdq::OffgridArgs a{};
   a.cfg = *cfg;
   a.n_inst = n_inst;

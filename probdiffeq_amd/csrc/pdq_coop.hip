@@ -1515,14 +1515,79 @@ __device__ void solve_instance(const Ctx<G>& c, int inst) {
   }
 }
 
-#ifndef PDQ_HOST_EMU
+// ---------------------------------------------------------- off-grid marginals
+// ProbabilisticSolver.offgrid_marginals (solvers.py:149-203) for the
+// factorisations without a lane kernel (dense; isotropic of other shapes),
+// one G-lane group per query time, with the solve's primitives and slot
+// workspace.  For t the group finds the grid interval (t0, t1] by
+// searchsorted(t_grid, t) (side "left", as the reference; the index is
+// clamped to [1, K - 1], so every access stays inside the arrays) and
+// evaluates
+// * the filter (estimators_and_losses.py:403-414): the marginal at t0
+//   predicted to t with the transition of t1's output scale;
+// * the fixed-interval smoother (:674-707): the filtering marginal at t0
+//   reverted to t and on to t1, then the smoothed marginal at t1
+//   marginalised back through the conditional t1 -> t.
 template <int G>
-__global__ void __launch_bounds__(64) solve_kernel(KArgs args) {
-  extern __shared__ double lds[];
-  const KArgs& a = args;
+__device__ void offgrid_query(const Ctx<G>& c, const OffgridCoopArgs& o, size_t q) {
+  const Layout& L = *c.L;
+  const int K = c.a->K, nc = c.nc, mc = c.mc, ms = nc * mc, cs = nc * nc;
+  const size_t inst = q / (size_t)o.n_query;
+  const double* ts = c.a->times + inst * K;
+  const double t = o.t_query[q];
+  int lo = 0, hi = K;
+  while (lo < hi) {  // first index with ts[i] >= t
+    const int mid = (lo + hi) / 2;
+    if (ts[mid] < t) lo = mid + 1;
+    else hi = mid;
+  }
+  const int i = lo < 1 ? 1 : (lo > K - 1 ? K - 1 : lo);
+  const double t0 = ts[i - 1], t1 = ts[i];
+  const double os = o.output_scale[inst * K + i];
+  const size_t r0 = inst * K + (i - 1), r1 = inst * K + i;
+  const bool filter = c.a->cfg.strategy == PDQ_FILTER;
+  const double* m0p = (filter ? o.mean : o.filt_mean) + r0 * ms;
+  const double* L0p = (filter ? o.chol : o.filt_chol) + r0 * cs;
+  double *X0m = c.slot(0) + L.s_m, *X0L = c.slot(0) + L.s_L;
+  double *Xm = c.slot(1) + L.s_m, *XL = c.slot(1) + L.s_L;
+  double *Zm = c.slot(2) + L.s_m, *ZL = c.slot(2) + L.s_L;
+  double *P = c.W(L.w_P), *PI = c.W(L.w_PI), *big = c.W(L.w_big), *tmp = c.W(L.w_T1);
+  double *Ym = c.W(L.w_Pm), *YL = c.W(L.w_PL);
+  double *bA = c.W(L.w_bA), *bm = c.W(L.w_bm), *bL = c.W(L.w_bL), *btl = c.W(L.w_btl), *bto = c.W(L.w_bto);
+  g_copy(c.g, X0m, m0p, ms);
+  g_copy(c.g, X0L, L0p, cs);
+  c.g.sync();
+  precon(c, t - t0, P, PI);
+  c.g.sync();
+  const double s0 = noise_scale(c, t - t0, os);
+  if (filter) {
+    marginalise_prior(c, P, PI, s0, X0m, X0L, Ym, YL, big);
+  } else {
+    revert_prior(c, P, PI, s0, X0m, X0L, Xm, XL, bA, bm, bL, btl, bto, big, tmp);
+    c.g.sync();
+    precon(c, t1 - t, P, PI);
+    c.g.sync();
+    const double s1 = noise_scale(c, t1 - t, os);
+    revert_prior(c, P, PI, s1, Xm, XL, Zm, ZL, bA, bm, bL, btl, bto, big, tmp);
+    g_copy(c.g, X0m, o.mean + r1 * ms, ms);
+    g_copy(c.g, X0L, o.chol + r1 * cs, cs);
+    c.g.sync();
+    marginalise_cond(c, bA, bm, bL, btl, bto, X0m, X0L, Ym, YL, big);
+  }
+  c.g.sync();
+  g_copy(c.g, o.mean_out + q * ms, Ym, ms);
+  g_copy(c.g, o.chol_out + q * cs, YL, cs);
+  c.g.sync();
+}
+
+#ifndef PDQ_HOST_EMU
+// Workgroup set-up shared by the cooperative kernels: the IWP matrices (and
+// the exponential prior's drift rows) in LDS, and the context of this lane's
+// group.
+template <int G>
+__device__ Ctx<G> coop_setup(const KArgs& a, double* lds) {
   const Layout& L = a.lay;
   const int nc = L.nc;
-  // shared (per workgroup) IWP matrices and the exponential prior's drift rows
   double* sA = lds;
   double* sQ = lds + nc * nc;
   double* sD = lds + 2 * nc * nc;
@@ -1552,6 +1617,23 @@ __global__ void __launch_bounds__(64) solve_kernel(KArgs args) {
   // the exponential prior's transition lives in the instance's workspace
   c.sA = L.expo ? c.ws + L.w_xA : sA;
   c.sQ = L.expo ? c.ws + L.w_xL : sQ;
+  return c;
+}
+
+template <int G>
+__global__ void __launch_bounds__(64) offgrid_coop_kernel(OffgridCoopArgs o) {
+  extern __shared__ double lds[];
+  const Ctx<G> c = coop_setup<G>(o.k, lds);
+  const size_t total = (size_t)o.k.n_inst * o.n_query;
+  const size_t groups = (size_t)gridDim.x * (64 / G);
+  for (size_t q = (size_t)blockIdx.x * (64 / G) + threadIdx.x / G; q < total; q += groups) offgrid_query(c, o, q);
+}
+
+template <int G>
+__global__ void __launch_bounds__(64) solve_kernel(KArgs args) {
+  extern __shared__ double lds[];
+  const KArgs& a = args;
+  const Ctx<G> c = coop_setup<G>(a, lds);
   for (;;) {
     int inst = 0;
     if (c.g.l == 0) inst = atomicAdd(a.queue, 1);
@@ -1626,6 +1708,33 @@ int launch_coop_g(KArgs& a, hipStream_t st) {
   return PDQ_OK;
 }
 
+template <int G>
+int launch_offgrid_coop_g(OffgridCoopArgs& o, hipStream_t st) {
+  const size_t lds = coop_lds_bytes(o.k.lay, G);
+  auto kern = offgrid_coop_kernel<G>;
+  PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per_cu = 0;
+  PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
+  if (per_cu < 1) return host_error(PDQ_ERR_UNSUPPORTED, "a query does not fit into LDS (%zu bytes)", lds);
+  const long need = ((long)o.k.n_inst * o.n_query + 64 / G - 1) / (64 / G);
+  const long cap = (long)per_cu * num_cus() * 4;
+  const int grid = (int)std::max<long>(1, std::min(need, cap));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, o);
+  PDQ_HIP_TRY(hipGetLastError());
+  return PDQ_OK;
+}
+
+int launch_offgrid_coop(OffgridCoopArgs& o, int G, hipStream_t st) {
+  switch (G) {
+    case 4: return launch_offgrid_coop_g<4>(o, st);
+    case 8: return launch_offgrid_coop_g<8>(o, st);
+    case 16: return launch_offgrid_coop_g<16>(o, st);
+    case 32: return launch_offgrid_coop_g<32>(o, st);
+    case 64: return launch_offgrid_coop_g<64>(o, st);
+  }
+  return host_error(PDQ_ERR_INVALID, "lanes per query must be one of 4, 8, 16, 32, 64 (got %d)", G);
+}
+
 int launch_coop(KArgs& a, int G, hipStream_t st) {
   switch (G) {
     case 4: return launch_coop_g<4>(a, st);
@@ -1639,9 +1748,9 @@ int launch_coop(KArgs& a, int G, hipStream_t st) {
 
 
 #else
-// CPU build of one instance (G = 1); ws must hold L.inst_size doubles and
-// shared must hold L.wg_shared doubles.
-inline void emu_solve_instance(const KArgs& a, int inst, double* shared, double* ws) {
+// CPU build (G = 1); ws must hold L.inst_size doubles and shared must hold
+// L.wg_shared doubles.
+inline Ctx<1> emu_ctx(const KArgs& a, double* shared, double* ws) {
   const Layout& L = a.lay;
   const int nc = L.nc;
   for (int i = 0; i < nc * nc; ++i) {
@@ -1667,8 +1776,18 @@ inline void emu_solve_instance(const KArgs& a, int inst, double* shared, double*
   c.sD = shared + 2 * nc * nc;
   c.sA = L.expo ? ws + L.w_xA : shared;
   c.sQ = L.expo ? ws + L.w_xL : shared + nc * nc;
+  return c;
+}
+
+inline void emu_solve_instance(const KArgs& a, int inst, double* shared, double* ws) {
+  const Ctx<1> c = emu_ctx(a, shared, ws);
   if (a.mode == MODE_LML) lml_instance(c, inst);
   else solve_instance(c, inst);
+}
+
+inline void emu_offgrid_query(const OffgridCoopArgs& o, size_t q, double* shared, double* ws) {
+  const Ctx<1> c = emu_ctx(o.k, shared, ws);
+  offgrid_query(c, o, q);
 }
 #endif
 }  // namespace pdq

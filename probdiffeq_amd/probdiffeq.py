@@ -663,8 +663,8 @@ class _Solver:
             x = x if batched else x[None]
             return x.reshape(B, K, width).contiguous()
 
-        N, d = cfg.N, cfg.ode_dim
-        out = {"mean": flat(solution.u.mean_flat, N * d), "chol": flat(solution.u.cholesky_flat, N * N)}
+        N, d, ms, cs = cfg.N, cfg.ode_dim, cfg.mean_size, cfg.chol_size
+        out = {"mean": flat(solution.u.mean_flat, ms), "chol": flat(solution.u.cholesky_flat, cs)}
         os_ = solution.output_scale if batched else solution.output_scale[None]
         os_ = os_.reshape(B, -1)
         if os_.shape[1] == K - 1:  # non-dynamic solvers: K - 1 rows; the reference's index K - 1 clamps
@@ -672,14 +672,15 @@ class _Solver:
         out["output_scale"] = os_.contiguous()
         if self.strategy.name == "fixedinterval":
             filt = solution.solution_full.filtering
-            out["filt_mean"] = flat(filt.mean_flat, N * d)
-            out["filt_chol"] = flat(filt.cholesky_flat, N * N)
+            out["filt_mean"] = flat(filt.mean_flat, ms)
+            out["filt_chol"] = flat(filt.cholesky_flat, cs)
         mean, chol = capi.offgrid_marginals(cfg, out, t_all.contiguous(), tq)
         lead = (B, tq.shape[1]) if batched else (tq.shape[1],)
-        m = mean.reshape(*lead, N, d) if batched else mean[0].reshape(*lead, N, d)
-        L = chol.reshape(*lead, N, N) if batched else chol[0].reshape(*lead, N, N)
+        mshape, lshape = ((cfg.n,), (cfg.n, cfg.n)) if cfg.factorisation == "dense" else ((N, d), (N, N))
+        m = (mean if batched else mean[0]).reshape(*lead, *mshape)
+        L = (chol if batched else chol[0]).reshape(*lead, *lshape)
         if scalar:
-            m, L = m[..., 0, :, :], L[..., 0, :, :]
+            m, L = m[..., 0, :] if cfg.factorisation == "dense" else m[..., 0, :, :], L[..., 0, :, :]
         return Normal(m, L, cfg.factorisation, N, d)
 
 

@@ -115,7 +115,7 @@ def offgrid(cfg, times, sol, t_query):
     Qn = t_query.shape[1]
     N, d = cfg.num_derivatives + 1, cfg.ode_dim
     mean_out = np.zeros((B, Qn, N * d))
-    chol_out = np.zeros((B, Qn, N * N))
+    chol_out = np.zeros((B, Qn, cfg.chol_size))
     m, L, os_ = arr(sol["mean"]), arr(sol["chol"]), arr(sol["output_scale"])
     fm = arr(sol["filt_mean"]) if "filt_mean" in sol else None
     fL = arr(sol["filt_chol"]) if "filt_chol" in sol else None

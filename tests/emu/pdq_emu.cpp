@@ -301,9 +301,33 @@ int pdqemu_offgrid(const pdq_config* cfg, int n_inst, int K, const double* times
                    const double* chol, const double* filt_mean, const double* filt_chol, const double* output_scale,
                    int n_query, const double* t_query, double* mean_out, double* chol_out) {
   const int N = cfg->num_derivatives + 1, D = cfg->ode_dim;
-  if (cfg->factorisation != PDQ_ISOTROPIC || D != 2 || (N != 4 && N != 5) || cfg->strategy == PDQ_FIXEDPOINT)
-    return 2;
+  if (cfg->factorisation == PDQ_BLOCKDIAG || cfg->strategy == PDQ_FIXEDPOINT) return 2;
   std::vector<double> A, Q;
+  if (cfg->factorisation == PDQ_DENSE || D != 2 || (N != 4 && N != 5) || cfg->lanes_per_instance > 1) {
+    // the cooperative kernel's off-grid code, one query after another
+    if (cfg->factorisation == PDQ_DENSE) pdq::iwp_dense(cfg->num_derivatives, D, cfg->base_output_scale, A, Q);
+    else pdq::iwp_matrices(cfg->num_derivatives, A, Q);
+    pdq::OffgridCoopArgs o{};
+    o.k.cfg = *cfg;
+    o.k.lay = pdq::make_layout(*cfg, 0);
+    o.k.n_inst = n_inst;
+    o.k.K = K;
+    o.k.times = times;
+    o.k.A = A.data();
+    o.k.Q = Q.data();
+    o.n_query = n_query;
+    o.t_query = t_query;
+    o.mean = mean;
+    o.chol = chol;
+    o.filt_mean = filt_mean;
+    o.filt_chol = filt_chol;
+    o.output_scale = output_scale;
+    o.mean_out = mean_out;
+    o.chol_out = chol_out;
+    std::vector<double> shared(o.k.lay.wg_shared), ws(o.k.lay.inst_size);
+    for (size_t q = 0; q < (size_t)n_inst * n_query; ++q) pdq::emu_offgrid_query(o, q, shared.data(), ws.data());
+    return 0;
+  }
   pdq::iwp_matrices(cfg->num_derivatives, A, Q);
   pdq::OffgridArgs a{};
   a.cfg = *cfg;

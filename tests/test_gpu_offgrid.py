@@ -33,6 +33,40 @@ def test_offgrid_marginals_match_the_oracle(oracle, nu, strategy, solver):
     OC.check(mean[0].cpu().numpy(), chol[0].cpu().numpy(), refs, f"gpu offgrid {strategy}/{solver} nu={nu}")
 
 
+def _matern_drift(length_scale, N, d):
+    from math import comb
+
+    z = np.sqrt(2 * (N - 0.5)) / length_scale
+    return np.concatenate([-comb(N, i) * z ** (N - i) * np.eye(d) for i in range(N)], axis=1)
+
+
+@pytest.mark.parametrize(
+    "fact,nu,strategy,solver,kw",
+    [
+        ("dense", 4, "filter", "dynamic", {}),
+        ("dense", 3, "fixedinterval", "dynamic", {}),
+        ("dense", 3, "fixedinterval", "solver", {}),
+        ("isotropic", 2, "fixedinterval", "dynamic", {}),
+        ("isotropic", 4, "filter", "dynamic", dict(lanes_per_instance=16)),
+        ("dense", 3, "fixedinterval", "dynamic", dict(prior="exponential", prior_drift=_matern_drift(8.0, 4, 2))),
+    ],
+)
+def test_offgrid_marginals_on_the_cooperative_kernel(oracle, fact, nu, strategy, solver, kw):
+    """Dense, isotropic shapes without a lane kernel, and exponential priors: one G-lane group per
+    query on the cooperative kernel's primitives."""
+    from probdiffeq_amd import capi
+
+    capi.require_device()
+    cfg, times, sol, tq, refs = OC.make_case(oracle, nu, strategy, solver, fact=fact, **kw)
+    if cfg.prior_drift is not None:
+        cfg.prior_drift = _dev(cfg.prior_drift)
+    print(f"coop offgrid case: {fact} nu={nu} {strategy}/{solver} {kw.keys()}", flush=True)
+    out = {k: _dev(v[None]) for k, v in sol.items()}
+    mean, chol = capi.offgrid_marginals(cfg, out, _dev(times), _dev(tq))
+    torch.cuda.synchronize()
+    OC.check(mean[0].cpu().numpy(), chol[0].cpu().numpy(), refs, f"gpu coop offgrid {fact}/{strategy}/{solver} nu={nu}", cov=fact == "dense")
+
+
 def test_offgrid_refuses_the_fixedpoint_smoother_and_other_shapes():
     from probdiffeq_amd import capi
 
@@ -42,12 +76,13 @@ def test_offgrid_refuses_the_fixedpoint_smoother_and_other_shapes():
     out = {"mean": _dev(np.zeros((1, 5, 10))), "chol": _dev(np.zeros((1, 5, 25))), "output_scale": _dev(np.ones((1, 5, 1)))}
     with pytest.raises(capi.PdqError, match="fixed-point smoother"):
         capi.offgrid_marginals(capi.Config(factorisation="isotropic", strategy="fixedpoint"), out, times, tq)
-    with pytest.raises(capi.PdqError, match="isotropic"):
-        capi.offgrid_marginals(capi.Config(factorisation="dense", strategy="filter"), out, times, tq)
+    with pytest.raises(capi.PdqError, match="dense and isotropic"):
+        capi.offgrid_marginals(capi.Config(factorisation="blockdiag", strategy="filter"), out, times, tq)
 
 
+@pytest.mark.parametrize("fact", ["isotropic", "dense"])
 @pytest.mark.parametrize("strategy", ["filter", "fixedinterval"])
-def test_api_offgrid_marginals_of_a_device_solve(oracle, strategy):
+def test_api_offgrid_marginals_of_a_device_solve(oracle, strategy, fact):
     """solver.offgrid_marginals(t, solution=) on a batched fixed-grid solve made on the device,
     against the oracle evaluated on that same solution's arrays."""
     import test_gpu_api as A
@@ -57,14 +92,17 @@ def test_api_offgrid_marginals_of_a_device_solve(oracle, strategy):
 
     rng = np.random.default_rng(4)
     y0 = torch.tensor(rng.uniform(10, 30, size=(3, 2)), dtype=torch.float64, device="cuda")
-    init, slv, _ = A._setup(pdx, "isotropic", strategy, "dynamic", y0)
+    init, slv, _ = A._setup(pdx, fact, strategy, "dynamic", y0)
     grid = torch.linspace(0.0, 4.0, 41, dtype=torch.float64, device="cuda")
     sol = ivpsolve.solve_fixed_grid(solver=slv)(init, grid=grid)
     tq = torch.tensor([0.37, 1.55, 3.93], dtype=torch.float64, device="cuda")
     print(f"offgrid api case: {strategy} B=3 K=41 Q=3", flush=True)
     rv = slv.offgrid_marginals(tq, solution=sol)
     torch.cuda.synchronize()
-    assert rv.mean_flat.shape == (3, 3, 5, 2) and rv.cholesky_flat.shape == (3, 3, 5, 5)
+    if fact == "dense":
+        assert rv.mean_flat.shape == (3, 3, 10) and rv.cholesky_flat.shape == (3, 3, 10, 10)
+    else:
+        assert rv.mean_flat.shape == (3, 3, 5, 2) and rv.cholesky_flat.shape == (3, 3, 5, 5)
     one = slv.offgrid_marginals(1.55, solution=sol)
     assert torch.equal(one.mean_flat, rv.mean_flat[:, 1])
     ocfg = T._oracle_cfg(oracle, sol.config)
@@ -83,8 +121,10 @@ def test_api_offgrid_marginals_of_a_device_solve(oracle, strategy):
                     mo, Lo = oracle.offgrid_marginal(ocfg, T.THETA, g[i - 1], g[i], t, os_[min(i, os_.size - 1)], fm[i - 1].ravel(), fL[i - 1].ravel(), m[i].ravel(), L[i].ravel())
                 per.append((np.asarray(mo, dtype=np.float64), np.asarray(Lo, dtype=np.float64)))
             (mr, Lr), (mf, Lf), (mt, Lt) = per
-            T.check_accuracy(rv.mean_flat[b, q].cpu().numpy().ravel(), mr, mf, mt, f"api offgrid {strategy} {b} {q} mean")
-            T.check_accuracy(rv.cholesky_flat[b, q].cpu().numpy().ravel(), Lr, Lf, Lt, f"api offgrid {strategy} {b} {q} chol")
+            T.check_accuracy(rv.mean_flat[b, q].cpu().numpy().ravel(), mr, mf, mt, f"api offgrid {fact}/{strategy} {b} {q} mean")
+            n = int(round(np.sqrt(Lr.size)))
+            cov = lambda x: np.asarray(x, dtype=np.float64).reshape(n, n) @ np.asarray(x, dtype=np.float64).reshape(n, n).T
+            T.check_accuracy(cov(rv.cholesky_flat[b, q].cpu().numpy()), cov(Lr), cov(Lf), cov(Lt), f"api offgrid {fact}/{strategy} {b} {q} cov")
     fp = pdx.solver_dynamic(strategy=pdx.strategy_smoother_fixedpoint(), constraint=slv.constraint)
     with pytest.raises(NotImplementedError):
         fp.offgrid_marginals(tq, solution=sol)

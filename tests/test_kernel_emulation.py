@@ -278,6 +278,14 @@ def test_lane_lml_value_only_matches_the_gradient_run(oracle, emu):
     assert np.array_equal(v0, v1)
 
 
+def _matern_drift(length_scale, N, d):
+    """prior_matern's drift (ssm_impl_api.py:443-477), pinned in tests/test_oracle_prior.py."""
+    from math import comb
+
+    z = np.sqrt(2 * (N - 0.5)) / length_scale
+    return np.concatenate([-comb(N, i) * z ** (N - i) * np.eye(d) for i in range(N)], axis=1)
+
+
 # ------------------------------------------------ off-grid marginals
 @pytest.mark.parametrize("nu,strategy,solver", [(4, "filter", "dynamic"), (3, "filter", "solver"), (4, "fixedinterval", "dynamic"), (3, "fixedinterval", "mle")])
 def test_offgrid_marginals_kernel_code(oracle, emu, nu, strategy, solver):
@@ -289,6 +297,28 @@ def test_offgrid_marginals_kernel_code(oracle, emu, nu, strategy, solver):
     batch = {k: v[None] for k, v in sol.items()}
     m, L = emu.offgrid(cfg, times, batch, tq)
     OC.check(m[0], L[0], refs, f"emulation offgrid {strategy}/{solver} nu={nu}")
+
+
+@pytest.mark.parametrize(
+    "fact,nu,strategy,solver,kw",
+    [
+        ("dense", 4, "filter", "dynamic", {}),
+        ("dense", 3, "fixedinterval", "dynamic", {}),
+        ("dense", 3, "fixedinterval", "solver", {}),
+        ("isotropic", 2, "fixedinterval", "dynamic", {}),
+        ("isotropic", 4, "filter", "dynamic", dict(lanes_per_instance=16)),
+        ("dense", 3, "fixedinterval", "dynamic", dict(prior="exponential", prior_drift=_matern_drift(8.0, 4, 2))),
+    ],
+)
+def test_offgrid_marginals_cooperative_code(oracle, emu, fact, nu, strategy, solver, kw):
+    """Off-grid marginals on the cooperative kernel's primitives (dense; isotropic shapes without a
+    lane kernel; exponential priors), against the oracle's offgrid_marginal in its three builds."""
+    import offgrid_cases as OC
+
+    cfg, times, sol, tq, refs = OC.make_case(oracle, nu, strategy, solver, fact=fact, **kw)
+    batch = {k: v[None] for k, v in sol.items()}
+    m, L = emu.offgrid(cfg, times, batch, tq)
+    OC.check(m[0], L[0], refs, f"emulation coop offgrid {fact}/{strategy}/{solver} nu={nu}", cov=fact == "dense")
 
 
 @pytest.mark.parametrize("strategy", ["filter", "fixedpoint"])
@@ -358,14 +388,6 @@ def test_lane_kernel_isotropic_ts1(oracle, emu, strategy, solver, vf, kw):
         cfg = capi.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver=solver, constraint="ts1",
                           rtol=1e-5, atol=1e-7, **kw)
     _run(oracle, emu, cfg, p, y, t, kernel="lane")
-
-
-def _matern_drift(length_scale, N, d):
-    """prior_matern's drift (ssm_impl_api.py:443-477), pinned in tests/test_oracle_prior.py."""
-    from math import comb
-
-    z = np.sqrt(2 * (N - 0.5)) / length_scale
-    return np.concatenate([-comb(N, i) * z ** (N - i) * np.eye(d) for i in range(N)], axis=1)
 
 
 @pytest.mark.parametrize(
