@@ -153,6 +153,10 @@ struct SampleArgs {
   double* samples;          // n_inst x n_samples x K x (N d)
 };
 int launch_sample(SampleArgs& a, int N, int D, hipStream_t st);
+// The same for any dense or isotropic shape with n * mc <= 32
+// (pdq_sample_generic.h, pdq_post.hip); base has n draws per step.
+constexpr int kSampleGenericMax = 32;  // n * mc
+int launch_sample_generic(SampleArgs& a, int n, int mc, int dense, hipStream_t st);
 
 // loss_lml_terminal_values on isotropic marginals (pdq_post.hip).
 int launch_terminal_lml(const pdq_config& cfg, int n_inst, int idx, const double* mean, const double* chol,

@@ -543,11 +543,13 @@ int32_t pdq_sample(const pdq_config* cfg, int32_t n_inst, int32_t K, const pdq_s
   if (cfg->strategy == PDQ_FILTER)
     return set_err(PDQ_ERR_UNSUPPORTED, "sampling needs a smoother's posterior (a filter's solution is no Markov sequence)");
   const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
-  if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(N, d))
-    return set_err(PDQ_ERR_UNSUPPORTED,
-                   "sampling is implemented for the isotropic factorisation with nu = 3 or 4 and d = 2 (got nu = %d, "
-                   "d = %d)",
-                   N - 1, d);
+  if (cfg->factorisation == PDQ_BLOCKDIAG)
+    return set_err(PDQ_ERR_UNSUPPORTED, "sampling is implemented for the dense and isotropic factorisations");
+  const bool lane = cfg->factorisation == PDQ_ISOTROPIC && pdq::lane_supported(N, d) && cfg->lanes_per_instance <= 1;
+  const pdq::Layout L = pdq::make_layout(*cfg, 0);
+  if (!lane && L.nc * L.mc > pdq::kSampleGenericMax)
+    return set_err(PDQ_ERR_UNSUPPORTED, "sampling supports (nu + 1) d <= %d (got %d)", pdq::kSampleGenericMax,
+                   L.nc * L.mc);
   if (n_inst == 0 || n_samples == 0) return PDQ_OK;
   const pdq_solution& s = *solution;
   if (!base || !samples || !s.term_mean || !s.term_chol || !s.cond_A || !s.cond_mean || !s.cond_chol || !s.cond_tl ||
@@ -567,6 +569,8 @@ int32_t pdq_sample(const pdq_config* cfg, int32_t n_inst, int32_t K, const pdq_s
   a.cond_to = s.cond_to;
   a.base = base;
   a.samples = samples;
+  if (!lane)
+    return pdq::launch_sample_generic(a, L.nc, L.mc, cfg->factorisation == PDQ_DENSE ? 1 : 0, (hipStream_t)stream);
   return pdq::launch_sample(a, N, d, (hipStream_t)stream);
 }
 

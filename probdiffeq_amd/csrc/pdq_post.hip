@@ -11,6 +11,7 @@
 #include <algorithm>
 
 #include "pdq_args.h"
+#include "pdq_sample_generic.h"
 
 namespace pdq {
 
@@ -45,6 +46,28 @@ int launch_terminal_lml(const pdq_config& cfg, int n_inst, int idx, const double
   const int N = cfg.num_derivatives + 1, d = cfg.ode_dim;
   hipLaunchKernelGGL(terminal_lml_kernel, dim3((n_inst + 63) / 64), dim3(64), 0, st, N, d, n_inst, idx, mean, chol, u,
                      std, value);
+  PDQ_HIP_TRY(hipGetLastError());
+  return PDQ_OK;
+}
+
+// MarkovSequence.sample for the shapes without a lane sampler: one lane per
+// (instance, sample), its three vectors element-major in LDS (stride 64, so
+// the lanes of a wavefront hit distinct banks).
+__global__ void __launch_bounds__(64) sample_generic_kernel(SampleArgs a, int n, int mc, int dense) {
+  __shared__ double v[3 * kSampleGenericMax * 64];
+  const size_t total = (size_t)a.n_inst * a.n_samples;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride)
+    sample_generic(a, g, n, mc, dense, v + threadIdx.x, 64);
+}
+
+int launch_sample_generic(SampleArgs& a, int n, int mc, int dense, hipStream_t st) {
+  if (n * mc > kSampleGenericMax) return host_error(PDQ_ERR_UNSUPPORTED, "sampling supports n * mc <= %d", kSampleGenericMax);
+  const size_t total = (size_t)a.n_inst * a.n_samples;
+  const long need = (long)((total + 63) / 64);
+  const long cap = 8L * num_cus();
+  const int grid = (int)std::max<long>(1, std::min(need, cap));
+  hipLaunchKernelGGL(sample_generic_kernel, dim3(grid), dim3(64), 0, st, a, n, mc, dense);
   PDQ_HIP_TRY(hipGetLastError());
   return PDQ_OK;
 }

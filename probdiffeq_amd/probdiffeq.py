@@ -365,49 +365,56 @@ class MarkovSequence:
     reverse: bool = True
 
     def sample(self, key, *, shape: tuple = ()):
-        """estimators_and_losses.py:233-273 on the device (isotropic).  ``key`` is an int seed or a
-        ``torch.Generator`` on the device (torch has no JAX keys); the standard normal draws come from
-        it, so samples match the reference's law, not its bits.  Returns (*shape, K, N, d), with a
-        leading instance axis for a batched solution."""
+        """estimators_and_losses.py:233-273 on the device (dense and isotropic).  ``key`` is an int seed
+        or a ``torch.Generator`` on the device (torch has no JAX keys); the standard normal draws come
+        from it, so samples match the reference's law, not its bits.  Returns (*shape, K, N, d)
+        (isotropic) or (*shape, K, N d) (dense, the derivative-major mean layout), with a leading
+        instance axis for a batched solution."""
         gen = key
         if not isinstance(key, torch.Generator):
             gen = torch.Generator(device=self.marginal.mean_flat.device)
             gen.manual_seed(int(key))
         shape = tuple(shape)
         n = int(np.prod(shape)) if shape else 1
-        batched = self.marginal.mean_flat.dim() == 3
+        dense = self.marginal.factorisation == "dense"
+        batched = self.marginal.mean_flat.dim() == (2 if dense else 3)
         B = self.marginal.mean_flat.shape[0] if batched else 1
         K = self.conditional.A.shape[-3] + 1
         N = self.marginal.num_coeffs
-        base = torch.randn(B, n, K, N, generator=gen, dtype=torch.float64, device=self.marginal.mean_flat.device)
+        draws = N * self.marginal.ode_dim if dense else N
+        base = torch.randn(B, n, K, draws, generator=gen, dtype=torch.float64, device=self.marginal.mean_flat.device)
         out = self.sample_from_draws(base if batched else base[0])
         lead = ((B,) if batched else ()) + shape
-        return out.reshape(*lead, *out.shape[-3:])
+        tail = out.shape[-2:] if dense else out.shape[-3:]
+        return out.reshape(*lead, *tail)
 
     def sample_from_draws(self, base):
-        """The sampler with given standard normal draws base (S, K, N), or (B, S, K, N) for a batched
-        solution: row K - 1 of each sample draws the terminal marginal, row k the conditional k."""
+        """The sampler with given standard normal draws base (S, K, n), or (B, S, K, n) for a batched
+        solution, n the covariance dimension (N isotropic, N d dense): row K - 1 of each sample draws
+        the terminal marginal, row k the conditional k."""
         m = self.marginal
-        if m.factorisation != "isotropic":
-            raise NotImplementedError("sampling is implemented for the isotropic factorisation")
-        batched = m.mean_flat.dim() == 3
+        if m.factorisation not in ("isotropic", "dense"):
+            raise NotImplementedError("sampling is implemented for the dense and isotropic factorisations")
+        dense = m.factorisation == "dense"
         N, d = m.num_coeffs, m.ode_dim
+        n = N * d if dense else N
+        batched = m.mean_flat.dim() == (2 if dense else 3)
         b = base if batched else base[None]
         c = self.conditional
         B = b.shape[0]
         K = c.A.shape[-3] + 1
         out = {
             "term_mean": (m.mean_flat if batched else m.mean_flat[None]).reshape(B, N * d).contiguous(),
-            "term_chol": (m.cholesky_flat if batched else m.cholesky_flat[None]).reshape(B, N * N).contiguous(),
-            "cond_A": (c.A if batched else c.A[None]).reshape(B, K - 1, N * N).contiguous(),
+            "term_chol": (m.cholesky_flat if batched else m.cholesky_flat[None]).reshape(B, n * n).contiguous(),
+            "cond_A": (c.A if batched else c.A[None]).reshape(B, K - 1, n * n).contiguous(),
             "cond_mean": (c.noise.mean_flat if batched else c.noise.mean_flat[None]).reshape(B, K - 1, N * d).contiguous(),
-            "cond_chol": (c.noise.cholesky_flat if batched else c.noise.cholesky_flat[None]).reshape(B, K - 1, N * N).contiguous(),
-            "cond_tl": (c.to_latent if batched else c.to_latent[None]).reshape(B, K - 1, N).contiguous(),
-            "cond_to": (c.to_observed if batched else c.to_observed[None]).reshape(B, K - 1, N).contiguous(),
+            "cond_chol": (c.noise.cholesky_flat if batched else c.noise.cholesky_flat[None]).reshape(B, K - 1, n * n).contiguous(),
+            "cond_tl": (c.to_latent if batched else c.to_latent[None]).reshape(B, K - 1, n).contiguous(),
+            "cond_to": (c.to_observed if batched else c.to_observed[None]).reshape(B, K - 1, n).contiguous(),
         }
-        cfg = capi.Config(factorisation="isotropic", num_derivatives=N - 1, ode_dim=d, strategy="fixedpoint")
+        cfg = capi.Config(factorisation=m.factorisation, num_derivatives=N - 1, ode_dim=d, strategy="fixedpoint")
         samples = capi.sample(cfg, out, K, b.contiguous())
-        samples = samples.reshape(B, b.shape[1], K, N, d)
+        samples = samples.reshape(B, b.shape[1], K, N * d) if dense else samples.reshape(B, b.shape[1], K, N, d)
         return samples if batched else samples[0]
 
 

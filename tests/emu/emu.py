@@ -130,12 +130,11 @@ def offgrid(cfg, times, sol, t_query):
 
 def sample(cfg, sol, base):
     """The posterior sampler's code on the CPU: sol = one instance's term_*/cond_* (solve()'s layouts),
-    base (S, K, N).  Returns (S, K, N d)."""
+    base (S, K, n) with n the covariance dimension.  Returns (S, K, mean_size)."""
     arr = lambda x: np.ascontiguousarray(x, dtype=np.float64)
     base = arr(base)
-    S, K, N = base.shape
-    d = cfg.ode_dim
-    out = np.zeros((S, K, N * d))
+    S, K, _ = base.shape
+    out = np.zeros((S, K, cfg.mean_size))
     keys = ("term_mean", "term_chol", "cond_A", "cond_mean", "cond_chol", "cond_tl", "cond_to")
     arrs = [arr(sol[k]) for k in keys]
     P = ctypes.c_void_p

@@ -13,6 +13,7 @@
 #include "../../probdiffeq_amd/csrc/pdq_lane_lml.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane_offgrid.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane_sample.h"
+#include "../../probdiffeq_amd/csrc/pdq_sample_generic.h"
 
 namespace pdq {
 int host_error(int code, const char*, ...) { return code; }
@@ -361,7 +362,8 @@ int pdqemu_sample(const pdq_config* cfg, int n_inst, int K, const double* term_m
                   const double* cA, const double* cm, const double* cL, const double* ctl, const double* cto,
                   int n_samples, const double* base, double* samples) {
   const int N = cfg->num_derivatives + 1, D = cfg->ode_dim;
-  if (cfg->factorisation != PDQ_ISOTROPIC || D != 2 || (N != 4 && N != 5)) return 2;
+  if (cfg->factorisation == PDQ_BLOCKDIAG) return 2;
+  const bool lane = cfg->factorisation == PDQ_ISOTROPIC && D == 2 && (N == 4 || N == 5) && cfg->lanes_per_instance <= 1;
   pdq::SampleArgs a{};
   a.cfg = *cfg;
   a.n_inst = n_inst;
@@ -376,6 +378,15 @@ int pdqemu_sample(const pdq_config* cfg, int n_inst, int K, const double* term_m
   a.cond_to = cto;
   a.base = base;
   a.samples = samples;
+  if (!lane) {
+    // the generic sampler (dense, other isotropic shapes), one sample after another
+    const int dense = cfg->factorisation == PDQ_DENSE;
+    const int n = dense ? N * D : N, mc = dense ? 1 : D;
+    if (n * mc > pdq::kSampleGenericMax) return 2;
+    std::vector<double> v(3 * pdq::kSampleGenericMax);
+    for (size_t g = 0; g < (size_t)n_inst * n_samples; ++g) pdq::sample_generic(a, g, n, mc, dense, v.data(), 1);
+    return 0;
+  }
   for (size_t g = 0; g < (size_t)n_inst * n_samples; ++g) {
     if (N == 4) pdq::lane_sample<4, 2>(a, g);
     else pdq::lane_sample<5, 2>(a, g);

@@ -6,10 +6,11 @@ import numpy as np
 import test_gpu_parity as T
 
 
-def make_case(oracle, nu, n_samples=6, K=20, seed=0):
+def make_case(oracle, nu, n_samples=6, K=20, seed=0, fact="isotropic", strategy="fixedpoint", **cfg_kw):
+    """base: (n_samples, K, n) draws, n the covariance dimension ((nu + 1) d dense, nu + 1 isotropic)."""
     from probdiffeq_amd import capi
 
-    cfg = capi.Config(factorisation="isotropic", num_derivatives=nu, ode_dim=2, strategy="fixedpoint", solver="dynamic")
+    cfg = capi.Config(factorisation=fact, num_derivatives=nu, ode_dim=2, strategy=strategy, solver="dynamic", **cfg_kw)
     ocfg = T._oracle_cfg(oracle, cfg)
     params, y0, _ = T._lv_batch(1, seed=seed)
     tc = oracle.taylor_coefficients("lotka_volterra", params[0], y0[0], 0.0, nu)
@@ -18,7 +19,7 @@ def make_case(oracle, nu, n_samples=6, K=20, seed=0):
     keys = ("term_mean", "term_chol", "cond_A", "cond_mean", "cond_chol", "cond_tl", "cond_to")
     sol = {k: np.asarray(sol[k], dtype=np.float64) for k in keys}
     rng = np.random.default_rng(seed + 7)
-    base = rng.normal(size=(n_samples, K, nu + 1))
+    base = rng.normal(size=(n_samples, K, cfg.n))
     refs = []
     for s in range(n_samples):
         per = []

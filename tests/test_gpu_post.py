@@ -81,6 +81,42 @@ def test_posterior_samples_match_the_oracle(oracle, nu):
     SC.check(samples[0].cpu().numpy(), refs, f"gpu sample nu={nu}")
 
 
+def _matern_drift(length_scale, N, d):
+    from math import comb
+
+    z = np.sqrt(2 * (N - 0.5)) / length_scale
+    return np.concatenate([-comb(N, i) * z ** (N - i) * np.eye(d) for i in range(N)], axis=1)
+
+
+@pytest.mark.parametrize(
+    "fact,nu,strategy,kw",
+    [
+        ("dense", 4, "fixedpoint", {}),
+        ("dense", 3, "fixedinterval", {}),
+        ("isotropic", 2, "fixedpoint", {}),
+        ("isotropic", 4, "fixedinterval", dict(lanes_per_instance=16)),
+        ("dense", 3, "fixedpoint", dict(prior="exponential", prior_drift=_matern_drift(8.0, 4, 2))),
+    ],
+)
+def test_generic_sampler_matches_the_oracle(oracle, fact, nu, strategy, kw):
+    """Sampling for the shapes without a lane sampler (dense with n draws per step; other isotropic
+    shapes) through the C ABI, against the oracle in its three builds."""
+    import sample_cases as SC
+
+    from probdiffeq_amd import capi
+
+    capi.require_device()
+    cfg, sol, base, refs = SC.make_case(oracle, nu, fact=fact, strategy=strategy, **kw)
+    if cfg.prior_drift is not None:
+        cfg.prior_drift = _dev(cfg.prior_drift)
+    K = base.shape[1]
+    print(f"generic sample case: {fact} nu={nu} {strategy} S={base.shape[0]} K={K}", flush=True)
+    out = {k: _dev(v[None]) for k, v in sol.items()}
+    samples = capi.sample(cfg, out, K, _dev(base[None]))
+    torch.cuda.synchronize()
+    SC.check(samples[0].cpu().numpy(), refs, f"gpu generic sample {fact}/{strategy} nu={nu}")
+
+
 def test_api_posterior_samples():
     """The mirror's MarkovSequence.sample: shapes, a seed's reproducibility, and the draws path equal
     to the C ABI's; a filter's solution has no posterior to sample."""
@@ -119,3 +155,43 @@ def test_api_posterior_samples():
     lz = torch.matmul(post.marginal.cholesky_flat[:, None].contiguous(), base[:, :, -1, :, None].contiguous())[..., 0]
     exp = post.marginal.mean_flat[:, None] + lz[..., None]
     assert torch.allclose(via_api[:, :, -1], exp, rtol=1e-14, atol=1e-12)
+
+
+def test_api_posterior_samples_dense(oracle):
+    """The mirror's MarkovSequence.sample on a dense fixed-point solve: shapes (K, N d), a seed's
+    reproducibility, and each sample equal to the oracle's with the same draws."""
+    import test_gpu_api as A
+
+    from probdiffeq_amd import ivpsolve
+    from probdiffeq_amd import probdiffeq as pdx
+
+    y0 = torch.tensor([[20.0, 20.0], [15.0, 25.0]], dtype=torch.float64, device="cuda")
+    init, slv, err = A._setup(pdx, "dense", "fixedpoint", "dynamic", y0)
+    save_at = torch.linspace(0.0, 6.0, 12, dtype=torch.float64, device="cuda")
+    sol = ivpsolve.solve_adaptive_save_at(solver=slv, error=err)(init, save_at=save_at, atol=1e-8, rtol=1e-6)
+    post = sol.solution_full.posterior
+    print("dense sample api case: B=2 K=12", flush=True)
+    s1 = post.sample(5, shape=(3,))
+    assert s1.shape == (2, 3, 12, 10)
+    assert torch.equal(s1, post.sample(5, shape=(3,)))
+    base = torch.randn(2, 2, 12, 10, dtype=torch.float64, device="cuda")
+    got = post.sample_from_draws(base).cpu().numpy()
+    ocfg = T._oracle_cfg(oracle, sol.config)
+    c = post.conditional
+    for b in range(2):
+        arrs = {
+            "term_mean": post.marginal.mean_flat[b].cpu().numpy(),
+            "term_chol": post.marginal.cholesky_flat[b].cpu().numpy(),
+            "cond_A": c.A[b].cpu().numpy(),
+            "cond_mean": c.noise.mean_flat[b].cpu().numpy(),
+            "cond_chol": c.noise.cholesky_flat[b].cpu().numpy(),
+            "cond_tl": c.to_latent[b].cpu().numpy(),
+            "cond_to": c.to_observed[b].cpu().numpy(),
+        }
+        for s in range(2):
+            z = base[b, s].cpu().numpy()
+            per = []
+            for build in ("strict", "fma", "extended"):
+                with oracle.rounding(build):
+                    per.append(np.asarray(oracle.sample(ocfg, T.THETA, arrs, z), dtype=np.float64))
+            T.check_accuracy(got[b, s], *per, f"dense api sample {b} {s}")

@@ -441,3 +441,23 @@ def test_per_instance_dt0(oracle, emu, fact, kernel):
         T._check_fields(cfg_i, T._fields(cfg_i, got), r, rf, rt, f"per-instance dt0 {kernel} instance {i}")
     r0 = oracle.solve_save_at(T._oracle_cfg(oracle, capi.Config(**{**cfg.__dict__, "dt0_per_instance": None})), p[1], tcs[1], t[1])
     assert r0["attempts"] != g["attempts"][1] or not np.allclose(r0["mean"], g["mean"][1], rtol=1e-12)
+
+
+@pytest.mark.parametrize(
+    "fact,nu,strategy,kw",
+    [
+        ("dense", 4, "fixedpoint", {}),
+        ("dense", 3, "fixedinterval", {}),
+        ("isotropic", 2, "fixedpoint", {}),
+        ("isotropic", 4, "fixedinterval", dict(lanes_per_instance=16)),
+        ("dense", 3, "fixedpoint", dict(prior="exponential", prior_drift=_matern_drift(8.0, 4, 2))),
+    ],
+)
+def test_generic_sampler_code(oracle, emu, fact, nu, strategy, kw):
+    """The sampler for the shapes without a lane sampler (pdq_sample_generic.h): dense (to applied to
+    the whole mean, n draws per step) and isotropic (the reference's isotropic forms), against the
+    oracle's MarkovSequence.sample (pinned in tests/test_oracle_sample.py) in its three builds."""
+    import sample_cases as SC
+
+    cfg, sol, base, refs = SC.make_case(oracle, nu, fact=fact, strategy=strategy, **kw)
+    SC.check(emu.sample(cfg, sol, base), refs, f"emulation generic sample {fact}/{strategy} nu={nu}")
