@@ -205,17 +205,20 @@ int32_t pdq_taylor_coefficients(int32_t vector_field, int32_t d, int32_t n_param
  *             and off the grid points (as the reference requires)
  *   mean_out: n_inst x n_query x mean_size, chol_out: ... x chol_size
  * Filter and fixed-interval smoother; the fixed-point smoother is refused
- * (the reference raises NotImplementedError).  Isotropic nu = 3, 4, d = 2. */
+ * (the reference raises NotImplementedError).  Isotropic nu = 3, 4, d = 2 on
+ * the lane kernel; dense, the other isotropic shapes and exponential priors
+ * on the cooperative kernel's primitives. */
 int32_t pdq_offgrid_marginals(const pdq_config* cfg, int32_t n_inst, int32_t K, const double* times,
                               const pdq_solution* solution, int32_t n_query, const double* t_query,
                               double* mean_out, double* chol_out, void* stream);
 
 /* Replaces loss_lml_terminal_values(tcoeff_index=idx)(u, marginals=, std=)
  * (probdiffeq/_probdiffeq/estimators_and_losses.py:20-50) for a batch of
- * isotropic marginals (e.g. the terminal values of a solve):
- *   mean: n_inst x (nu+1) x d, chol: n_inst x (nu+1) x (nu+1),
- *   u: n_inst x d data, std: n_inst observation noise (one scalar each, as
- *   marginals.std[idx] is for the isotropic factorisation),
+ * isotropic or dense marginals (e.g. the terminal values of a solve):
+ *   mean: n_inst x mean_size, chol: n_inst x chol_size,
+ *   u: n_inst x d data, std: the observation noise, shaped as
+ *   marginals.std[idx] is: one scalar per instance (isotropic) or d per
+ *   instance (dense, (nu+1) d + d) d <= 320),
  *   value: n_inst log-densities. */
 int32_t pdq_lml_terminal_values(const pdq_config* cfg, int32_t n_inst, const double* mean, const double* chol,
                                 const double* u, const double* std, int32_t tcoeff_index, double* value,
@@ -226,12 +229,14 @@ int32_t pdq_lml_terminal_values(const pdq_config* cfg, int32_t n_inst, const dou
  * posterior (solution.posterior: term_mean / term_chol and the K-1 backward
  * conditionals cond_*, as a solve with this configuration wrote them).  The
  * standard normal draws are an input instead of a JAX key:
- *   base:    n_inst x n_samples x K x (nu+1); row K-1 samples the terminal
- *            marginal, row k the conditional k (the isotropic sample_flat
- *            draws one (nu+1)-vector per row for all d columns)
+ *   base:    n_inst x n_samples x K x n, n = nu+1 (isotropic: sample_flat
+ *            draws one (nu+1)-vector per row for all d columns) or (nu+1) d
+ *            (dense); row K-1 samples the terminal marginal, row k the
+ *            conditional k
  *   samples: n_inst x n_samples x K x mean_size
- * Smoothers only (a filter's solution is no Markov sequence).  Isotropic
- * nu = 3, 4, d = 2. */
+ * Smoothers only (a filter's solution is no Markov sequence).  Dense and
+ * isotropic with (nu+1) d <= 32 (isotropic nu = 3, 4, d = 2 on the lane
+ * sampler, the other shapes on a generic one). */
 int32_t pdq_sample(const pdq_config* cfg, int32_t n_inst, int32_t K, const pdq_solution* solution,
                    int32_t n_samples, const double* base, double* samples, void* stream);
 

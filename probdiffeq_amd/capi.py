@@ -404,8 +404,9 @@ def sample(cfg: Config, out: dict, K: int, base, stream=None):
 
 
 def lml_terminal_values(cfg: Config, mean, chol, u, std, tcoeff_index=0, stream=None):
-    """loss_lml_terminal_values (estimators_and_losses.py:20-50) for a batch of isotropic marginals:
-    mean (B, N, d), chol (B, N, N), data u (B, d), std (B,).  Returns (B,) log-densities."""
+    """loss_lml_terminal_values (estimators_and_losses.py:20-50) for a batch of marginals: isotropic
+    mean (B, N, d), chol (B, N, N), std (B,); dense mean (B, N d), chol (B, N d, N d), std (B, d);
+    data u (B, d).  Returns (B,) log-densities."""
     B = mean.shape[0]
     value = torch.empty(B, dtype=torch.float64, device=mean.device)
     c = cfg.c()

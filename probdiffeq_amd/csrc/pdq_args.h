@@ -158,7 +158,10 @@ int launch_sample(SampleArgs& a, int N, int D, hipStream_t st);
 constexpr int kSampleGenericMax = 32;  // n * mc
 int launch_sample_generic(SampleArgs& a, int n, int mc, int dense, hipStream_t st);
 
-// loss_lml_terminal_values on isotropic marginals (pdq_post.hip).
+// loss_lml_terminal_values on isotropic and dense marginals (pdq_post.hip);
+// the dense kernel keeps a lane's (n + d) x d stack in LDS.
+constexpr int kTerminalDenseBlock = 32;
+constexpr int kTerminalDenseMax = 320;  // (n + d) d
 int launch_terminal_lml(const pdq_config& cfg, int n_inst, int idx, const double* mean, const double* chol,
                         const double* u, const double* std, double* value, hipStream_t st);
 

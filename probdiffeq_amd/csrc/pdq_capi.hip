@@ -579,8 +579,8 @@ int32_t pdq_lml_terminal_values(const pdq_config* cfg, int32_t n_inst, const dou
                                 void* stream) {
   if (!cfg) return set_err(PDQ_ERR_INVALID, "cfg is NULL");
   if (n_inst < 0) return set_err(PDQ_ERR_INVALID, "n_inst < 0");
-  if (cfg->factorisation != PDQ_ISOTROPIC)
-    return set_err(PDQ_ERR_UNSUPPORTED, "loss_lml_terminal_values is implemented for the isotropic factorisation");
+  if (cfg->factorisation == PDQ_BLOCKDIAG)
+    return set_err(PDQ_ERR_UNSUPPORTED, "loss_lml_terminal_values is implemented for the isotropic and dense factorisations");
   if (cfg->num_derivatives < 0 || cfg->num_derivatives > 7 || cfg->ode_dim < 1)
     return set_err(PDQ_ERR_INVALID, "invalid num_derivatives or ode_dim");
   if (tcoeff_index < 0 || tcoeff_index > cfg->num_derivatives)

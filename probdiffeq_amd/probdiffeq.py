@@ -839,12 +839,13 @@ def build_solution(cfg: capi.Config, out: dict, *, batched: bool) -> Probabilist
 # ---------------------------------------------------------------- the LML
 def loss_lml_terminal_values(*, tcoeff_index=0):
     """estimators_and_losses.py:20-50 on the device: the log-density of the data u under the
-    marginals' derivative `tcoeff_index` observed with noise `std`.  ``marginals`` is an isotropic
-    Normal (e.g. a terminal-value solution's ``u``), single or batched; ``u`` has shape (d,) or (B, d)."""
+    marginals' derivative `tcoeff_index` observed with noise `std`.  ``marginals`` is an isotropic or
+    dense Normal (e.g. a terminal-value solution's ``u``), single or batched; ``u`` has shape (d,) or
+    (B, d); ``std`` is shaped as ``marginals.std[tcoeff_index]`` (a scalar isotropic, d dense)."""
 
     def loss(u, /, *, marginals, std):
-        if marginals.factorisation != "isotropic":
-            raise NotImplementedError("loss_lml_terminal_values is implemented for the isotropic factorisation")
+        if marginals.factorisation not in ("isotropic", "dense"):
+            raise NotImplementedError("loss_lml_terminal_values is implemented for the isotropic and dense factorisations")
         dev = marginals.mean_flat.device
         u_t = _as_f64(u, dev)
         std_t = _as_f64(std, dev)
@@ -856,14 +857,14 @@ def loss_lml_terminal_values(*, tcoeff_index=0):
             msg += f" For reference: data-shape={tuple(u_t.shape)}."
             raise ValueError(msg)
         N, d = marginals.num_coeffs, marginals.ode_dim
-        batched = marginals.mean_flat.dim() == 3
+        dense = marginals.factorisation == "dense"
+        batched = marginals.mean_flat.dim() == (2 if dense else 3)
         m = marginals.mean_flat if batched else marginals.mean_flat[None]
         L = marginals.cholesky_flat if batched else marginals.cholesky_flat[None]
         B = m.shape[0]
-        cfg = capi.Config(factorisation="isotropic", num_derivatives=N - 1, ode_dim=d)
-        value = capi.lml_terminal_values(
-            cfg, m.contiguous(), L.contiguous(), u_t.reshape(B, d).contiguous(), std_t.reshape(B).contiguous(), tcoeff_index
-        )
+        cfg = capi.Config(factorisation=marginals.factorisation, num_derivatives=N - 1, ode_dim=d)
+        std_c = std_t.reshape(B, d) if dense else std_t.reshape(B)
+        value = capi.lml_terminal_values(cfg, m.contiguous(), L.contiguous(), u_t.reshape(B, d).contiguous(), std_c.contiguous(), tcoeff_index)
         return value if batched else value[0]
 
     return loss

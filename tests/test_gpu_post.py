@@ -38,6 +38,31 @@ def test_lml_terminal_values_match_the_oracle(oracle, nu, idx):
         T.check_accuracy(got[b], *vals, f"terminal lml nu={nu} idx={idx} instance {b}")
 
 
+@pytest.mark.parametrize("nu,idx,d", [(3, 0, 2), (4, 2, 2), (2, 1, 3)])
+def test_dense_lml_terminal_values_match_the_oracle(oracle, nu, idx, d):
+    """The dense form: derivative idx's d coordinates observed with noise diag(std), one QR of the
+    stacked factor, the d-dimensional Gaussian log-density (the oracle pinned to scipy)."""
+    from probdiffeq_amd import capi
+
+    capi.require_device()
+    rng = np.random.default_rng(10 * nu + idx)
+    B, n = 12, (nu + 1) * d
+    mean = rng.normal(size=(B, n))
+    chol = np.tril(rng.normal(size=(B, n, n))) + 2.0 * np.eye(n)
+    u = rng.normal(size=(B, d))
+    std = rng.uniform(0.1, 1.0, size=(B, d))
+    cfg = capi.Config(factorisation="dense", num_derivatives=nu, ode_dim=d)
+    print(f"dense terminal lml case: nu={nu} idx={idx} d={d} B={B}", flush=True)
+    got = capi.lml_terminal_values(cfg, _dev(mean), _dev(chol), _dev(u), _dev(std), idx).cpu().numpy()
+    ocfg = oracle.Config(fact="dense", nu=nu, d=d, strategy="filter")
+    for b in range(B):
+        vals = []
+        for build in ("strict", "fma", "extended"):
+            with oracle.rounding(build):
+                vals.append(float(oracle.lml_terminal(ocfg, T.THETA, mean[b], chol[b].ravel(), u[b], std[b], idx)))
+        T.check_accuracy(got[b], *vals, f"dense terminal lml nu={nu} idx={idx} instance {b}")
+
+
 def test_api_lml_terminal_values_of_a_terminal_value_solve(oracle):
     import test_gpu_api as A
 
@@ -61,6 +86,33 @@ def test_api_lml_terminal_values_of_a_terminal_value_solve(oracle):
             with oracle.rounding(build):
                 vals.append(float(oracle.lml_terminal(ocfg, T.THETA, m, L, data[b].cpu().numpy(), 0.5, 0)))
         T.check_accuracy(value[b].item(), *vals, f"terminal lml api instance {b}")
+
+
+def test_api_dense_lml_terminal_values(oracle):
+    """The mirror on a dense terminal-value solve: std is shaped as marginals.std[0] (d,) per instance."""
+    import test_gpu_api as A
+
+    from probdiffeq_amd import ivpsolve
+    from probdiffeq_amd import probdiffeq as pdx
+
+    y0 = torch.tensor([[20.0, 20.0], [15.0, 25.0]], dtype=torch.float64, device="cuda")
+    init, slv, err = A._setup(pdx, "dense", "filter", "dynamic", y0)
+    sol = ivpsolve.solve_adaptive_terminal_values(solver=slv, error=err)(init, t0=0.0, t1=5.0, atol=1e-8, rtol=1e-6)
+    data = torch.tensor([[10.0, 30.0], [12.0, 28.0]], dtype=torch.float64, device="cuda")
+    std = torch.tensor([[0.5, 0.3], [0.4, 0.6]], dtype=torch.float64, device="cuda")
+    print("dense terminal lml api case: B=2", flush=True)
+    with pytest.raises(ValueError, match="standard deviation container"):
+        pdx.loss_lml_terminal_values()(data, marginals=sol.u, std=std[:, 0])
+    value = pdx.loss_lml_terminal_values()(data, marginals=sol.u, std=std)
+    ocfg = oracle.Config(fact="dense", nu=4, d=2, strategy="filter")
+    for b in range(2):
+        m = sol.u.mean_flat[b].cpu().numpy().ravel()
+        L = sol.u.cholesky_flat[b].cpu().numpy().ravel()
+        vals = []
+        for build in ("strict", "fma", "extended"):
+            with oracle.rounding(build):
+                vals.append(float(oracle.lml_terminal(ocfg, T.THETA, m, L, data[b].cpu().numpy(), std[b].cpu().numpy(), 0)))
+        T.check_accuracy(value[b].item(), *vals, f"dense terminal lml api instance {b}")
 
 
 @pytest.mark.parametrize("nu", [3, 4])
