@@ -231,12 +231,6 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
   rc = device_mats(*cfg, false, m);
   if (rc) return rc;
   const int G = pick_lanes(*cfg, L, n_inst, mode);
-  if (G != 1 && mode != pdq::MODE_FIXED_GRID && cfg->error_estimator == PDQ_ERROR_STATE_STD)
-    return set_err(PDQ_ERR_UNSUPPORTED,
-                   "error_state_std is implemented in the lane-per-instance kernel (isotropic, nu = 3 or 4, d = 2)");
-  if (G != 1 && cfg->factorisation == PDQ_ISOTROPIC && cfg->constraint == PDQ_TS1)
-    return set_err(PDQ_ERR_UNSUPPORTED,
-                   "isotropic ts1 is implemented in the lane-per-instance kernel (nu = 3 or 4, d = 2)");
   if (G == 1) {
     if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(L.N, L.d))
       return set_err(PDQ_ERR_UNSUPPORTED,

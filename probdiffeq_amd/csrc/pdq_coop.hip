@@ -625,6 +625,19 @@ __device__ void linearize(const Ctx<G>& c, const double* Xm, double t, double* b
       else
         b[q * d + j] = v;
     }
+  } else if (!c.dense) {
+    // ts1 (isotropic, IsotropicResidual.linearize, ssm_impl_isotropic.py:321-355,
+    // with jacobian_materialize's exact trace, jacobians.py:100-106): the
+    // 1 x nc operator J = [h0, 1, 0...] with h0 = -tr(J_f) / d, and
+    // b = r(m) - J m for the residual r = m_1 - f(m_0).
+    double tr = 0.0;
+    for (int j = 0; j < d; ++j) tr += -vf_jac(id, d, y, c.prm, j, j);
+    const double h0 = tr / (double)d;
+    for (int k = c.g.l; k < c.nc; k += G) J[k] = k == 0 ? h0 : (k == 1 ? 1.0 : 0.0);
+    for (int j = c.g.l; j < d; j += G) {
+      const double r = Xm[d + j] - vf_component(id, d, y, c.prm, t, j);
+      b[j] = r - (h0 * Xm[j] + Xm[d + j]);
+    }
   } else {
     // ts1 (dense): J = [-J_f, I, 0...], b = (m_1 - f(m_0)) - J m
     const int nc = c.nc;
@@ -799,6 +812,68 @@ __device__ double error_power(const Ctx<G>& c, const double* oL, double sigma, c
   return pow(norm, -1.0 / (double)c.N);
 }
 
+// The std of row r of a factor (n columns): dense per-row qr_r (|beta| of
+// dlarfg, ssm_impl_dense.py:141-150), isotropic vector_norm (:189-197).
+template <int G>
+__device__ __forceinline__ double row_std(const Ctx<G>& c, const double* L, int r, int n) {
+  double ss = 0.0;
+  for (int j = 1; j < n; ++j) ss += L[r * n + j] * L[r * n + j];
+  const double alpha = L[r * n];
+  if (c.dense) return ss == 0.0 ? fabs(alpha) : fabs(hypot(alpha, sqrt(ss)));
+  return sqrt(alpha * alpha + ss);
+}
+
+// error_state_std (solvers.py:1041-1098): condition the output-scale-1
+// extrapolation (Um, UL) on the linearisation (b, J); the errors are the
+// calibrated std of derivative idx of the result (d values dense, one
+// isotropic), written to errs.  Xm, XL, mo, oL are scratch.
+template <int G>
+__device__ void state_std_errors(const Ctx<G>& c, const double* b, const double* J, const double* Um, const double* UL,
+                                 double* Xm, double* XL, double* big, double* gain, double* mo, double* oL,
+                                 double* errs) {
+  update(c, b, J, Um, UL, Xm, XL, big, gain, mo, oL);
+  const double os = rms_zero(c, mo, oL);
+  const int idx = c.a->cfg.error_derivative_idx;
+  const int ne = c.dense ? c.d : 1;
+  for (int k = c.g.l; k < ne; k += G) errs[k] = os * row_std(c, XL, c.dense ? idx * c.d + k : idx, c.nc);
+  c.g.sync();
+}
+
+// The error power of error_state_std: errs (d values dense, one isotropic) at
+// order n = idx (+1 per unit step), referenced to derivative idx.
+template <int G>
+__device__ double error_power_state(const Ctx<G>& c, const double* errs, const double* prev_m, const double* prop_m,
+                                    double dt) {
+  const pdq_config& cfg = c.a->cfg;
+  const int d = c.d, idx = cfg.error_derivative_idx, ne = c.dense ? d : 1;
+  const int nord = idx + (cfg.error_per_unit_step ? 1 : 0);
+  const double scale = pow(dt, (double)nord) / factorial_d(nord);
+  auto ref = [&](int k) {
+    const double u0 = c.dense ? prev_m[(idx * d + k) * c.mc] : prev_m[idx * c.mc + k];
+    const double u1 = c.dense ? prop_m[(idx * d + k) * c.mc] : prop_m[idx * c.mc + k];
+    return fmax(fabs(u0), fabs(u1));
+  };
+  double norm;
+  if (cfg.error_norm == PDQ_NORM_SCALE_THEN_RMS) {
+    double ss = 0.0;
+    for (int k = 0; k < d; ++k) {
+      const double e = errs[ne == 1 ? 0 : k] * scale;
+      const double rr = e / (cfg.atol + cfg.rtol * fabs(ref(k)));
+      ss += rr * rr;
+    }
+    norm = sqrt(ss) / sqrt((double)d);
+  } else {
+    double sa = 0.0, sr = 0.0;
+    for (int k = 0; k < ne; ++k) {
+      const double e = errs[k] * scale;
+      sa += e * e;
+    }
+    for (int k = 0; k < d; ++k) sr += ref(k) * ref(k);
+    norm = (sqrt(sa) / sqrt((double)ne)) / (cfg.atol + cfg.rtol * (sqrt(sr) / sqrt((double)d)));
+  }
+  return pow(norm, -1.0 / (double)c.N);
+}
+
 // controllers.py:66-84 / :24-63
 __device__ __forceinline__ double control_apply(const pdq_config& cfg, double dt, double& cstate, double ep) {
   if (cfg.control == PDQ_CONTROL_INTEGRAL) {
@@ -857,6 +932,7 @@ __device__ double step(const Ctx<G>& c, int from, int to, double dt, bool estima
   const double tn = t + dt;
   const bool smoother = cfg.strategy != PDQ_FILTER;
   const bool ident = S.sc[SC_IDENT] != 0.0;
+  const bool state_std = cfg.error_estimator == PDQ_ERROR_STATE_STD;
   precon(c, dt, P, PI);
   c.g.sync();
   double sigma = 1.0, ep = 0.0;
@@ -868,6 +944,9 @@ __device__ double step(const Ctx<G>& c, int from, int to, double dt, bool estima
     linearize(c, Um, tn, R.b, R.J, y0buf);
     marginalise_fx(c, R.b, R.J, Um, UL, om, oL, big);
     sigma = rms_zero(c, om, oL);
+    // error_state_std from the step's (first) linearisation, before the
+    // prediction takes Pm / PL; the errors stay in om
+    if (estimate && state_std) state_std_errors(c, R.b, R.J, Um, UL, Pm, PL, big, gain, mo, oL, om);
     const double s = noise_scale(c, dt, sigma);
     if (!smoother) {
       marginalise_prior(c, P, PI, s, S.m, S.L, Pm, PL, big);
@@ -893,7 +972,7 @@ __device__ double step(const Ctx<G>& c, int from, int to, double dt, bool estima
     // solvers.py:576-582: linearise again at the calibrated prediction
     if (cfg.re_linearize_after_calibration) linearize(c, Pm, tn, R.b, R.J, y0buf);
     update(c, R.b, R.J, Pm, PL, R.m, R.L, big, gain, mo, nullptr);
-    if (estimate) ep = error_power(c, oL, sigma, S.m, R.m, dt);
+    if (estimate) ep = state_std ? error_power_state(c, om, S.m, R.m, dt) : error_power(c, oL, sigma, S.m, R.m, dt);
   } else {
     const double s1 = noise_scale(c, dt, 1.0);
     if (!smoother) {
@@ -939,9 +1018,14 @@ __device__ double step(const Ctx<G>& c, int from, int to, double dt, bool estima
       // re_linearize_before_error (solvers.py:951-956): linearise at the
       // extrapolation instead of reusing the step's linearisation
       if (cfg.error_re_linearize) linearize(c, Um, tn, R.b, R.J, y0buf);
-      marginalise_fx(c, R.b, R.J, Um, UL, om, oL, big);
-      const double se = rms_zero(c, om, oL);
-      ep = error_power(c, oL, se, S.m, R.m, dt);
+      if (state_std) {
+        state_std_errors(c, R.b, R.J, Um, UL, Pm, PL, big, gain, mo, oL, om);
+        ep = error_power_state(c, om, S.m, R.m, dt);
+      } else {
+        marginalise_fx(c, R.b, R.J, Um, UL, om, oL, big);
+        const double se = rms_zero(c, om, oL);
+        ep = error_power(c, oL, se, S.m, R.m, dt);
+      }
     }
     sigma = mle ? S.sc[SC_OS] : 1.0;
   }

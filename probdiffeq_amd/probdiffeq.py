@@ -748,7 +748,7 @@ class error_residual_std:
 class error_state_std:
     """solvers.py:999-1098 (marked highly experimental in the reference): the calibrated std of
     derivative ``derivative_idx`` of the extrapolation conditioned on the step's linearisation.
-    On the device it runs in the lane-per-instance kernel (isotropic, nu = 3 or 4, d = 2)."""
+    Both device kernels implement it (dense and isotropic factorisations)."""
 
     kind = "state_std"
 

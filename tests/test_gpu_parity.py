@@ -358,10 +358,13 @@ def test_error_estimator_variants(oracle, pdq, fact, lanes, solver, kw):
         ("filter", "solver", dict(error_re_linearize=True)),
     ],
 )
-def test_isotropic_ts1(oracle, pdq, strategy, solver, kw):
+@pytest.mark.parametrize("lanes", [0, 16])
+def test_isotropic_ts1(oracle, pdq, strategy, solver, kw, lanes):
     """Isotropic ts1 with the exact trace (ssm_impl_isotropic.py:321-355, jacobians.py:100-106) on the
-    lane kernel, against the oracle pinned by the dense == isotropic equivalence."""
-    cfg = pdq.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver=solver, constraint="ts1", rtol=1e-5, atol=1e-7, **kw)
+    lane kernel (lanes 0, the default) and the cooperative kernel (16 lanes per instance), against the
+    oracle pinned by the dense == isotropic equivalence."""
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver=solver, constraint="ts1", rtol=1e-5, atol=1e-7,
+                     lanes_per_instance=lanes, **kw)
     params, y0s, times = _lv_batch(6, seed=43, t_lo=5.0, t_hi=10.0, K=12)
     _compare(oracle, pdq, cfg, params, y0s, times)
 
@@ -374,13 +377,17 @@ def test_isotropic_ts1_van_der_pol(oracle, pdq):
     _compare(oracle, pdq, cfg, mu, np.tile([2.0, 0.0], (6, 1)), np.tile(np.linspace(0.0, 6.3, 9), (6, 1)))
 
 
-def test_isotropic_ts1_is_refused_off_the_lane_kernel(pdq):
-    """The cooperative kernel and the LML implement isotropic ts0 only."""
+@pytest.mark.parametrize("nu,strategy", [(2, "fixedpoint"), (6, "fixedinterval")])
+def test_isotropic_ts1_shapes_without_a_lane_kernel(oracle, pdq, nu, strategy):
+    """Isotropic ts1 on shapes only the cooperative kernel runs (nu = 2 and 6, d = 2)."""
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=nu, strategy=strategy, solver="dynamic", constraint="ts1", rtol=1e-5, atol=1e-7)
+    params, y0s, times = _lv_batch(4, seed=53, t_lo=5.0, t_hi=10.0, K=9)
+    _compare(oracle, pdq, cfg, params, y0s, times)
+
+
+def test_isotropic_ts1_is_refused_by_the_lml(pdq):
+    """The LML implements isotropic ts0 only."""
     params, y0s, times = _lv_batch(1, seed=1, K=5)
-    tc = pdq.taylor_coefficients("lotka_volterra", _dev(params), _dev(y0s), 0.0, 4)
-    cfg = pdq.Config(factorisation="isotropic", num_derivatives=4, strategy="filter", constraint="ts1", lanes_per_instance=16)
-    with pytest.raises(pdq.PdqError, match="isotropic ts1"):
-        pdq.solve(cfg, _dev(params), tc, _dev(times))
     cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, strategy="fixedpoint", constraint="ts1")
     grid = np.linspace(0.0, 2.0, 11)
     with pytest.raises(pdq.PdqError, match="ts0"):
@@ -443,9 +450,21 @@ def test_exponential_prior_refusals(pdq):
         pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0s), _dev(grid), 5, _dev(np.ones((1, 3, 2))), _dev(np.ones((1, 3))), grad=False)
 
 
-def test_error_state_std_is_refused_by_the_cooperative_kernel(pdq):
-    cfg = pdq.Config(factorisation="dense", num_derivatives=4, strategy="filter", error_estimator="state_std")
-    params, y0s, times = _lv_batch(1, seed=1, K=5)
-    tc = pdq.taylor_coefficients("lotka_volterra", _dev(params), _dev(y0s), 0.0, 4)
-    with pytest.raises(pdq.PdqError, match="error_state_std"):
-        pdq.solve(cfg, _dev(params), tc, _dev(times))
+@pytest.mark.parametrize(
+    "fact,solver,lanes,kw",
+    [
+        ("dense", "dynamic", 0, dict(error_derivative_idx=0)),
+        ("dense", "dynamic", 0, dict(error_derivative_idx=2, re_linearize_after_calibration=True)),
+        ("dense", "solver", 0, dict(error_derivative_idx=0, error_re_linearize=True)),
+        ("dense", "mle", 0, dict(error_derivative_idx=3, error_per_unit_step=True, error_norm="rms_then_scale")),
+        ("isotropic", "dynamic", 16, dict(error_derivative_idx=2)),
+    ],
+)
+def test_error_state_std_on_the_cooperative_kernel(oracle, pdq, fact, solver, lanes, kw):
+    """error_state_std on the cooperative kernel: dense (d calibrated stds of derivative idx) and the
+    isotropic group path, against the oracle.  Derivative 1 is left out: the ODE constraint pins it,
+    so its posterior std is rounding noise and the step sequence depends on the rounding."""
+    cfg = pdq.Config(factorisation=fact, num_derivatives=4, strategy="fixedpoint", solver=solver, rtol=1e-5, atol=1e-7,
+                     error_estimator="state_std", lanes_per_instance=lanes, **kw)
+    params, y0s, times = _lv_batch(4, seed=47, t_lo=5.0, t_hi=10.0, K=12)
+    _compare(oracle, pdq, cfg, params, y0s, times)

@@ -372,9 +372,11 @@ def test_lane_kernel_error_estimators(oracle, emu, solver, kw):
         ("filter", "dynamic", "van_der_pol", {}),
     ],
 )
-def test_lane_kernel_isotropic_ts1(oracle, emu, strategy, solver, vf, kw):
+@pytest.mark.parametrize("kernel", ["lane", "coop"])
+def test_isotropic_ts1_kernels(oracle, emu, strategy, solver, vf, kw, kernel):
     """Isotropic ts1 with the exact trace (ssm_impl_isotropic.py:321-355, jacobians.py:100-106) in
-    the lane kernel's code (the oracle pinned by the dense == isotropic equivalence)."""
+    the lane kernel's and the cooperative kernel's code (the oracle pinned by the dense == isotropic
+    equivalence)."""
     from probdiffeq_amd import capi
 
     if vf == "van_der_pol":
@@ -387,7 +389,7 @@ def test_lane_kernel_isotropic_ts1(oracle, emu, strategy, solver, vf, kw):
         p, y, t = T._lv_batch(3, seed=6, t_lo=4.0, t_hi=8.0, K=10)
         cfg = capi.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver=solver, constraint="ts1",
                           rtol=1e-5, atol=1e-7, **kw)
-    _run(oracle, emu, cfg, p, y, t, kernel="lane")
+    _run(oracle, emu, cfg, p, y, t, kernel=kernel)
 
 
 @pytest.mark.parametrize(
@@ -461,3 +463,25 @@ def test_generic_sampler_code(oracle, emu, fact, nu, strategy, kw):
 
     cfg, sol, base, refs = SC.make_case(oracle, nu, fact=fact, strategy=strategy, **kw)
     SC.check(emu.sample(cfg, sol, base), refs, f"emulation generic sample {fact}/{strategy} nu={nu}")
+
+
+@pytest.mark.parametrize(
+    "fact,solver,kw",
+    [
+        ("dense", "dynamic", dict(error_derivative_idx=0)),
+        ("dense", "dynamic", dict(error_derivative_idx=2, re_linearize_after_calibration=True)),
+        ("dense", "solver", dict(error_derivative_idx=0, error_re_linearize=True)),
+        ("dense", "mle", dict(error_derivative_idx=3, error_per_unit_step=True, error_norm="rms_then_scale")),
+        ("isotropic", "dynamic", dict(error_derivative_idx=2)),
+    ],
+)
+def test_cooperative_kernel_error_state_std(oracle, emu, fact, solver, kw):
+    """error_state_std (solvers.py:999-1098) in the cooperative kernel's code: dense (d calibrated
+    stds of derivative idx) and isotropic, against the oracle.  Derivative 1 is left out: the ODE
+    constraint pins it, so its posterior std is rounding noise."""
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation=fact, num_derivatives=4, strategy="fixedpoint", solver=solver, rtol=1e-5, atol=1e-7,
+                      error_estimator="state_std", **kw)
+    p, y, t = T._lv_batch(3, seed=13, t_lo=4.0, t_hi=8.0, K=10)
+    _run(oracle, emu, cfg, p, y, t)
