@@ -114,6 +114,10 @@ typedef struct pdq_config {
    * dt0_adaptive, stepsize_initialisers.py:7-67); NULL: dt0 for every
    * instance. */
   const double* dt0_per_instance;
+  /* loss_lml_timeseries(tcoeff_index=) (estimators_and_losses.py:53-105):
+   * the Taylor coefficient the LML's data observe, in [0, num_derivatives];
+   * read by pdq_lml_value_and_grad only (0: the solution itself). */
+  int32_t lml_tcoeff_index;
 } pdq_config;
 
 /* Caller-allocated device outputs; NULL members are not written.

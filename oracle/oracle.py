@@ -72,6 +72,7 @@ def _cfg_struct(ct):
             ("error_per_unit_step", ctypes.c_int),
             ("prior", ctypes.c_int),
             ("prior_drift", ctypes.POINTER(ct)),
+            ("tcoeff_index", ctypes.c_int),
         ]
 
     return _Cfg
@@ -162,6 +163,8 @@ class Config:
     # d x (nu+1) d, derivative-major columns (ssm_impl_dense.py:541-629)
     prior: str = "iwp"
     prior_drift: object = None
+    # loss_lml_timeseries(tcoeff_index=): the Taylor coefficient the LML's data observe
+    tcoeff_index: int = 0
 
     def c(self):
         if self.prior not in ("iwp", "exponential"):
@@ -173,6 +176,9 @@ class Config:
             drift = np.ascontiguousarray(np.asarray(self.prior_drift, dtype=np.float64).reshape(self.d, self.N * self.d), dtype=_dtype())
         c = self._c()
         c.prior = 1 if drift is not None else 0
+        if not 0 <= self.tcoeff_index <= self.nu:
+            raise ValueError("tcoeff_index must lie in [0, nu]")
+        c.tcoeff_index = int(self.tcoeff_index)
         if drift is not None:
             c.prior_drift = drift.ctypes.data_as(ctypes.POINTER(_ct()))
             c._keep = drift  # the array lives as long as the struct

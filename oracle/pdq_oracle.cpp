@@ -667,6 +667,8 @@ struct pdqo_cfg {
   // columns) is prior_drift
   int prior;
   const Real* prior_drift;
+  // loss_lml_timeseries(tcoeff_index=): the Taylor coefficient the data observe
+  int tcoeff_index;
 };
 }
 
@@ -1673,16 +1675,18 @@ void finalize(const Problem& P, const Outputs& o, Final& F) {
   }
 }
 
-// to_derivative(0, std) observation model (ssm_impl_dense.py:206-217,
-// ssm_impl_isotropic.py:271-278, ssm_impl_blockdiag.py:367-377).
+// The observation model marginal.to_derivative(i, std) (ssm_impl_dense.py:
+// 206-217, ssm_impl_isotropic.py:271-278, ssm_impl_blockdiag.py:367-377) with
+// i = cfg.tcoeff_index: dense selects rows i d + k of the derivative-major
+// state, isotropic and blockdiag row i.
 BCond observation_model(const Problem& P, const Real* std_j, int nstd) {
   BCond out;
   out.b.resize(P.nblk);
-  const int d = P.d, N = P.N;
+  const int d = P.d, N = P.N, ix = P.cfg.tcoeff_index;
   if (P.cfg.fact == FACT_DENSE) {
     Cond& C = out.b[0];
     C.A = Mat(d, P.n);
-    for (int k = 0; k < d; ++k) C.A(k, k) = 1.0;
+    for (int k = 0; k < d; ++k) C.A(k, ix * d + k) = 1.0;
     C.noise.m = Mat(d, 1);
     C.noise.L = Mat(d, d);
     for (int k = 0; k < d; ++k) C.noise.L(k, k) = std_j[nstd == 1 ? 0 : k];
@@ -1691,7 +1695,7 @@ BCond observation_model(const Problem& P, const Real* std_j, int nstd) {
   } else if (P.cfg.fact == FACT_ISO) {
     Cond& C = out.b[0];
     C.A = Mat(1, N);
-    C.A(0, 0) = 1.0;
+    C.A(0, ix) = 1.0;
     C.noise.m = Mat(1, d);
     C.noise.L = Mat(1, 1);
     C.noise.L(0, 0) = std_j[0];
@@ -1701,7 +1705,7 @@ BCond observation_model(const Problem& P, const Real* std_j, int nstd) {
     for (int k = 0; k < d; ++k) {
       Cond& C = out.b[k];
       C.A = Mat(1, N);
-      C.A(0, 0) = 1.0;
+      C.A(0, ix) = 1.0;
       C.noise.m = Mat(1, 1);
       C.noise.L = Mat(1, 1);
       C.noise.L(0, 0) = std_j[nstd == 1 ? 0 : k];

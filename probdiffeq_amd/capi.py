@@ -72,6 +72,7 @@ class _Config(ctypes.Structure):
         ("prior", ctypes.c_int32),
         ("prior_drift", ctypes.c_void_p),
         ("dt0_per_instance", ctypes.c_void_p),
+        ("lml_tcoeff_index", ctypes.c_int32),
     ]
 
 
@@ -145,9 +146,12 @@ class Config:
     # optional initial step per instance: an (n_inst,) float64 device tensor (host array for the
     # CPU build of the kernels); None: dt0 for every instance
     dt0_per_instance: Any = None
+    # loss_lml_timeseries(tcoeff_index=): the Taylor coefficient the LML's data observe
+    lml_tcoeff_index: int = 0
 
     def c(self) -> _Config:
         c = self._c()
+        c.lml_tcoeff_index = int(self.lml_tcoeff_index)
         if self.prior not in PRIOR:
             raise ValueError(f"unknown prior {self.prior!r}")
         c.prior = PRIOR[self.prior]

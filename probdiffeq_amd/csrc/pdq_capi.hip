@@ -362,11 +362,15 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
     return set_err(PDQ_ERR_INVALID, "isotropic observation noise is one scalar per time point (n_std = 1)");
   if ((cfg->num_derivatives + 1) * cfg->ode_dim > 64) return set_err(PDQ_ERR_UNSUPPORTED, "Taylor init too large");
   const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
-  const bool lane_ok = cfg->factorisation == PDQ_ISOTROPIC && pdq::lane_lml_supported(N, d, cfg->solver);
+  if (cfg->lml_tcoeff_index < 0 || cfg->lml_tcoeff_index > cfg->num_derivatives)
+    return set_err(PDQ_ERR_INVALID, "lml_tcoeff_index (%d) must lie in [0, num_derivatives]", cfg->lml_tcoeff_index);
+  // the lane LML's QR is specialised to observations of coefficient 0
+  const bool lane_ok = cfg->factorisation == PDQ_ISOTROPIC && cfg->lml_tcoeff_index == 0 &&
+                       pdq::lane_lml_supported(N, d, cfg->solver);
   if (cfg->lanes_per_instance == 1 && !lane_ok)
     return set_err(PDQ_ERR_UNSUPPORTED,
                    "lanes_per_instance = 1 (lane-per-instance LML) needs the isotropic factorisation with nu = 3 or "
-                   "4, d = 2 and solver or solver_dynamic (got nu = %d, d = %d)",
+                   "4, d = 2, solver or solver_dynamic and lml_tcoeff_index = 0 (got nu = %d, d = %d)",
                    N - 1, d);
   if (n_inst == 0) return PDQ_OK;
   if (!params || !y0 || !grid || !data || !stds) return set_err(PDQ_ERR_INVALID, "null input pointer");

@@ -1399,20 +1399,23 @@ __device__ void lml_instance(const Ctx<G>& c, int inst) {
       g_copy(c.g, XL, YL, nc * nc);
       c.g.sync();
     }
-    // model_j = marginal.to_derivative(0, std_j); bayes_rule_and_logpdf_tree
+    // model_j = marginal.to_derivative(i, std_j) (ssm_impl_dense.py:206-217,
+    // ssm_impl_isotropic.py:271-278) with i = lml_tcoeff_index: the observed
+    // rows are i d + k (dense) and i (isotropic), i.e. ro + o with ro = i ko;
+    // bayes_rule_and_logpdf_tree
     const double* y = a.data + ((size_t)inst * Kobs + j) * d;
     const double* sd = a.stds + ((size_t)inst * Kobs + j) * a.n_std;
-    const int n2 = ko + nc;
+    const int n2 = ko + nc, ro = cfg.lml_tcoeff_index * ko;
     for (int e = c.g.l; e < n2 * n2; e += G) {
       const int r = e / n2, col = e % n2;
       double v;
       if (r < ko) v = col < ko ? (r == col ? sd[a.n_std == 1 ? 0 : r] : 0.0) : 0.0;
-      else v = col < ko ? XL[col * nc + (r - ko)] : XL[(col - ko) * nc + (r - ko)];
+      else v = col < ko ? XL[(ro + col) * nc + (r - ko)] : XL[(col - ko) * nc + (r - ko)];
       big[e] = v;
     }
     for (int e = c.g.l; e < ko * mc; e += G) {
       const int o = e / mc, col = e % mc;
-      mo[e] = Xm[o * mc + col] + 0.0;
+      mo[e] = Xm[(ro + o) * mc + col] + 0.0;
     }
     c.g.sync();
     g_qr_r(c.g, big, n2, n2, n2);

@@ -880,13 +880,14 @@ def loss_lml_timeseries(*, average_pdfs: bool = True, tcoeff_index=0, solve_triu
     gradient with respect to the vector-field parameters and the initial
     value comes from the same launch (forward-mode tangents).
     """
-    if tcoeff_index != 0:
-        raise NotImplementedError("observations of the zeroth Taylor coefficient only")
+    tcoeff_index = int(tcoeff_index)
 
     def loss(u, /, *, problem, params, y0, std):
         from probdiffeq_amd.torch_ops import lml
 
-        return lml(problem, u, std, params=params, y0=y0, average_pdfs=average_pdfs)
+        if not 0 <= tcoeff_index <= problem.num_derivatives:
+            raise ValueError(f"tcoeff_index must lie in [0, {problem.num_derivatives}]")
+        return lml(problem, u, std, params=params, y0=y0, average_pdfs=average_pdfs, tcoeff_index=tcoeff_index)
 
     return loss
 

@@ -56,8 +56,9 @@ class LML(torch.autograd.Function):
     """value = loss_lml_timeseries(data, posterior=solve(params, y0), std)."""
 
     @staticmethod
-    def forward(ctx, params, y0, data, stds, problem: LMLProblem, average: bool):
+    def forward(ctx, params, y0, data, stds, problem: LMLProblem, average: bool, tcoeff_index: int = 0):
         cfg = problem.config(params.shape[-1])
+        cfg.lml_tcoeff_index = int(tcoeff_index)
         need_grad = params.requires_grad or y0.requires_grad
         value, grad = capi.lml_value_and_grad(
             cfg,
@@ -78,12 +79,13 @@ class LML(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         if not ctx.has_grad:
-            return None, None, None, None, None, None
+            return None, None, None, None, None, None, None
         (grad,) = ctx.saved_tensors
         full = grad * g[:, None]
-        return full[:, : ctx.n_params], full[:, ctx.n_params :], None, None, None, None
+        return full[:, : ctx.n_params], full[:, ctx.n_params :], None, None, None, None, None
 
 
-def lml(problem: LMLProblem, data, std, *, params, y0, average_pdfs=True):
-    """Batched LML (one value per instance) with autograd support."""
-    return LML.apply(params, y0, data, std, problem, average_pdfs)
+def lml(problem: LMLProblem, data, std, *, params, y0, average_pdfs=True, tcoeff_index=0):
+    """Batched LML (one value per instance) with autograd support; the data observe Taylor
+    coefficient ``tcoeff_index``."""
+    return LML.apply(params, y0, data, std, problem, average_pdfs, tcoeff_index)

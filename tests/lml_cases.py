@@ -49,7 +49,7 @@ def oracle_value(oracle, ocfg, params, y0, grid, every, data, stds, average=True
     return v, vf, vt
 
 
-def richardson_grad(oracle, ocfg, params, y0, grid, every, data, stds, average=True):
+def richardson_grad(oracle, ocfg, params, y0, grid, every, data, stds, average=True, step=1e-4):
     """Richardson-extrapolated central differences (steps h and 2h) of the oracle's LML, evaluated
     in the long double build: its values carry about 19 digits, so a step of 1e-4 |x| keeps both the
     rounding error and the O(h^4) truncation error far below the 1e-10 tolerance the kernels are held to.
@@ -74,13 +74,22 @@ def richardson_grad(oracle, ocfg, params, y0, grid, every, data, stds, average=T
     g = np.zeros(x.size, dtype=np.longdouble)
     with oracle.rounding("extended"):
         for j in range(x.size):
-            h = np.longdouble(1e-4) * max(abs(x[j]), np.longdouble(1e-3))
+            h = np.longdouble(step) * max(abs(x[j]), np.longdouble(1e-3))
             e = np.zeros(x.size, dtype=np.longdouble)
             e[j] = h
             d1 = (f(x + e) - f(x - e)) / (2 * h)
             d2 = (f(x + 2 * e) - f(x - 2 * e)) / (4 * h)
             g[j] = (4 * d1 - d2) / 3
     return g.astype(np.float64)
+
+
+def grad_reference(oracle, ocfg, params, y0, grid, every, data, stds, average=True):
+    """(Richardson gradient at step 1e-4, its tolerance): the tolerance is GRAD_RTOL or, where the
+    oracle's LML is less well conditioned (data observing a high Taylor coefficient), ten times the
+    spread between the steps 1e-4 and 1e-3, the finite differences' own measured uncertainty."""
+    g = richardson_grad(oracle, ocfg, params, y0, grid, every, data, stds, average)
+    g3 = richardson_grad(oracle, ocfg, params, y0, grid, every, data, stds, average, step=1e-3)
+    return g, max(GRAD_RTOL, 10.0 * rel(g, g3))
 
 
 def rel(a, b):

@@ -30,7 +30,7 @@ def _dev(x):
     return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64).cuda()
 
 
-def _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, average=True, grad=True):
+def _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, average=True, grad=True, fd_spread_tol=False):
     ocfg = T._oracle_cfg(oracle, cfg)
     print(f"lml case: fact={cfg.factorisation} solver={cfg.solver} lanes={cfg.lanes_per_instance} B={params.shape[0]} K={grid.size}", flush=True)
     v, g = pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds), average=average, grad=grad)
@@ -40,7 +40,11 @@ def _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, average=True, 
     for i in range(params.shape[0]):
         vo, vf, vt = LC.oracle_value(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
         T.check_accuracy(v[i], vo, vf, vt, f"lml {cfg.factorisation}/{cfg.solver} instance {i}")
-        if grad:
+        if grad and fd_spread_tol:
+            gr, tol = LC.grad_reference(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
+            assert tol < 1e-8
+            assert LC.rel(g[i], gr) < tol, (i, g[i], gr, tol)
+        elif grad:
             gr = LC.richardson_grad(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
             assert LC.rel(g[i], gr) < LC.GRAD_RTOL, (i, g[i], gr)
     return v, g
@@ -61,6 +65,28 @@ def test_lml_variants(oracle, pdq, fact, solver, per_coordinate_std, average):
     params, y0, grid, every, data, stds = LC.lml_inputs(4, seed=5, t_end=10.0, n_grid=1001, every=10, per_coordinate_std=per_coordinate_std)
     cfg = pdq.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver=solver)
     _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, average=average)
+
+
+@pytest.mark.parametrize("fact,idx,lanes,grad", [("isotropic", 1, 0, True), ("isotropic", 2, 16, True), ("dense", 2, 0, True), ("dense", 3, 0, False)])
+def test_lml_observing_a_derivative(oracle, pdq, fact, idx, lanes, grad):
+    """loss_lml_timeseries(tcoeff_index=idx): the data observe Taylor coefficient idx.  Isotropic
+    nu = 3, d = 2 with lanes 0 would take the lane LML for idx = 0; idx > 0 runs the cooperative one.
+    idx = nu: value only (finite differences of the oracle cannot pin that gradient to 1e-10)."""
+    params, y0, grid, every, data, stds = LC.lml_inputs(4, seed=11, t_end=10.0, n_grid=1001, every=10)
+    cfg = pdq.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="dynamic",
+                     lanes_per_instance=lanes, lml_tcoeff_index=idx)
+    _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=grad, fd_spread_tol=True)
+
+
+def test_lml_tcoeff_index_is_checked(pdq):
+    params, y0, grid, every, data, stds = LC.lml_inputs(1, seed=1, t_end=2.0, n_grid=11, every=5)
+    for idx in (-1, 4):
+        cfg = pdq.Config(factorisation="dense", num_derivatives=3, ode_dim=2, strategy="fixedpoint", lml_tcoeff_index=idx)
+        with pytest.raises(pdq.PdqError, match="lml_tcoeff_index"):
+            pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds), grad=False)
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", lanes_per_instance=1, lml_tcoeff_index=1)
+    with pytest.raises(pdq.PdqError, match="lml_tcoeff_index = 0"):
+        pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds), grad=False)
 
 
 def test_lml_single_instance_value_only(oracle, pdq):
@@ -117,3 +143,29 @@ def test_lane_lml_refuses_other_shapes(pdq):
     cfg = pdq.Config(factorisation="dense", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver", lanes_per_instance=1)
     with pytest.raises(pdq.PdqError, match="lane-per-instance LML"):
         pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds))
+
+
+def test_mirror_loss_lml_timeseries_observing_a_derivative(pdq):
+    """probdiffeq.loss_lml_timeseries(tcoeff_index=1) through the torch op: value and autograd
+    gradient equal the C ABI's with lml_tcoeff_index = 1, and differ from tcoeff_index = 0."""
+    from probdiffeq_amd import probdiffeq as pd
+    from probdiffeq_amd.torch_ops import LMLProblem
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(3, seed=12, t_end=5.0, n_grid=251, every=10)
+    problem = LMLProblem(vector_field="lotka_volterra", num_derivatives=3, ode_dim=2, grid=_dev(grid), every=every,
+                         factorisation="dense", solver="dynamic")
+    p = _dev(params).requires_grad_(True)
+    y = _dev(y0).requires_grad_(True)
+    loss = pd.loss_lml_timeseries(tcoeff_index=1)
+    v = loss(_dev(data), problem=problem, params=p, y0=y, std=_dev(stds))
+    v.sum().backward()
+    cfg = problem.config(params.shape[1])
+    cfg.lml_tcoeff_index = 1
+    vr, gr = pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds))
+    torch.cuda.synchronize()
+    assert torch.equal(v.detach(), vr)
+    assert torch.equal(torch.cat([p.grad, y.grad], dim=1), gr)
+    v0 = pd.loss_lml_timeseries(tcoeff_index=0)(_dev(data), problem=problem, params=p.detach(), y0=y.detach(), std=_dev(stds))
+    assert torch.all((v0 - v.detach()).abs() > 1.0)
+    with pytest.raises(ValueError, match="tcoeff_index"):
+        pd.loss_lml_timeseries(tcoeff_index=4)(_dev(data), problem=problem, params=p, y0=y, std=_dev(stds))

@@ -111,6 +111,7 @@ def _oracle_cfg(oracle, cfg):
         error_per_unit_step=cfg.error_per_unit_step,
         prior=cfg.prior,
         prior_drift=None if cfg.prior_drift is None else np.asarray(cfg.prior_drift.cpu() if hasattr(cfg.prior_drift, "cpu") else cfg.prior_drift),
+        tcoeff_index=cfg.lml_tcoeff_index,
     )
 
 

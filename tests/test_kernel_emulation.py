@@ -133,6 +133,32 @@ def test_lml_value_and_gradient(oracle, emu, fact, solver, per_coordinate_std, a
         assert LC.rel(g[i], gr) < LC.GRAD_RTOL, (g[i], gr)
 
 
+@pytest.mark.parametrize("fact,idx,grad", [("isotropic", 1, True), ("isotropic", 2, True), ("isotropic", 3, False), ("dense", 1, True), ("dense", 2, True), ("dense", 3, False)])
+def test_lml_observing_a_derivative(oracle, emu, fact, idx, grad):
+    """loss_lml_timeseries(tcoeff_index=idx) in the cooperative kernel's code: the data observe Taylor
+    coefficient idx (to_derivative, ssm_impl_dense.py:206-217, ssm_impl_isotropic.py:271-278); value
+    against the oracle, forward-mode gradient against Richardson FD.  For idx = nu the long double
+    oracle's value moves by ~1e-13 relative under perturbations (the highest coefficient is the least
+    well conditioned), so finite differences cannot pin the gradient to 1e-10: value only."""
+    import lml_cases as LC
+    from probdiffeq_amd import capi
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(2, seed=9, t_end=10.0, n_grid=501, every=10)
+    cfg = capi.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="dynamic", lml_tcoeff_index=idx)
+    ocfg = T._oracle_cfg(oracle, cfg)
+    assert ocfg.tcoeff_index == idx
+    v, g = emu.lml(cfg, params, y0, grid, every, data, stds)
+    v0, _ = emu.lml(capi.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="dynamic"), params, y0, grid, every, data, stds, grad=False)
+    assert np.all(np.abs(v - v0) > 1.0)  # another coefficient observed: another likelihood
+    for i in range(2):
+        vo, vf, vt = LC.oracle_value(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i])
+        T.check_accuracy(v[i], vo, vf, vt, f"emulation lml tcoeff {idx} instance {i}")
+        if grad:
+            gr, tol = LC.grad_reference(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i])
+            assert tol < 1e-8
+            assert LC.rel(g[i], gr) < tol, (g[i], gr, tol)
+
+
 # ----------------------------------------------- lane-per-instance kernel
 @pytest.mark.parametrize("strategy", ["filter", "fixedpoint", "fixedinterval"])
 @pytest.mark.parametrize("solver", ["solver", "dynamic", "mle"])

@@ -249,8 +249,9 @@ def test_status_reports_max_attempts_and_nonfinite(oracle):
     assert r["status"] == 2
 
 
-def _lml_bruteforce(oracle, cfg, r, data, stds):
-    """log p(y_0..y_K) from the Markov sequence, assembled as one dense Gaussian."""
+def _lml_bruteforce(oracle, cfg, r, data, stds, idx=0):
+    """log p(y_0..y_K) from the Markov sequence, assembled as one dense Gaussian; the data observe
+    Taylor coefficient ``idx``."""
     K = len(r["t"])
     n = cfg.n
     d = cfg.d
@@ -283,13 +284,13 @@ def _lml_bruteforce(oracle, cfg, r, data, stds):
             H = np.zeros((K * d, dim))
             for j in range(K):
                 for k in range(d):
-                    H[j * d + k, j * n + k] = 1.0
+                    H[j * d + k, j * n + idx * d + k] = 1.0
             y = data.reshape(-1)
             R = np.diag(np.repeat(stds, d) ** 2) if np.ndim(stds) == 1 else np.diag(stds.reshape(-1) ** 2)
         else:
             H = np.zeros((K, dim))
             for j in range(K):
-                H[j, j * n] = 1.0
+                H[j, j * n + idx] = 1.0
             y = data[:, col]
             R = np.diag(stds**2)
         cov = H @ Mfull @ Mfull.T @ H.T + R
@@ -299,12 +300,15 @@ def _lml_bruteforce(oracle, cfg, r, data, stds):
     return total
 
 
+@pytest.mark.parametrize("idx", [0, 1, 2])
 @pytest.mark.parametrize("fact", ["isotropic", "dense"])
-def test_lml_equals_bruteforce_joint_density(oracle, fact):
+def test_lml_equals_bruteforce_joint_density(oracle, fact, idx):
+    """loss_lml_timeseries(tcoeff_index=idx) (estimators_and_losses.py:53-105) equals the joint
+    Gaussian density of the data under the Markov sequence, with H selecting coefficient idx."""
     tc = oracle.taylor_coefficients("lotka_volterra", THETA, [20.0, 20.0], 0.0, 2)
     grid = np.linspace(0.0, 2.0, 41)
     every = 10
-    cfg = oracle.Config(fact=fact, nu=2, strategy="fixedpoint", solver="solver")
+    cfg = oracle.Config(fact=fact, nu=2, strategy="fixedpoint", solver="solver", tcoeff_index=idx)
     r = oracle.solve_fixed_grid_checkpoints(cfg, THETA, tc, grid, every)
     K = len(r["t"])
     ref = lv_reference(grid[::every])
@@ -313,7 +317,7 @@ def test_lml_equals_bruteforce_joint_density(oracle, fact):
     stds = np.full(K, 0.3)
     lml_sum = oracle.lml(cfg, THETA, tc, grid, every, data, stds, average=False)
     lml_avg = oracle.lml(cfg, THETA, tc, grid, every, data, stds, average=True)
-    brute = _lml_bruteforce(oracle, cfg, r, data, stds)
+    brute = _lml_bruteforce(oracle, cfg, r, data, stds, idx)
     assert np.isclose(lml_sum, brute, rtol=1e-8)
     assert np.isclose(lml_avg * K, lml_sum, rtol=1e-12)
 
