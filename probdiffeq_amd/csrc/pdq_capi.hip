@@ -347,8 +347,6 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
     return set_err(PDQ_ERR_UNSUPPORTED, "the LML needs strategy_smoother_fixedpoint (a Markov sequence)");
   if (cfg->constraint != PDQ_TS0 && grad)
     return set_err(PDQ_ERR_UNSUPPORTED, "gradients need ts0 (the covariance recursion must not depend on them)");
-  if (cfg->constraint != PDQ_TS0 && cfg->factorisation == PDQ_ISOTROPIC)
-    return set_err(PDQ_ERR_UNSUPPORTED, "the isotropic LML is implemented for ts0");
   if (cfg->prior != PDQ_PRIOR_IWP)
     return set_err(PDQ_ERR_UNSUPPORTED, "the LML is implemented for the integrated Wiener process prior");
   if (cfg->solver == PDQ_SOLVER_MLE) return set_err(PDQ_ERR_UNSUPPORTED, "solver_mle is not supported by the LML path");
@@ -364,13 +362,14 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
   const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
   if (cfg->lml_tcoeff_index < 0 || cfg->lml_tcoeff_index > cfg->num_derivatives)
     return set_err(PDQ_ERR_INVALID, "lml_tcoeff_index (%d) must lie in [0, num_derivatives]", cfg->lml_tcoeff_index);
-  // the lane LML's QR is specialised to observations of coefficient 0
+  // the lane LML's QR is specialised to observations of coefficient 0 and its
+  // forward pass to ts0; the other cases run the cooperative LML
   const bool lane_ok = cfg->factorisation == PDQ_ISOTROPIC && cfg->lml_tcoeff_index == 0 &&
-                       pdq::lane_lml_supported(N, d, cfg->solver);
+                       cfg->constraint == PDQ_TS0 && pdq::lane_lml_supported(N, d, cfg->solver);
   if (cfg->lanes_per_instance == 1 && !lane_ok)
     return set_err(PDQ_ERR_UNSUPPORTED,
                    "lanes_per_instance = 1 (lane-per-instance LML) needs the isotropic factorisation with nu = 3 or "
-                   "4, d = 2, solver or solver_dynamic and lml_tcoeff_index = 0 (got nu = %d, d = %d)",
+                   "4, d = 2, ts0, solver or solver_dynamic and lml_tcoeff_index = 0 (got nu = %d, d = %d)",
                    N - 1, d);
   if (n_inst == 0) return PDQ_OK;
   if (!params || !y0 || !grid || !data || !stds) return set_err(PDQ_ERR_INVALID, "null input pointer");

@@ -78,6 +78,15 @@ def test_lml_observing_a_derivative(oracle, pdq, fact, idx, lanes, grad):
     _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=grad, fd_spread_tol=True)
 
 
+@pytest.mark.parametrize("fact,lanes", [("isotropic", 0), ("isotropic", 16), ("dense", 0)])
+def test_lml_value_with_ts1(oracle, pdq, fact, lanes):
+    """The LML value with ts1 (isotropic: the exact trace) runs the cooperative LML; the lane LML is ts0 only."""
+    params, y0, grid, every, data, stds = LC.lml_inputs(4, seed=13, t_end=10.0, n_grid=1001, every=10)
+    cfg = pdq.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="dynamic",
+                     constraint="ts1", lanes_per_instance=lanes)
+    _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=False)
+
+
 def test_lml_tcoeff_index_is_checked(pdq):
     params, y0, grid, every, data, stds = LC.lml_inputs(1, seed=1, t_end=2.0, n_grid=11, every=5)
     for idx in (-1, 4):

@@ -386,13 +386,13 @@ def test_isotropic_ts1_shapes_without_a_lane_kernel(oracle, pdq, nu, strategy):
     _compare(oracle, pdq, cfg, params, y0s, times)
 
 
-def test_isotropic_ts1_is_refused_by_the_lml(pdq):
-    """The LML implements isotropic ts0 only."""
+def test_ts1_lml_gradients_are_refused(pdq):
+    """LML gradients need ts0 (the forward-mode tangents share the covariance recursion); ts1 values run."""
     params, y0s, times = _lv_batch(1, seed=1, K=5)
     cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, strategy="fixedpoint", constraint="ts1")
     grid = np.linspace(0.0, 2.0, 11)
     with pytest.raises(pdq.PdqError, match="ts0"):
-        pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0s), _dev(grid), 5, _dev(np.ones((1, 3, 2))), _dev(np.ones((1, 3))), grad=False)
+        pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0s), _dev(grid), 5, _dev(np.ones((1, 3, 2))), _dev(np.ones((1, 3))), grad=True)
 
 
 def _matern_drift(length_scale, N, d):

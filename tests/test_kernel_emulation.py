@@ -159,6 +159,21 @@ def test_lml_observing_a_derivative(oracle, emu, fact, idx, grad):
             assert LC.rel(g[i], gr) < tol, (g[i], gr, tol)
 
 
+@pytest.mark.parametrize("fact", ["isotropic", "dense"])
+def test_lml_value_with_ts1(oracle, emu, fact):
+    """The cooperative LML's value with ts1 (isotropic: the exact trace) against the oracle."""
+    import lml_cases as LC
+    from probdiffeq_amd import capi
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(2, seed=14, t_end=10.0, n_grid=501, every=10)
+    cfg = capi.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="dynamic", constraint="ts1")
+    ocfg = T._oracle_cfg(oracle, cfg)
+    v, _ = emu.lml(cfg, params, y0, grid, every, data, stds, grad=False)
+    for i in range(2):
+        vo, vf, vt = LC.oracle_value(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i])
+        T.check_accuracy(v[i], vo, vf, vt, f"emulation lml ts1 instance {i}")
+
+
 # ----------------------------------------------- lane-per-instance kernel
 @pytest.mark.parametrize("strategy", ["filter", "fixedpoint", "fixedinterval"])
 @pytest.mark.parametrize("solver", ["solver", "dynamic", "mle"])
