@@ -118,6 +118,12 @@ typedef struct pdq_config {
    * the Taylor coefficient the LML's data observe, in [0, num_derivatives];
    * read by pdq_lml_value_and_grad only (0: the solution itself). */
   int32_t lml_tcoeff_index;
+  /* Optional initial standard deviation per covariance row (device, n
+   * doubles: dense (nu+1) d derivative-major, isotropic nu+1), shared by the
+   * instances: prior_wiener_integrated's element-wise is_exact and
+   * diffuse_derivatives (ssm_impl_isotropic.py:409-447, ssm_impl_dense.py:
+   * 458-500).  NULL: init_std on every row. */
+  const double* init_std_rows;
 } pdq_config;
 
 /* Caller-allocated device outputs; NULL members are not written.

@@ -73,6 +73,7 @@ def _cfg_struct(ct):
             ("prior", ctypes.c_int),
             ("prior_drift", ctypes.POINTER(ct)),
             ("tcoeff_index", ctypes.c_int),
+            ("init_std_rows", ctypes.POINTER(ct)),
         ]
 
     return _Cfg
@@ -165,6 +166,8 @@ class Config:
     prior_drift: object = None
     # loss_lml_timeseries(tcoeff_index=): the Taylor coefficient the LML's data observe
     tcoeff_index: int = 0
+    # optional initial std per covariance row (n values; dense (nu+1) d, isotropic nu+1)
+    init_std_rows: object = None
 
     def c(self):
         if self.prior not in ("iwp", "exponential"):
@@ -179,6 +182,12 @@ class Config:
         if not 0 <= self.tcoeff_index <= self.nu:
             raise ValueError("tcoeff_index must lie in [0, nu]")
         c.tcoeff_index = int(self.tcoeff_index)
+        if self.init_std_rows is not None:
+            rows = np.ascontiguousarray(np.asarray(self.init_std_rows, dtype=np.float64), dtype=_dtype())
+            if rows.shape != (self.n,):
+                raise ValueError(f"init_std_rows must have shape ({self.n},)")
+            c.init_std_rows = rows.ctypes.data_as(ctypes.POINTER(_ct()))
+            c._keep_rows = rows
         if drift is not None:
             c.prior_drift = drift.ctypes.data_as(ctypes.POINTER(_ct()))
             c._keep = drift  # the array lives as long as the struct

@@ -669,6 +669,10 @@ struct pdqo_cfg {
   const Real* prior_drift;
   // loss_lml_timeseries(tcoeff_index=): the Taylor coefficient the data observe
   int tcoeff_index;
+  // from_mean_and_std with a std per covariance row (element-wise is_exact,
+  // diffuse_derivatives; ssm_impl_isotropic.py:409-447): n values, or NULL
+  // for the scalar init_std on every row
+  const Real* init_std_rows;
 };
 }
 
@@ -1778,14 +1782,14 @@ BNormal prior_init(const Problem& P, const Real* tc, Real init_std) {
     x.L = Mat(P.n, P.n);
     for (int i = 0; i < N; ++i)
       for (int k = 0; k < d; ++k) x.m(i * d + k, 0) = tc[i * d + k];
-    for (int i = 0; i < P.n; ++i) x.L(i, i) = init_std;
+    for (int i = 0; i < P.n; ++i) x.L(i, i) = P.cfg.init_std_rows ? P.cfg.init_std_rows[i] : init_std;
   } else if (P.cfg.fact == FACT_ISO) {
     Normal& x = out.b[0];
     x.m = Mat(N, d);
     x.L = Mat(N, N);
     for (int i = 0; i < N; ++i)
       for (int k = 0; k < d; ++k) x.m(i, k) = tc[i * d + k];
-    for (int i = 0; i < N; ++i) x.L(i, i) = init_std;
+    for (int i = 0; i < N; ++i) x.L(i, i) = P.cfg.init_std_rows ? P.cfg.init_std_rows[i] : init_std;
   } else {
     for (int k = 0; k < d; ++k) {
       Normal& x = out.b[k];

@@ -73,6 +73,7 @@ class _Config(ctypes.Structure):
         ("prior_drift", ctypes.c_void_p),
         ("dt0_per_instance", ctypes.c_void_p),
         ("lml_tcoeff_index", ctypes.c_int32),
+        ("init_std_rows", ctypes.c_void_p),
     ]
 
 
@@ -148,10 +149,26 @@ class Config:
     dt0_per_instance: Any = None
     # loss_lml_timeseries(tcoeff_index=): the Taylor coefficient the LML's data observe
     lml_tcoeff_index: int = 0
+    # optional initial std per covariance row ((nu+1) d dense, nu+1 isotropic): a float64 device
+    # tensor (host array for the CPU build of the kernels); None: init_std on every row
+    init_std_rows: Any = None
 
     def c(self) -> _Config:
         c = self._c()
         c.lml_tcoeff_index = int(self.lml_tcoeff_index)
+        if self.init_std_rows is not None:
+            r = self.init_std_rows
+            n = (self.num_derivatives + 1) * (self.ode_dim if self.factorisation == "dense" else 1)
+            if r.shape != (n,):
+                raise ValueError(f"init_std_rows must have shape ({n},)")
+            if isinstance(r, np.ndarray):
+                r = np.ascontiguousarray(r, dtype=np.float64)
+                c.init_std_rows = r.ctypes.data
+            else:
+                if r.dtype != torch.float64 or not r.is_contiguous():
+                    raise ValueError("init_std_rows must be a contiguous float64 tensor")
+                c.init_std_rows = r.data_ptr()
+            c._keep_rows = r
         if self.prior not in PRIOR:
             raise ValueError(f"unknown prior {self.prior!r}")
         c.prior = PRIOR[self.prior]

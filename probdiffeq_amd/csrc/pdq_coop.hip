@@ -1337,7 +1337,7 @@ __device__ void lml_instance(const Ctx<G>& c, int inst) {
     if (c.dense) S0.m[(i * d + k) * mc + q] = tc[e];
     else S0.m[i * mc + q * d + k] = tc[e];
   }
-  for (int i = c.g.l; i < nc; i += G) S0.L[i * nc + i] = cfg.init_std;
+  for (int i = c.g.l; i < nc; i += G) S0.L[i * nc + i] = cfg.init_std_rows ? cfg.init_std_rows[i] : cfg.init_std;
   if (c.g.l == 0) {
     S0.sc[SC_T] = a.times[0];
     S0.sc[SC_OS] = 1.0;
@@ -1500,7 +1500,7 @@ __device__ void init_instance(const Ctx<G>& c, int inst, int slot0) {
     if (c.dense) S.m[(i * d + k) * mc] = tc[e];
     else S.m[i * mc + k] = tc[e];
   }
-  for (int i = c.g.l; i < nc; i += G) S.L[i * nc + i] = cfg.init_std;
+  for (int i = c.g.l; i < nc; i += G) S.L[i * nc + i] = cfg.init_std_rows ? cfg.init_std_rows[i] : cfg.init_std;
   if (c.g.l == 0) {
     S.sc[SC_T] = c.a->times[(size_t)inst * c.a->K];
     S.sc[SC_OS] = 1.0;

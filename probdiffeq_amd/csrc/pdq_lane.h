@@ -1167,7 +1167,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) L[i * N + j] = i == j ? cfg.init_std : 0.0;
+      for (int j = 0; j < N; ++j) L[i * N + j] = i == j ? (cfg.init_std_rows ? cfg.init_std_rows[i] : cfg.init_std) : 0.0;
 #pragma unroll
     for (int e = 0; e < SlotLayout<N, D>::size; ++e) w.at(sf, e) = 0.0;
     w.store_mL(sf, m, L);
@@ -1340,6 +1340,7 @@ __global__ void __launch_bounds__(64) lane_kernel(LArgs args) {
   a.cfg.strategy = STRATEGY;
   a.cfg.solver = SOLVER;
   if (!OPTS) {
+    a.cfg.init_std_rows = nullptr;
     a.cfg.constraint = PDQ_TS0;
     a.cfg.error_estimator = PDQ_ERROR_RESIDUAL_STD;
     a.cfg.error_re_linearize = 0;

@@ -19,7 +19,7 @@ int launch_lane_nd(LArgs& a, hipStream_t st);
 // without OPTS fold in.
 inline bool lane_default_options(const pdq_config& c) {
   return c.constraint == PDQ_TS0 && c.error_estimator == PDQ_ERROR_RESIDUAL_STD && !c.error_re_linearize &&
-         !c.error_per_unit_step && !c.re_linearize_after_calibration;
+         !c.error_per_unit_step && !c.re_linearize_after_calibration && !c.init_std_rows;
 }
 
 #ifdef PDQ_LANE_DEFINE_LAUNCH

@@ -157,7 +157,7 @@ __device__ __forceinline__ void lane_lml_instance(const LArgs& la, const LmlArgs
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) L[i * N + j] = i == j ? cfg.init_std : 0.0;
+      for (int j = 0; j < N; ++j) L[i * N + j] = i == j ? (cfg.init_std_rows ? cfg.init_std_rows[i] : cfg.init_std) : 0.0;
 #pragma unroll
     for (int e = 0; e < SlotLayout<N, MC>::size; ++e) w.at(sf, e) = 0.0;
     w.store_mL(sf, m, L);

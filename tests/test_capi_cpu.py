@@ -28,11 +28,12 @@ def test_library_exports_every_declared_symbol():
 
 def test_config_struct_matches_header_layout():
     """The ctypes mirror of pdq_config has the C layout (12 ints, 12 doubles, 8 ints, two pointers,
-    one int padded to the struct's 8-byte alignment)."""
+    one int padded to the struct's 8-byte alignment, one pointer)."""
     from probdiffeq_amd import capi
 
-    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8
+    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8
     assert capi._Config.lml_tcoeff_index.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8
+    assert capi._Config.init_std_rows.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8
     assert ctypes.sizeof(capi._Solution) == 17 * 8
 
 

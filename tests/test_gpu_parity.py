@@ -112,6 +112,7 @@ def _oracle_cfg(oracle, cfg):
         prior=cfg.prior,
         prior_drift=None if cfg.prior_drift is None else np.asarray(cfg.prior_drift.cpu() if hasattr(cfg.prior_drift, "cpu") else cfg.prior_drift),
         tcoeff_index=cfg.lml_tcoeff_index,
+        init_std_rows=None if cfg.init_std_rows is None else np.asarray(cfg.init_std_rows.cpu() if hasattr(cfg.init_std_rows, "cpu") else cfg.init_std_rows),
     )
 
 
@@ -468,4 +469,15 @@ def test_error_state_std_on_the_cooperative_kernel(oracle, pdq, fact, solver, la
     cfg = pdq.Config(factorisation=fact, num_derivatives=4, strategy="fixedpoint", solver=solver, rtol=1e-5, atol=1e-7,
                      error_estimator="state_std", lanes_per_instance=lanes, **kw)
     params, y0s, times = _lv_batch(4, seed=47, t_lo=5.0, t_hi=10.0, K=12)
+    _compare(oracle, pdq, cfg, params, y0s, times)
+
+
+@pytest.mark.parametrize("fact,lanes", [("dense", 0), ("isotropic", 0), ("isotropic", 16)])
+def test_init_std_rows(oracle, pdq, fact, lanes):
+    """A std per covariance row at t0 (element-wise is_exact, diffuse_derivatives): the lane kernel's
+    option instantiation (isotropic, lanes 0) and the cooperative kernel, against the oracle."""
+    rows = np.array([0.0, 0.0, 1e-3, 1e-3, 0.5, 0.5, 1.0, 1.0, 1.0, 1.0]) if fact == "dense" else np.array([0.0, 0.0, 1e-3, 0.5, 1.0])
+    cfg = pdq.Config(factorisation=fact, num_derivatives=4, strategy="fixedpoint", solver="dynamic", rtol=1e-5, atol=1e-7,
+                     lanes_per_instance=lanes, init_std_rows=_dev(rows))
+    params, y0s, times = _lv_batch(4, seed=57, t_lo=5.0, t_hi=10.0, K=12)
     _compare(oracle, pdq, cfg, params, y0s, times)

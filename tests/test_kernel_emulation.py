@@ -174,6 +174,21 @@ def test_lml_value_with_ts1(oracle, emu, fact):
         T.check_accuracy(v[i], vo, vf, vt, f"emulation lml ts1 instance {i}")
 
 
+@pytest.mark.parametrize("fact,kernel", [("dense", "coop"), ("isotropic", "coop"), ("isotropic", "lane")])
+def test_init_std_rows(oracle, emu, fact, kernel):
+    """A std per covariance row at t0 (element-wise is_exact, diffuse_derivatives) in both kernels'
+    code, against the oracle."""
+    from probdiffeq_amd import capi
+
+    n = 10 if fact == "dense" else 5
+    rows = np.array([0.0, 0.0, 1e-3, 1e-3, 0.5, 0.5, 1.0, 1.0, 1.0, 1.0])[:n] if fact == "dense" else np.array([0.0, 0.0, 1e-3, 0.5, 1.0])
+    cfg = capi.Config(factorisation=fact, num_derivatives=4, strategy="fixedpoint", solver="dynamic", rtol=1e-5, atol=1e-7, init_std_rows=rows)
+    ocfg = T._oracle_cfg(oracle, cfg)
+    assert ocfg.init_std_rows is not None
+    p, y, t = T._lv_batch(3, seed=21, t_lo=4.0, t_hi=8.0, K=10)
+    _run(oracle, emu, cfg, p, y, t, kernel=kernel)
+
+
 # ----------------------------------------------- lane-per-instance kernel
 @pytest.mark.parametrize("strategy", ["filter", "fixedpoint", "fixedinterval"])
 @pytest.mark.parametrize("solver", ["solver", "dynamic", "mle"])

@@ -379,3 +379,22 @@ def test_re_linearize_after_calibration_changes_only_the_rounding(oracle, fact, 
         truth = runs["extended"][key]
         e_ref = max(rel(runs["strict"][key], truth), rel(runs["fma"][key], truth))
         assert rel(r1[key], truth) <= max(1e-10, 32 * e_ref), key
+
+
+@pytest.mark.parametrize("fact", ["dense", "isotropic"])
+def test_init_std_rows_place_the_initial_stds(oracle, fact):
+    """from_mean_and_std with a std per covariance row (element-wise is_exact, diffuse_derivatives,
+    ssm_impl_isotropic.py:409-447): the solution at t0 is the prior itself, so its covariance is
+    diag(rows^2) in the state's row order (dense derivative-major); equal rows reproduce the scalar
+    init_std bitwise."""
+    tc = oracle.taylor_coefficients("lotka_volterra", THETA, [20.0, 20.0], 0.0, 3)
+    n = 8 if fact == "dense" else 4
+    rows = np.linspace(0.0, 0.7, n)
+    times = np.linspace(0.0, 2.0, 5)
+    cfg = oracle.Config(fact=fact, nu=3, d=2, strategy="filter", solver="solver", init_std_rows=rows)
+    r = oracle.solve_save_at(cfg, THETA, tc, times)
+    L0 = r["chol"][0].reshape(n, n)
+    assert np.array_equal(L0 @ L0.T, np.diag(rows**2))
+    same = oracle.solve_save_at(oracle.Config(fact=fact, nu=3, d=2, strategy="filter", solver="solver", init_std_rows=np.full(n, 0.3)), THETA, tc, times)
+    ref = oracle.solve_save_at(oracle.Config(fact=fact, nu=3, d=2, strategy="filter", solver="solver"), THETA, tc, times, init_std=0.3)
+    assert np.array_equal(same["mean"], ref["mean"]) and np.array_equal(same["chol"], ref["chol"])
