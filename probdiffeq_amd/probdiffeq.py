@@ -463,6 +463,7 @@ class Prior:
     batched: bool
     kind: str = "iwp"
     drift: torch.Tensor | None = None
+    init_std_rows: torch.Tensor | None = None  # a std per covariance row, or None (init_std)
 
     @property
     def num_derivatives(self) -> int:
@@ -497,14 +498,15 @@ class _StateSpaceModel:
             if c.shape != shape0:
                 msg = f"All leaves of the mean must have the same shape. However, leaf {i} has shape {tuple(c.shape)}, while leaf 0 has shape {tuple(shape0)}"
                 raise ValueError(msg)
-        if diffuse_derivatives:
-            raise NotImplementedError("diffuse derivatives are not implemented on the device path")
+        rows = None
+        if diffuse_derivatives or not isinstance(is_exact, bool):
+            rows = self._init_std_rows(coeffs, is_exact, inexact_eps, int(diffuse_derivatives), float(diffuse_eps))
+            coeffs = coeffs + [torch.zeros_like(coeffs[0]) for _ in range(int(diffuse_derivatives))]
+            is_exact = True
         batched = coeffs[0].dim() == 2
         tc = torch.stack(coeffs, dim=-2)
         if not batched:
             tc = tc[None]
-        if not isinstance(is_exact, bool):
-            raise NotImplementedError("element-wise is_exact is not implemented on the device path")
         init_std = 0.0 if is_exact else float(inexact_eps)
         base = 1.0
         if output_scale is not None:
@@ -516,7 +518,28 @@ class _StateSpaceModel:
             if s.size > 1 and not np.all(s == s.flat[0]):
                 raise NotImplementedError("non-uniform base output scales are not implemented on the device path")
             base = float(s.flat[0])
-        return Prior(self.factorisation, tc.contiguous(), init_std, base, batched)
+        return Prior(self.factorisation, tc.contiguous(), init_std, base, batched, init_std_rows=rows)
+
+    def _init_std_rows(self, coeffs, is_exact, inexact_eps, diffuse, diffuse_eps):
+        """_tcoeffs_standard_deviation and _add_diffuse_derivatives (ssm_impl_isotropic.py:519-567,
+        dense alike) as one std per covariance row: 0 for exact coefficients, inexact_eps otherwise,
+        diffuse_eps for the appended zero-mean derivatives.  Isotropic stds are one scalar per
+        derivative; dense ones one per element, derivative-major."""
+        if self.factorisation not in ("dense", "isotropic"):
+            raise NotImplementedError("element-wise is_exact and diffuse derivatives need the dense or isotropic factorisation")
+        d = coeffs[0].shape[-1]
+        width = d if self.factorisation == "dense" else 1
+        flags = [is_exact] * len(coeffs) if isinstance(is_exact, bool) else list(is_exact)
+        if len(flags) != len(coeffs):
+            raise ValueError(f"is_exact has {len(flags)} leaves; the mean has {len(coeffs)}")
+        rows = []
+        for e in flags:
+            e = np.asarray(e.cpu() if isinstance(e, torch.Tensor) else e, dtype=bool)
+            if self.factorisation == "isotropic" and e.ndim != 0:
+                raise ValueError("isotropic models take one is_exact flag per Taylor coefficient")
+            rows.append(np.broadcast_to(np.where(e, 0.0, float(inexact_eps)), (width,)))
+        rows += [np.full(width, diffuse_eps)] * diffuse
+        return torch.tensor(np.concatenate(rows), dtype=torch.float64, device=coeffs[0].device)
 
     def prior_exponential(self, ode, tcoeffs, /, *, is_exact=True, inexact_eps=1e-6, diffuse_derivatives=0, diffuse_eps=1.0, output_scale=None):
         """ssm_impl_dense.py:516-629: the prior of the linear SDE whose drift is the IWP's shift with
@@ -778,6 +801,7 @@ def make_config(prior: Prior, solver_obj: _Solver, error=None, control=None, cli
         clip_dt=clip_dt,
         base_output_scale=prior.base_output_scale,
         init_std=prior.init_std,
+        init_std_rows=prior.init_std_rows,
     )
     if prior.kind == "exponential":
         cfg.prior, cfg.prior_drift = "exponential", prior.drift

@@ -183,3 +183,4 @@ def test_system_matrices_and_preconditioner_match_the_oracle(oracle):
         assert np.allclose(p.numpy(), [dt**4 / 24, dt**3 / 6, dt**2 / 2, dt, 1.0], rtol=1e-15, atol=0)
         assert np.allclose((p * pinv).numpy(), 1.0, rtol=1e-15, atol=0)
     assert pdq.state_space_model_dense().constraint_ode_ts1(pdq.JetOde("lotka_volterra", None, 2), taylor_point=pdq.taylor_point_prior()).kind == "ts1"
+

@@ -219,3 +219,39 @@ def test_jet_expansion_algorithms_agree(api):
     for alg in (pdx.jetexpand_ode_unroll(num=6), pdx.jetexpand_ode_via_jvp(num=6), pdx.jetexpand_ode_doubling_unroll(num_doublings=2)):
         tc, _ = alg(vf, [u0], t=0.0)
         assert len(tc) == 7 and all(torch.equal(a, b) for a, b in zip(tc, ref))
+
+
+def test_mirror_prior_rows_for_elementwise_is_exact_and_diffuse_derivatives(api):
+    """prior_wiener_integrated(is_exact=[...], diffuse_derivatives=k) (ssm_impl_isotropic.py:409-567):
+    k zero-mean derivatives appended; one std per covariance row (isotropic: per derivative; dense:
+    derivative-major per element)."""
+    pdx, _ = api
+
+    tcs = [torch.tensor([1.0, 2.0], dtype=torch.float64), torch.tensor([3.0, 4.0], dtype=torch.float64)]
+    iso = pdx.state_space_model_isotropic().prior_wiener_integrated(tcs, is_exact=[True, False], inexact_eps=1e-3, diffuse_derivatives=2, diffuse_eps=0.5)
+    assert iso.tcoeffs.shape == (1, 4, 2) and torch.all(iso.tcoeffs[0, 2:] == 0)
+    assert iso.init_std_rows.tolist() == [0.0, 1e-3, 0.5, 0.5]
+    dense = pdx.state_space_model_dense().prior_wiener_integrated(tcs, is_exact=[True, [False, True]], inexact_eps=1e-3, diffuse_derivatives=1)
+    assert dense.init_std_rows.tolist() == [0.0, 0.0, 1e-3, 0.0, 1.0, 1.0]
+    plain = pdx.state_space_model_dense().prior_wiener_integrated(tcs)
+    assert plain.init_std_rows is None and plain.init_std == 0.0
+
+
+@pytest.mark.parametrize("fact", ["dense", "isotropic"])
+def test_diffuse_derivatives_solve(api, fact):
+    """prior_wiener_integrated(diffuse_derivatives=2) on three Taylor coefficients (nu = 4 in all):
+    the solve runs with init_std_rows and matches DOP853 to its tolerance."""
+    pdx, ivpsolve = api
+    vf = pdx.lotka_volterra(*T.THETA)
+    u0 = torch.tensor([20.0, 20.0], dtype=torch.float64, device="cuda")
+    tcoeffs, _ = pdx.jetexpand_ode_padded_scan(num=2)(vf, [u0], t=0.0)
+    ssm = {"dense": pdx.state_space_model_dense, "isotropic": pdx.state_space_model_isotropic}[fact]()
+    init = ssm.prior_wiener_integrated(tcoeffs, diffuse_derivatives=2, diffuse_eps=1.0)
+    assert init.tcoeffs.shape[-2] == 5 and init.init_std_rows is not None
+    constraint = ssm.constraint_ode_ts0(vf)
+    slv = pdx.solver_dynamic(strategy=pdx.strategy_filter(), constraint=constraint)
+    save_at = torch.linspace(0.0, 10.0, 20, dtype=torch.float64, device="cuda")
+    sol = ivpsolve.solve_adaptive_save_at(solver=slv, error=pdx.error_residual_std(constraint=constraint))(init, save_at=save_at, atol=1e-8, rtol=1e-6)
+    u = sol.u.mean[0].cpu().numpy()
+    truth = _dop853(T.THETA, [20.0, 20.0], save_at.cpu().numpy())
+    assert np.max(np.abs(u - truth) / (1e-8 + 1e-6 * np.abs(truth))) < 50.0
