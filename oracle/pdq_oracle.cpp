@@ -1077,12 +1077,32 @@ BCond linearize(const Problem& P, const BNormal& rv, Real t) {
     C.tl = ones(N);
     C.to = ones(1);
   } else {
+    // ts0: BlockDiagOdeTs0.linearize (ssm_impl_blockdiag.py:123-144).
+    // ts1: BlockDiagResidual.linearize (:147-183) with jacobian_materialize's
+    // calculate_diagonal_along_d (jacobians.py:108-113): for r = m_1 - f(m_0)
+    // block k's operator is [d r_k / d m_0k, d r_k / d m_1k, 0...] =
+    // [-J_f[k][k], 1, 0...] and fx = r_k(m) - operator . m_k.
+    Vec Jf;
+    if (P.cfg.constraint != CONSTRAINT_TS0) {
+      Jf.resize((size_t)d * d);
+      P.vf.jac(y0.data(), t, Jf.data());
+    }
     for (int k = 0; k < d; ++k) {
       Cond& C = out.b[k];
       C.A = Mat(1, N);
-      C.A(0, 1) = 1.0;
       C.noise.m = Mat(1, 1);
-      C.noise.m(0, 0) = -f[k];
+      if (P.cfg.constraint == CONSTRAINT_TS0) {
+        C.A(0, 1) = 1.0;
+        C.noise.m(0, 0) = -f[k];
+      } else {
+        C.A(0, 0) = -Jf[(size_t)k * d + k];
+        C.A(0, 1) = 1.0;
+        const Mat& m = rv.b[k].m;  // (N, 1)
+        const Real r = m(1, 0) - f[k];
+        Real lin = 0.0;
+        for (int n = 0; n < N; ++n) lin += C.A(0, n) * m(n, 0);
+        C.noise.m(0, 0) = r - lin;
+      }
       C.noise.L = Mat(1, 1);
       C.noise.L(0, 0) = damp;
       C.tl = ones(N);

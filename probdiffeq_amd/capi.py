@@ -153,45 +153,42 @@ class Config:
     # tensor (host array for the CPU build of the kernels); None: init_std on every row
     init_std_rows: Any = None
 
-    def c(self) -> _Config:
+    def c(self, host: bool = False) -> _Config:
+        """The C struct.  The kernels dereference init_std_rows, prior_drift and
+        dt0_per_instance on the device, so for the HIP library (host=False) they
+        must be contiguous float64 device tensors; host=True (the CPU build of the
+        kernels) takes numpy arrays instead."""
         c = self._c()
         c.lml_tcoeff_index = int(self.lml_tcoeff_index)
-        if self.init_std_rows is not None:
-            r = self.init_std_rows
-            n = (self.num_derivatives + 1) * (self.ode_dim if self.factorisation == "dense" else 1)
-            if r.shape != (n,):
-                raise ValueError(f"init_std_rows must have shape ({n},)")
-            if isinstance(r, np.ndarray):
-                r = np.ascontiguousarray(r, dtype=np.float64)
-                c.init_std_rows = r.ctypes.data
-            else:
-                if r.dtype != torch.float64 or not r.is_contiguous():
-                    raise ValueError("init_std_rows must be a contiguous float64 tensor")
-                c.init_std_rows = r.data_ptr()
-            c._keep_rows = r
+        n_rows = (self.num_derivatives + 1) * (self.ode_dim if self.factorisation == "dense" else 1)
         if self.prior not in PRIOR:
             raise ValueError(f"unknown prior {self.prior!r}")
         c.prior = PRIOR[self.prior]
-        if self.prior_drift is not None:
-            drift = self.prior_drift
-            if isinstance(drift, np.ndarray):
-                drift = np.ascontiguousarray(drift, dtype=np.float64)
-                c.prior_drift = drift.ctypes.data
+        for field, shape in (
+            ("init_std_rows", (n_rows,)),
+            ("prior_drift", (self.ode_dim, (self.num_derivatives + 1) * self.ode_dim)),
+            ("dt0_per_instance", None),
+        ):
+            v = getattr(self, field)
+            if v is None:
+                continue
+            if shape is not None and tuple(v.shape) != shape:
+                raise ValueError(f"{field} must have shape {shape} (got {tuple(v.shape)})")
+            if v.ndim != (1 if shape is None else len(shape)):
+                raise ValueError(f"{field} has {v.ndim} dimensions")
+            if host:
+                if not isinstance(v, np.ndarray):
+                    raise ValueError(f"{field}: the CPU build of the kernels takes a numpy array")
+                v = np.ascontiguousarray(v, dtype=np.float64)
+                ptr = v.ctypes.data
             else:
-                if drift.dtype != torch.float64 or not drift.is_contiguous():
-                    raise ValueError("prior_drift must be a contiguous float64 tensor")
-                c.prior_drift = drift.data_ptr()
-            c._keep = drift  # the memory lives as long as the struct
-        if self.dt0_per_instance is not None:
-            d0 = self.dt0_per_instance
-            if isinstance(d0, np.ndarray):
-                d0 = np.ascontiguousarray(d0, dtype=np.float64)
-                c.dt0_per_instance = d0.ctypes.data
-            else:
-                if d0.dtype != torch.float64 or not d0.is_contiguous():
-                    raise ValueError("dt0_per_instance must be a contiguous float64 tensor")
-                c.dt0_per_instance = d0.data_ptr()
-            c._keep_dt0 = d0
+                if not isinstance(v, torch.Tensor) or not v.is_cuda:
+                    raise PdqError(f"{field} must be a device (HIP) tensor: the kernels read it on the GPU")
+                if v.dtype != torch.float64 or not v.is_contiguous():
+                    raise ValueError(f"{field} must be a contiguous float64 tensor")
+                ptr = v.data_ptr()
+            setattr(c, field, ptr)
+            setattr(c, "_keep_" + field, v)  # the memory lives as long as the struct
         return c
 
     def _c(self) -> _Config:
@@ -297,13 +294,20 @@ def check(rc: int) -> None:
         raise PdqError(f"pdq error {rc}: {last_error()}")
 
 
-def _ptr(t):
+def _ptr(t, dtype=torch.float64, shape=None, name="tensor"):
+    """Device pointer of a contiguous tensor of the given dtype (and shape,
+    where given: None entries match any size)."""
     if t is None:
         return None
-    if not t.is_cuda:
-        raise PdqError("pdq takes device tensors (HIP); got a CPU tensor")
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise PdqError(f"pdq takes device tensors (HIP); {name} is not one")
     if not t.is_contiguous():
-        raise PdqError("pdq takes contiguous tensors")
+        raise PdqError(f"pdq takes contiguous tensors ({name} is not)")
+    if dtype is not None and t.dtype != dtype:
+        raise PdqError(f"{name} must be {dtype} (got {t.dtype})")
+    if shape is not None:
+        if t.dim() != len(shape) or any(s is not None and s != n for s, n in zip(shape, t.shape)):
+            raise PdqError(f"{name} must have shape {tuple('*' if s is None else s for s in shape)} (got {tuple(t.shape)})")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -349,23 +353,31 @@ def alloc_solution(cfg: Config, n_inst: int, K: int, device=None, *, smoother_st
     return out
 
 
+_INT_OUTPUTS = {"num_steps": torch.int64, "attempts": torch.int64, "accepted": torch.int64, "status": torch.int32}
+
+
 def _solution_struct(out: dict) -> _Solution:
     s = _Solution()
     for name, _ in _Solution._fields_:
         t = out.get(name)
-        setattr(s, name, _ptr(t).value if t is not None else None)
+        setattr(s, name, _ptr(t, _INT_OUTPUTS.get(name, torch.float64), name=name).value if t is not None else None)
     return s
 
 
 def solve(cfg: Config, params, tcoeffs, times, *, fixed_grid=False, out=None, stream=None):
     """Batched solve; params (B, P), tcoeffs (B, N, d), times (B, K) device tensors."""
+    if times.dim() != 2:
+        raise PdqError(f"times must have shape (B, K) (got {tuple(times.shape)})")
     n_inst, K = times.shape
+    p_ptr = _ptr(params, shape=(n_inst, cfg.n_params), name="params")
+    tc_ptr = _ptr(tcoeffs, shape=(n_inst, cfg.N, cfg.ode_dim), name="tcoeffs")
+    t_ptr = _ptr(times, name="times")
     if out is None:
         out = alloc_solution(cfg, n_inst, K, times.device)
     c = cfg.c()
     s = _solution_struct(out)
     fn = lib().pdq_solve_fixed_grid if fixed_grid else lib().pdq_solve_adaptive_save_at
-    check(fn(ctypes.byref(c), n_inst, _ptr(params), _ptr(tcoeffs), _ptr(times), K, ctypes.byref(s), _stream(stream)))
+    check(fn(ctypes.byref(c), n_inst, p_ptr, tc_ptr, t_ptr, K, ctypes.byref(s), _stream(stream)))
     return out
 
 
@@ -373,26 +385,24 @@ def lml_value_and_grad(cfg: Config, params, y0, grid, every, data, stds, *, aver
     """Batched marginal log-likelihood (and its gradient w.r.t. params and y0)."""
     n_inst = params.shape[0]
     dev = params.device
+    if grid.dim() != 1 or every < 1 or (grid.shape[0] - 1) % every != 0:
+        raise PdqError(f"grid must be (K_grid,) with (K_grid - 1) divisible by every = {every}")
+    k_obs = (grid.shape[0] - 1) // every + 1
+    n_std = stds.shape[-1] if stds.dim() == 3 else 1
+    args = (
+        _ptr(params, shape=(n_inst, cfg.n_params), name="params"),
+        _ptr(y0, shape=(n_inst, cfg.ode_dim), name="y0"),
+        _ptr(grid, name="grid"),
+        _ptr(data, shape=(n_inst, k_obs, cfg.ode_dim), name="data"),
+        _ptr(stds, shape=(n_inst, k_obs, n_std) if stds.dim() == 3 else (n_inst, k_obs), name="stds"),
+    )
     value = torch.zeros(n_inst, dtype=torch.float64, device=dev)
     g = torch.zeros(n_inst, cfg.n_params + cfg.ode_dim, dtype=torch.float64, device=dev) if grad else None
-    n_std = stds.shape[-1] if stds.dim() == 3 else 1
     c = cfg.c()
+    p, y, gr, da, st = args
     check(
         lib().pdq_lml_value_and_grad(
-            ctypes.byref(c),
-            n_inst,
-            _ptr(params),
-            _ptr(y0),
-            _ptr(grid),
-            grid.shape[0],
-            every,
-            _ptr(data),
-            _ptr(stds),
-            n_std,
-            int(average),
-            _ptr(value),
-            _ptr(g),
-            _stream(stream),
+            ctypes.byref(c), n_inst, p, y, gr, grid.shape[0], every, da, st, n_std, int(average), _ptr(value), _ptr(g), _stream(stream)
         )
     )
     return value, g
@@ -449,7 +459,16 @@ def taylor_coefficients(vector_field: str, params, y0, t0: float, num: int, stre
     out = torch.zeros(n_inst, num + 1, d, dtype=torch.float64, device=y0.device)
     check(
         lib().pdq_taylor_coefficients(
-            VF[vector_field], d, params.shape[1], n_inst, _ptr(params), _ptr(y0), ctypes.c_double(t0), num, _ptr(out), _stream(stream)
+            VF[vector_field],
+            d,
+            params.shape[1],
+            n_inst,
+            _ptr(params, shape=(n_inst, None), name="params"),
+            _ptr(y0, name="y0"),
+            ctypes.c_double(t0),
+            num,
+            _ptr(out),
+            _stream(stream),
         )
     )
     return out

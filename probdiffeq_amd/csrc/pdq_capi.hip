@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -24,7 +25,9 @@
 
 namespace {
 
-thread_local std::string g_err;
+// The last error message of the calling thread (a fixed buffer: setting it
+// allocates nothing).
+thread_local char g_err[512];
 
 }  // namespace
 
@@ -35,7 +38,7 @@ int host_error(int code, const char* fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
-  g_err = buf;
+  std::memcpy(g_err, buf, sizeof(g_err));
   return code;
 }
 
@@ -50,12 +53,10 @@ int num_cus() {
 namespace {
 
 int set_err(int code, const char* fmt, ...) {
-  char buf[512];
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
-  g_err = buf;
   return code;
 }
 
@@ -70,9 +71,35 @@ struct DevMats {
   double* Q = nullptr;
 };
 
+// Stream-ordered device buffers of one call, freed (stream ordered) on every
+// exit path of the call.
+struct CallBuffers {
+  hipStream_t st;
+  void* p[4] = {nullptr, nullptr, nullptr, nullptr};
+  int n = 0;
+  explicit CallBuffers(hipStream_t s) : st(s) {}
+  CallBuffers(const CallBuffers&) = delete;
+  CallBuffers& operator=(const CallBuffers&) = delete;
+  ~CallBuffers() {
+    for (int i = n - 1; i >= 0; --i) (void)hipFreeAsync(p[i], st);
+  }
+  template <class T>
+  hipError_t alloc(T** out, size_t bytes) {
+    void* q = nullptr;
+    const hipError_t e = hipMallocAsync(&q, bytes, st);
+    if (e != hipSuccess) return e;
+    p[n++] = q;
+    *out = static_cast<T*>(q);
+    return hipSuccess;
+  }
+};
+
 std::mutex g_mu;
 std::map<std::tuple<int, int, int, int, double, int>, DevMats> g_mats;  // (dev, fact, nu, d, base, kind)
-std::map<std::string, int> g_tuned;
+// (mode, factorisation, nu, d, strategy, solver, constraint, vector field,
+// prior, batch-size bucket): a POD key, so that a lookup allocates nothing
+using TuneKey = std::array<int, 10>;
+std::map<TuneKey, int> g_tuned;
 
 int device_mats(const pdq_config& cfg, bool latent_1d, DevMats& out) {
   int dev = 0;
@@ -113,24 +140,32 @@ int device_mats(const pdq_config& cfg, bool latent_1d, DevMats& out) {
   return PDQ_OK;
 }
 
+// The registry's requirements on a vector field's dimension and parameters
+// (pdq_device.h reads params[0 .. n_params) and y[0 .. d) only within them).
+int check_vector_field(int vf, int d, int n_params) {
+  if (vf < 0 || vf > PDQ_VF_LOGISTIC) return set_err(PDQ_ERR_UNSUPPORTED, "unknown vector field %d", vf);
+  if (d < 1) return set_err(PDQ_ERR_INVALID, "ode_dim must be positive");
+  if (n_params < 0 || n_params > pdq::kMaxParams) return set_err(PDQ_ERR_INVALID, "n_params out of range");
+  if ((vf == PDQ_VF_LOTKA_VOLTERRA || vf == PDQ_VF_VAN_DER_POL) && d != 2)
+    return set_err(PDQ_ERR_INVALID, "this vector field needs ode_dim == 2");
+  if (vf == PDQ_VF_LOTKA_VOLTERRA && n_params < 4) return set_err(PDQ_ERR_INVALID, "lotka_volterra needs 4 params");
+  if (vf == PDQ_VF_VAN_DER_POL && n_params < 1) return set_err(PDQ_ERR_INVALID, "van_der_pol needs 1 param");
+  if (vf == PDQ_VF_BURGERS && n_params < 2) return set_err(PDQ_ERR_INVALID, "burgers needs (viscosity, dx)");
+  if (vf == PDQ_VF_LINEAR && n_params < d) return set_err(PDQ_ERR_INVALID, "linear needs d params");
+  if (vf == PDQ_VF_LOGISTIC && n_params < 1) return set_err(PDQ_ERR_INVALID, "logistic needs 1 param");
+  return PDQ_OK;
+}
+
 int check_common(const pdq_config* cfg, int n_inst, int K) {
+  int rc = PDQ_OK;
   if (!cfg) return set_err(PDQ_ERR_INVALID, "cfg is NULL");
   if (n_inst < 0) return set_err(PDQ_ERR_INVALID, "n_inst < 0");
   if (K < 2) return set_err(PDQ_ERR_INVALID, "need at least two time points (got %d)", K);
   if (cfg->num_derivatives < 1 || cfg->num_derivatives > 7)
     return set_err(PDQ_ERR_UNSUPPORTED, "num_derivatives must be in [1, 7] (got %d)", cfg->num_derivatives);
   if (cfg->ode_dim < 1) return set_err(PDQ_ERR_INVALID, "ode_dim must be positive");
-  if (cfg->n_params < 0 || cfg->n_params > pdq::kMaxParams) return set_err(PDQ_ERR_INVALID, "n_params out of range");
-  const int vf = cfg->vector_field;
-  const int d = cfg->ode_dim;
-  if ((vf == PDQ_VF_LOTKA_VOLTERRA || vf == PDQ_VF_VAN_DER_POL) && d != 2)
-    return set_err(PDQ_ERR_INVALID, "this vector field needs ode_dim == 2");
-  if (vf == PDQ_VF_LOTKA_VOLTERRA && cfg->n_params < 4) return set_err(PDQ_ERR_INVALID, "lotka_volterra needs 4 params");
-  if (vf == PDQ_VF_VAN_DER_POL && cfg->n_params < 1) return set_err(PDQ_ERR_INVALID, "van_der_pol needs 1 param");
-  if (vf == PDQ_VF_BURGERS && cfg->n_params < 2) return set_err(PDQ_ERR_INVALID, "burgers needs (viscosity, dx)");
-  if (vf == PDQ_VF_LINEAR && cfg->n_params < d) return set_err(PDQ_ERR_INVALID, "linear needs d params");
-  if (vf == PDQ_VF_LOGISTIC && cfg->n_params < 1) return set_err(PDQ_ERR_INVALID, "logistic needs 1 param");
-  if (vf < 0 || vf > PDQ_VF_LOGISTIC) return set_err(PDQ_ERR_UNSUPPORTED, "unknown vector field %d", vf);
+  rc = check_vector_field(cfg->vector_field, cfg->ode_dim, cfg->n_params);
+  if (rc) return rc;
   if (cfg->strategy < 0 || cfg->strategy > 2 || cfg->solver < 0 || cfg->solver > 2 || cfg->constraint < 0 ||
       cfg->constraint > 1 || cfg->factorisation < 0 || cfg->factorisation > 2)
     return set_err(PDQ_ERR_INVALID, "invalid enum value in configuration");
@@ -167,13 +202,11 @@ int lds_limit() {
   return v;
 }
 
-std::string tune_key(const pdq_config& c, int n_inst, int mode) {
+TuneKey tune_key(const pdq_config& c, int n_inst, int mode) {
   int bucket = 1;
   while (bucket < n_inst) bucket <<= 1;
-  char buf[256];
-  snprintf(buf, sizeof(buf), "%d/%d/%d/%d/%d/%d/%d/%d/%d/%d", mode, c.factorisation, c.num_derivatives, c.ode_dim,
-           c.strategy, c.solver, c.constraint, c.vector_field, c.prior, bucket);
-  return buf;
+  return TuneKey{mode, c.factorisation, c.num_derivatives, c.ode_dim, c.strategy, c.solver, c.constraint,
+                 c.vector_field, c.prior, bucket};
 }
 
 // Heuristic lanes per instance: enough lanes that the batch fills the chip
@@ -205,8 +238,10 @@ int pick_lanes(const pdq_config& cfg, const pdq::Layout& L, int n_inst, int mode
   return heuristic_lanes(L, n_inst);
 }
 
-int solve_common(const pdq_config* cfg, int n_inst, const double* params, const double* tcoeffs,
-                 const double* times, int K, const pdq_solution* out, void* stream, int mode) {
+// Checks of a batched solve's configuration, inputs and outputs (before any
+// device work).
+int solve_validate(const pdq_config* cfg, int n_inst, const double* params, const double* tcoeffs,
+                   const double* times, int K, const pdq_solution* out) {
   int rc = check_common(cfg, n_inst, K);
   if (rc) return rc;
   if (!out) return set_err(PDQ_ERR_INVALID, "out is NULL");
@@ -219,17 +254,23 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
   if (n_inst == 0) return PDQ_OK;
   if (!tcoeffs || !times || (cfg->n_params > 0 && !params))
     return set_err(PDQ_ERR_INVALID, "tcoeffs, times and params must be device pointers");
-  hipStream_t st = (hipStream_t)stream;
-  pdq::Layout L = pdq::make_layout(*cfg, 0);
+  const pdq::Layout L = pdq::make_layout(*cfg, 0);
   if (L.nc > 32) return set_err(PDQ_ERR_UNSUPPORTED, "dense state dimension %d exceeds 32", L.nc);
   if (L.smoother && (!out->filt_mean || !out->filt_chol || !out->cond_A || !out->cond_mean || !out->cond_chol ||
                      !out->cond_tl || !out->cond_to))
     return set_err(PDQ_ERR_INVALID, "smoothers need filt_* and cond_* outputs (backward pass storage)");
   if (cfg->solver == PDQ_SOLVER_MLE && !out->num_steps)
     return set_err(PDQ_ERR_INVALID, "solver_mle needs the num_steps output");
-  DevMats m;
-  rc = device_mats(*cfg, false, m);
-  if (rc) return rc;
+  return PDQ_OK;
+}
+
+// Launch of a validated batched solve on the IWP matrices A, Q (device).
+// queue: the cooperative kernel's work-queue counter; null allocates one for
+// this call (stream ordered).
+int solve_run(const pdq_config* cfg, int n_inst, const double* params, const double* tcoeffs, const double* times,
+              int K, const pdq_solution* out, const double* A, const double* Q, int* queue, hipStream_t st,
+              int mode) {
+  const pdq::Layout L = pdq::make_layout(*cfg, 0);
   const int G = pick_lanes(*cfg, L, n_inst, mode);
   if (G == 1) {
     if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(L.N, L.d))
@@ -245,14 +286,13 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
     la.params = params;
     la.tcoeffs = tcoeffs;
     la.times = times;
-    la.A = m.A;
-    la.Q = m.Q;
+    la.A = A;
+    la.Q = Q;
     la.out = *out;
-    rc = pdq::launch_lane(la, L.N, L.d, st);
-    return rc;
+    return pdq::launch_lane(la, L.N, L.d, st);
   }
-  int* queue = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&queue, sizeof(int), st));
+  int* q = queue;
+  if (!q) HIP_TRY(hipMallocAsync((void**)&q, sizeof(int), st));
   pdq::KArgs a{};
   a.cfg = *cfg;
   a.lay = L;
@@ -262,13 +302,23 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
   a.params = params;
   a.tcoeffs = tcoeffs;
   a.times = times;
-  a.A = m.A;
-  a.Q = m.Q;
+  a.A = A;
+  a.Q = Q;
   a.out = *out;
-  a.queue = queue;
-  rc = pdq::launch_coop(a, G, st);
-  (void)hipFreeAsync(queue, st);
+  a.queue = q;
+  const int rc = pdq::launch_coop(a, G, st);
+  if (!queue) (void)hipFreeAsync(q, st);
   return rc;
+}
+
+int solve_common(const pdq_config* cfg, int n_inst, const double* params, const double* tcoeffs,
+                 const double* times, int K, const pdq_solution* out, void* stream, int mode) {
+  int rc = solve_validate(cfg, n_inst, params, tcoeffs, times, K, out);
+  if (rc || n_inst == 0) return rc;
+  DevMats m;
+  rc = device_mats(*cfg, false, m);
+  if (rc) return rc;
+  return solve_run(cfg, n_inst, params, tcoeffs, times, K, out, m.A, m.Q, nullptr, (hipStream_t)stream, mode);
 }
 
 __global__ void taylor_kernel(int vf, int d, int np, int n_inst, const double* params, const double* y0, double t0,
@@ -309,10 +359,10 @@ int32_t pdq_init(void) {
 
 size_t pdq_last_error(char* buf, size_t size) {
   if (buf && size) {
-    std::strncpy(buf, g_err.c_str(), size - 1);
+    std::strncpy(buf, g_err, size - 1);
     buf[size - 1] = 0;
   }
-  return g_err.size();
+  return std::strlen(g_err);
 }
 
 int32_t pdq_mean_size(const pdq_config* cfg) { return (cfg->num_derivatives + 1) * cfg->ode_dim; }
@@ -384,55 +434,48 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
     DevMats m;
     rc = device_mats(*cfg, false, m);
     if (rc) return rc;
+    CallBuffers bufs(st);
     double* tinit = nullptr;
-    double* scratch = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&tinit, sizeof(double) * (size_t)n_inst * N * d * (1 + n_tan), st));
-    if (hipMallocAsync((void**)&scratch, sizeof(double) * csz * (size_t)std::max(1, Kobs - 1) * n_lanes, st) !=
-        hipSuccess) {
-      (void)hipFreeAsync(tinit, st);
-      return set_err(PDQ_ERR_HIP, "hipMallocAsync of the LML scratch buffer failed");
-    }
+    double* cond_buf = nullptr;  // backward conditionals between observation nodes
+    HIP_TRY(bufs.alloc(&tinit, sizeof(double) * (size_t)n_inst * N * d * (1 + n_tan)));
+    HIP_TRY(bufs.alloc(&cond_buf, sizeof(double) * csz * (size_t)std::max(1, Kobs - 1) * n_lanes));
     rc = pdq::launch_taylor_tangents(cfg->vector_field, d, cfg->n_params, n_inst, cfg->num_derivatives, n_tan, params,
                                      y0, grid, tinit, st);
-    if (!rc) {
-      pdq::LmlArgs la{};
-      la.cfg = *cfg;
-      la.n_inst = n_inst;
-      la.K = K_grid;
-      la.every = every;
-      la.n_tan = n_tan;
-      la.average_pdfs = average_pdfs;
-      la.G = G;
-      la.params = params;
-      la.tcoeffs = tinit;
-      la.times = grid;
-      la.data = data;
-      la.stds = stds;
-      la.Q = m.Q;
-      la.value = value;
-      la.grad = grad;
-      la.scratch = scratch;
-      rc = pdq::launch_lane_lml(la, N, d, tpl, st);
-    }
-    (void)hipFreeAsync(scratch, st);
-    (void)hipFreeAsync(tinit, st);
-    return rc;
+    if (rc) return rc;
+    pdq::LmlArgs la{};
+    la.cfg = *cfg;
+    la.n_inst = n_inst;
+    la.K = K_grid;
+    la.every = every;
+    la.n_tan = n_tan;
+    la.average_pdfs = average_pdfs;
+    la.G = G;
+    la.params = params;
+    la.tcoeffs = tinit;
+    la.times = grid;
+    la.data = data;
+    la.stds = stds;
+    la.Q = m.Q;
+    la.value = value;
+    la.grad = grad;
+    la.scratch = cond_buf;
+    return pdq::launch_lane_lml(la, N, d, tpl, st);
   }
   pdq::Layout L = pdq::make_layout(*cfg, n_tan);
   if (L.nc > 32) return set_err(PDQ_ERR_UNSUPPORTED, "dense state dimension %d exceeds 32", L.nc);
   const size_t csz = 2 * (size_t)L.nc * L.nc + (size_t)L.nc * L.mc + 2 * (size_t)L.nc;
-  double* scratch = nullptr;
-  int* queue = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&queue, sizeof(int), st));
-  HIP_TRY(hipMallocAsync((void**)&scratch, sizeof(double) * csz * (size_t)std::max(1, Kobs - 1) * n_inst, st));
-  // prior means with tangents: Taylor expansion of y0 (jetexpand, num = nu)
-  double* tinit = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&tinit, sizeof(double) * (size_t)n_inst * N * d * (1 + n_tan), st));
-  rc = pdq::launch_taylor_tangents(cfg->vector_field, d, cfg->n_params, n_inst, cfg->num_derivatives, n_tan, params, y0,
-                                   grid, tinit, st);
-  if (rc) return rc;
   DevMats m;
   rc = device_mats(*cfg, false, m);
+  if (rc) return rc;
+  CallBuffers bufs(st);
+  double* cond_buf = nullptr;  // backward conditionals between observation nodes
+  int* queue = nullptr;
+  double* tinit = nullptr;  // prior means with tangents: Taylor expansion of y0 (jetexpand, num = nu)
+  HIP_TRY(bufs.alloc(&queue, sizeof(int)));
+  HIP_TRY(bufs.alloc(&cond_buf, sizeof(double) * csz * (size_t)std::max(1, Kobs - 1) * n_inst));
+  HIP_TRY(bufs.alloc(&tinit, sizeof(double) * (size_t)n_inst * N * d * (1 + n_tan)));
+  rc = pdq::launch_taylor_tangents(cfg->vector_field, d, cfg->n_params, n_inst, cfg->num_derivatives, n_tan, params, y0,
+                                   grid, tinit, st);
   if (rc) return rc;
   pdq::KArgs a{};
   a.cfg = *cfg;
@@ -453,14 +496,10 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
   a.Q = m.Q;
   a.lml_value = value;
   a.lml_grad = grad;
-  a.scratch = scratch;
+  a.scratch = cond_buf;
   a.queue = queue;
   const int G = pick_lanes(*cfg, L, n_inst, pdq::MODE_LML);
-  rc = pdq::launch_coop(a, G, st);
-  (void)hipFreeAsync(tinit, st);
-  (void)hipFreeAsync(scratch, st);
-  (void)hipFreeAsync(queue, st);
-  return rc;
+  return pdq::launch_coop(a, G, st);
 }
 
 int32_t pdq_offgrid_marginals(const pdq_config* cfg, int32_t n_inst, int32_t K, const double* times,
@@ -592,11 +631,15 @@ int32_t pdq_lml_terminal_values(const pdq_config* cfg, int32_t n_inst, const dou
 int32_t pdq_taylor_coefficients(int32_t vector_field, int32_t d, int32_t n_params, int32_t n_inst,
                                 const double* params, const double* y0, double t0, int32_t num, double* out,
                                 void* stream) {
-  if ((num + 1) * d > 8 * 64) return set_err(PDQ_ERR_UNSUPPORTED, "(num+1)*d must not exceed 512");
+  // taylor_kernel keeps (num + 1) d coefficients in a private array of 512
   if (num < 0 || d < 1 || n_inst < 0) return set_err(PDQ_ERR_INVALID, "invalid sizes");
+  if ((int64_t)(num + 1) * d > 8 * 64) return set_err(PDQ_ERR_UNSUPPORTED, "(num+1)*d must not exceed 512");
+  const int rc = check_vector_field(vector_field, d, n_params);
+  if (rc) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
   if (n_inst == 0) return PDQ_OK;
+  if (!y0 || !out || (n_params > 0 && !params)) return set_err(PDQ_ERR_INVALID, "null device pointer");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(taylor_kernel, dim3((n_inst + 63) / 64), dim3(64), 0, st, vector_field, d, n_params, n_inst,
                      params, y0, t0, num, out);

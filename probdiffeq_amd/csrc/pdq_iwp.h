@@ -20,25 +20,29 @@ inline double binom(int n, int k) {
   return r;
 }
 
-inline void iwp_matrices(int nu, std::vector<double>& A, std::vector<double>& Q) {
+constexpr int kIwpMaxN = 8;  // nu + 1 <= 8
+
+// Into caller arrays of (nu+1)^2 doubles; nu <= 7.  No heap allocation.
+inline void iwp_matrices_into(int nu, double* A, double* Q) {
   const int n = nu + 1;
-  A.assign(n * n, 0.0);
+  for (int i = 0; i < n * n; ++i) A[i] = 0.0;
   for (int i = 0; i < n; ++i)
     for (int j = i; j < n; ++j) A[i * n + j] = binom(nu - i, nu - j);
   // Kahan: U upper with H = U^T U ... columns scaled by sqrt(2i+1) f_j
-  std::vector<double> f(n, 1.0), U(n * n, 0.0);
+  double f[kIwpMaxN], U[kIwpMaxN * kIwpMaxN], Lh[kIwpMaxN * kIwpMaxN], M[kIwpMaxN * kIwpMaxN];
+  f[0] = 1.0;
   for (int k = 1; k < n; ++k) f[k] = (((f[k - 1] / k) * (2.0 * k)) / k) * (2.0 * k + 1.0);
+  for (int i = 0; i < n * n; ++i) U[i] = Lh[i] = 0.0;
   for (int i = 0; i < n; ++i) U[i * n + i] = 1.0;
   for (int j = 1; j < n; ++j)
     for (int i = j - 1; i >= 0; --i) U[i * n + j] = (U[(i + 1) * n + j] / (double)(j - i)) * (double)(i + j + 2);
-  std::vector<double> Lh(n * n, 0.0);  // lower factor = U^T scaled
+  // lower factor = U^T scaled
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
       const double u = U[i * n + j] * (std::sqrt(2.0 * i + 1.0) * (1.0 / f[j]));
       if (j >= i) Lh[j * n + i] = u;
     }
   // M = (row-flipped Lh)^T, triangularise, transpose back
-  std::vector<double> M(n * n);
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) M[i * n + j] = Lh[(n - 1 - j) * n + i];
   for (int i = 0; i < n - 1; ++i) {
@@ -58,7 +62,7 @@ inline void iwp_matrices(int nu, std::vector<double>& A, std::vector<double>& Q)
     }
     M[i * n + i] = beta;
   }
-  Q.assign(n * n, 0.0);
+  for (int i = 0; i < n * n; ++i) Q[i] = 0.0;
   for (int i = 0; i < n; ++i)
     for (int j = 0; j <= i; ++j) Q[i * n + j] = M[j * n + i];
   for (int j = 0; j < n; ++j) {
@@ -67,20 +71,33 @@ inline void iwp_matrices(int nu, std::vector<double>& A, std::vector<double>& Q)
   }
 }
 
+inline void iwp_matrices(int nu, std::vector<double>& A, std::vector<double>& Q) {
+  const int n = nu + 1;
+  A.assign(n * n, 0.0);
+  Q.assign(n * n, 0.0);
+  iwp_matrices_into(nu, A.data(), Q.data());
+}
 
-// Dense layout: A = kron(A1, I_d), Q = kron(Q1, base * I_d), derivative-major.
-inline void iwp_dense(int nu, int d, double base, std::vector<double>& A, std::vector<double>& Q) {
-  std::vector<double> A1, Q1;
-  iwp_matrices(nu, A1, Q1);
+// Dense layout into caller arrays of ((nu+1) d)^2 doubles (see iwp_dense).
+inline void iwp_dense_into(int nu, int d, double base, double* A, double* Q) {
+  double A1[kIwpMaxN * kIwpMaxN], Q1[kIwpMaxN * kIwpMaxN];
+  iwp_matrices_into(nu, A1, Q1);
   const int N = nu + 1, n = N * d;
-  A.assign((size_t)n * n, 0.0);
-  Q.assign((size_t)n * n, 0.0);
+  for (size_t i = 0; i < (size_t)n * n; ++i) A[i] = Q[i] = 0.0;
   for (int i = 0; i < N; ++i)
     for (int j = 0; j < N; ++j)
       for (int k = 0; k < d; ++k) {
         A[(size_t)(i * d + k) * n + j * d + k] = A1[i * N + j];
         Q[(size_t)(i * d + k) * n + j * d + k] = Q1[i * N + j] * base;
       }
+}
+
+// Dense layout: A = kron(A1, I_d), Q = kron(Q1, base * I_d), derivative-major.
+inline void iwp_dense(int nu, int d, double base, std::vector<double>& A, std::vector<double>& Q) {
+  const size_t n = (size_t)(nu + 1) * d;
+  A.assign(n * n, 0.0);
+  Q.assign(n * n, 0.0);
+  iwp_dense_into(nu, d, base, A.data(), Q.data());
 }
 
 }  // namespace pdq

@@ -553,10 +553,13 @@ class _StateSpaceModel:
             raise NotImplementedError(msg)
         base = self.prior_wiener_integrated(tcoeffs, is_exact=is_exact, inexact_eps=inexact_eps, diffuse_derivatives=diffuse_derivatives,
                                             diffuse_eps=diffuse_eps, output_scale=output_scale)
-        if ode.num_tcoeffs_in_args != len(tcoeffs):
+        # the order is checked against the coefficients after the diffuse padding
+        # (ssm_impl_dense.py:553-573)
+        n_coeffs = base.tcoeffs.shape[-2]
+        if ode.num_tcoeffs_in_args != n_coeffs:
             msg = "The exponential prior does not match the Taylor coefficients in the SSM."
             msg += f" The passed ODE has order {ode.num_tcoeffs_in_args}, whereas the state-space model"
-            msg += f" includes {len(tcoeffs)} Taylor coefficients."
+            msg += f" includes {n_coeffs} Taylor coefficients."
             raise TypeError(msg)
         tc = base.tcoeffs.detach().to("cpu")
         drifts = [_linear_drift(ode, tc[i]) for i in sorted({0, tc.shape[0] - 1})]

@@ -75,7 +75,7 @@ def solve(cfg, params, tcoeffs, times, fixed_grid=False, kernel="coop"):
     for name, _ in capi._Solution._fields_:
         a = out.get(name)
         setattr(s, name, a.ctypes.data if a is not None else None)
-    c = cfg.c()
+    c = cfg.c(host=True)
     P = ctypes.c_void_p
     fn = lib().pdqemu_solve_lane if kernel == "lane" else lib().pdqemu_solve
     rc = fn(ctypes.byref(c), B, P(params.ctypes.data), P(tcoeffs.ctypes.data), P(times.ctypes.data), K, ctypes.byref(s), int(fixed_grid))
@@ -92,7 +92,7 @@ def lml(cfg, params, y0, grid, every, data, stds, average=True, grad=True, kerne
     value = np.zeros(B)
     g = np.zeros((B, params.shape[1] + y0.shape[1])) if grad else None
     n_std = stds.shape[-1] if stds.ndim == 3 else 1
-    c = cfg.c()
+    c = cfg.c(host=True)
     P = ctypes.c_void_p
     if kernel == "lane":
         n_tan = params.shape[1] + y0.shape[1] if grad else 0
@@ -121,7 +121,7 @@ def offgrid(cfg, times, sol, t_query):
     fL = arr(sol["filt_chol"]) if "filt_chol" in sol else None
     P = ctypes.c_void_p
     ptr = lambda x: P(x.ctypes.data) if x is not None else None
-    c = cfg.c()
+    c = cfg.c(host=True)
     rc = lib().pdqemu_offgrid(ctypes.byref(c), B, K, ptr(times), ptr(m), ptr(L), ptr(fm), ptr(fL), ptr(os_), Qn, ptr(t_query), ptr(mean_out), ptr(chol_out))
     if rc:
         raise RuntimeError(f"emulation rc {rc}")
@@ -138,7 +138,7 @@ def sample(cfg, sol, base):
     keys = ("term_mean", "term_chol", "cond_A", "cond_mean", "cond_chol", "cond_tl", "cond_to")
     arrs = [arr(sol[k]) for k in keys]
     P = ctypes.c_void_p
-    c = cfg.c()
+    c = cfg.c(host=True)
     rc = lib().pdqemu_sample(ctypes.byref(c), 1, K, *[P(a.ctypes.data) for a in arrs], S, P(base.ctypes.data), P(out.ctypes.data))
     if rc:
         raise RuntimeError(f"emulation rc {rc}")

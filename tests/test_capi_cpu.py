@@ -184,3 +184,67 @@ def test_system_matrices_and_preconditioner_match_the_oracle(oracle):
         assert np.allclose((p * pinv).numpy(), 1.0, rtol=1e-15, atol=0)
     assert pdq.state_space_model_dense().constraint_ode_ts1(pdq.JetOde("lotka_volterra", None, 2), taylor_point=pdq.taylor_point_prior()).kind == "ts1"
 
+
+
+def test_device_read_config_fields_must_be_device_tensors():
+    """init_std_rows, prior_drift and dt0_per_instance are read by the kernels on the GPU: the
+    binding refuses host memory for them (numpy arrays, CPU tensors) and wrong shapes or dtypes,
+    before any call into the library; the CPU build of the kernels takes numpy (host=True)."""
+    import numpy as np
+    import torch
+
+    from probdiffeq_amd import capi
+
+    for field, value in [
+        ("init_std_rows", np.zeros(10)),
+        ("init_std_rows", torch.zeros(10, dtype=torch.float64)),
+        ("prior_drift", np.zeros((2, 10))),
+        ("dt0_per_instance", torch.full((3,), 0.1, dtype=torch.float64)),
+    ]:
+        cfg = capi.Config(factorisation="dense", **{field: value})
+        with pytest.raises(capi.PdqError, match="device"):
+            cfg.c()
+        cfg.c(host=True) if isinstance(value, np.ndarray) else None
+    with pytest.raises(ValueError, match="shape"):
+        capi.Config(factorisation="dense", init_std_rows=np.zeros(7)).c(host=True)
+    with pytest.raises(ValueError, match="shape"):
+        capi.Config(factorisation="dense", prior_drift=np.zeros((10, 2))).c(host=True)
+    with pytest.raises(ValueError, match="dimensions"):
+        capi.Config(dt0_per_instance=np.zeros((2, 2))).c(host=True)
+
+
+def test_solve_wrappers_refuse_host_tensors():
+    import torch
+
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config()
+    x = torch.zeros(2, 4, dtype=torch.float64)
+    with pytest.raises(capi.PdqError, match="device"):
+        capi.solve(cfg, x, torch.zeros(2, 5, 2, dtype=torch.float64), torch.zeros(2, 3, dtype=torch.float64))
+    with pytest.raises(capi.PdqError, match="device"):
+        capi.lml_value_and_grad(cfg, x, torch.zeros(2, 2), torch.zeros(11), 5, torch.zeros(2, 3, 2), torch.zeros(2, 3))
+
+
+@pytest.mark.parametrize(
+    "vf,d,n_params,num,match",
+    [
+        (0, 3, 4, 4, "ode_dim == 2"),  # lotka_volterra with d != 2
+        (0, 2, 3, 4, "4 params"),  # lotka_volterra with too few params
+        (1, 2, 0, 4, "1 param"),  # van_der_pol without mu
+        (3, 4, 2, 2, "d params"),  # linear with n_params < d
+        (9, 2, 4, 4, "unknown vector field"),
+        (0, 2, 4, -1, "invalid sizes"),
+        (0, 2, 4, 2**30, "512"),  # (num + 1) d overflows int without 64-bit arithmetic
+        (2, 300, 2, 1, "512"),
+    ],
+)
+def test_taylor_coefficients_checks_its_arguments(vf, d, n_params, num, match):
+    """pdq_taylor_coefficients applies the vector-field registry's requirements and bounds num
+    before any device work (its kernel indexes params and a 512-entry private array by them)."""
+    from probdiffeq_amd import capi
+
+    lib = capi.lib()
+    rc = lib.pdq_taylor_coefficients(vf, d, n_params, 1, None, None, ctypes.c_double(0.0), num, None, None)
+    assert rc in (1, 2), rc
+    assert match in capi.last_error()

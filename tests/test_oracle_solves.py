@@ -145,6 +145,40 @@ def test_vmap_dense_equals_blockdiag_for_diagonal_ode(oracle):
         assert np.allclose(rd["output_scale"][:, 0], rb["output_scale"][:, j], rtol=1e-12)
 
 
+@pytest.mark.parametrize("vf", ["linear", "logistic"])
+@pytest.mark.parametrize("strategy", ["filter", "fixedpoint"])
+def test_vmap_dense_equals_blockdiag_ts1_for_diagonal_ode(oracle, vf, strategy):
+    """Blockdiag ts1 (BlockDiagResidual.linearize, ssm_impl_blockdiag.py:147-183, with the exact
+    Jacobian diagonal of jacobian_materialize, jacobians.py:108-113) against dense ts1 on each
+    component: for an ODE whose Jacobian is diagonal the blocks are d independent dense d = 1
+    problems that share the grid (test_dynamic_across_factorisations.py:68-145 with ts1).  The
+    logistic field's Jacobian depends on the state, so a wrong sign or a missing diagonal entry
+    in the block operator (or in its offset fx) changes the calibration and the means."""
+    rng = np.random.default_rng(4)
+    d = 5
+    if vf == "linear":
+        prm = rng.normal(size=d)
+        u0 = rng.normal(size=d)
+    else:
+        prm = np.array([1.7])
+        u0 = rng.uniform(0.1, 0.9, size=d)
+    ts = np.linspace(0.0, 3.0, 13)
+    cfg_bd = oracle.Config(fact="blockdiag", nu=3, d=d, vf=vf, constraint="ts1", strategy=strategy)
+    tc = oracle.taylor_coefficients(vf, prm, u0, 0.0, 3)
+    rb = oracle.solve_fixed_grid(cfg_bd, prm, tc, ts)
+    cfg_b0 = oracle.Config(fact="blockdiag", nu=3, d=d, vf=vf, constraint="ts0", strategy=strategy)
+    r0 = oracle.solve_fixed_grid(cfg_b0, prm, tc, ts)
+    assert np.max(np.abs(r0["output_scale"] - rb["output_scale"])) > 1e-6  # ts1 is not ts0
+    for j in range(d):
+        pj = prm[j : j + 1] if vf == "linear" else prm
+        cfg_d = oracle.Config(fact="dense", nu=3, d=1, vf=vf, constraint="ts1", strategy=strategy)
+        tcj = oracle.taylor_coefficients(vf, pj, u0[j : j + 1], 0.0, 3)
+        rd = oracle.solve_fixed_grid(cfg_d, pj, tcj, ts)
+        assert np.allclose(rd["mean"], rb["mean"].reshape(13, d, 4)[:, j, :], rtol=1e-10, atol=1e-12)
+        assert np.allclose(rd["chol"].reshape(13, 4, 4), rb["chol"].reshape(13, d, 4, 4)[:, j], rtol=1e-10, atol=1e-12)
+        assert np.allclose(rd["output_scale"][:, 0], rb["output_scale"][:, j], rtol=1e-12)
+
+
 def _marginalise_np(A, nm, nc, tl, to, m, L):
     """Dense LatentCond.marginalise (ssm_impl_dense.py:24-33) on covariances."""
     mean = tl[:, None] * m
