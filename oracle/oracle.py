@@ -22,12 +22,16 @@ _LIB_PATHS = {
     "strict": _LIB_PATH,
     "fma": os.path.join(_HERE, "_build", "libpdq_oracle_fma.so"),
     "extended": os.path.join(_HERE, "_build", "libpdq_oracle_ext.so"),
+    # the strict build at -O3 for x86-64-v3 (AVX2, FMA units; contraction still off, so the
+    # arithmetic and the step sequence are the strict build's): the CPU baseline's timing leg
+    "fast": os.path.join(_HERE, "_build", "libpdq_oracle_fast.so"),
 }
 # scalar type of each build: (numpy dtype, ctypes type)
 _REAL = {
     "strict": (np.float64, ctypes.c_double),
     "fma": (np.float64, ctypes.c_double),
     "extended": (np.longdouble, ctypes.c_longdouble),
+    "fast": (np.float64, ctypes.c_double),
 }
 
 FACT = {"dense": 0, "isotropic": 1, "blockdiag": 2}

@@ -1,11 +1,8 @@
 """Compile the gfx950 HIP library (libpdq.so) in-tree with hipcc.
 
 The translation units (C ABI, cooperative dense/isotropic kernel,
-lane-per-instance isotropic kernel) compile in parallel and link into one
-shared library.  The per-block blockdiag kernel
-(csrc/pdq_blockdiag.hip) is not part of the device library: it did not finish
-on its first device run and stays off the device until that is explained; its
-code is still compiled for the CPU by the kernel-emulation tests.
+lane-per-instance isotropic kernel, block-diagonal kernel) compile in
+parallel and link into one shared library.
 """
 
 from __future__ import annotations
@@ -32,6 +29,10 @@ UNITS = [
     "pdq_lane_offgrid.hip",
     "pdq_post.hip",
     "pdq_lane_sample.hip",
+    "pdq_bd.hip",
+    "pdq_bd_filter.hip",
+    "pdq_bd_fixedpoint.hip",
+    "pdq_bd_fixedinterval.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -41,20 +41,6 @@ struct KArgs {
 
 enum { MODE_SAVE_AT = 0, MODE_FIXED_GRID = 1, MODE_LML = 2 };
 
-struct BArgs {
-  pdq_config cfg;
-  int n_inst, K, mode, nblk_threads;
-  const double* params;
-  const double* tcoeffs;
-  const double* times;
-  const double* A1;  // (nu+1)^2 latent transition
-  const double* Q1;  // (nu+1)^2 latent noise factor
-  pdq_solution out;
-  int* queue;
-};
-
-enum { BMODE_SAVE_AT = 0, BMODE_FIXED_GRID = 1 };
-
 // Arguments of the lane-per-instance isotropic kernel (pdq_lane.h).
 struct LArgs {
   pdq_config cfg;
@@ -76,7 +62,6 @@ int num_cus();
 
 // Launchers (one per translation unit, compiled in parallel).
 int launch_coop(KArgs& a, int G, hipStream_t st);
-int launch_bd(BArgs& a, hipStream_t st);
 // Lane-per-instance isotropic kernel (pdq_lane.hip): shapes (N = nu + 1, d)
 // it is instantiated for and its launcher.
 bool lane_supported(int N, int D);

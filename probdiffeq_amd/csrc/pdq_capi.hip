@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "pdq_args.h"
+#include "pdq_bd.h"
 #include "pdq_device.h"
 #include "pdq_iwp.h"
 
@@ -186,10 +187,6 @@ int check_common(const pdq_config* cfg, int n_inst, int K) {
       return set_err(PDQ_ERR_UNSUPPORTED, "the exponential prior supports (nu + 1) d <= 16 (got %d)",
                      (cfg->num_derivatives + 1) * cfg->ode_dim);
   }
-  if (cfg->constraint == PDQ_TS1 && cfg->factorisation == PDQ_BLOCKDIAG)
-    return set_err(PDQ_ERR_UNSUPPORTED,
-                   "ts1 is implemented for the dense and isotropic factorisations (blockdiag ts1 takes the "
-                   "Jacobian's diagonal, jacobians.py:108-113, 171-193)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
   return PDQ_OK;
@@ -245,12 +242,16 @@ int solve_validate(const pdq_config* cfg, int n_inst, const double* params, cons
   int rc = check_common(cfg, n_inst, K);
   if (rc) return rc;
   if (!out) return set_err(PDQ_ERR_INVALID, "out is NULL");
-  if (cfg->factorisation == PDQ_BLOCKDIAG)
-    // The per-block kernel (pdq_blockdiag.hip) did not finish on its first
-    // device run (LV, d = 2, nu = 4) and its cause is not shown; it is not
-    // linked into the device library and the factorisation is refused.
-    return set_err(PDQ_ERR_UNSUPPORTED,
-                   "the blockdiag factorisation is disabled on the device (its kernel did not finish on its first run)");
+  if (cfg->factorisation == PDQ_BLOCKDIAG) {
+    if (cfg->ode_dim > pdq::kBdMaxDim)
+      return set_err(PDQ_ERR_UNSUPPORTED, "blockdiag supports ode_dim <= %d (one lane per block; got %d)",
+                     pdq::kBdMaxDim, cfg->ode_dim);
+    if (cfg->num_derivatives > 5)
+      return set_err(PDQ_ERR_UNSUPPORTED, "blockdiag supports num_derivatives in [1, 5] (got %d)", cfg->num_derivatives);
+    if (cfg->init_std_rows)
+      return set_err(PDQ_ERR_UNSUPPORTED, "init_std_rows (a std per row) is implemented for the dense and isotropic "
+                                          "factorisations; blockdiag takes init_std");
+  }
   if (n_inst == 0) return PDQ_OK;
   if (!tcoeffs || !times || (cfg->n_params > 0 && !params))
     return set_err(PDQ_ERR_INVALID, "tcoeffs, times and params must be device pointers");
@@ -271,6 +272,21 @@ int solve_run(const pdq_config* cfg, int n_inst, const double* params, const dou
               int K, const pdq_solution* out, const double* A, const double* Q, int* queue, hipStream_t st,
               int mode) {
   const pdq::Layout L = pdq::make_layout(*cfg, 0);
+  if (cfg->factorisation == PDQ_BLOCKDIAG) {
+    // one lane per block, one lane group per instance (pdq_bd.h)
+    pdq::BdArgs ba{};
+    ba.cfg = *cfg;
+    ba.n_inst = n_inst;
+    ba.K = K;
+    ba.mode = mode == pdq::MODE_FIXED_GRID ? pdq::LMODE_FIXED_GRID : pdq::LMODE_SAVE_AT;
+    ba.G = pdq::bd_group_lanes(cfg->ode_dim);
+    ba.params = params;
+    ba.tcoeffs = tcoeffs;
+    ba.times = times;
+    ba.Q = Q;
+    ba.out = *out;
+    return pdq::launch_bd(ba, st);
+  }
   const int G = pick_lanes(*cfg, L, n_inst, mode);
   if (G == 1) {
     if (cfg->factorisation != PDQ_ISOTROPIC || !pdq::lane_supported(L.N, L.d))
@@ -321,17 +337,30 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
   return solve_run(cfg, n_inst, params, tcoeffs, times, K, out, m.A, m.Q, nullptr, (hipStream_t)stream, mode);
 }
 
+// One thread per instance.  Up to 512 coefficients ((num + 1) d) the
+// normalised coefficients c_k = u^(k) / k! live in a private array; beyond,
+// they are formed in place in the instance's output rows, with the same
+// arithmetic, and scaled by k! in a last pass.
 __global__ void taylor_kernel(int vf, int d, int np, int n_inst, const double* params, const double* y0, double t0,
                               int num, double* out) {
   const int inst = blockIdx.x * blockDim.x + threadIdx.x;
   if (inst >= n_inst) return;
-  double c[8 * 64];
   const double* prm = params + (size_t)inst * np;
-  for (int j = 0; j < d; ++j) c[j] = y0[(size_t)inst * d + j];
+  double* o = out + (size_t)inst * (num + 1) * d;
+  if ((num + 1) * d <= 8 * 64) {
+    double c[8 * 64];
+    for (int j = 0; j < d; ++j) c[j] = y0[(size_t)inst * d + j];
+    for (int k = 0; k < num; ++k)
+      for (int j = 0; j < d; ++j) c[(k + 1) * d + j] = pdq::taylor_rhs(vf, d, c, prm, t0, k, j) / (double)(k + 1);
+    for (int k = 0; k <= num; ++k)
+      for (int j = 0; j < d; ++j) o[(size_t)k * d + j] = pdq::factorial_d(k) * c[k * d + j];
+    return;
+  }
+  for (int j = 0; j < d; ++j) o[j] = y0[(size_t)inst * d + j];
   for (int k = 0; k < num; ++k)
-    for (int j = 0; j < d; ++j) c[(k + 1) * d + j] = pdq::taylor_rhs(vf, d, c, prm, t0, k, j) / (double)(k + 1);
+    for (int j = 0; j < d; ++j) o[(size_t)(k + 1) * d + j] = pdq::taylor_rhs(vf, d, (const double*)o, prm, t0, k, j) / (double)(k + 1);
   for (int k = 0; k <= num; ++k)
-    for (int j = 0; j < d; ++j) out[((size_t)inst * (num + 1) + k) * d + j] = pdq::factorial_d(k) * c[k * d + j];
+    for (int j = 0; j < d; ++j) o[(size_t)k * d + j] = pdq::factorial_d(k) * o[(size_t)k * d + j];
 }
 
 }  // namespace
@@ -631,9 +660,9 @@ int32_t pdq_lml_terminal_values(const pdq_config* cfg, int32_t n_inst, const dou
 int32_t pdq_taylor_coefficients(int32_t vector_field, int32_t d, int32_t n_params, int32_t n_inst,
                                 const double* params, const double* y0, double t0, int32_t num, double* out,
                                 void* stream) {
-  // taylor_kernel keeps (num + 1) d coefficients in a private array of 512
   if (num < 0 || d < 1 || n_inst < 0) return set_err(PDQ_ERR_INVALID, "invalid sizes");
-  if ((int64_t)(num + 1) * d > 8 * 64) return set_err(PDQ_ERR_UNSUPPORTED, "(num+1)*d must not exceed 512");
+  if (num > 7) return set_err(PDQ_ERR_UNSUPPORTED, "num must not exceed 7 (got %d)", num);
+  if (d > (1 << 20)) return set_err(PDQ_ERR_UNSUPPORTED, "d must not exceed 2^20 (got %d)", d);
   const int rc = check_vector_field(vector_field, d, n_params);
   if (rc) return rc;
   int ndev = 0;

@@ -19,7 +19,7 @@ def build(extra=()) -> str:
     # compiled to a private name and renamed into place, so that concurrent
     # test processes never load a half-written library
     tmp = f"{LIB}.{os.getpid()}.tmp"
-    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-DPDQ_HOST_EMU", *extra, "-o", tmp, src]
+    cmd = ["g++", "-O1", "-std=c++20", "-pthread", "-fPIC", "-shared", "-DPDQ_HOST_EMU", *extra, "-o", tmp, src]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB
@@ -100,6 +100,21 @@ def lml(cfg, params, y0, grid, every, data, stds, average=True, grad=True, kerne
         rc = lib().pdqemu_lml_lane(ctypes.byref(c), B, P(params.ctypes.data), P(y0.ctypes.data), P(grid.ctypes.data), grid.size, every, P(data.ctypes.data), P(stds.ctypes.data), int(average), P(value.ctypes.data), P(g.ctypes.data) if grad else None, tpl)
     else:
         rc = lib().pdqemu_lml(ctypes.byref(c), B, P(params.ctypes.data), P(y0.ctypes.data), P(grid.ctypes.data), grid.size, every, P(data.ctypes.data), P(stds.ctypes.data), n_std, int(average), P(value.ctypes.data), P(g.ctypes.data) if grad else None)
+    if rc:
+        raise RuntimeError(f"emulation rc {rc}")
+    return value, g
+
+
+def lml_adjoint(cfg, params, y0, grid, every, data, stds, average=True, grad=True, lanes=3):
+    """The reverse-mode adjoint LML kernel's code (pdq_lane_adjoint.h) on the CPU; `lanes` scratch
+    columns shared by the instances in grid-stride order, as on the device."""
+    params, y0, grid, data, stds = (np.ascontiguousarray(x, dtype=np.float64) for x in (params, y0, grid, data, stds))
+    B = params.shape[0]
+    value = np.zeros(B)
+    g = np.zeros((B, params.shape[1] + y0.shape[1])) if grad else None
+    c = cfg.c(host=True)
+    P = ctypes.c_void_p
+    rc = lib().pdqemu_lml_adjoint(ctypes.byref(c), B, P(params.ctypes.data), P(y0.ctypes.data), P(grid.ctypes.data), grid.size, every, P(data.ctypes.data), P(stds.ctypes.data), int(average), P(value.ctypes.data), P(g.ctypes.data) if grad else None, lanes)
     if rc:
         raise RuntimeError(f"emulation rc {rc}")
     return value, g

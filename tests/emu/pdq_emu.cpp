@@ -2,14 +2,17 @@
 // Compiled with g++ -DPDQ_HOST_EMU from the product's device sources; runs
 // every instance with one lane (G = 1) so that the kernel logic can be
 // compared with the oracle on the CPU.
+#include <barrier>
 #include <cstring>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
-#include "../../probdiffeq_amd/csrc/pdq_blockdiag.hip"
+#include "../../probdiffeq_amd/csrc/pdq_bd.h"
 #include "../../probdiffeq_amd/csrc/pdq_coop.hip"
 #include "../../probdiffeq_amd/csrc/pdq_iwp.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane.h"
+#include "../../probdiffeq_amd/csrc/pdq_lane_adjoint.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane_lml.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane_offgrid.h"
 #include "../../probdiffeq_amd/csrc/pdq_lane_sample.h"
@@ -19,6 +22,72 @@ namespace pdq {
 int host_error(int code, const char*, ...) { return code; }
 int num_cus() { return 1; }
 }  // namespace pdq
+
+// The block-diagonal kernel's lane group on the CPU: one thread per lane, a
+// real barrier, and lane exchanges through a shared array with HIP's
+// __shfl_xor / __shfl semantics (a source lane outside the caller's width
+// segment reads the caller's own value).
+namespace {
+std::barrier<>* g_group_barrier = nullptr;
+double g_group_xchg[pdq::kBdMaxDim];
+thread_local int t_group_lane = 0;
+}  // namespace
+
+void pdq_emu_group_barrier() { g_group_barrier->arrive_and_wait(); }
+
+double pdq_emu_shfl_xor(double v, int mask, int width) {
+  const int self = t_group_lane;
+  g_group_xchg[self] = v;
+  g_group_barrier->arrive_and_wait();
+  int idx = self ^ mask;
+  if (idx >= ((self + width) & ~(width - 1))) idx = self;
+  const double r = g_group_xchg[idx];
+  g_group_barrier->arrive_and_wait();
+  return r;
+}
+
+double pdq_emu_shfl(double v, int src, int width) {
+  const int self = t_group_lane;
+  g_group_xchg[self] = v;
+  g_group_barrier->arrive_and_wait();
+  const double r = g_group_xchg[(self & ~(width - 1)) + src];
+  g_group_barrier->arrive_and_wait();
+  return r;
+}
+
+namespace {
+template <int N, int S>
+void bd_emu_instance(const pdq::BdArgs& a, int inst) {
+  const int G = a.G;
+  std::vector<double> y(G, 0.0), red(pdq::kBdMaxDim / 64, 0.0);
+  std::barrier<> bar(G);
+  g_group_barrier = &bar;
+  std::vector<std::thread> th;
+  th.reserve(G);
+  for (int l = 0; l < G; ++l)
+    th.emplace_back([&, l] {
+      t_group_lane = l;
+      pdq::BdGroup g;
+      g.G = G;
+      g.lane = l;
+      g.multi = G > 64;
+      g.y = y.data();
+      g.red = red.data();
+      pdq::bd_solve_instance<N, S>(a, g, inst);
+    });
+  for (auto& t : th) t.join();
+  g_group_barrier = nullptr;
+}
+
+template <int N>
+void bd_emu_n(const pdq::BdArgs& a, int inst) {
+  switch (a.cfg.strategy) {
+    case PDQ_FILTER: return bd_emu_instance<N, PDQ_FILTER>(a, inst);
+    case PDQ_FIXEDPOINT: return bd_emu_instance<N, PDQ_FIXEDPOINT>(a, inst);
+    default: return bd_emu_instance<N, PDQ_FIXEDINTERVAL>(a, inst);
+  }
+}
+}  // namespace
 
 extern "C" {
 
@@ -105,33 +174,31 @@ int pdqemu_solve(const pdq_config* cfg, int n_inst, const double* params, const 
                  int K, const pdq_solution* out, int mode) {
   int queue = 0;
   if (cfg->factorisation == PDQ_BLOCKDIAG) {
+    // the block-diagonal kernel's code, one instance at a time, each as a
+    // group of G threads (one per block)
     std::vector<double> A1, Q1;
     pdq::iwp_matrices(cfg->num_derivatives, A1, Q1);
-    pdq::BArgs a{};
+    pdq::BdArgs a{};
     a.cfg = *cfg;
     a.n_inst = n_inst;
     a.K = K;
-    a.mode = mode;
+    a.mode = mode == 1 ? pdq::LMODE_FIXED_GRID : pdq::LMODE_SAVE_AT;
+    a.G = pdq::bd_group_lanes(cfg->ode_dim);
     a.params = params;
     a.tcoeffs = tcoeffs;
     a.times = times;
-    a.A1 = A1.data();
-    a.Q1 = Q1.data();
+    a.Q = Q1.data();
     a.out = *out;
-    a.queue = &queue;
-    std::vector<double> y(cfg->ode_dim), red(16);
-    const int N = cfg->num_derivatives + 1, d = cfg->ode_dim;
+    if (cfg->ode_dim > pdq::kBdMaxDim) return 2;
     for (int i = 0; i < n_inst; ++i) {
-#define PDQ_EMU_BD(NN, BB)                                               \
-  if (N == NN && d <= BB) {                                              \
-    pdq::emu_solve_instance_bd<NN, BB>(a, i, y.data(), red.data());       \
-    continue;                                                            \
-  }
-      PDQ_EMU_BD(3, 2)
-      PDQ_EMU_BD(3, 64)
-      PDQ_EMU_BD(5, 2)
-      PDQ_EMU_BD(4, 2)
-      return 2;
+      switch (cfg->num_derivatives + 1) {
+        case 2: bd_emu_n<2>(a, i); break;
+        case 3: bd_emu_n<3>(a, i); break;
+        case 4: bd_emu_n<4>(a, i); break;
+        case 5: bd_emu_n<5>(a, i); break;
+        case 6: bd_emu_n<6>(a, i); break;
+        default: return 2;
+      }
     }
     return 0;
   }
@@ -295,6 +362,52 @@ int pdqemu_lml_lane(const pdq_config* cfg, int n_inst, const double* params, con
   return 0;
 }
 
+
+// The reverse-mode adjoint of the lane LML (pdq_lane_adjoint.h), one lane per
+// instance, `lanes` scratch columns (instances grid-stride over them as on
+// the device).
+int pdqemu_lml_adjoint(const pdq_config* cfg, int n_inst, const double* params, const double* y0, const double* grid,
+                       int K, int every, const double* data, const double* stds, int average, double* value,
+                       double* grad, int lanes) {
+  const int n_tan = grad ? cfg->n_params + cfg->ode_dim : 0;
+  const int N = cfg->num_derivatives + 1, D = cfg->ode_dim;
+  if (cfg->factorisation != PDQ_ISOTROPIC || D != 2 || (N != 4 && N != 5) || lanes < 1) return 2;
+  std::vector<double> A, Q;
+  pdq::iwp_matrices(cfg->num_derivatives, A, Q);
+  std::vector<double> tinit = emu_taylor_tangents(cfg, n_inst, params, y0, grid, n_tan);
+  const size_t lsz = N == 4 ? pdq::AdjLayout<4, 2>(K, every).size : pdq::AdjLayout<5, 2>(K, every).size;
+  std::vector<double> scratch(lsz * (size_t)lanes, 0.0);
+  pdq::LmlArgs a{};
+  a.cfg = *cfg;
+  a.cfg.strategy = PDQ_FIXEDPOINT;
+  a.n_inst = n_inst;
+  a.K = K;
+  a.every = every;
+  a.n_tan = n_tan;
+  a.average_pdfs = average;
+  a.G = 1;
+  a.params = params;
+  a.tcoeffs = tinit.data();
+  a.times = grid;
+  a.data = data;
+  a.stds = stds;
+  a.Q = Q.data();
+  a.value = value;
+  a.grad = grad;
+  a.scratch = scratch.data();
+  pdq::LArgs la{};
+  la.cfg = a.cfg;
+  la.n_inst = n_inst;
+  la.K = K;
+  la.Q = Q.data();
+  std::vector<double> slots(64 * 2 * 256);
+  for (int inst = 0; inst < n_inst; ++inst) {
+    const size_t lane = (size_t)inst % lanes;
+    if (N == 4) pdq::lane_adj_instance<4, 2>(la, a, inst, slots.data(), lane, (size_t)lanes);
+    else pdq::lane_adj_instance<5, 2>(la, a, inst, slots.data(), lane, (size_t)lanes);
+  }
+  return 0;
+}
 
 // The off-grid marginal kernel (pdq_lane_offgrid.h) on the CPU, one query
 // after another.  Returns 2 for a shape or strategy it does not cover.

@@ -65,12 +65,12 @@ def test_validation_errors_are_reported():
     from probdiffeq_amd import capi
 
     lib = capi.lib()
-    cfg = capi.Config(factorisation="blockdiag", constraint="ts1")
+    cfg = capi.Config(factorisation="isotropic", prior="exponential")
     c = cfg.c()
     sol = capi._Solution()
     rc = lib.pdq_solve_adaptive_save_at(ctypes.byref(c), 1, None, None, None, 10, ctypes.byref(sol), None)
     assert rc == 2  # PDQ_ERR_UNSUPPORTED
-    assert "ts1" in capi.last_error()
+    assert "dense factorisation" in capi.last_error()
     cfg = capi.Config(num_derivatives=12)
     c = cfg.c()
     rc = lib.pdq_solve_adaptive_save_at(ctypes.byref(c), 1, None, None, None, 10, ctypes.byref(sol), None)
@@ -235,13 +235,13 @@ def test_solve_wrappers_refuse_host_tensors():
         (3, 4, 2, 2, "d params"),  # linear with n_params < d
         (9, 2, 4, 4, "unknown vector field"),
         (0, 2, 4, -1, "invalid sizes"),
-        (0, 2, 4, 2**30, "512"),  # (num + 1) d overflows int without 64-bit arithmetic
-        (2, 300, 2, 1, "512"),
+        (0, 2, 4, 2**30, "num must not exceed 7"),  # (num + 1) d would overflow an int
+        (2, 2**21, 2, 1, "2^20"),
     ],
 )
 def test_taylor_coefficients_checks_its_arguments(vf, d, n_params, num, match):
-    """pdq_taylor_coefficients applies the vector-field registry's requirements and bounds num
-    before any device work (its kernel indexes params and a 512-entry private array by them)."""
+    """pdq_taylor_coefficients applies the vector-field registry's requirements and bounds num and d
+    before any device work (its kernel indexes params and the coefficient rows by them)."""
     from probdiffeq_amd import capi
 
     lib = capi.lib()

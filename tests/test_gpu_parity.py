@@ -157,14 +157,16 @@ def check_accuracy(got, ref, fma, truth, label):
     assert e_gpu <= tol and d <= tol, f"{label}: rel(gpu, truth) {e_gpu:.3e}, rel(gpu, ref) {d:.3e} > tol {tol:.3e} (e_ref {e_ref:.3e})"
 
 
-def _check_fields(cfg, got, ref, fma, truth, label):
+def _check_fields(cfg, got, ref, fma, truth, label, skip=()):
     fr, ff, ft = _fields(cfg, ref), _fields(cfg, fma), _fields(cfg, truth)
     for key, exp in fr.items():
+        if key in skip:
+            continue
         g = got[key][: exp.shape[0]] if key == "output_scale" else got[key]
         check_accuracy(g, exp, ff[key], ft[key], f"{label} {key}")
 
 
-def _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=False, check_factor_signs=False):
+def _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=False, check_factor_signs=False, skip=()):
     """Solve a batch on the GPU and every instance with the oracle; compare."""
     B, K = times.shape
     ocfg = _oracle_cfg(oracle, cfg)
@@ -181,7 +183,7 @@ def _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=False, check_facto
         assert np.array_equal(g["num_steps"][i], r["num_steps"])
         assert _rel(g["t"][i], r["t"]) < 1e-14
         got = {k: g[k][i] for k in ("mean", "chol", "output_scale", "filt_mean", "filt_chol", "cond_A", "cond_mean", "term_mean") if k in g}
-        _check_fields(cfg, _fields(cfg, got), r, rf, rt, f"{cfg.factorisation}/{cfg.strategy}/{cfg.solver} instance {i}")
+        _check_fields(cfg, _fields(cfg, got), r, rf, rt, f"{cfg.factorisation}/{cfg.strategy}/{cfg.solver} instance {i}", skip)
         if check_factor_signs:
             # The R-only QR's sign convention ties each column's sign to an
             # element that can be at cancellation level: the strict and FMA
@@ -253,14 +255,14 @@ def test_fixed_grid(oracle, pdq, fact, lanes):
     _compare(oracle, pdq, cfg, params, y0, times, fixed_grid=True)
 
 
-@pytest.mark.skip(reason="blockdiag device kernel did not finish within the 120 s test limit on its first GPU run; the cause is not shown, so the factorisation is disabled on the device (its kernel logic is checked by the CPU emulation tests)")
+@pytest.mark.skip(reason="this case did not finish within the 120 s test limit on the first blockdiag kernel's first GPU run and its cause was not shown; that kernel is removed and the case is not run on the device again (the current blockdiag kernel is tested by test_gpu_blockdiag.py)")
 def test_blockdiag_lotka_volterra(oracle, pdq):
     cfg = pdq.Config(factorisation="blockdiag", num_derivatives=4, strategy="filter", solver="dynamic")
     params, y0, times = _lv_batch(5, seed=5)
     _compare(oracle, pdq, cfg, params, y0, times)
 
 
-@pytest.mark.skip(reason="blockdiag device kernel did not finish within the 120 s test limit on its first GPU run; the cause is not shown, so the factorisation is disabled on the device (its kernel logic is checked by the CPU emulation tests)")
+@pytest.mark.skip(reason="this case did not finish within the 120 s test limit on the first blockdiag kernel's first GPU run and its cause was not shown; that kernel is removed and the case is not run on the device again (the current blockdiag kernel is tested by test_gpu_blockdiag.py)")
 def test_blockdiag_burgers_small_grid(oracle, pdq):
     """C4 kind at a small grid: periodic Burgers, blockdiag nu=2, filter."""
     Ng = 64
@@ -274,13 +276,18 @@ def test_blockdiag_burgers_small_grid(oracle, pdq):
     _compare(oracle, pdq, cfg, params, y0, times)
 
 
-def test_blockdiag_is_refused_on_the_device(pdq):
-    """The C ABI refuses blockdiag before any allocation or launch."""
-    cfg = pdq.Config(factorisation="blockdiag", num_derivatives=4, strategy="filter", solver="dynamic")
-    params, y0, times = _lv_batch(1, seed=5)
-    tcs = np.zeros((1, 5, 2))
-    with pytest.raises(pdq.PdqError, match="blockdiag"):
-        pdq.solve(cfg, _dev(params), _dev(tcs), _dev(times))
+def test_blockdiag_limits_are_refused(pdq):
+    """The C ABI refuses block-diagonal shapes beyond the kernel's limits before any launch."""
+    tcs = np.zeros((1, 3, 600))
+    for kw, match in [
+        (dict(ode_dim=600, vector_field="burgers", n_params=2, num_derivatives=2), "ode_dim"),
+        (dict(ode_dim=2, num_derivatives=6), "num_derivatives"),
+    ]:
+        cfg = pdq.Config(factorisation="blockdiag", strategy="filter", solver="dynamic", **kw)
+        p = np.zeros((1, cfg.n_params))
+        tc = np.zeros((1, cfg.N, cfg.ode_dim))
+        with pytest.raises(pdq.PdqError, match=match):
+            pdq.solve(cfg, _dev(p), _dev(tc), _dev(np.array([[0.0, 1.0]])))
 
 
 @pytest.mark.parametrize(

@@ -99,6 +99,93 @@ def test_blockdiag_burgers(oracle, emu):
     _run(oracle, emu, cfg, params, y0, np.tile([0.0, 0.2], (2, 1)))
 
 
+def _burgers(Ng, B, seed=0, nmodes=8):
+    """The C4 initial condition at Ng points: sum of 8 sinusoids with N(0, 1/k^2) amplitudes."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(Ng) / Ng
+    y0 = np.stack([sum(rng.normal(0, 1 / k) * np.sin(2 * np.pi * k * x) for k in range(1, nmodes + 1)) for _ in range(B)])
+    return np.tile([0.01, 1.0 / Ng], (B, 1)), y0
+
+
+@pytest.mark.parametrize("strategy", ["filter", "fixedpoint", "fixedinterval"])
+@pytest.mark.parametrize("solver", ["solver", "dynamic", "mle"])
+def test_blockdiag_strategy_solver_grid(oracle, emu, strategy, solver):
+    """The block-diagonal kernel (one lane per block, a group per instance) for every strategy and
+    solver: Lotka-Volterra, d = 2 (G = 2), nu = 3."""
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="blockdiag", num_derivatives=3, strategy=strategy, solver=solver, rtol=1e-5, atol=1e-7)
+    p, y, t = T._lv_batch(3, seed=17, t_lo=3.0, t_hi=8.0, K=9)
+    _run(oracle, emu, cfg, p, y, t)
+
+
+@pytest.mark.parametrize("strategy", ["filter", "fixedpoint"])
+def test_blockdiag_multi_wave_group(oracle, emu, strategy):
+    """d = 100 > 64: one group of G = 128 lanes per instance (two wavefronts' worth, 28 padding
+    lanes), workgroup barriers and the two-level sum; Burgers, nu = 2."""
+    from probdiffeq_amd import capi
+
+    params, y0 = _burgers(100, 2, seed=3)
+    cfg = capi.Config(factorisation="blockdiag", num_derivatives=2, ode_dim=100, vector_field="burgers", n_params=2, strategy=strategy, rtol=1e-4, atol=1e-6)
+    _run(oracle, emu, cfg, params, y0, np.tile([0.0, 0.05, 0.1], (2, 1)))
+
+
+@pytest.mark.parametrize("strategy", ["filter", "fixedinterval"])
+def test_blockdiag_fixed_grid(oracle, emu, strategy):
+    from probdiffeq_amd import capi
+
+    p, y, _ = T._lv_batch(2, seed=23)
+    cfg = capi.Config(factorisation="blockdiag", num_derivatives=4, strategy=strategy, solver="dynamic")
+    _run(oracle, emu, cfg, p, y, np.tile(np.linspace(0.0, 4.0, 41), (2, 1)), fixed_grid=True)
+
+
+@pytest.mark.parametrize(
+    "vf,d,strategy",
+    [("logistic", 5, "filter"), ("logistic", 5, "fixedpoint"), ("lotka_volterra", 2, "filter"), ("burgers", 16, "filter")],
+)
+def test_blockdiag_ts1(oracle, emu, vf, d, strategy):
+    """Blockdiag ts1 with the exact Jacobian diagonal (BlockDiagResidual.linearize,
+    ssm_impl_blockdiag.py:147-183, jacobians.py:108-113)."""
+    from probdiffeq_amd import capi
+
+    rng = np.random.default_rng(29)
+    B = 2
+    if vf == "logistic":
+        params = np.full((B, 1), 1.5)
+        y0 = rng.uniform(0.1, 0.9, size=(B, d))
+        times = np.tile(np.linspace(0.0, 4.0, 6), (B, 1))
+        n_params = 1
+    elif vf == "burgers":
+        params, y0 = _burgers(d, B, seed=5, nmodes=3)
+        times = np.tile([0.0, 0.1], (B, 1))
+        n_params = 2
+    else:
+        params, y0, times = T._lv_batch(B, seed=31, t_lo=3.0, t_hi=6.0, K=7)
+        n_params = 4
+    cfg = capi.Config(factorisation="blockdiag", num_derivatives=3, ode_dim=d, vector_field=vf, n_params=n_params, constraint="ts1", strategy=strategy, rtol=1e-5, atol=1e-7)
+    _run(oracle, emu, cfg, params, y0, times)
+
+
+@pytest.mark.parametrize(
+    "solver,kw",
+    [
+        ("dynamic", dict(clip_dt=True)),
+        ("dynamic", dict(error_norm="rms_then_scale")),
+        ("dynamic", dict(control="proportional_integral")),
+        ("dynamic", dict(error_estimator="state_std", error_derivative_idx=2)),
+        ("solver", dict(error_estimator="state_std", error_re_linearize=True)),
+        ("mle", dict(error_re_linearize=True, error_per_unit_step=True)),
+        ("dynamic", dict(re_linearize_after_calibration=True)),
+    ],
+)
+def test_blockdiag_options(oracle, emu, solver, kw):
+    from probdiffeq_amd import capi
+
+    cfg = capi.Config(factorisation="blockdiag", num_derivatives=3, strategy="fixedpoint", solver=solver, rtol=1e-5, atol=1e-7, **kw)
+    p, y, t = T._lv_batch(2, seed=37, t_lo=3.0, t_hi=6.0, K=7)
+    _run(oracle, emu, cfg, p, y, t)
+
+
 def test_status_codes(emu):
     from oracle import oracle as O
     from probdiffeq_amd import capi
@@ -131,6 +218,28 @@ def test_lml_value_and_gradient(oracle, emu, fact, solver, per_coordinate_std, a
         T.check_accuracy(v[i], vo, vf, vt, f"emulation lml instance {i}")
         gr = LC.richardson_grad(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
         assert LC.rel(g[i], gr) < LC.GRAD_RTOL, (g[i], gr)
+
+
+@pytest.mark.parametrize("nu,solver,average", [(3, "solver", True), (3, "dynamic", False), (4, "solver", True)])
+def test_lml_reverse_adjoint(oracle, emu, nu, solver, average):
+    """The reverse-mode adjoint of the lane LML (pdq_lane_adjoint.h: checkpoints every 50 grid steps,
+    the forward factors recomputed per segment, the scan's and the steps' adjoints): its value is the
+    forward LML's bitwise, its gradient equals the forward-mode tangents to rounding and the long
+    double Richardson differences of the oracle to 1e-10.  Two scratch columns for three instances
+    exercise the reuse of a lane's storage by the next instance."""
+    import lml_cases as LC
+    from probdiffeq_amd import capi
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(3, seed=3, t_end=10.0, n_grid=501, every=10)
+    cfg = capi.Config(factorisation="isotropic", num_derivatives=nu, ode_dim=2, strategy="fixedpoint", solver=solver)
+    v_f, g_f = emu.lml(cfg, params, y0, grid, every, data, stds, average=average, kernel="lane")
+    v_r, g_r = emu.lml_adjoint(cfg, params, y0, grid, every, data, stds, average=average, lanes=2)
+    assert np.array_equal(v_r, v_f)
+    assert LC.rel(g_r, g_f) < 1e-13
+    ocfg = T._oracle_cfg(oracle, cfg)
+    for i in range(2):
+        gr = LC.richardson_grad(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i], average)
+        assert LC.rel(g_r[i], gr) < LC.GRAD_RTOL, (g_r[i], gr)
 
 
 @pytest.mark.parametrize("fact,idx,grad", [("isotropic", 1, True), ("isotropic", 2, True), ("isotropic", 3, False), ("dense", 1, True), ("dense", 2, True), ("dense", 3, False)])
