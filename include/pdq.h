@@ -176,6 +176,21 @@ int32_t pdq_solve_adaptive_save_at(const pdq_config* cfg, int32_t n_inst, const 
                                    const double* tcoeffs, const double* save_at, int32_t K,
                                    const pdq_solution* out, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): pdq_solve_adaptive_save_at with the same arguments and results,
+ * but it calls no hipMalloc, hipFree or other allocating routine.  What the
+ * allocating entry point allocates (the IWP matrices, the work-queue counter)
+ * lives in a fixed 16 640-byte device array per GPU, held until the process
+ * ends; its address is looked up by pdq_init or by the first call on a device.
+ * Limit: state dimension n <= 32 (dense (nu+1) d, isotropic nu+1), the
+ * kernels' own limit; larger n returns PDQ_ERR_UNSUPPORTED.  Every call on a
+ * GPU rewrites that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller. */
+int32_t pdq_solve_adaptive_save_at_noalloc(const pdq_config* cfg, int32_t n_inst, const double* params,
+                                           const double* tcoeffs, const double* save_at, int32_t K,
+                                           const pdq_solution* out, void* stream);
+
 /* Replaces ivpsolve.solve_fixed_grid(solver=)(prior, grid=, damp=)
  * (probdiffeq/_ivpsolve/solvers_via_fixed_steps.py:11-34).
  *   grid: n_inst x K */

@@ -254,6 +254,7 @@ EXPORTED = [
     "pdq_init",
     "pdq_last_error",
     "pdq_solve_adaptive_save_at",
+    "pdq_solve_adaptive_save_at_noalloc",
     "pdq_solve_fixed_grid",
     "pdq_lml_value_and_grad",
     "pdq_taylor_coefficients",
@@ -364,8 +365,12 @@ def _solution_struct(out: dict) -> _Solution:
     return s
 
 
-def solve(cfg: Config, params, tcoeffs, times, *, fixed_grid=False, out=None, stream=None):
-    """Batched solve; params (B, P), tcoeffs (B, N, d), times (B, K) device tensors."""
+def solve(cfg: Config, params, tcoeffs, times, *, fixed_grid=False, out=None, stream=None, no_alloc=False):
+    """Batched solve; params (B, P), tcoeffs (B, N, d), times (B, K) device tensors.
+
+    no_alloc=True calls pdq_solve_adaptive_save_at_noalloc (save_at only): the
+    library allocates no device memory for the call (out, if given, is the
+    caller's)."""
     if times.dim() != 2:
         raise PdqError(f"times must have shape (B, K) (got {tuple(times.shape)})")
     n_inst, K = times.shape
@@ -376,7 +381,12 @@ def solve(cfg: Config, params, tcoeffs, times, *, fixed_grid=False, out=None, st
         out = alloc_solution(cfg, n_inst, K, times.device)
     c = cfg.c()
     s = _solution_struct(out)
-    fn = lib().pdq_solve_fixed_grid if fixed_grid else lib().pdq_solve_adaptive_save_at
+    if no_alloc:
+        if fixed_grid:
+            raise PdqError("the no-allocation entry point solves on save_at points (pdq_solve_adaptive_save_at_noalloc)")
+        fn = lib().pdq_solve_adaptive_save_at_noalloc
+    else:
+        fn = lib().pdq_solve_fixed_grid if fixed_grid else lib().pdq_solve_adaptive_save_at
     check(fn(ctypes.byref(c), n_inst, p_ptr, tc_ptr, t_ptr, K, ctypes.byref(s), _stream(stream)))
     return out
 

@@ -337,6 +337,72 @@ int solve_common(const pdq_config* cfg, int n_inst, const double* params, const 
   return solve_run(cfg, n_inst, params, tcoeffs, times, K, out, m.A, m.Q, nullptr, (hipStream_t)stream, mode);
 }
 
+// ------------------------------------------------ no-allocation solve
+// pdq_solve_adaptive_save_at_noalloc keeps what pdq_solve_adaptive_save_at
+// allocates (the cooperative kernel's work-queue counter, and the IWP
+// matrices that device_mats caches) in `scratch`, a fixed array in device
+// memory, one per GPU for the life of the process:
+//   bytes [0, 256)                 work-queue counter (int), zeroed by a kernel per call
+//   bytes [256, 256 + 8 n^2)       A (n x n, n = the state dimension of one block)
+//   bytes [256 + 8 n^2, ...)       Q
+// Host code addresses it only through the pointer hipGetSymbolAddress returns
+// for the current device (g_noalloc_ptr), never by the array's name.
+constexpr int kNoAllocMaxDevices = 64;
+constexpr int kNoAllocMaxState = 32;  // n <= 32, the solve kernels' own limit
+constexpr size_t kNoAllocQueueBytes = 256;
+constexpr size_t kNoAllocScratchBytes =
+    kNoAllocQueueBytes + 2 * sizeof(double) * kNoAllocMaxState * kNoAllocMaxState;  // 16 640 B
+unsigned char* g_noalloc_ptr[kNoAllocMaxDevices];
+
+// The matrices travel as kernel arguments, chunk by chunk, so that filling
+// the workspace needs neither a host staging buffer nor a copy engine call.
+constexpr int kFillChunk = 448;
+struct FillChunk {
+  double* dst;
+  int n;
+  double v[kFillChunk];
+};
+
+__global__ void scratch_fill_kernel(FillChunk c) {
+  for (int i = threadIdx.x; i < c.n; i += blockDim.x) c.dst[i] = c.v[i];
+}
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// The current device's address of `scratch` (looked up once per device).
+int noalloc_scratch(unsigned char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+    return set_err(PDQ_ERR_UNSUPPORTED, "device %d: the no-allocation workspace serves devices 0..%d", dev,
+                   kNoAllocMaxDevices - 1);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_err(PDQ_ERR_HIP, "hipGetSymbolAddress returned no address for the workspace");
+    g_noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return PDQ_OK;
+}
+
+int fill_device(double* dst, const double* src, int count, hipStream_t st) {
+  for (int off = 0; off < count; off += kFillChunk) {
+    FillChunk c;
+    c.dst = dst + off;
+    c.n = std::min(kFillChunk, count - off);
+    std::memcpy(c.v, src + off, sizeof(double) * c.n);
+    hipLaunchKernelGGL(scratch_fill_kernel, dim3(1), dim3(256), 0, st, c);
+    HIP_TRY(hipGetLastError());
+  }
+  return PDQ_OK;
+}
+
 // One thread per instance.  Up to 512 coefficients ((num + 1) d) the
 // normalised coefficients c_k = u^(k) / k! live in a private array; beyond,
 // they are formed in place in the instance's output rows, with the same
@@ -383,7 +449,8 @@ int32_t pdq_system_matrices_1d_iwp(int32_t num_derivatives, double* A, double* Q
 int32_t pdq_init(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
-  return PDQ_OK;
+  unsigned char* ws = nullptr;
+  return noalloc_scratch(&ws);  // the current device's no-allocation workspace
 }
 
 size_t pdq_last_error(char* buf, size_t size) {
@@ -407,6 +474,36 @@ int32_t pdq_solve_adaptive_save_at(const pdq_config* cfg, int32_t n_inst, const 
                                    const double* tcoeffs, const double* save_at, int32_t K,
                                    const pdq_solution* out, void* stream) {
   return solve_common(cfg, n_inst, params, tcoeffs, save_at, K, out, stream, pdq::MODE_SAVE_AT);
+}
+
+int32_t pdq_solve_adaptive_save_at_noalloc(const pdq_config* cfg, int32_t n_inst, const double* params,
+                                           const double* tcoeffs, const double* save_at, int32_t K,
+                                           const pdq_solution* out, void* stream) {
+  int rc = solve_validate(cfg, n_inst, params, tcoeffs, save_at, K, out);
+  if (rc || n_inst == 0) return rc;
+  const pdq::Layout L = pdq::make_layout(*cfg, 0);
+  const int n = L.nc;
+  if (n > kNoAllocMaxState)
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "the no-allocation solve holds state dimensions up to %d in its %zu-byte workspace (got %d)",
+                   kNoAllocMaxState, kNoAllocScratchBytes, n);
+  unsigned char* ws = nullptr;
+  rc = noalloc_scratch(&ws);
+  if (rc) return rc;
+  double hA[kNoAllocMaxState * kNoAllocMaxState], hQ[kNoAllocMaxState * kNoAllocMaxState];
+  if (cfg->factorisation == PDQ_DENSE)
+    pdq::iwp_dense_into(cfg->num_derivatives, cfg->ode_dim, cfg->base_output_scale, hA, hQ);
+  else
+    pdq::iwp_matrices_into(cfg->num_derivatives, hA, hQ);
+  double* A = reinterpret_cast<double*>(ws + kNoAllocQueueBytes);
+  double* Q = A + (size_t)n * n;
+  hipStream_t st = (hipStream_t)stream;
+  rc = fill_device(A, hA, n * n, st);
+  if (!rc) rc = fill_device(Q, hQ, n * n, st);
+  if (rc) return rc;
+  // the queue counter is zeroed by the cooperative launcher's own kernel
+  return solve_run(cfg, n_inst, params, tcoeffs, save_at, K, out, A, Q, reinterpret_cast<int*>(ws), st,
+                   pdq::MODE_SAVE_AT);
 }
 
 int32_t pdq_solve_fixed_grid(const pdq_config* cfg, int32_t n_inst, const double* params, const double* tcoeffs,
