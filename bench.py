@@ -182,6 +182,32 @@ def bench_c5(args, torch, capi, flops):
         "traffic_source": C5_TRAFFIC_SOURCE,
         "ms_per_launch": ms,
     }
+    # the same batch through the reverse-mode adjoint (checkpoints every 50 grid steps)
+    cfg_r = capi.Config(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver", lml_gradient="reverse")
+    for _ in range(max(1, args.warmup)):
+        capi.lml_value_and_grad(cfg_r, P, Y0, G, 20, D, S)
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    e0.record()
+    for _ in range(reps):
+        val_r, grad_r = capi.lml_value_and_grad(cfg_r, P, Y0, G, 20, D, S)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_r = e0.elapsed_time(e1) / reps
+    fl_r = P.shape[0] * flops.lml_reverse(cfg_r, grid.size - 1, n_obs)
+    res["lml_grads_per_s_reverse"] = P.shape[0] / (ms_r / 1e3)
+    res["lml_reverse_ms_per_batch"] = ms_r
+    res["lml_reverse_vs_forward_grad_max_rel"] = float(((grad_r - grad).abs().amax(dim=1) / grad.abs().amax(dim=1)).max())
+    res["lml_reverse_roofline"] = {
+        "bound": "fp64_valu",
+        "achieved": fl_r / (ms_r / 1e3) / 1e12,
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": fl_r / (ms_r / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+        "flops_per_launch": fl_r,
+        "algorithmic_bytes": flops.lml_reverse_bytes(cfg_r, P.shape[0], n_obs, grid.size),
+        "ms_per_launch": ms_r,
+    }
     # plain: central differences (h = 1e-6) of single-instance value launches, 13 per gradient
     n_plain = min(args.plain_sample, 8)
     e0, e1 = _events()
@@ -485,7 +511,7 @@ def main():
     _progress(f"C2 done: {c2['ms_per_step']:.2f} ms per batch, plain {c2['plain_solves_per_s']:.1f} solves/s")
     c5 = {} if args.skip_lml else bench_c5(args, torch, capi, flops)
     if c5:
-        _progress(f"C5 done: {c5['lml_grads_per_s']:.1f} gradients/s")
+        _progress(f"C5 done: {c5['lml_grads_per_s']:.1f} gradients/s forward, {c5['lml_grads_per_s_reverse']:.1f} reverse")
     og = bench_offgrid(args, torch, capi, flops, params, y0, save_at) if not args.skip_offgrid else {}
     if og:
         _progress(f"off-grid marginals done: {og['offgrid_marginals_per_s']:.3e} per s")
@@ -552,6 +578,9 @@ def main():
             plain_fd_grads_per_s=c5["plain_fd_grads_per_s"],
             adjoint_vs_plain=c5["lml_grads_per_s"] / c5["plain_fd_grads_per_s"],
             lml_roofline=c5["lml_roofline"],
+            lml_grads_per_s_reverse=c5["lml_grads_per_s_reverse"],
+            lml_reverse_vs_forward_grad_max_rel=c5["lml_reverse_vs_forward_grad_max_rel"],
+            lml_reverse_roofline=c5["lml_reverse_roofline"],
         )
     line.update(og)
     print(json.dumps(line), flush=True)

@@ -44,6 +44,10 @@ enum { PDQ_ERROR_RESIDUAL_STD = 0, PDQ_ERROR_STATE_STD = 1 };
 /* Priors: integrated Wiener process (ssm_impl_*.py prior_wiener_integrated)
  * and the dense exponential-integrator prior (ssm_impl_dense.py:392-629). */
 enum { PDQ_PRIOR_IWP = 0, PDQ_PRIOR_EXPONENTIAL = 1 };
+/* LML gradient modes (pdq_config.lml_gradient) */
+enum { PDQ_GRAD_FORWARD = 0, PDQ_GRAD_REVERSE = 1 };
+/* LML gain solves (pdq_config.lml_solve) */
+enum { PDQ_LML_SOLVE_LSTSQ = 0, PDQ_LML_SOLVE_TRIU = 1 };
 /* device vector-field registry (problems.py:283-295 convention u' = f(u, t)) */
 enum {
   PDQ_VF_LOTKA_VOLTERRA = 0, /* params (a, b, c, d), d = 2 */
@@ -124,6 +128,17 @@ typedef struct pdq_config {
    * diffuse_derivatives (ssm_impl_isotropic.py:409-447, ssm_impl_dense.py:
    * 458-500).  NULL: init_std on every row. */
   const double* init_std_rows;
+  /* pdq_lml_value_and_grad's gradient: PDQ_GRAD_FORWARD (forward-mode tangents, one lane per
+   * direction) or PDQ_GRAD_REVERSE (the reverse-mode adjoint of the forward pass and the reverse
+   * scan, one lane per instance, with checkpoints every 50 grid steps and the forward factors
+   * recomputed: isotropic nu = 3, 4 with d = 2, ts0, lml_tcoeff_index 0). */
+  int32_t lml_gradient;
+  /* loss_lml_timeseries(solve_triu=) (estimators_and_losses.py:53-54): the solve behind the LML's
+   * gains, PDQ_LML_SOLVE_LSTSQ (linalg.lstsq_svd, the reference's default: the minimum-norm
+   * solution, singular values at or below eps k s_max dropped) or PDQ_LML_SOLVE_TRIU
+   * (linalg.solve_triu).  With one observed row per node (isotropic) the two differ only for an
+   * exactly zero factor, where the log-density is not finite either way. */
+  int32_t lml_solve;
 } pdq_config;
 
 /* Caller-allocated device outputs; NULL members are not written.

@@ -78,6 +78,7 @@ def _cfg_struct(ct):
             ("prior_drift", ctypes.POINTER(ct)),
             ("tcoeff_index", ctypes.c_int),
             ("init_std_rows", ctypes.POINTER(ct)),
+            ("lml_solve", ctypes.c_int),
         ]
 
     return _Cfg
@@ -172,6 +173,8 @@ class Config:
     tcoeff_index: int = 0
     # optional initial std per covariance row (n values; dense (nu+1) d, isotropic nu+1)
     init_std_rows: object = None
+    # loss_lml_timeseries(solve_triu=): "lstsq_svd" (the default) or "solve_triu"
+    lml_solve: str = "lstsq_svd"
 
     def c(self):
         if self.prior not in ("iwp", "exponential"):
@@ -186,6 +189,9 @@ class Config:
         if not 0 <= self.tcoeff_index <= self.nu:
             raise ValueError("tcoeff_index must lie in [0, nu]")
         c.tcoeff_index = int(self.tcoeff_index)
+        if self.lml_solve not in ("lstsq_svd", "solve_triu"):
+            raise ValueError(f"unknown lml_solve {self.lml_solve!r}")
+        c.lml_solve = 1 if self.lml_solve == "solve_triu" else 0
         if self.init_std_rows is not None:
             rows = np.ascontiguousarray(np.asarray(self.init_std_rows, dtype=np.float64), dtype=_dtype())
             if rows.shape != (self.n,):
@@ -335,6 +341,18 @@ def qr_r(M):
     R = _zeros((min(m, n), n))
     lib().pdqo_qr_r(m, n, _p(M), _p(R))
     return R
+
+
+def lstsq_svd(A, B):
+    """jnp.linalg.lstsq(A, B)[0] (backend/linalg.py:60-61): the minimum-norm least-squares
+    solution, singular values at or below eps * max(m, n) * s_max treated as zero."""
+    A, B = _f64(A), _f64(B)
+    vec = B.ndim == 1
+    B = _f64(B.reshape(B.shape[0], -1))
+    m, n = A.shape
+    X = _zeros((n, B.shape[1]))
+    lib().pdqo_lstsq_svd(m, n, _p(A), B.shape[1], _p(B), _p(X))
+    return X[:, 0] if vec else X
 
 
 def revert_conditional(R_X_F, R_X, R_YX, lstsq=False):

@@ -673,6 +673,9 @@ struct pdqo_cfg {
   // diffuse_derivatives; ssm_impl_isotropic.py:409-447): n values, or NULL
   // for the scalar init_std on every row
   const Real* init_std_rows;
+  // loss_lml_timeseries(solve_triu=): 0 linalg.lstsq_svd (the default), 1
+  // linalg.solve_triu
+  int lml_solve;
 };
 }
 
@@ -1744,7 +1747,7 @@ BCond observation_model(const Problem& P, const Real* std_j, int nstd) {
 Real bayes_logpdf(const Problem& P, const BCond& model, const Real* y, const BNormal& rv, BNormal& updated) {
   BNormal obs;
   BCond rev;
-  revert(P, model, rv, SOLVE_LSTSQ, obs, rev);
+  revert(P, model, rv, P.cfg.lml_solve == 1 ? SOLVE_TRIU : SOLVE_LSTSQ, obs, rev);
   Real lp = 0.0;
   updated.b.resize(P.nblk);
   for (int i = 0; i < P.nblk; ++i) {
@@ -1947,6 +1950,15 @@ void pdqo_qr_r(int m, int n, const Real* M, Real* R) {
   std::memcpy(A.a.data(), M, sizeof(Real) * m * n);
   Mat r = qr_r(A);
   std::memcpy(R, r.a.data(), sizeof(Real) * r.a.size());
+}
+
+// lstsq_svd (backend/linalg.py:60-61) of A (m x n) and B (m x nb): X (n x nb).
+void pdqo_lstsq_svd(int m, int n, const Real* A, int nb, const Real* B, Real* X) {
+  Mat a(m, n), b(m, nb);
+  std::memcpy(a.a.data(), A, sizeof(Real) * m * n);
+  std::memcpy(b.a.data(), B, sizeof(Real) * m * nb);
+  Mat x = lstsq_svd(a, b);
+  std::memcpy(X, x.a.data(), sizeof(Real) * n * nb);
 }
 
 void pdqo_revert_conditional(int n, int k, const Real* R_X_F, const Real* R_X, const Real* R_YX, int mode,

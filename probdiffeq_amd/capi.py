@@ -26,6 +26,8 @@ ERROR_NORM = {"scale_then_rms": 0, "rms_then_scale": 1}
 CONTROL = {"integral": 0, "proportional_integral": 1}
 ERROR_ESTIMATOR = {"residual_std": 0, "state_std": 1}
 PRIOR = {"iwp": 0, "exponential": 1}
+LML_GRADIENT = {"forward": 0, "reverse": 1}
+LML_SOLVE = {"lstsq_svd": 0, "solve_triu": 1}
 VF = {"lotka_volterra": 0, "van_der_pol": 1, "burgers": 2, "linear": 3, "logistic": 4}
 STATUS = {0: "ok", 1: "max_attempts", 2: "nonfinite", 3: "dt_underflow"}
 
@@ -74,6 +76,8 @@ class _Config(ctypes.Structure):
         ("dt0_per_instance", ctypes.c_void_p),
         ("lml_tcoeff_index", ctypes.c_int32),
         ("init_std_rows", ctypes.c_void_p),
+        ("lml_gradient", ctypes.c_int32),
+        ("lml_solve", ctypes.c_int32),
     ]
 
 
@@ -152,6 +156,10 @@ class Config:
     # optional initial std per covariance row ((nu+1) d dense, nu+1 isotropic): a float64 device
     # tensor (host array for the CPU build of the kernels); None: init_std on every row
     init_std_rows: Any = None
+    # pdq_lml_value_and_grad's gradient: "forward" (tangents) or "reverse" (the adjoint)
+    lml_gradient: str = "forward"
+    # loss_lml_timeseries(solve_triu=): "lstsq_svd" (the reference's default) or "solve_triu"
+    lml_solve: str = "lstsq_svd"
 
     def c(self, host: bool = False) -> _Config:
         """The C struct.  The kernels dereference init_std_rows, prior_drift and
@@ -160,6 +168,12 @@ class Config:
         kernels) takes numpy arrays instead."""
         c = self._c()
         c.lml_tcoeff_index = int(self.lml_tcoeff_index)
+        if self.lml_gradient not in LML_GRADIENT:
+            raise ValueError(f"unknown lml_gradient {self.lml_gradient!r}")
+        c.lml_gradient = LML_GRADIENT[self.lml_gradient]
+        if self.lml_solve not in LML_SOLVE:
+            raise ValueError(f"unknown lml_solve {self.lml_solve!r}")
+        c.lml_solve = LML_SOLVE[self.lml_solve]
         n_rows = (self.num_derivatives + 1) * (self.ode_dim if self.factorisation == "dense" else 1)
         if self.prior not in PRIOR:
             raise ValueError(f"unknown prior {self.prior!r}")

@@ -87,6 +87,10 @@ int lane_lml_tangents_per_lane(int n_tan);
 bool lane_lml_supported(int N, int D, int solver);
 size_t lane_lml_cond_size(int N, int D, int tpl);
 int launch_lane_lml(LmlArgs& a, int N, int D, int tpl, hipStream_t st);
+// The reverse-mode adjoint (pdq_lane_adjoint.h): one lane per instance; the
+// scratch buffer holds lane_adj_scratch_doubles(...) doubles.
+size_t lane_adj_scratch_doubles(int N, int D, int K, int every, int n_inst);
+int launch_lane_adj(LmlArgs& a, int N, int D, hipStream_t st);
 // Off-grid marginals (pdq_lane_offgrid.hip), isotropic shapes of the lane
 // kernel, filter and fixed-interval smoother.
 struct OffgridArgs {

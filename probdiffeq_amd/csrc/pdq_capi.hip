@@ -157,7 +157,15 @@ int check_vector_field(int vf, int d, int n_params) {
   return PDQ_OK;
 }
 
-int check_common(const pdq_config* cfg, int n_inst, int K) {
+int check_device() {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
+  return PDQ_OK;
+}
+
+// The configuration checks shared by the entry points; with `device` the
+// presence of a HIP device is checked last.
+int check_common(const pdq_config* cfg, int n_inst, int K, bool device = true) {
   int rc = PDQ_OK;
   if (!cfg) return set_err(PDQ_ERR_INVALID, "cfg is NULL");
   if (n_inst < 0) return set_err(PDQ_ERR_INVALID, "n_inst < 0");
@@ -187,9 +195,7 @@ int check_common(const pdq_config* cfg, int n_inst, int K) {
       return set_err(PDQ_ERR_UNSUPPORTED, "the exponential prior supports (nu + 1) d <= 16 (got %d)",
                      (cfg->num_derivatives + 1) * cfg->ode_dim);
   }
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(PDQ_ERR_NO_DEVICE, "no HIP device");
-  return PDQ_OK;
+  return device ? check_device() : PDQ_OK;
 }
 
 int lds_limit() {
@@ -515,7 +521,8 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
                                const double* grid, int32_t K_grid, int32_t every, const double* data,
                                const double* stds, int32_t n_std, int32_t average_pdfs, double* value, double* grad,
                                void* stream) {
-  int rc = check_common(cfg, n_inst, K_grid);
+  // the device is checked after the LML's own configuration checks
+  int rc = check_common(cfg, n_inst, K_grid, false);
   if (rc) return rc;
   if (cfg->factorisation == PDQ_BLOCKDIAG)
     return set_err(PDQ_ERR_UNSUPPORTED, "the LML path supports the dense and isotropic factorisations");
@@ -547,10 +554,55 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
                    "lanes_per_instance = 1 (lane-per-instance LML) needs the isotropic factorisation with nu = 3 or "
                    "4, d = 2, ts0, solver or solver_dynamic and lml_tcoeff_index = 0 (got nu = %d, d = %d)",
                    N - 1, d);
+  if (cfg->lml_solve != PDQ_LML_SOLVE_LSTSQ && cfg->lml_solve != PDQ_LML_SOLVE_TRIU)
+    return set_err(PDQ_ERR_INVALID, "unknown lml_solve %d", cfg->lml_solve);
+  if (cfg->lml_gradient != PDQ_GRAD_FORWARD && cfg->lml_gradient != PDQ_GRAD_REVERSE)
+    return set_err(PDQ_ERR_INVALID, "unknown lml_gradient %d", cfg->lml_gradient);
+  const bool reverse = grad && cfg->lml_gradient == PDQ_GRAD_REVERSE;
+  if (reverse && !lane_ok)
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "the reverse adjoint (lml_gradient = PDQ_GRAD_REVERSE) covers the isotropic factorisation with "
+                   "nu = 3 or 4, d = 2, ts0, solver or solver_dynamic and lml_tcoeff_index = 0 (got nu = %d, d = %d)",
+                   N - 1, d);
+  if (reverse && cfg->lanes_per_instance > 1)
+    return set_err(PDQ_ERR_UNSUPPORTED, "the reverse adjoint runs one lane per instance (lanes_per_instance 0 or 1)");
+  rc = check_device();
+  if (rc) return rc;
   if (n_inst == 0) return PDQ_OK;
   if (!params || !y0 || !grid || !data || !stds) return set_err(PDQ_ERR_INVALID, "null input pointer");
   hipStream_t st = (hipStream_t)stream;
   const int Kobs = (K_grid - 1) / every + 1;
+  if (reverse) {
+    DevMats m;
+    rc = device_mats(*cfg, false, m);
+    if (rc) return rc;
+    CallBuffers bufs(st);
+    double* tinit = nullptr;
+    double* adj_buf = nullptr;  // conditionals, node data, checkpoints and one segment per lane
+    HIP_TRY(bufs.alloc(&tinit, sizeof(double) * (size_t)n_inst * N * d * (1 + n_tan)));
+    HIP_TRY(bufs.alloc(&adj_buf, sizeof(double) * pdq::lane_adj_scratch_doubles(N, d, K_grid, every, n_inst)));
+    rc = pdq::launch_taylor_tangents(cfg->vector_field, d, cfg->n_params, n_inst, cfg->num_derivatives, n_tan, params,
+                                     y0, grid, tinit, st);
+    if (rc) return rc;
+    pdq::LmlArgs la{};
+    la.cfg = *cfg;
+    la.n_inst = n_inst;
+    la.K = K_grid;
+    la.every = every;
+    la.n_tan = n_tan;
+    la.average_pdfs = average_pdfs;
+    la.G = 1;
+    la.params = params;
+    la.tcoeffs = tinit;
+    la.times = grid;
+    la.data = data;
+    la.stds = stds;
+    la.Q = m.Q;
+    la.value = value;
+    la.grad = grad;
+    la.scratch = adj_buf;
+    return pdq::launch_lane_adj(la, N, d, st);
+  }
   if (lane_ok && cfg->lanes_per_instance <= 1) {
     // lane-per-instance LML: G lanes per instance, `tpl` tangent columns each
     const int tpl = pdq::lane_lml_tangents_per_lane(n_tan);

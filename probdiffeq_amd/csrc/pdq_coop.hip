@@ -88,19 +88,89 @@ __device__ void g_qr_r(const Grp<G>& g, double* A, int m, int n, int ld) {
   g.sync();
 }
 
-// G := solve_triu(R_Y, R12)^T (cholesky_util.py:77-78): R_Y k x k upper
-// (leading dim ldr), R12 k x n (same array, columns k..k+n-1), out n x k.
-// mode 1 = lstsq_svd semantics for k == 1 (pseudo-inverse of a scalar).
+// Singular value decomposition of the k x k upper-triangular R_Y (rows of R,
+// leading dim ldr) by one-sided Jacobi rotations (Hestenes): on return U
+// (k x k, row-major) holds R_Y V with mutually orthogonal columns, V (k x k)
+// the accumulated rotations and s[j] the norm of column j of U.  Sweeps stop
+// once every pair's cosine is below 1e-16 (at most 60).  One lane runs it.
+__device__ inline void jacobi_svd(const double* R, int ldr, int k, double* U, double* V, double* s) {
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) {
+      U[i * k + j] = R[i * ldr + j];
+      V[i * k + j] = i == j ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        double alpha = 0.0, beta = 0.0, gamma = 0.0;
+        for (int i = 0; i < k; ++i) {
+          alpha += U[i * k + p] * U[i * k + p];
+          beta += U[i * k + q] * U[i * k + q];
+          gamma += U[i * k + p] * U[i * k + q];
+        }
+        if (gamma == 0.0) continue;
+        off = fmax(off, fabs(gamma) / sqrt(alpha * beta));
+        // the rotation that makes columns p and q orthogonal (the smaller root of
+        // t^2 + 2 zeta t - 1 = 0)
+        const double zeta = (beta - alpha) / (2.0 * gamma);
+        const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+        for (int i = 0; i < k; ++i) {
+          const double up = U[i * k + p], uq = U[i * k + q];
+          U[i * k + p] = cs * up - sn * uq;
+          U[i * k + q] = sn * up + cs * uq;
+          const double vp = V[i * k + p], vq = V[i * k + q];
+          V[i * k + p] = cs * vp - sn * vq;
+          V[i * k + q] = sn * vp + cs * vq;
+        }
+      }
+    if (off < 1e-16) break;
+  }
+  for (int j = 0; j < k; ++j) {
+    double ss = 0.0;
+    for (int i = 0; i < k; ++i) ss += U[i * k + j] * U[i * k + j];
+    s[j] = sqrt(ss);
+  }
+}
+
+// G := solve(R_Y, R12)^T (cholesky_util.py:77-78): R_Y k x k upper (leading
+// dim ldr), R12 k x n (same array, columns k..k+n-1), out n x k.
+// mode 0: solve_triu (back substitution).
+// mode 1: lstsq_svd (backend/linalg.py:60-61, jnp.linalg.lstsq): the
+// minimum-norm least-squares solution, singular values at or below
+// eps * k * s_max treated as zero; ws holds 2 k^2 + k doubles of the group's
+// LDS.  Lane 0 decomposes R_Y, then each lane solves its columns.
 template <int G>
-__device__ void g_gain(const Grp<G>& g, const double* R, int ldr, int k, int n, double* out, int mode) {
+__device__ void g_gain(const Grp<G>& g, const double* R, int ldr, int k, int n, double* out, int mode,
+                       double* ws = nullptr) {
+  if (mode == 1) {
+    double* U = ws;
+    double* V = ws + k * k;
+    double* s = ws + 2 * k * k;
+    if (g.l == 0) jacobi_svd(R, ldr, k, U, V, s);
+    g.sync();
+    double smax = 0.0;
+    for (int j = 0; j < k; ++j) smax = fmax(smax, s[j]);
+    const double cutoff = 2.220446049250313e-16 * (double)k * smax;
+    for (int j = g.l; j < n; j += G) {
+      for (int i = 0; i < k; ++i) out[j * k + i] = 0.0;
+      for (int c = 0; c < k; ++c) {
+        if (!(s[c] > cutoff)) continue;
+        double ub = 0.0;
+        for (int i = 0; i < k; ++i) ub += (U[i * k + c] / s[c]) * R[i * ldr + k + j];
+        ub /= s[c];
+        for (int i = 0; i < k; ++i) out[j * k + i] += V[i * k + c] * ub;
+      }
+    }
+    return;
+  }
   for (int j = g.l; j < n; j += G) {
     // back substitution for column j of R12, written straight into out
     for (int i = k - 1; i >= 0; --i) {
       double s = R[i * ldr + k + j];
       for (int c = i + 1; c < k; ++c) s -= R[i * ldr + c] * out[j * k + c];
-      const double r = R[i * ldr + i];
-      // mode 1: lstsq_svd for a 1 x 1 factor (pseudo-inverse of zero is zero)
-      out[j * k + i] = (mode == 1 && r == 0.0) ? 0.0 : s / r;
+      out[j * k + i] = s / R[i * ldr + i];
     }
   }
 }
@@ -1419,7 +1489,7 @@ __device__ void lml_instance(const Ctx<G>& c, int inst) {
     }
     c.g.sync();
     g_qr_r(c.g, big, n2, n2, n2);
-    g_gain(c.g, big, n2, ko, nc, gain, 1);
+    g_gain(c.g, big, n2, ko, nc, gain, cfg.lml_solve == PDQ_LML_SOLVE_TRIU ? 0 : 1, c.W(L.w_svd));
     c.g.sync();
     if (c.g.l == 0) {
       // observed.logpdf (ssm_impl_dense.py:170-180 / isotropic :217-236);

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <atomic>
 
+#include "pdq_lane_adjoint.h"
 #include "pdq_lane_lml.h"
 
 namespace pdq {
@@ -50,7 +51,42 @@ int launch_lml_shape(LmlArgs& a, int tpl, hipStream_t st) {
   return host_error(PDQ_ERR_UNSUPPORTED, "no lane LML kernel with %d tangents per lane", tpl);
 }
 
+template <int N, int D, int S>
+int launch_adj_nd(LmlArgs& a, hipStream_t st) {
+  auto kern = lane_adj_kernel<N, D, S>;
+  const size_t lds = lane_lml_lds_bytes<N, D>();
+  static std::atomic<int> done[kLmlMaxDevices];
+  int dev = 0;
+  PDQ_HIP_TRY(hipGetDevice(&dev));
+  if (!(dev >= 0 && dev < kLmlMaxDevices && done[dev].load())) {
+    PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (dev >= 0 && dev < kLmlMaxDevices) done[dev].store(1);
+  }
+  // one lane per instance: the grid covers every instance once (the scratch
+  // buffer has a column per lane of this grid)
+  const int grid = (a.n_inst + 63) / 64;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, a);
+  PDQ_HIP_TRY(hipGetLastError());
+  return PDQ_OK;
+}
+
 }  // namespace
+
+size_t lane_adj_scratch_doubles(int N, int D, int K, int every, int n_inst) {
+  const size_t lanes = (size_t)((n_inst + 63) / 64) * 64;
+  if (D == 2 && N == 4) return AdjLayout<4, 2>(K, every).size * lanes;
+  if (D == 2 && N == 5) return AdjLayout<5, 2>(K, every).size * lanes;
+  return 0;
+}
+
+int launch_lane_adj(LmlArgs& a, int N, int D, hipStream_t st) {
+  if (D == 2 && (N == 4 || N == 5)) {
+    const bool dyn = a.cfg.solver == PDQ_SOLVER_DYNAMIC;
+    if (N == 4) return dyn ? launch_adj_nd<4, 2, PDQ_SOLVER_DYNAMIC>(a, st) : launch_adj_nd<4, 2, PDQ_SOLVER>(a, st);
+    return dyn ? launch_adj_nd<5, 2, PDQ_SOLVER_DYNAMIC>(a, st) : launch_adj_nd<5, 2, PDQ_SOLVER>(a, st);
+  }
+  return host_error(PDQ_ERR_UNSUPPORTED, "no adjoint LML kernel for nu + 1 = %d, d = %d", N, D);
+}
 
 // One tangent column per lane: G = n_tan lanes per instance.  Against three
 // per lane (G = 2) this is 1.44x faster on the C5 batch (13.9 vs 20.0 ms):

@@ -15,6 +15,7 @@ struct Layout {
   int w_P, w_PI, w_big, w_T1, w_Um, w_UL, w_Pm, w_PL, w_om, w_oL, w_G, w_mo, w_y0;
   int w_bA, w_bm, w_bL, w_btl, w_btl_pad, w_bto;
   int w_eA, w_em, w_eL, w_etl, w_eto, w_prm, w_tri, w_lml;
+  int w_svd;  // the LML gain's singular value decomposition of R_Y (U, V, s)
   // exponential prior: the transition of the current step (e^A, Gramian
   // factor) and the scratch of its computation
   int w_xA, w_xL, w_xa, w_xB, w_xA2, w_xA4, w_xA6, w_xA8, w_xT, w_xV, w_xVmU, w_xP, w_xPB, w_xrhs, w_xtmp,
@@ -88,6 +89,7 @@ PDQ_HD inline Layout make_layout(const pdq_config& cfg, int n_tan) {
   L.w_prm = o; o += cfg.n_params > 0 ? cfg.n_params : 1;
   L.w_tri = o; o += ko * mc;
   L.w_lml = o; o += 4 * (1 + n_tan) + 4 * ko;
+  L.w_svd = o; o += 2 * ko * ko + ko;
   if (L.expo) {
     const int nn = nc * nc, rows = 10 * L.d > 2 * nc ? 10 * L.d : 2 * nc;
     L.w_xA = o; o += nn;

@@ -111,6 +111,41 @@ def lml_forward(cfg, grid_steps: int, observations: int, n_tan: int = 0) -> floa
     return nb * (grid_steps * step) + nb * lml
 
 
+ADJ_SEGMENT = 50  # grid steps per checkpoint segment of the reverse adjoint
+
+
+def lml_reverse(cfg, grid_steps: int, observations: int) -> float:
+    """FLOPs the reverse-mode adjoint of the isotropic lane LML executes: the forward pass, the
+    recomputation of every checkpoint segment and of every step's factors in the backward sweep
+    (three forward passes' worth of factor work), and the adjoint of the mean recursions, which
+    costs what one more set of mean columns costs in the forward direction (step and scan)."""
+    nc, mc, ko, nb = dims(cfg)
+    step0 = lml_forward(cfg, grid_steps, 0, 0) / max(1, grid_steps)
+    step1 = lml_forward(cfg, grid_steps, 0, 1) / max(1, grid_steps)
+    scan0 = lml_forward(cfg, 0, observations, 0)
+    scan1 = lml_forward(cfg, 0, observations, 1)
+    return nb * (3 * grid_steps * step0 + grid_steps * (step1 - step0) + scan0 + (scan1 - scan0))
+
+
+def lml_reverse_bytes(cfg, n_inst: int, observations: int, grid_points: int) -> float:
+    """Algorithmic HBM bytes of the reverse adjoint: inputs and outputs as in the forward LML
+    (the Taylor coefficients carry their n_params + d sensitivities), and per instance its storage
+    written once and read once: the interval conditionals, each observation node's gain, factor
+    and residuals, the adjoints of the conditional means, the checkpoints every 50 steps, and
+    every step's slot within its segment."""
+    N, d = cfg.num_derivatives + 1, cfg.ode_dim
+    tri = N * (N + 1) // 2
+    slot = 8 + N * d + tri + N * N + N * d + tri + 2 * N
+    cond = N * N + N * d + tri + 2 * N
+    steps = grid_points - 1
+    per_inst = (observations - 1) * cond + observations * (N + 1 + d) + (observations - 1) * N * d
+    per_inst += ((steps - 1) // ADJ_SEGMENT + 1) * slot + steps * slot
+    n_theta = cfg.n_params + d
+    inputs = n_inst * (cfg.n_params + d + observations * (d + 1) + N * d * (1 + n_theta)) + grid_points
+    outputs = n_inst * (1 + n_theta)
+    return 8.0 * (inputs + outputs + 2 * n_inst * per_inst)
+
+
 def solve_bytes(cfg, n_inst: int, K: int) -> float:
     """Algorithmic HBM bytes of one batched save_at solve: inputs, outputs written
     once, and the stored backward conditionals read back by the smoother's backward pass."""

@@ -897,7 +897,15 @@ def loss_lml_terminal_values(*, tcoeff_index=0):
     return loss
 
 
-def loss_lml_timeseries(*, average_pdfs: bool = True, tcoeff_index=0, solve_triu=None):
+class linalg:  # noqa: N801 (the reference's module name)
+    """The two solves loss_lml_timeseries accepts as ``solve_triu`` (backend/linalg.py:48-61):
+    the device implements these and no other callable."""
+
+    lstsq_svd = "lstsq_svd"
+    solve_triu = "solve_triu"
+
+
+def loss_lml_timeseries(*, average_pdfs: bool = True, tcoeff_index=0, solve_triu=linalg.lstsq_svd):
     """estimators_and_losses.py:53-105 on the device, as a differentiable torch op.
 
     Returns ``loss(u, /, *, problem, std)`` where ``problem`` is an
@@ -905,16 +913,23 @@ def loss_lml_timeseries(*, average_pdfs: bool = True, tcoeff_index=0, solve_triu
     the reference differentiates through a solve that depends on the
     parameters; here the solve and the likelihood run in one launch and the
     gradient with respect to the vector-field parameters and the initial
-    value comes from the same launch (forward-mode tangents).
+    value comes from the same launch (forward-mode tangents, or the reverse
+    adjoint with ``problem.lml_gradient = "reverse"``).  ``solve_triu`` is
+    ``linalg.lstsq_svd`` (the reference's default) or ``linalg.solve_triu``;
+    any other callable raises NotImplementedError.
     """
     tcoeff_index = int(tcoeff_index)
+    if solve_triu is None:
+        solve_triu = linalg.lstsq_svd
+    if solve_triu not in (linalg.lstsq_svd, linalg.solve_triu):
+        raise NotImplementedError("solve_triu must be linalg.lstsq_svd or linalg.solve_triu (the device's two gain solves)")
 
     def loss(u, /, *, problem, params, y0, std):
         from probdiffeq_amd.torch_ops import lml
 
         if not 0 <= tcoeff_index <= problem.num_derivatives:
             raise ValueError(f"tcoeff_index must lie in [0, {problem.num_derivatives}]")
-        return lml(problem, u, std, params=params, y0=y0, average_pdfs=average_pdfs, tcoeff_index=tcoeff_index)
+        return lml(problem, u, std, params=params, y0=y0, average_pdfs=average_pdfs, tcoeff_index=tcoeff_index, solve=solve_triu)
 
     return loss
 

@@ -36,6 +36,8 @@ class LMLProblem:
     base_output_scale: float = 1.0
     init_std: float = 0.0
     lanes_per_instance: int = 0
+    # the gradient: "forward" (tangents) or "reverse" (the adjoint with checkpoints)
+    lml_gradient: str = "forward"
 
     def config(self, n_params: int) -> capi.Config:
         return capi.Config(
@@ -49,6 +51,7 @@ class LMLProblem:
             base_output_scale=self.base_output_scale,
             init_std=self.init_std,
             lanes_per_instance=self.lanes_per_instance,
+            lml_gradient=self.lml_gradient,
         )
 
 
@@ -56,9 +59,11 @@ class LML(torch.autograd.Function):
     """value = loss_lml_timeseries(data, posterior=solve(params, y0), std)."""
 
     @staticmethod
-    def forward(ctx, params, y0, data, stds, problem: LMLProblem, average: bool, tcoeff_index: int = 0):
+    def forward(ctx, params, y0, data, stds, problem: LMLProblem, average: bool, tcoeff_index: int = 0,
+                solve: str = "lstsq_svd"):
         cfg = problem.config(params.shape[-1])
         cfg.lml_tcoeff_index = int(tcoeff_index)
+        cfg.lml_solve = solve
         need_grad = params.requires_grad or y0.requires_grad
         value, grad = capi.lml_value_and_grad(
             cfg,
@@ -79,13 +84,13 @@ class LML(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         if not ctx.has_grad:
-            return None, None, None, None, None, None, None
+            return None, None, None, None, None, None, None, None
         (grad,) = ctx.saved_tensors
         full = grad * g[:, None]
-        return full[:, : ctx.n_params], full[:, ctx.n_params :], None, None, None, None, None
+        return full[:, : ctx.n_params], full[:, ctx.n_params :], None, None, None, None, None, None
 
 
-def lml(problem: LMLProblem, data, std, *, params, y0, average_pdfs=True, tcoeff_index=0):
+def lml(problem: LMLProblem, data, std, *, params, y0, average_pdfs=True, tcoeff_index=0, solve="lstsq_svd"):
     """Batched LML (one value per instance) with autograd support; the data observe Taylor
-    coefficient ``tcoeff_index``."""
-    return LML.apply(params, y0, data, std, problem, average_pdfs, tcoeff_index)
+    coefficient ``tcoeff_index``; ``solve`` ("lstsq_svd" or "solve_triu") is the gains' solve."""
+    return LML.apply(params, y0, data, std, problem, average_pdfs, tcoeff_index, solve)

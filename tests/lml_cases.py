@@ -95,3 +95,16 @@ def grad_reference(oracle, ocfg, params, y0, grid, every, data, stds, average=Tr
 def rel(a, b):
     a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def lml_dropped_singular_value_inputs(n_inst=2, seed=3):
+    """Dense LML data on a coarse grid (11 points on [0, 5], every point observed) where coordinate 1
+    of every third observation has std 1e17 (how missing data are often encoded) and coordinate 0
+    there has std 0.1.  The observed factor R_Y at those nodes has singular values ~1e17 and ~1:
+    lstsq_svd's cutoff eps * 2 * s_max (~44) drops the small one, so its gain ignores coordinate 0
+    as well, while solve_triu's gain uses it.  The grid is coarse so that the state is uncertain
+    enough at the nodes for the gain to matter to the likelihood."""
+    params, y0, grid, every, data, stds = lml_inputs(n_inst, seed=seed, t_end=5.0, n_grid=11, every=1, per_coordinate_std=True)
+    stds[:, 1::3, 1] = 1e17
+    stds[:, 1::3, 0] = 0.1
+    return params, y0, grid, every, data, stds

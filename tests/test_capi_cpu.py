@@ -28,10 +28,12 @@ def test_library_exports_every_declared_symbol():
 
 def test_config_struct_matches_header_layout():
     """The ctypes mirror of pdq_config has the C layout (12 ints, 12 doubles, 8 ints, two pointers,
-    one int padded to the struct's 8-byte alignment, one pointer)."""
+    one int padded to the struct's 8-byte alignment, one pointer, two ints)."""
     from probdiffeq_amd import capi
 
-    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8
+    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8 + 8
+    assert capi._Config.lml_gradient.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8
+    assert capi._Config.lml_solve.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8 + 4
     assert capi._Config.lml_tcoeff_index.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8
     assert capi._Config.init_std_rows.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8
     assert ctypes.sizeof(capi._Solution) == 17 * 8
@@ -248,3 +250,32 @@ def test_taylor_coefficients_checks_its_arguments(vf, d, n_params, num, match):
     rc = lib.pdq_taylor_coefficients(vf, d, n_params, 1, None, None, ctypes.c_double(0.0), num, None, None)
     assert rc in (1, 2), rc
     assert match in capi.last_error()
+
+
+def test_lml_gradient_mode_is_checked_before_device_work():
+    """pdq_config.lml_gradient: the binding accepts "forward" and "reverse"; the C ABI refuses an
+    unknown mode and the shapes the reverse adjoint does not cover before touching the device."""
+    from probdiffeq_amd import capi
+
+    with pytest.raises(ValueError, match="lml_gradient"):
+        capi.Config(lml_gradient="backward").c()
+    lib = capi.lib()
+    fake = ctypes.c_void_p(256)  # never dereferenced: the checks fail first
+
+    def call(c):
+        return lib.pdq_lml_value_and_grad(ctypes.byref(c), 1, fake, fake, fake, 11, 5, fake, fake, 1, 1, fake, fake, None)
+
+    base = dict(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver")
+    c = capi.Config(**base).c()
+    c.lml_gradient = 7
+    assert call(c) == 1 and "unknown lml_gradient" in capi.last_error()
+    for kw, match in [
+        (dict(factorisation="dense"), "reverse adjoint"),
+        (dict(num_derivatives=5), "reverse adjoint"),
+        (dict(lml_tcoeff_index=2), "reverse adjoint"),
+        (dict(lanes_per_instance=8), "one lane per instance"),
+    ]:
+        cfg = dict(base, lml_gradient="reverse")
+        cfg.update(kw)
+        assert call(capi.Config(**cfg).c()) == 2, kw
+        assert match in capi.last_error(), (kw, capi.last_error())

@@ -178,3 +178,108 @@ def test_mirror_loss_lml_timeseries_observing_a_derivative(pdq):
     assert torch.all((v0 - v.detach()).abs() > 1.0)
     with pytest.raises(ValueError, match="tcoeff_index"):
         pd.loss_lml_timeseries(tcoeff_index=4)(_dev(data), problem=problem, params=p, y0=y, std=_dev(stds))
+
+
+@pytest.mark.parametrize(
+    "nu,solver,average,n_grid,every",
+    [(3, "solver", True, 1001, 10), (3, "dynamic", False, 761, 20), (4, "solver", True, 1001, 10), (4, "dynamic", True, 761, 20)],
+)
+def test_reverse_adjoint_against_finite_differences(oracle, pdq, nu, solver, average, n_grid, every):
+    """lml_gradient = "reverse" (pdq_lane_adjoint.h: checkpoints every 50 grid steps, the forward
+    factors recomputed per segment): the value against the oracle with the solve parity bound, the
+    gradient against long double Richardson differences of the oracle to 1e-10.  761 grid points make
+    the last checkpoint segment a partial one (760 = 15 x 50 + 10 steps)."""
+    params, y0, grid, _, data, stds = LC.lml_inputs(3, seed=40 + nu, t_end=10.0, n_grid=n_grid, every=every)
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=nu, ode_dim=2, strategy="fixedpoint", solver=solver, lml_gradient="reverse")
+    _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, average=average)
+
+
+def test_reverse_adjoint_matches_forward_tangents_at_the_c5_shape(pdq):
+    """The C5 workload (1024 instances, 2001 grid points, 101 observations, nu = 3): the reverse
+    adjoint's value equals the forward-mode lane LML's to 1e-13 and its gradient the forward-mode
+    tangents' to 1e-12, normwise relative per instance."""
+    import bench
+
+    params, y0, grid, data, stds = bench.c5_inputs(1024)
+    every = 20
+    base = dict(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver")
+    args = (_dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds))
+    print("lml case: C5 shape, reverse vs forward, B=1024", flush=True)
+    v_f, g_f = pdq.lml_value_and_grad(pdq.Config(**base), *args)
+    v_r, g_r = pdq.lml_value_and_grad(pdq.Config(**base, lml_gradient="reverse"), *args)
+    torch.cuda.synchronize()
+    assert torch.isfinite(v_r).all() and torch.isfinite(g_r).all()
+    assert ((v_r - v_f).abs() / v_f.abs()).max().item() < 1e-13
+    err = ((g_r - g_f).abs().amax(dim=1) / g_f.abs().amax(dim=1)).max().item()
+    print(f"reverse vs forward gradient: max normwise relative difference {err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+def test_reverse_adjoint_value_only_and_instance_counts(pdq):
+    """Without a gradient the reverse flag changes nothing (the forward LML runs); instance counts
+    that do not fill a wavefront (1, 65) give each instance the value and gradient it has alone."""
+    params, y0, grid, every, data, stds = LC.lml_inputs(65, seed=7, t_end=5.0, n_grid=251, every=10)
+    base = dict(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="dynamic")
+    args = (_dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds))
+    print("lml case: reverse adjoint, B=65 and B=1", flush=True)
+    v0, _ = pdq.lml_value_and_grad(pdq.Config(**base), *args, grad=False)
+    v1, _ = pdq.lml_value_and_grad(pdq.Config(**base, lml_gradient="reverse"), *args, grad=False)
+    assert torch.equal(v0, v1)
+    v65, g65 = pdq.lml_value_and_grad(pdq.Config(**base, lml_gradient="reverse"), *args)
+    one = (_dev(params[64:]), _dev(y0[64:]), _dev(grid), every, _dev(data[64:]), _dev(stds[64:]))
+    v1, g1 = pdq.lml_value_and_grad(pdq.Config(**base, lml_gradient="reverse"), *one)
+    torch.cuda.synchronize()
+    assert torch.equal(v65[64:], v1) and torch.equal(g65[64:], g1)
+
+
+@pytest.mark.parametrize(
+    "kw,match",
+    [
+        (dict(factorisation="dense"), "reverse adjoint"),
+        (dict(num_derivatives=2), "reverse adjoint"),
+        (dict(lml_tcoeff_index=1), "reverse adjoint"),
+        (dict(lanes_per_instance=16), "one lane per instance"),
+    ],
+)
+def test_reverse_adjoint_refuses_other_shapes(pdq, kw, match):
+    params, y0, grid, every, data, stds = LC.lml_inputs(1, seed=2, t_end=5.0, n_grid=251, every=10)
+    base = dict(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver", lml_gradient="reverse")
+    base.update(kw)
+    with pytest.raises(pdq.PdqError, match=match):
+        pdq.lml_value_and_grad(pdq.Config(**base), _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds))
+
+
+def test_lml_gain_solves_with_a_dropped_singular_value(oracle, pdq):
+    """loss_lml_timeseries(solve_triu=...) on the device (dense, two observed rows per node): where
+    R_Y's small singular value falls below lstsq_svd's cutoff the Jacobi-SVD gain matches the
+    oracle's lstsq_svd and the back substitution the oracle's solve_triu (value, and gradient
+    against Richardson differences to 1e-10); the two solves' likelihoods differ by more than 0.1 %."""
+    params, y0, grid, every, data, stds = LC.lml_dropped_singular_value_inputs()
+    vals = {}
+    for solve in ("lstsq_svd", "solve_triu"):
+        cfg = pdq.Config(factorisation="dense", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="dynamic", lml_solve=solve)
+        v, _ = _check(oracle, pdq, cfg, params, y0, grid, every, data, stds)
+        vals[solve] = v
+    assert np.all(np.abs(vals["lstsq_svd"] - vals["solve_triu"]) > 1e-3 * np.abs(vals["solve_triu"]))
+
+
+def test_mirror_loss_lml_timeseries_solve_choices(pdq):
+    """probdiffeq.loss_lml_timeseries(solve_triu=linalg.solve_triu) reaches the device's back
+    substitution (the value equals the C ABI's with lml_solve = "solve_triu"); another callable is
+    refused."""
+    from probdiffeq_amd import probdiffeq as pd
+    from probdiffeq_amd.torch_ops import LMLProblem
+
+    params, y0, grid, every, data, stds = LC.lml_dropped_singular_value_inputs()
+    problem = LMLProblem(vector_field="lotka_volterra", num_derivatives=3, ode_dim=2, grid=_dev(grid), every=every,
+                         factorisation="dense", solver="dynamic")
+    for solve in (pd.linalg.lstsq_svd, pd.linalg.solve_triu):
+        loss = pd.loss_lml_timeseries(solve_triu=solve)
+        v = loss(_dev(data), problem=problem, params=_dev(params), y0=_dev(y0), std=_dev(stds))
+        cfg = problem.config(params.shape[1])
+        cfg.lml_solve = solve
+        vr, _ = pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds), grad=False)
+        torch.cuda.synchronize()
+        assert torch.equal(v.detach(), vr)
+    with pytest.raises(NotImplementedError, match="solve_triu"):
+        pd.loss_lml_timeseries(solve_triu=lambda a, b: b)

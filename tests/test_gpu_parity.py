@@ -113,6 +113,7 @@ def _oracle_cfg(oracle, cfg):
         prior_drift=None if cfg.prior_drift is None else np.asarray(cfg.prior_drift.cpu() if hasattr(cfg.prior_drift, "cpu") else cfg.prior_drift),
         tcoeff_index=cfg.lml_tcoeff_index,
         init_std_rows=None if cfg.init_std_rows is None else np.asarray(cfg.init_std_rows.cpu() if hasattr(cfg.init_std_rows, "cpu") else cfg.init_std_rows),
+        lml_solve=cfg.lml_solve,
     )
 
 

@@ -242,6 +242,32 @@ def test_lml_reverse_adjoint(oracle, emu, nu, solver, average):
         assert LC.rel(g_r[i], gr) < LC.GRAD_RTOL, (g_r[i], gr)
 
 
+def test_lml_gain_solves_with_a_dropped_singular_value(oracle, emu):
+    """loss_lml_timeseries(solve_triu=linalg.lstsq_svd / linalg.solve_triu) in the cooperative
+    kernel's code (dense, two observed rows per node).  Where R_Y's small singular value falls below
+    lstsq_svd's cutoff (lml_dropped_singular_value_inputs) the kernel's Jacobi-SVD gain matches the
+    oracle's lstsq_svd and its back substitution the oracle's solve_triu, value and gradient; the
+    two solves' likelihoods differ by far more than the tolerance, so each comparison tells them
+    apart."""
+    import lml_cases as LC
+    from probdiffeq_amd import capi
+
+    params, y0, grid, every, data, stds = LC.lml_dropped_singular_value_inputs()
+    vals = {}
+    for solve in ("lstsq_svd", "solve_triu"):
+        cfg = capi.Config(factorisation="dense", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="dynamic", lml_solve=solve)
+        ocfg = T._oracle_cfg(oracle, cfg)
+        assert ocfg.lml_solve == solve
+        v, g = emu.lml(cfg, params, y0, grid, every, data, stds)
+        for i in range(2):
+            vo, vf, vt = LC.oracle_value(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i])
+            T.check_accuracy(v[i], vo, vf, vt, f"emulation lml {solve} instance {i}")
+            gr = LC.richardson_grad(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i])
+            assert LC.rel(g[i], gr) < LC.GRAD_RTOL, (solve, g[i], gr)
+        vals[solve] = v
+    assert np.all(np.abs(vals["lstsq_svd"] - vals["solve_triu"]) > 1e-3 * np.abs(vals["solve_triu"]))
+
+
 @pytest.mark.parametrize("fact,idx,grad", [("isotropic", 1, True), ("isotropic", 2, True), ("isotropic", 3, False), ("dense", 1, True), ("dense", 2, True), ("dense", 3, False)])
 def test_lml_observing_a_derivative(oracle, emu, fact, idx, grad):
     """loss_lml_timeseries(tcoeff_index=idx) in the cooperative kernel's code: the data observe Taylor

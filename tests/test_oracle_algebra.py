@@ -245,3 +245,40 @@ def test_vector_field_jacobians_match_finite_differences(oracle):
             ym[j] -= h
             Jfd[:, j] = (oracle.vector_field(vf, prm, yp)[0] - oracle.vector_field(vf, prm, ym)[0]) / (2 * h)
         assert np.allclose(J, Jfd, rtol=1e-6, atol=1e-6), vf
+
+
+@pytest.mark.parametrize(
+    "case",
+    ["full_rank", "rank_one_triangular", "zero", "cutoff_hit", "cutoff_missed", "tall_rank_deficient", "rank_two_of_three"],
+)
+def test_lstsq_svd_matches_numpy(oracle, case):
+    """lstsq_svd (backend/linalg.py:60-61) is jnp.linalg.lstsq(A, B)[0]: the minimum-norm solution
+    with singular values at or below eps * max(m, n) * s_max dropped, which numpy.linalg.lstsq with
+    its default rcond computes by LAPACK's SVD.  The cutoff cases sit a factor 10 either side of the
+    threshold (eps * 2 * 1e16 ~ 4.4): a wrong cutoff (relative to s_min, or without max(m, n)) or a
+    back substitution instead fails one of them."""
+    rng = _rng()
+    if case == "full_rank":
+        A = np.triu(rng.normal(size=(3, 3))) + 2.0 * np.eye(3)
+    elif case == "rank_one_triangular":
+        A = np.array([[1.0, 2.0], [0.0, 0.0]])
+    elif case == "zero":
+        A = np.zeros((2, 2))
+    elif case == "cutoff_hit":
+        A = np.array([[1e16, 3.0], [0.0, 0.44]])
+    elif case == "cutoff_missed":
+        A = np.array([[1e16, 3.0], [0.0, 44.0]])
+    elif case == "tall_rank_deficient":
+        u = rng.normal(size=(5, 1))
+        A = np.hstack([u, 2.0 * u, rng.normal(size=(5, 1))])
+    else:
+        A = rng.normal(size=(3, 2)) @ rng.normal(size=(2, 3))
+    B = rng.normal(size=(A.shape[0], 4))
+    X = oracle.lstsq_svd(A, B)
+    X_np = np.linalg.lstsq(A, B, rcond=None)[0]
+    assert np.allclose(X, X_np, rtol=1e-12, atol=1e-12 * np.abs(X_np).max()), (X, X_np)
+    if case == "cutoff_hit":
+        # the small singular value is dropped: the solution has no component along it
+        assert not np.allclose(X, np.linalg.solve(A, B), rtol=1e-3)
+    if case == "cutoff_missed":
+        assert np.allclose(X, np.linalg.solve(A, B), rtol=1e-10)
