@@ -92,7 +92,8 @@ def _full_batch(oracle, pdq, cfg, params, y0s, times, label, allow_rounding_crit
         tc = oracle.taylor_coefficients(cfg.vector_field, params[i], y0s[i], times[i, 0], cfg.num_derivatives)
         ref, fma, truth = T._oracle_solve(oracle, ocfg, params[i], tc, times[i], False)
         T.check_accuracy(u_gpu[i], ref["mean"][-1, :d], fma["mean"][-1, :d], truth["mean"][-1, :d], f"{label} instance {i} u(t_end)")
-    T.REPORT.append({"case": f"{label} u(t_end), {B} instances", "e_gpu": float(rel.max()), "tol": 1e-10})
+    # a screening record, not a bound: the instances above 1e-10 were checked one by one above
+    T.REPORT.append({"case": f"{label} u(t_end), {B} instances", "e_gpu": float(rel.max()), "screen": 1e-10})
     return att
 
 
