@@ -36,6 +36,87 @@ namespace pdq {
 #define PDQ_PHASE_BOUNDARY() __builtin_amdgcn_sched_barrier(0)
 #endif
 
+// Reciprocal and square root of operands known to be finite and non-zero,
+// and the inverse N-th root, from the hardware estimates refined by FMA
+// iterations.  The IEEE sequences the compiler emits for 1/b and sqrt(x) are
+// 9-15 instructions with long dependency chains; the Householder reflectors,
+// the preconditioner's and conditionals' scalings and the gains need about 60
+// of them per iteration of the hot loop.
+// * lane_rcp_nz: v_rcp_f64, then two Newton steps y += y (1 - b y), each an
+//   FMA pair; within an ulp of 1/b for finite non-zero b (zero or infinite b
+//   give NaN: callers pass such y only to lane_div, which then divides).
+// * lane_sqrt_pos: v_rsq_f64, one Goldschmidt step for (sqrt x, 1/(2 sqrt x))
+//   and a final Newton correction of the root (Markstein, IA-64 and
+//   Elementary Functions, 2000, ch. 9); within an ulp of sqrt(x) for finite
+//   positive x.
+// * lane_root_inv<N>: x^(-1/N) from a single-precision estimate of the
+//   scaled operand (x = xs 2^(N k), xs within single range), then two Newton
+//   steps on x y^N = 1, y += (y / N) (1 - x y^N); within a few ulps of
+//   pow(x, -1/N).  x = 0 gives +inf and x = +inf gives 0, as pow does.
+// The host build (tests/emu) computes the IEEE operations and pow.
+#ifdef PDQ_HOST_EMU
+__device__ __forceinline__ double lane_rcp_nz(double b) { return 1.0 / b; }
+__device__ __forceinline__ double lane_sqrt_pos(double x) { return sqrt(x); }
+template <int N>
+__device__ __forceinline__ double lane_root_inv(double x) {
+  return pow(x, -1.0 / (double)N);
+}
+#else
+__device__ __forceinline__ double lane_rcp_nz(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double lane_sqrt_pos(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  return fma(fma(-g, g, x), h, g);
+}
+template <int N>
+__device__ __forceinline__ double lane_root_inv(double x) {
+  const int k = __builtin_amdgcn_frexp_exp(x) / N;
+  const double xs = __builtin_ldexp(x, -N * k);
+  double y = __builtin_ldexp((double)__builtin_amdgcn_exp2f(-__builtin_amdgcn_logf((float)xs) / (float)N), -k);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    double t = y;
+#pragma unroll
+    for (int q = 1; q < N; ++q) t *= y;
+    y = fma(y * (1.0 / (double)N), fma(-x, t, 1.0), y);
+  }
+  y = x == 0.0 ? __builtin_inf() : y;
+  return x == __builtin_inf() ? 0.0 : y;
+}
+#endif
+
+// q = a / b from y = lane_rcp_nz(b): q = a y, corrected by one FMA step
+// against the residual a - b q (Markstein); within an ulp for finite non-zero
+// b and a finite quotient.
+__device__ __forceinline__ double lane_div_nz(double a, double b, double y) {
+  const double q = a * y;
+  return fma(fma(-q, b, a), y, q);
+}
+
+// a / b given y = 1 / b (correctly rounded): q = a y, then one FMA
+// correction step against the exact residual a - b q (Markstein).  For a
+// correctly rounded, normal y and a finite quotient the result is the
+// correctly rounded quotient, as a division gives it, for three instructions
+// once y is known; used where many quotients share one divisor.  Any other
+// case (b = 0, infinities, NaN, y outside the normal range) takes the
+// division itself, so special values behave exactly as IEEE division.
+__device__ __forceinline__ double lane_div(double a, double b, double y) {
+  const double q = a * y;
+  const double r = fma(-q, b, a);
+  const double res = r == 0.0 ? q : fma(r, y, q);  // exact q keeps the sign of a zero quotient
+  if (__builtin_expect(!(__builtin_isnormal(y) && isfinite(res)), 0)) return a / b;
+  return res;
+}
+
 // Slot scalars.
 enum { LS_T = 0, LS_OS = 1, LS_NSTEPS = 2, LS_MLE_RUN = 3, LS_MLE_N = 4, LS_IDENT = 5, LS_NSC = 8 };
 
@@ -186,9 +267,11 @@ __device__ __forceinline__ void lane_qr(double* A) {
       if (S::nz(i, r)) ss += A[r * C + i] * A[r * C + i];
     if (ss != 0.0) {
       const double alpha = A[i * C + i];
-      const double beta = -copysign(sqrt(fma(alpha, alpha, ss)), alpha);
-      const double tau = (beta - alpha) / beta;
-      const double scal = 1.0 / (alpha - beta);
+      // ss > 0: alpha^2 + ss, beta and alpha - beta are finite and non-zero
+      // for finite data
+      const double beta = -copysign(lane_sqrt_pos(fma(alpha, alpha, ss)), alpha);
+      const double tau = lane_div_nz(beta - alpha, beta, lane_rcp_nz(beta));
+      const double scal = lane_rcp_nz(alpha - beta);
 #pragma unroll
       for (int r = i + 1; r < M; ++r)
         if (S::nz(i, r)) A[r * C + i] *= scal;
@@ -211,21 +294,6 @@ __device__ __forceinline__ void lane_qr(double* A) {
   for (int i = 0; i < k; ++i)
 #pragma unroll
     for (int j = 0; j < i; ++j) A[i * C + j] = 0.0;
-}
-
-// a / b given y = 1 / b (correctly rounded): q = a y, then one FMA
-// correction step against the exact residual a - b q (Markstein).  For a
-// correctly rounded, normal y and a finite quotient the result is the
-// correctly rounded quotient, as a division gives it, for three instructions
-// once y is known; used where many quotients share one divisor.  Any other
-// case (b = 0, infinities, NaN, y outside the normal range) takes the
-// division itself, so special values behave exactly as IEEE division.
-__device__ __forceinline__ double lane_div(double a, double b, double y) {
-  const double q = a * y;
-  const double r = fma(-q, b, a);
-  const double res = r == 0.0 ? q : fma(r, y, q);  // exact q keeps the sign of a zero quotient
-  if (__builtin_expect(!(__builtin_isnormal(y) && isfinite(res)), 0)) return a / b;
-  return res;
 }
 
 // [dense N x N; upper-triangular N x N] stacks (sum_of_sqrtm_factors of a
@@ -311,7 +379,7 @@ __device__ __forceinline__ void lane_precon(double dt, double* P, double* PI) {
   if constexpr (i < N) {
     constexpr int pw = N - 1 - i;
     const double s = factorial_d(pw);
-    P[i] = lane_ipow<pw>(dt) / s;
+    P[i] = lane_div_nz(lane_ipow<pw>(dt), s, 1.0 / s);  // 1 / s correctly rounded at compile time
     PI[i] = lane_ipow<-pw>(dt) * s;
     lane_precon<N, i + 1>(dt, P, PI);
   }
@@ -443,7 +511,7 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ Q,
   // gain = solve_triu(R_Y, R12)^T
   double G[N * N], rdiag[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i) rdiag[i] = 1.0 / big[i * n2 + i];
+  for (int i = 0; i < N; ++i) rdiag[i] = lane_rcp_nz(big[i * n2 + i]);
 #pragma unroll
   for (int j = 0; j < N; ++j)
 #pragma unroll
@@ -473,8 +541,8 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ Q,
     }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    bw.tl(i) = 1.0 / P[i];
-    bw.to(i) = 1.0 / PI[i];
+    bw.tl(i) = lane_rcp_nz(P[i]);
+    bw.to(i) = lane_rcp_nz(PI[i]);
   }
 }
 
@@ -680,7 +748,7 @@ __device__ __forceinline__ void lane_update(const pdq_config& cfg, const double*
   for (int c = 0; c < D; ++c) mo[c] = lane_obs_row<D>(cfg, h0, Pm, c) + b[c];
   lane_qr<n2, n2, UpdateRows>(big);
   double gain[N];
-  const double r0 = 1.0 / big[0];
+  const double r0 = lane_rcp_nz(big[0]);
 #pragma unroll
   for (int j = 0; j < N; ++j) gain[j] = lane_div(big[1 + j], big[0], r0);
 #pragma unroll
@@ -730,7 +798,7 @@ __device__ __forceinline__ double lane_error_power(const pdq_config& cfg, double
     for (int k = 0; k < D; ++k) sr += ref[k] * ref[k];
     norm = (sqrt(sa) / sqrt(1.0)) / (cfg.atol + cfg.rtol * (sqrt(sr) / sqrt((double)D)));
   }
-  return pow(norm, -1.0 / (double)N);
+  return lane_root_inv<N>(norm);
 }
 
 // error_residual_std's calibrated std of the observed residual: the
