@@ -1212,197 +1212,377 @@ __device__ __forceinline__ void lane_finalize(const LaneCtx<N, D>& c, int sf) {
   }
 }
 
-// One instance: init, the save_at / fixed-grid loop, finalize, counters.
-// `slots` is the lane's slot storage (its LDS column on the device).
+// The adaptive loop's state of one lane (solvers_via_adaptive_steps.py:
+// 46-375): the three slots' roles, the checkpoint index, the controller.
+struct LaneLoop {
+  int sf, ifr, pr, k, att, status;
+  bool second_half;
+  double dt, cstate;
+  long attempts, accepted;
+};
+
+// The fixed-point merge an iteration asks for: cond(res) = cond(self) o
+// cond(other), with `ident` when cond(self) is the identity (lane_merge).
+struct LaneMergeReq {
+  int on, ident, self, other, res;
+};
+
+// What the first part of an iteration hands to the second.
 template <int N, int D>
-__device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, double* slots) {
-  const pdq_config& cfg = a.cfg;
+struct LaneIterVals {
+  int op;
+  double t1, tsf, h, tn, sigma, err, h0, os;
+  double P[N], PI[N], b[D], Xm[N * D], Pm[N * D], PL[N * N];
+};
+
+enum { LOP_STEP = 0, LOP_BEYOND1 = 1, LOP_BEYOND2 = 2, LOP_AT = 3 };
+
+// prior init (from_mean_and_std): mean = Taylor coefficients, chol =
+// init_std I, in slot sf; interp_from = a copy; checkpoint 0.
+template <int N, int D>
+__device__ __forceinline__ void lane_init_instance(const LaneCtx<N, D>& c, const double* ts, int sf, int ifr) {
+  const pdq_config& cfg = c.a->cfg;
+  const LaneWs<N, D>& w = c.w;
+  const double* tc = c.a->tcoeffs + (size_t)c.inst * N * D;
+  double m[N * D], L[N * N];
+#pragma unroll
+  for (int e = 0; e < N * D; ++e) m[e] = tc[e];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) L[i * N + j] = i == j ? (cfg.init_std_rows ? cfg.init_std_rows[i] : cfg.init_std) : 0.0;
+#pragma unroll
+  for (int e = 0; e < SlotLayout<N, D>::size; ++e) w.at(sf, e) = 0.0;
+  w.store_mL(sf, m, L);
+  w.at(sf, LS_T) = ts[0];
+  w.at(sf, LS_OS) = 1.0;
+  w.at(sf, LS_NSTEPS) = 0.0;
+  w.at(sf, LS_IDENT) = cfg.strategy != PDQ_FILTER ? 1.0 : 0.0;
+  w.copy_slot(ifr, sf);
+  lane_write_checkpoint<N, D, CondRef<N, D>>(c, 0, ts[0], m, L, 1.0, 0.0, nullptr, false);
+}
+
+// `valid`: inst names an instance of the launch (per-instance inputs are read).
+__device__ __forceinline__ LaneLoop lane_loop_start(const pdq_config& cfg, int inst, bool valid = true) {
+  LaneLoop s;
+  s.sf = 0;
+  s.ifr = 1;
+  s.pr = 2;
+  s.k = 1;
+  s.att = 0;
+  s.status = PDQ_STATUS_OK;
+  s.second_half = false;
+  s.dt = cfg.dt0_per_instance && valid ? cfg.dt0_per_instance[inst] : cfg.dt0;
+  s.cstate = 1.0;
+  s.attempts = 0;
+  s.accepted = 0;
+  return s;
+}
+
+// The adaptive loop as a state machine: every iteration performs one
+// operation per lane -- a step attempt, one half of interpolate_fwd beyond
+// t1, or interpolate_fwd_at_t1 -- in the reference's order.  Step attempts
+// and both halves of the interpolation share one prediction (revert +
+// merge), so lanes of a wavefront that are at different points of their
+// solves execute it together instead of one after the other.
+//
+// Part 1: choose the operation, calibrate (solver_dynamic steps), and
+// predict: the filter marginalises the transition; the smoothers revert it
+// into the conditional of slot `bws`.  The fixed-point smoother's merge of
+// that conditional behind another one is returned as a request: it writes
+// only conditional storage that part 2 neither reads nor writes, so it may
+// run before part 2 or beside it.  interpolate_fwd_at_t1 is done here whole
+// (op LOP_AT, no part 2).
+template <int N, int D>
+__device__ __forceinline__ void lane_adapt_part1(const LaneCtx<N, D>& c, const double* ts, LaneLoop& s,
+                                                 LaneIterVals<N, D>& v, LaneMergeReq& rq) {
+  const pdq_config& cfg = c.a->cfg;
+  const LaneWs<N, D>& w = c.w;
+  rq.on = 0;
+  v.t1 = ts[s.k];
+  v.tsf = w.at(s.sf, LS_T);
+  if (s.second_half) v.op = LOP_BEYOND2;
+  else if (v.tsf + cfg.eps < v.t1) v.op = LOP_STEP;
+  else if (v.tsf > v.t1 + cfg.eps) v.op = LOP_BEYOND1;
+  else v.op = LOP_AT;
+  if (v.op == LOP_AT) {
+    lane_interp_at<N, D>(c, s.k, s.ifr, s.sf);
+    ++s.k;
+    return;
+  }
+  // inputs of the shared prediction
+  const int xs = v.op == LOP_STEP ? s.sf : s.ifr;
+  double XL[N * N];
+  w.load_mL(xs, v.Xm, XL);
+  v.tn = 0.0;
+  v.sigma = 1.0;
+  v.err = 0.0;
+  v.h0 = 0.0;
+  if (v.op == LOP_STEP) {
+    v.h = s.dt;
+    if (cfg.clip_dt) v.h = fmin(v.h, v.t1 - v.tsf);
+    v.tn = v.tsf + v.h;
+  } else if (v.op == LOP_BEYOND1) {
+    v.h = v.t1 - w.at(s.ifr, LS_T);  // interp_from -> t1
+  } else {
+    v.h = v.tsf - v.t1;  // t1 -> step_from
+  }
+  lane_precon<N>(v.h, v.P, v.PI);
+  if (v.op == LOP_STEP && cfg.solver == PDQ_SOLVER_DYNAMIC)
+    lane_calibrate<N, D>(c, v.Xm, v.P, v.PI, v.h, v.tn, v.b, v.h0, v.sigma, v.err, true);
+  v.os = w.at(s.sf, LS_OS);
+  const double scale = v.op == LOP_STEP ? v.sigma : v.os;
+  // step: revert into proposed, merge step_from's conditional in front;
+  // first half: revert into step_from's conditional (fixed point; it is
+  // rewritten by the second half) and merge interp_from's in front, in
+  // place (fixed interval: revert straight into interp_from's);
+  // second half: revert into step_from's conditional behind an identity.
+  int bws;
+  if (v.op == LOP_STEP) {
+    bws = s.pr;
+    rq.self = s.sf;
+    rq.res = s.pr;
+    rq.ident = w.at(s.sf, LS_IDENT) != 0.0;
+  } else if (v.op == LOP_BEYOND1) {
+    bws = cfg.strategy == PDQ_FIXEDPOINT ? s.sf : s.ifr;
+    rq.self = s.ifr;
+    rq.res = s.ifr;
+    rq.ident = w.at(s.ifr, LS_IDENT) != 0.0;
+  } else {
+    bws = s.sf;
+    rq.self = s.sf;
+    rq.res = s.sf;
+    rq.ident = 1;
+  }
+  rq.other = bws;
+  rq.on = cfg.strategy == PDQ_FIXEDPOINT;
+  PDQ_PHASE_BOUNDARY();
+  const double ns = lane_noise_scale(cfg, v.h, scale);
+  if (cfg.strategy == PDQ_FILTER)
+    lane_marginalise_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL);
+  else
+    lane_revert_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL, w.cond(bws));
+  PDQ_PHASE_BOUNDARY();
+}
+
+// The requested merge (ssm_impl_isotropic.py:91-105).
+template <int N, int D>
+__device__ __forceinline__ void lane_run_merge(const LaneWs<N, D>& w, const LaneMergeReq& rq) {
+  if (rq.on) lane_merge<N, D>(rq.ident != 0, w.cond(rq.self), w.cond(rq.other), w.cond(rq.res));
+}
+
+// Part 2: the step's update, error estimate and control decision, or the
+// interpolation's bookkeeping and checkpoint.
+template <int N, int D>
+__device__ __forceinline__ void lane_adapt_part2(const LaneCtx<N, D>& c, LaneLoop& s, LaneIterVals<N, D>& v) {
+  const pdq_config& cfg = c.a->cfg;
+  const LaneWs<N, D>& w = c.w;
+  const bool smoother = cfg.strategy != PDQ_FILTER;
+  if (v.op == LOP_STEP) {
+    const double ep = lane_finish_step<N, D>(c, s.sf, s.pr, v.h, v.tn, v.P, v.PI, v.Xm, v.Pm, v.PL, v.b, v.h0,
+                                             v.sigma, v.err, true);
+    s.dt = lane_control(cfg, v.h, s.cstate, ep);
+    ++s.att;
+    ++s.attempts;
+    if (!isfinite(ep) || !isfinite(s.dt)) {
+      s.status = PDQ_STATUS_NONFINITE;
+    } else if (ep < 1.0 && (s.dt == 0.0 || v.tsf + s.dt == v.tsf)) {
+      s.status = PDQ_STATUS_DT_UNDERFLOW;
+    } else if (ep >= 1.0 && w.at(s.pr, LS_T) == v.tsf) {
+      s.status = PDQ_STATUS_DT_UNDERFLOW;  // accepted without advancing t
+    } else if ((cfg.max_attempts > 0 && s.att >= cfg.max_attempts && ep < 1.0) || s.attempts >= kMaxTotalAttempts) {
+      s.status = PDQ_STATUS_MAX_ATTEMPTS;
+    } else if (ep >= 1.0) {
+      ++s.accepted;
+      s.att = 0;
+      const int old_if = s.ifr;  // interp_from <- step_from, step_from <- proposed
+      s.ifr = s.sf;
+      s.sf = s.pr;
+      s.pr = old_if;
+    }
+  } else if (v.op == LOP_BEYOND1) {
+    const double nsteps = w.at(s.sf, LS_NSTEPS);
+    if (!smoother) {
+      lane_write_checkpoint<N, D, CondRef<N, D>>(c, s.k, v.t1, v.Pm, v.PL, v.os, nsteps, nullptr, false);
+      w.store_mL(s.ifr, v.Pm, v.PL);
+      w.at(s.ifr, LS_T) = v.t1;
+      ++s.k;
+    } else {
+      w.store_mL(s.ifr, v.Pm, v.PL);  // ext_t's marginal: the new interp_from's
+      s.second_half = true;
+    }
+  } else {
+    // interpolated = (ext_t.marginal, ext_t.conditional), the checkpoint
+    double Em[N * D], EL[N * N];
+    w.load_mL(s.ifr, Em, EL);
+    const CondRef<N, D> ext = w.cond(s.ifr);
+    lane_write_checkpoint<N, D>(c, s.k, v.t1, Em, EL, v.os, w.at(s.sf, LS_NSTEPS), &ext, false);
+    w.at(s.ifr, LS_T) = v.t1;
+    w.at(s.ifr, LS_IDENT) = cfg.strategy == PDQ_FIXEDPOINT ? 1.0 : 0.0;
+    w.at(s.sf, LS_IDENT) = 0.0;
+    s.second_half = false;
+    ++s.k;
+  }
+}
+
+template <int N, int D>
+__device__ __forceinline__ void lane_end_instance(const LaneCtx<N, D>& c, const LaneLoop& s) {
+  if (s.status == PDQ_STATUS_OK) lane_finalize<N, D>(c, s.sf);
+  const pdq_solution& o = c.a->out;
+  const size_t inst = (size_t)c.inst;
+  if (o.attempts) o.attempts[inst] = s.attempts;
+  if (o.accepted) o.accepted[inst] = s.accepted;
+  if (o.status) o.status[inst] = s.status;
+}
+
+template <int N, int D>
+__device__ __forceinline__ LaneCtx<N, D> lane_ctx(const LArgs& a, int inst, double* slots) {
   LaneCtx<N, D> c;
   c.a = &a;
   c.inst = inst;
-  c.prm = a.params + (size_t)inst * cfg.n_params;
+  c.prm = a.params + (size_t)inst * a.cfg.n_params;
   c.w.base = slots;
-  const LaneWs<N, D>& w = c.w;
-  const int K = a.K;
-  const double* ts = a.times + (size_t)inst * K;
-  int sf = 0, ifr = 1, pr = 2;
-  // prior init (from_mean_and_std): mean = Taylor coefficients, chol = init_std I
-  {
-    const double* tc = a.tcoeffs + (size_t)inst * N * D;
-    double m[N * D], L[N * N];
-#pragma unroll
-    for (int e = 0; e < N * D; ++e) m[e] = tc[e];
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int j = 0; j < N; ++j) L[i * N + j] = i == j ? (cfg.init_std_rows ? cfg.init_std_rows[i] : cfg.init_std) : 0.0;
-#pragma unroll
-    for (int e = 0; e < SlotLayout<N, D>::size; ++e) w.at(sf, e) = 0.0;
-    w.store_mL(sf, m, L);
-    w.at(sf, LS_T) = ts[0];
-    w.at(sf, LS_OS) = 1.0;
-    w.at(sf, LS_NSTEPS) = 0.0;
-    w.at(sf, LS_IDENT) = cfg.strategy != PDQ_FILTER ? 1.0 : 0.0;
-    w.copy_slot(ifr, sf);
-    lane_write_checkpoint<N, D, CondRef<N, D>>(c, 0, ts[0], m, L, 1.0, 0.0, nullptr, false);
-  }
-  long attempts = 0, accepted = 0;
-  int status = PDQ_STATUS_OK;
-  if (a.mode == LMODE_FIXED_GRID) {
-    for (int k = 1; k < K; ++k) {
-      lane_step<N, D>(c, sf, pr, ts[k] - ts[k - 1], false);
-      ++attempts;
-      ++accepted;
-      lane_write_slot<N, D>(c, k, pr);
-      const int tmp = sf;
-      sf = pr;
-      pr = tmp;
-    }
-  } else {
-    // The adaptive loop (solvers_via_adaptive_steps.py:46-375) as a state
-    // machine: every iteration performs one operation per lane -- a step
-    // attempt, one half of interpolate_fwd beyond t1, or
-    // interpolate_fwd_at_t1 -- in the reference's order.  Step attempts and
-    // both halves of the interpolation share one prediction (revert + merge),
-    // so lanes of a wavefront that are at different points of their solves
-    // execute it together instead of one after the other.
-    enum { OP_STEP = 0, OP_BEYOND1 = 1, OP_BEYOND2 = 2, OP_AT = 3 };
-    const bool smoother = cfg.strategy != PDQ_FILTER;
-    double dt = cfg.dt0_per_instance ? cfg.dt0_per_instance[c.inst] : cfg.dt0, cstate = 1.0;
-    int k = 1, att = 0;
-    bool second_half = false;
-    while (k < K && status == PDQ_STATUS_OK) {
-      const double t1 = ts[k];
-      const double tsf = w.at(sf, LS_T);
-      int op;
-      if (second_half) op = OP_BEYOND2;
-      else if (tsf + cfg.eps < t1) op = OP_STEP;
-      else if (tsf > t1 + cfg.eps) op = OP_BEYOND1;
-      else op = OP_AT;
-      if (op == OP_AT) {
-        lane_interp_at<N, D>(c, k, ifr, sf);
-        ++k;
-        continue;
-      }
-      // inputs of the shared prediction
-      const int xs = op == OP_STEP ? sf : ifr;
-      double Xm[N * D], XL[N * N], P[N], PI[N], b[D], Pm[N * D], PL[N * N];
-      w.load_mL(xs, Xm, XL);
-      double h, tn = 0.0, sigma = 1.0, err = 0.0, h0 = 0.0;
-      if (op == OP_STEP) {
-        h = dt;
-        if (cfg.clip_dt) h = fmin(h, t1 - tsf);
-        tn = tsf + h;
-      } else if (op == OP_BEYOND1) {
-        h = t1 - w.at(ifr, LS_T);  // interp_from -> t1
-      } else {
-        h = tsf - t1;  // t1 -> step_from
-      }
-      lane_precon<N>(h, P, PI);
-      if (op == OP_STEP && cfg.solver == PDQ_SOLVER_DYNAMIC)
-        lane_calibrate<N, D>(c, Xm, P, PI, h, tn, b, h0, sigma, err, true);
-      const double os = w.at(sf, LS_OS);
-      const double scale = op == OP_STEP ? sigma : os;
-      // step: revert into proposed, merge step_from's conditional in front;
-      // first half: revert into step_from's conditional (fixed point; it is
-      // rewritten by the second half) and merge interp_from's in front, in
-      // place (fixed interval: revert straight into interp_from's);
-      // second half: revert into step_from's conditional behind an identity.
-      int bws, mself, mres;
-      bool mident;
-      if (op == OP_STEP) {
-        bws = pr;
-        mself = sf;
-        mres = pr;
-        mident = w.at(sf, LS_IDENT) != 0.0;
-      } else if (op == OP_BEYOND1) {
-        bws = cfg.strategy == PDQ_FIXEDPOINT ? sf : ifr;
-        mself = ifr;
-        mres = ifr;
-        mident = w.at(ifr, LS_IDENT) != 0.0;
-      } else {
-        bws = sf;
-        mself = sf;
-        mres = sf;
-        mident = true;
-      }
-      PDQ_PHASE_BOUNDARY();
-      lane_predict<N, D>(c, P, PI, lane_noise_scale(cfg, h, scale), Xm, XL, Pm, PL, bws,
-                         cfg.strategy == PDQ_FIXEDPOINT, mident, mself, mres);
-      if (op == OP_STEP) {
-        const double ep = lane_finish_step<N, D>(c, sf, pr, h, tn, P, PI, Xm, Pm, PL, b, h0, sigma, err, true);
-        dt = lane_control(cfg, h, cstate, ep);
-        ++att;
-        ++attempts;
-        if (!isfinite(ep) || !isfinite(dt)) {
-          status = PDQ_STATUS_NONFINITE;
-        } else if (ep < 1.0 && (dt == 0.0 || tsf + dt == tsf)) {
-          status = PDQ_STATUS_DT_UNDERFLOW;
-        } else if (ep >= 1.0 && w.at(pr, LS_T) == tsf) {
-          status = PDQ_STATUS_DT_UNDERFLOW;  // accepted without advancing t
-        } else if ((cfg.max_attempts > 0 && att >= cfg.max_attempts && ep < 1.0) || attempts >= kMaxTotalAttempts) {
-          status = PDQ_STATUS_MAX_ATTEMPTS;
-        } else if (ep >= 1.0) {
-          ++accepted;
-          att = 0;
-          const int old_if = ifr;  // interp_from <- step_from, step_from <- proposed
-          ifr = sf;
-          sf = pr;
-          pr = old_if;
-        }
-      } else if (op == OP_BEYOND1) {
-        const double nsteps = w.at(sf, LS_NSTEPS);
-        if (!smoother) {
-          lane_write_checkpoint<N, D, CondRef<N, D>>(c, k, t1, Pm, PL, os, nsteps, nullptr, false);
-          w.store_mL(ifr, Pm, PL);
-          w.at(ifr, LS_T) = t1;
-          ++k;
-        } else {
-          w.store_mL(ifr, Pm, PL);  // ext_t's marginal: the new interp_from's
-          second_half = true;
-        }
-      } else {
-        // interpolated = (ext_t.marginal, ext_t.conditional), the checkpoint
-        double Em[N * D], EL[N * N];
-        w.load_mL(ifr, Em, EL);
-        const CondRef<N, D> ext = w.cond(ifr);
-        lane_write_checkpoint<N, D>(c, k, t1, Em, EL, os, w.at(sf, LS_NSTEPS), &ext, false);
-        w.at(ifr, LS_T) = t1;
-        w.at(ifr, LS_IDENT) = cfg.strategy == PDQ_FIXEDPOINT ? 1.0 : 0.0;
-        w.at(sf, LS_IDENT) = 0.0;
-        second_half = false;
-        ++k;
-      }
-    }
-  }
-  if (status == PDQ_STATUS_OK) lane_finalize<N, D>(c, sf);
-  const pdq_solution& o = a.out;
-  if (o.attempts) o.attempts[inst] = attempts;
-  if (o.accepted) o.accepted[inst] = accepted;
-  if (o.status) o.status[inst] = status;
+  return c;
 }
 
-// LDS bytes of one 64-lane workgroup: three slots per lane.
+// One instance: init, the save_at / fixed-grid loop, finalize, counters.
+// `slots` is the lane's slot storage (its LDS column on the device).  The
+// adaptive loop runs part 1, the merge and part 2 of every iteration in turn.
 template <int N, int D>
+__device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, double* slots) {
+  const LaneCtx<N, D> c = lane_ctx<N, D>(a, inst, slots);
+  const int K = a.K;
+  const double* ts = a.times + (size_t)inst * K;
+  LaneLoop s = lane_loop_start(a.cfg, inst);
+  lane_init_instance<N, D>(c, ts, s.sf, s.ifr);
+  if (a.mode == LMODE_FIXED_GRID) {
+    for (int k = 1; k < K; ++k) {
+      lane_step<N, D>(c, s.sf, s.pr, ts[k] - ts[k - 1], false);
+      ++s.attempts;
+      ++s.accepted;
+      lane_write_slot<N, D>(c, k, s.pr);
+      const int tmp = s.sf;
+      s.sf = s.pr;
+      s.pr = tmp;
+    }
+  } else {
+    while (s.k < K && s.status == PDQ_STATUS_OK) {
+      LaneIterVals<N, D> v;
+      LaneMergeReq rq;
+      lane_adapt_part1<N, D>(c, ts, s, v, rq);
+      if (v.op == LOP_AT) continue;
+      lane_run_merge<N, D>(c.w, rq);
+      lane_adapt_part2<N, D>(c, s, v);
+    }
+  }
+  lane_end_instance<N, D>(c, s);
+}
+
+// Threads of a workgroup: the fixed-point smoother runs its merges on a
+// second wavefront (lane_solve_split), the other strategies on one.
+template <int STRATEGY>
+constexpr int lane_block_threads() {
+  return STRATEGY == PDQ_FIXEDPOINT ? 2 * kLaneBlock : kLaneBlock;
+}
+
+// LDS bytes of one workgroup: three slots per lane; with the second
+// wavefront, 64 merge requests and the loop flag after them.
+template <int N, int D, int STRATEGY = PDQ_FILTER>
 constexpr size_t lane_lds_bytes() {
-  return sizeof(double) * kLaneBlock * 3 * SlotLayout<N, D>::size;
+  return sizeof(double) * kLaneBlock * 3 * SlotLayout<N, D>::size +
+         (STRATEGY == PDQ_FIXEDPOINT ? sizeof(double) * (kLaneBlock / 2 + 1) : 0);
 }
 
 #ifndef PDQ_HOST_EMU
-// One lane per instance, grid-stride over instances.  Each lane's slots live
-// in LDS, element-major across the workgroup's lanes, read and written by
-// that lane only.  The strategy and the solver are compile-time constants of
-// the instantiation: the kernel overwrites them in its own copy of the
-// arguments, so every test of them folds and the other strategies' and
-// solvers' code (with the registers it would hold) is not compiled in.
-// Without OPTS the linearisation, the error estimator's and the
-// calibration's options take their defaults the same way (ts0,
+// Packed merge requests: on, ident, self, other, res (2 bits per slot).
+__device__ __forceinline__ int lane_pack_req(const LaneMergeReq& r) {
+  return r.on | (r.ident << 1) | (r.self << 2) | (r.other << 4) | (r.res << 6);
+}
+__device__ __forceinline__ LaneMergeReq lane_unpack_req(int v) {
+  LaneMergeReq r;
+  r.on = v & 1;
+  r.ident = (v >> 1) & 1;
+  r.self = (v >> 2) & 3;
+  r.other = (v >> 4) & 3;
+  r.res = (v >> 6) & 3;
+  return r;
+}
+
+// The fixed-point smoother's adaptive solve on two wavefronts: wavefront 0
+// runs parts 1 and 2 of every iteration for the workgroup's 64 instances
+// (one lane each, as lane_solve_instance); wavefront 1 runs the merges that
+// part 1 asks for while wavefront 0 runs part 2.  The merge writes only the
+// conditional of one slot, which part 2 neither reads nor writes (part 2
+// touches means, factors and scalars, and the conditional of a slot the
+// merge does not write), so the result is that of the serial order.
+// Lane j of both wavefronts works on instance j's LDS column only.
+// Per iteration: part 1 and the request (wave 0) | barrier | merge (wave 1)
+// and part 2 (wave 0), then wave 0 posts whether any of its lanes goes on |
+// barrier.  Both wavefronts read the posted flag after the same barrier, so
+// they pass the same number of barriers and leave the loop together; the
+// request and the flag are rewritten only after the barrier that follows
+// their last read.
+template <int N, int D>
+__device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
+  const int l = threadIdx.x & (kLaneBlock - 1);
+  const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / kLaneBlock);  // wave-uniform
+  double* const slots = lds + l;
+  int* const req = reinterpret_cast<int*>(lds + kLaneBlock * 3 * SlotLayout<N, D>::size);
+  volatile int* const flag = req + kLaneBlock;
+  const int inst = blockIdx.x * kLaneBlock + l;
+  const bool mine = inst < a.n_inst;
+  const LaneCtx<N, D> c = lane_ctx<N, D>(a, mine ? inst : 0, slots);
+  if (role == 0) {
+    const int K = a.K;
+    const double* ts = a.times + (size_t)(mine ? inst : 0) * K;
+    LaneLoop s = lane_loop_start(a.cfg, inst, mine);
+    if (mine) lane_init_instance<N, D>(c, ts, s.sf, s.ifr);
+    bool run = mine && s.k < K;
+    const int any0 = __any(run);
+    if (l == 0) *flag = any0;
+    __syncthreads();
+    while (__builtin_amdgcn_readfirstlane(*flag)) {
+      LaneIterVals<N, D> v;
+      LaneMergeReq rq;
+      rq.on = 0;
+      v.op = LOP_AT;
+      if (run) lane_adapt_part1<N, D>(c, ts, s, v, rq);
+      req[l] = lane_pack_req(rq);
+      __syncthreads();
+      if (run && v.op != LOP_AT) lane_adapt_part2<N, D>(c, s, v);
+      run = run && s.k < K && s.status == PDQ_STATUS_OK;
+      const int any = __any(run);
+      if (l == 0) *flag = any;
+      __syncthreads();
+    }
+    if (mine) lane_end_instance<N, D>(c, s);
+  } else {
+    __syncthreads();
+    while (__builtin_amdgcn_readfirstlane(*flag)) {
+      __syncthreads();
+      lane_run_merge<N, D>(c.w, lane_unpack_req(req[l]));
+      __syncthreads();
+    }
+  }
+}
+
+// One lane per instance.  Each lane's slots live in LDS, element-major across
+// the workgroup's lanes, read and written by that lane only (and, for the
+// fixed-point smoother's adaptive solve, by the same lane of the merging
+// wavefront: lane_solve_split).  The strategy and the solver are
+// compile-time constants of the instantiation: the kernel overwrites them in
+// its own copy of the arguments, so every test of them folds and the other
+// strategies' and solvers' code (with the registers it would hold) is not
+// compiled in.  Without OPTS the linearisation, the error estimator's and
+// the calibration's options take their defaults the same way (ts0,
 // error_residual_std without re-linearisation or per-unit-step scaling, no
 // re-linearisation after the calibration), and the launcher picks this
 // instantiation only for configurations that hold them; with OPTS they are
-// read at run time.
+// read at run time.  The fixed-point smoother's launches cover every
+// instance with one lane of wavefront 0 (no grid stride); the others stride
+// over instances.
 template <int N, int D, int STRATEGY, int SOLVER, bool OPTS>
-__global__ void __launch_bounds__(64) lane_kernel(LArgs args) {
+__global__ void __launch_bounds__(lane_block_threads<STRATEGY>()) lane_kernel(LArgs args) {
   extern __shared__ double lds[];
   LArgs a = args;
   a.cfg.strategy = STRATEGY;
@@ -1415,10 +1595,19 @@ __global__ void __launch_bounds__(64) lane_kernel(LArgs args) {
     a.cfg.error_per_unit_step = 0;
     a.cfg.re_linearize_after_calibration = 0;
   }
-  double* const slots = lds + threadIdx.x;
-  const int stride = gridDim.x * blockDim.x;
-  for (int inst = blockIdx.x * blockDim.x + threadIdx.x; inst < a.n_inst; inst += stride)
-    lane_solve_instance<N, D>(a, inst, slots);
+  if constexpr (STRATEGY == PDQ_FIXEDPOINT) {
+    if (a.mode != LMODE_FIXED_GRID) {
+      lane_solve_split<N, D>(a, lds);
+    } else if (threadIdx.x < kLaneBlock) {
+      const int inst = blockIdx.x * kLaneBlock + threadIdx.x;
+      if (inst < a.n_inst) lane_solve_instance<N, D>(a, inst, lds + threadIdx.x);
+    }
+  } else {
+    double* const slots = lds + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    for (int inst = blockIdx.x * blockDim.x + threadIdx.x; inst < a.n_inst; inst += stride)
+      lane_solve_instance<N, D>(a, inst, slots);
+  }
 }
 #endif
 

@@ -33,10 +33,10 @@ int lane_blocks_per_cu(int& per_cu) {
   PDQ_HIP_TRY(hipGetDevice(&dev));
   if (dev >= 0 && dev < kLaneMaxDevices && (per_cu = cache[dev].load()) > 0) return PDQ_OK;
   auto kern = lane_kernel<N, D, S, V, O>;
-  const size_t lds = lane_lds_bytes<N, D>();
+  const size_t lds = lane_lds_bytes<N, D, S>();
   PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   per_cu = 0;
-  PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
+  PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, lane_block_threads<S>(), lds));
   if (per_cu < 1) return host_error(PDQ_ERR_UNSUPPORTED, "lane kernel does not fit (%zu bytes of LDS)", lds);
   if (dev >= 0 && dev < kLaneMaxDevices) cache[dev].store(per_cu);
   return PDQ_OK;
@@ -45,14 +45,17 @@ int lane_blocks_per_cu(int& per_cu) {
 template <int N, int D, int S, int V, bool O>
 int launch_lane_nd(LArgs& a, hipStream_t st) {
   auto kern = lane_kernel<N, D, S, V, O>;
-  const size_t lds = lane_lds_bytes<N, D>();
+  const size_t lds = lane_lds_bytes<N, D, S>();
   int per_cu = 0;
   const int rc = lane_blocks_per_cu<N, D, S, V, O>(per_cu);
   if (rc) return rc;
   const long need = ((long)a.n_inst + 63) / 64;
-  const long cap = (long)per_cu * num_cus();
+  if (S == PDQ_FIXEDPOINT && need > 0x7fffffffL) return host_error(PDQ_ERR_UNSUPPORTED, "too many instances");
+  // the fixed-point kernel gives every instance a lane of its own (no grid
+  // stride); the others stride over at most the resident workgroups
+  const long cap = S == PDQ_FIXEDPOINT ? need : (long)per_cu * num_cus();
   const int grid = (int)std::max<long>(1, std::min(need, cap));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, a);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(lane_block_threads<S>()), lds, st, a);
   PDQ_HIP_TRY(hipGetLastError());
   return PDQ_OK;
 }
