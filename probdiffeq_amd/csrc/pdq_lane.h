@@ -474,9 +474,13 @@ __device__ __forceinline__ void lane_marginalise_prior(const double* __restrict_
 // cholesky_util.py:27-82): observed (Ym, YL) and backward conditional bw
 // (written to any conditional storage).
 template <int N, int D, class Cond>
+// With `ident_rescale` the conditional is stored already merged behind an
+// identity (lane_merge with an identity `self`: rows scaled by to_observed,
+// to_observed = 1), bitwise as the revert followed by that merge.
 __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ Q,
                                                   const double* P, const double* PI, double s, const double* Xm,
-                                                  const double* XL, double* Ym, double* YL, Cond&& bw) {
+                                                  const double* XL, double* Ym, double* YL, Cond&& bw,
+                                                  bool ident_rescale = false) {
   constexpr int n2 = 2 * N;
   double big[n2 * n2];
   double tmp[N * D];
@@ -528,21 +532,22 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ Q,
       double acc = 0.0;
 #pragma unroll
       for (int k = 0; k < N; ++k) acc += G[i * N + k] * tmp[k * D + col];
-      bw.m(i, col) = PI[i] * Xm[i * D + col] - acc;
+      const double mv = PI[i] * Xm[i * D + col] - acc;
+      bw.m(i, col) = ident_rescale ? (1.0 * lane_rcp_nz(PI[i])) * mv + 0.0 : mv;
       Ym[i * D + col] = P[i] * tmp[i * D + col];
     }
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int i = 0; i < N; ++i) {
+    const double to = lane_rcp_nz(PI[i]);
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      bw.A(i, j) = G[i * N + j];
-      bw.setL(i, j, big[(N + j) * n2 + N + i]);
+      bw.A(i, j) = ident_rescale ? (1.0 * to) * G[i * N + j] : G[i * N + j];
+      const double lv = big[(N + j) * n2 + N + i];
+      bw.setL(i, j, ident_rescale ? fabs(1.0 * to) * lv : lv);
       YL[i * N + j] = fabs(P[i]) * big[j * n2 + i];
     }
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
     bw.tl(i) = lane_rcp_nz(P[i]);
-    bw.to(i) = lane_rcp_nz(PI[i]);
+    bw.to(i) = ident_rescale ? 1.0 : to;
   }
 }
 
@@ -1230,7 +1235,7 @@ struct LaneMergeReq {
 // What the first part of an iteration hands to the second.
 template <int N, int D>
 struct LaneIterVals {
-  int op;
+  int op, ident;
   double t1, tsf, h, tn, sigma, err, h0, os;
   double P[N], PI[N], b[D], Xm[N * D], Pm[N * D], PL[N * N];
 };
@@ -1279,37 +1284,55 @@ __device__ __forceinline__ LaneLoop lane_loop_start(const pdq_config& cfg, int i
   return s;
 }
 
+// The operation of the next iteration of the adaptive loop (needs s.k < K).
+template <int N, int D>
+__device__ __forceinline__ int lane_next_op(const pdq_config& cfg, const double* ts, const LaneLoop& s,
+                                            const LaneWs<N, D>& w) {
+  const double t1 = ts[s.k];
+  const double tsf = w.at(s.sf, LS_T);
+  if (s.second_half) return LOP_BEYOND2;
+  if (tsf + cfg.eps < t1) return LOP_STEP;
+  if (tsf > t1 + cfg.eps) return LOP_BEYOND1;
+  return LOP_AT;
+}
+
 // The adaptive loop as a state machine: every iteration performs one
 // operation per lane -- a step attempt, one half of interpolate_fwd beyond
 // t1, or interpolate_fwd_at_t1 -- in the reference's order.  Step attempts
-// and both halves of the interpolation share one prediction (revert +
-// merge), so lanes of a wavefront that are at different points of their
-// solves execute it together instead of one after the other.
+// and both halves of the interpolation share one prediction (revert), so
+// lanes of a wavefront that are at different points of their solves execute
+// it together instead of one after the other.
 //
 // Part 1: choose the operation, calibrate (solver_dynamic steps), and
 // predict: the filter marginalises the transition; the smoothers revert it
-// into the conditional of slot `bws`.  The fixed-point smoother's merge of
-// that conditional behind another one is returned as a request: it writes
-// only conditional storage that part 2 neither reads nor writes, so it may
-// run before part 2 or beside it.  interpolate_fwd_at_t1 is done here whole
-// (op LOP_AT, no part 2).
+// into the conditional of slot `bws`.
+//
+// Part 2: the step's update, error estimate and control decision, or the
+// interpolation's bookkeeping and checkpoint (interpolate_fwd_at_t1 is done
+// here whole).  It returns the fixed-point smoother's merge of the new
+// backward conditional behind the accumulated one as a request, to be run
+// after part 2 and before the next iteration's part 1 or beside that part 1:
+// * a step's merge, cond(proposed) = cond(step_from) o cond(proposed), is
+//   wanted only if the step is accepted and the next operation is not the
+//   first half of an interpolation (which replaces that conditional);
+// * the first half reverts into the proposed slot (unused between steps);
+//   its merge is cond(interp_from) = cond(interp_from) o cond(proposed);
+// * the second half reverts into step_from's conditional, already merged
+//   behind an identity (lane_revert_prior's rescaling), and asks for none.
+// The requested merge reads and writes only conditionals that the next part
+// 1 neither reads nor writes: after an accepted step the next step reverts
+// into the new proposed slot, the old interp_from; the second half writes
+// step_from's conditional while the first half's merge uses interp_from's
+// and proposed's; interpolate_fwd_at_t1 does nothing in part 1.
 template <int N, int D>
 __device__ __forceinline__ void lane_adapt_part1(const LaneCtx<N, D>& c, const double* ts, LaneLoop& s,
-                                                 LaneIterVals<N, D>& v, LaneMergeReq& rq) {
+                                                 LaneIterVals<N, D>& v) {
   const pdq_config& cfg = c.a->cfg;
   const LaneWs<N, D>& w = c.w;
-  rq.on = 0;
   v.t1 = ts[s.k];
   v.tsf = w.at(s.sf, LS_T);
-  if (s.second_half) v.op = LOP_BEYOND2;
-  else if (v.tsf + cfg.eps < v.t1) v.op = LOP_STEP;
-  else if (v.tsf > v.t1 + cfg.eps) v.op = LOP_BEYOND1;
-  else v.op = LOP_AT;
-  if (v.op == LOP_AT) {
-    lane_interp_at<N, D>(c, s.k, s.ifr, s.sf);
-    ++s.k;
-    return;
-  }
+  v.op = lane_next_op<N, D>(cfg, ts, s, w);
+  if (v.op == LOP_AT) return;
   // inputs of the shared prediction
   const int xs = v.op == LOP_STEP ? s.sf : s.ifr;
   double XL[N * N];
@@ -1332,36 +1355,23 @@ __device__ __forceinline__ void lane_adapt_part1(const LaneCtx<N, D>& c, const d
     lane_calibrate<N, D>(c, v.Xm, v.P, v.PI, v.h, v.tn, v.b, v.h0, v.sigma, v.err, true);
   v.os = w.at(s.sf, LS_OS);
   const double scale = v.op == LOP_STEP ? v.sigma : v.os;
-  // step: revert into proposed, merge step_from's conditional in front;
-  // first half: revert into step_from's conditional (fixed point; it is
-  // rewritten by the second half) and merge interp_from's in front, in
-  // place (fixed interval: revert straight into interp_from's);
-  // second half: revert into step_from's conditional behind an identity.
+  // step: revert into proposed; first half: revert into proposed (fixed
+  // point, merged behind interp_from's conditional in part 2's request) or
+  // straight into interp_from's (fixed interval); second half: revert into
+  // step_from's, for the fixed point smoother already merged behind an
+  // identity
+  const bool fp = cfg.strategy == PDQ_FIXEDPOINT;
   int bws;
-  if (v.op == LOP_STEP) {
-    bws = s.pr;
-    rq.self = s.sf;
-    rq.res = s.pr;
-    rq.ident = w.at(s.sf, LS_IDENT) != 0.0;
-  } else if (v.op == LOP_BEYOND1) {
-    bws = cfg.strategy == PDQ_FIXEDPOINT ? s.sf : s.ifr;
-    rq.self = s.ifr;
-    rq.res = s.ifr;
-    rq.ident = w.at(s.ifr, LS_IDENT) != 0.0;
-  } else {
-    bws = s.sf;
-    rq.self = s.sf;
-    rq.res = s.sf;
-    rq.ident = 1;
-  }
-  rq.other = bws;
-  rq.on = cfg.strategy == PDQ_FIXEDPOINT;
+  if (v.op == LOP_STEP) bws = s.pr;
+  else if (v.op == LOP_BEYOND1) bws = fp ? s.pr : s.ifr;
+  else bws = s.sf;
+  v.ident = w.at(v.op == LOP_BEYOND1 ? s.ifr : s.sf, LS_IDENT) != 0.0;
   PDQ_PHASE_BOUNDARY();
   const double ns = lane_noise_scale(cfg, v.h, scale);
   if (cfg.strategy == PDQ_FILTER)
     lane_marginalise_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL);
   else
-    lane_revert_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL, w.cond(bws));
+    lane_revert_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL, w.cond(bws), fp && v.op == LOP_BEYOND2);
   PDQ_PHASE_BOUNDARY();
 }
 
@@ -1371,14 +1381,21 @@ __device__ __forceinline__ void lane_run_merge(const LaneWs<N, D>& w, const Lane
   if (rq.on) lane_merge<N, D>(rq.ident != 0, w.cond(rq.self), w.cond(rq.other), w.cond(rq.res));
 }
 
-// Part 2: the step's update, error estimate and control decision, or the
-// interpolation's bookkeeping and checkpoint.
 template <int N, int D>
-__device__ __forceinline__ void lane_adapt_part2(const LaneCtx<N, D>& c, LaneLoop& s, LaneIterVals<N, D>& v) {
+__device__ __forceinline__ LaneMergeReq lane_adapt_part2(const LaneCtx<N, D>& c, const double* ts, LaneLoop& s,
+                                                         LaneIterVals<N, D>& v) {
   const pdq_config& cfg = c.a->cfg;
   const LaneWs<N, D>& w = c.w;
   const bool smoother = cfg.strategy != PDQ_FILTER;
-  if (v.op == LOP_STEP) {
+  const bool fp = cfg.strategy == PDQ_FIXEDPOINT;
+  const int K = c.a->K;
+  LaneMergeReq rq;
+  rq.on = 0;
+  rq.ident = v.ident;
+  if (v.op == LOP_AT) {
+    lane_interp_at<N, D>(c, s.k, s.ifr, s.sf);
+    ++s.k;
+  } else if (v.op == LOP_STEP) {
     const double ep = lane_finish_step<N, D>(c, s.sf, s.pr, v.h, v.tn, v.P, v.PI, v.Xm, v.Pm, v.PL, v.b, v.h0,
                                              v.sigma, v.err, true);
     s.dt = lane_control(cfg, v.h, s.cstate, ep);
@@ -1395,10 +1412,14 @@ __device__ __forceinline__ void lane_adapt_part2(const LaneCtx<N, D>& c, LaneLoo
     } else if (ep >= 1.0) {
       ++s.accepted;
       s.att = 0;
+      rq.self = s.sf;
+      rq.other = s.pr;
+      rq.res = s.pr;
       const int old_if = s.ifr;  // interp_from <- step_from, step_from <- proposed
       s.ifr = s.sf;
       s.sf = s.pr;
       s.pr = old_if;
+      rq.on = fp && lane_next_op<N, D>(cfg, ts, s, w) != LOP_BEYOND1;
     }
   } else if (v.op == LOP_BEYOND1) {
     const double nsteps = w.at(s.sf, LS_NSTEPS);
@@ -1410,6 +1431,10 @@ __device__ __forceinline__ void lane_adapt_part2(const LaneCtx<N, D>& c, LaneLoo
     } else {
       w.store_mL(s.ifr, v.Pm, v.PL);  // ext_t's marginal: the new interp_from's
       s.second_half = true;
+      rq.self = s.ifr;
+      rq.other = s.pr;
+      rq.res = s.ifr;
+      rq.on = fp;
     }
   } else {
     // interpolated = (ext_t.marginal, ext_t.conditional), the checkpoint
@@ -1418,11 +1443,13 @@ __device__ __forceinline__ void lane_adapt_part2(const LaneCtx<N, D>& c, LaneLoo
     const CondRef<N, D> ext = w.cond(s.ifr);
     lane_write_checkpoint<N, D>(c, s.k, v.t1, Em, EL, v.os, w.at(s.sf, LS_NSTEPS), &ext, false);
     w.at(s.ifr, LS_T) = v.t1;
-    w.at(s.ifr, LS_IDENT) = cfg.strategy == PDQ_FIXEDPOINT ? 1.0 : 0.0;
+    w.at(s.ifr, LS_IDENT) = fp ? 1.0 : 0.0;
     w.at(s.sf, LS_IDENT) = 0.0;
     s.second_half = false;
     ++s.k;
   }
+  (void)K;
+  return rq;
 }
 
 template <int N, int D>
@@ -1447,7 +1474,7 @@ __device__ __forceinline__ LaneCtx<N, D> lane_ctx(const LArgs& a, int inst, doub
 
 // One instance: init, the save_at / fixed-grid loop, finalize, counters.
 // `slots` is the lane's slot storage (its LDS column on the device).  The
-// adaptive loop runs part 1, the merge and part 2 of every iteration in turn.
+// adaptive loop runs part 1, part 2 and the merge part 2 asks for, in turn.
 template <int N, int D>
 __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, double* slots) {
   const LaneCtx<N, D> c = lane_ctx<N, D>(a, inst, slots);
@@ -1468,11 +1495,8 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
   } else {
     while (s.k < K && s.status == PDQ_STATUS_OK) {
       LaneIterVals<N, D> v;
-      LaneMergeReq rq;
-      lane_adapt_part1<N, D>(c, ts, s, v, rq);
-      if (v.op == LOP_AT) continue;
-      lane_run_merge<N, D>(c.w, rq);
-      lane_adapt_part2<N, D>(c, s, v);
+      lane_adapt_part1<N, D>(c, ts, s, v);
+      lane_run_merge<N, D>(c.w, lane_adapt_part2<N, D>(c, ts, s, v));
     }
   }
   lane_end_instance<N, D>(c, s);
@@ -1510,18 +1534,18 @@ __device__ __forceinline__ LaneMergeReq lane_unpack_req(int v) {
 
 // The fixed-point smoother's adaptive solve on two wavefronts: wavefront 0
 // runs parts 1 and 2 of every iteration for the workgroup's 64 instances
-// (one lane each, as lane_solve_instance); wavefront 1 runs the merges that
-// part 1 asks for while wavefront 0 runs part 2.  The merge writes only the
-// conditional of one slot, which part 2 neither reads nor writes (part 2
-// touches means, factors and scalars, and the conditional of a slot the
-// merge does not write), so the result is that of the serial order.
-// Lane j of both wavefronts works on instance j's LDS column only.
-// Per iteration: part 1 and the request (wave 0) | barrier | merge (wave 1)
-// and part 2 (wave 0), then wave 0 posts whether any of its lanes goes on |
-// barrier.  Both wavefronts read the posted flag after the same barrier, so
-// they pass the same number of barriers and leave the loop together; the
-// request and the flag are rewritten only after the barrier that follows
-// their last read.
+// (one lane each, as lane_solve_instance); wavefront 1 runs the merge that
+// part 2 of an iteration asks for while wavefront 0 runs part 1 of the next
+// iteration, which touches none of the merge's conditionals (see
+// lane_adapt_part2).  Lane j of both wavefronts works on instance j's LDS
+// column only.  Per iteration: [part 1 (wave 0) | the previous request's
+// merge (wave 1)] barrier [part 2, the new request and whether any lane goes
+// on (wave 0)] barrier.  Both wavefronts read the posted flag after the same
+// barrier, so they pass the same number of barriers and leave the loop
+// together; a request and the flag are rewritten only after the barrier that
+// follows their last read.  The loop ends after an interpolation (every lane
+// that reached the end) or a failed step, neither of which leaves a merge
+// the solution needs.
 template <int N, int D>
 __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
   const int l = threadIdx.x & (kLaneBlock - 1);
@@ -1538,18 +1562,18 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
     LaneLoop s = lane_loop_start(a.cfg, inst, mine);
     if (mine) lane_init_instance<N, D>(c, ts, s.sf, s.ifr);
     bool run = mine && s.k < K;
+    req[l] = 0;
     const int any0 = __any(run);
     if (l == 0) *flag = any0;
     __syncthreads();
     while (__builtin_amdgcn_readfirstlane(*flag)) {
       LaneIterVals<N, D> v;
+      if (run) lane_adapt_part1<N, D>(c, ts, s, v);
+      __syncthreads();
       LaneMergeReq rq;
       rq.on = 0;
-      v.op = LOP_AT;
-      if (run) lane_adapt_part1<N, D>(c, ts, s, v, rq);
+      if (run) rq = lane_adapt_part2<N, D>(c, ts, s, v);
       req[l] = lane_pack_req(rq);
-      __syncthreads();
-      if (run && v.op != LOP_AT) lane_adapt_part2<N, D>(c, s, v);
       run = run && s.k < K && s.status == PDQ_STATUS_OK;
       const int any = __any(run);
       if (l == 0) *flag = any;
@@ -1559,8 +1583,8 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
   } else {
     __syncthreads();
     while (__builtin_amdgcn_readfirstlane(*flag)) {
-      __syncthreads();
       lane_run_merge<N, D>(c.w, lane_unpack_req(req[l]));
+      __syncthreads();
       __syncthreads();
     }
   }
