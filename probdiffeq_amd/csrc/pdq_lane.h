@@ -1120,17 +1120,6 @@ __device__ __forceinline__ void lane_interp_beyond(const LaneCtx<N, D>& c, int k
   w.at(to, LS_IDENT) = 0.0;
 }
 
-// ProbabilisticSolver.interpolate_fwd_at_t1 (solvers.py:271-315).
-template <int N, int D>
-__device__ __forceinline__ void lane_interp_at(const LaneCtx<N, D>& c, int k, int from, int to) {
-  lane_write_slot<N, D>(c, k, to);
-  c.w.copy_slot(from, to);
-  if (c.a->cfg.strategy == PDQ_FIXEDPOINT) {
-    c.w.at(to, LS_IDENT) = 1.0;
-    c.w.at(from, LS_IDENT) = 1.0;
-  }
-}
-
 // userfriendly_output + strategy.finalize (solvers.py:439-480, 601-633,
 // 735-767; estimators_and_losses.py:380-391, 437-470).
 template <int N, int D>
@@ -1226,10 +1215,14 @@ struct LaneLoop {
   long attempts, accepted;
 };
 
-// The fixed-point merge an iteration asks for: cond(res) = cond(self) o
-// cond(other), with `ident` when cond(self) is the identity (lane_merge).
+// The work an iteration leaves for after its part 2:
+// * on: the fixed-point merge cond(res) = cond(self) o cond(other), with
+//   `ident` when cond(self) is the identity (lane_merge);
+// * ckpt: checkpoint k from slot `self` (marginal and conditional, the
+//   conditional an identity with `ident`), with the output scale and step
+//   count of slot `other` and the time of slot `self` (own_t) or ts[k].
 struct LaneMergeReq {
-  int on, ident, self, other, res;
+  int on, ident, self, other, res, ckpt, own_t, k;
 };
 
 // What the first part of an iteration hands to the second.
@@ -1241,6 +1234,34 @@ struct LaneIterVals {
 };
 
 enum { LOP_STEP = 0, LOP_BEYOND1 = 1, LOP_BEYOND2 = 2, LOP_AT = 3 };
+
+__device__ __forceinline__ LaneMergeReq lane_no_req() {
+  LaneMergeReq r;
+  r.on = r.ident = r.self = r.other = r.res = r.ckpt = r.own_t = r.k = 0;
+  return r;
+}
+
+// A request packed into two ints (flags and 2-bit slot indices, then k), as
+// the merging wavefront reads it from LDS; the serial schedule takes the
+// same round trip.
+__device__ __forceinline__ void lane_pack_req(const LaneMergeReq& r, int* dst) {
+  dst[0] = (r.on & 1) | ((r.ident & 1) << 1) | ((r.self & 3) << 2) | ((r.other & 3) << 4) | ((r.res & 3) << 6) |
+           ((r.ckpt & 1) << 8) | ((r.own_t & 1) << 9);
+  dst[1] = r.k;
+}
+__device__ __forceinline__ LaneMergeReq lane_unpack_req(const int* src) {
+  const int v = src[0];
+  LaneMergeReq r;
+  r.on = v & 1;
+  r.ident = (v >> 1) & 1;
+  r.self = (v >> 2) & 3;
+  r.other = (v >> 4) & 3;
+  r.res = (v >> 6) & 3;
+  r.ckpt = (v >> 8) & 1;
+  r.own_t = (v >> 9) & 1;
+  r.k = src[1];
+  return r;
+}
 
 // prior init (from_mean_and_std): mean = Taylor coefficients, chol =
 // init_std I, in slot sf; interp_from = a copy; checkpoint 0.
@@ -1375,10 +1396,24 @@ __device__ __forceinline__ void lane_adapt_part1(const LaneCtx<N, D>& c, const d
   PDQ_PHASE_BOUNDARY();
 }
 
-// The requested merge (ssm_impl_isotropic.py:91-105).
+// The work part 2 left: the merge (ssm_impl_isotropic.py:91-105) or the
+// checkpoint.
 template <int N, int D>
-__device__ __forceinline__ void lane_run_merge(const LaneWs<N, D>& w, const LaneMergeReq& rq) {
-  if (rq.on) lane_merge<N, D>(rq.ident != 0, w.cond(rq.self), w.cond(rq.other), w.cond(rq.res));
+__device__ __forceinline__ void lane_run_merge(const LaneCtx<N, D>& c, const double* ts, const LaneMergeReq& rq) {
+  const LaneWs<N, D>& w = c.w;
+  if (rq.on) {
+    lane_merge<N, D>(rq.ident != 0, w.cond(rq.self), w.cond(rq.other), w.cond(rq.res));
+  } else if (rq.ckpt) {
+#ifdef PDQ_HOST_EMU
+    if (rq.k < 0 || rq.k >= c.a->K || rq.self < 0 || rq.self > 2 || rq.other < 0 || rq.other > 2) __builtin_trap();
+#endif
+    double m[N * D], L[N * N];
+    w.load_mL(rq.self, m, L);
+    const CondRef<N, D> cond = w.cond(rq.self);
+    const double t = rq.own_t ? w.at(rq.self, LS_T) : ts[rq.k];
+    lane_write_checkpoint<N, D>(c, rq.k, t, m, L, w.at(rq.other, LS_OS), w.at(rq.other, LS_NSTEPS), &cond,
+                                rq.ident != 0);
+  }
 }
 
 template <int N, int D>
@@ -1389,11 +1424,21 @@ __device__ __forceinline__ LaneMergeReq lane_adapt_part2(const LaneCtx<N, D>& c,
   const bool smoother = cfg.strategy != PDQ_FILTER;
   const bool fp = cfg.strategy == PDQ_FIXEDPOINT;
   const int K = c.a->K;
-  LaneMergeReq rq;
-  rq.on = 0;
-  rq.ident = v.ident;
+  LaneMergeReq rq = lane_no_req();
   if (v.op == LOP_AT) {
-    lane_interp_at<N, D>(c, s.k, s.ifr, s.sf);
+    // interpolate_fwd_at_t1 (solvers.py:271-315): checkpoint k is step_from
+    // (written after part 2), interp_from becomes a copy of it
+    rq.ckpt = 1;
+    rq.own_t = 1;
+    rq.k = s.k;
+    rq.self = s.sf;
+    rq.other = s.sf;
+    rq.ident = w.at(s.sf, LS_IDENT) != 0.0;
+    w.copy_slot(s.ifr, s.sf);
+    if (fp) {
+      w.at(s.sf, LS_IDENT) = 1.0;
+      w.at(s.ifr, LS_IDENT) = 1.0;
+    }
     ++s.k;
   } else if (v.op == LOP_STEP) {
     const double ep = lane_finish_step<N, D>(c, s.sf, s.pr, v.h, v.tn, v.P, v.PI, v.Xm, v.Pm, v.PL, v.b, v.h0,
@@ -1412,6 +1457,7 @@ __device__ __forceinline__ LaneMergeReq lane_adapt_part2(const LaneCtx<N, D>& c,
     } else if (ep >= 1.0) {
       ++s.accepted;
       s.att = 0;
+      rq.ident = v.ident;
       rq.self = s.sf;
       rq.other = s.pr;
       rq.res = s.pr;
@@ -1431,6 +1477,7 @@ __device__ __forceinline__ LaneMergeReq lane_adapt_part2(const LaneCtx<N, D>& c,
     } else {
       w.store_mL(s.ifr, v.Pm, v.PL);  // ext_t's marginal: the new interp_from's
       s.second_half = true;
+      rq.ident = v.ident;
       rq.self = s.ifr;
       rq.other = s.pr;
       rq.res = s.ifr;
@@ -1438,10 +1485,13 @@ __device__ __forceinline__ LaneMergeReq lane_adapt_part2(const LaneCtx<N, D>& c,
     }
   } else {
     // interpolated = (ext_t.marginal, ext_t.conditional), the checkpoint
-    double Em[N * D], EL[N * N];
-    w.load_mL(s.ifr, Em, EL);
-    const CondRef<N, D> ext = w.cond(s.ifr);
-    lane_write_checkpoint<N, D>(c, s.k, v.t1, Em, EL, v.os, w.at(s.sf, LS_NSTEPS), &ext, false);
+    // (written after part 2)
+    rq.ckpt = 1;
+    rq.own_t = 0;
+    rq.k = s.k;
+    rq.self = s.ifr;
+    rq.other = s.sf;
+    rq.ident = 0;
     w.at(s.ifr, LS_T) = v.t1;
     w.at(s.ifr, LS_IDENT) = fp ? 1.0 : 0.0;
     w.at(s.sf, LS_IDENT) = 0.0;
@@ -1496,7 +1546,9 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
     while (s.k < K && s.status == PDQ_STATUS_OK) {
       LaneIterVals<N, D> v;
       lane_adapt_part1<N, D>(c, ts, s, v);
-      lane_run_merge<N, D>(c.w, lane_adapt_part2<N, D>(c, ts, s, v));
+      int packed[2];
+      lane_pack_req(lane_adapt_part2<N, D>(c, ts, s, v), packed);
+      lane_run_merge<N, D>(c, ts, lane_unpack_req(packed));
     }
   }
   lane_end_instance<N, D>(c, s);
@@ -1510,27 +1562,14 @@ constexpr int lane_block_threads() {
 }
 
 // LDS bytes of one workgroup: three slots per lane; with the second
-// wavefront, 64 merge requests and the loop flag after them.
+// wavefront, 64 requests (two ints each) and the loop flag after them.
 template <int N, int D, int STRATEGY = PDQ_FILTER>
 constexpr size_t lane_lds_bytes() {
   return sizeof(double) * kLaneBlock * 3 * SlotLayout<N, D>::size +
-         (STRATEGY == PDQ_FIXEDPOINT ? sizeof(double) * (kLaneBlock / 2 + 1) : 0);
+         (STRATEGY == PDQ_FIXEDPOINT ? sizeof(double) * (kLaneBlock + 1) : 0);
 }
 
 #ifndef PDQ_HOST_EMU
-// Packed merge requests: on, ident, self, other, res (2 bits per slot).
-__device__ __forceinline__ int lane_pack_req(const LaneMergeReq& r) {
-  return r.on | (r.ident << 1) | (r.self << 2) | (r.other << 4) | (r.res << 6);
-}
-__device__ __forceinline__ LaneMergeReq lane_unpack_req(int v) {
-  LaneMergeReq r;
-  r.on = v & 1;
-  r.ident = (v >> 1) & 1;
-  r.self = (v >> 2) & 3;
-  r.other = (v >> 4) & 3;
-  r.res = (v >> 6) & 3;
-  return r;
-}
 
 // The fixed-point smoother's adaptive solve on two wavefronts: wavefront 0
 // runs parts 1 and 2 of every iteration for the workgroup's 64 instances
@@ -1543,16 +1582,16 @@ __device__ __forceinline__ LaneMergeReq lane_unpack_req(int v) {
 // on (wave 0)] barrier.  Both wavefronts read the posted flag after the same
 // barrier, so they pass the same number of barriers and leave the loop
 // together; a request and the flag are rewritten only after the barrier that
-// follows their last read.  The loop ends after an interpolation (every lane
-// that reached the end) or a failed step, neither of which leaves a merge
-// the solution needs.
+// follows their last read.  A request posted in the final iteration is not
+// taken up by wavefront 1 (it leaves the loop); wavefront 0 runs it after
+// the loop.
 template <int N, int D>
 __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
   const int l = threadIdx.x & (kLaneBlock - 1);
   const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / kLaneBlock);  // wave-uniform
   double* const slots = lds + l;
   int* const req = reinterpret_cast<int*>(lds + kLaneBlock * 3 * SlotLayout<N, D>::size);
-  volatile int* const flag = req + kLaneBlock;
+  volatile int* const flag = req + 2 * kLaneBlock;
   const int inst = blockIdx.x * kLaneBlock + l;
   const bool mine = inst < a.n_inst;
   const LaneCtx<N, D> c = lane_ctx<N, D>(a, mine ? inst : 0, slots);
@@ -1562,28 +1601,37 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
     LaneLoop s = lane_loop_start(a.cfg, inst, mine);
     if (mine) lane_init_instance<N, D>(c, ts, s.sf, s.ifr);
     bool run = mine && s.k < K;
-    req[l] = 0;
+    req[2 * l] = 0;
+    req[2 * l + 1] = 0;
     const int any0 = __any(run);
     if (l == 0) *flag = any0;
     __syncthreads();
+    LaneMergeReq last = lane_no_req();
     while (__builtin_amdgcn_readfirstlane(*flag)) {
       LaneIterVals<N, D> v;
       if (run) lane_adapt_part1<N, D>(c, ts, s, v);
       __syncthreads();
-      LaneMergeReq rq;
-      rq.on = 0;
-      if (run) rq = lane_adapt_part2<N, D>(c, ts, s, v);
-      req[l] = lane_pack_req(rq);
+      last = lane_no_req();
+      if (run) last = lane_adapt_part2<N, D>(c, ts, s, v);
+      lane_pack_req(last, req + 2 * l);
       run = run && s.k < K && s.status == PDQ_STATUS_OK;
       const int any = __any(run);
       if (l == 0) *flag = any;
       __syncthreads();
     }
-    if (mine) lane_end_instance<N, D>(c, s);
+    // the merging wavefront has left: the request of the final iteration
+    // (the last checkpoint of the lanes that ended in it) is run here
+    if (mine) {
+      int packed[2];
+      lane_pack_req(last, packed);
+      lane_run_merge<N, D>(c, ts, lane_unpack_req(packed));
+      lane_end_instance<N, D>(c, s);
+    }
   } else {
+    const double* ts = a.times + (size_t)(mine ? inst : 0) * a.K;
     __syncthreads();
     while (__builtin_amdgcn_readfirstlane(*flag)) {
-      lane_run_merge<N, D>(c.w, lane_unpack_req(req[l]));
+      lane_run_merge<N, D>(c, ts, lane_unpack_req(req + 2 * l));
       __syncthreads();
       __syncthreads();
     }
