@@ -79,57 +79,162 @@ __device__ __forceinline__ void lane_lml_linearize(const pdq_config& cfg, const 
   (void)MC;
 }
 
+// What the prediction part of a fixed-grid step hands to its update part.
+template <int N, int MC>
+struct LmlStepVals {
+  double P[N], PI[N], b[MC], Pm[N * MC], PL[N * N], sigma, tn;
+};
+
 // One fixed-grid step of the fixed-point smoother (solver.step /
 // solver_dynamic.step, solvers.py:549-599 and 702-733) from slot `from` to
-// slot `to` with tangent columns.
+// slot `to` with tangent columns, in two parts: (1) calibration and the
+// revert of the transition into cond(to); (2) the update into slot `to`'s
+// marginal.  Between them (or beside part 1 of the next step) the merge
+// cond(to) = cond(from) o cond(to), which neither part reads or writes.
 template <int N, int D, int TPL>
-__device__ __forceinline__ void lane_lml_step(const LaneCtx<N, D * (1 + TPL)>& c, int q0, int from, int to,
-                                              double h) {
+__device__ __forceinline__ void lane_lml_step_part1(const LaneCtx<N, D * (1 + TPL)>& c, int q0, int from, int to,
+                                                    double h, LmlStepVals<N, D * (1 + TPL)>& v) {
   constexpr int MC = D * (1 + TPL);
   const pdq_config& cfg = c.a->cfg;
   const double* __restrict__ Q = c.a->Q;
   const LaneWs<N, MC>& w = c.w;
-  double Sm[N * MC], SLf[N * N], P[N], PI[N], b[MC], Pm[N * MC], PL[N * N];
+  double Sm[N * MC], SLf[N * N];
   w.load_mL(from, Sm, SLf);
-  const double tn = w.at(from, LS_T) + h;
-  lane_precon<N>(h, P, PI);
-  double sigma = 1.0;
+  v.tn = w.at(from, LS_T) + h;
+  lane_precon<N>(h, v.P, v.PI);
+  v.sigma = 1.0;
   if (cfg.solver == PDQ_SOLVER_DYNAMIC) {
     // calibration at the output-scale-1 extrapolation; sigma is a constant
     // for the tangents (stop_gradient)
     const double s1 = lane_noise_scale(cfg, h, 1.0);
     double Um[N * MC], col[N + 1];
-    lane_apply_flat_mean<N, MC>(P, PI, Sm, Um);
-    lane_lml_linearize<N, D, TPL>(cfg, c.prm, q0, Um, tn, b);
+    lane_apply_flat_mean<N, MC>(v.P, v.PI, Sm, Um);
+    lane_lml_linearize<N, D, TPL>(cfg, c.prm, q0, Um, v.tn, v.b);
 #pragma unroll
-    for (int r = 0; r < N; ++r) col[r] = fabs(P[1]) * (s1 * lane_iwp_q<N>(Q, 1, r));
+    for (int r = 0; r < N; ++r) col[r] = fabs(v.P[1]) * (s1 * lane_iwp_q<N>(Q, 1, r));
     col[N] = cfg.damp;
     double om[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) om[k] = Um[1 * MC + k] + b[k];
+    for (int k = 0; k < D; ++k) om[k] = Um[1 * MC + k] + v.b[k];
     lane_qr<N + 1, 1, FxColumnRows<N>>(col);
-    sigma = lane_rms_zero<D>(om, col[0]);
+    v.sigma = lane_rms_zero<D>(om, col[0]);
   }
   PDQ_PHASE_BOUNDARY();
-  lane_predict<N, MC>(c, P, PI, lane_noise_scale(cfg, h, sigma), Sm, SLf, Pm, PL, to, true,
-                      w.at(from, LS_IDENT) != 0.0, from, to);
+  lane_revert_prior<N, MC>(Q, v.P, v.PI, lane_noise_scale(cfg, h, v.sigma), Sm, SLf, v.Pm, v.PL, w.cond(to));
+  PDQ_PHASE_BOUNDARY();
+}
+
+template <int N, int D, int TPL>
+__device__ __forceinline__ void lane_lml_step_part2(const LaneCtx<N, D * (1 + TPL)>& c, int q0, int from, int to,
+                                                    LmlStepVals<N, D * (1 + TPL)>& v) {
+  constexpr int MC = D * (1 + TPL);
+  const pdq_config& cfg = c.a->cfg;
+  const LaneWs<N, MC>& w = c.w;
   if (cfg.solver != PDQ_SOLVER_DYNAMIC || cfg.re_linearize_after_calibration)
-    lane_lml_linearize<N, D, TPL>(cfg, c.prm, q0, Pm, tn, b);
+    lane_lml_linearize<N, D, TPL>(cfg, c.prm, q0, v.Pm, v.tn, v.b);
   double Rm[N * MC], RL[N * N], mo[MC], oLu;
-  lane_update<N, MC>(cfg, b, 0.0, Pm, PL, Rm, RL, mo, oLu);
+  lane_update<N, MC>(cfg, v.b, 0.0, v.Pm, v.PL, Rm, RL, mo, oLu);
   w.store_mL(to, Rm, RL);
-  w.at(to, LS_T) = tn;
-  w.at(to, LS_OS) = sigma;
+  w.at(to, LS_T) = v.tn;
+  w.at(to, LS_OS) = v.sigma;
   w.at(to, LS_NSTEPS) = w.at(from, LS_NSTEPS) + 1.0;
   w.at(to, LS_IDENT) = 0.0;
 }
 
+template <int N, int D, int TPL>
+__device__ __forceinline__ void lane_lml_step(const LaneCtx<N, D * (1 + TPL)>& c, int q0, int from, int to,
+                                              double h) {
+  constexpr int MC = D * (1 + TPL);
+  const LaneWs<N, MC>& w = c.w;
+  LmlStepVals<N, MC> v;
+  lane_lml_step_part1<N, D, TPL>(c, q0, from, to, h, v);
+  lane_merge<N, MC>(w.at(from, LS_IDENT) != 0.0, w.cond(from), w.cond(to), w.cond(to));
+  PDQ_PHASE_BOUNDARY();
+  lane_lml_step_part2<N, D, TPL>(c, q0, from, to, v);
+}
+
+#ifndef PDQ_HOST_EMU
+// The forward pass on two wavefronts (the LML kernel's SPLIT form): wavefront
+// 0 runs the steps' two parts for 64 lanes, wavefront 1 runs each step's
+// merge, and at an observation node the copy of the merged conditional to
+// HBM, beside wavefront 0's next prediction part.  Three slots rotate (the
+// step from `sf` reverts into `pr`; then sf <- pr, pr <- spare, spare <- sf),
+// so the next prediction reads only the marginal of the merge's result slot
+// and reverts into the third slot.  Every lane takes the same K - 1 steps, so
+// both wavefronts pass 2 (K - 1) + 1 barriers; the last step's request is run
+// by wavefront 0 after the loop.  Returns step_from's slot.
+template <int N, int D, int TPL>
+__device__ __forceinline__ int lane_lml_forward_split(const LaneCtx<N, D * (1 + TPL)>& c, const LmlArgs& a, int q0,
+                                                      size_t glane, size_t n_lanes, bool active, int* req) {
+  constexpr int MC = D * (1 + TPL);
+  using CL = CondLayout<N, MC>;
+  const LaneWs<N, MC>& w = c.w;
+  const int K = a.K, every = a.every;
+  const int l = threadIdx.x & (kLaneBlock - 1);
+  const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / kLaneBlock);
+  // request: [0] = on | ident << 1 | self << 2 | res << 4, [1] = node index or -1
+  auto run = [&](const int* r) {
+    const int f = r[0];
+    if (!(f & 1)) return;
+    const int self = (f >> 2) & 3, res = (f >> 4) & 3;
+    lane_merge<N, MC>(((f >> 1) & 1) != 0, w.cond(self), w.cond(res), w.cond(res));
+    if (r[1] >= 0) {
+      const CondStrided<N, MC> dst{a.scratch + (size_t)r[1] * CL::size * n_lanes + glane, n_lanes};
+      cond_copy<N, MC>(dst, w.cond(res));
+    }
+  };
+  int* const mine = req + 2 * l;
+  if (role == 0) {
+    int sf = 0, pr = 1, sp = 2;
+    mine[0] = 0;
+    mine[1] = -1;
+    __syncthreads();
+    int last[2] = {0, -1};
+    for (int k = 1; k < K; ++k) {
+      LmlStepVals<N, MC> v;
+      if (active) lane_lml_step_part1<N, D, TPL>(c, q0, sf, pr, a.times[k] - a.times[k - 1], v);
+      __syncthreads();
+      last[0] = 0;
+      last[1] = -1;
+      if (active) {
+        lane_lml_step_part2<N, D, TPL>(c, q0, sf, pr, v);
+        const int ident = w.at(sf, LS_IDENT) != 0.0;
+        last[0] = 1 | (ident << 1) | (sf << 2) | (pr << 4);
+        last[1] = k % every == 0 ? k / every - 1 : -1;
+        const int old_sf = sf;
+        sf = pr;
+        pr = sp;
+        sp = old_sf;
+        if (k % every == 0) w.at(sf, LS_IDENT) = 1.0;  // restart behind an identity
+      }
+      mine[0] = last[0];
+      mine[1] = last[1];
+      __syncthreads();
+    }
+    run(last);
+    return sf;
+  }
+  __syncthreads();
+  for (int k = 1; k < K; ++k) {
+    run(mine);
+    __syncthreads();
+    __syncthreads();
+  }
+  return 0;
+}
+#endif
+
 // Lane g of instance `inst`: the forward pass, the reverse scan, and its
 // share of the outputs.  `slots` is the lane's LDS column (stride 64);
-// `glane` / `n_lanes` index the lane in the HBM scratch buffer.
-template <int N, int D, int TPL>
+// `glane` / `n_lanes` index the lane in the HBM scratch buffer.  With SPLIT
+// the forward pass runs on two wavefronts (lane_lml_forward_split); `active`
+// tells whether the lane has work (every lane of both wavefronts calls this,
+// the merging wavefront's lanes only for the forward pass), `lds_req` is the
+// workgroup's request array.
+template <int N, int D, int TPL, bool SPLIT = false>
 __device__ __forceinline__ void lane_lml_instance(const LArgs& la, const LmlArgs& a, int inst, int g, double* slots,
-                                                  size_t glane, size_t n_lanes) {
+                                                  size_t glane, size_t n_lanes, bool active = true,
+                                                  int* lds_req = nullptr) {
   constexpr int MC = D * (1 + TPL);
   using CL = CondLayout<N, MC>;
   const pdq_config& cfg = a.cfg;
@@ -142,7 +247,7 @@ __device__ __forceinline__ void lane_lml_instance(const LArgs& la, const LmlArgs
   const int ntan = a.n_tan, q0 = g * TPL;
   const int K = a.K, every = a.every, Kobs = (K - 1) / every + 1;
   int sf = 0, pr = 1;
-  {
+  if (active && !(SPLIT && threadIdx.x >= kLaneBlock)) {
     // prior: mean = Taylor coefficients (with tangents), chol = init_std I
     const double* tc = a.tcoeffs + (size_t)inst * N * D * (1 + ntan);
     double m[N * MC], L[N * N];
@@ -165,18 +270,26 @@ __device__ __forceinline__ void lane_lml_instance(const LArgs& la, const LmlArgs
     w.at(sf, LS_OS) = 1.0;
     w.at(sf, LS_IDENT) = 1.0;
   }
-  for (int k = 1; k < K; ++k) {
-    lane_lml_step<N, D, TPL>(c, q0, sf, pr, a.times[k] - a.times[k - 1]);
-    const int tmp = sf;
-    sf = pr;
-    pr = tmp;
-    if (k % every == 0) {
-      // the reset: store the interval's conditional, restart behind an identity
-      const CondStrided<N, MC> dst{a.scratch + (size_t)(k / every - 1) * CL::size * n_lanes + glane, n_lanes};
-      cond_copy<N, MC>(dst, w.cond(sf));
-      w.at(sf, LS_IDENT) = 1.0;
+  if constexpr (SPLIT) {
+#ifndef PDQ_HOST_EMU
+    sf = lane_lml_forward_split<N, D, TPL>(c, a, q0, glane, n_lanes, active, lds_req);
+    if (threadIdx.x >= kLaneBlock) return;  // the merging wavefront's part ends here
+#endif
+  } else {
+    for (int k = 1; k < K; ++k) {
+      lane_lml_step<N, D, TPL>(c, q0, sf, pr, a.times[k] - a.times[k - 1]);
+      const int tmp = sf;
+      sf = pr;
+      pr = tmp;
+      if (k % every == 0) {
+        // the reset: store the interval's conditional, restart behind an identity
+        const CondStrided<N, MC> dst{a.scratch + (size_t)(k / every - 1) * CL::size * n_lanes + glane, n_lanes};
+        cond_copy<N, MC>(dst, w.cond(sf));
+        w.at(sf, LS_IDENT) = 1.0;
+      }
     }
   }
+  if (!active) return;
   // terminal marginal: posterior1.conditional.marginalise(posterior1.marginal)
   double Xm[N * MC], XL[N * N];
   {
@@ -279,18 +392,28 @@ __device__ __forceinline__ void lane_lml_instance(const LArgs& la, const LmlArgs
     for (int tt = 0; tt < TPL; ++tt) a.grad[(size_t)inst * ntan + q0 + tt] = lp[1 + tt];
 }
 
-// LDS bytes of one 64-lane workgroup: two slots per lane.
-template <int N, int MC>
+// LDS bytes of one workgroup: two slots per lane, or with the merging
+// wavefront (SPLIT) three slots and 64 requests of two ints.
+template <int N, int MC, bool SPLIT = false>
 constexpr size_t lane_lml_lds_bytes() {
-  return sizeof(double) * kLaneBlock * 2 * SlotLayout<N, MC>::size;
+  return sizeof(double) * kLaneBlock * (SPLIT ? 3 : 2) * SlotLayout<N, MC>::size +
+         (SPLIT ? sizeof(double) * kLaneBlock : 0);
+}
+
+// Whether the SPLIT form's LDS fits a workgroup (160 KB on gfx950).
+template <int N, int MC>
+constexpr bool lane_lml_split_fits() {
+  return lane_lml_lds_bytes<N, MC, true>() <= 160 * 1024;
 }
 
 #ifndef PDQ_HOST_EMU
 // One lane per (instance, tangent group), grid-stride over the n_inst G
-// lanes.  The solver is a compile-time constant of the instantiation; the
+// lanes, or with SPLIT one lane of wavefront 0 per (instance, tangent group)
+// and the forward pass's merges on wavefront 1 (lane_lml_forward_split).
+// The solver is a compile-time constant of the instantiation; the
 // linearisation is ts0 (the C ABI refuses ts1 for the LML).
-template <int N, int D, int TPL, int SOLVER>
-__global__ void __launch_bounds__(64) lane_lml_kernel(LmlArgs args) {
+template <int N, int D, int TPL, int SOLVER, bool SPLIT>
+__global__ void __launch_bounds__(SPLIT ? 2 * kLaneBlock : kLaneBlock) lane_lml_kernel(LmlArgs args) {
   extern __shared__ double lds[];
   LmlArgs a = args;
   a.cfg.strategy = PDQ_FIXEDPOINT;
@@ -301,11 +424,21 @@ __global__ void __launch_bounds__(64) lane_lml_kernel(LmlArgs args) {
   la.n_inst = a.n_inst;
   la.K = a.K;
   la.Q = a.Q;
-  double* const slots = lds + threadIdx.x;
   const size_t n_lanes = (size_t)a.n_inst * a.G;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t gl = (size_t)blockIdx.x * blockDim.x + threadIdx.x; gl < n_lanes; gl += stride)
-    lane_lml_instance<N, D, TPL>(la, a, (int)(gl / a.G), (int)(gl % a.G), slots, gl, n_lanes);
+  if constexpr (SPLIT) {
+    constexpr int MC = D * (1 + TPL);
+    const int l = threadIdx.x & (kLaneBlock - 1);
+    const size_t gl = (size_t)blockIdx.x * kLaneBlock + l;
+    const bool active = gl < n_lanes;
+    const size_t g2 = active ? gl : 0;
+    int* const req = reinterpret_cast<int*>(lds + kLaneBlock * 3 * SlotLayout<N, MC>::size);
+    lane_lml_instance<N, D, TPL, true>(la, a, (int)(g2 / a.G), (int)(g2 % a.G), lds + l, gl, n_lanes, active, req);
+  } else {
+    double* const slots = lds + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t gl = (size_t)blockIdx.x * blockDim.x + threadIdx.x; gl < n_lanes; gl += stride)
+      lane_lml_instance<N, D, TPL>(la, a, (int)(gl / a.G), (int)(gl % a.G), slots, gl, n_lanes);
+  }
 }
 #endif
 

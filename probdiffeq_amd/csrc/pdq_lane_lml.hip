@@ -14,8 +14,12 @@ constexpr int kLmlMaxDevices = 64;
 template <int N, int D, int TPL, int S>
 int launch_lml_nd(LmlArgs& a, hipStream_t st) {
   constexpr int MC = D * (1 + TPL);
-  auto kern = lane_lml_kernel<N, D, TPL, S>;
-  const size_t lds = lane_lml_lds_bytes<N, MC>();
+  // the forward pass's merges on a second wavefront where three slots with a
+  // tangent column fit (decided per shape, so that the value-only and the
+  // gradient launches of one shape run the same form)
+  constexpr bool split = lane_lml_split_fits<N, D * 2>();
+  auto kern = lane_lml_kernel<N, D, TPL, S, split>;
+  const size_t lds = lane_lml_lds_bytes<N, MC, split>();
   // LDS attribute and occupancy once per device and instantiation
   static std::atomic<int> cache[kLmlMaxDevices];
   int dev = 0;
@@ -23,15 +27,17 @@ int launch_lml_nd(LmlArgs& a, hipStream_t st) {
   int per_cu = (dev >= 0 && dev < kLmlMaxDevices) ? cache[dev].load() : 0;
   if (per_cu <= 0) {
     PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
+    PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, split ? 128 : 64, lds));
     if (per_cu < 1) return host_error(PDQ_ERR_UNSUPPORTED, "lane LML kernel does not fit (%zu bytes of LDS)", lds);
     if (dev >= 0 && dev < kLmlMaxDevices) cache[dev].store(per_cu);
   }
   const long lanes = (long)a.n_inst * a.G;
   const long need = (lanes + 63) / 64;
-  const long cap = (long)per_cu * num_cus();
+  if (need > 0x7fffffffL) return host_error(PDQ_ERR_UNSUPPORTED, "too many instances");
+  // SPLIT: one lane of wavefront 0 per (instance, tangent group), no stride
+  const long cap = split ? need : (long)per_cu * num_cus();
   const int grid = (int)std::max<long>(1, std::min(need, cap));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, a);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(split ? 128 : 64), lds, st, a);
   PDQ_HIP_TRY(hipGetLastError());
   return PDQ_OK;
 }
