@@ -530,8 +530,9 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
     return set_err(PDQ_ERR_UNSUPPORTED, "the LML needs strategy_smoother_fixedpoint (a Markov sequence)");
   if (cfg->constraint != PDQ_TS0 && grad)
     return set_err(PDQ_ERR_UNSUPPORTED, "gradients need ts0 (the covariance recursion must not depend on them)");
-  if (cfg->prior != PDQ_PRIOR_IWP)
-    return set_err(PDQ_ERR_UNSUPPORTED, "the LML is implemented for the integrated Wiener process prior");
+  // exponential priors (dense only, check_common) run the cooperative LML: their
+  // transition depends on the step and the drift, not on the parameters, so the
+  // forward-mode tangents stay exact
   if (cfg->solver == PDQ_SOLVER_MLE) return set_err(PDQ_ERR_UNSUPPORTED, "solver_mle is not supported by the LML path");
   if (every < 1 || (K_grid - 1) % every != 0)
     return set_err(PDQ_ERR_INVALID, "every (%d) must divide K_grid - 1 (%d)", every, K_grid - 1);

@@ -11,6 +11,7 @@ gradient.  There is no CPU path.
 from __future__ import annotations
 
 import dataclasses
+from typing import Any
 
 import torch
 
@@ -38,6 +39,10 @@ class LMLProblem:
     lanes_per_instance: int = 0
     # the gradient: "forward" (tangents) or "reverse" (the adjoint with checkpoints)
     lml_gradient: str = "forward"
+    # "iwp" or "exponential" (dense): prior_drift is the drift's bottom block, a device tensor
+    # (d, (num_derivatives + 1) d), as in capi.Config
+    prior: str = "iwp"
+    prior_drift: Any = None
 
     def config(self, n_params: int) -> capi.Config:
         return capi.Config(
@@ -52,6 +57,8 @@ class LMLProblem:
             init_std=self.init_std,
             lanes_per_instance=self.lanes_per_instance,
             lml_gradient=self.lml_gradient,
+            prior=self.prior,
+            prior_drift=self.prior_drift,
         )
 
 

@@ -283,3 +283,25 @@ def test_mirror_loss_lml_timeseries_solve_choices(pdq):
         assert torch.equal(v.detach(), vr)
     with pytest.raises(NotImplementedError, match="solve_triu"):
         pd.loss_lml_timeseries(solve_triu=lambda a, b: b)
+
+
+@pytest.mark.parametrize("solver", ["solver", "dynamic"])
+def test_lml_with_an_exponential_prior(oracle, pdq, solver):
+    """loss_lml_timeseries on a solve with a Matern prior (prior_matern, length scale 0.3; the
+    dense exponential transition every step), through the C ABI and the torch op: value against
+    the oracle, gradient against Richardson FD to 1e-10."""
+    import bench
+    from probdiffeq_amd.torch_ops import LMLProblem, lml
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(3, seed=6, t_end=5.0, n_grid=101, every=5, per_coordinate_std=True)
+    drift = _dev(bench.matern_drift(0.3, 4, 2))
+    cfg = pdq.Config(factorisation="dense", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver=solver,
+                     prior="exponential", prior_drift=drift)
+    v, g = _check(oracle, pdq, cfg, params, y0, grid, every, data, stds)
+    problem = LMLProblem(vector_field="lotka_volterra", num_derivatives=3, ode_dim=2, grid=_dev(grid), every=every,
+                         factorisation="dense", solver=solver, prior="exponential", prior_drift=drift)
+    P = _dev(params).requires_grad_(True)
+    vt = lml(problem, _dev(data), _dev(stds), params=P, y0=_dev(y0))
+    vt.sum().backward()
+    assert torch.equal(vt.detach().cpu(), torch.as_tensor(v))
+    assert torch.equal(P.grad.cpu(), torch.as_tensor(g[:, :4]))

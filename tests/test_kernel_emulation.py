@@ -242,6 +242,31 @@ def test_lml_reverse_adjoint(oracle, emu, nu, solver, average):
         assert LC.rel(g_r[i], gr) < LC.GRAD_RTOL, (g_r[i], gr)
 
 
+@pytest.mark.parametrize("solver", ["solver", "dynamic"])
+def test_lml_with_an_exponential_prior(oracle, emu, solver):
+    """loss_lml_timeseries on a solve with a Matern prior (prior_matern, ssm_impl_api.py:443-477;
+    the dense exponential transition every step) in the cooperative kernel's code: value against
+    the oracle, forward-mode gradient against Richardson FD (the transition depends on the step
+    and the drift, not on the parameters)."""
+    import bench
+    import lml_cases as LC
+    from probdiffeq_amd import capi
+
+    params, y0, grid, every, data, stds = LC.lml_inputs(2, seed=6, t_end=5.0, n_grid=101, every=5, per_coordinate_std=True)
+    drift = bench.matern_drift(0.3, 4, 2)
+    cfg = capi.Config(factorisation="dense", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver=solver,
+                      prior="exponential", prior_drift=drift)
+    ocfg = T._oracle_cfg(oracle, cfg)
+    v, g = emu.lml(cfg, params, y0, grid, every, data, stds)
+    v_iwp, _ = emu.lml(capi.Config(factorisation="dense", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver=solver), params, y0, grid, every, data, stds, grad=False)
+    assert np.all(np.abs(v - v_iwp) > 1e-3 * np.abs(v_iwp))  # another prior: another likelihood
+    for i in range(2):
+        vo, vf, vt = LC.oracle_value(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i])
+        T.check_accuracy(v[i], vo, vf, vt, f"emulation lml matern {solver} instance {i}")
+        gr = LC.richardson_grad(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i])
+        assert LC.rel(g[i], gr) < LC.GRAD_RTOL, (g[i], gr)
+
+
 def test_lml_gain_solves_with_a_dropped_singular_value(oracle, emu):
     """loss_lml_timeseries(solve_triu=linalg.lstsq_svd / linalg.solve_triu) in the cooperative
     kernel's code (dense, two observed rows per node).  Where R_Y's small singular value falls below
