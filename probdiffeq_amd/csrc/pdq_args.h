@@ -160,8 +160,13 @@ size_t coop_lds_bytes(const Layout& L, int G);
 
 }  // namespace pdq
 
-#define PDQ_HIP_TRY(expr)                                                                               \
-  do {                                                                                                  \
-    hipError_t e_ = (expr);                                                                             \
-    if (e_ != hipSuccess) return ::pdq::host_error(PDQ_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+// A failed runtime call is reported, and the thread's last HIP error is
+// cleared so that the caller's next HIP call does not pick it up.
+#define PDQ_HIP_TRY(expr)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      (void)hipGetLastError();                                                             \
+      return ::pdq::host_error(PDQ_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));       \
+    }                                                                                      \
   } while (0)

@@ -61,10 +61,15 @@ int set_err(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                       \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess) return set_err(PDQ_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+// A failed runtime call is reported, and the thread's last HIP error is
+// cleared so that the caller's next HIP call does not pick it up.
+#define HIP_TRY(expr)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      (void)hipGetLastError();                                                   \
+      return set_err(PDQ_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));       \
+    }                                                                            \
   } while (0)
 
 struct DevMats {

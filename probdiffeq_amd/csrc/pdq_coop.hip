@@ -1851,7 +1851,13 @@ template <int G>
 int launch_coop_g(KArgs& a, hipStream_t st) {
   const size_t lds = coop_lds_bytes(a.lay, G);
   auto kern = solve_kernel<G>;
-  PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return host_error(PDQ_ERR_UNSUPPORTED,
+                      "lanes_per_instance = %d puts %d instances and %zu bytes of LDS in a workgroup, more than the "
+                      "device allows; take more lanes per instance",
+                      G, 64 / G, lds);
+  }
   int per_cu = 0;
   PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
   if (per_cu < 1) return host_error(PDQ_ERR_UNSUPPORTED, "instance does not fit into LDS (%zu bytes)", lds);
@@ -1869,7 +1875,10 @@ template <int G>
 int launch_offgrid_coop_g(OffgridCoopArgs& o, hipStream_t st) {
   const size_t lds = coop_lds_bytes(o.k.lay, G);
   auto kern = offgrid_coop_kernel<G>;
-  PDQ_HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return host_error(PDQ_ERR_UNSUPPORTED, "%d queries of %zu bytes of LDS do not fit a workgroup", 64 / G, lds);
+  }
   int per_cu = 0;
   PDQ_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
   if (per_cu < 1) return host_error(PDQ_ERR_UNSUPPORTED, "a query does not fit into LDS (%zu bytes)", lds);

@@ -291,6 +291,22 @@ def test_blockdiag_limits_are_refused(pdq):
             pdq.solve(cfg, _dev(p), _dev(tc), _dev(np.array([[0.0, 1.0]])))
 
 
+def test_oversized_lane_groups_are_refused_and_leave_no_hip_error(oracle, pdq):
+    """lanes_per_instance = 4 puts 16 dense nu = 4 instances into one workgroup, more LDS than the
+    device allows: the C ABI refuses it as unsupported (no launch), clears the runtime's last error,
+    and the next device work (a torch allocation, the same solve with 16 lanes) runs normally."""
+    params, y0s, times = _lv_batch(2, seed=3, t_lo=3.0, t_hi=5.0, K=6)
+    cfg = pdq.Config(factorisation="dense", num_derivatives=4, strategy="filter", solver="dynamic", lanes_per_instance=4)
+    tcs = np.stack([oracle.taylor_coefficients("lotka_volterra", params[i], y0s[i], 0.0, 4) for i in range(2)])
+    print("refusal case: dense nu=4 with 4 lanes per instance", flush=True)
+    with pytest.raises(pdq.PdqError, match="LDS"):
+        pdq.solve(cfg, _dev(params), _dev(tcs), _dev(times))
+    z = torch.zeros(8, dtype=torch.float64, device="cuda") + 1.0
+    torch.cuda.synchronize()
+    assert float(z.sum()) == 8.0
+    _compare(oracle, pdq, pdq.Config(factorisation="dense", num_derivatives=4, strategy="filter", solver="dynamic", lanes_per_instance=16), params, y0s, times)
+
+
 @pytest.mark.parametrize(
     "kw",
     [
