@@ -48,6 +48,15 @@ enum { PDQ_PRIOR_IWP = 0, PDQ_PRIOR_EXPONENTIAL = 1 };
 enum { PDQ_GRAD_FORWARD = 0, PDQ_GRAD_REVERSE = 1 };
 /* LML gain solves (pdq_config.lml_solve) */
 enum { PDQ_LML_SOLVE_LSTSQ = 0, PDQ_LML_SOLVE_TRIU = 1 };
+/* Jacobian handlers of the ts1 linearisation (pdq_config.jacobian): the exact
+ * trace / diagonal (jacobian_materialize, jacobians.py:78-113), the
+ * reverse-mode Monte Carlo estimate (jacobian_monte_carlo_rev, :195-271, the
+ * reference's default for ode()) or the forward-mode one
+ * (jacobian_monte_carlo_fwd, :116-192).  The two estimates of the trace are
+ * the same v^T J v; of the diagonal, the reverse mode takes (v^T J)_k v_k and
+ * the forward mode (J v)_k v_k.  The dense factorisation always takes the
+ * exact Jacobian (materialize_dense). */
+enum { PDQ_JAC_EXACT = 0, PDQ_JAC_MONTE_CARLO_REV = 1, PDQ_JAC_MONTE_CARLO_FWD = 2 };
 /* device vector-field registry (problems.py:283-295 convention u' = f(u, t)) */
 enum {
   PDQ_VF_LOTKA_VOLTERRA = 0, /* params (a, b, c, d), d = 2 */
@@ -139,6 +148,18 @@ typedef struct pdq_config {
    * (linalg.solve_triu).  With one observed row per node (isotropic) the two differ only for an
    * exactly zero factor, where the log-density is not finite either way. */
   int32_t lml_solve;
+  /* ts1 Jacobian handler (PDQ_JAC_*), and for the Monte Carlo handlers the number of Rademacher
+   * probes (1..64) and the seed of their stream.  Each probe entry is a sign drawn by a
+   * counter-based generator from (seed, set, probe, coordinate), with `set` the number of
+   * accepted steps before the linearisation (twice that for solver_dynamic with
+   * re_linearize_after_calibration, whose second linearisation takes set + 1): a rejected
+   * attempt's retry reuses its probes, as the reference's PRNG key, carried in the solution, is
+   * discarded with the rejected proposal.  error_re_linearize is refused with these handlers
+   * (the reference draws that linearisation's probes from the error estimator's own key).  The stream is this library's counter-based
+   * generator, not JAX's threefry: values follow the same estimator, not the same draws. */
+  int32_t jacobian;
+  int32_t jac_probes;
+  uint32_t jac_seed;
 } pdq_config;
 
 /* Caller-allocated device outputs; NULL members are not written.

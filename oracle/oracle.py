@@ -40,6 +40,7 @@ STRATEGY = {"filter": 0, "fixedpoint": 1, "fixedinterval": 2}
 SOLVER = {"solver": 0, "dynamic": 1, "mle": 2}
 ERROR_NORM = {"scale_then_rms": 0, "rms_then_scale": 1}
 CONTROL = {"integral": 0, "proportional_integral": 1}
+JACOBIAN = {"exact": 0, "monte_carlo_rev": 1, "monte_carlo_fwd": 2}
 VF = {"lotka_volterra": 0, "van_der_pol": 1, "burgers": 2, "linear": 3, "logistic": 4}
 
 
@@ -79,6 +80,9 @@ def _cfg_struct(ct):
             ("tcoeff_index", ctypes.c_int),
             ("init_std_rows", ctypes.POINTER(ct)),
             ("lml_solve", ctypes.c_int),
+            ("jacobian", ctypes.c_int),
+            ("jac_probes", ctypes.c_int),
+            ("jac_seed", ctypes.c_uint32),
         ]
 
     return _Cfg
@@ -175,6 +179,11 @@ class Config:
     init_std_rows: object = None
     # loss_lml_timeseries(solve_triu=): "lstsq_svd" (the default) or "solve_triu"
     lml_solve: str = "lstsq_svd"
+    # ts1 Jacobian handler: "exact" (jacobian_materialize), "monte_carlo_rev" or
+    # "monte_carlo_fwd" (jacobian_monte_carlo_rev / _fwd(seed=jac_seed, num_probes=jac_probes))
+    jacobian: str = "exact"
+    jac_probes: int = 10
+    jac_seed: int = 1
 
     def c(self):
         if self.prior not in ("iwp", "exponential"):
@@ -192,6 +201,11 @@ class Config:
         if self.lml_solve not in ("lstsq_svd", "solve_triu"):
             raise ValueError(f"unknown lml_solve {self.lml_solve!r}")
         c.lml_solve = 1 if self.lml_solve == "solve_triu" else 0
+        if self.jacobian not in JACOBIAN:
+            raise ValueError(f"unknown jacobian {self.jacobian!r}")
+        if self.jacobian != "exact" and not 1 <= self.jac_probes <= 64:
+            raise ValueError("jac_probes must lie in [1, 64]")
+        c.jacobian, c.jac_probes, c.jac_seed = JACOBIAN[self.jacobian], int(self.jac_probes), int(self.jac_seed) & 0xFFFFFFFF
         if self.init_std_rows is not None:
             rows = np.ascontiguousarray(np.asarray(self.init_std_rows, dtype=np.float64), dtype=_dtype())
             if rows.shape != (self.n,):
@@ -307,6 +321,8 @@ def lib():
         L.pdqo_step_error_power.restype = ct
         L.pdqo_logpdf.restype = ct
         L.pdqo_control.restype = ct
+        L.pdqo_rademacher.restype = ct
+        L.pdqo_rademacher.argtypes = [ctypes.c_uint32] * 4
         L.pdqo_solve_save_at_batch.restype = ctypes.c_int
         _libs[_variant] = L
     return L
@@ -416,6 +432,23 @@ def vector_field(vf, params, y, t=0.0):
     f, J = _zeros(d), _zeros((d, d))
     lib().pdqo_vf(VF[vf], d, _p(params), _p(y), _s(t), _p(f), _p(J))
     return f, J
+
+
+def rademacher(seed, set_, probe, coord):
+    """One entry of the Monte Carlo handlers' probe stream (+1.0 or -1.0)."""
+    return float(lib().pdqo_rademacher(seed & 0xFFFFFFFF, set_ & 0xFFFFFFFF, probe, coord))
+
+
+def mc_jacobian_diagonal(vf, params, y, *, seed=1, set_=0, probes=10, mode="rev", t=0.0):
+    """The Monte Carlo estimate of the vector field's Jacobian diagonal at y (calculate_diagonal_along_d)."""
+    params, y = _f64(params), _f64(y)
+    d = y.size
+    out = _zeros(d)
+    if mode not in ("rev", "fwd"):
+        raise ValueError(f"unknown mode {mode!r}")
+    lib().pdqo_mc_diag(VF[vf], d, _p(params), _p(y), _s(t), ctypes.c_uint32(seed & 0xFFFFFFFF),
+                       ctypes.c_uint32(set_ & 0xFFFFFFFF), int(probes), int(mode == "fwd"), _p(out))
+    return out
 
 
 def _alloc(cfg: Config, K: int):

@@ -32,6 +32,12 @@
 //       closed forms, Gaussian log-density (tests/test_oracle_offgrid.py)
 //   posterior samples -- smoothing means and covariances through the
 //       sampler's affine dependence on its draws (tests/test_oracle_sample.py)
+//   Monte Carlo Jacobian handlers -- the estimator against numpy's v^T J
+//       products on the same probes, exact for diagonal Jacobians, the
+//       probes' balance, the limit of many probes, and rejected attempts
+//       reusing their probes (tests/test_oracle_jacobian.py); the probe
+//       stream is this project's generator, not JAX's threefry, so the
+//       reference's draws are not reproduced
 // ===========================================================================
 #include <algorithm>
 #include <cmath>
@@ -676,6 +682,11 @@ struct pdqo_cfg {
   // loss_lml_timeseries(solve_triu=): 0 linalg.lstsq_svd (the default), 1
   // linalg.solve_triu
   int lml_solve;
+  // ts1 Jacobian handler: 0 jacobian_materialize (exact), 1
+  // jacobian_monte_carlo_rev, 2 jacobian_monte_carlo_fwd; the number of
+  // Rademacher probes and the seed of their stream
+  int jacobian, jac_probes;
+  uint32_t jac_seed;
 };
 }
 
@@ -1012,11 +1023,58 @@ Vec coeff_of(const Problem& P, const BNormal& rv, int i) {
   return u;
 }
 
+// ------------------------------------------- Monte Carlo Jacobian handlers
+enum { JAC_EXACT = 0, JAC_MC_REV = 1, JAC_MC_FWD = 2 };
+
+// One Rademacher entry: the sign of the top bit of the SplitMix64 finaliser
+// (Steele, Lea and Flood, OOPSLA 2014) of a counter mixing the seed, the
+// probe set, the probe and the coordinate.  The reference draws its probes
+// with random.rademacher from a threefry key split at every linearisation
+// (jacobians.py:236-238, 255-257); this stream replaces that key chain with
+// one probe set per linearisation.
+Real rademacher(uint32_t seed, uint32_t set, uint32_t probe, uint32_t coord) {
+  uint64_t z = (uint64_t)seed * 0x9E3779B97F4A7C15ull + (uint64_t)set * 0xD1B54A32D192ED03ull +
+               (uint64_t)probe * 0x8CB92BA72F3D8DD7ull + (uint64_t)coord * 0xABC98388FB8FAC03ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (z >> 63) ? -1.0 : 1.0;
+}
+
+// calculate_diagonal_along_d for one output row (jacobians.py:250-271 in
+// reverse mode: vjp(v) * v; :171-192 in forward mode: jvp(v) * v), averaged
+// over the probes, with the dense Jacobian Jf (row-major d x d).  Summed
+// over k it is calculate_trace_along_d (:230-248, :151-169).
+Real mc_diag(const Real* Jf, int d, int k, uint32_t seed, uint32_t set, int probes, bool fwd) {
+  Real acc = 0.0;
+  for (int s = 0; s < probes; ++s) {
+    Real w = 0.0;  // (v^T J)_k or (J v)_k
+    for (int j = 0; j < d; ++j) w += rademacher(seed, set, (uint32_t)s, (uint32_t)j) * (fwd ? Jf[k * d + j] : Jf[j * d + k]);
+    acc += w * rademacher(seed, set, (uint32_t)s, (uint32_t)k);
+  }
+  return acc / (Real)probes;
+}
+
+// The Jacobian diagonal entry k that the configured handler reports.
+Real jac_diag(const pdqo_cfg& c, const Real* Jf, int d, int k, uint32_t set) {
+  if (c.jacobian == JAC_EXACT) return Jf[k * d + k];
+  return mc_diag(Jf, d, k, c.jac_seed, set, c.jac_probes, c.jacobian == JAC_MC_FWD);
+}
+
+// Probe sets a step uses: solver_dynamic with re_linearize_after_calibration
+// linearises twice (solvers.py:561-582), every other step once.
+uint32_t lin_sets_per_step(const pdqo_cfg& c) {
+  return (c.solver == SOLVER_DYNAMIC && c.re_linearize) ? 2u : 1u;
+}
+
 // Linearisations.  ts0: *OdeTs0.linearize (ssm_impl_dense.py:237-259,
 // ssm_impl_isotropic.py:298-317, ssm_impl_blockdiag.py:123-144).
 // ts1: DenseResidual.linearize (ssm_impl_dense.py:262-334) with
 // residual_from_ode (problems.py:518-532) and the prior-mean Taylor point.
-BCond linearize(const Problem& P, const BNormal& rv, Real t) {
+// `set` is the probe set of the Monte Carlo handlers (isotropic traces and
+// blockdiag diagonals; the dense factorisation materialises the Jacobian,
+// jacobians.py:221-228, and draws none).
+BCond linearize(const Problem& P, const BNormal& rv, Real t, uint32_t set = 0) {
   const int d = P.d, N = P.N;
   Vec y0 = u0_of(P, rv);
   Vec f(d);
@@ -1065,7 +1123,7 @@ BCond linearize(const Problem& P, const BNormal& rv, Real t) {
       Vec Jf(d * d);
       P.vf.jac(y0.data(), t, Jf.data());
       Real tr = 0.0;
-      for (int k = 0; k < d; ++k) tr += -Jf[k * d + k];
+      for (int k = 0; k < d; ++k) tr += -jac_diag(P.cfg, Jf.data(), d, k, set);
       C.A(0, 0) = tr / (Real)d;
       C.A(0, 1) = (Real)d / (Real)d;
       const Mat& m = rv.b[0].m;
@@ -1098,7 +1156,7 @@ BCond linearize(const Problem& P, const BNormal& rv, Real t) {
         C.A(0, 1) = 1.0;
         C.noise.m(0, 0) = -f[k];
       } else {
-        C.A(0, 0) = -Jf[(size_t)k * d + k];
+        C.A(0, 0) = -jac_diag(P.cfg, Jf.data(), d, k, set);
         C.A(0, 1) = 1.0;
         const Mat& m = rv.b[k].m;  // (N, 1)
         const Real r = m(1, 0) - f[k];
@@ -1144,6 +1202,7 @@ struct Sol {
   Vec os;
   long num_steps = 0;
   BCond fx;             // fun_evals (cached linearisation)
+  uint32_t lin_set = 0;  // the Monte Carlo handlers' next probe set (the auxiliary key)
   Vec mle_running;      // solver_mle auxiliary
   Real mle_num_data = 0;
 };
@@ -1224,7 +1283,7 @@ Sol solver_step(const Problem& P, const Sol& st, Real dt) {
   if (P.cfg.solver == SOLVER_DYNAMIC) {
     BCond tr = transition(P, dt, prototype_os(P));
     BNormal u = apply_flat(P, tr, st.u);
-    BCond fx = linearize(P, u, tn);
+    BCond fx = linearize(P, u, tn, st.lin_set);
     BNormal observed = marginalise(P, fx, u);
     Vec os = residual_whitened_rms_zero(P, observed);
     if (g_tape_mode != TAPE_OFF) {
@@ -1241,7 +1300,7 @@ Sol solver_step(const Problem& P, const Sol& st, Real dt) {
     predict(P, st.post, tr2, upred, pred);
     // solvers.py:576-582: optionally linearise again at the calibrated
     // prediction; the step still reports the initial linearisation (:597)
-    const BCond fx_upd = P.cfg.re_linearize ? linearize(P, upred, tn) : fx;
+    const BCond fx_upd = P.cfg.re_linearize ? linearize(P, upred, tn, st.lin_set + 1) : fx;
     BNormal upd = bayes_rule_zero(P, fx_upd, upred, nullptr);
     out.u = upd;
     out.post.marg = upd;
@@ -1254,7 +1313,7 @@ Sol solver_step(const Problem& P, const Sol& st, Real dt) {
     BNormal upred;
     Post pred;
     predict(P, st.post, tr, upred, pred);
-    BCond fx = linearize(P, upred, tn);
+    BCond fx = linearize(P, upred, tn, st.lin_set);
     BNormal observed;
     BNormal upd = bayes_rule_zero(P, fx, upred, &observed);
     out.u = upd;
@@ -1277,6 +1336,7 @@ Sol solver_step(const Problem& P, const Sol& st, Real dt) {
   }
   out.t = tn;
   out.num_steps = st.num_steps + 1;
+  out.lin_set = st.lin_set + lin_sets_per_step(P.cfg);
   return out;
 }
 
@@ -2032,6 +2092,23 @@ void pdqo_vf(int vf_id, int d, const Real* params, const Real* y, Real t, Real* 
   vf.prm = params;
   if (f) vf.eval(y, t, f);
   if (J) vf.jac(y, t, J);
+}
+
+// The Monte Carlo handlers' pieces: one probe entry, and the diagonal
+// estimates of the vector field's Jacobian at y (out: d entries).
+Real pdqo_rademacher(uint32_t seed, uint32_t set, uint32_t probe, uint32_t coord) {
+  return rademacher(seed, set, probe, coord);
+}
+
+void pdqo_mc_diag(int vf_id, int d, const Real* params, const Real* y, Real t, uint32_t seed, uint32_t set,
+                  int probes, int fwd, Real* out) {
+  Vf vf;
+  vf.id = vf_id;
+  vf.d = d;
+  vf.prm = params;
+  std::vector<Real> J((size_t)d * d);
+  vf.jac(y, t, J.data());
+  for (int k = 0; k < d; ++k) out[k] = mc_diag(J.data(), d, k, seed, set, probes, fwd != 0);
 }
 
 static void write_outputs(const Problem& P, const Outputs& o, const Final& F, pdqo_out* out) {

@@ -28,6 +28,7 @@ ERROR_ESTIMATOR = {"residual_std": 0, "state_std": 1}
 PRIOR = {"iwp": 0, "exponential": 1}
 LML_GRADIENT = {"forward": 0, "reverse": 1}
 LML_SOLVE = {"lstsq_svd": 0, "solve_triu": 1}
+JACOBIAN = {"exact": 0, "monte_carlo_rev": 1, "monte_carlo_fwd": 2}
 VF = {"lotka_volterra": 0, "van_der_pol": 1, "burgers": 2, "linear": 3, "logistic": 4}
 STATUS = {0: "ok", 1: "max_attempts", 2: "nonfinite", 3: "dt_underflow"}
 
@@ -78,6 +79,9 @@ class _Config(ctypes.Structure):
         ("init_std_rows", ctypes.c_void_p),
         ("lml_gradient", ctypes.c_int32),
         ("lml_solve", ctypes.c_int32),
+        ("jacobian", ctypes.c_int32),
+        ("jac_probes", ctypes.c_int32),
+        ("jac_seed", ctypes.c_uint32),
     ]
 
 
@@ -160,6 +164,12 @@ class Config:
     lml_gradient: str = "forward"
     # loss_lml_timeseries(solve_triu=): "lstsq_svd" (the reference's default) or "solve_triu"
     lml_solve: str = "lstsq_svd"
+    # ts1 Jacobian handler: "exact" (jacobian_materialize) or "monte_carlo_rev"
+    # or "monte_carlo_fwd" (jacobian_monte_carlo_rev / _fwd(seed=jac_seed, num_probes=jac_probes));
+    # isotropic and blockdiag only (the dense factorisation takes the exact Jacobian)
+    jacobian: str = "exact"
+    jac_probes: int = 10
+    jac_seed: int = 1
 
     def c(self, host: bool = False) -> _Config:
         """The C struct.  The kernels dereference init_std_rows, prior_drift and
@@ -174,6 +184,11 @@ class Config:
         if self.lml_solve not in LML_SOLVE:
             raise ValueError(f"unknown lml_solve {self.lml_solve!r}")
         c.lml_solve = LML_SOLVE[self.lml_solve]
+        if self.jacobian not in JACOBIAN:
+            raise ValueError(f"unknown jacobian {self.jacobian!r}")
+        c.jacobian = JACOBIAN[self.jacobian]
+        c.jac_probes = int(self.jac_probes)
+        c.jac_seed = int(self.jac_seed) & 0xFFFFFFFF
         n_rows = (self.num_derivatives + 1) * (self.ode_dim if self.factorisation == "dense" else 1)
         if self.prior not in PRIOR:
             raise ValueError(f"unknown prior {self.prior!r}")

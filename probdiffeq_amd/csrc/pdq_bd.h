@@ -168,16 +168,20 @@ struct BdCtx {
   BdGroup g;
   int inst;
   bool act;  // lane < d (padding lanes write nothing)
+  uint32_t lin_set;  // the step's probe set of the Monte Carlo Jacobian handlers (pdq_lin_set)
 };
 
 // The linearisation at the block means Xm (ts0: BlockDiagOdeTs0.linearize,
 // ssm_impl_blockdiag.py:123-144; ts1: BlockDiagResidual.linearize, :147-183,
 // with the exact Jacobian diagonal of jacobian_materialize, jacobians.py:
 // 108-113): b = -f_k(u) (ts0) or r_k(m) - [h0, 1] m with h0 = -J_f[k][k]
-// (ts1).  Every lane of the group calls it (it synchronises twice).
+// (ts1).  With the Monte Carlo handlers (calculate_diagonal_along_d of
+// jacobian_monte_carlo_rev and _fwd, jacobians.py:250-271, 171-192) h0 is minus
+// the probe mean of (v^T J)_k v_k or (J v)_k v_k over the probe set c.lin_set.  Every lane of the group calls it (it
+// synchronises twice).
 template <int N, int STRATEGY>
 __device__ __forceinline__ void bd_linearize(const BdCtx<N, STRATEGY>& c, const double* Xm, double t, double& b,
-                                             double& h0) {
+                                             double& h0, uint32_t set_offset = 0) {
   const pdq_config& cfg = c.a->cfg;
   const int d = cfg.ode_dim, k = c.g.lane;
   if (c.act) c.g.y[k] = Xm[0];
@@ -185,7 +189,11 @@ __device__ __forceinline__ void bd_linearize(const BdCtx<N, STRATEGY>& c, const 
   double f = 0.0, jkk = 0.0;
   if (c.act) {
     f = vf_component(cfg.vector_field, d, c.g.y, c.prm, t, k);
-    if (cfg.constraint == PDQ_TS1) jkk = vf_jac(cfg.vector_field, d, c.g.y, c.prm, k, k);
+    if (cfg.constraint == PDQ_TS1)
+      jkk = cfg.jacobian != PDQ_JAC_EXACT
+                ? vf_mc_diag(cfg.vector_field, d, c.g.y, c.prm, k, cfg.jac_seed, c.lin_set + set_offset, cfg.jac_probes,
+                             cfg.jacobian == PDQ_JAC_MONTE_CARLO_FWD)
+                : vf_jac(cfg.vector_field, d, c.g.y, c.prm, k, k);
   }
   c.g.sync();  // every lane has read y before it is written again
   if (cfg.constraint != PDQ_TS1) {
@@ -419,7 +427,7 @@ __device__ __forceinline__ double bd_step(const BdCtx<N, STRATEGY>& c, const BdS
   double Rm[N], RL[N * N], mo, oLu;
   if (cfg.solver == PDQ_SOLVER_DYNAMIC) {
     double bu = b, hu = h0;
-    if (cfg.re_linearize_after_calibration) bd_linearize<N, STRATEGY>(c, Pm, tn, bu, hu);
+    if (cfg.re_linearize_after_calibration) bd_linearize<N, STRATEGY>(c, Pm, tn, bu, hu, 1);
     lane_update<N, 1>(cfg, &bu, hu, Pm, PL, Rm, RL, &mo, oLu);
     pr.mle_run = 0.0;
     zp.mle_n = 0.0;
@@ -615,6 +623,7 @@ __device__ __forceinline__ void bd_solve_instance(const BdArgs& a, const BdGroup
   c.g = g;
   c.inst = inst;
   c.act = blk < d;
+  c.lin_set = 0;
   const double* ts = a.times + (size_t)inst * K;
   BdSlot<N> sf, fr, pr;
   BdScal zs, zf, zp;
@@ -645,6 +654,7 @@ __device__ __forceinline__ void bd_solve_instance(const BdArgs& a, const BdGroup
   int status = PDQ_STATUS_OK;
   if (a.mode == LMODE_FIXED_GRID) {
     for (int k = 1; k < K; ++k) {
+      c.lin_set = pdq_lin_set(cfg, k - 1);
       bd_step<N, STRATEGY>(c, sf, zs, pr, zp, ts[k] - ts[k - 1], false);
       ++attempts;
       ++accepted;
@@ -661,6 +671,7 @@ __device__ __forceinline__ void bd_solve_instance(const BdArgs& a, const BdGroup
         // one step attempt (RejectionLoop.step_attempt)
         double h = dt;
         if (cfg.clip_dt) h = fmin(h, t1 - zs.t);
+        c.lin_set = pdq_lin_set(cfg, accepted);
         const double ep = bd_step<N, STRATEGY>(c, sf, zs, pr, zp, h, true);
         dt = lane_control(cfg, h, cstate, ep);
         ++att;

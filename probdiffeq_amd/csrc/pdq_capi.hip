@@ -200,6 +200,19 @@ int check_common(const pdq_config* cfg, int n_inst, int K, bool device = true) {
       return set_err(PDQ_ERR_UNSUPPORTED, "the exponential prior supports (nu + 1) d <= 16 (got %d)",
                      (cfg->num_derivatives + 1) * cfg->ode_dim);
   }
+  if (cfg->jacobian < PDQ_JAC_EXACT || cfg->jacobian > PDQ_JAC_MONTE_CARLO_FWD)
+    return set_err(PDQ_ERR_INVALID, "unknown jacobian handler %d", cfg->jacobian);
+  if (cfg->jacobian != PDQ_JAC_EXACT) {
+    if (cfg->jac_probes < 1 || cfg->jac_probes > 64)
+      return set_err(PDQ_ERR_INVALID, "jac_probes must be in [1, 64] (got %d)", cfg->jac_probes);
+    // re_linearize_before_error draws from the error estimator's own key
+    // (solvers.py:946-950), which the probe sets do not model (the dense
+    // factorisation materialises the exact Jacobian and draws none,
+    // jacobians.py:221-228)
+    if (cfg->constraint == PDQ_TS1 && cfg->factorisation != PDQ_DENSE && cfg->error_re_linearize)
+      return set_err(PDQ_ERR_UNSUPPORTED,
+                     "the Monte Carlo Jacobian handlers do not support re_linearize_before_error");
+  }
   return device ? check_device() : PDQ_OK;
 }
 

@@ -33,6 +33,9 @@ struct Ctx {
   int expo;          // exponential prior (dense)
   double* ws;   // instance workspace base
   double* prm;  // vector-field parameters (LDS copy)
+  // probe set of the Monte Carlo Jacobian handler (accepted steps so far), set
+  // by the solve loop before each step
+  mutable uint32_t lin_set = 0;
   __device__ double* slot(int s) const { return ws + s * L->s_size; }
   __device__ double* W(int off) const { return ws + off; }
 };
@@ -666,7 +669,8 @@ __device__ __forceinline__ int sel_row(const Ctx<G>& c, int obs) {
 // ssm_impl_isotropic.py:298-317): writes b (ko x mc) and, for ts1, J (ko x nc).
 // With tangent columns (MODE_LML) b also carries d b / d theta.
 template <int G>
-__device__ void linearize(const Ctx<G>& c, const double* Xm, double t, double* b, double* J, double* y0buf) {
+__device__ void linearize(const Ctx<G>& c, const double* Xm, double t, double* b, double* J, double* y0buf,
+                          uint32_t set_offset = 0) {
   const int d = c.d, mc = c.mc;
   const int id = c.a->cfg.vector_field;
   // gather u(t) (zeroth Taylor coefficient) and, with tangents, its tangents
@@ -699,9 +703,17 @@ __device__ void linearize(const Ctx<G>& c, const double* Xm, double t, double* b
     // ts1 (isotropic, IsotropicResidual.linearize, ssm_impl_isotropic.py:321-355,
     // with jacobian_materialize's exact trace, jacobians.py:100-106): the
     // 1 x nc operator J = [h0, 1, 0...] with h0 = -tr(J_f) / d, and
-    // b = r(m) - J m for the residual r = m_1 - f(m_0).
+    // b = r(m) - J m for the residual r = m_1 - f(m_0).  With
+    // jacobian_monte_carlo_rev / _fwd (:230-248, :151-169) tr J_f is the probe
+    // mean of v^T J_f v.
     double tr = 0.0;
-    for (int j = 0; j < d; ++j) tr += -vf_jac(id, d, y, c.prm, j, j);
+    if (c.a->cfg.jacobian != PDQ_JAC_EXACT) {
+      const bool fwd = c.a->cfg.jacobian == PDQ_JAC_MONTE_CARLO_FWD;
+      for (int j = 0; j < d; ++j)
+        tr += -vf_mc_diag(id, d, y, c.prm, j, c.a->cfg.jac_seed, c.lin_set + set_offset, c.a->cfg.jac_probes, fwd);
+    } else {
+      for (int j = 0; j < d; ++j) tr += -vf_jac(id, d, y, c.prm, j, j);
+    }
     const double h0 = tr / (double)d;
     for (int k = c.g.l; k < c.nc; k += G) J[k] = k == 0 ? h0 : (k == 1 ? 1.0 : 0.0);
     for (int j = c.g.l; j < d; j += G) {
@@ -1040,7 +1052,7 @@ __device__ double step(const Ctx<G>& c, int from, int to, double dt, bool estima
     }
     c.g.sync();
     // solvers.py:576-582: linearise again at the calibrated prediction
-    if (cfg.re_linearize_after_calibration) linearize(c, Pm, tn, R.b, R.J, y0buf);
+    if (cfg.re_linearize_after_calibration) linearize(c, Pm, tn, R.b, R.J, y0buf, 1);
     update(c, R.b, R.J, Pm, PL, R.m, R.L, big, gain, mo, nullptr);
     if (estimate) ep = state_std ? error_power_state(c, om, S.m, R.m, dt) : error_power(c, oL, sigma, S.m, R.m, dt);
   } else {
@@ -1417,6 +1429,7 @@ __device__ void lml_instance(const Ctx<G>& c, int inst) {
   const int csz = 2 * nc * nc + nc * mc + 2 * nc;
   double* store = a.scratch + (size_t)inst * (Kobs - 1) * csz;
   for (int k = 1; k < K; ++k) {
+    c.lin_set = pdq_lin_set(c.a->cfg, k - 1);
     step(c, sf, pr, a.times[k] - a.times[k - 1], false);
     const int t = sf;
     sf = pr;
@@ -1600,6 +1613,7 @@ __device__ void solve_instance(const Ctx<G>& c, int inst) {
   if (c.a->mode == MODE_FIXED_GRID) {
     for (int k = 1; k < K && status == PDQ_STATUS_OK; ++k) {
       const double dt = ts[k] - ts[k - 1];
+      c.lin_set = pdq_lin_set(c.a->cfg, k - 1);
       step(c, sf, pr, dt, false);
       ++attempts;
       ++accepted;
@@ -1622,6 +1636,7 @@ __device__ void solve_instance(const Ctx<G>& c, int inst) {
           while (ep < 1.0) {
             double dtc = dt;
             if (cfg.clip_dt) dtc = fmin(dtc, t1 - slotv(c, sf).sc[SC_T]);
+            c.lin_set = pdq_lin_set(c.a->cfg, accepted);
             ep = step(c, sf, pr, dtc, true);
             dt = control_apply(cfg, dtc, cstate, ep);
             ++att;

@@ -74,6 +74,89 @@ __device__ __forceinline__ double vf_jac(int id, int d, const double* __restrict
   return 0.0;
 }
 
+// Rademacher probes of the Monte Carlo Jacobian handler (jacobian_monte_carlo_rev,
+// jacobians.py:195-271): +1 or -1 from the top bit of the SplitMix64 finaliser
+// (Steele, Lea and Flood, "Fast splittable pseudorandom number generators",
+// OOPSLA 2014) applied to a counter that combines the seed, the probe set (the
+// accepted steps before the linearisation), the probe and the coordinate.
+__host__ __device__ inline double pdq_rademacher(uint32_t seed, uint32_t set, uint32_t probe, uint32_t coord) {
+  uint64_t z = (uint64_t)seed * 0x9E3779B97F4A7C15ull + (uint64_t)set * 0xD1B54A32D192ED03ull +
+               (uint64_t)probe * 0x8CB92BA72F3D8DD7ull + (uint64_t)coord * 0xABC98388FB8FAC03ull;
+  z ^= z >> 30;
+  z *= 0xBF58476D1CE4E5B9ull;
+  z ^= z >> 27;
+  z *= 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (z >> 63) ? -1.0 : 1.0;
+}
+
+// The probe set of a step's first linearisation after `steps` accepted steps.
+// The reference carries the handler's PRNG key in the solution and splits it
+// at every linearisation (jacobians.py:230-271): solver_dynamic with
+// re_linearize_after_calibration linearises twice per step (solvers.py:
+// 561-582), its second linearisation taking set + 1.  A rejected proposal's
+// key is discarded with it, so a retry draws the same probes.
+__host__ __device__ inline uint32_t pdq_lin_set(const pdq_config& cfg, long steps) {
+  const uint32_t per = (cfg.solver == PDQ_SOLVER_DYNAMIC && cfg.re_linearize_after_calibration) ? 2u : 1u;
+  return (uint32_t)steps * per;
+}
+
+// The rows j of the Jacobian's column k that can be non-zero (d f_j / d y_k),
+// in ascending order: at most three.
+__device__ __forceinline__ int vf_col_rows(int id, int d, int k, int* rows) {
+  switch (id) {
+    case PDQ_VF_LOTKA_VOLTERRA:
+    case PDQ_VF_VAN_DER_POL:
+      rows[0] = 0;
+      rows[1] = 1;
+      return 2;
+    case PDQ_VF_BURGERS: {
+      if (d <= 3) {
+        for (int j = 0; j < d; ++j) rows[j] = j;
+        return d;
+      }
+      const int km = (k + d - 1) % d, kp = (k + 1) % d;
+      // ascending: the wrapped neighbour goes first (k = 0) or last (k = d - 1)
+      if (k == 0) {
+        rows[0] = 0, rows[1] = 1, rows[2] = km;
+      } else if (k == d - 1) {
+        rows[0] = kp, rows[1] = km, rows[2] = k;
+      } else {
+        rows[0] = km, rows[1] = k, rows[2] = kp;
+      }
+      return 3;
+    }
+  }
+  rows[0] = k;
+  return 1;
+}
+
+// The Monte Carlo estimate of the Jacobian's diagonal entry k: the mean over
+// `probes` Rademacher vectors v of
+//  * reverse mode (calculate_diagonal_along_d of jacobian_monte_carlo_rev,
+//    jacobians.py:250-271): (v^T J)_k v_k = (sum_j v_j J[j][k]) v_k;
+//  * forward mode (jacobian_monte_carlo_fwd, :171-192): (J v)_k v_k =
+//    (sum_j J[k][j] v_j) v_k;
+// with J[j][k] = d f_j / d y_k.  In both modes its sum over k is the trace
+// estimate v^T J v of calculate_trace_along_d (:151-169, :230-248).  The
+// registry's Jacobians have symmetric sparsity patterns, so row k and column k
+// have their non-zeros at the same indices.
+__device__ __forceinline__ double vf_mc_diag(int id, int d, const double* __restrict__ y,
+                                             const double* __restrict__ prm, int k, uint32_t seed, uint32_t set,
+                                             int probes, bool fwd) {
+  int rows[3];
+  const int nr = vf_col_rows(id, d, k, rows);
+  double col[3];
+  for (int r = 0; r < nr; ++r) col[r] = fwd ? vf_jac(id, d, y, prm, k, rows[r]) : vf_jac(id, d, y, prm, rows[r], k);
+  double acc = 0.0;
+  for (int s = 0; s < probes; ++s) {
+    double w = 0.0;
+    for (int r = 0; r < nr; ++r) w += pdq_rademacher(seed, set, (uint32_t)s, (uint32_t)rows[r]) * col[r];
+    acc += w * pdq_rademacher(seed, set, (uint32_t)s, (uint32_t)k);
+  }
+  return acc / (double)probes;
+}
+
 // d f_k / d prm_q
 __device__ __forceinline__ double vf_jac_param(int id, int d, const double* __restrict__ y,
                                                const double* __restrict__ prm, int k, int q) {

@@ -676,10 +676,12 @@ __device__ __forceinline__ void lane_marginalise_cond(const Cond& c, const doubl
 // * ts1 (IsotropicResidual.linearize, ssm_impl_isotropic.py:321-355) with
 //   jacobian_materialize's exact trace (jacobians.py:100-106): the trace over
 //   d of d r / d x for r = x_1 - f(x_0) is [-tr J_f, d, 0...], divided by d,
-//   and b = r(Xm) - [h0, 1] Xm.
+//   and b = r(Xm) - [h0, 1] Xm.  With jacobian_monte_carlo_rev or _fwd (:230-248, :151-169)
+//   tr J_f is the probe mean of v^T J_f v (probe set `set`); the entry of x_1,
+//   the mean of v^T v = d, stays exact.
 template <int N, int D>
 __device__ __forceinline__ void lane_linearize(const pdq_config& cfg, const double* __restrict__ prm, const double* Xm,
-                                               double t, double* b, double& h0) {
+                                               double t, double* b, double& h0, uint32_t set = 0) {
   double y[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) y[j] = Xm[j];
@@ -690,8 +692,14 @@ __device__ __forceinline__ void lane_linearize(const pdq_config& cfg, const doub
     return;
   }
   double tr = 0.0;
+  if (cfg.jacobian != PDQ_JAC_EXACT) {
+    const bool fwd = cfg.jacobian == PDQ_JAC_MONTE_CARLO_FWD;
+    for (int j = 0; j < D; ++j)
+      tr += -vf_mc_diag(cfg.vector_field, D, y, prm, j, cfg.jac_seed, set, cfg.jac_probes, fwd);
+  } else {
 #pragma unroll
-  for (int j = 0; j < D; ++j) tr += -vf_jac(cfg.vector_field, D, y, prm, j, j);
+    for (int j = 0; j < D; ++j) tr += -vf_jac(cfg.vector_field, D, y, prm, j, j);
+  }
   h0 = tr / (double)D;
 #pragma unroll
   for (int j = 0; j < D; ++j) {
@@ -855,6 +863,7 @@ struct LaneCtx {
   const double* prm;
   LaneWs<N, D> w;
   int inst;
+  uint32_t lin_set;  // the step's probe set of the Monte Carlo Jacobian handlers (pdq_lin_set)
 };
 
 // The three parts of a step attempt: solver_dynamic.step (solvers.py:549-599),
@@ -879,7 +888,7 @@ __device__ __forceinline__ void lane_calibrate(const LaneCtx<N, D>& c, const dou
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) UL[i * N + j] = fabs(P[i]) * (s1 * lane_iwp_q<N>(Q, i, j));
-  lane_linearize<N, D>(cfg, c.prm, Um, tn, b, h0);
+  lane_linearize<N, D>(cfg, c.prm, Um, tn, b, h0, c.lin_set);
   double om[D], oL;
   lane_marginalise_fx<N, D>(cfg, b, h0, Um, UL, om, oL);
   sigma = lane_rms_zero<D>(om, oL);
@@ -930,11 +939,11 @@ __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int f
     double bu[D], hu = h0;
 #pragma unroll
     for (int k = 0; k < D; ++k) bu[k] = b[k];
-    if (cfg.re_linearize_after_calibration) lane_linearize<N, D>(cfg, c.prm, Pm, tn, bu, hu);
+    if (cfg.re_linearize_after_calibration) lane_linearize<N, D>(cfg, c.prm, Pm, tn, bu, hu, c.lin_set + 1);
     lane_update<N, D>(cfg, bu, hu, Pm, PL, Rm, RL, mo, oLu);
   } else {
     const double s1 = lane_noise_scale(cfg, h, 1.0);
-    lane_linearize<N, D>(cfg, c.prm, Pm, tn, b, h0);
+    lane_linearize<N, D>(cfg, c.prm, Pm, tn, b, h0, c.lin_set);
     if (estimate) {
       // the error's calibrated std, before the update so that its operands
       // and the posterior are not live together
@@ -948,7 +957,7 @@ __device__ __forceinline__ double lane_finish_step(const LaneCtx<N, D>& c, int f
       // (re_linearize_before_error, solvers.py:951-956)
 #pragma unroll
       for (int k = 0; k < D; ++k) be[k] = b[k];
-      if (cfg.error_re_linearize) lane_linearize<N, D>(cfg, c.prm, Um, tn, be, he);
+      if (cfg.error_re_linearize) lane_linearize<N, D>(cfg, c.prm, Um, tn, be, he, c.lin_set);
       if (state_std) {
         err = lane_state_err<N, D>(cfg, be, he, Um, UL, cfg.error_derivative_idx);
       } else {
@@ -1519,6 +1528,7 @@ __device__ __forceinline__ LaneCtx<N, D> lane_ctx(const LArgs& a, int inst, doub
   c.inst = inst;
   c.prm = a.params + (size_t)inst * a.cfg.n_params;
   c.w.base = slots;
+  c.lin_set = 0;
   return c;
 }
 
@@ -1527,13 +1537,14 @@ __device__ __forceinline__ LaneCtx<N, D> lane_ctx(const LArgs& a, int inst, doub
 // adaptive loop runs part 1, part 2 and the merge part 2 asks for, in turn.
 template <int N, int D>
 __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, double* slots) {
-  const LaneCtx<N, D> c = lane_ctx<N, D>(a, inst, slots);
+  LaneCtx<N, D> c = lane_ctx<N, D>(a, inst, slots);
   const int K = a.K;
   const double* ts = a.times + (size_t)inst * K;
   LaneLoop s = lane_loop_start(a.cfg, inst);
   lane_init_instance<N, D>(c, ts, s.sf, s.ifr);
   if (a.mode == LMODE_FIXED_GRID) {
     for (int k = 1; k < K; ++k) {
+      c.lin_set = pdq_lin_set(a.cfg, k - 1);
       lane_step<N, D>(c, s.sf, s.pr, ts[k] - ts[k - 1], false);
       ++s.attempts;
       ++s.accepted;
@@ -1545,6 +1556,7 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
   } else {
     while (s.k < K && s.status == PDQ_STATUS_OK) {
       LaneIterVals<N, D> v;
+      c.lin_set = pdq_lin_set(a.cfg, s.accepted);
       lane_adapt_part1<N, D>(c, ts, s, v);
       int packed[2];
       lane_pack_req(lane_adapt_part2<N, D>(c, ts, s, v), packed);
@@ -1594,7 +1606,7 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
   volatile int* const flag = req + 2 * kLaneBlock;
   const int inst = blockIdx.x * kLaneBlock + l;
   const bool mine = inst < a.n_inst;
-  const LaneCtx<N, D> c = lane_ctx<N, D>(a, mine ? inst : 0, slots);
+  LaneCtx<N, D> c = lane_ctx<N, D>(a, mine ? inst : 0, slots);
   if (role == 0) {
     const int K = a.K;
     const double* ts = a.times + (size_t)(mine ? inst : 0) * K;
@@ -1609,6 +1621,7 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
     LaneMergeReq last = lane_no_req();
     while (__builtin_amdgcn_readfirstlane(*flag)) {
       LaneIterVals<N, D> v;
+      c.lin_set = pdq_lin_set(a.cfg, s.accepted);
       if (run) lane_adapt_part1<N, D>(c, ts, s, v);
       __syncthreads();
       last = lane_no_req();

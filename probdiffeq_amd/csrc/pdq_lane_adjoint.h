@@ -167,6 +167,7 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
   c.inst = inst;
   c.prm = a.params + (size_t)inst * cfg.n_params;
   c.w.base = slots;
+  c.lin_set = 0;
   const LaneWs<N, D>& w = c.w;
   const int K = a.K, every = a.every, Kobs = (K - 1) / every + 1, ntan = a.n_tan;
   const AdjLayout<N, D> lay(K, every);

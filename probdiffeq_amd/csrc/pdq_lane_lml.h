@@ -243,6 +243,7 @@ __device__ __forceinline__ void lane_lml_instance(const LArgs& la, const LmlArgs
   c.inst = inst;
   c.prm = a.params + (size_t)inst * cfg.n_params;
   c.w.base = slots;
+  c.lin_set = 0;
   const LaneWs<N, MC>& w = c.w;
   const int ntan = a.n_tan, q0 = g * TPL;
   const int K = a.K, every = a.every, Kobs = (K - 1) / every + 1;

@@ -150,12 +150,15 @@ class jacobian_materialize:
         return f"{self.__class__.__name__}(jacfun={self.jacfun})"
 
 
-class jacobian_monte_carlo_fwd:
-    """jacobians.py:116-192: an exact dense Jacobian; Hutchinson estimates of traces and diagonals.
+class _JacobianMonteCarlo:
+    """An exact dense Jacobian (dense ts1); Hutchinson estimates of the isotropic trace and the
+    blockdiag diagonal from `num_probes` Rademacher probes (1..64).  The kernels draw one probe set
+    per linearisation from a counter-based stream keyed by `seed` and the accepted steps (a
+    rejected attempt's retry reuses its probes, as the reference's key travels with the accepted
+    solution); it is not JAX's threefry stream, so single estimates differ from the reference's
+    while following the same estimator."""
 
-    The device path uses the dense Jacobian (dense ts1).  The estimates draw Rademacher probes from
-    JAX's random stream, which the device path does not reproduce: isotropic ts1 with this handler
-    raises NotImplementedError (pass jacobian_materialize() for the exact trace)."""
+    mode = ""
 
     def __init__(self, *, seed=1, num_probes=10) -> None:
         self.seed = seed
@@ -165,8 +168,32 @@ class jacobian_monte_carlo_fwd:
         return f"{self.__class__.__name__}(seed={self.seed}, num_probes={self.num_probes})"
 
 
-class jacobian_monte_carlo_rev(jacobian_monte_carlo_fwd):
-    """jacobians.py:195-271 (reverse mode; the reference's default handler of ode())."""
+class jacobian_monte_carlo_fwd(_JacobianMonteCarlo):
+    """jacobians.py:116-192 (forward mode): diagonal entry k is the probe mean of (J v)_k v_k."""
+
+    mode = "monte_carlo_fwd"
+
+
+class jacobian_monte_carlo_rev(_JacobianMonteCarlo):
+    """jacobians.py:195-271 (reverse mode; the reference's default handler of ode(),
+    problems.py:283-295): diagonal entry k is the probe mean of (v^T J)_k v_k."""
+
+    mode = "monte_carlo_rev"
+
+
+def _jacobian_handler(jac):
+    """(pdq_config.jacobian, probes, seed) of a Jacobian handler; None is ode()'s default,
+    jacobian_monte_carlo_rev(seed=1, num_probes=10) (problems.py:283-295)."""
+    if jac is None:
+        jac = jacobian_monte_carlo_rev()
+    if isinstance(jac, jacobian_materialize):
+        return "exact", 10, 1
+    if isinstance(jac, _JacobianMonteCarlo):
+        probes = int(jac.num_probes)
+        if not 1 <= probes <= 64:
+            raise NotImplementedError(f"the kernels draw 1 to 64 probes per linearisation (got num_probes={probes})")
+        return jac.mode, probes, int(jac.seed) & 0xFFFFFFFF
+    raise TypeError(f"unknown Jacobian handler {jac!r}")
 
 
 class JetOdeAutonomous:
@@ -587,20 +614,15 @@ class _StateSpaceModel:
         """ssm_impl_api.py:536-558 (prior-mean Taylor point).
 
         Dense: the exact Jacobian (every handler materialises it).  Isotropic
-        (ssm_impl_isotropic.py:321-355): the trace over d of the Jacobian, exact with
-        jacobian_materialize (jacobians.py:100-106); the reference's default handler estimates
-        it from JAX random probes, which the device path does not reproduce."""
+        (ssm_impl_isotropic.py:321-355): the trace over d of the Jacobian; blockdiag
+        (ssm_impl_blockdiag.py:147-183): its diagonal.  Both are exact with jacobian_materialize
+        (jacobians.py:100-113) and Hutchinson estimates with jacobian_monte_carlo_rev (ode()'s
+        default) or jacobian_monte_carlo_fwd."""
         if not isinstance(ode, JetOde):
             raise TypeError(ode)
         if taylor_point is not None and not isinstance(taylor_point, taylor_point_prior):
             raise NotImplementedError("only the prior-mean Taylor point (taylor_point_prior) is implemented")
-        if self.factorisation == "blockdiag":
-            msg = "ts1 for blockdiag models (Jacobian diagonals, jacobians.py:108-113, 171-193) is not implemented on the device path"
-            raise NotImplementedError(msg)
-        if self.factorisation == "isotropic" and not isinstance(ode.jacobian, jacobian_materialize):
-            msg = "isotropic ts1 runs with the exact trace: construct the ODE with jacobian=jacobian_materialize()"
-            msg += f" (got {ode.jacobian!r}; Hutchinson estimates from JAX's random stream are not reproduced)"
-            raise NotImplementedError(msg)
+        _jacobian_handler(ode.jacobian)  # unknown handlers and probe counts fail here
         return Constraint("ts1", ode)
 
 
@@ -808,6 +830,8 @@ def make_config(prior: Prior, solver_obj: _Solver, error=None, control=None, cli
     )
     if prior.kind == "exponential":
         cfg.prior, cfg.prior_drift = "exponential", prior.drift
+    if solver_obj.constraint.kind == "ts1":
+        cfg.jacobian, cfg.jac_probes, cfg.jac_seed = _jacobian_handler(ode_.jacobian)
     if isinstance(solver_obj, solver_mle):
         cfg.correct_asymptotic_underconfidence = solver_obj.correct_asymptotic_underconfidence
     if isinstance(solver_obj, solver_dynamic):

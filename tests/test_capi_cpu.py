@@ -28,12 +28,13 @@ def test_library_exports_every_declared_symbol():
 
 def test_config_struct_matches_header_layout():
     """The ctypes mirror of pdq_config has the C layout (12 ints, 12 doubles, 8 ints, two pointers,
-    one int padded to the struct's 8-byte alignment, one pointer, two ints)."""
+    one int padded to the struct's 8-byte alignment, one pointer, five ints padded)."""
     from probdiffeq_amd import capi
 
-    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8 + 8
+    assert ctypes.sizeof(capi._Config) == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8 + 24
     assert capi._Config.lml_gradient.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8
     assert capi._Config.lml_solve.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8 + 4
+    assert capi._Config.jac_seed.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8 + 8 + 16
     assert capi._Config.lml_tcoeff_index.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8
     assert capi._Config.init_std_rows.offset == 12 * 4 + 12 * 8 + 8 * 4 + 2 * 8 + 8
     assert ctypes.sizeof(capi._Solution) == 17 * 8
@@ -93,17 +94,19 @@ def test_api_objects_mirror_the_reference():
     with pytest.raises(NotImplementedError):
         pdq.ode(lambda y, *, t: y)
     assert pdq.solver_dynamic(strategy=pdq.strategy_filter(), constraint=None, re_linearize_after_calibration=True).re_linearize_after_calibration
-    # isotropic ts1 takes the exact trace (jacobian_materialize); the reference's default
-    # Hutchinson handler draws JAX random probes and is refused
+    # ts1 takes ode()'s default handler (jacobian_monte_carlo_rev(seed=1, num_probes=10)) or the
+    # one the ODE names, for every factorisation
     lv = pdq.JetOde("lotka_volterra", None, 2)  # registry entry; its parameters stay unset here
-    with pytest.raises(NotImplementedError, match="jacobian_materialize"):
-        ssm.constraint_ode_ts1(lv)
-    with pytest.raises(NotImplementedError, match="jacobian_materialize"):
-        ssm.constraint_ode_ts1(pdq.ode(lv, jacobian=pdq.jacobian_monte_carlo_rev()))
-    assert ssm.constraint_ode_ts1(pdq.ode(lv, jacobian=pdq.jacobian_materialize())).kind == "ts1"
-    assert pdq.state_space_model_dense().constraint_ode_ts1(lv).kind == "ts1"
-    with pytest.raises(NotImplementedError):
-        pdq.state_space_model_blockdiag().constraint_ode_ts1(lv)
+    assert pdq._jacobian_handler(lv.jacobian) == ("monte_carlo_rev", 10, 1)
+    assert pdq._jacobian_handler(pdq.jacobian_monte_carlo_fwd(seed=7, num_probes=3)) == ("monte_carlo_fwd", 3, 7)
+    assert pdq._jacobian_handler(pdq.jacobian_materialize())[0] == "exact"
+    for model in (ssm, pdq.state_space_model_dense(), pdq.state_space_model_blockdiag()):
+        assert model.constraint_ode_ts1(lv).kind == "ts1"
+        assert model.constraint_ode_ts1(pdq.ode(lv, jacobian=pdq.jacobian_materialize())).kind == "ts1"
+    with pytest.raises(NotImplementedError, match="probes"):
+        ssm.constraint_ode_ts1(pdq.ode(lv, jacobian=pdq.jacobian_monte_carlo_rev(num_probes=65)))
+    with pytest.raises(TypeError):
+        ssm.constraint_ode_ts1(pdq.ode(lv, jacobian=object()))
 
 
 def test_fixed_grid_warns_for_fixedpoint_smoother():
@@ -279,3 +282,33 @@ def test_lml_gradient_mode_is_checked_before_device_work():
         cfg.update(kw)
         assert call(capi.Config(**cfg).c()) == 2, kw
         assert match in capi.last_error(), (kw, capi.last_error())
+
+
+def test_jacobian_handler_is_checked_before_device_work():
+    """pdq_config.jacobian / jac_probes: unknown handlers, probe counts outside 1..64 and the
+    re-linearisation before the error estimate (which draws from the error estimator's own key)
+    are refused before the device is touched."""
+    from probdiffeq_amd import capi
+
+    with pytest.raises(ValueError, match="jacobian"):
+        capi.Config(jacobian="hutchinson").c()
+    lib = capi.lib()
+    sol = capi._Solution()
+
+    def call(c):
+        return lib.pdq_solve_adaptive_save_at(ctypes.byref(c), 1, None, None, None, 10, ctypes.byref(sol), None)
+
+    base = dict(factorisation="isotropic", num_derivatives=3, constraint="ts1", jacobian="monte_carlo_rev")
+    c = capi.Config(**base).c()
+    c.jacobian = 5
+    assert call(c) == 1 and "unknown jacobian" in capi.last_error()
+    for probes in (0, 65):
+        assert call(capi.Config(**base, jac_probes=probes).c()) == 1
+        assert "jac_probes" in capi.last_error()
+    for fact in ("isotropic", "blockdiag"):
+        cfg = dict(base, factorisation=fact, error_re_linearize=True, solver="solver")
+        assert call(capi.Config(**cfg).c()) == 2 and "re_linearize_before_error" in capi.last_error()
+    # the dense factorisation materialises the Jacobian: the option combination passes the checks
+    # (and then stops at the missing device or the null inputs)
+    cfg = dict(base, factorisation="dense", error_re_linearize=True, solver="solver")
+    assert call(capi.Config(**cfg).c()) not in (0, 2) or "re_linearize" not in capi.last_error()

@@ -100,6 +100,29 @@ def test_ts1_exact_jacobian_diagonal(oracle, pdq, vf, d, strategy):
     T._compare(oracle, pdq, cfg, params, y0, times)
 
 
+@pytest.mark.parametrize(
+    "mode,vf,d,strategy,kw",
+    [
+        ("monte_carlo_rev", "lotka_volterra", 2, "fixedpoint", {}),
+        ("monte_carlo_fwd", "lotka_volterra", 2, "filter", dict(re_linearize_after_calibration=True)),
+        ("monte_carlo_rev", "burgers", 16, "filter", {}),
+        ("monte_carlo_fwd", "burgers", 70, "fixedinterval", {}),
+    ],
+)
+def test_ts1_monte_carlo_jacobian_diagonal(oracle, pdq, mode, vf, d, strategy, kw):
+    """Blockdiag ts1 with the Hutchinson diagonal of jacobian_monte_carlo_rev / _fwd (jacobians.py:
+    171-192, 250-271): every block's lane draws the oracle's probes (d = 70 spans two wavefronts)."""
+    B = 3
+    if vf == "burgers":
+        params, y0 = _burgers(d, B, seed=6, nmodes=3)
+        times = np.tile([0.0, 0.05, 0.1], (B, 1))
+    else:
+        params, y0, times = T._lv_batch(B, seed=47, t_lo=3.0, t_hi=6.0, K=7)
+    cfg = pdq.Config(factorisation="blockdiag", num_derivatives=3, ode_dim=d, vector_field=vf, n_params=params.shape[1],
+                     constraint="ts1", strategy=strategy, rtol=1e-5, atol=1e-7, jacobian=mode, jac_probes=5, jac_seed=4, **kw)
+    T._compare(oracle, pdq, cfg, params, y0, times)
+
+
 @pytest.mark.parametrize("strategy", ["filter", "fixedinterval"])
 def test_fixed_grid(oracle, pdq, strategy):
     params, y0, _ = T._lv_batch(4, seed=47)

@@ -87,6 +87,16 @@ def test_lml_value_with_ts1(oracle, pdq, fact, lanes):
     _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=False)
 
 
+@pytest.mark.parametrize("lanes", [0, 16])
+def test_lml_value_with_ts1_monte_carlo_jacobian(oracle, pdq, lanes):
+    """The LML value with isotropic ts1 and ode()'s default handler, jacobian_monte_carlo_rev: on the
+    fixed grid step k draws probe set k - 1, as in the oracle."""
+    params, y0, grid, every, data, stds = LC.lml_inputs(4, seed=15, t_end=10.0, n_grid=1001, every=10)
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="dynamic",
+                     constraint="ts1", lanes_per_instance=lanes, jacobian="monte_carlo_rev", jac_probes=10, jac_seed=1)
+    _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=False)
+
+
 def test_lml_tcoeff_index_is_checked(pdq):
     params, y0, grid, every, data, stds = LC.lml_inputs(1, seed=1, t_end=2.0, n_grid=11, every=5)
     for idx in (-1, 4):

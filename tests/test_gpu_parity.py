@@ -114,6 +114,9 @@ def _oracle_cfg(oracle, cfg):
         tcoeff_index=cfg.lml_tcoeff_index,
         init_std_rows=None if cfg.init_std_rows is None else np.asarray(cfg.init_std_rows.cpu() if hasattr(cfg.init_std_rows, "cpu") else cfg.init_std_rows),
         lml_solve=cfg.lml_solve,
+        jacobian=cfg.jacobian,
+        jac_probes=cfg.jac_probes,
+        jac_seed=cfg.jac_seed,
     )
 
 
@@ -411,6 +414,28 @@ def test_isotropic_ts1_shapes_without_a_lane_kernel(oracle, pdq, nu, strategy):
     _compare(oracle, pdq, cfg, params, y0s, times)
 
 
+@pytest.mark.parametrize("mode", ["monte_carlo_rev", "monte_carlo_fwd"])
+@pytest.mark.parametrize(
+    "lanes,strategy,kw",
+    [
+        (0, "fixedpoint", {}),
+        (16, "fixedpoint", {}),
+        (0, "filter", dict(clip_dt=True)),
+        (8, "fixedinterval", dict(re_linearize_after_calibration=True)),
+    ],
+)
+def test_isotropic_ts1_monte_carlo_jacobian(oracle, pdq, mode, lanes, strategy, kw):
+    """Isotropic ts1 with the Hutchinson trace of jacobian_monte_carlo_rev / _fwd (jacobians.py:
+    151-169, 230-248) on the lane kernel (lanes 0) and the cooperative kernel: the same probes as
+    the oracle, one set per linearisation, a rejected attempt's retry reusing its set."""
+    cfg = pdq.Config(factorisation="isotropic", num_derivatives=4, strategy=strategy, solver="dynamic", constraint="ts1",
+                     rtol=1e-5, atol=1e-7, lanes_per_instance=lanes, jacobian=mode, jac_probes=6, jac_seed=9, **kw)
+    params, y0s, times = _lv_batch(6, seed=45, t_lo=5.0, t_hi=10.0, K=12)
+    g = _compare(oracle, pdq, cfg, params, y0s, times)
+    assert np.all(g["attempts"] > g["accepted"])
+    _compare(oracle, pdq, cfg, params, y0s, times, fixed_grid=True)
+
+
 def test_ts1_lml_gradients_are_refused(pdq):
     """LML gradients need ts0 (the forward-mode tangents share the covariance recursion); ts1 values run."""
     params, y0s, times = _lv_batch(1, seed=1, K=5)
@@ -459,7 +484,8 @@ def test_exponential_prior(oracle, pdq, nu, prior, strategy, solver, lanes):
 
 
 def test_exponential_prior_refusals(pdq):
-    """Dense only (as the reference), (nu + 1) d <= 16, a drift pointer, and no LML."""
+    """Dense only (as the reference), (nu + 1) d <= 16 and a drift pointer.  (The LML runs them:
+    tests/test_gpu_lml.py.)"""
     params, y0s, times = _lv_batch(1, seed=1, K=5)
     tc = pdq.taylor_coefficients("lotka_volterra", _dev(params), _dev(y0s), 0.0, 3)
     drift = _dev(np.zeros((2, 8)))
@@ -470,10 +496,12 @@ def test_exponential_prior_refusals(pdq):
         cfg = pdq.Config(num_derivatives=3, prior="exponential", **kw)
         with pytest.raises(pdq.PdqError, match=match):
             pdq.solve(cfg, _dev(params), tc, _dev(times))
-    cfg = pdq.Config(factorisation="dense", num_derivatives=3, strategy="fixedpoint", prior="exponential", prior_drift=drift)
-    grid = np.linspace(0.0, 2.0, 11)
-    with pytest.raises(pdq.PdqError, match="Wiener"):
-        pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0s), _dev(grid), 5, _dev(np.ones((1, 3, 2))), _dev(np.ones((1, 3))), grad=False)
+    # (nu + 1) d = 18
+    lp, ly = _dev(np.full((1, 1), 1.5)), _dev(np.full((1, 3), 0.3))
+    cfg = pdq.Config(factorisation="dense", num_derivatives=5, ode_dim=3, vector_field="logistic", n_params=1,
+                     prior="exponential", prior_drift=_dev(np.zeros((3, 18))))
+    with pytest.raises(pdq.PdqError, match="<= 16"):
+        pdq.solve(cfg, lp, pdq.taylor_coefficients("logistic", lp, ly, 0.0, 5), _dev(times))
 
 
 @pytest.mark.parametrize(

@@ -609,6 +609,45 @@ def test_isotropic_ts1_kernels(oracle, emu, strategy, solver, vf, kw, kernel):
 
 
 @pytest.mark.parametrize(
+    "fact,kernel,strategy,kw",
+    [
+        ("isotropic", "lane", "fixedpoint", {}),
+        ("isotropic", "coop", "fixedpoint", {}),
+        ("isotropic", "lane", "filter", dict(re_linearize_after_calibration=True)),
+        ("isotropic", "coop", "fixedinterval", dict(re_linearize_after_calibration=True)),
+        ("blockdiag", "coop", "fixedpoint", {}),
+        ("blockdiag", "coop", "filter", dict(re_linearize_after_calibration=True, clip_dt=True)),
+    ],
+)
+@pytest.mark.parametrize("mode", ["monte_carlo_rev", "monte_carlo_fwd"])
+def test_monte_carlo_jacobian_kernels(oracle, emu, fact, kernel, strategy, kw, mode):
+    """ts1 with the Monte Carlo Jacobian handlers (jacobians.py:116-271): the kernels draw the
+    oracle's probes (one set per linearisation, indexed by the accepted steps) and follow its
+    solves, rejected attempts included."""
+    from probdiffeq_amd import capi
+
+    p, y, t = T._lv_batch(3, seed=8, t_lo=4.0, t_hi=8.0, K=10)
+    cfg = capi.Config(factorisation=fact, num_derivatives=3, strategy=strategy, solver="dynamic", constraint="ts1",
+                      rtol=1e-5, atol=1e-7, jacobian=mode, jac_probes=6, jac_seed=3, **kw)
+    g = _run(oracle, emu, cfg, p, y, t, kernel=kernel)
+    assert np.all(g["attempts"] > g["accepted"])  # rejected attempts reuse their probe sets
+    fx = _run(oracle, emu, cfg, p, y, t, fixed_grid=True, kernel=kernel)
+    assert np.all(fx["status"] == 0)
+
+
+def test_monte_carlo_jacobian_blockdiag_burgers(oracle, emu):
+    """Blockdiag ts1 on Burgers (three non-zeros per Jacobian column, wrapped at the ends) with
+    the reverse-mode diagonal estimate."""
+    from probdiffeq_amd import capi
+
+    params, y0 = _burgers(16, 2, seed=5, nmodes=3)
+    times = np.tile([0.0, 0.05, 0.1], (2, 1))
+    cfg = capi.Config(factorisation="blockdiag", num_derivatives=3, ode_dim=16, vector_field="burgers", n_params=2,
+                      constraint="ts1", strategy="filter", rtol=1e-5, atol=1e-7, jacobian="monte_carlo_rev", jac_probes=10)
+    _run(oracle, emu, cfg, params, y0, times)
+
+
+@pytest.mark.parametrize(
     "prior,strategy,solver",
     [
         ("zero", "filter", "dynamic"),
