@@ -551,7 +551,12 @@ int32_t pdq_lml_value_and_grad(const pdq_config* cfg, int32_t n_inst, const doub
   // exponential priors (dense only, check_common) run the cooperative LML: their
   // transition depends on the step and the drift, not on the parameters, so the
   // forward-mode tangents stay exact
-  if (cfg->solver == PDQ_SOLVER_MLE) return set_err(PDQ_ERR_UNSUPPORTED, "solver_mle is not supported by the LML path");
+  // solver_mle: the value runs (the posterior rescaled by the calibrated output
+  // scale); its gradient would carry that scale's dependence on the parameters
+  // through every factor of the reverse scan, which the tangents do not
+  if (cfg->solver == PDQ_SOLVER_MLE && grad)
+    return set_err(PDQ_ERR_UNSUPPORTED,
+                   "LML gradients with solver_mle are not supported (the value runs: pass grad = NULL)");
   if (every < 1 || (K_grid - 1) % every != 0)
     return set_err(PDQ_ERR_INVALID, "every (%d) must divide K_grid - 1 (%d)", every, K_grid - 1);
   const int n_tan = grad ? cfg->n_params + cfg->ode_dim : 0;

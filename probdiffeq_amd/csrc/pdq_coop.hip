@@ -1448,6 +1448,27 @@ __device__ void lml_instance(const Ctx<G>& c, int inst) {
     }
   }
   __threadfence_block();
+  if (cfg.solver == PDQ_SOLVER_MLE) {
+    // solver_mle's userfriendly_output rescales the posterior by the
+    // calibrated output scale before the loss sees it (solvers.py:439-480,
+    // strategy.finalize): every stored conditional's noise factor and the
+    // terminal marginal and conditional.  With tangents refused, the factors
+    // are the only thing the scale touches.
+    c.g.sync();
+    SlotV S = slotv(c, sf);
+    double scale = S.sc[SC_MLE_RUN];
+    if (cfg.correct_asymptotic_underconfidence) scale = scale / sqrt((double)(K - 1));
+    for (int e = c.g.l; e < nc * nc; e += G) {
+      S.L[e] = scale * S.L[e];
+      S.cL[e] = scale * S.cL[e];
+    }
+    for (int j = 0; j < Kobs - 1; ++j) {
+      double* nl = store + (size_t)j * csz + nc * nc + nc * mc;
+      for (int e = c.g.l; e < nc * nc; e += G) nl[e] = scale * nl[e];
+    }
+    __threadfence_block();
+    c.g.sync();
+  }
   // terminal marginal: posterior1.conditional.marginalise(posterior1.marginal)
   double* big = c.W(L.w_big);
   double* Xm = c.W(L.w_Pm);

@@ -39,6 +39,13 @@ def lml_inputs(n_inst, seed=0, t_end=20.0, n_grid=2001, every=20, per_coordinate
     return params, y0, grid, every, data, stds
 
 
+def lml_inputs_scale_sensitive(n_inst, seed=0):
+    """A coarse grid (dt = 0.25) and observation stds of 1e-3: the solver's own uncertainty
+    dominates the observed covariances, so the LML depends on the posterior's output scale."""
+    params, y0, grid, every, data, _ = lml_inputs(n_inst, seed=seed, t_end=6.0, n_grid=25, every=2)
+    return params, y0, grid, every, data, np.full(data.shape[:2] + (1,), 1e-3)
+
+
 def oracle_value(oracle, ocfg, params, y0, grid, every, data, stds, average=True):
     """(FP64 strict, FP64 FMA, long double) values of the oracle's LML on the same float64 inputs."""
     v = oracle.lml_from_y0(ocfg, params, y0, grid, every, data, stds, average)

@@ -97,6 +97,19 @@ def test_lml_value_with_ts1_monte_carlo_jacobian(oracle, pdq, lanes):
     _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=False)
 
 
+@pytest.mark.parametrize("fact,lanes,correct", [("isotropic", 0, True), ("isotropic", 16, False), ("dense", 0, True)])
+def test_lml_value_with_solver_mle(oracle, pdq, fact, lanes, correct):
+    """The LML value with solver_mle runs the cooperative LML on the posterior rescaled by the
+    calibrated output scale (solvers.py:439-480); inputs where that scale moves the value.  Its
+    gradient is refused."""
+    params, y0, grid, every, data, stds = LC.lml_inputs_scale_sensitive(4, seed=17)
+    cfg = pdq.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="mle",
+                     lanes_per_instance=lanes, correct_asymptotic_underconfidence=correct)
+    _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=False)
+    with pytest.raises(pdq.PdqError, match="solver_mle"):
+        pdq.lml_value_and_grad(cfg, _dev(params), _dev(y0), _dev(grid), every, _dev(data), _dev(stds), grad=True)
+
+
 def test_lml_tcoeff_index_is_checked(pdq):
     params, y0, grid, every, data, stds = LC.lml_inputs(1, seed=1, t_end=2.0, n_grid=11, every=5)
     for idx in (-1, 4):

@@ -104,6 +104,7 @@ def _oracle_cfg(oracle, cfg):
         damp=cfg.damp,
         base_scale=cfg.base_output_scale,
         max_attempts=cfg.max_attempts,
+        correct_asymptotic_underconfidence=cfg.correct_asymptotic_underconfidence,
         re_linearize_after_calibration=cfg.re_linearize_after_calibration,
         error_estimator=cfg.error_estimator,
         error_re_linearize=cfg.error_re_linearize,

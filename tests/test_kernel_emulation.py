@@ -334,6 +334,27 @@ def test_lml_value_with_ts1(oracle, emu, fact):
         T.check_accuracy(v[i], vo, vf, vt, f"emulation lml ts1 instance {i}")
 
 
+@pytest.mark.parametrize("fact,correct,constraint", [("isotropic", True, "ts0"), ("dense", False, "ts0"), ("isotropic", True, "ts1")])
+def test_lml_value_with_solver_mle(oracle, emu, fact, correct, constraint):
+    """The cooperative LML's value with solver_mle: the posterior rescaled by the calibrated output
+    scale (solvers.py:439-480) before the reverse scan, against the oracle."""
+    import lml_cases as LC
+    from probdiffeq_amd import capi
+
+    params, y0, grid, every, data, stds = LC.lml_inputs_scale_sensitive(2, seed=16)
+    cfg = capi.Config(factorisation=fact, num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="mle",
+                      constraint=constraint, correct_asymptotic_underconfidence=correct)
+    ocfg = T._oracle_cfg(oracle, cfg)
+    v, _ = emu.lml(cfg, params, y0, grid, every, data, stds, grad=False)
+    for i in range(2):
+        vo, vf, vt = LC.oracle_value(oracle, ocfg, params[i], y0[i], grid, every, data[i], stds[i])
+        T.check_accuracy(v[i], vo, vf, vt, f"emulation lml mle instance {i}")
+        # the rescaling is live: the uncalibrated solver's LML is far off
+        plain = oracle.lml_from_y0(T._oracle_cfg(oracle, capi.Config(**{**cfg.__dict__, "solver": "solver"})),
+                                   params[i], y0[i], grid, every, data[i], stds[i])
+        assert abs(plain - vo) > 1e-3 * abs(vo)
+
+
 @pytest.mark.parametrize("fact,kernel", [("dense", "coop"), ("isotropic", "coop"), ("isotropic", "lane")])
 def test_init_std_rows(oracle, emu, fact, kernel):
     """A std per covariance row at t0 (element-wise is_exact, diffuse_derivatives) in both kernels'

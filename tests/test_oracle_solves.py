@@ -334,15 +334,17 @@ def _lml_bruteforce(oracle, cfg, r, data, stds, idx=0):
     return total
 
 
+@pytest.mark.parametrize("solver", ["solver", "mle"])
 @pytest.mark.parametrize("idx", [0, 1, 2])
 @pytest.mark.parametrize("fact", ["isotropic", "dense"])
-def test_lml_equals_bruteforce_joint_density(oracle, fact, idx):
+def test_lml_equals_bruteforce_joint_density(oracle, fact, idx, solver):
     """loss_lml_timeseries(tcoeff_index=idx) (estimators_and_losses.py:53-105) equals the joint
-    Gaussian density of the data under the Markov sequence, with H selecting coefficient idx."""
+    Gaussian density of the data under the Markov sequence, with H selecting coefficient idx
+    (solver_mle: the sequence its userfriendly_output rescales)."""
     tc = oracle.taylor_coefficients("lotka_volterra", THETA, [20.0, 20.0], 0.0, 2)
     grid = np.linspace(0.0, 2.0, 41)
     every = 10
-    cfg = oracle.Config(fact=fact, nu=2, strategy="fixedpoint", solver="solver", tcoeff_index=idx)
+    cfg = oracle.Config(fact=fact, nu=2, strategy="fixedpoint", solver=solver, tcoeff_index=idx)
     r = oracle.solve_fixed_grid_checkpoints(cfg, THETA, tc, grid, every)
     K = len(r["t"])
     ref = lv_reference(grid[::every])
@@ -354,6 +356,31 @@ def test_lml_equals_bruteforce_joint_density(oracle, fact, idx):
     brute = _lml_bruteforce(oracle, cfg, r, data, stds, idx)
     assert np.isclose(lml_sum, brute, rtol=1e-8)
     assert np.isclose(lml_avg * K, lml_sum, rtol=1e-12)
+
+
+@pytest.mark.parametrize("correct", [True, False])
+@pytest.mark.parametrize("fact", ["isotropic", "dense"])
+def test_lml_with_solver_mle_is_the_lml_at_the_calibrated_scale(oracle, fact, correct):
+    """solver_mle (solvers.py:395-480) rescales the posterior by its calibrated output scale s.
+    With noise-free ts0 observations and an exact initial state the means do not depend on the
+    prior's scale and every covariance scales with s^2, so the LML of the rescaled sequence is the
+    plain solver's LML under a prior of output scale s -- and differs from it at scale 1."""
+    tc = oracle.taylor_coefficients("lotka_volterra", THETA, [20.0, 20.0], 0.0, 3)
+    grid = np.linspace(0.0, 3.0, 13)  # coarse, so that the solver's own uncertainty shows
+    every = 2
+    data = lv_reference(grid[::every]) + 0.05
+    stds = np.full(data.shape[0], 0.01)
+    mle = oracle.Config(fact=fact, nu=3, strategy="fixedpoint", solver="mle", correct_asymptotic_underconfidence=correct)
+    r = oracle.solve_fixed_grid_checkpoints(mle, THETA, tc, grid, every)
+    scale = float(np.ravel(r["output_scale"])[-1])
+    assert scale > 0 and abs(scale - 1.0) > 1e-3
+    plain = oracle.Config(fact=fact, nu=3, strategy="fixedpoint", solver="solver", base_scale=scale)
+    got = oracle.lml(mle, THETA, tc, grid, every, data, stds, average=False)
+    want = oracle.lml(plain, THETA, tc, grid, every, data, stds, average=False)
+    assert np.isclose(got, want, rtol=1e-10)
+    unit = oracle.lml(oracle.Config(fact=fact, nu=3, strategy="fixedpoint", solver="solver"), THETA, tc, grid, every,
+                      data, stds, average=False)
+    assert not np.isclose(got, unit, rtol=1e-3)
 
 
 def test_calibration_tape_replays_and_freezes_the_output_scales(oracle):
