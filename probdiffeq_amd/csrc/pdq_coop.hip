@@ -16,6 +16,26 @@
 namespace pdq {
 
 // ---------------------------------------------------------------- context
+#ifndef PDQ_HOST_EMU
+// The dynamic LDS of every cooperative launch.  A pointer into it that reaches
+// a function through memory (the context) or as an argument of a function
+// that is not inlined is a generic address, and its loads and stores compile
+// to flat instructions, each waited for on both the vector-memory and the LDS
+// counters.  lds_ptr rebuilds such a pointer from this symbol (the same
+// address), so that the compiler emits LDS instructions for it.  Only
+// pointers into the launch's LDS may pass through it.
+extern __shared__ double pdq_coop_lds[];
+template <class T>
+__device__ __forceinline__ T* lds_ptr(T* p) {
+  const double* base = pdq_coop_lds;
+  return reinterpret_cast<T*>(pdq_coop_lds + (reinterpret_cast<const double*>(p) - base));
+}
+#else
+template <class T>
+inline T* lds_ptr(T* p) {
+  return p;
+}
+#endif
 
 
 // Slot scalar indices.
@@ -27,17 +47,22 @@ struct Ctx {
   const KArgs* a;
   const Layout* L;
   int nc, mc, mcp, ko, N, d, dense;  // mcp: primal mean columns (mc includes tangents)
-  double* sA;   // shared A (nc x nc); the exponential prior's e^A of the current step
-  double* sQ;   // shared Q (nc x nc); the exponential prior's Gramian factor
-  const double* sD;  // exponential prior: the drift's last d rows (d x nc, shared)
-  int expo;          // exponential prior (dense)
-  double* ws;   // instance workspace base
-  double* prm;  // vector-field parameters (LDS copy)
+  double* sA_;   // shared A (nc x nc); the exponential prior's e^A of the current step
+  double* sQ_;   // shared Q (nc x nc); the exponential prior's Gramian factor
+  const double* sD_;  // exponential prior: the drift's last d rows (d x nc, shared)
+  int expo;           // exponential prior (dense)
+  double* ws;    // instance workspace base
+  double* prm_;  // vector-field parameters (LDS copy)
   // probe set of the Monte Carlo Jacobian handler (accepted steps so far), set
   // by the solve loop before each step
   mutable uint32_t lin_set = 0;
-  __device__ double* slot(int s) const { return ws + s * L->s_size; }
-  __device__ double* W(int off) const { return ws + off; }
+  // every pointer the context holds is into LDS
+  __device__ double* slot(int s) const { return lds_ptr(ws + s * L->s_size); }
+  __device__ double* W(int off) const { return lds_ptr(ws + off); }
+  __device__ double* sA() const { return lds_ptr(sA_); }
+  __device__ double* sQ() const { return lds_ptr(sQ_); }
+  __device__ const double* sD() const { return lds_ptr(sD_); }
+  __device__ double* prm() const { return lds_ptr(prm_); }
 };
 
 // ------------------------------------------------------------- primitives
@@ -286,8 +311,8 @@ template <int G>
 __device__ void exp_transition(const Ctx<G>& c, double dt, const double* P, const double* PI) {
   const Layout& L = *c.L;
   const int n = c.nc, d = c.d, N = c.N, m10 = 10 * d;
-  double* eA = c.sA;
-  double* U = c.sQ;
+  double* eA = c.sA();
+  double* U = c.sQ();
   double* Ap = c.W(L.w_xa);
   double* Bp = c.W(L.w_xB);
   double* A2 = c.W(L.w_xA2);
@@ -305,7 +330,7 @@ __device__ void exp_transition(const Ctx<G>& c, double dt, const double* P, cons
   const double base = c.a->cfg.base_output_scale;
   for (int e = c.g.l; e < n * n; e += G) {
     const int i = e / n, j = e % n;
-    const double a = i < n - d ? (j == i + d ? 1.0 : 0.0) : c.sD[(i - (n - d)) * n + j];
+    const double a = i < n - d ? (j == i + d ? 1.0 : 0.0) : c.sD()[(i - (n - d)) * n + j];
     Ap[e] = ((dt * PI[i]) * a) * P[j];
   }
   for (int e = c.g.l; e < n * d; e += G) {
@@ -442,7 +467,9 @@ __device__ void exp_transition(const Ctx<G>& c, double dt, const double* P, cons
 // preconditioner_taylor (utilities.py:74-84) expanded to nc entries; with the
 // exponential prior also that step's transition (exp_transition).
 template <int G>
-__device__ void precon(const Ctx<G>& c, double dt, double* P, double* PI) {
+__device__ void precon(const Ctx<G>& c, double dt, double* P_, double* PI_) {
+  double* const P = lds_ptr(P_);
+  double* const PI = lds_ptr(PI_);
   for (int i = c.g.l; i < c.nc; i += G) {
     const int deriv = c.dense ? i / c.d : i;
     const int pw = c.N - 1 - deriv;
@@ -477,10 +504,10 @@ __device__ void apply_flat_mean(const Ctx<G>& c, const double* P, const double* 
     const int i = e / mc, col = e % mc;
     double acc = 0.0;
     if (c.dense) {
-      for (int k = 0; k < nc; ++k) acc += c.sA[i * nc + k] * (PI[k] * x[k * mc + col]);
+      for (int k = 0; k < nc; ++k) acc += c.sA()[i * nc + k] * (PI[k] * x[k * mc + col]);
       out[e] = P[i] * (acc + 0.0);
     } else {
-      for (int k = 0; k < nc; ++k) acc += (P[i] * c.sA[i * nc + k]) * (PI[k] * x[k * mc + col]);
+      for (int k = 0; k < nc; ++k) acc += (P[i] * c.sA()[i * nc + k]) * (PI[k] * x[k * mc + col]);
       out[e] = acc + 0.0;
     }
   }
@@ -492,7 +519,7 @@ __device__ void apply_flat_chol(const Ctx<G>& c, const double* P, double s, doub
   const int nc = c.nc;
   for (int e = c.g.l; e < nc * nc; e += G) {
     const int i = e / nc;
-    out[e] = fabs(P[i]) * (s * c.sQ[e]);
+    out[e] = fabs(P[i]) * (s * c.sQ()[e]);
   }
 }
 
@@ -506,14 +533,14 @@ __device__ void marginalise_prior(const Ctx<G>& c, const double* P, const double
   for (int e = c.g.l; e < nc * nc; e += G) {
     const int r = e / nc, col = e % nc;  // big[r][col] = (A chol_l)[col][r]
     double acc = 0.0;
-    for (int k = 0; k < nc; ++k) acc += c.sA[col * nc + k] * (PI[k] * XL[k * nc + r]);
+    for (int k = 0; k < nc; ++k) acc += c.sA()[col * nc + k] * (PI[k] * XL[k * nc + r]);
     big[r * nc + col] = acc;
-    big[(nc + r) * nc + col] = s * c.sQ[col * nc + r];
+    big[(nc + r) * nc + col] = s * c.sQ()[col * nc + r];
   }
   for (int e = c.g.l; e < nc * mc; e += G) {
     const int i = e / mc, col = e % mc;
     double acc = 0.0;
-    for (int k = 0; k < nc; ++k) acc += c.sA[i * nc + k] * (PI[k] * Xm[k * mc + col]);
+    for (int k = 0; k < nc; ++k) acc += c.sA()[i * nc + k] * (PI[k] * Xm[k * mc + col]);
     Ym[e] = P[i] * (acc + 0.0);
   }
   c.g.sync();
@@ -534,10 +561,10 @@ __device__ void revert_prior(const Ctx<G>& c, const double* P, const double* PI,
   // [[R_YX, 0], [R_X_F, R_X]]; R_YX = (sQ)^T, R_X_F = (A chol_l)^T, R_X = chol_l^T
   for (int e = c.g.l; e < nc * nc; e += G) {
     const int r = e / nc, col = e % nc;
-    big[r * n2 + col] = s * c.sQ[col * nc + r];
+    big[r * n2 + col] = s * c.sQ()[col * nc + r];
     big[r * n2 + nc + col] = 0.0;
     double acc = 0.0;
-    for (int k = 0; k < nc; ++k) acc += c.sA[col * nc + k] * (fabs(PI[k]) * XL[k * nc + r]);
+    for (int k = 0; k < nc; ++k) acc += c.sA()[col * nc + k] * (fabs(PI[k]) * XL[k * nc + r]);
     big[(nc + r) * n2 + col] = acc;
     big[(nc + r) * n2 + nc + col] = fabs(PI[col]) * XL[col * nc + r];
   }
@@ -545,7 +572,7 @@ __device__ void revert_prior(const Ctx<G>& c, const double* P, const double* PI,
   for (int e = c.g.l; e < nc * mc; e += G) {
     const int i = e / mc, col = e % mc;
     double acc = 0.0;
-    for (int k = 0; k < nc; ++k) acc += c.sA[i * nc + k] * (PI[k] * Xm[k * mc + col]);
+    for (int k = 0; k < nc; ++k) acc += c.sA()[i * nc + k] * (PI[k] * Xm[k * mc + col]);
     tmp[e] = acc + 0.0;
   }
   c.g.sync();
@@ -669,8 +696,12 @@ __device__ __forceinline__ int sel_row(const Ctx<G>& c, int obs) {
 // ssm_impl_isotropic.py:298-317): writes b (ko x mc) and, for ts1, J (ko x nc).
 // With tangent columns (MODE_LML) b also carries d b / d theta.
 template <int G>
-__device__ void linearize(const Ctx<G>& c, const double* Xm, double t, double* b, double* J, double* y0buf,
+__device__ void linearize(const Ctx<G>& c, const double* Xm_, double t, double* b_, double* J_, double* y0buf_,
                           uint32_t set_offset = 0) {
+  const double* const Xm = lds_ptr(Xm_);
+  double* const b = lds_ptr(b_);
+  double* const J = lds_ptr(J_);
+  double* const y0buf = lds_ptr(y0buf_);
   const int d = c.d, mc = c.mc;
   const int id = c.a->cfg.vector_field;
   // gather u(t) (zeroth Taylor coefficient) and, with tangents, its tangents
@@ -686,12 +717,12 @@ __device__ void linearize(const Ctx<G>& c, const double* Xm, double t, double* b
       const int q = k / d, j = k % d;
       double v;
       if (q == 0) {
-        v = -vf_component(id, d, y, c.prm, t, j);
+        v = -vf_component(id, d, y, c.prm(), t, j);
       } else {
         double acc = 0.0;
-        for (int jj = 0; jj < d; ++jj) acc += vf_jac(id, d, y, c.prm, j, jj) * y[q * d + jj];
+        for (int jj = 0; jj < d; ++jj) acc += vf_jac(id, d, y, c.prm(), j, jj) * y[q * d + jj];
         const int qq = q - 1;
-        if (qq < c.a->cfg.n_params) acc += vf_jac_param(id, d, y, c.prm, j, qq);
+        if (qq < c.a->cfg.n_params) acc += vf_jac_param(id, d, y, c.prm(), j, qq);
         v = -acc;
       }
       if (c.dense)
@@ -710,14 +741,14 @@ __device__ void linearize(const Ctx<G>& c, const double* Xm, double t, double* b
     if (c.a->cfg.jacobian != PDQ_JAC_EXACT) {
       const bool fwd = c.a->cfg.jacobian == PDQ_JAC_MONTE_CARLO_FWD;
       for (int j = 0; j < d; ++j)
-        tr += -vf_mc_diag(id, d, y, c.prm, j, c.a->cfg.jac_seed, c.lin_set + set_offset, c.a->cfg.jac_probes, fwd);
+        tr += -vf_mc_diag(id, d, y, c.prm(), j, c.a->cfg.jac_seed, c.lin_set + set_offset, c.a->cfg.jac_probes, fwd);
     } else {
-      for (int j = 0; j < d; ++j) tr += -vf_jac(id, d, y, c.prm, j, j);
+      for (int j = 0; j < d; ++j) tr += -vf_jac(id, d, y, c.prm(), j, j);
     }
     const double h0 = tr / (double)d;
     for (int k = c.g.l; k < c.nc; k += G) J[k] = k == 0 ? h0 : (k == 1 ? 1.0 : 0.0);
     for (int j = c.g.l; j < d; j += G) {
-      const double r = Xm[d + j] - vf_component(id, d, y, c.prm, t, j);
+      const double r = Xm[d + j] - vf_component(id, d, y, c.prm(), t, j);
       b[j] = r - (h0 * Xm[j] + Xm[d + j]);
     }
   } else {
@@ -726,13 +757,13 @@ __device__ void linearize(const Ctx<G>& c, const double* Xm, double t, double* b
     for (int e = c.g.l; e < d * nc; e += G) {
       const int i = e / nc, j = e % nc;
       double v = 0.0;
-      if (j < d) v = -vf_jac(id, d, y, c.prm, i, j);
+      if (j < d) v = -vf_jac(id, d, y, c.prm(), i, j);
       else if (j == d + i) v = 1.0;
       J[e] = v;
     }
     c.g.sync();
     for (int i = c.g.l; i < d; i += G) {
-      const double f = vf_component(id, d, y, c.prm, t, i);
+      const double f = vf_component(id, d, y, c.prm(), t, i);
       double jm = 0.0;
       for (int k = 0; k < nc; ++k) jm += J[i * nc + k] * Xm[k * mc];
       b[i * mc] = (Xm[(d + i) * mc] - f) - jm;
@@ -846,8 +877,11 @@ __device__ void update(const Ctx<G>& c, const double* b, const double* J, const 
 // error_residual_std.estimate_error_norm (solvers.py:925-996) given the
 // observed marginal of the error model and its local scale sigma.
 template <int G>
-__device__ double error_power(const Ctx<G>& c, const double* oL, double sigma, const double* prev_m,
-                              const double* prop_m, double dt) {
+__device__ double error_power(const Ctx<G>& c, const double* oL_, double sigma, const double* prev_m_,
+                              const double* prop_m_, double dt) {
+  const double* const oL = lds_ptr(oL_);
+  const double* const prev_m = lds_ptr(prev_m_);
+  const double* const prop_m = lds_ptr(prop_m_);
   const int ko = c.ko, d = c.d;
   const pdq_config& cfg = c.a->cfg;
   // std of rescaled observed rows: dense per-row qr_r (|beta| of dlarfg),
@@ -910,9 +944,12 @@ __device__ __forceinline__ double row_std(const Ctx<G>& c, const double* L, int 
 // calibrated std of derivative idx of the result (d values dense, one
 // isotropic), written to errs.  Xm, XL, mo, oL are scratch.
 template <int G>
-__device__ void state_std_errors(const Ctx<G>& c, const double* b, const double* J, const double* Um, const double* UL,
-                                 double* Xm, double* XL, double* big, double* gain, double* mo, double* oL,
-                                 double* errs) {
+__device__ void state_std_errors(const Ctx<G>& c, const double* b_, const double* J_, const double* Um_,
+                                 const double* UL_, double* Xm_, double* XL_, double* big_, double* gain_, double* mo_,
+                                 double* oL_, double* errs_) {
+  const double *const b = lds_ptr(b_), *const J = lds_ptr(J_), *const Um = lds_ptr(Um_), *const UL = lds_ptr(UL_);
+  double *const Xm = lds_ptr(Xm_), *const XL = lds_ptr(XL_), *const big = lds_ptr(big_), *const gain = lds_ptr(gain_);
+  double *const mo = lds_ptr(mo_), *const oL = lds_ptr(oL_), *const errs = lds_ptr(errs_);
   update(c, b, J, Um, UL, Xm, XL, big, gain, mo, oL);
   const double os = rms_zero(c, mo, oL);
   const int idx = c.a->cfg.error_derivative_idx;
@@ -924,8 +961,11 @@ __device__ void state_std_errors(const Ctx<G>& c, const double* b, const double*
 // The error power of error_state_std: errs (d values dense, one isotropic) at
 // order n = idx (+1 per unit step), referenced to derivative idx.
 template <int G>
-__device__ double error_power_state(const Ctx<G>& c, const double* errs, const double* prev_m, const double* prop_m,
-                                    double dt) {
+__device__ double error_power_state(const Ctx<G>& c, const double* errs_, const double* prev_m_,
+                                    const double* prop_m_, double dt) {
+  const double* const errs = lds_ptr(errs_);
+  const double* const prev_m = lds_ptr(prev_m_);
+  const double* const prop_m = lds_ptr(prop_m_);
   const pdq_config& cfg = c.a->cfg;
   const int d = c.d, idx = cfg.error_derivative_idx, ne = c.dense ? d : 1;
   const int nord = idx + (cfg.error_per_unit_step ? 1 : 0);
@@ -1405,7 +1445,7 @@ __device__ void lml_instance(const Ctx<G>& c, int inst) {
   const int ntan = mc / mcp - 1;
   const int np = cfg.n_params;
   const int K = a.K, every = a.every, Kobs = (K - 1) / every + 1;
-  for (int i = c.g.l; i < np; i += G) c.prm[i] = a.params[(size_t)inst * np + i];
+  for (int i = c.g.l; i < np; i += G) c.prm()[i] = a.params[(size_t)inst * np + i];
   int sf = 0, pr = 2;
   g_fill(c.g, c.slot(sf), 0.0, L.s_size);
   c.g.sync();
@@ -1621,7 +1661,7 @@ __device__ void solve_instance(const Ctx<G>& c, int inst) {
   const int K = c.a->K;
   const double* ts = c.a->times + (size_t)inst * K;
   // vector-field parameters
-  for (int i = c.g.l; i < cfg.n_params; i += G) c.prm[i] = c.a->params[(size_t)inst * cfg.n_params + i];
+  for (int i = c.g.l; i < cfg.n_params; i += G) c.prm()[i] = c.a->params[(size_t)inst * cfg.n_params + i];
   int sf = 0, ifr = 1, pr = 2;
   init_instance(c, inst, sf);
   copy_slot(c, ifr, sf);
@@ -1804,18 +1844,18 @@ __device__ Ctx<G> coop_setup(const KArgs& a, double* lds) {
   c.d = L.d;
   c.dense = a.cfg.factorisation == PDQ_DENSE;
   c.ws = lds + L.wg_shared + grp * L.inst_size;
-  c.prm = c.ws + L.w_prm;
+  c.prm_ = c.ws + L.w_prm;
   c.expo = L.expo;
-  c.sD = sD;
+  c.sD_ = sD;
   // the exponential prior's transition lives in the instance's workspace
-  c.sA = L.expo ? c.ws + L.w_xA : sA;
-  c.sQ = L.expo ? c.ws + L.w_xL : sQ;
+  c.sA_ = L.expo ? c.ws + L.w_xA : sA;
+  c.sQ_ = L.expo ? c.ws + L.w_xL : sQ;
   return c;
 }
 
 template <int G>
 __global__ void __launch_bounds__(64) offgrid_coop_kernel(OffgridCoopArgs o) {
-  extern __shared__ double lds[];
+  double* const lds = pdq_coop_lds;
   const Ctx<G> c = coop_setup<G>(o.k, lds);
   const size_t total = (size_t)o.k.n_inst * o.n_query;
   const size_t groups = (size_t)gridDim.x * (64 / G);
@@ -1824,7 +1864,7 @@ __global__ void __launch_bounds__(64) offgrid_coop_kernel(OffgridCoopArgs o) {
 
 template <int G>
 __global__ void __launch_bounds__(64) solve_kernel(KArgs args) {
-  extern __shared__ double lds[];
+  double* const lds = pdq_coop_lds;
   const KArgs& a = args;
   const Ctx<G> c = coop_setup<G>(a, lds);
   for (;;) {
@@ -1973,11 +2013,11 @@ inline Ctx<1> emu_ctx(const KArgs& a, double* shared, double* ws) {
   c.d = L.d;
   c.dense = a.cfg.factorisation == PDQ_DENSE;
   c.ws = ws;
-  c.prm = ws + L.w_prm;
+  c.prm_ = ws + L.w_prm;
   c.expo = L.expo;
-  c.sD = shared + 2 * nc * nc;
-  c.sA = L.expo ? ws + L.w_xA : shared;
-  c.sQ = L.expo ? ws + L.w_xL : shared + nc * nc;
+  c.sD_ = shared + 2 * nc * nc;
+  c.sA_ = L.expo ? ws + L.w_xA : shared;
+  c.sQ_ = L.expo ? ws + L.w_xL : shared + nc * nc;
   return c;
 }
 
