@@ -38,6 +38,8 @@ C2_MEASURED_TRAFFIC_BYTES = round((228_576.6 * 2 + 940_478.8) * 1024)
 C2_TRAFFIC_SOURCE = "profiles/lane_kernel_c2_isotropic_nu4_4096.md"
 C5_MEASURED_TRAFFIC_BYTES = round((121_083.3 * 2 + 240_058.0) * 1024)
 C5_TRAFFIC_SOURCE = "profiles/lane_lml_kernel_c5_isotropic_nu3_1024.md"
+# the same for the C5 batch's reverse adjoint (lane_adj_kernel, two wavefronts)
+C5_REVERSE_MEASURED_TRAFFIC_BYTES = round((326_791.3 * 2 + 1_155_472.0) * 1024)
 
 
 def c2_inputs(n_inst=4096, K=200, seed=0):
@@ -206,6 +208,8 @@ def bench_c5(args, torch, capi, flops):
         "frac": fl_r / (ms_r / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
         "flops_per_launch": fl_r,
         "algorithmic_bytes": flops.lml_reverse_bytes(cfg_r, P.shape[0], n_obs, grid.size),
+        "traffic": C5_REVERSE_MEASURED_TRAFFIC_BYTES if P.shape[0] == 1024 else None,
+        "traffic_source": C5_TRAFFIC_SOURCE,
         "ms_per_launch": ms_r,
     }
     # plain: central differences (h = 1e-6) of single-instance value launches, 13 per gradient

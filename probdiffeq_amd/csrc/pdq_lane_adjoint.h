@@ -32,7 +32,16 @@
 // Storage per lane, in HBM element-major across the launch's lanes:
 // conditionals (Kobs - 1) x |cond|, nodes Kobs x (N + 1 + D), adjoints of the
 // conditional means (Kobs - 1) x N D, checkpoints ceil((K - 1) / C) x |slot|,
-// and one segment of C slots.
+// and two segment buffers of C slots.
+//
+// Two-wavefront form (SPLIT).  Wavefront 0 runs the lane's whole adjoint
+// except the recomputation of the checkpoint segments; wavefront 1, on its own
+// two LDS slots, recomputes segment sg - 1 into one segment buffer while
+// wavefront 0 walks segment sg backwards from the other, and recomputes the
+// last segment while wavefront 0 runs the reverse scan and its adjoint.  The
+// workgroup barriers sit at phase boundaries that every lane reaches: lanes
+// without an instance of their own compute an existing one and write no
+// output.
 #pragma once
 
 #include "pdq_lane_lml.h"
@@ -61,7 +70,7 @@ struct AdjLayout {
     ckpt = o;
     o += nck * SL::size;
     seg = o;
-    o += (size_t)kAdjSegment * SL::size;
+    o += 2 * (size_t)kAdjSegment * SL::size;
     size = o;
   }
 };
@@ -93,6 +102,31 @@ template <int N, int D>
 __device__ __forceinline__ void adj_slot_in(const LaneWs<N, D>& w, int s, const double* p, size_t stride) {
 #pragma unroll
   for (int e = 0; e < SlotLayout<N, D>::size; ++e) w.at(s, e) = p[(size_t)e * stride];
+}
+
+// Recomputes checkpoint segment sg's slots (grid steps k0 .. k1 - 1, from the
+// checkpoint stored by the forward pass) into segment buffer `buf`, stepping
+// on the lane's LDS slots 0 and 1 as the forward pass did.
+template <int N, int D, class At>
+__device__ __forceinline__ void adj_recompute_segment(const LaneCtx<N, D>& c, const LmlArgs& a,
+                                                      const AdjLayout<N, D>& lay, At at, size_t n_lanes, int sg,
+                                                      int buf) {
+  using SL = SlotLayout<N, D>;
+  const LaneWs<N, D>& w = c.w;
+  const int K = a.K, every = a.every;
+  const int k0 = sg * kAdjSegment;
+  const int k1 = (k0 + kAdjSegment < K - 1) ? k0 + kAdjSegment : K - 1;
+  int sf = 0, pr = 1;
+  adj_slot_in<N, D>(w, sf, at(lay.ckpt + (size_t)sg * SL::size), n_lanes);
+  for (int k = k0; k < k1; ++k) {
+    adj_slot_out<N, D>(w, sf, at(lay.seg + ((size_t)buf * kAdjSegment + (k - k0)) * SL::size), n_lanes);
+    if (k + 1 == k1) break;
+    lane_lml_step<N, D, 0>(c, 0, sf, pr, a.times[k + 1] - a.times[k]);
+    const int tmp = sf;
+    sf = pr;
+    pr = tmp;
+    if ((k + 1) % every == 0) w.at(sf, LS_IDENT) = 1.0;
+  }
 }
 
 // The quantities of one forward step (from slot `from`, step h) that its
@@ -155,10 +189,13 @@ __device__ __forceinline__ void adj_recompute_step(const LaneCtx<N, D>& c, int f
 }
 
 // One lane: the forward pass with checkpoints, the reverse scan (value), the
-// scan's adjoint, and the backward sweep over the checkpoint segments.
-template <int N, int D>
+// scan's adjoint, and the backward sweep over the checkpoint segments.  With
+// SPLIT this is wavefront 0's part (lane_adj_recompute_role is wavefront 1's);
+// `active` is false for a lane that computes an instance it does not own (it
+// writes no output).
+template <int N, int D, bool SPLIT = false>
 __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs& a, int inst, double* slots,
-                                                  size_t lane, size_t n_lanes) {
+                                                  size_t lane, size_t n_lanes, bool active = true) {
   using CL = CondLayout<N, D>;
   using SL = SlotLayout<N, D>;
   const pdq_config& cfg = a.cfg;
@@ -206,6 +243,7 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
       w.at(sf, LS_IDENT) = 1.0;
     }
   }
+  if constexpr (SPLIT) __syncthreads();  // every checkpoint is stored: wavefront 1 starts recomputing
   // terminal marginal: posterior1.conditional.marginalise(posterior1.marginal)
   const bool term_ident = w.at(sf, LS_IDENT) != 0.0;
   double Xm[N * D], XL[N * N];
@@ -301,7 +339,7 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
 #pragma unroll
     for (int e = 0; e < N * N; ++e) XL[e] = YL[e];
   }
-  if (a.value) a.value[inst] = lp;
+  if (a.value && active) a.value[inst] = lp;
   if (!a.grad) return;
   // ---- adjoint of the scan, forward over the nodes
   const double cw = a.average_pdfs ? 1.0 / (double)Kobs : 1.0;  // d lml / d logpdf_j
@@ -382,17 +420,11 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int k0 = sg * kAdjSegment;
     const int k1 = (k0 + kAdjSegment < K - 1) ? k0 + kAdjSegment : K - 1;
-    // recompute the segment's slots from its checkpoint
-    adj_slot_in<N, D>(w, sf, at(lay.ckpt + (size_t)sg * SL::size), n_lanes);
-    for (int k = k0; k < k1; ++k) {
-      adj_slot_out<N, D>(w, sf, at(lay.seg + (size_t)(k - k0) * SL::size), n_lanes);
-      if (k + 1 == k1) break;
-      lane_lml_step<N, D, 0>(c, 0, sf, pr, a.times[k + 1] - a.times[k]);
-      const int tmp = sf;
-      sf = pr;
-      pr = tmp;
-      if ((k + 1) % every == 0) w.at(sf, LS_IDENT) = 1.0;
-    }
+    // the segment's slots from its checkpoint: recomputed here, or by
+    // wavefront 1 into buffer sg & 1 before this barrier
+    const int buf = SPLIT ? (sg & 1) : 0;
+    if constexpr (SPLIT) __syncthreads();
+    else adj_recompute_segment<N, D>(c, a, lay, at, n_lanes, sg, 0);
     for (int k = k1 - 1; k >= k0; --k) {
       // after this step a reset stored the running conditional as interval j's:
       // its adjoint comes from the scan, and the identity behind it has none
@@ -401,7 +433,7 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
 #pragma unroll
         for (int e = 0; e < N * D; ++e) cbar[e] = xb[(size_t)e * n_lanes];
       }
-      adj_slot_in<N, D>(w, sf, at(lay.seg + (size_t)(k - k0) * SL::size), n_lanes);
+      adj_slot_in<N, D>(w, sf, at(lay.seg + ((size_t)buf * kAdjSegment + (k - k0)) * SL::size), n_lanes);
       AdjStep<N, D> s;
       adj_recompute_step<N, D>(c, sf, a.times[k + 1] - a.times[k], s);
       const bool ident = w.at(sf, LS_IDENT) != 0.0;
@@ -490,15 +522,50 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
     double g = q < np ? tbar[q] : 0.0;
 #pragma unroll
     for (int e = 0; e < N * D; ++e) g += mbar[e] * tc[(size_t)e * (1 + ntan) + 1 + q];
-    a.grad[(size_t)inst * ntan + q] = g;
+    if (active) a.grad[(size_t)inst * ntan + q] = g;
+  }
+}
+
+// Wavefront 1 of the two-wavefront adjoint: the recomputation of the
+// checkpoint segments, one segment ahead of wavefront 0's backward walk, on
+// the lane's own LDS slots (`slots`), for the same instance and HBM column as
+// its partner lane of wavefront 0.  Its barriers pair with
+// lane_adj_instance<N, D, true>'s.
+template <int N, int D>
+__device__ __forceinline__ void lane_adj_recompute_role(const LArgs& la, const LmlArgs& a, int inst, double* slots,
+                                                        size_t lane, size_t n_lanes) {
+  LaneCtx<N, D> c;
+  c.a = &la;
+  c.inst = inst;
+  c.prm = a.params + (size_t)inst * a.cfg.n_params;
+  c.w.base = slots;
+  c.lin_set = 0;
+  const AdjLayout<N, D> lay(a.K, a.every);
+  double* const base = a.scratch + lane;
+  auto at = [&](size_t off) -> double* { return base + off * n_lanes; };
+  const int nseg = (a.K - 2) / kAdjSegment + 1;
+  __syncthreads();  // the forward pass has stored every checkpoint
+  if (!a.grad) return;
+  adj_recompute_segment<N, D>(c, a, lay, at, n_lanes, nseg - 1, (nseg - 1) & 1);
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    __syncthreads();  // segment sg is in its buffer: wavefront 0 walks it now
+    if (sg > 0) adj_recompute_segment<N, D>(c, a, lay, at, n_lanes, sg - 1, (sg - 1) & 1);
   }
 }
 
 #ifndef PDQ_HOST_EMU
-// One lane per instance, grid-stride over instances; slots in LDS as in the
-// forward LML kernel.
-template <int N, int D, int SOLVER>
-__global__ void __launch_bounds__(64) lane_adj_kernel(LmlArgs args) {
+// Bytes of LDS of the adjoint kernel: two slots per lane of each wavefront.
+template <int N, int D, bool SPLIT>
+constexpr size_t lane_adj_lds_bytes() {
+  return sizeof(double) * kLaneBlock * 2 * SlotLayout<N, D>::size * (SPLIT ? 2 : 1);
+}
+
+// One lane per instance (the grid covers every instance once); slots in LDS
+// as in the forward LML kernel.  With SPLIT the workgroup is two wavefronts:
+// lane l of wavefront 0 runs the adjoint of instance blockIdx.x * 64 + l and
+// lane l of wavefront 1 recomputes its checkpoint segments on its own slots.
+template <int N, int D, int SOLVER, bool SPLIT>
+__global__ void __launch_bounds__(SPLIT ? 2 * kLaneBlock : kLaneBlock) lane_adj_kernel(LmlArgs args) {
   extern __shared__ double lds[];
   LmlArgs a = args;
   a.cfg.strategy = PDQ_FIXEDPOINT;
@@ -509,10 +576,24 @@ __global__ void __launch_bounds__(64) lane_adj_kernel(LmlArgs args) {
   la.n_inst = a.n_inst;
   la.K = a.K;
   la.Q = a.Q;
-  double* const slots = lds + threadIdx.x;
-  const size_t n_lanes = (size_t)gridDim.x * blockDim.x;
-  const size_t lane = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (size_t inst = lane; inst < (size_t)a.n_inst; inst += n_lanes) lane_adj_instance<N, D>(la, a, (int)inst, slots, lane, n_lanes);
+  if constexpr (SPLIT) {
+    const int l = threadIdx.x & (kLaneBlock - 1);
+    const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / kLaneBlock);
+    double* const slots = lds + l + (role ? 2 * SlotLayout<N, D>::size * kLaneBlock : 0);
+    const size_t n_lanes = (size_t)gridDim.x * kLaneBlock;
+    const size_t lane = (size_t)blockIdx.x * kLaneBlock + l;
+    // a lane past the last instance computes the last one (uniform control
+    // flow for the barriers) and writes nothing
+    const bool active = lane < (size_t)a.n_inst;
+    const int inst = active ? (int)lane : a.n_inst - 1;
+    if (role == 0) lane_adj_instance<N, D, true>(la, a, inst, slots, lane, n_lanes, active);
+    else lane_adj_recompute_role<N, D>(la, a, inst, slots, lane, n_lanes);
+  } else {
+    double* const slots = lds + threadIdx.x;
+    const size_t n_lanes = (size_t)gridDim.x * blockDim.x;
+    const size_t lane = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t inst = lane; inst < (size_t)a.n_inst; inst += n_lanes) lane_adj_instance<N, D>(la, a, (int)inst, slots, lane, n_lanes);
+  }
 }
 #endif
 

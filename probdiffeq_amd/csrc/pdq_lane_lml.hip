@@ -59,8 +59,11 @@ int launch_lml_shape(LmlArgs& a, int tpl, hipStream_t st) {
 
 template <int N, int D, int S>
 int launch_adj_nd(LmlArgs& a, hipStream_t st) {
-  auto kern = lane_adj_kernel<N, D, S>;
-  const size_t lds = lane_lml_lds_bytes<N, D>();
+  // two wavefronts per workgroup (the segment recomputation beside the
+  // backward walk) where their four slots per lane fit the LDS
+  constexpr bool split = lane_adj_lds_bytes<N, D, true>() <= 160 * 1024;
+  auto kern = lane_adj_kernel<N, D, S, split>;
+  const size_t lds = lane_adj_lds_bytes<N, D, split>();
   static std::atomic<int> done[kLmlMaxDevices];
   int dev = 0;
   PDQ_HIP_TRY(hipGetDevice(&dev));
@@ -69,9 +72,9 @@ int launch_adj_nd(LmlArgs& a, hipStream_t st) {
     if (dev >= 0 && dev < kLmlMaxDevices) done[dev].store(1);
   }
   // one lane per instance: the grid covers every instance once (the scratch
-  // buffer has a column per lane of this grid)
+  // buffer has a column per lane of wavefront 0 of this grid)
   const int grid = (a.n_inst + 63) / 64;
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, a);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(split ? 128 : 64), lds, st, a);
   PDQ_HIP_TRY(hipGetLastError());
   return PDQ_OK;
 }
