@@ -34,14 +34,16 @@
 // conditional means (Kobs - 1) x N D, checkpoints ceil((K - 1) / C) x |slot|,
 // and two segment buffers of C slots.
 //
-// Two-wavefront form (SPLIT).  Wavefront 0 runs the lane's whole adjoint
-// except the recomputation of the checkpoint segments; wavefront 1, on its own
-// two LDS slots, recomputes segment sg - 1 into one segment buffer while
-// wavefront 0 walks segment sg backwards from the other, and recomputes the
-// last segment while wavefront 0 runs the reverse scan and its adjoint.  The
-// workgroup barriers sit at phase boundaries that every lane reaches: lanes
-// without an instance of their own compute an existing one and write no
-// output.
+// Two-wavefront form (SPLIT).  The forward pass is the LML kernel's split one
+// (lane_lml_forward_split: wavefront 1 runs each step's merge and the copy of
+// each observation interval's conditional beside wavefront 0's next step;
+// three slots rotate, the third in wavefront 1's region of the LDS, and
+// wavefront 0 stores the checkpoints).  Then wavefront 1, on its own two LDS
+// slots, recomputes segment sg - 1 into one segment buffer while wavefront 0
+// walks segment sg backwards from the other, and recomputes the last segment
+// while wavefront 0 runs the reverse scan and its adjoint.  The workgroup
+// barriers sit at phase boundaries that every lane reaches: lanes without an
+// instance of their own compute an existing one and write no output.
 #pragma once
 
 #include "pdq_lane_lml.h"
@@ -56,6 +58,7 @@ struct AdjLayout {
   using CL = CondLayout<N, D>;
   using SL = SlotLayout<N, D>;
   static constexpr int node = N + 1 + D;  // gain, r00, whitened residuals
+  static constexpr bool conds_first = true;  // conds sits at offset 0 (the split forward pass relies on it)
   size_t conds, nodes, xibar, ckpt, seg, size;
   __device__ __host__ AdjLayout(int K, int every) {
     const size_t kobs = (size_t)(K - 1) / every + 1;
@@ -190,12 +193,14 @@ __device__ __forceinline__ void adj_recompute_step(const LaneCtx<N, D>& c, int f
 
 // One lane: the forward pass with checkpoints, the reverse scan (value), the
 // scan's adjoint, and the backward sweep over the checkpoint segments.  With
-// SPLIT this is wavefront 0's part (lane_adj_recompute_role is wavefront 1's);
-// `active` is false for a lane that computes an instance it does not own (it
-// writes no output).
+// SPLIT this is wavefront 0's part (lane_adj_recompute_role is wavefront 1's),
+// `req` the workgroup's request array of the split forward pass; `active` is
+// false for a lane that computes an instance it does not own (it writes no
+// output).
 template <int N, int D, bool SPLIT = false>
 __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs& a, int inst, double* slots,
-                                                  size_t lane, size_t n_lanes, bool active = true) {
+                                                  size_t lane, size_t n_lanes, bool active = true,
+                                                  int* req = nullptr) {
   using CL = CondLayout<N, D>;
   using SL = SlotLayout<N, D>;
   const pdq_config& cfg = a.cfg;
@@ -231,19 +236,27 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
     w.at(sf, LS_IDENT) = 1.0;
   }
   // ---- forward pass: checkpoints every kAdjSegment steps, interval conditionals
-  for (int k = 0; k < K - 1; ++k) {
-    if (k % kAdjSegment == 0) adj_slot_out<N, D>(w, sf, at(lay.ckpt + (size_t)(k / kAdjSegment) * SL::size), n_lanes);
-    lane_lml_step<N, D, 0>(c, 0, sf, pr, a.times[k + 1] - a.times[k]);
-    const int tmp = sf;
-    sf = pr;
-    pr = tmp;
-    if ((k + 1) % every == 0) {
-      const CondStrided<N, D> dst{at(lay.conds + (size_t)((k + 1) / every - 1) * CL::size), n_lanes};
-      cond_copy<N, D>(dst, w.cond(sf));
-      w.at(sf, LS_IDENT) = 1.0;
+  if constexpr (SPLIT) {
+#ifndef PDQ_HOST_EMU
+    // the interval conditionals land where the LML kernel puts them: lay.conds
+    // is the storage's first region
+    static_assert(AdjLayout<N, D>::conds_first, "the split forward pass writes the conditionals at offset 0");
+    sf = lane_lml_forward_split<N, D, 0>(c, a, 0, lane, n_lanes, true, req, kAdjSegment, at(lay.ckpt));
+#endif
+  } else {
+    for (int k = 0; k < K - 1; ++k) {
+      if (k % kAdjSegment == 0) adj_slot_out<N, D>(w, sf, at(lay.ckpt + (size_t)(k / kAdjSegment) * SL::size), n_lanes);
+      lane_lml_step<N, D, 0>(c, 0, sf, pr, a.times[k + 1] - a.times[k]);
+      const int tmp = sf;
+      sf = pr;
+      pr = tmp;
+      if ((k + 1) % every == 0) {
+        const CondStrided<N, D> dst{at(lay.conds + (size_t)((k + 1) / every - 1) * CL::size), n_lanes};
+        cond_copy<N, D>(dst, w.cond(sf));
+        w.at(sf, LS_IDENT) = 1.0;
+      }
     }
   }
-  if constexpr (SPLIT) __syncthreads();  // every checkpoint is stored: wavefront 1 starts recomputing
   // terminal marginal: posterior1.conditional.marginalise(posterior1.marginal)
   const bool term_ident = w.at(sf, LS_IDENT) != 0.0;
   double Xm[N * D], XL[N * N];
@@ -267,6 +280,13 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
     } else {
       lane_marginalise_cond<N, D>(fc, Sm, SLf, Xm, XL);
     }
+  }
+  if constexpr (SPLIT) {
+    // every checkpoint is stored and the terminal slot read (with SPLIT it may
+    // be the third slot, in wavefront 1's region): wavefront 1 starts
+    // recomputing; wavefront 0 walks on slots 0 and 1 from here on
+    __syncthreads();
+    sf = 0;
   }
   // ---- reverse scan (evaluate_lml): value, and each node's gain, factor, residuals
   double lp = 0.0, num = 0.0;
@@ -526,25 +546,30 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
   }
 }
 
-// Wavefront 1 of the two-wavefront adjoint: the recomputation of the
+// Wavefront 1 of the two-wavefront adjoint: the merges of the split forward
+// pass on wavefront 0's slots (`fwd_slots`), then the recomputation of the
 // checkpoint segments, one segment ahead of wavefront 0's backward walk, on
 // the lane's own LDS slots (`slots`), for the same instance and HBM column as
 // its partner lane of wavefront 0.  Its barriers pair with
 // lane_adj_instance<N, D, true>'s.
 template <int N, int D>
-__device__ __forceinline__ void lane_adj_recompute_role(const LArgs& la, const LmlArgs& a, int inst, double* slots,
-                                                        size_t lane, size_t n_lanes) {
+__device__ __forceinline__ void lane_adj_recompute_role(const LArgs& la, const LmlArgs& a, int inst, double* fwd_slots,
+                                                        double* slots, size_t lane, size_t n_lanes, int* req) {
   LaneCtx<N, D> c;
   c.a = &la;
   c.inst = inst;
   c.prm = a.params + (size_t)inst * a.cfg.n_params;
-  c.w.base = slots;
+  c.w.base = fwd_slots;
   c.lin_set = 0;
+#ifndef PDQ_HOST_EMU
+  lane_lml_forward_split<N, D, 0>(c, a, 0, lane, n_lanes, true, req);
+#endif
+  c.w.base = slots;
   const AdjLayout<N, D> lay(a.K, a.every);
   double* const base = a.scratch + lane;
   auto at = [&](size_t off) -> double* { return base + off * n_lanes; };
   const int nseg = (a.K - 2) / kAdjSegment + 1;
-  __syncthreads();  // the forward pass has stored every checkpoint
+  __syncthreads();  // the checkpoints are stored and wavefront 0 has read the terminal slot
   if (!a.grad) return;
   adj_recompute_segment<N, D>(c, a, lay, at, n_lanes, nseg - 1, (nseg - 1) & 1);
   for (int sg = nseg - 1; sg >= 0; --sg) {
@@ -554,10 +579,12 @@ __device__ __forceinline__ void lane_adj_recompute_role(const LArgs& la, const L
 }
 
 #ifndef PDQ_HOST_EMU
-// Bytes of LDS of the adjoint kernel: two slots per lane of each wavefront.
+// Bytes of LDS of the adjoint kernel: two slots per lane of each wavefront,
+// and with SPLIT the forward pass's request array (two ints per lane).
 template <int N, int D, bool SPLIT>
 constexpr size_t lane_adj_lds_bytes() {
-  return sizeof(double) * kLaneBlock * 2 * SlotLayout<N, D>::size * (SPLIT ? 2 : 1);
+  return sizeof(double) * kLaneBlock * 2 * SlotLayout<N, D>::size * (SPLIT ? 2 : 1) +
+         (SPLIT ? 2 * sizeof(int) * kLaneBlock : 0);
 }
 
 // One lane per instance (the grid covers every instance once); slots in LDS
@@ -579,15 +606,19 @@ __global__ void __launch_bounds__(SPLIT ? 2 * kLaneBlock : kLaneBlock) lane_adj_
   if constexpr (SPLIT) {
     const int l = threadIdx.x & (kLaneBlock - 1);
     const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / kLaneBlock);
+    // wavefront 0's slots 0, 1 (and, in the forward pass, 2), then wavefront
+    // 1's slots 0, 1 (the forward pass's slot 2 among them), then the requests
+    double* const fwd_slots = lds + l;
     double* const slots = lds + l + (role ? 2 * SlotLayout<N, D>::size * kLaneBlock : 0);
+    int* const req = reinterpret_cast<int*>(lds + 4 * SlotLayout<N, D>::size * kLaneBlock);
     const size_t n_lanes = (size_t)gridDim.x * kLaneBlock;
     const size_t lane = (size_t)blockIdx.x * kLaneBlock + l;
     // a lane past the last instance computes the last one (uniform control
     // flow for the barriers) and writes nothing
     const bool active = lane < (size_t)a.n_inst;
     const int inst = active ? (int)lane : a.n_inst - 1;
-    if (role == 0) lane_adj_instance<N, D, true>(la, a, inst, slots, lane, n_lanes, active);
-    else lane_adj_recompute_role<N, D>(la, a, inst, slots, lane, n_lanes);
+    if (role == 0) lane_adj_instance<N, D, true>(la, a, inst, slots, lane, n_lanes, active, req);
+    else lane_adj_recompute_role<N, D>(la, a, inst, fwd_slots, slots, lane, n_lanes, req);
   } else {
     double* const slots = lds + threadIdx.x;
     const size_t n_lanes = (size_t)gridDim.x * blockDim.x;

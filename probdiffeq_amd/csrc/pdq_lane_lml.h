@@ -162,10 +162,15 @@ __device__ __forceinline__ void lane_lml_step(const LaneCtx<N, D * (1 + TPL)>& c
 // so the next prediction reads only the marginal of the merge's result slot
 // and reverts into the third slot.  Every lane takes the same K - 1 steps, so
 // both wavefronts pass 2 (K - 1) + 1 barriers; the last step's request is run
-// by wavefront 0 after the loop.  Returns step_from's slot.
+// by wavefront 0 after the loop.  Returns step_from's slot.  With ckpt_every
+// > 0 wavefront 0 also stores step_from's whole slot before every
+// ckpt_every-th step (checkpoint j, before step j ckpt_every, at ckpt + j |slot|
+// n_lanes, element-major with stride n_lanes), once the previous step's merge
+// has completed.
 template <int N, int D, int TPL>
 __device__ __forceinline__ int lane_lml_forward_split(const LaneCtx<N, D * (1 + TPL)>& c, const LmlArgs& a, int q0,
-                                                      size_t glane, size_t n_lanes, bool active, int* req) {
+                                                      size_t glane, size_t n_lanes, bool active, int* req,
+                                                      int ckpt_every = 0, double* ckpt = nullptr) {
   constexpr int MC = D * (1 + TPL);
   using CL = CondLayout<N, MC>;
   const LaneWs<N, MC>& w = c.w;
@@ -194,6 +199,12 @@ __device__ __forceinline__ int lane_lml_forward_split(const LaneCtx<N, D * (1 + 
       LmlStepVals<N, MC> v;
       if (active) lane_lml_step_part1<N, D, TPL>(c, q0, sf, pr, a.times[k] - a.times[k - 1], v);
       __syncthreads();
+      if (ckpt_every > 0 && active && (k - 1) % ckpt_every == 0) {
+        // part 1 wrote only pr; the merge into sf's conditional is complete
+        double* dst = ckpt + (size_t)((k - 1) / ckpt_every) * SlotLayout<N, MC>::size * n_lanes;
+#pragma unroll
+        for (int e = 0; e < SlotLayout<N, MC>::size; ++e) dst[(size_t)e * n_lanes] = w.at(sf, e);
+      }
       last[0] = 0;
       last[1] = -1;
       if (active) {
