@@ -92,9 +92,11 @@ __device__ void g_qr_r(const Grp<G>& g, double* A, int m, int n, int ld) {
     ss = g.sum(ss);
     if (ss == 0.0) continue;
     const double alpha = A[i * ld + i];
-    const double beta = -copysign(hypot(alpha, sqrt(ss)), alpha);
-    const double tau = (beta - alpha) / beta;
-    const double scal = 1.0 / (alpha - beta);
+    // beta != 0 and alpha - beta != 0 (ss > 0): the refined hardware
+    // estimates stand in for hypot and the divisions, as in the lane kernel
+    const double beta = -copysign(lane_sqrt_pos(fma(alpha, alpha, ss)), alpha);
+    const double tau = lane_div_nz(beta - alpha, beta, lane_rcp_nz(beta));
+    const double scal = lane_rcp_nz(alpha - beta);
     g.sync();
     for (int r = i + 1 + g.l; r < m; r += G) A[r * ld + i] *= scal;
     g.sync();

@@ -23,6 +23,48 @@ constexpr int kMaxTangents = 8;  // forward-mode directions of the LML gradient
 // Safety net that bounds every instance's loop (the reference has none).
 constexpr long kMaxTotalAttempts = 1L << 22;  // per solve: every persistent wave ends in bounded time
 
+// ------------------------------------------------- refined reciprocal and root
+// Reciprocal, quotient and square root of operands known to be finite and
+// non-zero, from the hardware estimates refined by FMA iterations.  The IEEE
+// sequences the compiler emits for 1/b and sqrt(x) are 9-15 instructions with
+// long dependency chains; the Householder reflectors and the scalings of the
+// hot loops need dozens of them per step.
+// * lane_rcp_nz: v_rcp_f64, then two Newton steps y += y (1 - b y), each an
+//   FMA pair; within an ulp of 1/b for finite non-zero b (zero or infinite b
+//   give NaN: callers pass such y only to a checked division).
+// * lane_sqrt_pos: v_rsq_f64, one Goldschmidt step for (sqrt x, 1/(2 sqrt x))
+//   and a final Newton correction of the root (Markstein, IA-64 and
+//   Elementary Functions, 2000, ch. 9); within an ulp of sqrt(x) for finite
+//   positive x.
+// * lane_div_nz: q = a / b from y = lane_rcp_nz(b): q = a y, corrected by one
+//   FMA step against the residual a - b q (Markstein); within an ulp for
+//   finite non-zero b and a finite quotient.
+// The host build (tests/emu) computes the IEEE operations.
+#ifdef PDQ_HOST_EMU
+__device__ __forceinline__ double lane_rcp_nz(double b) { return 1.0 / b; }
+__device__ __forceinline__ double lane_sqrt_pos(double x) { return sqrt(x); }
+#else
+__device__ __forceinline__ double lane_rcp_nz(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double lane_sqrt_pos(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  return fma(fma(-g, g, x), h, g);
+}
+#endif
+__device__ __forceinline__ double lane_div_nz(double a, double b, double y) {
+  const double q = a * y;
+  return fma(fma(-q, b, a), y, q);
+}
+
 // ------------------------------------------------------------ vector fields
 // The registry restates problems.py:283-295's u' = f(u, t) for the device.
 // Component-wise evaluation lets every lane of a group produce its own f_k.

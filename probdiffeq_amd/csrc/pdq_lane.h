@@ -36,47 +36,21 @@ namespace pdq {
 #define PDQ_PHASE_BOUNDARY() __builtin_amdgcn_sched_barrier(0)
 #endif
 
-// Reciprocal and square root of operands known to be finite and non-zero,
-// and the inverse N-th root, from the hardware estimates refined by FMA
-// iterations.  The IEEE sequences the compiler emits for 1/b and sqrt(x) are
-// 9-15 instructions with long dependency chains; the Householder reflectors,
-// the preconditioner's and conditionals' scalings and the gains need about 60
-// of them per iteration of the hot loop.
-// * lane_rcp_nz: v_rcp_f64, then two Newton steps y += y (1 - b y), each an
-//   FMA pair; within an ulp of 1/b for finite non-zero b (zero or infinite b
-//   give NaN: callers pass such y only to lane_div, which then divides).
-// * lane_sqrt_pos: v_rsq_f64, one Goldschmidt step for (sqrt x, 1/(2 sqrt x))
-//   and a final Newton correction of the root (Markstein, IA-64 and
-//   Elementary Functions, 2000, ch. 9); within an ulp of sqrt(x) for finite
-//   positive x.
+// The inverse N-th root, from a hardware estimate refined by FMA iterations
+// (the reciprocal and square root of the same kind, lane_rcp_nz,
+// lane_sqrt_pos and lane_div_nz, are in pdq_device.h, shared with the
+// cooperative kernel).
 // * lane_root_inv<N>: x^(-1/N) from a single-precision estimate of the
 //   scaled operand (x = xs 2^(N k), xs within single range), then two Newton
 //   steps on x y^N = 1, y += (y / N) (1 - x y^N); within a few ulps of
 //   pow(x, -1/N).  x = 0 gives +inf and x = +inf gives 0, as pow does.
-// The host build (tests/emu) computes the IEEE operations and pow.
+// The host build (tests/emu) computes pow.
 #ifdef PDQ_HOST_EMU
-__device__ __forceinline__ double lane_rcp_nz(double b) { return 1.0 / b; }
-__device__ __forceinline__ double lane_sqrt_pos(double x) { return sqrt(x); }
 template <int N>
 __device__ __forceinline__ double lane_root_inv(double x) {
   return pow(x, -1.0 / (double)N);
 }
 #else
-__device__ __forceinline__ double lane_rcp_nz(double b) {
-  double y = __builtin_amdgcn_rcp(b);
-  double e = fma(-b, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-b, y, 1.0);
-  return fma(y, e, y);
-}
-__device__ __forceinline__ double lane_sqrt_pos(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
-  double g = x * y, h = 0.5 * y;
-  const double r = fma(-g, h, 0.5);
-  g = fma(g, r, g);
-  h = fma(h, r, h);
-  return fma(fma(-g, g, x), h, g);
-}
 template <int N>
 __device__ __forceinline__ double lane_root_inv(double x) {
   const int k = __builtin_amdgcn_frexp_exp(x) / N;
@@ -93,14 +67,6 @@ __device__ __forceinline__ double lane_root_inv(double x) {
   return x == __builtin_inf() ? 0.0 : y;
 }
 #endif
-
-// q = a / b from y = lane_rcp_nz(b): q = a y, corrected by one FMA step
-// against the residual a - b q (Markstein); within an ulp for finite non-zero
-// b and a finite quotient.
-__device__ __forceinline__ double lane_div_nz(double a, double b, double y) {
-  const double q = a * y;
-  return fma(fma(-q, b, a), y, q);
-}
 
 // a / b given y = 1 / b (correctly rounded): q = a y, then one FMA
 // correction step against the exact residual a - b q (Markstein).  For a
