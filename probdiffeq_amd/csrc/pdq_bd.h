@@ -50,6 +50,7 @@ namespace pdq {
 // Largest block-diagonal dimension (one lane per block, <= 512 lanes per
 // instance).
 constexpr int kBdMaxDim = 512;
+constexpr int kBdRedStride = kBdMaxDim / 64;  // one group sum buffer: a partial per wavefront
 
 // Arguments of the block-diagonal kernel.
 struct BdArgs {
@@ -68,8 +69,14 @@ struct BdGroup {
   int G;       // lanes (blocks incl. padding)
   int lane;    // = block index
   bool multi;  // the group is the whole (multi-wavefront) workgroup
-  double* y;   // G doubles of LDS: the published zeroth coefficients
-  double* red; // multi: one partial sum per wavefront
+  double* y;   // 2 G doubles of LDS: the published zeroth coefficients, two buffers
+  double* red; // multi: one partial sum per wavefront, two buffers of kBdRedStride
+  // The buffer each of the next linearisation and group sum uses.  They
+  // alternate, so a write never meets a read of the same buffer from the
+  // previous call: between the two calls that share a buffer lies the other
+  // call's barrier, which every lane reaches after its reads.  Every lane
+  // makes the same calls, so the toggles stay uniform.
+  mutable int ybuf = 0, rbuf = 0;
 
   __device__ __forceinline__ void sync() const {
 #ifdef PDQ_HOST_EMU
@@ -110,11 +117,12 @@ struct BdGroup {
     for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m, 64);
     v = shfl(v, 0, 64);
     const int w = lane / 64, nw = G / 64;
-    sync();  // earlier readers of red are done
-    if ((lane & 63) == 0) red[w] = v;
+    double* const r = red + rbuf * kBdRedStride;
+    rbuf ^= 1;
+    if ((lane & 63) == 0) r[w] = v;
     sync();
     double s = 0.0;
-    for (int i = 0; i < nw; ++i) s += red[i];
+    for (int i = 0; i < nw; ++i) s += r[i];
     return s;
   }
 };
@@ -184,18 +192,19 @@ __device__ __forceinline__ void bd_linearize(const BdCtx<N, STRATEGY>& c, const 
                                              double& h0, uint32_t set_offset = 0) {
   const pdq_config& cfg = c.a->cfg;
   const int d = cfg.ode_dim, k = c.g.lane;
-  if (c.act) c.g.y[k] = Xm[0];
+  double* const y = c.g.y + c.g.ybuf * c.g.G;  // the buffers alternate: no barrier after the reads
+  c.g.ybuf ^= 1;
+  if (c.act) y[k] = Xm[0];
   c.g.sync();
   double f = 0.0, jkk = 0.0;
   if (c.act) {
-    f = vf_component(cfg.vector_field, d, c.g.y, c.prm, t, k);
+    f = vf_component(cfg.vector_field, d, y, c.prm, t, k);
     if (cfg.constraint == PDQ_TS1)
       jkk = cfg.jacobian != PDQ_JAC_EXACT
-                ? vf_mc_diag(cfg.vector_field, d, c.g.y, c.prm, k, cfg.jac_seed, c.lin_set + set_offset, cfg.jac_probes,
+                ? vf_mc_diag(cfg.vector_field, d, y, c.prm, k, cfg.jac_seed, c.lin_set + set_offset, cfg.jac_probes,
                              cfg.jacobian == PDQ_JAC_MONTE_CARLO_FWD)
-                : vf_jac(cfg.vector_field, d, c.g.y, c.prm, k, k);
+                : vf_jac(cfg.vector_field, d, y, c.prm, k, k);
   }
-  c.g.sync();  // every lane has read y before it is written again
   if (cfg.constraint != PDQ_TS1) {
     h0 = 0.0;
     b = -f;
@@ -723,7 +732,7 @@ __device__ __forceinline__ void bd_solve_instance(const BdArgs& a, const BdGroup
 // whole group shares, so each group leaves the loop as a unit.
 template <int N, int STRATEGY>
 __global__ void __launch_bounds__(64) bd_wave_kernel(BdArgs args) {
-  __shared__ double y[64];
+  __shared__ double y[2 * 64];
   BdArgs a = args;
   a.cfg.strategy = STRATEGY;
   const int G = a.G;
@@ -732,7 +741,7 @@ __global__ void __launch_bounds__(64) bd_wave_kernel(BdArgs args) {
   g.G = G;
   g.lane = threadIdx.x % G;
   g.multi = false;
-  g.y = y + (threadIdx.x / G) * G;
+  g.y = y + (threadIdx.x / G) * 2 * G;
   g.red = nullptr;
   const int stride = gridDim.x * gpw;
   for (int inst = blockIdx.x * gpw + threadIdx.x / G; inst < a.n_inst; inst += stride)
@@ -743,8 +752,8 @@ __global__ void __launch_bounds__(64) bd_wave_kernel(BdArgs args) {
 // stride over instances (uniform across the workgroup).
 template <int N, int STRATEGY>
 __global__ void __launch_bounds__(512) bd_block_kernel(BdArgs args) {
-  __shared__ double y[kBdMaxDim];
-  __shared__ double red[kBdMaxDim / 64];
+  __shared__ double y[2 * kBdMaxDim];
+  __shared__ double red[2 * kBdRedStride];
   BdArgs a = args;
   a.cfg.strategy = STRATEGY;
   BdGroup g;

@@ -59,7 +59,7 @@ namespace {
 template <int N, int S>
 void bd_emu_instance(const pdq::BdArgs& a, int inst) {
   const int G = a.G;
-  std::vector<double> y(G, 0.0), red(pdq::kBdMaxDim / 64, 0.0);
+  std::vector<double> y(2 * G, 0.0), red(2 * pdq::kBdRedStride, 0.0);
   std::barrier<> bar(G);
   g_group_barrier = &bar;
   std::vector<std::thread> th;
