@@ -240,7 +240,7 @@ __device__ __forceinline__ double bd_error_power(const BdCtx<N, STRATEGY>& c, do
     const double sr = c.g.sum(c.act ? ref * ref : 0.0);
     norm = (sqrt(sa) / sqrt(d)) / (cfg.atol + cfg.rtol * (sqrt(sr) / sqrt(d)));
   }
-  return pow(norm, -1.0 / (double)N);
+  return lane_root_inv<N>(norm);  // pow(norm, -1/N), as the lane kernel's error power takes it
 }
 
 // The transition's marginalise with the block-diagonal |to_latent| on the

@@ -36,37 +36,6 @@ namespace pdq {
 #define PDQ_PHASE_BOUNDARY() __builtin_amdgcn_sched_barrier(0)
 #endif
 
-// The inverse N-th root, from a hardware estimate refined by FMA iterations
-// (the reciprocal and square root of the same kind, lane_rcp_nz,
-// lane_sqrt_pos and lane_div_nz, are in pdq_device.h, shared with the
-// cooperative kernel).
-// * lane_root_inv<N>: x^(-1/N) from a single-precision estimate of the
-//   scaled operand (x = xs 2^(N k), xs within single range), then two Newton
-//   steps on x y^N = 1, y += (y / N) (1 - x y^N); within a few ulps of
-//   pow(x, -1/N).  x = 0 gives +inf and x = +inf gives 0, as pow does.
-// The host build (tests/emu) computes pow.
-#ifdef PDQ_HOST_EMU
-template <int N>
-__device__ __forceinline__ double lane_root_inv(double x) {
-  return pow(x, -1.0 / (double)N);
-}
-#else
-template <int N>
-__device__ __forceinline__ double lane_root_inv(double x) {
-  const int k = __builtin_amdgcn_frexp_exp(x) / N;
-  const double xs = __builtin_ldexp(x, -N * k);
-  double y = __builtin_ldexp((double)__builtin_amdgcn_exp2f(-__builtin_amdgcn_logf((float)xs) / (float)N), -k);
-#pragma unroll
-  for (int it = 0; it < 2; ++it) {
-    double t = y;
-#pragma unroll
-    for (int q = 1; q < N; ++q) t *= y;
-    y = fma(y * (1.0 / (double)N), fma(-x, t, 1.0), y);
-  }
-  y = x == 0.0 ? __builtin_inf() : y;
-  return x == __builtin_inf() ? 0.0 : y;
-}
-#endif
 
 // a / b given y = 1 / b (correctly rounded): q = a y, then one FMA
 // correction step against the exact residual a - b q (Markstein).  For a
@@ -291,54 +260,6 @@ template <int N>
 struct FxColumnRows {
   static constexpr bool nz(int, int r) { return r <= 1 || r == N; }
 };
-
-// preconditioner_taylor (utilities.py:74-84): dt^k with libm pow semantics.
-// dt^k for an exponent known at compile time, computed as an unevaluated sum
-// hi + lo of exact FMA products (double-double), then rounded once: the
-// result is within about half an ulp of dt^k, as a correctly rounded pow is,
-// at a few instructions instead of a general pow.  k = 0, 1, 2, -1 are single
-// correctly rounded operations.  A negative exponent is the reciprocal of the
-// double-double power: q = 1/hi refined by one FMA step against hi + lo.
-struct LaneDD {
-  double hi, lo;
-};
-
-template <int k>
-__device__ __forceinline__ LaneDD lane_dd_pow(double x) {
-  static_assert(k >= 1 && k <= 8, "lane_dd_pow covers the exponents of num_derivatives <= 7");
-  if constexpr (k == 1) {
-    return {x, 0.0};
-  } else if constexpr (k % 2 == 0) {
-    const LaneDD h = lane_dd_pow<k / 2>(x);
-    const double p = h.hi * h.hi;
-    return {p, fma(h.hi, h.hi, -p) + 2.0 * h.hi * h.lo};
-  } else {
-    const LaneDD h = lane_dd_pow<k - 1>(x);
-    const double p = h.hi * x;
-    return {p, fma(h.hi, x, -p) + h.lo * x};
-  }
-}
-
-template <int k>
-__device__ __forceinline__ double lane_ipow(double dt) {
-  if constexpr (k == 0) {
-    return 1.0;
-  } else if constexpr (k == 1) {
-    return dt;
-  } else if constexpr (k == 2) {
-    return dt * dt;
-  } else if constexpr (k == -1) {
-    return 1.0 / dt;
-  } else if constexpr (k > 0) {
-    const LaneDD p = lane_dd_pow<k>(dt);
-    return p.hi + p.lo;
-  } else {
-    const LaneDD p = lane_dd_pow<-k>(dt);
-    const double q = 1.0 / p.hi;
-    const double r = fma(-p.hi, q, 1.0) - p.lo * q;
-    return isfinite(q) ? fma(q, r, q) : q;  // dt^k = 0 or overflowed: 1/hi as pow gives it
-  }
-}
 
 template <int N, int i = 0>
 __device__ __forceinline__ void lane_precon(double dt, double* P, double* PI) {
