@@ -177,7 +177,22 @@ struct BdCtx {
   int inst;
   bool act;  // lane < d (padding lanes write nothing)
   uint32_t lin_set;  // the step's probe set of the Monte Carlo Jacobian handlers (pdq_lin_set)
+  // Constant denominators of the step and their reciprocals (correctly
+  // rounded, so that lane_div returns the correctly rounded quotient, bitwise
+  // the division's): sqrt(d) of the error norm, and Burgers' 2 dx and dx dx.
+  double sqd, rsqd, bg_den1, bg_rden1, bg_den2, bg_rden2;
 };
+
+// Burgers' component k (the registry's formula, vf_component in
+// pdq_device.h): the periodic neighbours without an integer modulo, the two
+// quotients from the instance's reciprocals.  Bitwise the registry's value.
+template <int N, int STRATEGY>
+__device__ __forceinline__ double bd_burgers(const BdCtx<N, STRATEGY>& c, const double* y, int d, int k) {
+  const double visc = c.prm[0];
+  const int km = k == 0 ? d - 1 : k - 1, kp = k + 1 == d ? 0 : k + 1;
+  const double um = y[km], u0 = y[k], up = y[kp];
+  return -u0 * lane_div(up - um, c.bg_den1, c.bg_rden1) + visc * lane_div(up - 2.0 * u0 + um, c.bg_den2, c.bg_rden2);
+}
 
 // The linearisation at the block means Xm (ts0: BlockDiagOdeTs0.linearize,
 // ssm_impl_blockdiag.py:123-144; ts1: BlockDiagResidual.linearize, :147-183,
@@ -198,7 +213,7 @@ __device__ __forceinline__ void bd_linearize(const BdCtx<N, STRATEGY>& c, const 
   c.g.sync();
   double f = 0.0, jkk = 0.0;
   if (c.act) {
-    f = vf_component(cfg.vector_field, d, y, c.prm, t, k);
+    f = cfg.vector_field == PDQ_VF_BURGERS ? bd_burgers(c, y, d, k) : vf_component(cfg.vector_field, d, y, c.prm, t, k);
     if (cfg.constraint == PDQ_TS1)
       jkk = cfg.jacobian != PDQ_JAC_EXACT
                 ? vf_mc_diag(cfg.vector_field, d, y, c.prm, k, cfg.jac_seed, c.lin_set + set_offset, cfg.jac_probes,
@@ -234,7 +249,7 @@ __device__ __forceinline__ double bd_error_power(const BdCtx<N, STRATEGY>& c, do
   if (cfg.error_norm == PDQ_NORM_SCALE_THEN_RMS) {
     const double rr = e / (cfg.atol + cfg.rtol * fabs(ref));
     const double tot = c.g.sum(c.act ? rr * rr : 0.0);
-    norm = sqrt(tot) / sqrt(d);
+    norm = lane_div(sqrt(tot), c.sqd, c.rsqd);  // sqrt(tot) / sqrt(d)
   } else {
     const double sa = c.g.sum(c.act ? e * e : 0.0);
     const double sr = c.g.sum(c.act ? ref * ref : 0.0);
@@ -633,6 +648,17 @@ __device__ __forceinline__ void bd_solve_instance(const BdArgs& a, const BdGroup
   c.inst = inst;
   c.act = blk < d;
   c.lin_set = 0;
+  c.sqd = sqrt((double)d);
+  c.rsqd = 1.0 / c.sqd;
+  if (cfg.vector_field == PDQ_VF_BURGERS) {
+    const double dx = c.prm[1];
+    c.bg_den1 = 2.0 * dx;
+    c.bg_rden1 = 1.0 / c.bg_den1;
+    c.bg_den2 = dx * dx;
+    c.bg_rden2 = 1.0 / c.bg_den2;
+  } else {
+    c.bg_den1 = c.bg_rden1 = c.bg_den2 = c.bg_rden2 = 1.0;
+  }
   const double* ts = a.times + (size_t)inst * K;
   BdSlot<N> sf, fr, pr;
   BdScal zs, zf, zp;
