@@ -190,12 +190,18 @@ struct DenseRows {
   static constexpr bool nz(int, int) { return true; }
 };
 
+//
+// With `tau_out`, tau_out[i] receives reflection i's coefficient, 0 where the
+// column needed none (for ss > 0 the coefficient is (beta - alpha) / beta with
+// beta and alpha of opposite signs, never 0); the reflectors' entries stay in
+// A below the diagonal of rows min(M,C).., as the updates left them.
 template <int M, int C, class S = DenseRows>
-__device__ __forceinline__ void lane_qr(double* A) {
+__device__ __forceinline__ void lane_qr(double* A, double* tau_out = nullptr) {
   constexpr int k = M < C ? M : C;
 #pragma unroll
   for (int i = 0; i < k; ++i) {
     if (M - i <= 1) break;
+    if (tau_out) tau_out[i] = 0.0;
     double ss = 0.0;
 #pragma unroll
     for (int r = i + 1; r < M; ++r)
@@ -206,6 +212,7 @@ __device__ __forceinline__ void lane_qr(double* A) {
       // for finite data
       const double beta = -copysign(lane_sqrt_pos(fma(alpha, alpha, ss)), alpha);
       const double tau = lane_div_nz(beta - alpha, beta, lane_rcp_nz(beta));
+      if (tau_out) tau_out[i] = tau;
       const double scal = lane_rcp_nz(alpha - beta);
 #pragma unroll
       for (int r = i + 1; r < M; ++r)
@@ -432,6 +439,187 @@ __device__ __forceinline__ void lane_revert_prior(const double* __restrict__ Q,
       const double lv = big[(N + j) * n2 + N + i];
       bw.setL(i, j, ident_rescale ? fabs(1.0 * to) * lv : lv);
       YL[i * N + j] = fabs(P[i]) * big[j * n2 + i];
+    }
+    bw.tl(i) = lane_rcp_nz(P[i]);
+    bw.to(i) = ident_rescale ? 1.0 : to;
+  }
+}
+
+// lane_revert_prior in two halves, for the fixed-point smoother's adaptive
+// loop: the left half gives the prediction (what the step needs) and goes on
+// to the revert block's right columns, the right half finishes the backward
+// conditional (what only the merges and checkpoints read), so that a second
+// wavefront can do it beside the step's update.  Reflection i < N of the
+// revert block has its non-zeros in row i and rows N..2N-1 (RevertRows), and
+// a reflection updates each column from that column and the reflector alone.
+// So the left half's QR of the left N columns gives lane_revert_prior's R_Y,
+// and its reflections of the right N columns (in lane_qr's order) and the QR
+// of their rows N..2N-1 give R12 and R22: bitwise those of
+// lane_revert_prior, as are the right half's gain and conditional.
+//
+// Between the halves the stage below sits in storage the caller provides
+// (StageRef: the conditional's own storage, which the right half then
+// overwrites, and a tail beyond it): R12, the upper triangles of R22 and
+// R_Y, and the preconditioner P, PI.
+template <int N, int D>
+struct RevertStage {
+  static constexpr int T = N * (N + 1) / 2;
+  static constexpr int r12 = 0, r22 = N * N, r = r22 + T, P = r + T, PI = P + N, size = PI + N;
+  static constexpr int tail = size > CondLayout<N, D>::size ? size - CondLayout<N, D>::size : 0;
+};
+
+// The stage's storage: element e < the conditional's size in the
+// conditional (stride kLaneBlock), the rest in `x` (stride `xs`).
+template <int N, int D>
+struct StageRef {
+  double* c;
+  double* x;
+  int xs;
+  __device__ __forceinline__ double& at(int e) const {
+    constexpr int cs = CondLayout<N, D>::size;
+    return e < cs ? c[e * kLaneBlock] : x[(e - cs) * xs];
+  }
+};
+
+template <int N, int D>
+__device__ __forceinline__ void lane_revert_left(const double* __restrict__ Q, const double* P, const double* PI,
+                                                 double s, const double* Xm, const double* XL, double* Ym, double* YL,
+                                                 const StageRef<N, D>& st) {
+  using RS = RevertStage<N, D>;
+  double big[2 * N * N], tau[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int col = 0; col < N; ++col) {
+      big[r * N + col] = s * lane_iwp_q<N>(Q, col, r);
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const double a = lane_iwp_a<N>(col, k);
+        if (a != 0.0) acc += a * (fabs(PI[k]) * XL[k * N + r]);
+      }
+      big[(N + r) * N + col] = acc;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const double a = lane_iwp_a<N>(i, k);
+        if (a != 0.0) acc += a * (PI[k] * Xm[k * D + col]);
+      }
+      Ym[i * D + col] = P[i] * (acc + 0.0);
+    }
+  lane_qr<2 * N, N, RevertRows<N>>(big, tau);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      YL[i * N + j] = fabs(P[i]) * big[j * N + i];
+      if (j >= i) st.at(RS::r + tri(j, i)) = big[i * N + j];
+    }
+  // the right columns: zeros on top, (|PI| XL)^T below, then lane_qr's
+  // column update for reflections 0..N-1
+  double rt[2 * N * N];
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int col = 0; col < N; ++col) {
+      rt[r * N + col] = 0.0;
+      rt[(N + r) * N + col] = fabs(PI[col]) * XL[col * N + r];
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (tau[i] != 0.0) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double w = rt[i * N + j];
+#pragma unroll
+        for (int r = 0; r < N; ++r) w += big[(N + r) * N + i] * rt[(N + r) * N + j];
+        w *= tau[i];
+        rt[i * N + j] -= w;
+#pragma unroll
+        for (int r = 0; r < N; ++r) rt[(N + r) * N + j] -= big[(N + r) * N + i] * w;
+      }
+    }
+  }
+  lane_qr<N, N>(rt + N * N);  // reflections N..2N-1: the dense bottom-right block
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) st.at(RS::r12 + e) = rt[e];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = i; j < N; ++j) st.at(RS::r22 + tri(j, i)) = rt[(N + i) * N + j];
+    st.at(RS::P + i) = P[i];
+    st.at(RS::PI + i) = PI[i];
+  }
+}
+
+// The right half: Xm is the reverted state's mean, as the left half had it;
+// the stage is read before `bw`, the conditional, is written (with
+// `ident_rescale` as in lane_revert_prior).
+template <int N, int D, class Cond>
+__device__ __forceinline__ void lane_revert_right(const double* Xm, const StageRef<N, D>& st, Cond&& bw,
+                                                  bool ident_rescale) {
+  using RS = RevertStage<N, D>;
+  double rt[2 * N * N], R[N * N], P[N], PI[N];
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) rt[e] = st.at(RS::r12 + e);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      R[i * N + j] = j >= i ? st.at(RS::r + tri(j, i)) : 0.0;
+      rt[(N + i) * N + j] = j >= i ? st.at(RS::r22 + tri(j, i)) : 0.0;
+    }
+    P[i] = st.at(RS::P + i);
+    PI[i] = st.at(RS::PI + i);
+  }
+  double tmp[N * D];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const double a = lane_iwp_a<N>(i, k);
+        if (a != 0.0) acc += a * (PI[k] * Xm[k * D + col]);
+      }
+      tmp[i * D + col] = acc + 0.0;
+    }
+  double G[N * N], rdiag[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) rdiag[i] = lane_rcp_nz(R[i * N + i]);
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+      double sacc = rt[i * N + j];
+#pragma unroll
+      for (int c = i + 1; c < N; ++c) sacc -= R[i * N + c] * G[j * N + c];
+      G[j * N + i] = lane_div(sacc, R[i * N + i], rdiag[i]);
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += G[i * N + k] * tmp[k * D + col];
+      const double mv = PI[i] * Xm[i * D + col] - acc;
+      bw.m(i, col) = ident_rescale ? (1.0 * lane_rcp_nz(PI[i])) * mv + 0.0 : mv;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double to = lane_rcp_nz(PI[i]);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      bw.A(i, j) = ident_rescale ? (1.0 * to) * G[i * N + j] : G[i * N + j];
+      const double lv = rt[(N + j) * N + i];
+      bw.setL(i, j, ident_rescale ? fabs(1.0 * to) * lv : lv);
     }
     bw.tl(i) = lane_rcp_nz(P[i]);
     bw.to(i) = ident_rescale ? 1.0 : to;
@@ -751,6 +939,8 @@ struct LaneCtx {
   LaneWs<N, D> w;
   int inst;
   uint32_t lin_set;  // the step's probe set of the Monte Carlo Jacobian handlers (pdq_lin_set)
+  double* stage_tail;  // the revert stage's tail (StageRef), element stride stage_stride
+  int stage_stride;
 };
 
 // The three parts of a step attempt: solver_dynamic.step (solvers.py:549-599),
@@ -1125,11 +1315,23 @@ struct LaneMergeReq {
 template <int N, int D>
 struct LaneIterVals {
   int op, ident;
+  int fin;  // the conditional lane_revert_finish completes: lane_fin_pack, or -1
   double t1, tsf, h, tn, sigma, err, h0, os;
   double P[N], PI[N], b[D], Xm[N * D], Pm[N * D], PL[N * N];
 };
 
 enum { LOP_STEP = 0, LOP_BEYOND1 = 1, LOP_BEYOND2 = 2, LOP_AT = 3 };
+
+template <int N, int D>
+__device__ __forceinline__ StageRef<N, D> lane_stage(const LaneCtx<N, D>& c, int slot) {
+  return StageRef<N, D>{c.w.cond(slot).p, c.stage_tail, c.stage_stride};
+}
+
+// What lane_revert_finish completes: the conditional's slot, the slot holding
+// the reverted state's mean and factor, and lane_revert_prior's rescaling.
+__device__ __forceinline__ int lane_fin_pack(int bws, int src, bool rescale) {
+  return bws | (src << 2) | ((rescale ? 1 : 0) << 4);
+}
 
 __device__ __forceinline__ LaneMergeReq lane_no_req() {
   LaneMergeReq r;
@@ -1248,6 +1450,7 @@ __device__ __forceinline__ void lane_adapt_part1(const LaneCtx<N, D>& c, const d
   const LaneWs<N, D>& w = c.w;
   v.t1 = ts[s.k];
   v.tsf = w.at(s.sf, LS_T);
+  v.fin = -1;
   v.op = lane_next_op<N, D>(cfg, ts, s, w);
   if (v.op == LOP_AT) return;
   // inputs of the shared prediction
@@ -1285,11 +1488,40 @@ __device__ __forceinline__ void lane_adapt_part1(const LaneCtx<N, D>& c, const d
   v.ident = w.at(v.op == LOP_BEYOND1 ? s.ifr : s.sf, LS_IDENT) != 0.0;
   PDQ_PHASE_BOUNDARY();
   const double ns = lane_noise_scale(cfg, v.h, scale);
-  if (cfg.strategy == PDQ_FILTER)
+  if (cfg.strategy == PDQ_FILTER) {
     lane_marginalise_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL);
-  else
-    lane_revert_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL, w.cond(bws), fp && v.op == LOP_BEYOND2);
+  } else if (fp) {
+    // the prediction here, the conditional in lane_revert_finish (beside
+    // part 2); the first half's source mean, interp_from's, is overwritten
+    // by part 2, so it goes with the stage into the proposed slot (whose
+    // mean nothing reads until a step writes it)
+    lane_revert_left<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL, lane_stage(c, bws));
+    int src = xs;
+    if (v.op == LOP_BEYOND1) {
+#pragma unroll
+      for (int i = 0; i < N * D; ++i) w.at(s.pr, LaneWs<N, D>::SL::m + i) = v.Xm[i];
+      src = s.pr;
+    }
+    v.fin = lane_fin_pack(bws, src, v.op == LOP_BEYOND2);
+  } else {
+    lane_revert_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL, w.cond(bws), false);
+  }
   PDQ_PHASE_BOUNDARY();
+}
+
+// The fixed-point smoother's backward conditional of an iteration's part 1
+// (lane_revert_right from the stage part 1 left in the conditional's storage
+// and the stage tail, and the reverted state's mean in slot `src`).  Part 2
+// reads none of these, and writes neither the conditional, the tail nor that
+// slot's mean: this may run beside it.
+template <int N, int D>
+__device__ __forceinline__ void lane_revert_finish(const LaneCtx<N, D>& c, int fin) {
+  if (fin < 0) return;
+  double m[N * D];
+  const int src = (fin >> 2) & 3;
+#pragma unroll
+  for (int i = 0; i < N * D; ++i) m[i] = c.w.at(src, LaneWs<N, D>::SL::m + i);
+  lane_revert_right<N, D>(m, lane_stage(c, fin & 3), c.w.cond(fin & 3), ((fin >> 4) & 1) != 0);
 }
 
 // The work part 2 left: the merge (ssm_impl_isotropic.py:91-105) or the
@@ -1416,6 +1648,8 @@ __device__ __forceinline__ LaneCtx<N, D> lane_ctx(const LArgs& a, int inst, doub
   c.prm = a.params + (size_t)inst * a.cfg.n_params;
   c.w.base = slots;
   c.lin_set = 0;
+  c.stage_tail = nullptr;
+  c.stage_stride = 1;
   return c;
 }
 
@@ -1441,10 +1675,13 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
       s.pr = tmp;
     }
   } else {
+    double tail[RevertStage<N, D>::tail > 0 ? RevertStage<N, D>::tail : 1];
+    c.stage_tail = tail;
     while (s.k < K && s.status == PDQ_STATUS_OK) {
       LaneIterVals<N, D> v;
       c.lin_set = pdq_lin_set(a.cfg, s.accepted);
       lane_adapt_part1<N, D>(c, ts, s, v);
+      lane_revert_finish<N, D>(c, v.fin);
       int packed[2];
       lane_pack_req(lane_adapt_part2<N, D>(c, ts, s, v), packed);
       lane_run_merge<N, D>(c, ts, lane_unpack_req(packed));
@@ -1461,11 +1698,15 @@ constexpr int lane_block_threads() {
 }
 
 // LDS bytes of one workgroup: three slots per lane; with the second
-// wavefront, 64 requests (two ints each) and the loop flag after them.
+// wavefront, 64 requests (two ints each), the loop flag (in an eight-byte
+// cell), 64 lane_revert_finish codes and the lanes' revert stage tails
+// (element-major) after them.
 template <int N, int D, int STRATEGY = PDQ_FILTER>
 constexpr size_t lane_lds_bytes() {
   return sizeof(double) * kLaneBlock * 3 * SlotLayout<N, D>::size +
-         (STRATEGY == PDQ_FIXEDPOINT ? sizeof(double) * (kLaneBlock + 1) : 0);
+         (STRATEGY == PDQ_FIXEDPOINT ? sizeof(double) * (kLaneBlock + 1) + sizeof(int) * kLaneBlock +
+                                           sizeof(double) * kLaneBlock * RevertStage<N, D>::tail
+                                     : 0);
 }
 
 #ifndef PDQ_HOST_EMU
@@ -1491,9 +1732,13 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
   double* const slots = lds + l;
   int* const req = reinterpret_cast<int*>(lds + kLaneBlock * 3 * SlotLayout<N, D>::size);
   volatile int* const flag = req + 2 * kLaneBlock;
+  int* const fin = req + 2 * kLaneBlock + 2;
+  double* const tail = reinterpret_cast<double*>(fin + kLaneBlock);
   const int inst = blockIdx.x * kLaneBlock + l;
   const bool mine = inst < a.n_inst;
   LaneCtx<N, D> c = lane_ctx<N, D>(a, mine ? inst : 0, slots);
+  c.stage_tail = tail + l;
+  c.stage_stride = kLaneBlock;
   if (role == 0) {
     const int K = a.K;
     const double* ts = a.times + (size_t)(mine ? inst : 0) * K;
@@ -1509,7 +1754,9 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
     while (__builtin_amdgcn_readfirstlane(*flag)) {
       LaneIterVals<N, D> v;
       c.lin_set = pdq_lin_set(a.cfg, s.accepted);
+      v.fin = -1;
       if (run) lane_adapt_part1<N, D>(c, ts, s, v);
+      fin[l] = v.fin;
       __syncthreads();
       last = lane_no_req();
       if (run) last = lane_adapt_part2<N, D>(c, ts, s, v);
@@ -1533,6 +1780,7 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
     while (__builtin_amdgcn_readfirstlane(*flag)) {
       lane_run_merge<N, D>(c, ts, lane_unpack_req(req + 2 * l));
       __syncthreads();
+      lane_revert_finish<N, D>(c, fin[l]);
       __syncthreads();
     }
   }
@@ -1556,6 +1804,7 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
 template <int N, int D, int STRATEGY, int SOLVER, bool OPTS>
 __global__ void __launch_bounds__(lane_block_threads<STRATEGY>()) lane_kernel(LArgs args) {
   extern __shared__ double lds[];
+  static_assert(lane_lds_bytes<N, D, STRATEGY>() <= 160 * 1024, "a workgroup's LDS fits in one CU's 160 KB");
   LArgs a = args;
   a.cfg.strategy = STRATEGY;
   a.cfg.solver = SOLVER;

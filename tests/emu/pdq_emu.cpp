@@ -114,6 +114,66 @@ int pdqemu_iwp_mismatches(void) {
   return bad;
 }
 
+// Entries (bitwise) where lane_revert_left followed by lane_revert_right
+// differs from lane_revert_prior (pdq_lane.h), over `trials` random reverts
+// of the IWP prior with nu + 1 = N = 2..6, D = 2: prediction mean and factor
+// and every entry of the conditional.  Trial 0 reverts a zero factor (no
+// reflections at all), trial 1 a factor with a zero column.
+int pdqemu_revert_halves_mismatch(int N_, int trials, unsigned seed, int rescale) {
+  int bad = 0;
+  unsigned long long st = 0x9E3779B97F4A7C15ull ^ seed;
+  auto unif = [&]() {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    return (double)(st >> 11) * 0x1.0p-53;
+  };
+  auto check = [&](auto n_tag) {
+    constexpr int N = decltype(n_tag)::value, D = 2;
+    using Cond = pdq::LCond<N, D>;
+    std::vector<double> A, Qv;
+    pdq::iwp_matrices(N - 1, A, Qv);
+    for (int t = 0; t < trials; ++t) {
+      const double h = 1e-3 + unif() * 0.5, s = 0.1 + unif() * 10.0;
+      double P[N], PI[N], Xm[N * D], XL[N * N];
+      pdq::lane_precon<N>(h, P, PI);
+      for (int i = 0; i < N * D; ++i) Xm[i] = 4.0 * unif() - 2.0;
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+          XL[i * N + j] = (j <= i && t != 0 && !(t == 1 && j == 1)) ? (2.0 * unif() - 1.0) * 1e-2 : 0.0;
+      double Ym0[N * D], YL0[N * N], Ym1[N * D], YL1[N * N];
+      Cond c0;
+      std::memset(&c0, 0, sizeof c0);
+      pdq::lane_revert_prior<N, D>(Qv.data(), P, PI, s, Xm, XL, Ym0, YL0, c0, rescale != 0);
+      // as in the kernel: the stage in the conditional's own storage (lane stride) and a tail
+      std::vector<double> cbuf(pdq::CondLayout<N, D>::size * pdq::kLaneBlock, 0.0);
+      std::vector<double> tail(pdq::RevertStage<N, D>::tail + 1, 0.0);
+      const pdq::StageRef<N, D> st{cbuf.data(), tail.data(), 1};
+      const pdq::CondRef<N, D> c1{cbuf.data()};
+      pdq::lane_revert_left<N, D>(Qv.data(), P, PI, s, Xm, XL, Ym1, YL1, st);
+      pdq::lane_revert_right<N, D>(Xm, st, c1, rescale != 0);
+      bad += std::memcmp(Ym0, Ym1, sizeof Ym0) != 0;
+      bad += std::memcmp(YL0, YL1, sizeof YL0) != 0;
+      for (int e = 0; e < pdq::CondLayout<N, D>::size; ++e) {
+        if (e >= pdq::CondLayout<N, D>::L && e < pdq::CondLayout<N, D>::tl) continue;  // L: below
+        bad += std::memcmp(&c0.v[e], &c1.at(e), sizeof(double)) != 0;
+      }
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j <= i; ++j) {
+          const double x0 = c0.L(i, j), x1 = c1.L(i, j);
+          bad += std::memcmp(&x0, &x1, sizeof(double)) != 0;
+        }
+    }
+  };
+  switch (N_) {
+    case 2: check(std::integral_constant<int, 2>{}); break;
+    case 3: check(std::integral_constant<int, 3>{}); break;
+    case 4: check(std::integral_constant<int, 4>{}); break;
+    case 5: check(std::integral_constant<int, 5>{}); break;
+    case 6: check(std::integral_constant<int, 6>{}); break;
+    default: return -1;
+  }
+  return bad;
+}
+
 // The lane kernel's shared-divisor quotient (pdq_lane.h: lane_div).
 double pdqemu_lane_div(double a, double b) { return pdq::lane_div(a, b, 1.0 / b); }
 

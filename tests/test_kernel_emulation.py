@@ -454,6 +454,17 @@ def test_lane_compile_time_transition_matches_the_host_matrices(emu):
     assert emu.lib().pdqemu_iwp_mismatches() == 0
 
 
+@pytest.mark.parametrize("rescale", [0, 1])
+def test_lane_revert_halves_are_bitwise_the_revert(emu, rescale):
+    """The fixed-point smoother's adaptive loop splits the revert (ssm_impl_isotropic.py:107-133)
+    into the prediction (lane_revert_left) and the backward conditional (lane_revert_right, on the
+    merging wavefront): together they give lane_revert_prior's prediction and conditional bit for
+    bit, for nu = 1..5, with and without the identity merge's rescaling, including a zero factor
+    (no reflections) and a factor with a zero column."""
+    for n in range(2, 7):
+        assert emu.lib().pdqemu_revert_halves_mismatch(n, 200, 11 + n, rescale) == 0, n
+
+
 def test_lane_shared_divisor_quotient_is_the_correctly_rounded_division(emu):
     """lane_div (one reciprocal per divisor, then an FMA correction) returns exactly a / b:
     random operands over a wide exponent range, hard cases near ties, and IEEE special values."""
