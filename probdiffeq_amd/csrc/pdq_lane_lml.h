@@ -91,9 +91,12 @@ struct LmlStepVals {
 // revert of the transition into cond(to); (2) the update into slot `to`'s
 // marginal.  Between them (or beside part 1 of the next step) the merge
 // cond(to) = cond(from) o cond(to), which neither part reads or writes.
+// With `defer` the revert stops after its left half (lane_revert_left): the
+// conditional of slot `to` is finished by lane_revert_right from the stage
+// left in its storage (lane_lml_forward_split's second wavefront).
 template <int N, int D, int TPL>
 __device__ __forceinline__ void lane_lml_step_part1(const LaneCtx<N, D * (1 + TPL)>& c, int q0, int from, int to,
-                                                    double h, LmlStepVals<N, D * (1 + TPL)>& v) {
+                                                    double h, LmlStepVals<N, D * (1 + TPL)>& v, bool defer = false) {
   constexpr int MC = D * (1 + TPL);
   const pdq_config& cfg = c.a->cfg;
   const double* __restrict__ Q = c.a->Q;
@@ -120,7 +123,11 @@ __device__ __forceinline__ void lane_lml_step_part1(const LaneCtx<N, D * (1 + TP
     v.sigma = lane_rms_zero<D>(om, col[0]);
   }
   PDQ_PHASE_BOUNDARY();
-  lane_revert_prior<N, MC>(Q, v.P, v.PI, lane_noise_scale(cfg, h, v.sigma), Sm, SLf, v.Pm, v.PL, w.cond(to));
+  if (defer)
+    lane_revert_left<N, MC>(Q, v.P, v.PI, lane_noise_scale(cfg, h, v.sigma), Sm, SLf, v.Pm, v.PL,
+                            StageRef<N, MC>{w.cond(to).p, nullptr, 1});
+  else
+    lane_revert_prior<N, MC>(Q, v.P, v.PI, lane_noise_scale(cfg, h, v.sigma), Sm, SLf, v.Pm, v.PL, w.cond(to));
   PDQ_PHASE_BOUNDARY();
 }
 
@@ -160,7 +167,11 @@ __device__ __forceinline__ void lane_lml_step(const LaneCtx<N, D * (1 + TPL)>& c
 // HBM, beside wavefront 0's next prediction part.  Three slots rotate (the
 // step from `sf` reverts into `pr`; then sf <- pr, pr <- spare, spare <- sf),
 // so the next prediction reads only the marginal of the merge's result slot
-// and reverts into the third slot.  Every lane takes the same K - 1 steps, so
+// and reverts into the third slot.  When the revert's stage fits in a
+// conditional's storage, wavefront 1 also finishes each step's backward
+// conditional (lane_revert_right) beside wavefront 0's update: the update
+// writes neither that conditional nor step_from's mean, which it reads.
+// Wavefront 1 follows the slot rotation itself.  Every lane takes the same K - 1 steps, so
 // both wavefronts pass 2 (K - 1) + 1 barriers; the last step's request is run
 // by wavefront 0 after the loop.  Returns step_from's slot.  With ckpt_every
 // > 0 wavefront 0 also stores step_from's whole slot before every
@@ -173,6 +184,7 @@ __device__ __forceinline__ int lane_lml_forward_split(const LaneCtx<N, D * (1 + 
                                                       int ckpt_every = 0, double* ckpt = nullptr) {
   constexpr int MC = D * (1 + TPL);
   using CL = CondLayout<N, MC>;
+  constexpr bool defer = RevertStage<N, MC>::tail == 0;
   const LaneWs<N, MC>& w = c.w;
   const int K = a.K, every = a.every;
   const int l = threadIdx.x & (kLaneBlock - 1);
@@ -197,7 +209,7 @@ __device__ __forceinline__ int lane_lml_forward_split(const LaneCtx<N, D * (1 + 
     int last[2] = {0, -1};
     for (int k = 1; k < K; ++k) {
       LmlStepVals<N, MC> v;
-      if (active) lane_lml_step_part1<N, D, TPL>(c, q0, sf, pr, a.times[k] - a.times[k - 1], v);
+      if (active) lane_lml_step_part1<N, D, TPL>(c, q0, sf, pr, a.times[k] - a.times[k - 1], v, defer);
       __syncthreads();
       if (ckpt_every > 0 && active && (k - 1) % ckpt_every == 0) {
         // part 1 wrote only pr; the merge into sf's conditional is complete
@@ -226,9 +238,22 @@ __device__ __forceinline__ int lane_lml_forward_split(const LaneCtx<N, D * (1 + 
     return sf;
   }
   __syncthreads();
+  int sf = 0, pr = 1, sp = 2;
   for (int k = 1; k < K; ++k) {
     run(mine);
     __syncthreads();
+    if (defer && active) {
+      double Xm[N * MC];
+#pragma unroll
+      for (int i = 0; i < N * MC; ++i) Xm[i] = w.at(sf, SlotLayout<N, MC>::m + i);
+      lane_revert_right<N, MC>(Xm, StageRef<N, MC>{w.cond(pr).p, nullptr, 1}, w.cond(pr), false);
+    }
+    if (active) {
+      const int old_sf = sf;
+      sf = pr;
+      pr = sp;
+      sp = old_sf;
+    }
     __syncthreads();
   }
   return 0;

@@ -116,18 +116,19 @@ int pdqemu_iwp_mismatches(void) {
 
 // Entries (bitwise) where lane_revert_left followed by lane_revert_right
 // differs from lane_revert_prior (pdq_lane.h), over `trials` random reverts
-// of the IWP prior with nu + 1 = N = 2..6, D = 2: prediction mean and factor
+// of the IWP prior with nu + 1 = N = 2..6 and D = 2 (the isotropic solve) or
+// D = 4 (the LML's primal and tangent columns): prediction mean and factor
 // and every entry of the conditional.  Trial 0 reverts a zero factor (no
 // reflections at all), trial 1 a factor with a zero column.
-int pdqemu_revert_halves_mismatch(int N_, int trials, unsigned seed, int rescale) {
+int pdqemu_revert_halves_mismatch(int N_, int D_, int trials, unsigned seed, int rescale) {
   int bad = 0;
   unsigned long long st = 0x9E3779B97F4A7C15ull ^ seed;
   auto unif = [&]() {
     st = st * 6364136223846793005ull + 1442695040888963407ull;
     return (double)(st >> 11) * 0x1.0p-53;
   };
-  auto check = [&](auto n_tag) {
-    constexpr int N = decltype(n_tag)::value, D = 2;
+  auto check = [&](auto n_tag, auto d_tag) {
+    constexpr int N = decltype(n_tag)::value, D = decltype(d_tag)::value;
     using Cond = pdq::LCond<N, D>;
     std::vector<double> A, Qv;
     pdq::iwp_matrices(N - 1, A, Qv);
@@ -163,12 +164,17 @@ int pdqemu_revert_halves_mismatch(int N_, int trials, unsigned seed, int rescale
         }
     }
   };
+  auto with_d = [&](auto n_tag) {
+    if (D_ == 2) check(n_tag, std::integral_constant<int, 2>{});
+    else if (D_ == 4) check(n_tag, std::integral_constant<int, 4>{});
+    else bad = -1;
+  };
   switch (N_) {
-    case 2: check(std::integral_constant<int, 2>{}); break;
-    case 3: check(std::integral_constant<int, 3>{}); break;
-    case 4: check(std::integral_constant<int, 4>{}); break;
-    case 5: check(std::integral_constant<int, 5>{}); break;
-    case 6: check(std::integral_constant<int, 6>{}); break;
+    case 2: with_d(std::integral_constant<int, 2>{}); break;
+    case 3: with_d(std::integral_constant<int, 3>{}); break;
+    case 4: with_d(std::integral_constant<int, 4>{}); break;
+    case 5: with_d(std::integral_constant<int, 5>{}); break;
+    case 6: with_d(std::integral_constant<int, 6>{}); break;
     default: return -1;
   }
   return bad;

@@ -126,7 +126,9 @@ def test_lml_single_instance_value_only(oracle, pdq):
     cfg = pdq.Config(factorisation="isotropic", num_derivatives=3, ode_dim=2, strategy="fixedpoint", solver="solver")
     v_only, _ = _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=False)
     v_grad, _ = _check(oracle, pdq, cfg, params, y0, grid, every, data, stds, grad=True)
-    assert v_only[0] == v_grad[0]
+    # the gradient kernel finishes each step's backward conditional on its second wavefront
+    # (lane_revert_right) and rounds differently from the value-only kernel: 1e-13 relative
+    assert abs(v_only[0] - v_grad[0]) <= 1e-13 * abs(v_only[0])
 
 
 def test_torch_autograd_function(oracle, pdq):

@@ -459,10 +459,11 @@ def test_lane_revert_halves_are_bitwise_the_revert(emu, rescale):
     """The fixed-point smoother's adaptive loop splits the revert (ssm_impl_isotropic.py:107-133)
     into the prediction (lane_revert_left) and the backward conditional (lane_revert_right, on the
     merging wavefront): together they give lane_revert_prior's prediction and conditional bit for
-    bit, for nu = 1..5, with and without the identity merge's rescaling, including a zero factor
-    (no reflections) and a factor with a zero column."""
+    bit, for nu = 1..5, two and four mean columns, with and without the identity merge's
+    rescaling, including a zero factor (no reflections) and a factor with a zero column."""
     for n in range(2, 7):
-        assert emu.lib().pdqemu_revert_halves_mismatch(n, 200, 11 + n, rescale) == 0, n
+        for d in (2, 4):
+            assert emu.lib().pdqemu_revert_halves_mismatch(n, d, 200, 11 + n, rescale) == 0, (n, d)
 
 
 def test_lane_shared_divisor_quotient_is_the_correctly_rounded_division(emu):
