@@ -200,7 +200,7 @@ __device__ __forceinline__ void adj_recompute_step(const LaneCtx<N, D>& c, int f
 template <int N, int D, bool SPLIT = false>
 __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs& a, int inst, double* slots,
                                                   size_t lane, size_t n_lanes, bool active = true,
-                                                  int* req = nullptr) {
+                                                  int* req = nullptr, double* stage_tail = nullptr) {
   using CL = CondLayout<N, D>;
   using SL = SlotLayout<N, D>;
   const pdq_config& cfg = a.cfg;
@@ -241,7 +241,7 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
     // the interval conditionals land where the LML kernel puts them: lay.conds
     // is the storage's first region
     static_assert(AdjLayout<N, D>::conds_first, "the split forward pass writes the conditionals at offset 0");
-    sf = lane_lml_forward_split<N, D, 0>(c, a, 0, lane, n_lanes, true, req, kAdjSegment, at(lay.ckpt));
+    sf = lane_lml_forward_split<N, D, 0>(c, a, 0, lane, n_lanes, true, req, kAdjSegment, at(lay.ckpt), stage_tail);
 #endif
   } else {
     for (int k = 0; k < K - 1; ++k) {
@@ -554,7 +554,8 @@ __device__ __forceinline__ void lane_adj_instance(const LArgs& la, const LmlArgs
 // lane_adj_instance<N, D, true>'s.
 template <int N, int D>
 __device__ __forceinline__ void lane_adj_recompute_role(const LArgs& la, const LmlArgs& a, int inst, double* fwd_slots,
-                                                        double* slots, size_t lane, size_t n_lanes, int* req) {
+                                                        double* slots, size_t lane, size_t n_lanes, int* req,
+                                                        double* stage_tail) {
   LaneCtx<N, D> c;
   c.a = &la;
   c.inst = inst;
@@ -562,7 +563,7 @@ __device__ __forceinline__ void lane_adj_recompute_role(const LArgs& la, const L
   c.w.base = fwd_slots;
   c.lin_set = 0;
 #ifndef PDQ_HOST_EMU
-  lane_lml_forward_split<N, D, 0>(c, a, 0, lane, n_lanes, true, req);
+  lane_lml_forward_split<N, D, 0>(c, a, 0, lane, n_lanes, true, req, 0, nullptr, stage_tail);
 #endif
   c.w.base = slots;
   const AdjLayout<N, D> lay(a.K, a.every);
@@ -584,7 +585,7 @@ __device__ __forceinline__ void lane_adj_recompute_role(const LArgs& la, const L
 template <int N, int D, bool SPLIT>
 constexpr size_t lane_adj_lds_bytes() {
   return sizeof(double) * kLaneBlock * 2 * SlotLayout<N, D>::size * (SPLIT ? 2 : 1) +
-         (SPLIT ? 2 * sizeof(int) * kLaneBlock : 0);
+         (SPLIT ? 2 * sizeof(int) * kLaneBlock + sizeof(double) * kLaneBlock * RevertStage<N, D>::tail : 0);
 }
 
 // One lane per instance (the grid covers every instance once); slots in LDS
@@ -611,14 +612,18 @@ __global__ void __launch_bounds__(SPLIT ? 2 * kLaneBlock : kLaneBlock) lane_adj_
     double* const fwd_slots = lds + l;
     double* const slots = lds + l + (role ? 2 * SlotLayout<N, D>::size * kLaneBlock : 0);
     int* const req = reinterpret_cast<int*>(lds + 4 * SlotLayout<N, D>::size * kLaneBlock);
+    // after the requests: the lane's revert stage tail (the forward pass's
+    // conditionals are finished on wavefront 1)
+    double* const stage_tail =
+        RevertStage<N, D>::tail > 0 ? reinterpret_cast<double*>(req + 2 * kLaneBlock) + l : nullptr;
     const size_t n_lanes = (size_t)gridDim.x * kLaneBlock;
     const size_t lane = (size_t)blockIdx.x * kLaneBlock + l;
     // a lane past the last instance computes the last one (uniform control
     // flow for the barriers) and writes nothing
     const bool active = lane < (size_t)a.n_inst;
     const int inst = active ? (int)lane : a.n_inst - 1;
-    if (role == 0) lane_adj_instance<N, D, true>(la, a, inst, slots, lane, n_lanes, active, req);
-    else lane_adj_recompute_role<N, D>(la, a, inst, fwd_slots, slots, lane, n_lanes, req);
+    if (role == 0) lane_adj_instance<N, D, true>(la, a, inst, slots, lane, n_lanes, active, req, stage_tail);
+    else lane_adj_recompute_role<N, D>(la, a, inst, fwd_slots, slots, lane, n_lanes, req, stage_tail);
   } else {
     double* const slots = lds + threadIdx.x;
     const size_t n_lanes = (size_t)gridDim.x * blockDim.x;

@@ -93,10 +93,12 @@ struct LmlStepVals {
 // cond(to) = cond(from) o cond(to), which neither part reads or writes.
 // With `defer` the revert stops after its left half (lane_revert_left): the
 // conditional of slot `to` is finished by lane_revert_right from the stage
-// left in its storage (lane_lml_forward_split's second wavefront).
+// left in its storage and `stage_tail` (lane_lml_forward_split's second
+// wavefront).
 template <int N, int D, int TPL>
 __device__ __forceinline__ void lane_lml_step_part1(const LaneCtx<N, D * (1 + TPL)>& c, int q0, int from, int to,
-                                                    double h, LmlStepVals<N, D * (1 + TPL)>& v, bool defer = false) {
+                                                    double h, LmlStepVals<N, D * (1 + TPL)>& v, bool defer = false,
+                                                    double* stage_tail = nullptr) {
   constexpr int MC = D * (1 + TPL);
   const pdq_config& cfg = c.a->cfg;
   const double* __restrict__ Q = c.a->Q;
@@ -125,7 +127,7 @@ __device__ __forceinline__ void lane_lml_step_part1(const LaneCtx<N, D * (1 + TP
   PDQ_PHASE_BOUNDARY();
   if (defer)
     lane_revert_left<N, MC>(Q, v.P, v.PI, lane_noise_scale(cfg, h, v.sigma), Sm, SLf, v.Pm, v.PL,
-                            StageRef<N, MC>{w.cond(to).p, nullptr, 1});
+                            StageRef<N, MC>{w.cond(to).p, stage_tail, kLaneBlock});
   else
     lane_revert_prior<N, MC>(Q, v.P, v.PI, lane_noise_scale(cfg, h, v.sigma), Sm, SLf, v.Pm, v.PL, w.cond(to));
   PDQ_PHASE_BOUNDARY();
@@ -168,7 +170,8 @@ __device__ __forceinline__ void lane_lml_step(const LaneCtx<N, D * (1 + TPL)>& c
 // step from `sf` reverts into `pr`; then sf <- pr, pr <- spare, spare <- sf),
 // so the next prediction reads only the marginal of the merge's result slot
 // and reverts into the third slot.  When the revert's stage fits in a
-// conditional's storage, wavefront 1 also finishes each step's backward
+// conditional's storage, or the caller gives the lane's stage tail (stride
+// 64) for the rest, wavefront 1 also finishes each step's backward
 // conditional (lane_revert_right) beside wavefront 0's update: the update
 // writes neither that conditional nor step_from's mean, which it reads.
 // Wavefront 1 follows the slot rotation itself.  Every lane takes the same K - 1 steps, so
@@ -181,10 +184,11 @@ __device__ __forceinline__ void lane_lml_step(const LaneCtx<N, D * (1 + TPL)>& c
 template <int N, int D, int TPL>
 __device__ __forceinline__ int lane_lml_forward_split(const LaneCtx<N, D * (1 + TPL)>& c, const LmlArgs& a, int q0,
                                                       size_t glane, size_t n_lanes, bool active, int* req,
-                                                      int ckpt_every = 0, double* ckpt = nullptr) {
+                                                      int ckpt_every = 0, double* ckpt = nullptr,
+                                                      double* stage_tail = nullptr) {
   constexpr int MC = D * (1 + TPL);
   using CL = CondLayout<N, MC>;
-  constexpr bool defer = RevertStage<N, MC>::tail == 0;
+  const bool defer = RevertStage<N, MC>::tail == 0 || stage_tail != nullptr;
   const LaneWs<N, MC>& w = c.w;
   const int K = a.K, every = a.every;
   const int l = threadIdx.x & (kLaneBlock - 1);
@@ -209,7 +213,7 @@ __device__ __forceinline__ int lane_lml_forward_split(const LaneCtx<N, D * (1 + 
     int last[2] = {0, -1};
     for (int k = 1; k < K; ++k) {
       LmlStepVals<N, MC> v;
-      if (active) lane_lml_step_part1<N, D, TPL>(c, q0, sf, pr, a.times[k] - a.times[k - 1], v, defer);
+      if (active) lane_lml_step_part1<N, D, TPL>(c, q0, sf, pr, a.times[k] - a.times[k - 1], v, defer, stage_tail);
       __syncthreads();
       if (ckpt_every > 0 && active && (k - 1) % ckpt_every == 0) {
         // part 1 wrote only pr; the merge into sf's conditional is complete
@@ -246,7 +250,7 @@ __device__ __forceinline__ int lane_lml_forward_split(const LaneCtx<N, D * (1 + 
       double Xm[N * MC];
 #pragma unroll
       for (int i = 0; i < N * MC; ++i) Xm[i] = w.at(sf, SlotLayout<N, MC>::m + i);
-      lane_revert_right<N, MC>(Xm, StageRef<N, MC>{w.cond(pr).p, nullptr, 1}, w.cond(pr), false);
+      lane_revert_right<N, MC>(Xm, StageRef<N, MC>{w.cond(pr).p, stage_tail, kLaneBlock}, w.cond(pr), false);
     }
     if (active) {
       const int old_sf = sf;
