@@ -34,12 +34,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per C2 launch of the lane kernel, from rocprofv3 PMC runs
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, scales from a 1 GiB copy); see
 # profiles/lane_kernel_c2_isotropic_nu4_4096.md.  Re-measure when the kernel changes.
-C2_MEASURED_TRAFFIC_BYTES = round((228_576.6 * 2 + 940_478.8) * 1024)
+C2_MEASURED_TRAFFIC_BYTES = round((228_549.9 * 2 + 940_406.3) * 1024)
 C2_TRAFFIC_SOURCE = "profiles/lane_kernel_c2_isotropic_nu4_4096.md"
-C5_MEASURED_TRAFFIC_BYTES = round((121_178.1 * 2 + 240_058.0) * 1024)
+C5_MEASURED_TRAFFIC_BYTES = round((121_189.7 * 2 + 240_058.0) * 1024)
 C5_TRAFFIC_SOURCE = "profiles/lane_lml_kernel_c5_isotropic_nu3_1024.md"
 # the same for the C5 batch's reverse adjoint (lane_adj_kernel, two wavefronts)
-C5_REVERSE_MEASURED_TRAFFIC_BYTES = round((324_672.5 * 2 + 1_155_472.0) * 1024)
+C5_REVERSE_MEASURED_TRAFFIC_BYTES = round((322_318.0 * 2 + 1_155_472.0) * 1024)
 
 
 def c2_inputs(n_inst=4096, K=200, seed=0):
