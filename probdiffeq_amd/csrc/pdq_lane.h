@@ -1490,11 +1490,11 @@ __device__ __forceinline__ void lane_adapt_part1(const LaneCtx<N, D>& c, const d
   const double ns = lane_noise_scale(cfg, v.h, scale);
   if (cfg.strategy == PDQ_FILTER) {
     lane_marginalise_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL);
-  } else if (fp) {
-    // the prediction here, the conditional in lane_revert_finish (beside
-    // part 2); the first half's source mean, interp_from's, is overwritten
-    // by part 2, so it goes with the stage into the proposed slot (whose
-    // mean nothing reads until a step writes it)
+  } else {
+    // the smoothers: the prediction here, the conditional in
+    // lane_revert_finish (beside part 2); the first half's source mean,
+    // interp_from's, is overwritten by part 2, so it goes with the stage into
+    // the proposed slot (whose mean nothing reads until a step writes it)
     lane_revert_left<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL, lane_stage(c, bws));
     int src = xs;
     if (v.op == LOP_BEYOND1) {
@@ -1502,9 +1502,7 @@ __device__ __forceinline__ void lane_adapt_part1(const LaneCtx<N, D>& c, const d
       for (int i = 0; i < N * D; ++i) w.at(s.pr, LaneWs<N, D>::SL::m + i) = v.Xm[i];
       src = s.pr;
     }
-    v.fin = lane_fin_pack(bws, src, v.op == LOP_BEYOND2);
-  } else {
-    lane_revert_prior<N, D>(c.a->Q, v.P, v.PI, ns, v.Xm, XL, v.Pm, v.PL, w.cond(bws), false);
+    v.fin = lane_fin_pack(bws, src, fp && v.op == LOP_BEYOND2);
   }
   PDQ_PHASE_BOUNDARY();
 }
@@ -1690,11 +1688,16 @@ __device__ __forceinline__ void lane_solve_instance(const LArgs& a, int inst, do
   lane_end_instance<N, D>(c, s);
 }
 
-// Threads of a workgroup: the fixed-point smoother runs its merges on a
-// second wavefront (lane_solve_split), the other strategies on one.
+// The smoothers' adaptive solves run on two wavefronts (lane_solve_split):
+// the second finishes the reverts' conditionals and runs the merges and
+// checkpoint writes; the filter runs on one.
+constexpr bool lane_split_strategy(int strategy) {
+  return strategy == PDQ_FIXEDPOINT || strategy == PDQ_FIXEDINTERVAL;
+}
+
 template <int STRATEGY>
 constexpr int lane_block_threads() {
-  return STRATEGY == PDQ_FIXEDPOINT ? 2 * kLaneBlock : kLaneBlock;
+  return lane_split_strategy(STRATEGY) ? 2 * kLaneBlock : kLaneBlock;
 }
 
 // LDS bytes of one workgroup: three slots per lane; with the second
@@ -1704,16 +1707,16 @@ constexpr int lane_block_threads() {
 template <int N, int D, int STRATEGY = PDQ_FILTER>
 constexpr size_t lane_lds_bytes() {
   return sizeof(double) * kLaneBlock * 3 * SlotLayout<N, D>::size +
-         (STRATEGY == PDQ_FIXEDPOINT ? sizeof(double) * (kLaneBlock + 1) + sizeof(int) * kLaneBlock +
+         (lane_split_strategy(STRATEGY) ? sizeof(double) * (kLaneBlock + 1) + sizeof(int) * kLaneBlock +
                                            sizeof(double) * kLaneBlock * RevertStage<N, D>::tail
                                      : 0);
 }
 
 #ifndef PDQ_HOST_EMU
 
-// The fixed-point smoother's adaptive solve on two wavefronts: wavefront 0
-// runs parts 1 and 2 of every iteration for the workgroup's 64 instances
-// (one lane each, as lane_solve_instance); wavefront 1 runs the merge that
+// The smoothers' adaptive solve on two wavefronts: wavefront 0 runs parts 1
+// and 2 of every iteration for the workgroup's 64 instances (one lane each,
+// as lane_solve_instance); wavefront 1 runs the merge or checkpoint that
 // part 2 of an iteration asks for while wavefront 0 runs part 1 of the next
 // iteration, which touches none of the merge's conditionals (see
 // lane_adapt_part2).  Lane j of both wavefronts works on instance j's LDS
@@ -1788,7 +1791,7 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
 
 // One lane per instance.  Each lane's slots live in LDS, element-major across
 // the workgroup's lanes, read and written by that lane only (and, for the
-// fixed-point smoother's adaptive solve, by the same lane of the merging
+// smoothers' adaptive solves, by the same lane of the merging
 // wavefront: lane_solve_split).  The strategy and the solver are
 // compile-time constants of the instantiation: the kernel overwrites them in
 // its own copy of the arguments, so every test of them folds and the other
@@ -1798,9 +1801,8 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
 // error_residual_std without re-linearisation or per-unit-step scaling, no
 // re-linearisation after the calibration), and the launcher picks this
 // instantiation only for configurations that hold them; with OPTS they are
-// read at run time.  The fixed-point smoother's launches cover every
-// instance with one lane of wavefront 0 (no grid stride); the others stride
-// over instances.
+// read at run time.  The smoothers' launches cover every instance with one
+// lane of wavefront 0 (no grid stride); the filter's strides over instances.
 template <int N, int D, int STRATEGY, int SOLVER, bool OPTS>
 __global__ void __launch_bounds__(lane_block_threads<STRATEGY>()) lane_kernel(LArgs args) {
   extern __shared__ double lds[];
@@ -1816,7 +1818,7 @@ __global__ void __launch_bounds__(lane_block_threads<STRATEGY>()) lane_kernel(LA
     a.cfg.error_per_unit_step = 0;
     a.cfg.re_linearize_after_calibration = 0;
   }
-  if constexpr (STRATEGY == PDQ_FIXEDPOINT) {
+  if constexpr (lane_split_strategy(STRATEGY)) {
     if (a.mode != LMODE_FIXED_GRID) {
       lane_solve_split<N, D>(a, lds);
     } else if (threadIdx.x < kLaneBlock) {

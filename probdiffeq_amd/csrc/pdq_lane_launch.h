@@ -50,10 +50,10 @@ int launch_lane_nd(LArgs& a, hipStream_t st) {
   const int rc = lane_blocks_per_cu<N, D, S, V, O>(per_cu);
   if (rc) return rc;
   const long need = ((long)a.n_inst + 63) / 64;
-  if (S == PDQ_FIXEDPOINT && need > 0x7fffffffL) return host_error(PDQ_ERR_UNSUPPORTED, "too many instances");
-  // the fixed-point kernel gives every instance a lane of its own (no grid
-  // stride); the others stride over at most the resident workgroups
-  const long cap = S == PDQ_FIXEDPOINT ? need : (long)per_cu * num_cus();
+  if (lane_split_strategy(S) && need > 0x7fffffffL) return host_error(PDQ_ERR_UNSUPPORTED, "too many instances");
+  // the smoothers' kernels give every instance a lane of its own (no grid
+  // stride); the filter's strides over at most the resident workgroups
+  const long cap = lane_split_strategy(S) ? need : (long)per_cu * num_cus();
   const int grid = (int)std::max<long>(1, std::min(need, cap));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(lane_block_threads<S>()), lds, st, a);
   PDQ_HIP_TRY(hipGetLastError());
