@@ -1697,7 +1697,7 @@ constexpr bool lane_split_strategy(int strategy) {
 
 template <int STRATEGY>
 constexpr int lane_block_threads() {
-  return lane_split_strategy(STRATEGY) ? 2 * kLaneBlock : kLaneBlock;
+  return lane_split_strategy(STRATEGY) ? 3 * kLaneBlock : kLaneBlock;
 }
 
 // LDS bytes of one workgroup: three slots per lane; with the second
@@ -1778,12 +1778,17 @@ __device__ __forceinline__ void lane_solve_split(const LArgs& a, double* lds) {
       lane_end_instance<N, D>(c, s);
     }
   } else {
+    // wavefront 1: the merges, then the conditionals; wavefront 2: the
+    // checkpoint writes (a lane's request is one or the other)
     const double* ts = a.times + (size_t)(mine ? inst : 0) * a.K;
     __syncthreads();
     while (__builtin_amdgcn_readfirstlane(*flag)) {
-      lane_run_merge<N, D>(c, ts, lane_unpack_req(req + 2 * l));
+      LaneMergeReq rq = lane_unpack_req(req + 2 * l);
+      if (role == 1) rq.ckpt = 0;
+      else rq.on = 0;
+      lane_run_merge<N, D>(c, ts, rq);
       __syncthreads();
-      lane_revert_finish<N, D>(c, fin[l]);
+      if (role == 1) lane_revert_finish<N, D>(c, fin[l]);
       __syncthreads();
     }
   }
